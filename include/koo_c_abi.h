@@ -1,0 +1,340 @@
+/*
+ * koo_c_abi.h -- C ABI of the MI355X (gfx950) LMM Monte Carlo pricing library
+ * (libkoo_amd.so).
+ *
+ * Every entry point keeps the name and the parameter order of the reference
+ * host-callable launcher it replaces (cited per function, paths relative to
+ * kooderive-code-r256/), with three uniform changes:
+ *   - it returns int status (KOO_OK = 0) instead of void / exit() on error;
+ *   - it takes a trailing `void* stream` (a hipStream_t; NULL = null stream)
+ *     and never synchronises, so calls are stream-ordered and capturable;
+ *   - C++-typed parameters (std::vector<int>&, cudaStream_t, bool) become
+ *     plain `const int*` + count, `void*` and `int`.
+ * All pointers are device pointers unless stated; the caller owns every
+ * buffer.  Layouts are the reference's path-minor ones: [D][P] for Sobol
+ * draws, (S,F,P) for bridged increments, (S,N,P) for rates, (S,N+1,P) for
+ * discount ratios, (S,N,F) for pseudo-roots.
+ */
+#ifndef KOO_C_ABI_H
+#define KOO_C_ABI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KOO_OK 0
+#define KOO_ERR_INVALID (-1) /* bad argument: nothing launched */
+#define KOO_ERR_HIP (-2)     /* HIP launch / runtime failure */
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* koo_last_error(void);
+/* Library version string. */
+const char* koo_version(void);
+
+/* =====================================================================
+ * Stage (a): Sobol integers, scrambling, inverse normal
+ * =================================================================== */
+
+/* Host: 32 Bratley-Fox direction numbers per dimension (Joe-Kuo table),
+ * written to host memory [n_dimensions][32].
+ * Replaces initSobolDirectionVectors, gold/sobol_gold.cpp:118. */
+int initSobolDirectionVectors(int n_dimensions, unsigned int* directions_host);
+
+/* d_output[d][i] = Sobol integer of Gray-code point i + path_offset + 1 in
+ * dimension d, for i < n_vectors, d < n_dimensions; d_directions is
+ * [n_dimensions][32].  Replaces sobolGPUInts, sobol_int_gpu.h:68. */
+int sobolGPUInts(int n_vectors, int n_dimensions, unsigned path_offset,
+                 unsigned int* d_directions, unsigned int* d_output, void* stream);
+
+/* In place: device_data[d][p] ^= scramblers[d] for d < number_dim, p < paths.
+ * Replaces scramble_gpu, scramble_gpu.h:8. */
+int scramble_gpu(unsigned int* device_data, unsigned int* scramblers, int number_dim, int paths,
+                 void* stream);
+
+/* out[i] = normal quantile of in[i] (tail-exact Shaw-Brickman, FP32).
+ * Replaces the thrust::transform of inverseCumulativeShawBrickmanUnsignedInt
+ * in LMM_evolver_full.cu:330-334 (no named launcher in the reference). */
+int inverse_normal_u32_gpu(const unsigned int* in, float* out, long long n, void* stream);
+
+/* Fused generator: out[d][i] = invnorm(sobol(d, i + path_offset + 1) ^ s_d)
+ * with s_d = scramblers[d] for d < number_scrambled, else 0.  One write of
+ * FP32 normals; no integer buffer.  New (no reference counterpart): fuses
+ * sobolGPUInts + scramble_gpu + the inverse-normal transform. */
+int sobol_scramble_normal_gpu(int n_vectors, int n_dimensions, unsigned path_offset,
+                              const unsigned int* d_directions, const unsigned int* scramblers,
+                              int number_scrambled, float* out, void* stream);
+
+/* =====================================================================
+ * Host set-up (FP64, host memory): correlation, PCA pseudo-roots, drifts
+ * =================================================================== */
+
+/* corr[i][j] = L + (1-L) exp(-beta |t_i - t_j|), corr[i][i] = 1.
+ * Replaces GenerateExpLongCorrMatrix, gold/volstructs_gold.cpp:157. */
+int koo_exp_corr(const double* times, int n, double L, double beta, double* corr);
+
+/* Rank-`factors` PCA root of a symmetric n x n matrix (largest eigenvalues),
+ * rows rescaled to keep the diagonal when normalise != 0; out is n x factors.
+ * Replaces pseudoSqrtSchur, gold/math/pseudoSquareRoot.cpp:76. */
+int koo_pseudo_sqrt(int n, const double* cov, int factors, int normalise, double* out);
+
+/* A (steps, n_rates, factors): per-step pseudo-roots of the flat-vol
+ * covariance with exponential correlation on rate_times[0..n_rates).
+ * rate_times has n_rates+1 entries.  Replaces FlatVolPseudoRootsOfCovariances,
+ * gold/volstructs_gold.cpp:236. */
+int FlatVolPseudoRootsOfCovariances(const double* rate_times, const double* evol_times,
+                                    const double* vols, int n_rates, int steps, int factors,
+                                    double L, double beta, double* A);
+
+/* Factor model A(s,r,f) = sigma_r sqrt(T_s - T_{s-1}) B(r,f) for r >= s, 0
+ * otherwise; B (n_rates, factors) is the renormalised PCA root of the rate
+ * correlation (optional output).  New: makes dA/dsigma_r = A/sigma_r exact. */
+int koo_factor_pseudo_roots(const double* rate_times, const double* vols, int n_rates,
+                            int factors, double L, double beta, double* A, double* B);
+
+/* out(s,r) = -0.5 sum_f A(s,r,f)^2.  Replaces the fixed-drift loop of
+ * gold/LMM_evolver_classes_gold.h:193-205. */
+int koo_fixed_drifts(const double* A, int steps, int n_rates, int factors, double* out);
+
+/* Spot-measure drifts of one step.  Replaces spotDriftComputer::getDrifts,
+ * gold/LMM_evolver_gold.h:178-231. */
+int spotDriftComputer_getDrifts(const double* A_step, const double* rates, const double* taus,
+                                const double* displacements, int factors, int n_rates, int alive,
+                                double* drifts);
+
+/* =====================================================================
+ * Stage (b): Brownian bridge, correlation + drift, LMM evolution
+ * =================================================================== */
+
+/* Host: diagonal allocation of the F*S Sobol dimensions to (factor, step),
+ * indices [F][S].  Replaces bridge_allocate_diagonal,
+ * gold/Bridge_allocator_gold.cpp:38. */
+int bridge_allocate_diagonal(int factors, int steps, int* indices_host);
+
+/* Jaeckel bridge on t_i = i+1 plus successive differencing, per factor:
+ * in [D][P] (dimension indices(f,i) feeds bridge variate i of factor f),
+ * out (S,F,P).  Replaces brownian_bridge::transform, brownian_bridge.h:49
+ * (a C++ class in the reference; this is its C shim). */
+int brownian_bridge_gpu(int steps, const int* allocation_indices, const float* in, float* out,
+                        int paths, int factors, void* stream);
+
+/* Fused stage (a)+(b): Sobol point -> scramble -> normal -> bridge, written
+ * once as (S,F,P) increments; D = steps*factors dimensions, dimension
+ * indices(f,i) scrambled with scramblers[indices(f,i)] when it is below
+ * number_scrambled.  New (no reference counterpart): replaces the chain
+ * SobDeviceInts -> scramble_main -> transform -> bb.transform of
+ * LMM_evolver_full.cu:312-346. */
+int sobol_bridge_gpu(int paths, unsigned path_offset, int steps, int factors,
+                     const unsigned int* d_directions, const unsigned int* scramblers,
+                     int number_scrambled, const int* allocation_indices, float* out,
+                     void* stream);
+
+/* out(s,r,p) = sum_f A(s,r,f) in(s,p,f) + drift_fixed(s,r) for r >= alive[s].
+ * Replaces correlated_drift_paths_GPU, Correlate_drift_gpu.h:12. */
+int correlated_drift_paths_GPU(float* input_data_device, float* output_data_device,
+                               float* A_device, int* alive_device, float* drift_fixed_device,
+                               int A_offset_per_step, int factors, int out_dimensions,
+                               int number_paths, int steps, void* stream);
+
+/* Predictor-corrector log-Euler over all steps (spot measure, displaced
+ * diffusion) with fused discount ratios.  variates (S,F,P) when
+ * transposeData != 0, else (S,P,F).  Writes rates and log-rates (S,N,P) for
+ * alive rates and, when doDiscounts != 0, discount ratios (S,N+1,P).
+ * e_buffer_device / e_buffer_pred_device / sharedMemForLogs / numberThreads
+ * are accepted for signature compatibility and ignored (the running drift
+ * sums live in registers).  Replaces LMM_evolver_pc_all_gpu,
+ * LMM_evolver_all.h:22-46. */
+int LMM_evolver_pc_all_gpu(float* initial_rates_device, float* initial_log_rates_device,
+                           float* taus_device, float* variates_device, float* pseudo_roots_device,
+                           float* initial_drifts_device, float* fixed_drifts_device,
+                           float* displacements_device, int* alive_device, int paths,
+                           int factors, int steps, int rates, float* e_buffer_device,
+                           float* e_buffer_pred_device, float* evolved_rates_device,
+                           float* evolved_log_rates_device, float* discounts_device,
+                           int sharedMemForLogs, int transposeData, void* stream,
+                           int numberThreads, int doDiscounts);
+
+/* P(s,alive) = 1, P(s,r+1) = P(s,r)/(1 + tau_r f(s,r)).  alive_host has
+ * `steps` entries (host copy of the reference's aliveIndices vector),
+ * alive_device the same on the device.  Replaces
+ * discount_ratios_computation_gpu, LMM_evolver_gpu.h:55-64. */
+int discount_ratios_computation_gpu(float* evolved_rates_device, float* taus_device,
+                                    const int* alive_host, int* alive_device, int paths,
+                                    int steps, int rates, float* discounts_device,
+                                    int allStepsAtOnce, void* stream);
+
+/* Ann(s,r) = sum_{j>=r} tau_j P(s,j+1).  Replaces
+ * coterminal_annuity_ratios_computation_gpu, LMM_evolver_gpu.h:68-75. */
+int coterminal_annuity_ratios_computation_gpu(float* discounts_device, float* taus_device,
+                                              const int* alive_host, int paths, int steps,
+                                              int rates, float* annuities_device, void* stream);
+
+/* SR(s,r) = (P(s,r) - P(s,N)) / Ann(s,r).  Replaces
+ * coterminal_swap_rate_computation_gpu, LMM_evolver_gpu.h:78-85. */
+int coterminal_swap_rate_computation_gpu(float* discounts_device, float* annuities_device,
+                                         const int* alive_host, int paths, int steps, int rates,
+                                         float* cot_swap_rates_device, void* stream);
+
+/* Swap rate from startIndex to endIndex on the curve of step_for_offset_in,
+ * written to row step_for_offset_out of a (.,P) matrix.  Replaces
+ * swap_rate_computation_gpu, LMM_evolver_gpu.h:88-97. */
+int swap_rate_computation_gpu(float* discounts_device, float* taus_device, int startIndex,
+                              int endIndex, int paths, int step_for_offset_in,
+                              int step_for_offset_out, int rates, float* swap_rate_device,
+                              void* stream);
+
+/* Discretely compounded money-market numeraire, B(0) = 1,
+ * B(i) = B(i-1) P(i-1,i-1)/P(i-1,i); out (N,P).  Replaces
+ * spot_measure_numeraires_computation_gpu, LMM_evolver_gpu.h:100-103. */
+int spot_measure_numeraires_computation_gpu(float* discount_factors_global,
+                                            float* numeraire_values_global, int paths,
+                                            int n_rates, void* stream);
+
+/* As above, written at column offset pathsOffsetForOutput of a
+ * (N, pathsForOutput) matrix.  Replaces
+ * spot_measure_numeraires_computation_offset_gpu, LMM_evolver_gpu.h:104-111. */
+int spot_measure_numeraires_computation_offset_gpu(float* discount_factors_global,
+                                                   float* numeraire_values_global, int paths,
+                                                   int pathsForOutput, int pathsOffsetForOutput,
+                                                   int n_rates, void* stream);
+
+/* out(s,p) = f(s, forwardIndices[s], p).  forwardIndices_host has `steps`
+ * entries.  Replaces forward_rate_extraction_gpu, LMM_evolver_gpu.h:115-121. */
+int forward_rate_extraction_gpu(float* all_forwards_device, const int* forwardIndices_host,
+                                int paths, int steps, int rates, float* select_forwards_device,
+                                void* stream);
+
+/* out(i,p) = f(stepIndices[i], forwardIndices[i], p) for i < n_indices.
+ * Replaces forward_rate_extraction_selecting_step_gpu,
+ * LMM_evolver_gpu.h:123-130. */
+int forward_rate_extraction_selecting_step_gpu(float* all_forwards_device,
+                                               const int* forwardIndices_host,
+                                               const int* stepIndices_host, int n_indices,
+                                               int paths, int rates,
+                                               float* select_forwards_device, void* stream);
+
+/* out[offsetForOutput + p + pathsOutput*i] = all[p + pathsInput*exerciseIndices[i]].
+ * exerciseIndices is a device array.  Replaces
+ * spot_measure_numeraires_extraction_gpu, LMM_evolver_gpu.h:132-140. */
+int spot_measure_numeraires_extraction_gpu(float* all_numeraire_values_global,
+                                           float* some_numeraire_values_global, int pathsInput,
+                                           int pathsOutput, int offsetForOutput,
+                                           int numberRates, int numberExerciseTimes,
+                                           int* exerciseIndices_vec, void* stream);
+
+/* =====================================================================
+ * Stage (c): cash flows, discounting, aggregation, reductions
+ * =================================================================== */
+
+/* Swap of rate1 against a fixed coupon; aux_data = [K, payReceive,
+ * accruals1[numberSteps], accruals2[numberSteps]] (device).
+ * Replaces cashFlowGeneratorCallerSwap, cashFlowGeneration_product_gpu.h:190. */
+int cashFlowGeneratorCallerSwap(float* genFlows1, float* genFlows2, float* aux_data, int paths,
+                                int numberSteps, float* rates1, float* rates2, float* rates3,
+                                float* forwards, float* discRatios, void* stream);
+
+/* Target redemption note; aux_data = [totalCoupon, K, m, accruals1[S],
+ * accruals2[S]].  Replaces cashFlowGeneratorCallerTARN,
+ * cashFlowGeneration_product_gpu.h:111. */
+int cashFlowGeneratorCallerTARN(float* genFlows1, float* genFlows2, float* aux_data, int paths,
+                                int numberSteps, float* rates1, float* rates2, float* rates3,
+                                float* forwards, float* discRatios, void* stream);
+
+/* Caplet strip on rate1 (new product, same interface as Swap):
+ * aux_data = [K, accruals[numberSteps]]; genFlows1 = acc max(rate1 - K, 0),
+ * genFlows2 = 0. */
+int cashFlowGeneratorCallerCaplet(float* genFlows1, float* genFlows2, float* aux_data, int paths,
+                                  int numberSteps, float* rates1, float* rates2, float* rates3,
+                                  float* forwards, float* discRatios, void* stream);
+
+/* Discount flows generated at step s, paid between rate times
+ * firstIndex[s] and secondIndex[s] (log-linear weight theta[s]), to the
+ * numeraire: discountedFlows(s,p) and summedDiscountedFlows(p).  The two
+ * flags are accepted and ignored.  Replaces cashFlowDiscounting_gpu,
+ * cashFlowDiscounting_gpu.h:14-27. */
+int cashFlowDiscounting_gpu(int* firstIndex, int* secondIndex, float* theta,
+                            float* discountRatios, float* genFlows, float* numeraireValues,
+                            int paths, int numberSteps, int useTextures, int useShared,
+                            float* discountedFlows, float* summedDiscountedFlows, void* stream);
+
+/* Discount each of numberFlowsPerPath flows (generated at step
+ * generationIndices[i]) to rate time targetIndices[i] via the numeraire.
+ * Replaces cashFlowDiscounting_partial_gpu, cashFlowDiscounting_gpu.h:29-42. */
+int cashFlowDiscounting_partial_gpu(int numberFlowsPerPath, int* generationIndices_dev,
+                                    int* firstIndex_dev, int* secondIndex_dev, float* theta_dev,
+                                    int* targetIndices_dev, float* discountRatios_dev,
+                                    float* genFlows_dev, float* numeraireValues_dev, int paths,
+                                    int numberSteps, float* discountedFlows_dev, void* stream);
+
+/* aggregatedFlows(precedingIndex[s], offSetForOutput + p) += genFlows(s,p)
+ * for start <= s < numberSteps (accumulates, never overwrites).
+ * Replaces AggregateFlows_gpu, cashFlowAggregation_gpu.h:18-27. */
+int AggregateFlows_gpu(float* aggregatedFlows, int totalPaths, int numberExerciseDates,
+                       float* genFlows, int batchPaths, int offSetForOutput, int numberSteps,
+                       int* precedingIndex, int start, void* stream);
+
+/* Partial sums: out[i] = sum of data[8i .. 8i+7], repeated until at most
+ * stopPoint values remain; *result receives the pointer into workspace
+ * (>= points floats) that holds them and *remaining their count.  As in the
+ * reference, a level whose length is not a multiple of 8 drops its last
+ * (length % 8) values.
+ * Replaces PartialReduce_using8s_gpu, reductions_gpu.h:9-14. */
+int PartialReduce_using8s_gpu(const float* data, float* workspace, int points, int threads,
+                              int stopPoint, float** result, int* remaining, void* stream);
+/* Same with 16-way sums.  Replaces PartialReduce_using16s_gpu,
+ * reductions_gpu.h:15-19. */
+int PartialReduce_using16s_gpu(const float* data, float* workspace, int points, int threads,
+                               int stopPoint, float** result, int* remaining, void* stream);
+
+/* *final = sum of in[0..size); wsp holds >= 1024 floats.  threads/blocks are
+ * hints and may be 0.  Replaces reduceSinglePass, reduction_thread_fence.h:13. */
+int reduceSinglePass(int size, int threads, int blocks, float* in, float* wsp, float* final,
+                     void* stream);
+
+/* FP64 compensated sum of n floats or doubles into *out (device double);
+ * workspace holds at least koo_sum_workspace_doubles(n) doubles.  New. */
+long long koo_sum_workspace_doubles(long long n);
+int koo_sum_f32_to_f64(const float* in, long long n, double* workspace, double* out, void* stream);
+int koo_sum_f64(const double* in, long long n, double* workspace, double* out, void* stream);
+
+/* =====================================================================
+ * Fused LMM pricing with adjoint Greeks (new; the reference lists adjoint
+ * Greeks as an open task, doc/todo.txt:10).
+ *
+ * Model: N rates, steps == N, alive(s) = s, pseudo-roots A (S,N,F) with
+ * fixed drift -0.5 sum_f A^2, step-0 drift from the initial rates, spot
+ * measure.  Product KOO_PRODUCT_CAPLETS: caplet i fixes f_i at step i and pays
+ * tau_i max(f_i - K, 0) at T_{i+1} (product_aux = [K]).  Prices are in units
+ * of the numeraire at T_0 (B_0 = 1).
+ *
+ * Forward: price (device double) and, when checkpoints != NULL, the
+ * log-rate state at the start of every checkpoint_every-th step plus the
+ * numeraire there.  Backward: one reverse sweep per path from those
+ * checkpoints (recomputing the steps in between into segment_scratch), giving
+ * greeks_out[0..N) = dV/df_i(0) and greeks_out[N..2N) = dV/dsigma_i with
+ * A(s,i,f) proportional to sigma_i.  Sizes of all device buffers come from
+ * lmm_price_workspace().  product_aux is a HOST array (read at call time);
+ * every other pointer is a device pointer.  variates are (S,F,P); price_out
+ * and greeks_out receive means over paths (FP64).
+ * =================================================================== */
+#define KOO_PRODUCT_CAPLETS 1
+
+int lmm_price_workspace(int paths, int rates, int factors, int checkpoint_every,
+                        size_t* checkpoint_bytes, size_t* scratch_bytes, size_t* partial_bytes);
+
+int lmm_price_fwd(int paths, int rates, int factors, int checkpoint_every, const float* variates,
+                  const float* initial_rates, const float* taus, const float* displacements,
+                  const float* pseudo_roots, int product, const float* product_aux,
+                  float* checkpoints, double* partials, double* price_out, void* stream);
+
+int lmm_price_bwd(int paths, int rates, int factors, int checkpoint_every, const float* variates,
+                  const float* initial_rates, const float* taus, const float* displacements,
+                  const float* pseudo_roots, const float* vols, int product,
+                  const float* product_aux, const float* checkpoints, float* segment_scratch,
+                  double* partials, double* greeks_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KOO_C_ABI_H */

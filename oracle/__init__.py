@@ -1,0 +1,311 @@
+"""FP64 CPU oracle for the LMM Monte Carlo pricing path (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this package, and only as the checker: the product library
+(useful_code_amd) never imports, links or calls it.
+
+``lib()`` loads ``oracle/liboracle.so`` (the C restatement in koo_oracle.c);
+``ref()`` loads ``oracle/_ref/libkoo_ref.so`` (the reference's own gold
+routines, built from the reference sources by oracle/Makefile) when present.
+The thin numpy wrappers below keep the reference's array layouts.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_c_int, _c_dbl, _c_u32, _c_i64 = ctypes.c_int, ctypes.c_double, ctypes.c_uint32, ctypes.c_int64
+
+
+def build(ref=True):
+    """Compile liboracle.so (and the reference gold library when the
+    reference sources are present)."""
+    targets = ["all"]
+    if ref and os.path.isdir("/root/reference/kooderive-code-r256/LINUX/gold"):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", _HERE] + targets, check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build(ref=False)
+        _LIB = ctypes.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+def ref_available():
+    return os.path.exists(os.path.join(_HERE, "_ref", "libkoo_ref.so"))
+
+
+def ref():
+    global _REF
+    if _REF is None:
+        _REF = ctypes.CDLL(os.path.join(_HERE, "_ref", "libkoo_ref.so"))
+        _declare_ref(_REF)
+    return _REF
+
+
+class CapletModel(ctypes.Structure):
+    _fields_ = [("n_rates", _c_int), ("factors", _c_int), ("paths", _c_int),
+                ("f0", ctypes.c_void_p), ("taus", ctypes.c_void_p), ("disp", ctypes.c_void_p),
+                ("A", ctypes.c_void_p), ("sigma", ctypes.c_void_p), ("strike", _c_dbl)]
+
+
+def _declare(L):
+    sig = {
+        "orc_sobol_directions": (_c_int, [_c_int, _u32p]),
+        "orc_sobol_ints": (None, [_c_int, _c_int, _c_u32, _u32p, _u32p]),
+        "orc_scramble": (None, [_u32p, _u32p, _c_int, _c_int]),
+        "orc_invnorm_sb_ref_f32": (ctypes.c_float, [_c_u32]),
+        "orc_invnorm_u32": (_c_dbl, [_c_u32]),
+        "orc_invnorm": (_c_dbl, [_c_dbl]),
+        "orc_invnorm_u32_array": (None, [_u32p, _f64p, _c_i64]),
+        "orc_bridge_allocate_diagonal": (None, [_c_int, _c_int, _i32p]),
+        "orc_bridge_tables": (None, [_c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p]),
+        "orc_bridge_transform": (None, [_c_int, _c_int, _c_int, _i32p, _f64p, _f64p]),
+        "orc_correlate_drift": (None, [_f64p, _f64p, _f64p, _i32p, _f64p, _c_int, _c_int,
+                                       _c_int, _c_int, _c_int]),
+        "orc_spot_drifts": (None, [_f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _f64p]),
+        "orc_lmm_pc_evolve": (None, [_f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                     _i32p, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                     _f64p, _f64p, _f64p]),
+        "orc_discount_ratios": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "orc_coterminal_annuities": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "orc_coterminal_swap_rates": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "orc_swap_rate": (None, [_f64p, _f64p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                 _f64p]),
+        "orc_spot_numeraires": (None, [_f64p, _c_int, _c_int, _f64p]),
+        "orc_forward_extraction": (None, [_f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "orc_swap_flows": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _f64p]),
+        "orc_tarn_flows": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _f64p, _f64p]),
+        "orc_caplet_flows": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _f64p]),
+        "orc_cash_flow_indices_weights": (None, [_f64p, _f64p, _c_int, _i32p, _i32p, _f64p]),
+        "orc_cash_flow_discounting": (None, [_i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _c_int,
+                                             _c_int, _f64p, _f64p]),
+        "orc_cash_flow_discounting_partial": (None, [_c_int, _i32p, _i32p, _i32p, _f64p, _i32p,
+                                                     _f64p, _f64p, _f64p, _c_int, _c_int, _f64p]),
+        "orc_sum_compensated": (_c_dbl, [_f64p, _c_i64]),
+        "orc_sym_eigen": (_c_int, [_c_int, _f64p, _f64p, _f64p]),
+        "orc_exp_corr": (None, [_f64p, _c_int, _c_dbl, _c_dbl, _f64p]),
+        "orc_pseudo_sqrt": (_c_int, [_c_int, _f64p, _c_int, _c_int, _f64p]),
+        "orc_flat_vol_pseudo_roots": (_c_int, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int,
+                                               _c_dbl, _c_dbl, _f64p]),
+        "orc_factor_pseudo_roots": (_c_int, [_f64p, _f64p, _c_int, _c_int, _c_dbl, _c_dbl,
+                                             _f64p, ctypes.c_void_p]),
+        "orc_caplet_price_paths": (None, [ctypes.POINTER(CapletModel), _f64p, _f64p]),
+        "orc_caplet_adjoint_paths": (None, [ctypes.POINTER(CapletModel), _f64p, _f64p, _f64p,
+                                            _f64p]),
+        "orc_caplet_pipeline": (None, [ctypes.POINTER(CapletModel), _u32p, _c_int,
+                                       ctypes.POINTER(_c_dbl), _f64p, _f64p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _declare_ref(R):
+    sig = {
+        "ref_sobol_directions": (None, [_c_int, _u32p]),
+        "ref_sobol_ints_offset": (None, [_c_int, _c_int, _c_int, _u32p, _u32p]),
+        "ref_scramble": (None, [_u32p, _u32p, _u32p, _c_int, _c_int]),
+        "ref_invnorm_sb": (ctypes.c_float, [_c_u32]),
+        "ref_bridge_allocate_diagonal": (None, [_c_int, _c_int, _i32p]),
+        "ref_correlate_drift": (None, [_f64p, _f64p, _f64p, _c_int, _f64p, _c_int, _c_int,
+                                       _c_int, _c_int]),
+        "ref_spot_drifts": (None, [_f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _f64p]),
+        "ref_lmm_pc": (None, [_f64p, _f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _c_int,
+                              _f64p, _f64p]),
+        "ref_discount_ratios": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "ref_annuities": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "ref_cot_swap_rates": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "ref_swap_rate": (None, [_f64p, _f64p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                 _c_int, _f64p]),
+        "ref_numeraires": (None, [_f64p, _c_int, _c_int, _f64p]),
+        "ref_forward_extraction": (None, [_f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
+        "ref_swap_flows": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _f64p]),
+        "ref_tarn_flows": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _f64p, _f64p]),
+        "ref_cash_flow_indices_weights": (None, [_f64p, _c_int, _f64p, _c_int, _i32p, _i32p,
+                                                 _f64p]),
+        "ref_cash_flow_discounting": (None, [_i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _c_int,
+                                             _c_int, _f64p, _f64p]),
+        "ref_flat_vol_pseudo_roots": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _c_dbl,
+                                             _c_dbl, _f64p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(R, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+# ---------------------------------------------------------------- wrappers
+
+def sobol_directions(n_dims):
+    d = np.zeros(n_dims * 32, np.uint32)
+    if lib().orc_sobol_directions(n_dims, d) != 0:
+        raise ValueError("n_dims out of table range")
+    return d
+
+
+def sobol_ints(n_vectors, n_dims, offset=0, dirs=None):
+    dirs = sobol_directions(n_dims) if dirs is None else dirs
+    out = np.zeros(n_vectors * n_dims, np.uint32)
+    lib().orc_sobol_ints(n_vectors, n_dims, offset, dirs, out)
+    return out.reshape(n_dims, n_vectors)
+
+
+def scramble(data, scramblers, number_dim):
+    data = np.ascontiguousarray(data, np.uint32).copy()
+    lib().orc_scramble(data.reshape(-1), np.ascontiguousarray(scramblers, np.uint32),
+                       number_dim, data.shape[-1])
+    return data
+
+
+def invnorm_u32(x):
+    x = np.ascontiguousarray(x, np.uint32)
+    out = np.zeros(x.size, np.float64)
+    lib().orc_invnorm_u32_array(x.reshape(-1), out, x.size)
+    return out.reshape(x.shape)
+
+
+def bridge_allocate_diagonal(factors, steps):
+    idx = np.zeros(factors * steps, np.int32)
+    lib().orc_bridge_allocate_diagonal(factors, steps, idx)
+    return idx
+
+
+def bridge_tables(steps):
+    bi, li, ri = (np.zeros(steps, np.int32) for _ in range(3))
+    lw, rw, sd = (np.zeros(steps, np.float64) for _ in range(3))
+    lib().orc_bridge_tables(steps, bi, li, ri, lw, rw, sd)
+    return bi, li, ri, lw, rw, sd
+
+
+def bridge_transform(normals, steps, factors, indices=None):
+    """normals [D][P] -> increments (S,F,P)."""
+    normals = np.ascontiguousarray(normals, np.float64)
+    paths = normals.shape[-1]
+    idx = bridge_allocate_diagonal(factors, steps) if indices is None else np.ascontiguousarray(indices, np.int32)
+    out = np.zeros(steps * factors * paths, np.float64)
+    lib().orc_bridge_transform(steps, factors, paths, idx, normals.reshape(-1), out)
+    return out.reshape(steps, factors, paths)
+
+
+def exp_corr(times, L, beta):
+    t = np.ascontiguousarray(times, np.float64)
+    out = np.zeros(t.size * t.size)
+    lib().orc_exp_corr(t, t.size, L, beta, out)
+    return out.reshape(t.size, t.size)
+
+
+def pseudo_sqrt(cov, factors, normalise=True):
+    cov = np.ascontiguousarray(cov, np.float64)
+    n = cov.shape[0]
+    out = np.zeros(n * factors)
+    lib().orc_pseudo_sqrt(n, cov.reshape(-1), factors, int(normalise), out)
+    return out.reshape(n, factors)
+
+
+def flat_vol_pseudo_roots(rate_times, evol_times, vols, factors, L, beta):
+    rt = np.ascontiguousarray(rate_times, np.float64)
+    et = np.ascontiguousarray(evol_times, np.float64)
+    v = np.ascontiguousarray(vols, np.float64)
+    n, s = v.size, et.size
+    A = np.zeros(s * n * factors)
+    lib().orc_flat_vol_pseudo_roots(rt, et, v, n, s, factors, L, beta, A)
+    return A.reshape(s, n, factors)
+
+
+def factor_pseudo_roots(rate_times, vols, factors, L, beta):
+    rt = np.ascontiguousarray(rate_times, np.float64)
+    v = np.ascontiguousarray(vols, np.float64)
+    n = v.size
+    A = np.zeros(n * n * factors)
+    B = np.zeros(n * factors)
+    lib().orc_factor_pseudo_roots(rt, v, n, factors, L, beta, A, B.ctypes.data)
+    return A.reshape(n, n, factors), B.reshape(n, factors)
+
+
+def fixed_drifts(A):
+    return -0.5 * np.sum(np.asarray(A, np.float64) ** 2, axis=2)
+
+
+def spot_drifts(A_step, rates, taus, disp, alive=0):
+    A_step = np.ascontiguousarray(A_step, np.float64)
+    n, F = A_step.shape
+    out = np.zeros(n)
+    lib().orc_spot_drifts(A_step.reshape(-1), np.ascontiguousarray(rates, np.float64),
+                          np.ascontiguousarray(taus, np.float64),
+                          np.ascontiguousarray(disp, np.float64), F, n, alive, out)
+    return out
+
+
+def lmm_pc_evolve(f0, taus, variates, A, disp, alive, transposed=True, initial_drifts=None,
+                  fixed=None):
+    """Kernel-semantics PC evolver; variates (S,F,P) if transposed else (S,P,F)."""
+    f0 = np.ascontiguousarray(f0, np.float64)
+    taus = np.ascontiguousarray(taus, np.float64)
+    disp = np.ascontiguousarray(disp, np.float64)
+    A = np.ascontiguousarray(A, np.float64)
+    S, N, F = A.shape
+    v = np.ascontiguousarray(variates, np.float64)
+    P = v.shape[2] if transposed else v.shape[1]
+    alive = np.ascontiguousarray(alive, np.int32)
+    logs0 = np.log(f0 + disp)
+    mu0 = spot_drifts(A[0], f0, taus, disp, int(alive[0])) if initial_drifts is None else \
+        np.ascontiguousarray(initial_drifts, np.float64)
+    fx = fixed_drifts(A) if fixed is None else np.ascontiguousarray(fixed, np.float64)
+    rates = np.zeros(S * N * P)
+    logs = np.zeros(S * N * P)
+    disc = np.zeros(S * (N + 1) * P)
+    lib().orc_lmm_pc_evolve(f0, logs0, taus, v.reshape(-1), A.reshape(-1), mu0,
+                            fx.reshape(-1), disp, alive, P, F, S, N, int(transposed),
+                            rates, logs, disc)
+    return rates.reshape(S, N, P), logs.reshape(S, N, P), disc.reshape(S, N + 1, P)
+
+
+def caplet_model(f0, taus, disp, A, sigma, strike, paths):
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (f0, taus, disp, A, sigma)]
+    m = CapletModel(len(arrs[0]), arrs[3].shape[2], paths, arrs[0].ctypes.data,
+                    arrs[1].ctypes.data, arrs[2].ctypes.data, arrs[3].ctypes.data,
+                    arrs[4].ctypes.data, float(strike))
+    m._keep = arrs
+    return m
+
+
+def caplet_price_paths(model, z):
+    z = np.ascontiguousarray(z, np.float64)
+    out = np.zeros(model.paths)
+    lib().orc_caplet_price_paths(ctypes.byref(model), z.reshape(-1), out)
+    return out
+
+
+def caplet_adjoint_paths(model, z):
+    z = np.ascontiguousarray(z, np.float64)
+    pp = np.zeros(model.paths)
+    d = np.zeros(model.n_rates)
+    v = np.zeros(model.n_rates)
+    lib().orc_caplet_adjoint_paths(ctypes.byref(model), z.reshape(-1), pp, d, v)
+    return pp, d, v
+
+
+def caplet_pipeline(model, ints, adjoint=False):
+    ints = np.ascontiguousarray(ints, np.uint32)
+    price = ctypes.c_double(0.0)
+    d = np.zeros(model.n_rates)
+    v = np.zeros(model.n_rates)
+    lib().orc_caplet_pipeline(ctypes.byref(model), ints.reshape(-1), int(adjoint),
+                              ctypes.byref(price), d, v)
+    return price.value, d, v

@@ -1,0 +1,973 @@
+/*
+ * koo_oracle.c -- FP64 CPU oracle for the LMM Monte Carlo pricing path.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py, and only as the checker.  The product library
+ * never links or calls it.
+ *
+ * Each function restates, in double precision and plain C, the reference
+ * routine cited above it (paths relative to kooderive-code-r256/).  How each
+ * one is pinned (golden vectors, the compiled reference gold in oracle/_ref,
+ * closed forms or invariants) is recorded beside its test in
+ * tests/test_oracle_*.py; functions with no such check are marked
+ * "parity unpinned" here.
+ */
+#include "koo_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* The Joe-Kuo (degree, a, m[]) table: data shared with the product. */
+struct koo_sobol_primitive {
+  unsigned degree;
+  unsigned a;
+  unsigned m[18];
+};
+#include "../useful_code_amd/csrc/sobol_joe_kuo.inc"
+
+/* ======================================================================
+ * Stage (a)
+ * ==================================================================== */
+
+/* Bratley-Fox direction numbers, 32 per dimension, Q0.32.
+ * Follows gold/sobol_gold.cpp:118-164 (initSobolDirectionVectors). */
+int orc_sobol_directions(int n_dims, uint32_t *dirs) {
+  if (n_dims < 1 || n_dims > KOO_SOBOL_TABLE_DIMS) return -1;
+  for (int d = 0; d < n_dims; ++d) {
+    uint32_t *v = dirs + 32 * d;
+    if (d == 0) {
+      for (int i = 0; i < 32; ++i) v[i] = 1u << (31 - i);
+      continue;
+    }
+    int deg = (int)koo_sobol_table[d].degree;
+    uint32_t a = koo_sobol_table[d].a;
+    for (int i = 0; i < deg && i < 32; ++i) v[i] = koo_sobol_table[d].m[i] << (31 - i);
+    for (int i = deg; i < 32; ++i) {
+      uint32_t x = v[i - deg] ^ (v[i - deg] >> deg);
+      for (int j = 1; j < deg; ++j)
+        if ((a >> (deg - 1 - j)) & 1u) x ^= v[i - j];
+      v[i] = x;
+    }
+  }
+  return 0;
+}
+
+/* Sequential Gray-code Sobol: x_0 = 0, x_i = x_{i-1} ^ v[ctz(i)], output
+ * points offset+1 .. offset+n.  Follows gold/sobol_gold.cpp:166-204
+ * (sobolCPUints / sobolCPUintsOffset), skipping ahead with the Gray code
+ * g(n) = n ^ (n >> 1) instead of generating the discarded prefix. */
+void orc_sobol_ints(int n_vectors, int n_dims, uint32_t path_offset, const uint32_t *dirs,
+                    uint32_t *out) {
+  for (int d = 0; d < n_dims; ++d) {
+    const uint32_t *v = dirs + 32 * d;
+    uint32_t n = path_offset + 1u;
+    uint32_t g = n ^ (n >> 1);
+    uint32_t x = 0;
+    for (int k = 0; k < 32; ++k)
+      if ((g >> k) & 1u) x ^= v[k];
+    uint32_t *o = out + (size_t)d * n_vectors;
+    for (int i = 0; i < n_vectors; ++i) {
+      o[i] = x;
+      n += 1u;
+      x ^= v[__builtin_ctz(n)];
+    }
+  }
+}
+
+/* Digital shift of the first number_dim rows.  gold/scramble_gold.cpp:11-24. */
+void orc_scramble(uint32_t *data, const uint32_t *scramblers, int number_dim, int paths) {
+  for (int d = 0; d < number_dim; ++d)
+    for (int p = 0; p < paths; ++p) data[(size_t)d * paths + p] ^= scramblers[d];
+}
+
+/* Shaw-Brickman rational approximation of the normal quantile (W. T. Shaw and
+ * N. Brickman, "Differential equations for Monte Carlo recycling and a
+ * GPU-optimized normal quantile", 2009); coefficients as published and as
+ * used by inverseCumulatives.h:146-172 and gold/math/normals_gold.h:191-216. */
+static const double SB_P[6] = {1.2533136835212087879, 1.9797154223229267471,
+                               0.80002295072483916762, 0.087403248265958578062,
+                               0.0020751409553756572917, 4.744820732427972462e-6};
+static const double SB_Q[6] = {1.0, 2.0795584360534589311, 1.2499328117341603014,
+                               0.23668431621373705623, 0.0120098270559197768,
+                               0.00010590620919921025259};
+
+/* The reference's FP32 formula with its own u = float(x) * 2^-32 mapping
+ * (inverseCumulatives.h:146-172), in float arithmetic, edge cases included
+ * (x = 0 and x >= 2^32-128 give a non-finite value in the reference). */
+float orc_invnorm_sb_ref_f32(uint32_t x) {
+  const float P1 = 1.2533136835212087879f, P2 = 1.9797154223229267471f,
+              P3 = 0.80002295072483916762f, P4 = 0.087403248265958578062f,
+              P5 = 0.0020751409553756572917f, P6 = 4.744820732427972462e-6f;
+  const float Q1 = 1.0f, Q2 = 2.0795584360534589311f, Q3 = 1.2499328117341603014f,
+              Q4 = 0.23668431621373705623f, Q5 = 0.0120098270559197768f,
+              Q6 = 0.00010590620919921025259f;
+  volatile float v = (float)x * 2.3283064E-10F;
+  int sgn = (v >= 0.5f);
+  sgn = sgn - !sgn;
+  float z = -logf(1.0f - ((float)sgn * ((2.0f * v) - 1.0f)));
+  float p = P1 + z * (P2 + z * (P3 + z * (P4 + (P5 + P6 * z) * z)));
+  float q = Q1 + z * (Q2 + z * (Q3 + z * (Q4 + (Q5 + Q6 * z) * z)));
+  return (float)sgn * z * p / q;
+}
+
+/* Lower-tail probability t in (0, 1/2] and sign for the quantile solve. */
+static double norm_cdf_lower(double x) { return 0.5 * erfc(-x * M_SQRT1_2); }
+
+/* Accurate quantile of a lower-tail probability t in (0, 1/2]: Shaw-Brickman
+ * start, then two Halley steps on Phi(x) = t using libm erfc.  Returns x <= 0. */
+static double quantile_lower(double t) {
+  if (t >= 0.5) return 0.0;
+  double z = -log(2.0 * t);
+  double p = SB_P[0] + z * (SB_P[1] + z * (SB_P[2] + z * (SB_P[3] + z * (SB_P[4] + z * SB_P[5]))));
+  double q = SB_Q[0] + z * (SB_Q[1] + z * (SB_Q[2] + z * (SB_Q[3] + z * (SB_Q[4] + z * SB_Q[5]))));
+  double x = -z * p / q;
+  for (int it = 0; it < 2; ++it) {
+    double e = norm_cdf_lower(x) - t;
+    double dens = exp(-0.5 * x * x) / sqrt(2.0 * M_PI);
+    double u = e / dens;
+    x = x - u / (1.0 + 0.5 * x * u);
+  }
+  return x;
+}
+
+double orc_invnorm(double u) {
+  if (u <= 0.0 || u >= 1.0) return u <= 0.0 ? -INFINITY : INFINITY;
+  if (u < 0.5) return quantile_lower(u);
+  return -quantile_lower(1.0 - u);
+}
+
+/* The product's uniform: u(x) = x * 2^-32, with x = 0 mapped to 2^-33; the
+ * upper tail is taken as 1 - u = (2^32 - x) * 2^-32 exactly. */
+double orc_invnorm_u32(uint32_t x) {
+  const double two_m32 = 2.3283064365386962890625e-10;
+  if (x == 0u) return quantile_lower(0.5 * two_m32);
+  if (x < 0x80000000u) return quantile_lower((double)x * two_m32);
+  if (x == 0x80000000u) return 0.0;
+  return -quantile_lower((double)(0x100000000ull - (uint64_t)x) * two_m32);
+}
+
+void orc_invnorm_u32_array(const uint32_t *in, double *out, int64_t n) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) out[i] = orc_invnorm_u32(in[i]);
+}
+
+/* ======================================================================
+ * Stage (b)
+ * ==================================================================== */
+
+/* Diagonal allocation of Sobol dimensions to (factor, step) pairs.
+ * gold/Bridge_allocator_gold.cpp:38-76 (bridge_allocate_diagonal). */
+void orc_bridge_allocate_diagonal(int factors, int steps, int *indices) {
+  int i0 = 0, j0 = 0, i = 0, j = 0, counter = 0;
+  while (counter < factors * steps) {
+    indices[i * steps + j] = counter++;
+    if (i == 0 || j == steps - 1) {
+      if (i0 < factors - 1) {
+        i0 += 1;
+        j0 = 0;
+      } else {
+        i0 = factors - 1;
+        j0 += 1;
+      }
+      i = i0;
+      j = j0;
+    } else {
+      i -= 1;
+      j += 1;
+    }
+  }
+}
+
+/* Jaeckel bridge construction on the unit grid t_i = i+1.
+ * brownian_bridge.cu:355-367 and 639-700 (constructor and initialize()). */
+void orc_bridge_tables(int steps, int *bridge_index, int *left_index, int *right_index,
+                       double *left_weight, double *right_weight, double *std_dev) {
+  double *t = (double *)malloc(sizeof(double) * steps);
+  int *map = (int *)calloc(steps, sizeof(int));
+  for (int i = 0; i < steps; ++i) t[i] = (double)(i + 1);
+  map[steps - 1] = 1;
+  bridge_index[0] = steps - 1;
+  left_index[0] = right_index[0] = 0;
+  std_dev[0] = sqrt(t[steps - 1]);
+  left_weight[0] = right_weight[0] = 0.0;
+  for (int j = 0, i = 1; i < steps; ++i) {
+    while (map[j]) ++j;
+    int k = j;
+    while (!map[k]) ++k;
+    int l = j + ((k - 1 - j) >> 1);
+    map[l] = i;
+    bridge_index[i] = l;
+    left_index[i] = j;
+    right_index[i] = k;
+    if (j != 0) {
+      left_weight[i] = (t[k] - t[l]) / (t[k] - t[j - 1]);
+      right_weight[i] = (t[l] - t[j - 1]) / (t[k] - t[j - 1]);
+      std_dev[i] = sqrt(((t[l] - t[j - 1]) * (t[k] - t[l])) / (t[k] - t[j - 1]));
+    } else {
+      left_weight[i] = (t[k] - t[l]) / t[k];
+      right_weight[i] = t[l] / t[k];
+      std_dev[i] = sqrt(t[l] * (t[k] - t[l]) / t[k]);
+    }
+    j = k + 1;
+    if (j >= steps) j = 0;
+  }
+  free(map);
+  free(t);
+}
+
+/* Bridge then successive differences; output (S,F,P).
+ * brownian_bridge.cu:401-456 (transformCPU). */
+void orc_bridge_transform(int steps, int factors, int paths, const int *indices,
+                          const double *in, double *out) {
+  int *bi = (int *)malloc(sizeof(int) * steps * 3);
+  double *w = (double *)malloc(sizeof(double) * steps * 3);
+  orc_bridge_tables(steps, bi, bi + steps, bi + 2 * steps, w, w + steps, w + 2 * steps);
+#pragma omp parallel
+  {
+    double *path = (double *)malloc(sizeof(double) * steps);
+#pragma omp for schedule(static)
+    for (int p = 0; p < paths; ++p) {
+      for (int f = 0; f < factors; ++f) {
+        const int *idx = indices + f * steps;
+        path[steps - 1] = w[2 * steps] * in[(size_t)idx[0] * paths + p];
+        for (int i = 1; i < steps; ++i) {
+          int j = bi[steps + i], k = bi[2 * steps + i], l = bi[i];
+          double zz = in[(size_t)idx[i] * paths + p];
+          if (j != 0)
+            path[l] = w[i] * path[j - 1] + w[steps + i] * path[k] + w[2 * steps + i] * zz;
+          else
+            path[l] = w[steps + i] * path[k] + w[2 * steps + i] * zz;
+        }
+        for (int i = steps - 1; i >= 1; --i) path[i] -= path[i - 1];
+        for (int i = 0; i < steps; ++i) out[((size_t)i * factors + f) * paths + p] = path[i];
+      }
+    }
+    free(path);
+  }
+  free(bi);
+  free(w);
+}
+
+/* y(s,r,p) = sum_f A(s,r,f) z(s,p,f) + mu_fixed(s,r) for r >= alive(s).
+ * correlate_drift_gpu.cu:142-220 (correlation_drift_kernel); gold twin
+ * gold/correlate_drift_gold.h:22-61 (which fills every r). */
+void orc_correlate_drift(const double *in, double *out, const double *A, const int *alive,
+                         const double *drift_fixed, int a_offset_per_step, int factors,
+                         int out_dims, int paths, int steps) {
+  for (int s = 0; s < steps; ++s)
+    for (int p = 0; p < paths; ++p)
+      for (int r = alive[s]; r < out_dims; ++r) {
+        double t = 0.0;
+        for (int f = 0; f < factors; ++f)
+          t += A[(size_t)s * a_offset_per_step + r * factors + f] *
+               in[((size_t)s * paths + p) * factors + f];
+        out[((size_t)s * out_dims + r) * paths + p] = t + drift_fixed[s * out_dims + r];
+      }
+}
+
+/* Spot-measure drift mu_r = sum_f A_rf sum_{j=alive..r} g_j A_jf,
+ * g_j = (f_j + d_j) tau_j / (1 + f_j tau_j).
+ * gold/LMM_evolver_gold.h:178-203 (spotDriftComputer::getDrifts), started at
+ * the alive index. */
+void orc_spot_drifts(const double *A_step, const double *rates, const double *taus,
+                     const double *displacements, int factors, int n_rates, int alive,
+                     double *drifts) {
+  double e[64];
+  for (int f = 0; f < factors; ++f) e[f] = 0.0;
+  for (int r = 0; r < alive; ++r) drifts[r] = 0.0;
+  for (int r = alive; r < n_rates; ++r) {
+    double g = (rates[r] + displacements[r]) * taus[r] / (1.0 + rates[r] * taus[r]);
+    double mu = 0.0;
+    for (int f = 0; f < factors; ++f) {
+      e[f] += A_step[r * factors + f] * g;
+      mu += A_step[r * factors + f] * e[f];
+    }
+    drifts[r] = mu;
+  }
+}
+
+/* Predictor-corrector log-Euler over all steps with fused discount ratios.
+ * LMM_evolver_all.cu:506-747 (LMM_evolver_all_steps_pc_kernel_devicelog_discounts):
+ * step 0 uses the host initial drifts, later steps the drifts of the previous
+ * step's rates; only alive rates (r >= alive(s)) are written. */
+void orc_lmm_pc_evolve(const double *initial_rates, const double *initial_log_rates,
+                       const double *taus, const double *variates, const double *A,
+                       const double *initial_drifts, const double *fixed_drifts,
+                       const double *displacements, const int *alive, int paths, int factors,
+                       int steps, int rates, int transposed_variates, double *rates_out,
+                       double *log_rates_out, double *discounts_out) {
+  (void)initial_rates;
+  const size_t SP = (size_t)paths;
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < paths; ++p) {
+    double e[64], ep[64];
+    for (int s = 0; s < steps; ++s) {
+      const int a = alive[s];
+      const double *As = A + (size_t)s * rates * factors;
+      double zf[64];
+      for (int f = 0; f < factors; ++f)
+        zf[f] = transposed_variates ? variates[((size_t)s * factors + f) * SP + p]
+                                    : variates[((size_t)s * SP + p) * factors + f];
+      for (int f = 0; f < factors; ++f) e[f] = ep[f] = 0.0;
+      double df = 1.0;
+      discounts_out[((size_t)s * (rates + 1) + a) * SP + p] = 1.0;
+      for (int r = a; r < rates; ++r) {
+        const double *Ar = As + r * factors;
+        const double tau = taus[r], d = displacements[r];
+        double mu, logf0;
+        if (s == 0) {
+          mu = initial_drifts[r];
+          logf0 = initial_log_rates[r];
+        } else {
+          double fr = rates_out[((size_t)(s - 1) * rates + r) * SP + p];
+          double g = (fr + d) * tau / (1.0 + fr * tau);
+          mu = 0.0;
+          for (int f = 0; f < factors; ++f) {
+            e[f] += g * Ar[f];
+            mu += Ar[f] * e[f];
+          }
+          logf0 = log_rates_out[((size_t)(s - 1) * rates + r) * SP + p];
+        }
+        double dW = 0.0;
+        for (int f = 0; f < factors; ++f) dW += Ar[f] * zf[f];
+        double lf = logf0 + mu + dW + fixed_drifts[s * rates + r];
+        double Xp = exp(lf);
+        double gp = Xp * tau / (1.0 + (Xp - d) * tau);
+        double mup = 0.0;
+        for (int f = 0; f < factors; ++f) {
+          ep[f] += gp * Ar[f];
+          mup += Ar[f] * ep[f];
+        }
+        lf += 0.5 * (mup - mu);
+        double rate = exp(lf) - d;
+        log_rates_out[((size_t)s * rates + r) * SP + p] = lf;
+        rates_out[((size_t)s * rates + r) * SP + p] = rate;
+        df /= (1.0 + rate * tau);
+        discounts_out[((size_t)s * (rates + 1) + r + 1) * SP + p] = df;
+      }
+    }
+  }
+}
+
+/* P(s,alive) = 1; P(s,r+1) = P(s,r) / (1 + tau_r f(s,r)).
+ * gold/LMM_evolver_gold.h:460-492 (discount_ratios_computation_gold). */
+void orc_discount_ratios(const double *rates, const double *taus, const int *alive, int paths,
+                         int steps, int n_rates, double *discounts) {
+  for (int s = 0; s < steps; ++s)
+    for (int p = 0; p < paths; ++p) {
+      double df = 1.0;
+      discounts[((size_t)s * (n_rates + 1) + alive[s]) * paths + p] = df;
+      for (int r = alive[s]; r < n_rates; ++r) {
+        df /= 1.0 + taus[r] * rates[((size_t)s * n_rates + r) * paths + p];
+        discounts[((size_t)s * (n_rates + 1) + r + 1) * paths + p] = df;
+      }
+    }
+}
+
+/* A(s,r) = sum_{j>=r} tau_j P(s,j+1).
+ * gold/LMM_evolver_gold.h:494-522 (coterminal_annuity_ratios_computation_gold). */
+void orc_coterminal_annuities(const double *discounts, const double *taus, const int *alive,
+                              int paths, int steps, int n_rates, double *annuities) {
+  for (int p = 0; p < paths; ++p)
+    for (int s = 0; s < steps; ++s) {
+      double ann = 0.0;
+      for (int r = n_rates - 1; r >= alive[s]; --r) {
+        ann += taus[r] * discounts[((size_t)s * (n_rates + 1) + r + 1) * paths + p];
+        annuities[((size_t)s * n_rates + r) * paths + p] = ann;
+      }
+    }
+}
+
+/* SR(s,r) = (P(s,r) - P(s,N)) / A(s,r).
+ * gold/LMM_evolver_gold.h:524-552 (coterminal_swap_rates_computation_gold). */
+void orc_coterminal_swap_rates(const double *discounts, const double *annuities, const int *alive,
+                               int paths, int steps, int n_rates, double *swap_rates) {
+  for (int p = 0; p < paths; ++p)
+    for (int s = 0; s < steps; ++s)
+      for (int r = alive[s]; r < n_rates; ++r) {
+        double ann = annuities[((size_t)s * n_rates + r) * paths + p];
+        double df2 = discounts[((size_t)s * (n_rates + 1) + n_rates) * paths + p];
+        double df1 = discounts[((size_t)s * (n_rates + 1) + r) * paths + p];
+        swap_rates[((size_t)s * n_rates + r) * paths + p] = (df1 - df2) / ann;
+      }
+}
+
+/* Single swap rate from discount ratios of one step, written at step_out.
+ * gold/LMM_evolver_gold.h:554-584 (swap_rate_computation_gold). */
+void orc_swap_rate(const double *discounts, const double *taus, int start_index, int end_index,
+                   int paths, int step_in, int step_out, int n_rates, double *swap_rate) {
+  for (int p = 0; p < paths; ++p) {
+    const double *D = discounts + (size_t)step_in * (n_rates + 1) * paths + p;
+    double num = D[(size_t)start_index * paths] - D[(size_t)end_index * paths];
+    double den = 0.0;
+    for (int r = start_index; r < end_index; ++r) den += taus[r] * D[(size_t)(r + 1) * paths];
+    swap_rate[(size_t)step_out * paths + p] = num / den;
+  }
+}
+
+/* B(0) = 1; B(i) = B(i-1) P(i-1,i-1) / P(i-1,i).
+ * gold/LMM_evolver_gold.h:586-604 (spot_measure_numeraires_computation_gold). */
+void orc_spot_numeraires(const double *discounts, int paths, int n_rates, double *numeraires) {
+  for (int p = 0; p < paths; ++p) {
+    double b = 1.0;
+    numeraires[p] = b;
+    for (int r = 1; r < n_rates; ++r) {
+      const double *D = discounts + (size_t)(r - 1) * (n_rates + 1) * paths + p;
+      b *= D[(size_t)(r - 1) * paths] / D[(size_t)r * paths];
+      numeraires[(size_t)r * paths + p] = b;
+    }
+  }
+}
+
+/* out(s,p) = f(s, indices[s], p).
+ * gold/LMM_evolver_gold.h:642-668 (forward_rate_extraction_gold). */
+void orc_forward_extraction(const double *forwards, const int *indices, int paths, int steps,
+                            int n_rates, double *out) {
+  for (int s = 0; s < steps; ++s)
+    for (int p = 0; p < paths; ++p)
+      out[(size_t)s * paths + p] = forwards[((size_t)s * n_rates + indices[s]) * paths + p];
+}
+
+/* ======================================================================
+ * Stage (c)
+ * ==================================================================== */
+
+/* Swap: aux = [K, payReceive, accrual1[S], accrual2[S]];
+ * flow1 = -K acc1 pr, flow2 = acc2 rate1 pr.
+ * cashFlowGeneration_product_gpu.h:128-188 (class Swap) with the driver loop of
+ * cashFlowGeneration_gpu.h:23-77; gold twin
+ * gold/cashFlowGeneration_product_gold.h. */
+void orc_swap_flows(double *flows1, double *flows2, const double *aux, int paths, int steps,
+                    const double *rates1) {
+  double K = aux[0], pr = aux[1];
+  for (int p = 0; p < paths; ++p)
+    for (int s = 0; s < steps; ++s) {
+      flows1[(size_t)s * paths + p] = -K * aux[2 + s] * pr;
+      flows2[(size_t)s * paths + p] = aux[2 + steps + s] * rates1[(size_t)s * paths + p] * pr;
+    }
+}
+
+/* TARN: aux = [totalCoupon, K, m, accrual1[S], accrual2[S]].
+ * cashFlowGeneration_product_gpu.h:19-108 (class TARN). */
+void orc_tarn_flows(double *flows1, double *flows2, const double *aux, int paths, int steps,
+                    const double *rates1, const double *rates2) {
+  double total = aux[0], K = aux[1], m = aux[2];
+  const double *acc1 = aux + 3, *acc2 = aux + 3 + steps;
+  for (int p = 0; p < paths; ++p) {
+    double paid = 0.0;
+    int done = 0, index = 0;
+    for (int s = 0; s < steps; ++s) {
+      double fl1 = 0.0, fl2 = 0.0;
+      if (!done) {
+        fl2 = acc2[index] * rates2[(size_t)s * paths + p];
+        if (index + 1 == steps) {
+          fl1 = paid - total;
+          done = 1;
+        } else {
+          double unc = (K - m * rates1[(size_t)s * paths + p]) * acc1[index];
+          if (unc < 0.0) unc = 0.0;
+          double nt = paid + unc;
+          if (nt >= total) {
+            done = 1;
+            fl1 = paid - total;
+          } else {
+            fl1 = -unc;
+            paid = nt;
+            ++index;
+          }
+        }
+      }
+      flows1[(size_t)s * paths + p] = fl1;
+      flows2[(size_t)s * paths + p] = fl2;
+    }
+  }
+}
+
+/* Caplet strip (no reference counterpart; same interface as Swap):
+ * aux = [K, accrual[S]]; flow1 = accrual max(rate1 - K, 0), flow2 = 0. */
+void orc_caplet_flows(double *flows1, double *flows2, const double *aux, int paths, int steps,
+                      const double *rates1) {
+  double K = aux[0];
+  for (int p = 0; p < paths; ++p)
+    for (int s = 0; s < steps; ++s) {
+      double x = rates1[(size_t)s * paths + p] - K;
+      flows1[(size_t)s * paths + p] = aux[1 + s] * (x > 0.0 ? x : 0.0);
+      flows2[(size_t)s * paths + p] = 0.0;
+    }
+}
+
+/* Payment times -> bracketing rate-time indices and log-linear weights.
+ * gold/cashFlowDiscounting_gold.h:210-245 (generateCashFlowIndicesAndWeights). */
+void orc_cash_flow_indices_weights(const double *rate_times, const double *payment_times,
+                                   int n_times, int *first_index, int *second_index,
+                                   double *thetas) {
+  for (int i = 0; i < n_times; ++i) {
+    double t = payment_times[i];
+    if (t < rate_times[0]) {
+      first_index[i] = 0;
+      second_index[i] = 1;
+    } else if (t > rate_times[n_times - 1]) {
+      first_index[i] = n_times - 1;
+      second_index[i] = n_times;
+    } else {
+      int j = 1;
+      while (t > rate_times[j]) ++j;
+      first_index[i] = j - 1;
+      second_index[i] = j;
+    }
+    thetas[i] = (t - rate_times[first_index[i]]) /
+                (rate_times[second_index[i]] - rate_times[first_index[i]]);
+  }
+}
+
+/* df = (P1/P_s)^(1-theta) (P2/P_s)^theta / B_s, plus the per-path sum.
+ * gold/cashFlowDiscounting_gold.h:20-95 (cashFlowDiscounting_gold). */
+void orc_cash_flow_discounting(const int *first_index, const int *second_index,
+                               const double *thetas, const double *discounts,
+                               const double *flows, const double *numeraires, int paths,
+                               int steps, double *discounted, double *summed) {
+  for (int p = 0; p < paths; ++p) {
+    double total = 0.0;
+    for (int s = 0; s < steps; ++s) {
+      const double *D = discounts + (size_t)s * (steps + 1) * paths + p;
+      double b = D[(size_t)s * paths];
+      double r1 = D[(size_t)first_index[s] * paths] / b;
+      double r2 = D[(size_t)second_index[s] * paths] / b;
+      double df = pow(r1, 1.0 - thetas[s]) * pow(r2, thetas[s]);
+      double cf = flows[(size_t)s * paths + p] * df / numeraires[(size_t)s * paths + p];
+      discounted[(size_t)s * paths + p] = cf;
+      total += cf;
+    }
+    summed[p] = total;
+  }
+}
+
+/* Discount each flow to its generation step with the curve of that step, then
+ * to targetIndices[i] with the numeraire ratio.
+ * cashFlowDiscounting_gpu.cu:369-440 (cashFlowDiscounting_partial_kernel),
+ * which divides the interpolated ratio once more by P(i,i) (== 1 for the
+ * evolver's discount ratios); gold twin gold/cashFlowDiscounting_gold.h:97-205. */
+void orc_cash_flow_discounting_partial(int flows_per_path, const int *gen_indices,
+                                       const int *first_index, const int *second_index,
+                                       const double *thetas, const int *target_indices,
+                                       const double *discounts, const double *flows,
+                                       const double *numeraires, int paths, int steps,
+                                       double *discounted) {
+  for (int p = 0; p < paths; ++p)
+    for (int k = 0; k < flows_per_path; ++k) {
+      int i = gen_indices[k];
+      const double *D = discounts + (size_t)i * (steps + 1) * paths + p;
+      double now = D[(size_t)i * paths];
+      double r1 = D[(size_t)first_index[k] * paths] / now;
+      double r2 = D[(size_t)second_index[k] * paths] / now;
+      double rdf = pow(r1, 1.0 - thetas[k]) * pow(r2, thetas[k]);
+      double fl = flows[(size_t)k * paths + p] * rdf / now;
+      fl *= numeraires[(size_t)target_indices[k] * paths + p] / numeraires[(size_t)i * paths + p];
+      discounted[(size_t)k * paths + p] = fl;
+    }
+}
+
+/* Neumaier-compensated sum (reference sums are thrust::reduce in FP32;
+ * the oracle sums exactly enough that its own error is negligible). */
+double orc_sum_compensated(const double *x, int64_t n) {
+  double s = 0.0, c = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double t = s + x[i];
+    if (fabs(s) >= fabs(x[i]))
+      c += (s - t) + x[i];
+    else
+      c += (x[i] - t) + s;
+    s = t;
+  }
+  return s + c;
+}
+
+/* ======================================================================
+ * Host set-up: correlation and pseudo-roots
+ * ==================================================================== */
+
+/* Cyclic Jacobi eigen-decomposition of a symmetric matrix; eigenvalues in
+ * descending order; each eigenvector's largest-magnitude component made
+ * positive.  Plays the role of gold/math/Schur_gold.cpp
+ * (SymmetricSchurDecomposition). */
+int orc_sym_eigen(int n, const double *a, double *evals, double *evecs) {
+  double *m = (double *)malloc(sizeof(double) * n * n);
+  double *v = (double *)malloc(sizeof(double) * n * n);
+  memcpy(m, a, sizeof(double) * n * n);
+  for (int i = 0; i < n * n; ++i) v[i] = 0.0;
+  for (int i = 0; i < n; ++i) v[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        tot += m[i * n + j] * m[i * n + j];
+        if (i != j) off += m[i * n + j] * m[i * n + j];
+      }
+    if (off <= 1e-30 * (tot > 0 ? tot : 1.0)) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = m[p * n + q];
+        if (fabs(apq) < 1e-300) continue;
+        double app = m[p * n + p], aqq = m[q * n + q];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          double mkp = m[k * n + p], mkq = m[k * n + q];
+          m[k * n + p] = c * mkp - s * mkq;
+          m[k * n + q] = s * mkp + c * mkq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double mpk = m[p * n + k], mqk = m[q * n + k];
+          m[p * n + k] = c * mpk - s * mqk;
+          m[q * n + k] = s * mpk + c * mqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = v[k * n + p], vkq = v[k * n + q];
+          v[k * n + p] = c * vkp - s * vkq;
+          v[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  int *order = (int *)malloc(sizeof(int) * n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (m[order[j] * n + order[j]] > m[order[i] * n + order[i]]) {
+        int t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+      }
+  for (int c = 0; c < n; ++c) {
+    int src = order[c];
+    evals[c] = m[src * n + src];
+    int imax = 0;
+    for (int k = 1; k < n; ++k)
+      if (fabs(v[k * n + src]) > fabs(v[imax * n + src]) + 1e-12) imax = k;
+    double sg = v[imax * n + src] < 0 ? -1.0 : 1.0;
+    for (int k = 0; k < n; ++k) evecs[k * n + c] = sg * v[k * n + src];
+  }
+  free(order);
+  free(v);
+  free(m);
+  return 0;
+}
+
+/* rho_ij = L + (1-L) exp(-beta |t_i - t_j|).
+ * gold/volstructs_gold.cpp:157-175 (GenerateExpLongCorrMatrix). */
+void orc_exp_corr(const double *times, int n, double L, double beta, double *corr) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      corr[i * n + j] = (i == j) ? 1.0 : L + (1.0 - L) * exp(-beta * fabs(times[i] - times[j]));
+}
+
+/* Top-F eigenpairs scaled by sqrt(lambda), rows renormalised to keep the
+ * diagonal.  gold/math/pseudoSquareRoot.cpp:76-110 (pseudoSqrtSchur). */
+int orc_pseudo_sqrt(int n, const double *cov, int factors, int normalise, double *out) {
+  double *ev = (double *)malloc(sizeof(double) * n);
+  double *vec = (double *)malloc(sizeof(double) * n * n);
+  orc_sym_eigen(n, cov, ev, vec);
+  for (int f = 0; f < factors; ++f) {
+    double rl = ev[f] > 0.0 ? sqrt(ev[f]) : 0.0;
+    for (int i = 0; i < n; ++i) out[i * factors + f] = vec[i * n + f] * rl;
+  }
+  if (normalise)
+    for (int i = 0; i < n; ++i) {
+      double nrm = 0.0;
+      for (int f = 0; f < factors; ++f) nrm += out[i * factors + f] * out[i * factors + f];
+      if (nrm > 0.0) {
+        double adj = sqrt(cov[i * n + i] / nrm);
+        for (int f = 0; f < factors; ++f) out[i * factors + f] *= adj;
+      }
+    }
+  free(vec);
+  free(ev);
+  return 0;
+}
+
+/* Per-step PCA pseudo-roots of the flat-vol covariance over each evolution
+ * step.  gold/volstructs_gold.cpp:177-258 (FlatVolPseudoRootsOfCovariances),
+ * correlation on rate start times rate_times[0..N-1]. */
+int orc_flat_vol_pseudo_roots(const double *rate_times, const double *evol_times,
+                              const double *vols, int n_rates, int steps, int factors, double L,
+                              double beta, double *A) {
+  double *corr = (double *)malloc(sizeof(double) * n_rates * n_rates);
+  double *cov = (double *)malloc(sizeof(double) * n_rates * n_rates);
+  orc_exp_corr(rate_times, n_rates, L, beta, corr);
+  double start = 0.0;
+  for (int s = 0; s < steps; ++s) {
+    for (int r = 0; r < n_rates; ++r) {
+      double tr = fmin(evol_times[s], rate_times[r]) - fmin(start, rate_times[r]);
+      for (int q = 0; q <= r; ++q) {
+        double tq = fmin(evol_times[s], rate_times[q]) - fmin(start, rate_times[q]);
+        double c = corr[r * n_rates + q] * vols[r] * vols[q] * fmin(tr, tq);
+        cov[r * n_rates + q] = cov[q * n_rates + r] = c;
+      }
+    }
+    start = evol_times[s];
+    orc_pseudo_sqrt(n_rates, cov, factors, 1, A + (size_t)s * n_rates * factors);
+  }
+  free(cov);
+  free(corr);
+  return 0;
+}
+
+/* Factor model A(s,r,f) = sigma_r sqrt(dt_s) B(r,f) for alive r (r >= s),
+ * zero otherwise, with B the renormalised rank-F PCA root of the correlation
+ * on rate start times; evolution times are the rate start times.  No
+ * reference counterpart: the vol-row scaling makes d A / d sigma_r = A/sigma_r
+ * exact, which is how vegas are defined. */
+int orc_factor_pseudo_roots(const double *rate_times, const double *vols, int n_rates,
+                            int factors, double L, double beta, double *A, double *B) {
+  double *corr = (double *)malloc(sizeof(double) * n_rates * n_rates);
+  double *b = (double *)malloc(sizeof(double) * n_rates * factors);
+  orc_exp_corr(rate_times, n_rates, L, beta, corr);
+  orc_pseudo_sqrt(n_rates, corr, factors, 1, b);
+  double prev = 0.0;
+  for (int s = 0; s < n_rates; ++s) {
+    double dt = rate_times[s] - prev;
+    prev = rate_times[s];
+    for (int r = 0; r < n_rates; ++r)
+      for (int f = 0; f < factors; ++f)
+        A[((size_t)s * n_rates + r) * factors + f] =
+            r >= s ? vols[r] * sqrt(dt) * b[r * factors + f] : 0.0;
+  }
+  if (B) memcpy(B, b, sizeof(double) * n_rates * factors);
+  free(b);
+  free(corr);
+  return 0;
+}
+
+/* ======================================================================
+ * Fused caplet-portfolio pipeline (price and pathwise adjoint).
+ *
+ * No reference counterpart: the reference lists adjoint Greeks as an open
+ * task (doc/todo.txt:10).  The forward step is the A7 recurrence
+ * (LMM_evolver_all.cu:571-742) with the step-0 drift computed from the
+ * initial rates exactly as the host does for the reference kernel
+ * (kooexample/bermudanIRDPricer.cu:446).  Product: caplet i fixes f_i at
+ * step i and pays tau_i max(f_i - K, 0) at T_{i+1}; it is valued at T_i as
+ * tau_i max(f_i - K, 0) / (1 + tau_i f_i) and deflated by the spot numeraire
+ * B_i = prod_{j<i} (1 + tau_j f_j(T_j)) with B_0 = 1, so prices are in units
+ * of the numeraire at T_0.
+ * ==================================================================== */
+
+typedef struct {
+  double g, mu, gp, mup, x, xp, lp;
+} step_rate;
+
+/* One forward PC step from start log-state L (rates r >= s), writing the
+ * end-of-step log-state into Lout and, if rec != NULL, the per-rate
+ * intermediates and the running sums e(r), e^(r) (F each). */
+static void pc_step(const orc_caplet_model *m, int s, const double *L, const double *z_s,
+                    double *Lout, step_rate *rec, double *erec, double *eprec) {
+  const int N = m->n_rates, F = m->factors;
+  const double *As = m->A + (size_t)s * N * F;
+  double e[64], ep[64];
+  for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0;
+  for (int r = s; r < N; ++r) {
+    const double *Ar = As + r * F;
+    double tau = m->taus[r], d = m->disp[r];
+    double x = exp(L[r]);
+    double g = tau * x / (1.0 + tau * (x - d));
+    double mu = 0.0, dW = 0.0, c = 0.0;
+    for (int f = 0; f < F; ++f) {
+      e[f] += g * Ar[f];
+      mu += Ar[f] * e[f];
+      dW += Ar[f] * z_s[f];
+      c -= 0.5 * Ar[f] * Ar[f];
+    }
+    double lh = L[r] + mu + c + dW;
+    double xp = exp(lh);
+    double gp = tau * xp / (1.0 + tau * (xp - d));
+    double mup = 0.0;
+    for (int f = 0; f < F; ++f) {
+      ep[f] += gp * Ar[f];
+      mup += Ar[f] * ep[f];
+    }
+    Lout[r] = lh + 0.5 * (mup - mu);
+    if (rec) {
+      rec[r].g = g;
+      rec[r].mu = mu;
+      rec[r].gp = gp;
+      rec[r].mup = mup;
+      rec[r].x = x;
+      rec[r].xp = xp;
+      rec[r].lp = lh;
+      for (int f = 0; f < F; ++f) {
+        erec[r * F + f] = e[f];
+        eprec[r * F + f] = ep[f];
+      }
+    }
+  }
+}
+
+static double caplet_value(double f, double tau, double K) {
+  double x = f - K;
+  return x > 0.0 ? tau * x / (1.0 + tau * f) : 0.0;
+}
+
+static double caplet_value_deriv(double f, double tau, double K) {
+  if (f <= K) return 0.0;
+  double den = 1.0 + tau * f;
+  return tau * (1.0 + tau * K) / (den * den);
+}
+
+static double caplet_path_price(const orc_caplet_model *m, const double *z, int p, double *L,
+                                double *L2) {
+  const int N = m->n_rates, F = m->factors, P = m->paths;
+  for (int r = 0; r < N; ++r) L[r] = log(m->f0[r] + m->disp[r]);
+  double B = 1.0, y = 0.0, zs[64];
+  for (int s = 0; s < N; ++s) {
+    for (int f = 0; f < F; ++f) zs[f] = z[((size_t)s * F + f) * P + p];
+    pc_step(m, s, L, zs, L2, NULL, NULL, NULL);
+    for (int r = s; r < N; ++r) L[r] = L2[r];
+    double fs = exp(L[s]) - m->disp[s];
+    y += caplet_value(fs, m->taus[s], m->strike) / B;
+    B *= 1.0 + m->taus[s] * fs;
+  }
+  return y;
+}
+
+void orc_caplet_price_paths(const orc_caplet_model *m, const double *z, double *price_per_path) {
+  const int N = m->n_rates;
+#pragma omp parallel
+  {
+    double *L = (double *)malloc(sizeof(double) * N * 2);
+#pragma omp for schedule(static)
+    for (int p = 0; p < m->paths; ++p) price_per_path[p] = caplet_path_price(m, z, p, L, L + N);
+    free(L);
+  }
+}
+
+/* Reverse sweep of one path; adds d(price)/d f0 into delta and
+ * sum_{s,f} Abar(s,r,f) A(s,r,f) into vw (vega numerator); returns price. */
+static double caplet_path_adjoint(const orc_caplet_model *m, const double *z, int p,
+                                  double *states /* (S+1)*N */, double *lam, double *delta,
+                                  double *vw, step_rate *rec, double *erec, double *eprec) {
+  const int N = m->n_rates, F = m->factors, P = m->paths;
+  double *Bs = states + (size_t)(N + 1) * N; /* N numeraires */
+  for (int r = 0; r < N; ++r) states[r] = log(m->f0[r] + m->disp[r]);
+  double B = 1.0, y = 0.0, zs[64];
+  for (int s = 0; s < N; ++s) {
+    for (int f = 0; f < F; ++f) zs[f] = z[((size_t)s * F + f) * P + p];
+    double *Ls = states + (size_t)s * N, *Ln = states + (size_t)(s + 1) * N;
+    for (int r = 0; r < N; ++r) Ln[r] = Ls[r];
+    pc_step(m, s, Ls, zs, Ln, NULL, NULL, NULL);
+    Bs[s] = B;
+    double fs = exp(Ln[s]) - m->disp[s];
+    y += caplet_value(fs, m->taus[s], m->strike) / B;
+    B *= 1.0 + m->taus[s] * fs;
+  }
+  for (int r = 0; r < N; ++r) lam[r] = 0.0;
+  double tail = 0.0;
+  for (int s = N - 1; s >= 0; --s) {
+    for (int f = 0; f < F; ++f) zs[f] = z[((size_t)s * F + f) * P + p];
+    const double *Ls = states + (size_t)s * N;
+    double Lend[64];
+    pc_step(m, s, Ls, zs, Lend, rec, erec, eprec);
+    /* payoff and numeraire adjoint of the rate fixing at the end of step s */
+    double xs = exp(Lend[s]), fs = xs - m->disp[s], tau = m->taus[s];
+    double v = caplet_value(fs, tau, m->strike);
+    double dy = caplet_value_deriv(fs, tau, m->strike) / Bs[s] - tau / (1.0 + tau * fs) * tail;
+    lam[s] += dy * xs;
+    tail += v / Bs[s];
+    /* reverse of the PC step */
+    const double *As = m->A + (size_t)s * N * F;
+    double Eh[64], E[64];
+    for (int f = 0; f < F; ++f) Eh[f] = E[f] = 0.0;
+    for (int r = N - 1; r >= s; --r) {
+      const double *Ar = As + r * F;
+      double d = m->disp[r];
+      double lp = lam[r];
+      step_rate q = rec[r];
+      double gph = 0.0;
+      for (int f = 0; f < F; ++f) {
+        Eh[f] += 0.5 * lp * Ar[f];
+        gph += Ar[f] * Eh[f];
+      }
+      double fh = q.xp - d;
+      double lh = lp + gph * q.gp * (1.0 - m->taus[r] * d) / (1.0 + m->taus[r] * fh);
+      double mub = lh - 0.5 * lp;
+      double gb = 0.0;
+      for (int f = 0; f < F; ++f) {
+        E[f] += mub * Ar[f];
+        gb += Ar[f] * E[f];
+      }
+      double fr = q.x - d;
+      double lnew = lh + gb * q.g * (1.0 - m->taus[r] * d) / (1.0 + m->taus[r] * fr);
+      double acc = 0.0;
+      for (int f = 0; f < F; ++f) {
+        double abar = 0.5 * lp * eprec[r * F + f] + q.gp * Eh[f] + mub * erec[r * F + f] +
+                      q.g * E[f] + lh * zs[f] - lh * Ar[f];
+        acc += abar * Ar[f];
+      }
+      vw[r] += acc;
+      lam[r] = lnew;
+    }
+  }
+  for (int r = 0; r < N; ++r) delta[r] += lam[r] / (m->f0[r] + m->disp[r]);
+  return y;
+}
+
+void orc_caplet_adjoint_paths(const orc_caplet_model *m, const double *z, double *price_per_path,
+                              double *delta_sum, double *vega_sum) {
+  const int N = m->n_rates, F = m->factors;
+  for (int r = 0; r < N; ++r) delta_sum[r] = vega_sum[r] = 0.0;
+#pragma omp parallel
+  {
+    double *states = (double *)malloc(sizeof(double) * ((size_t)(N + 1) * N + N));
+    double *lam = (double *)malloc(sizeof(double) * N);
+    double *dl = (double *)calloc(N, sizeof(double));
+    double *vw = (double *)calloc(N, sizeof(double));
+    step_rate *rec = (step_rate *)malloc(sizeof(step_rate) * N);
+    double *erec = (double *)malloc(sizeof(double) * N * F * 2);
+#pragma omp for schedule(static)
+    for (int p = 0; p < m->paths; ++p)
+      price_per_path[p] = caplet_path_adjoint(m, z, p, states, lam, dl, vw, rec, erec, erec + N * F);
+#pragma omp critical
+    {
+      for (int r = 0; r < N; ++r) {
+        delta_sum[r] += dl[r];
+        vega_sum[r] += vw[r] / m->sigma[r];
+      }
+    }
+    free(erec);
+    free(rec);
+    free(vw);
+    free(dl);
+    free(lam);
+    free(states);
+  }
+}
+
+/* Integers -> FP64 normals -> diagonal-allocation bridge -> price (and
+ * adjoint Greeks): the whole path in FP64 on the same Sobol points.  Outputs
+ * are means over paths. */
+void orc_caplet_pipeline(const orc_caplet_model *m, const uint32_t *ints, int do_adjoint,
+                         double *price, double *deltas, double *vegas) {
+  const int N = m->n_rates, F = m->factors, P = m->paths;
+  const size_t D = (size_t)N * F;
+  double *u = (double *)malloc(sizeof(double) * D * P);
+  double *z = (double *)malloc(sizeof(double) * D * P);
+  double *pp = (double *)malloc(sizeof(double) * P);
+  int *idx = (int *)malloc(sizeof(int) * D);
+  orc_invnorm_u32_array(ints, u, (int64_t)D * P);
+  orc_bridge_allocate_diagonal(F, N, idx);
+  orc_bridge_transform(N, F, P, idx, u, z);
+  if (do_adjoint) {
+    orc_caplet_adjoint_paths(m, z, pp, deltas, vegas);
+    for (int r = 0; r < N; ++r) {
+      deltas[r] /= P;
+      vegas[r] /= P;
+    }
+  } else {
+    orc_caplet_price_paths(m, z, pp);
+  }
+  *price = orc_sum_compensated(pp, P) / P;
+  free(idx);
+  free(pp);
+  free(z);
+  free(u);
+}

@@ -1,0 +1,119 @@
+/*
+ * koo_oracle.h -- FP64 CPU oracle for the LMM Monte Carlo pricing path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links or calls
+ * this code: it is loaded by the tests, by __graft_entry__.smoke() and by the
+ * cpu_baseline leg of bench.py, and only as the checker.  Every function is a
+ * plain-C restatement of a reference gold routine or kernel (cited per
+ * function in koo_oracle.c), computed in double precision.
+ *
+ * Layout conventions (the reference's, "path-minor"):
+ *   Sobol integers / normals       [D][P]          D = steps*factors
+ *   bridged increments             (S,F,P)
+ *   correlated increments          (S,N,P)
+ *   rates, log-rates               (S,N,P)
+ *   discount ratios                (S,N+1,P)
+ *   pseudo-roots A                 (S,N,F)
+ *   fixed drifts                   (S,N)
+ *   spot-measure numeraires        (S,P)   (S == N)
+ */
+#ifndef KOO_ORACLE_H
+#define KOO_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- stage (a): Sobol, scramble, inverse normal ---------------------- */
+int orc_sobol_directions(int n_dims, uint32_t *dirs /* [n_dims][32] */);
+void orc_sobol_ints(int n_vectors, int n_dims, uint32_t path_offset,
+                    const uint32_t *dirs, uint32_t *out /* [n_dims][n_vectors] */);
+void orc_scramble(uint32_t *data, const uint32_t *scramblers, int number_dim, int paths);
+float orc_invnorm_sb_ref_f32(uint32_t x);  /* reference FP32 formula, verbatim semantics */
+double orc_invnorm_u32(uint32_t x);          /* accurate FP64 quantile of the product's u(x) */
+double orc_invnorm(double u);                /* accurate FP64 quantile, 0<u<1 */
+void orc_invnorm_u32_array(const uint32_t *in, double *out, int64_t n);
+
+/* ---- stage (b): bridge, correlate, evolve ----------------------------- */
+void orc_bridge_allocate_diagonal(int factors, int steps, int *indices /* [F][S] */);
+void orc_bridge_tables(int steps, int *bridge_index, int *left_index, int *right_index,
+                       double *left_weight, double *right_weight, double *std_dev);
+void orc_bridge_transform(int steps, int factors, int paths, const int *indices,
+                          const double *in /* [D][P] */, double *out /* (S,F,P) */);
+void orc_correlate_drift(const double *in /* (S,P,F) */, double *out /* (S,N,P) */,
+                         const double *A, const int *alive, const double *drift_fixed,
+                         int a_offset_per_step, int factors, int out_dims, int paths, int steps);
+void orc_spot_drifts(const double *A_step /* (N,F) */, const double *rates, const double *taus,
+                     const double *displacements, int factors, int n_rates, int alive,
+                     double *drifts);
+void orc_lmm_pc_evolve(const double *initial_rates, const double *initial_log_rates,
+                       const double *taus, const double *variates, const double *A,
+                       const double *initial_drifts, const double *fixed_drifts,
+                       const double *displacements, const int *alive, int paths, int factors,
+                       int steps, int rates, int transposed_variates,
+                       double *rates_out, double *log_rates_out, double *discounts_out);
+void orc_discount_ratios(const double *rates, const double *taus, const int *alive, int paths,
+                         int steps, int n_rates, double *discounts);
+void orc_coterminal_annuities(const double *discounts, const double *taus, const int *alive,
+                              int paths, int steps, int n_rates, double *annuities);
+void orc_coterminal_swap_rates(const double *discounts, const double *annuities, const int *alive,
+                               int paths, int steps, int n_rates, double *swap_rates);
+void orc_swap_rate(const double *discounts, const double *taus, int start_index, int end_index,
+                   int paths, int step_in, int step_out, int n_rates, double *swap_rate);
+void orc_spot_numeraires(const double *discounts, int paths, int n_rates, double *numeraires);
+void orc_forward_extraction(const double *forwards, const int *indices, int paths, int steps,
+                            int n_rates, double *out);
+
+/* ---- stage (c): cash flows, discounting, reductions ------------------- */
+void orc_swap_flows(double *flows1, double *flows2, const double *aux, int paths, int steps,
+                    const double *rates1);
+void orc_tarn_flows(double *flows1, double *flows2, const double *aux, int paths, int steps,
+                    const double *rates1, const double *rates2);
+void orc_caplet_flows(double *flows1, double *flows2, const double *aux, int paths, int steps,
+                      const double *rates1);
+void orc_cash_flow_indices_weights(const double *rate_times, const double *payment_times,
+                                   int n_times, int *first_index, int *second_index,
+                                   double *thetas);
+void orc_cash_flow_discounting(const int *first_index, const int *second_index,
+                               const double *thetas, const double *discounts,
+                               const double *flows, const double *numeraires, int paths,
+                               int steps, double *discounted, double *summed);
+void orc_cash_flow_discounting_partial(int flows_per_path, const int *gen_indices,
+                                       const int *first_index, const int *second_index,
+                                       const double *thetas, const int *target_indices,
+                                       const double *discounts, const double *flows,
+                                       const double *numeraires, int paths, int steps,
+                                       double *discounted);
+double orc_sum_compensated(const double *x, int64_t n);
+
+/* ---- host set-up: correlation, PCA pseudo-roots ----------------------- */
+int orc_sym_eigen(int n, const double *a /* n*n */, double *evals, double *evecs /* n*n, col=vec */);
+void orc_exp_corr(const double *times, int n, double L, double beta, double *corr);
+int orc_pseudo_sqrt(int n, const double *cov, int factors, int normalise, double *out /* n*F */);
+int orc_flat_vol_pseudo_roots(const double *rate_times /* N+1 */, const double *evol_times /* S */,
+                              const double *vols, int n_rates, int steps, int factors, double L,
+                              double beta, double *A /* (S,N,F) */);
+int orc_factor_pseudo_roots(const double *rate_times /* N+1 */, const double *vols, int n_rates,
+                            int factors, double L, double beta, double *A /* (S,N,F) */,
+                            double *B /* (N,F) or NULL */);
+
+/* ---- fused pipeline: caplet portfolio price and pathwise adjoint ------ */
+typedef struct {
+  int n_rates, factors, paths;
+  const double *f0, *taus, *disp, *A, *sigma; /* A (S,N,F), S = N */
+  double strike;
+} orc_caplet_model;
+
+void orc_caplet_price_paths(const orc_caplet_model *m, const double *z /* (S,F,P) */,
+                            double *price_per_path);
+void orc_caplet_adjoint_paths(const orc_caplet_model *m, const double *z,
+                              double *price_per_path, double *delta_sum /* N */,
+                              double *vega_sum /* N */);
+void orc_caplet_pipeline(const orc_caplet_model *m, const uint32_t *ints /* [D][P] */,
+                         int do_adjoint, double *price, double *deltas, double *vegas);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

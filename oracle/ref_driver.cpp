@@ -1,0 +1,194 @@
+// ref_driver.cpp -- C entry points onto the reference's own gold CPU routines.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile.ref against the
+// reference sources where they lie (kooderive-code-r256/LINUX/gold, the
+// Linux-portable copy of gold/) into oracle/_ref/libkoo_ref.so.  The tests
+// use it to pin the FP64 restatement in koo_oracle.c against the reference's
+// actual code; the product never touches it.  Every function below only
+// marshals plain arrays into the gold routine named in its comment.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <gold/Bridge_allocator_gold.h>
+#include <gold/LMM_evolver_gold.h>
+#include <gold/cashFlowDiscounting_gold.h>
+#include <gold/cashFlowGeneration_product_gold.h>
+#include <gold/correlate_drift_gold.h>
+#include <gold/math/Normals_gold.h>
+#include <gold/scramble_gold.h>
+#include <gold/sobol_gold.h>
+#include <gold/volstructs_gold.h>
+
+template <class T>
+static std::vector<T> vec(const T* p, size_t n) { return std::vector<T>(p, p + n); }
+
+extern "C" {
+
+// gold/sobol_gold.cpp:118 initSobolDirectionVectors
+void ref_sobol_directions(int n_dims, unsigned* dirs) { initSobolDirectionVectors(n_dims, dirs); }
+
+// gold/sobol_gold.cpp:191 sobolCPUintsOffset
+void ref_sobol_ints_offset(int n_vectors, int n_dims, int offset, unsigned* dirs, unsigned* out) {
+  sobolCPUintsOffset(n_vectors, n_dims, offset, dirs, out);
+}
+
+// gold/scramble_gold.cpp:11 scramble_gold
+void ref_scramble(const unsigned* in, unsigned* out, const unsigned* scr, int dims, int paths) {
+  std::vector<unsigned> i(in, in + (size_t)dims * paths), o((size_t)dims * paths), s(scr, scr + dims);
+  scramble_gold(i, o, s, dims, paths);
+  std::memcpy(out, o.data(), o.size() * sizeof(unsigned));
+}
+
+// gold/math/normals_gold.h:191 inverseCumulativeShawBrickmanUnsignedIntGold<float>
+float ref_invnorm_sb(unsigned x) { return inverseCumulativeShawBrickmanUnsignedIntGold<float>()(x); }
+
+// gold/Bridge_allocator_gold.cpp:38 bridge_allocate_diagonal
+void ref_bridge_allocate_diagonal(int factors, int steps, int* idx) {
+  std::vector<int> v(factors * steps);
+  bridge_allocate_diagonal(v, factors, steps);
+  std::memcpy(idx, v.data(), v.size() * sizeof(int));
+}
+
+// gold/correlate_drift_gold.h:22 correlate_drift_paths_gold<double>
+void ref_correlate_drift(const double* in, double* out, const double* A, int a_off, const double* drift,
+                         int factors, int out_dims, int paths, int steps) {
+  std::vector<double> i = vec(in, (size_t)steps * paths * factors);
+  std::vector<double> o((size_t)steps * out_dims * paths);
+  std::vector<double> a = vec(A, (size_t)steps * a_off);
+  std::vector<double> d = vec(drift, (size_t)steps * out_dims);
+  correlate_drift_paths_gold<double>(i, o, a, a_off, d, factors, out_dims, paths, steps);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:178 spotDriftComputer<double>::getDrifts
+void ref_spot_drifts(const double* A_step, const double* rates, const double* taus, const double* disp,
+                     int factors, int n_rates, double* drifts) {
+  std::vector<double> t = vec(taus, n_rates), d = vec(disp, n_rates), r = vec(rates, n_rates);
+  std::vector<double> out(n_rates);
+  spotDriftComputer<double> dc(t, factors, d);
+  MatrixConstFacade<double> A(A_step, n_rates, factors);
+  dc.getDrifts(A, r, out);
+  std::memcpy(drifts, out.data(), n_rates * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:391 LMM_evolver_pc_gold<double>
+void ref_lmm_pc(const double* f0, const double* taus, const double* incr, const double* A, const double* disp,
+                int paths, int factors, int steps, int rates, double* rates_out, double* logs_out) {
+  std::vector<double> i = vec(f0, rates), t = vec(taus, rates), d = vec(disp, rates);
+  std::vector<double> c = vec(incr, (size_t)steps * rates * paths);
+  std::vector<double> a = vec(A, (size_t)steps * rates * factors);
+  std::vector<double> ro, lo;
+  LMM_evolver_pc_gold<double>(i, t, c, a, d, paths, factors, steps, rates, ro, lo, 100.0);
+  std::memcpy(rates_out, ro.data(), ro.size() * sizeof(double));
+  std::memcpy(logs_out, lo.data(), lo.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:460 discount_ratios_computation_gold<double>
+void ref_discount_ratios(const double* rates, const double* taus, const int* alive, int paths, int steps,
+                         int n_rates, double* discounts) {
+  std::vector<double> r = vec(rates, (size_t)steps * n_rates * paths), t = vec(taus, n_rates);
+  std::vector<int> al = vec(alive, steps);
+  std::vector<double> out((size_t)steps * (n_rates + 1) * paths, 0.0);
+  discount_ratios_computation_gold<double>(r, t, al, paths, steps, n_rates, out, 0);
+  std::memcpy(discounts, out.data(), out.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:494 coterminal_annuity_ratios_computation_gold<double>
+void ref_annuities(const double* disc, const double* taus, const int* alive, int paths, int steps, int n_rates,
+                   double* ann) {
+  std::vector<double> d = vec(disc, (size_t)steps * (n_rates + 1) * paths), t = vec(taus, n_rates);
+  std::vector<int> al = vec(alive, steps);
+  std::vector<double> out((size_t)steps * n_rates * paths, 0.0);
+  coterminal_annuity_ratios_computation_gold<double>(d, t, al, paths, steps, n_rates, out, 0);
+  std::memcpy(ann, out.data(), out.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:525 coterminal_swap_rates_computation_gold<double>
+void ref_cot_swap_rates(const double* disc, const double* ann, const int* alive, int paths, int steps,
+                        int n_rates, double* sr) {
+  std::vector<double> d = vec(disc, (size_t)steps * (n_rates + 1) * paths);
+  std::vector<double> a = vec(ann, (size_t)steps * n_rates * paths);
+  std::vector<int> al = vec(alive, steps);
+  std::vector<double> out((size_t)steps * n_rates * paths, 0.0);
+  coterminal_swap_rates_computation_gold<double>(d, a, al, paths, steps, n_rates, out, 0);
+  std::memcpy(sr, out.data(), out.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:560 swap_rate_computation_gold<double>
+void ref_swap_rate(const double* disc, const double* taus, int start, int end, int paths, int step_in,
+                   int step_out, int steps, int n_rates, double* out) {
+  std::vector<double> d = vec(disc, (size_t)steps * (n_rates + 1) * paths), t = vec(taus, n_rates);
+  std::vector<double> o((size_t)steps * paths, 0.0);
+  swap_rate_computation_gold<double>(d, t, start, end, paths, step_in, step_out, steps, n_rates, o);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:614 spot_measure_numeraires_computation_offset_gold<double>
+void ref_numeraires(const double* disc, int paths, int n_rates, double* out) {
+  std::vector<double> d = vec(disc, (size_t)n_rates * (n_rates + 1) * paths);
+  std::vector<double> o((size_t)n_rates * paths, 0.0);
+  spot_measure_numeraires_computation_offset_gold<double>(d, paths, paths, 0, n_rates, o);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:642 forward_rate_extraction_gold<double>
+void ref_forward_extraction(const double* fwd, const int* idx, int paths, int steps, int n_rates, double* out) {
+  std::vector<double> f = vec(fwd, (size_t)steps * n_rates * paths);
+  std::vector<int> ix = vec(idx, steps);
+  std::vector<double> o((size_t)steps * paths, 0.0);
+  forward_rate_extraction_gold<double>(f, ix, paths, steps, n_rates, o, 0);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// gold/cashFlowGeneration_product_gold.h cashFlowGeneratorCallerSwap_gold<double>
+void ref_swap_flows(double* f1, double* f2, const double* aux, int aux_n, int paths, int steps, const double* r1) {
+  std::vector<double> o1((size_t)steps * paths), o2((size_t)steps * paths), a = vec(aux, aux_n);
+  std::vector<double> rr = vec(r1, (size_t)steps * paths), dummy(1, 0.0);
+  cashFlowGeneratorCallerSwap_gold<double>(o1, o2, a, paths, steps, rr, rr, rr, dummy, dummy);
+  std::memcpy(f1, o1.data(), o1.size() * sizeof(double));
+  std::memcpy(f2, o2.data(), o2.size() * sizeof(double));
+}
+
+// gold/cashFlowGeneration_product_gold.h cashFlowGeneratorCallerTARN_gold<double>
+void ref_tarn_flows(double* f1, double* f2, const double* aux, int aux_n, int paths, int steps, const double* r1,
+                    const double* r2) {
+  std::vector<double> o1((size_t)steps * paths), o2((size_t)steps * paths), a = vec(aux, aux_n);
+  std::vector<double> x1 = vec(r1, (size_t)steps * paths), x2 = vec(r2, (size_t)steps * paths), dummy(1, 0.0);
+  cashFlowGeneratorCallerTARN_gold<double>(o1, o2, a, paths, steps, x1, x2, x2, dummy, dummy);
+  std::memcpy(f1, o1.data(), o1.size() * sizeof(double));
+  std::memcpy(f2, o2.data(), o2.size() * sizeof(double));
+}
+
+// gold/cashFlowDiscounting_gold.h:210 generateCashFlowIndicesAndWeights<double>
+void ref_cash_flow_indices_weights(const double* rate_times, int n_rate_times, const double* pay, int n,
+                                   int* first, int* second, double* thetas) {
+  std::vector<int> a, b;
+  std::vector<double> th, rt = vec(rate_times, n_rate_times), pt = vec(pay, n);
+  generateCashFlowIndicesAndWeights<double>(a, b, th, rt, pt);
+  std::memcpy(first, a.data(), n * sizeof(int));
+  std::memcpy(second, b.data(), n * sizeof(int));
+  std::memcpy(thetas, th.data(), n * sizeof(double));
+}
+
+// gold/cashFlowDiscounting_gold.h:20 cashFlowDiscounting_gold<double>
+void ref_cash_flow_discounting(const int* first, const int* second, const double* thetas, const double* disc,
+                               const double* flows, const double* num, int paths, int steps, double* out,
+                               double* summed) {
+  std::vector<int> a = vec(first, steps), b = vec(second, steps);
+  std::vector<double> th = vec(thetas, steps), d = vec(disc, (size_t)steps * (steps + 1) * paths);
+  std::vector<double> f = vec(flows, (size_t)steps * paths), nv = vec(num, (size_t)steps * paths), o, s;
+  cashFlowDiscounting_gold<double>(a, b, th, d, f, nv, paths, steps, o, s, 0);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+  std::memcpy(summed, s.data(), s.size() * sizeof(double));
+}
+
+// gold/volstructs_gold.cpp:236 FlatVolPseudoRootsOfCovariances (exp-long correlation)
+void ref_flat_vol_pseudo_roots(const double* rate_times, const double* evol_times, const double* vols,
+                               int n_rates, int steps, int factors, double L, double beta, double* A) {
+  std::vector<double> rt = vec(rate_times, n_rates + 1), et = vec(evol_times, steps), v = vec(vols, n_rates);
+  Cube_gold<double> c = FlatVolPseudoRootsOfCovariances(rt, et, v, factors, L, beta);
+  std::memcpy(A, &c(0, 0, 0), sizeof(double) * steps * n_rates * factors);
+}
+
+}  // extern "C"

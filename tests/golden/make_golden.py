@@ -1,0 +1,146 @@
+"""Writes the golden fixtures in this folder from the reference's own gold CPU
+routines, compiled from the reference sources by oracle/Makefile into
+oracle/_ref/libkoo_ref.so.  Run in a container that has the reference tree:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Fixtures are data only (inputs and the reference's outputs), small, and
+seeded; tests/test_oracle_golden.py checks the oracle against them and the GPU
+tests check the library against the oracle.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+
+
+def sobol(R):
+    D, P = 200, 64
+    dirs = np.zeros(210 * 32, np.uint32)
+    R.ref_sobol_directions(210, dirs)
+    out = {"dirs210": dirs}
+    for off in (0, 1000, 65535):
+        o = np.zeros(D * P, np.uint32)
+        R.ref_sobol_ints_offset(P, D, off, dirs[: D * 32].copy(), o)
+        out[f"ints_off{off}"] = o.reshape(D, P)
+    # scramble: the reference test's fixture formula (scramble_test.cu:74-88)
+    Ds, Ps = 16, 256
+    data = (np.arange(Ds, dtype=np.uint64)[:, None] * Ps + np.arange(Ps)[None, :]).astype(np.uint32)
+    scr = np.array([6 * i * i + 2 * i + 1 for i in range(Ds)], np.uint32)
+    o = np.zeros(Ds * Ps, np.uint32)
+    R.ref_scramble(data.reshape(-1).copy(), o, scr, Ds, Ps)
+    out.update(scramble_in=data, scramblers=scr, scramble_out=o.reshape(Ds, Ps))
+    return out
+
+
+def invnorm(R):
+    rng = np.random.RandomState(7)
+    x = np.concatenate([np.array([1, 2, 3, 1000, 2**31 - 1, 2**31, 2**31 + 1, 2**32 - 129,
+                                  2**32 - 256, 2**24, 2**20], np.uint64),
+                        rng.randint(0, 2**32, size=4096, dtype=np.uint64)]).astype(np.uint32)
+    y = np.array([R.ref_invnorm_sb(int(v)) for v in x], np.float32)
+    return {"x": x, "sb_ref_f32": y}
+
+
+def lmm(R):
+    N, F, P = 10, 3, 32
+    S = N
+    rng = np.random.RandomState(11)
+    tau = 0.5
+    rate_times = tau * np.arange(1, N + 2)
+    vols = 0.18 + 0.04 * rng.uniform(size=N)
+    f0 = 0.04 + 0.002 * np.arange(N)
+    taus = np.full(N, tau)
+    disp = np.full(N, 0.01)
+    A = np.zeros(S * N * F)
+    R.ref_flat_vol_pseudo_roots(rate_times, rate_times[:N].copy(), vols, N, S, F, 0.1, 0.08, A)
+    A = A.reshape(S, N, F)
+    fixed = -0.5 * np.sum(A * A, axis=2)
+    z = rng.normal(size=(S, P, F))
+    incr = np.zeros(S * N * P)
+    R.ref_correlate_drift(z.reshape(-1).copy(), incr, A.reshape(-1).copy(), N * F,
+                          fixed.reshape(-1).copy(), F, N, P, S)
+    rates = np.zeros(S * N * P)
+    logs = np.zeros(S * N * P)
+    R.ref_lmm_pc(f0, taus, incr, A.reshape(-1).copy(), disp, P, F, S, N, rates, logs)
+    alive = np.arange(S, dtype=np.int32)
+    disc = np.zeros(S * (N + 1) * P)
+    R.ref_discount_ratios(rates, taus, alive, P, S, N, disc)
+    ann = np.zeros(S * N * P)
+    R.ref_annuities(disc, taus, alive, P, S, N, ann)
+    sr = np.zeros(S * N * P)
+    R.ref_cot_swap_rates(disc, ann, alive, P, S, N, sr)
+    num = np.zeros(S * P)
+    R.ref_numeraires(disc, P, N, num)
+    fidx = np.minimum(np.arange(S) + 1, N - 1).astype(np.int32)
+    fx = np.zeros(S * P)
+    R.ref_forward_extraction(rates, fidx, P, S, N, fx)
+    swr = np.zeros(S * P)
+    R.ref_swap_rate(disc, taus, 3, 8, P, 2, 5, S, N, swr)
+    mu0 = np.zeros(N)
+    R.ref_spot_drifts(A[0].reshape(-1).copy(), f0, taus, disp, F, N, mu0)
+    return dict(N=N, F=F, P=P, rate_times=rate_times, vols=vols, f0=f0, taus=taus, disp=disp, A=A,
+                fixed=fixed, z=z, incr=incr.reshape(S, N, P), rates=rates.reshape(S, N, P),
+                logs=logs.reshape(S, N, P), disc=disc.reshape(S, N + 1, P),
+                ann=ann.reshape(S, N, P), sr=sr.reshape(S, N, P), num=num.reshape(S, P),
+                fidx=fidx, fx=fx.reshape(S, P), swr=swr.reshape(S, P), mu0=mu0)
+
+
+def cashflows(R, g):
+    N, P = g["N"], g["P"]
+    S = N
+    rates = g["rates"]
+    r1 = np.array([rates[s, s] for s in range(S)])  # rate fixing at each step, (S,P)
+    acc = np.full(S, 0.5)
+    swap_aux = np.concatenate([[0.045, 1.0], acc, acc])
+    f1 = np.zeros(S * P)
+    f2 = np.zeros(S * P)
+    R.ref_swap_flows(f1, f2, swap_aux, swap_aux.size, P, S, r1.reshape(-1).copy())
+    tarn_aux = np.concatenate([[0.1, 0.12, 2.0], acc, acc])
+    t1 = np.zeros(S * P)
+    t2 = np.zeros(S * P)
+    R.ref_tarn_flows(t1, t2, tarn_aux, tarn_aux.size, P, S, r1.reshape(-1).copy(),
+                     r1.reshape(-1).copy())
+    pay = g["rate_times"][1:S + 1] - 0.1
+    first = np.zeros(S, np.int32)
+    second = np.zeros(S, np.int32)
+    th = np.zeros(S)
+    R.ref_cash_flow_indices_weights(g["rate_times"], S + 1, pay, S, first, second, th)
+    dfl = np.zeros(S * P)
+    summed = np.zeros(P)
+    R.ref_cash_flow_discounting(first, second, th, g["disc"].reshape(-1).copy(), f2,
+                                g["num"].reshape(-1).copy(), P, S, dfl, summed)
+    return dict(r1=r1, swap_aux=swap_aux, swap_f1=f1.reshape(S, P), swap_f2=f2.reshape(S, P),
+                tarn_aux=tarn_aux, tarn_f1=t1.reshape(S, P), tarn_f2=t2.reshape(S, P),
+                pay_times=pay, first=first, second=second, thetas=th,
+                discounted=dfl.reshape(S, P), summed=summed)
+
+
+def pseudo_roots(R):
+    N, F = 20, 3
+    rate_times = 0.5 * np.arange(1, N + 2)
+    vols = np.full(N, 0.2)
+    A = np.zeros(N * N * F)
+    R.ref_flat_vol_pseudo_roots(rate_times, rate_times[:N].copy(), vols, N, N, F, 0.0, 0.1, A)
+    return dict(rate_times=rate_times, vols=vols, A=A.reshape(N, N, F))
+
+
+def main():
+    oracle.build(ref=True)
+    R = oracle.ref()
+    np.savez_compressed(os.path.join(HERE, "sobol_ref.npz"), **sobol(R))
+    np.savez_compressed(os.path.join(HERE, "invnorm_ref.npz"), **invnorm(R))
+    g = lmm(R)
+    np.savez_compressed(os.path.join(HERE, "lmm_gold_small.npz"), **g)
+    np.savez_compressed(os.path.join(HERE, "cashflow_ref.npz"), **cashflows(R, g))
+    np.savez_compressed(os.path.join(HERE, "pseudo_roots_ref.npz"), **pseudo_roots(R))
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,68 @@
+"""The C-ABI library loads, exports every entry point include/koo_c_abi.h
+declares, and rejects bad arguments with a status and a message before any
+device work (so these run without a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from useful_code_amd import koo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    hdr = open(os.path.join(ROOT, "include", "koo_c_abi.h")).read()
+    return set(re.findall(r"^(?:int|long long|const char\*)\s+(\w+)\(", hdr, re.M))
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", koo.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    syms = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    missing = declared() - syms
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    assert declared() == set(koo.exported_symbols())
+
+
+def test_library_loads_and_binds():
+    L = koo.lib()
+    assert b"gfx950" in L.koo_version()
+    for name in declared():
+        assert hasattr(L, name)
+
+
+def _status(name, *args):
+    return getattr(koo.lib(), name)(*args), koo.lib().koo_last_error().decode()
+
+
+@pytest.mark.parametrize("call,args", [
+    ("brownian_bridge_gpu", (129, None, None, None, 16, 1, None)),
+    ("sobol_bridge_gpu", (16, 0, 0, 2, None, None, 0, None, None, None)),
+    ("sobolGPUInts", (-1, 3, 0, None, None, None)),
+    ("LMM_evolver_pc_all_gpu", (None,) * 9 + (16, 9, 4, 4) + (None,) * 5 + (0, 1, None, 0, 1)),
+    ("lmm_price_fwd", (16, 200, 5, 8) + (None,) * 5 + (1,) + (None,) * 5),
+    ("lmm_price_fwd", (16, 40, 5, 8) + (None,) * 5 + (7,) + (None,) * 5),
+    ("inverse_normal_u32_gpu", (None, None, -3, None)),
+])
+def test_invalid_arguments_are_rejected(call, args):
+    st, msg = _status(call, *args)
+    assert st == -1 and call.split("_")[0] in msg
+
+
+def test_empty_work_is_a_no_op():
+    assert koo.lib().sobolGPUInts(0, 4, 0, None, None, None) == 0
+    assert koo.lib().scramble_gpu(None, None, 0, 0, None) == 0
+
+
+def test_workspace_sizes():
+    ck, sc, pa = koo.lmm_price_workspace(2**20, 40, 5, 8)
+    assert ck == 4 * 5 * 41 * 2**20
+    assert sc == 4 * 8 * (41 + 10) * 2**20
+    assert pa >= 8 * (2**20 // 256) * 80

@@ -1,0 +1,110 @@
+"""Stage (a) on the GPU through the C ABI against the oracle: Sobol integers
+(bit-exact, config shapes and offsets), scrambling (bit-exact, the reference
+test's fixture formula), the normal quantile (FP64 oracle tolerance) and the
+fused generator."""
+import numpy as np
+import pytest
+
+import oracle as O
+from useful_code_amd import koo
+
+pytestmark = pytest.mark.gpu
+
+
+def dev_u32(torch, a):
+    return torch.as_tensor(np.ascontiguousarray(a, np.uint32).view(np.int32)).cuda()
+
+
+def host_u32(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("P,D,off", [(4096, 60, 0), (5000, 3, 17), (1 << 20, 200, 0),
+                                     (1 << 20, 200, 123457), (777, 1, 5), (64, 2048, 0)])
+def test_sobol_ints_bit_exact(torch_cuda, P, D, off):
+    torch = torch_cuda
+    print(f"sobolGPUInts P={P} D={D} offset={off}", flush=True)
+    dirs = dev_u32(torch, koo.sobol_directions(D))
+    out = torch.empty(P * D, dtype=torch.int32, device="cuda")
+    koo.sobolGPUInts(P, D, off, dirs, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(host_u32(out).reshape(D, P), O.sobol_ints(P, D, off))
+
+
+def test_sobol_split_batches(torch_cuda):
+    torch = torch_cuda
+    D = 32
+    dirs = dev_u32(torch, koo.sobol_directions(D))
+    a = torch.empty(D * 3000, dtype=torch.int32, device="cuda")
+    b = torch.empty(D * 5000, dtype=torch.int32, device="cuda")
+    koo.sobolGPUInts(3000, D, 0, dirs, a)
+    koo.sobolGPUInts(5000, D, 3000, dirs, b)
+    full = O.sobol_ints(8000, D, 0)
+    got = np.concatenate([host_u32(a).reshape(D, -1), host_u32(b).reshape(D, -1)], axis=1)
+    assert np.array_equal(got, full)
+
+
+@pytest.mark.parametrize("D,P,nd", [(128, 1 << 18, 128), (16, 1003, 7), (10, 256, 10)])
+def test_scramble_bit_exact(torch_cuda, D, P, nd):
+    torch = torch_cuda
+    # scramble_test.cu:74-88 fixture: scrambler 6i^2+2i+1, data i*P+p
+    data = (np.arange(D, dtype=np.uint64)[:, None] * P + np.arange(P)[None, :]).astype(np.uint32)
+    scr = np.array([6 * i * i + 2 * i + 1 for i in range(D)], np.uint64).astype(np.uint32)
+    t = dev_u32(torch, data)
+    koo.scramble_gpu(t, dev_u32(torch, scr), nd, P)
+    assert np.array_equal(host_u32(t).reshape(D, P), O.scramble(data, scr, nd))
+
+
+def test_inverse_normal_vs_fp64_oracle(torch_cuda):
+    torch = torch_cuda
+    rng = np.random.RandomState(5)
+    edges = np.array([0, 1, 2, 3, 255, 256, 2**24 - 1, 2**24, 2**31 - 1, 2**31, 2**31 + 1,
+                      2**32 - 256, 2**32 - 129, 2**32 - 128, 2**32 - 2, 2**32 - 1], np.uint64)
+    x = np.concatenate([edges, rng.randint(0, 2**32, size=(1 << 20) - 16, dtype=np.uint64)])
+    x = x.astype(np.uint32)
+    out = torch.empty(x.size, dtype=torch.float32, device="cuda")
+    koo.inverse_normal_u32_gpu(dev_u32(torch, x), out)
+    got = out.cpu().numpy().astype(np.float64)
+    exact = O.invnorm_u32(x)
+    assert np.all(np.isfinite(got))
+    # Shaw-Brickman's own error (<= 2.1e-6 absolute, 3.6e-7 relative) plus FP32.
+    err = np.abs(got - exact)
+    assert np.all(err <= 3e-6 + 6e-7 * np.abs(exact)), float(err.max())
+    assert np.mean(err) < 2e-7
+    # odd symmetry of the integer mapping: x and 2^32 - x give +-z
+    y = x[(x > 0)]
+    out2 = torch.empty(y.size, dtype=torch.float32, device="cuda")
+    koo.inverse_normal_u32_gpu(dev_u32(torch, (0 - y.astype(np.uint64)) & 0xFFFFFFFF), out2)
+    koo.inverse_normal_u32_gpu(dev_u32(torch, y), out[: y.size])
+    assert np.array_equal(out2.cpu().numpy(), -out[: y.size].cpu().numpy())
+
+
+def test_inverse_normal_agrees_with_reference_formula_in_the_body(torch_cuda):
+    torch = torch_cuda
+    g = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                           "invnorm_ref.npz"))
+    x, ref = g["x"], g["sb_ref_f32"].astype(np.float64)
+    out = torch.empty(x.size, dtype=torch.float32, device="cuda")
+    koo.inverse_normal_u32_gpu(dev_u32(torch, x), out)
+    got = out.cpu().numpy().astype(np.float64)
+    u = x.astype(np.float64) * 2.0**-32
+    w = 2 * np.minimum(u, 1 - u)
+    body = np.isfinite(ref) & (w > 1e-3)
+    # The reference forms 1 - |2u - 1| in FP32 (error up to 2^-24 absolute in w);
+    # away from the tails that moves z by at most ~2^-24 / w.
+    assert np.all(np.abs(got[body] - ref[body]) <= 4e-7 + 2.0**-23 / w[body])
+
+
+def test_fused_generator_equals_staged(torch_cuda):
+    torch = torch_cuda
+    P, D, off, nscr = 50000, 60, 99, 20
+    dirs = dev_u32(torch, koo.sobol_directions(D))
+    scr = dev_u32(torch, np.random.RandomState(1).randint(0, 2**32, D, dtype=np.uint64))
+    ints = torch.empty(P * D, dtype=torch.int32, device="cuda")
+    koo.sobolGPUInts(P, D, off, dirs, ints)
+    koo.scramble_gpu(ints, scr, nscr, P)
+    staged = torch.empty(P * D, dtype=torch.float32, device="cuda")
+    koo.inverse_normal_u32_gpu(ints, staged)
+    fused = torch.empty_like(staged)
+    koo.sobol_scramble_normal_gpu(P, D, off, dirs, scr, nscr, fused)
+    assert torch.equal(staged, fused)

@@ -1,0 +1,314 @@
+"""Pins the FP64 oracle (oracle/koo_oracle.c) before anything is checked
+against it: the reference's own gold routines (committed fixtures written by
+tests/golden/make_golden.py from oracle/_ref), SciPy's independent Sobol and
+normal-quantile implementations, and closed forms / invariants."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(G, name))
+
+
+# ---------------------------------------------------------------- stage (a)
+
+def test_sobol_directions_match_reference_gold():
+    g = load("sobol_ref.npz")
+    assert np.array_equal(O.sobol_directions(210), g["dirs210"])
+
+
+@pytest.mark.parametrize("off", [0, 1000, 65535])
+def test_sobol_ints_match_reference_gold(off):
+    g = load("sobol_ref.npz")
+    assert np.array_equal(O.sobol_ints(64, 200, off), g[f"ints_off{off}"])
+
+
+def test_sobol_ints_match_scipy():
+    from scipy.stats import qmc
+    import warnings
+    D, P = 200, 4096
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pts = qmc.Sobol(D, scramble=False, bits=32).random(P + 1)
+    ref = (pts * 2.0**32).astype(np.uint64).astype(np.uint32)[1:].T
+    assert np.array_equal(O.sobol_ints(P, D, 0), ref)
+
+
+def test_sobol_split_batches_equal_one_batch():
+    # Sobol_int_test.cu:37-206: two batches with a path offset equal one run.
+    full = O.sobol_ints(3000, 17, 0)
+    a = O.sobol_ints(1000, 17, 0)
+    b = O.sobol_ints(2000, 17, 1000)
+    assert np.array_equal(np.concatenate([a, b], axis=1), full)
+
+
+def test_scramble_matches_reference_gold():
+    g = load("sobol_ref.npz")
+    out = O.scramble(g["scramble_in"], g["scramblers"], g["scramble_in"].shape[0])
+    assert np.array_equal(out, g["scramble_out"])
+
+
+def test_scramble_only_first_rows():
+    data = np.arange(6 * 8, dtype=np.uint32).reshape(6, 8)
+    out = O.scramble(data, np.full(6, 0xFFFFFFFF, np.uint32), 2)
+    assert np.array_equal(out[:2], ~data[:2]) and np.array_equal(out[2:], data[2:])
+
+
+def test_reference_sb_formula_restated_bit_for_bit():
+    g = load("invnorm_ref.npz")
+    mine = np.array([O.lib().orc_invnorm_sb_ref_f32(int(v)) for v in g["x"]], np.float32)
+    ref = g["sb_ref_f32"]
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(mine), fin)
+    assert np.array_equal(mine[fin].view(np.uint32), ref[fin].view(np.uint32))
+
+
+def test_fp64_quantile_matches_scipy():
+    from scipy.special import ndtri
+    u = np.concatenate([np.logspace(-12, -0.31, 400), 1 - np.logspace(-12, -0.31, 400),
+                        np.linspace(0.01, 0.99, 301)])
+    mine = np.array([O.lib().orc_invnorm(float(v)) for v in u])
+    assert np.allclose(mine, ndtri(u), rtol=2e-14, atol=1e-14)
+
+
+def test_fp64_quantile_of_integers():
+    from scipy.special import ndtri
+    x = np.array([1, 2, 1000, 2**31 - 1, 2**31 + 1, 2**32 - 1, 2**32 - 128], np.uint64)
+    mine = O.invnorm_u32(x.astype(np.uint32))
+    lower = x < 2**31
+    tail = np.where(lower, x, 2**32 - x).astype(np.float64) * 2.0**-32
+    exact = np.where(lower, ndtri(tail), -ndtri(tail))
+    assert np.allclose(mine, exact, rtol=1e-14)
+    assert O.invnorm_u32(np.array([2**31], np.uint32))[0] == 0.0
+    assert np.isclose(O.invnorm_u32(np.array([0], np.uint32))[0], ndtri(2.0**-33), rtol=1e-14)
+
+
+# ---------------------------------------------------------------- stage (b)
+
+def test_bridge_allocation_is_a_permutation_and_diagonal():
+    F, S = 5, 40
+    idx = O.bridge_allocate_diagonal(F, S).reshape(F, S)
+    assert sorted(idx.reshape(-1)) == list(range(F * S))
+    assert idx[0, 0] == 0 and idx[1, 0] == 1 and idx[0, 1] == 2  # first diagonals
+    if O.ref_available():
+        ref = np.zeros(F * S, np.int32)
+        O.ref().ref_bridge_allocate_diagonal(F, S, ref)
+        assert np.array_equal(ref, idx.reshape(-1))
+
+
+@pytest.mark.parametrize("steps", [1, 2, 7, 20, 40, 64])
+def test_bridge_increments_are_orthonormal(steps):
+    # The bridge maps i.i.d. N(0,1) to Brownian increments on the unit grid,
+    # i.e. its matrix M (increment_i = sum_j M_ij z_j) satisfies M M^T = I.
+    eye = np.eye(steps)
+    z = eye  # [D][P] with D = steps, P = steps columns = unit vectors
+    out = O.bridge_transform(z, steps, 1, indices=np.arange(steps, dtype=np.int32))
+    M = out[:, 0, :]  # (S, P): column j = response to unit vector j
+    assert np.allclose(M @ M.T, np.eye(steps), atol=1e-13)
+    # first variate builds the terminal point W(S) = sqrt(S) z0
+    assert np.isclose(M[:, 0].sum(), np.sqrt(steps))
+
+
+def test_correlate_drift_matches_reference_gold():
+    g = load("lmm_gold_small.npz")
+    N, F, P = int(g["N"]), int(g["F"]), int(g["P"])
+    S = N
+    out = np.zeros(S * N * P)
+    O.lib().orc_correlate_drift(np.ascontiguousarray(g["z"]).reshape(-1), out,
+                                np.ascontiguousarray(g["A"]).reshape(-1), np.zeros(S, np.int32),
+                                np.ascontiguousarray(g["fixed"]).reshape(-1), N * F, F, N, P, S)
+    assert np.allclose(out.reshape(S, N, P), g["incr"], rtol=0, atol=1e-15)
+
+
+def test_spot_drifts_match_reference_gold():
+    g = load("lmm_gold_small.npz")
+    mu0 = O.spot_drifts(g["A"][0], g["f0"], g["taus"], g["disp"])
+    assert np.allclose(mu0, g["mu0"], rtol=1e-14, atol=1e-18)
+
+
+def test_pc_evolver_matches_reference_gold():
+    g = load("lmm_gold_small.npz")
+    N = int(g["N"])
+    rates, logs, disc = O.lmm_pc_evolve(g["f0"], g["taus"], g["z"], g["A"], g["disp"],
+                                        np.arange(N, dtype=np.int32), transposed=False)
+    for s in range(N):
+        assert np.allclose(rates[s, s:], g["rates"][s, s:], rtol=1e-12, atol=1e-15)
+        assert np.allclose(logs[s, s:], g["logs"][s, s:], rtol=1e-12, atol=1e-15)
+        assert np.allclose(disc[s, s:], g["disc"][s, s:], rtol=1e-12)
+
+
+def test_zero_vol_freezes_forwards():
+    N, F, P = 8, 2, 5
+    f0 = 0.03 + 0.001 * np.arange(N)
+    taus = np.full(N, 0.5)
+    z = np.random.RandomState(0).normal(size=(N, F, P))
+    rates, _, disc = O.lmm_pc_evolve(f0, taus, z, np.zeros((N, N, F)), np.zeros(N),
+                                     np.arange(N, dtype=np.int32))
+    for s in range(N):
+        assert np.allclose(rates[s, s:], f0[s:, None], rtol=1e-14)
+        want = np.concatenate([[1.0], np.cumprod(1.0 / (1.0 + 0.5 * f0[s:]))])
+        assert np.allclose(disc[s, s:], want[:, None], rtol=1e-14)
+
+
+def test_discounts_annuities_swap_rates_numeraires_match_reference_gold():
+    g = load("lmm_gold_small.npz")
+    N, P = int(g["N"]), int(g["P"])
+    S = N
+    L = O.lib()
+    alive = np.arange(S, dtype=np.int32)
+    disc = np.zeros(S * (N + 1) * P)
+    L.orc_discount_ratios(np.ascontiguousarray(g["rates"]).reshape(-1), g["taus"], alive, P, S, N, disc)
+    ann = np.zeros(S * N * P)
+    L.orc_coterminal_annuities(disc, g["taus"], alive, P, S, N, ann)
+    sr = np.zeros(S * N * P)
+    L.orc_coterminal_swap_rates(disc, ann, alive, P, S, N, sr)
+    num = np.zeros(S * P)
+    L.orc_spot_numeraires(disc, P, N, num)
+    fx = np.zeros(S * P)
+    L.orc_forward_extraction(np.ascontiguousarray(g["rates"]).reshape(-1), g["fidx"], P, S, N, fx)
+    swr = np.zeros(S * P)
+    L.orc_swap_rate(disc, g["taus"], 3, 8, P, 2, 5, N, swr)
+    disc = disc.reshape(S, N + 1, P)
+    ann, sr = ann.reshape(S, N, P), sr.reshape(S, N, P)
+    for s in range(S):
+        assert np.allclose(disc[s, s:], g["disc"][s, s:], rtol=1e-14)
+        assert np.allclose(ann[s, s:], g["ann"][s, s:], rtol=1e-14)
+        assert np.allclose(sr[s, s:], g["sr"][s, s:], rtol=1e-13)
+    assert np.allclose(num.reshape(S, P), g["num"], rtol=1e-14)
+    assert np.allclose(fx.reshape(S, P), g["fx"], rtol=0, atol=0)
+    assert np.allclose(swr.reshape(S, P)[5], g["swr"][5], rtol=1e-14)
+
+
+# ---------------------------------------------------------------- stage (c)
+
+def test_swap_and_tarn_flows_match_reference_gold():
+    c = load("cashflow_ref.npz")
+    S, P = c["r1"].shape
+    L = O.lib()
+    f1, f2 = np.zeros(S * P), np.zeros(S * P)
+    L.orc_swap_flows(f1, f2, c["swap_aux"], P, S, np.ascontiguousarray(c["r1"]).reshape(-1))
+    assert np.allclose(f1, c["swap_f1"].reshape(-1), atol=1e-16)
+    assert np.allclose(f2, c["swap_f2"].reshape(-1), atol=1e-16)
+    t1, t2 = np.zeros(S * P), np.zeros(S * P)
+    r1 = np.ascontiguousarray(c["r1"]).reshape(-1)
+    L.orc_tarn_flows(t1, t2, c["tarn_aux"], P, S, r1, r1)
+    assert np.allclose(t1, c["tarn_f1"].reshape(-1), atol=1e-16)
+    assert np.allclose(t2, c["tarn_f2"].reshape(-1), atol=1e-16)
+
+
+def test_discounting_matches_reference_gold():
+    c, g = load("cashflow_ref.npz"), load("lmm_gold_small.npz")
+    S, P = c["r1"].shape
+    L = O.lib()
+    first, second = np.zeros(S, np.int32), np.zeros(S, np.int32)
+    th = np.zeros(S)
+    L.orc_cash_flow_indices_weights(g["rate_times"], c["pay_times"], S, first, second, th)
+    assert np.array_equal(first, c["first"]) and np.array_equal(second, c["second"])
+    assert np.allclose(th, c["thetas"], rtol=1e-15)
+    out, summed = np.zeros(S * P), np.zeros(P)
+    L.orc_cash_flow_discounting(first, second, th, np.ascontiguousarray(g["disc"]).reshape(-1),
+                                np.ascontiguousarray(c["swap_f2"]).reshape(-1),
+                                np.ascontiguousarray(g["num"]).reshape(-1), P, S, out, summed)
+    assert np.allclose(out.reshape(S, P), c["discounted"], rtol=1e-14, atol=1e-18)
+    assert np.allclose(summed, c["summed"], rtol=1e-14)
+
+
+def test_compensated_sum_is_exact_on_cancellation():
+    x = np.array([1e16, 1.0, -1e16, 1.0] * 1000)
+    assert O.lib().orc_sum_compensated(x, x.size) == 2000.0
+
+
+# ---------------------------------------------------------------- set-up
+
+def test_flat_vol_pseudo_roots_match_reference_gold():
+    p = load("pseudo_roots_ref.npz")
+    N, _, F = p["A"].shape
+    A = O.flat_vol_pseudo_roots(p["rate_times"], p["rate_times"][:N], p["vols"], F, 0.0, 0.1)
+    for s in range(N):
+        assert np.allclose(A[s] @ A[s].T, p["A"][s] @ p["A"][s].T, rtol=1e-9, atol=1e-15)
+        assert np.allclose(np.abs(A[s]), np.abs(p["A"][s]), rtol=1e-7, atol=1e-12)
+
+
+def test_pseudo_sqrt_matches_numpy_eigh():
+    C = O.exp_corr(0.5 * np.arange(1, 41), 0.0, 0.05)
+    B = O.pseudo_sqrt(C, 5, normalise=False)
+    w, v = np.linalg.eigh(C)
+    top = np.argsort(w)[::-1][:5]
+    ref = v[:, top] * np.sqrt(w[top])
+    assert np.allclose(np.abs(B), np.abs(ref), atol=1e-10)
+    Bn = O.pseudo_sqrt(C, 5, normalise=True)
+    assert np.allclose(np.sum(Bn * Bn, axis=1), 1.0, atol=1e-13)
+
+
+# ---------------------------------------------------------------- fused caplets
+
+def _small_caplet_model(P=48, N=10, F=3, seed=3):
+    rt = 0.5 * np.arange(1, N + 2)
+    sig = 0.15 + 0.1 * np.exp(-0.2 * rt[:N])
+    A, _ = O.factor_pseudo_roots(rt, sig, F, 0.0, 0.05)
+    rng = np.random.RandomState(seed)
+    f0 = 0.03 + 0.0005 * np.arange(N) + rng.normal(0, 0.001, N)
+    z = rng.normal(size=(N, F, P))
+    return f0, np.full(N, 0.5), np.zeros(N), A, sig, z
+
+
+def test_caplet_path_price_equals_evolver_plus_payoff():
+    f0, taus, disp, A, sig, z = _small_caplet_model()
+    P, N = z.shape[2], f0.size
+    m = O.caplet_model(f0, taus, disp, A, sig, 0.032, P)
+    pp = O.caplet_price_paths(m, z)
+    rates, _, _ = O.lmm_pc_evolve(f0, taus, z, A, disp, np.arange(N, dtype=np.int32))
+    fix = np.array([rates[s, s] for s in range(N)])  # (N, P)
+    val = 0.5 * np.maximum(fix - 0.032, 0) / (1 + 0.5 * fix)
+    B = np.vstack([np.ones(P), np.cumprod(1 + 0.5 * fix, axis=0)[:-1]])
+    assert np.allclose(pp, np.sum(val / B, axis=0), rtol=1e-12)
+
+
+def test_caplet_adjoint_matches_central_fd():
+    f0, taus, disp, A, sig, z = _small_caplet_model()
+    P, N = z.shape[2], f0.size
+    K, h = 0.032, 1e-6
+    m = O.caplet_model(f0, taus, disp, A, sig, K, P)
+    pp, d, v = O.caplet_adjoint_paths(m, z)
+    for i in range(N):
+        fp, fm = f0.copy(), f0.copy()
+        fp[i] += h
+        fm[i] -= h
+        fd = (O.caplet_price_paths(O.caplet_model(fp, taus, disp, A, sig, K, P), z).sum()
+              - O.caplet_price_paths(O.caplet_model(fm, taus, disp, A, sig, K, P), z).sum()) / (2 * h)
+        assert abs(fd - d[i]) <= 1e-7 * abs(d[i]) + 1e-10
+        sp, sm = sig.copy(), sig.copy()
+        sp[i] += h
+        sm[i] -= h
+        Ap, Am = A.copy(), A.copy()
+        Ap[:, i, :] *= sp[i] / sig[i]
+        Am[:, i, :] *= sm[i] / sig[i]
+        fv = (O.caplet_price_paths(O.caplet_model(f0, taus, disp, Ap, sp, K, P), z).sum()
+              - O.caplet_price_paths(O.caplet_model(f0, taus, disp, Am, sm, K, P), z).sum()) / (2 * h)
+        assert abs(fv - v[i]) <= 1e-7 * abs(v[i]) + 1e-10
+
+
+def test_caplet_price_near_black76():
+    # 2^14 scrambled-free Sobol paths: MC + discretisation error well below 1%.
+    from scipy.stats import norm
+    N, F, P = 10, 3, 2**14
+    rt = 0.5 * np.arange(1, N + 2)
+    sig = np.full(N, 0.2)
+    A, _ = O.factor_pseudo_roots(rt, sig, F, 0.0, 0.05)
+    f0 = np.full(N, 0.04)
+    K = 0.04
+    ints = O.sobol_ints(P, N * F, 0)
+    m = O.caplet_model(f0, np.full(N, 0.5), np.zeros(N), A, sig, K, P)
+    price, _, _ = O.caplet_pipeline(m, ints)
+    T = rt[:N]
+    d1 = (np.log(f0 / K) + 0.5 * sig**2 * T) / (sig * np.sqrt(T))
+    d2 = d1 - sig * np.sqrt(T)
+    df = np.cumprod(1 / (1 + 0.5 * f0))  # P(T0, T_{i+1}) / P(0, T0)
+    black = np.sum(df * 0.5 * (f0 * norm.cdf(d1) - K * norm.cdf(d2)))
+    assert abs(price / black - 1) < 5e-3

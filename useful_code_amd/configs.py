@@ -1,0 +1,87 @@
+"""Synthetic workloads of the LMM pricing path (host-side inputs, FP64).
+
+All randomness is seeded: numpy's MT19937 (``np.random.RandomState``) draws
+the initial curve perturbations and the 32-bit scramblers, so a workload is
+reproducible from its seed.  Rate times are T_i = tau * (i + 1), i = 0..N
+(rate i accrues over [T_i, T_{i+1}]); evolution times are the reset times
+T_0..T_{N-1}, so steps == rates and alive(s) = s.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import koo
+
+
+@dataclass
+class LMMWorkload:
+    name: str
+    paths: int
+    n_rates: int
+    factors: int
+    tau: float
+    f0: np.ndarray
+    vols: np.ndarray
+    rate_times: np.ndarray  # N + 1
+    A: np.ndarray  # (S, N, F) FP64
+    strike: float
+    product: str
+    scramblers: np.ndarray  # uint32, one per Sobol dimension
+    checkpoint_every: int = 8
+    displacements: np.ndarray = field(default=None)
+    path_offset: int = 0
+
+    @property
+    def steps(self):
+        return self.n_rates
+
+    @property
+    def dims(self):
+        return self.steps * self.factors
+
+    @property
+    def taus(self):
+        return np.full(self.n_rates, self.tau)
+
+    def disp(self):
+        return np.zeros(self.n_rates) if self.displacements is None else self.displacements
+
+    def alive(self):
+        return np.arange(self.steps, dtype=np.int32)
+
+
+def _scramblers(rng, dims):
+    return rng.randint(0, 2**32, size=dims, dtype=np.uint64).astype(np.uint32)
+
+
+def caplet_portfolio(paths=2**20, seed=2024, checkpoint_every=8):
+    """LMM price plus adjoint Greeks: 40 forwards, tau = 0.5,
+    f0_i = 0.03 + 0.0005 i + N(0, 0.001), sigma_i = 0.15 + 0.1 exp(-0.2 T_i),
+    5 PCA factors of exp(-0.05 |T_i - T_j|), 40 caplets at K = 0.045,
+    2^20 Sobol paths, 40 steps, checkpoint every 8 steps."""
+    N, F, tau = 40, 5, 0.5
+    rng = np.random.RandomState(seed)
+    f0 = 0.03 + 0.0005 * np.arange(N) + rng.normal(0.0, 0.001, N)
+    rate_times = tau * np.arange(1, N + 2)
+    vols = 0.15 + 0.1 * np.exp(-0.2 * rate_times[:N])
+    A, _ = koo.factor_pseudo_roots(rate_times, vols, F, 0.0, 0.05)
+    return LMMWorkload("lmm_caplets_40x5", paths, N, F, tau, f0, vols, rate_times, A, 0.045,
+                       "caplets", _scramblers(rng, N * F), checkpoint_every)
+
+
+def payer_swap(paths=2**15, seed=2023):
+    """Plain LMM, CPU-runnable case: 20 forwards, tau = 0.5,
+    f0 = 0.05 + U(-0.005, 0.005), flat vol 0.20, exp(-0.1 |T_i - T_j|)
+    reduced to 3 factors by PCA per step, 2^15 paths, payer swap at 0.05."""
+    N, F, tau = 20, 3, 0.5
+    rng = np.random.RandomState(seed)
+    f0 = 0.05 + rng.uniform(-0.005, 0.005, N)
+    rate_times = tau * np.arange(1, N + 2)
+    vols = np.full(N, 0.20)
+    A = koo.flat_vol_pseudo_roots(rate_times, rate_times[:N], vols, F, 0.0, 0.1)
+    return LMMWorkload("lmm_payer_swap_20x3", paths, N, F, tau, f0, vols, rate_times, A, 0.05,
+                       "swap", _scramblers(rng, N * F))
+
+
+def bridge_allocation(w):
+    return koo.bridge_allocate_diagonal(w.factors, w.steps)

@@ -1,0 +1,212 @@
+// bridge.hip -- stage (a)+(b): Brownian bridge (Jaeckel construction on the
+// unit grid) with successive differencing, and the fused
+// Sobol -> scramble -> normal -> bridge generator, for gfx950.
+//
+// One lane owns one (path, factor) pair; the S bridge points of that pair
+// live in LDS laid out [step][lane] so every LDS access of a wavefront is
+// bank-conflict free, and every global load/store is a contiguous 256-byte
+// row of the path-minor layouts ([D][P] in, (S,F,P) out).  The construction
+// tables travel in the kernel arguments (scalar loads), so a call allocates
+// nothing.
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "koo_common.h"
+
+namespace koo {
+
+namespace {
+
+constexpr int kBridgeMaxSteps = 128;
+
+// Construction schedule of brownian_bridge::initialize (brownian_bridge.cu:
+// 639-700) on t_i = i + 1 (brownian_bridge.cu:359-362), kept in float with
+// the same operations so the weights are the reference's bit for bit.
+struct BridgeTables {
+  int16_t bridge[kBridgeMaxSteps];
+  int16_t left[kBridgeMaxSteps];
+  int16_t right[kBridgeMaxSteps];
+  float lw[kBridgeMaxSteps];
+  float rw[kBridgeMaxSteps];
+  float sd[kBridgeMaxSteps];
+};
+
+BridgeTables make_tables(int size) {
+  BridgeTables T{};
+  std::vector<float> t(size);
+  for (int i = 0; i < size; ++i) t[i] = static_cast<float>(i + 1);
+  std::vector<int> map(size, 0);
+  map[size - 1] = 1;
+  T.bridge[0] = static_cast<int16_t>(size - 1);
+  T.sd[0] = std::sqrt(t[size - 1]);
+  T.lw[0] = T.rw[0] = 0.0f;
+  for (int j = 0, i = 1; i < size; ++i) {
+    while (map[j]) ++j;
+    int k = j;
+    while (!map[k]) ++k;
+    const int l = j + ((k - 1 - j) >> 1);
+    map[l] = i;
+    T.bridge[i] = static_cast<int16_t>(l);
+    T.left[i] = static_cast<int16_t>(j);
+    T.right[i] = static_cast<int16_t>(k);
+    if (j != 0) {
+      T.lw[i] = (t[k] - t[l]) / (t[k] - t[j - 1]);
+      T.rw[i] = (t[l] - t[j - 1]) / (t[k] - t[j - 1]);
+      T.sd[i] = std::sqrt(((t[l] - t[j - 1]) * (t[k] - t[l])) / (t[k] - t[j - 1]));
+    } else {
+      T.lw[i] = (t[k] - t[l]) / t[k];
+      T.rw[i] = t[l] / t[k];
+      T.sd[i] = std::sqrt(t[l] * (t[k] - t[l]) / t[k]);
+    }
+    j = k + 1;
+    if (j >= size) j = 0;
+  }
+  return T;
+}
+
+const BridgeTables& tables_for(int steps) {
+  static std::mutex mu;
+  static std::unordered_map<int, BridgeTables> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(steps);
+  if (it == cache.end()) it = cache.emplace(steps, make_tables(steps)).first;
+  return it->second;
+}
+
+constexpr int kBridgeThreads = 256;
+
+// Builds one factor's path for this lane's path index from the variate
+// source `z(i)` (bridge variate i), differences it and stores (S,F,P).
+template <class Source>
+__device__ __forceinline__ void bridge_lane(const BridgeTables& T, int steps, int factors,
+                                            int paths, int p, float* lds, Source z,
+                                            float* __restrict__ out) {
+  const int f = blockIdx.y;
+  const int lane = threadIdx.x;
+  float* path = lds + lane;  // path[i * blockDim.x]
+  const int bd = blockDim.x;
+  path[(steps - 1) * bd] = T.sd[0] * z(0);
+  for (int i = 1; i < steps; ++i) {
+    const int j = T.left[i], k = T.right[i], l = T.bridge[i];
+    float v = T.rw[i] * path[k * bd];
+    if (j != 0) v = T.lw[i] * path[(j - 1) * bd] + v;
+    path[l * bd] = v + T.sd[i] * z(i);
+  }
+  float prev = 0.0f;
+  for (int i = 0; i < steps; ++i) {
+    const float w = path[i * bd];
+    out[(static_cast<size_t>(i) * factors + f) * paths + p] = w - prev;
+    prev = w;
+  }
+}
+
+__global__ void __launch_bounds__(kBridgeThreads)
+    k_bridge(const BridgeTables T, int steps, int factors, int paths,
+             const int* __restrict__ alloc, const float* __restrict__ in,
+             float* __restrict__ out) {
+  extern __shared__ float lds[];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const int* idx = alloc + blockIdx.y * steps;
+  bridge_lane(T, steps, factors, paths, p, lds,
+              [&](int i) { return in[static_cast<size_t>(idx[i]) * paths + p]; }, out);
+}
+
+// Fused generator: the bridge variate i of factor f is the normal quantile of
+// Sobol dimension d = alloc(f, i) at Gray index p + off1, scrambled when
+// d < number_scrambled.  Direction numbers of the block's S dimensions sit in
+// LDS behind the path storage.
+__global__ void __launch_bounds__(kBridgeThreads)
+    k_sobol_bridge(const BridgeTables T, int steps, int factors, int paths, uint32_t off1,
+                   const uint32_t* __restrict__ dirs, const uint32_t* __restrict__ scr,
+                   int number_scrambled, const int* __restrict__ alloc, float* __restrict__ out) {
+  extern __shared__ float lds[];
+  uint32_t* v = reinterpret_cast<uint32_t*>(lds + steps * blockDim.x);
+  const int* idx = alloc + blockIdx.y * steps;
+  for (int w = threadIdx.x; w < steps * 32; w += blockDim.x)
+    v[w] = dirs[static_cast<size_t>(idx[w >> 5]) * 32 + (w & 31)];
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const uint32_t n = static_cast<uint32_t>(p) + off1;
+  bridge_lane(T, steps, factors, paths, p, lds,
+              [&](int i) {
+                const int d = idx[i];
+                const uint32_t s = d < number_scrambled ? scr[d] : 0u;
+                return invnorm_u32(sobol_point(v + 32 * i, n) ^ s);
+              },
+              out);
+}
+
+}  // namespace
+}  // namespace koo
+
+using namespace koo;
+
+// Diagonal allocation; the walk of gold/Bridge_allocator_gold.cpp:38-76.
+extern "C" int bridge_allocate_diagonal(int factors, int steps, int* indices) {
+  if (factors < 1 || steps < 1 || !indices)
+    return set_error(KOO_ERR_INVALID, "bridge_allocate_diagonal: bad arguments");
+  int i0 = 0, j0 = 0, i = 0, j = 0, counter = 0;
+  while (counter < factors * steps) {
+    indices[i * steps + j] = counter++;
+    if (i == 0 || j == steps - 1) {
+      if (i0 < factors - 1) {
+        i0 += 1;
+        j0 = 0;
+      } else {
+        i0 = factors - 1;
+        j0 += 1;
+      }
+      i = i0;
+      j = j0;
+    } else {
+      i -= 1;
+      j += 1;
+    }
+  }
+  return KOO_OK;
+}
+
+// Lanes per block so that the LDS path storage stays within 64 KiB.
+static int bridge_threads(int steps) { return steps <= 48 ? 256 : (steps <= 96 ? 128 : 64); }
+
+static int bridge_check(const char* who, int steps, int paths, int factors) {
+  if (steps < 1 || steps > kBridgeMaxSteps)
+    return set_error(KOO_ERR_INVALID, "%s: steps %d outside [1,%d]", who, steps, kBridgeMaxSteps);
+  if (paths < 0 || factors < 1 || factors > 65535)
+    return set_error(KOO_ERR_INVALID, "%s: bad paths/factors (%d, %d)", who, paths, factors);
+  return KOO_OK;
+}
+
+extern "C" int brownian_bridge_gpu(int steps, const int* allocation_indices, const float* in,
+                                   float* out, int paths, int factors, void* stream) {
+  if (int e = bridge_check("brownian_bridge_gpu", steps, paths, factors)) return e;
+  if (paths == 0) return KOO_OK;
+  if (!allocation_indices || !in || !out)
+    return set_error(KOO_ERR_INVALID, "brownian_bridge_gpu: null pointer");
+  const int threads = bridge_threads(steps);
+  const size_t lds = sizeof(float) * steps * threads;
+  hipLaunchKernelGGL(k_bridge, dim3((paths + threads - 1) / threads, factors), dim3(threads), lds,
+                     as_stream(stream), tables_for(steps), steps, factors, paths,
+                     allocation_indices, in, out);
+  return check_launch("brownian_bridge_gpu");
+}
+
+extern "C" int sobol_bridge_gpu(int paths, unsigned path_offset, int steps, int factors,
+                                const unsigned int* d_directions, const unsigned int* scramblers,
+                                int number_scrambled, const int* allocation_indices, float* out,
+                                void* stream) {
+  if (int e = bridge_check("sobol_bridge_gpu", steps, paths, factors)) return e;
+  if (paths == 0) return KOO_OK;
+  if (!d_directions || !allocation_indices || !out || (number_scrambled > 0 && !scramblers))
+    return set_error(KOO_ERR_INVALID, "sobol_bridge_gpu: null pointer");
+  const int threads = bridge_threads(steps);
+  const size_t lds = sizeof(float) * steps * threads + sizeof(uint32_t) * 32 * steps;
+  hipLaunchKernelGGL(k_sobol_bridge, dim3((paths + threads - 1) / threads, factors), dim3(threads),
+                     lds, as_stream(stream), tables_for(steps), steps,
+                     factors, paths, path_offset + 1u, d_directions, scramblers,
+                     number_scrambled, allocation_indices, out);
+  return check_launch("sobol_bridge_gpu");
+}

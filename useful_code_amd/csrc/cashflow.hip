@@ -1,0 +1,253 @@
+// cashflow.hip -- stage (c): product cash-flow generation, discounting to the
+// numeraire and aggregation onto exercise dates, for gfx950.
+//
+// One lane per path; product constants and per-step index/weight tables are
+// read with uniform addresses (scalar unit), per-path data is path-minor so
+// every wavefront access is one contiguous row.
+#include "koo_common.h"
+
+namespace koo {
+
+namespace {
+
+// Product state machines: getCashFlows(flow1, flow2, rate1, rate2, rate3)
+// returns true once the product has terminated on this path.
+
+// Swap of rate1 against a fixed coupon; aux = [K, payReceive, acc1[S], acc2[S]]
+// (cashFlowGeneration_product_gpu.h:128-188).
+struct SwapProduct {
+  const float* aux;
+  int steps, index;
+  float strike, pay_receive;
+  __device__ SwapProduct(const float* a, int s)
+      : aux(a), steps(s), index(0), strike(a[0]), pay_receive(a[1]) {}
+  __device__ bool flows(float& f1, float& f2, float r1, float, float) {
+    f2 = aux[2 + steps + index] * r1 * pay_receive;
+    f1 = -strike * aux[2 + index] * pay_receive;
+    ++index;
+    return index == steps;
+  }
+};
+
+// Target redemption note; aux = [total coupon, K, m, acc1[S], acc2[S]]
+// (cashFlowGeneration_product_gpu.h:19-108).
+struct TarnProduct {
+  const float* aux;
+  int steps, index;
+  float total, strike, mult, paid;
+  __device__ TarnProduct(const float* a, int s)
+      : aux(a), steps(s), index(0), total(a[0]), strike(a[1]), mult(a[2]), paid(0.0f) {}
+  __device__ bool flows(float& f1, float& f2, float r1, float r2, float) {
+    f2 = aux[3 + steps + index] * r2;
+    if (index + 1 == steps) {
+      f1 = paid - total;
+      return true;
+    }
+    float unc = (strike - mult * r1) * aux[3 + index];
+    if (unc < 0.0f) unc = 0.0f;
+    const float nt = paid + unc;
+    if (nt >= total) {
+      f1 = paid - total;
+      return true;
+    }
+    f1 = -unc;
+    paid = nt;
+    ++index;
+    return false;
+  }
+};
+
+// Caplet strip on rate1; aux = [K, acc[S]].  No reference counterpart.
+struct CapletProduct {
+  const float* aux;
+  int steps, index;
+  float strike;
+  __device__ CapletProduct(const float* a, int s) : aux(a), steps(s), index(0), strike(a[0]) {}
+  __device__ bool flows(float& f1, float& f2, float r1, float, float) {
+    f1 = aux[1 + index] * fmaxf(r1 - strike, 0.0f);
+    f2 = 0.0f;
+    ++index;
+    return index == steps;
+  }
+};
+
+// Driver loop of cashFlowGenerator_kernel (cashFlowGeneration_gpu.h:23-77).
+template <class Product>
+__global__ void __launch_bounds__(256)
+    k_cashflows(float* __restrict__ f1, float* __restrict__ f2, const float* __restrict__ aux,
+                int paths, int steps, const float* __restrict__ r1, const float* __restrict__ r2,
+                const float* __restrict__ r3) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  Product obj(aux, steps);
+  bool done = false;
+  size_t o = p;
+  for (int i = 0; i < steps; ++i, o += paths) {
+    float a = 0.0f, b = 0.0f;
+    if (!done) done = obj.flows(a, b, r1[o], r2 ? r2[o] : 0.0f, r3 ? r3[o] : 0.0f);
+    f1[o] = a;
+    f2[o] = b;
+  }
+}
+
+// df = (P1/P_s)^(1-theta) (P2/P_s)^theta / B_s plus per-path sum
+// (cashFlowDiscounting_gpu.cu:170-258).
+__global__ void __launch_bounds__(256)
+    k_discounting(const int* __restrict__ first, const int* __restrict__ second,
+                  const float* __restrict__ theta, const float* __restrict__ disc,
+                  const float* __restrict__ flows, const float* __restrict__ num, int paths,
+                  int steps, float* __restrict__ out, float* __restrict__ summed) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const size_t P = static_cast<size_t>(paths);
+  float total = 0.0f;
+  for (int s = 0; s < steps; ++s) {
+    const float* dcol = disc + static_cast<size_t>(s) * (steps + 1) * P + p;
+    const float now = dcol[static_cast<size_t>(s) * P];
+    const float r1 = dcol[static_cast<size_t>(first[s]) * P] / now;
+    const float r2 = dcol[static_cast<size_t>(second[s]) * P] / now;
+    const float th = theta[s];
+    const float rdf = powf(r1, 1.0f - th) * powf(r2, th);
+    const size_t o = static_cast<size_t>(s) * P + p;
+    const float fl = flows[o] * (rdf / num[o]);
+    total += fl;
+    out[o] = fl;
+  }
+  summed[p] = total;
+}
+
+// Partial discounting to targetIndices via the numeraire
+// (cashFlowDiscounting_gpu.cu:369-440).
+__global__ void __launch_bounds__(256)
+    k_discounting_partial(int n_flows, const int* __restrict__ gen, const int* __restrict__ first,
+                          const int* __restrict__ second, const float* __restrict__ theta,
+                          const int* __restrict__ target, const float* __restrict__ disc,
+                          const float* __restrict__ flows, const float* __restrict__ num,
+                          int paths, int steps, float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const size_t P = static_cast<size_t>(paths);
+  for (int k = 0; k < n_flows; ++k) {
+    const int i = gen[k];
+    const float* dcol = disc + static_cast<size_t>(i) * (steps + 1) * P + p;
+    const float now = dcol[static_cast<size_t>(i) * P];
+    const float r1 = dcol[static_cast<size_t>(first[k]) * P] / now;
+    const float r2 = dcol[static_cast<size_t>(second[k]) * P] / now;
+    const float th = theta[k];
+    const float rdf = powf(r1, 1.0f - th) * powf(r2, th);
+    float fl = flows[static_cast<size_t>(k) * P + p] * (rdf / now);
+    fl *= num[static_cast<size_t>(target[k]) * P + p] / num[static_cast<size_t>(i) * P + p];
+    out[static_cast<size_t>(k) * P + p] = fl;
+  }
+}
+
+// aggregated(j, offset + p) += flows(i, p), j = preceding[i]
+// (cashFlowAggregation_gpu.cu:22-53); negative indices (flows before the
+// first exercise date) are skipped as the gold twin does.
+__global__ void __launch_bounds__(256)
+    k_aggregate(float* __restrict__ agg, int total_paths, int n_ex,
+                const float* __restrict__ flows, int batch, int offset, int steps,
+                const int* __restrict__ preceding, int start) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= batch) return;
+  for (int i = start; i < steps; ++i) {
+    const int j = preceding[i];
+    if (j < 0 || j >= n_ex) continue;
+    agg[static_cast<size_t>(j) * total_paths + p + offset] += flows[static_cast<size_t>(i) * batch + p];
+  }
+}
+
+template <class Product>
+int launch_cashflows(const char* who, float* f1, float* f2, float* aux, int paths, int steps,
+                     float* r1, float* r2, float* r3, void* stream) {
+  if (paths < 0 || steps < 0) return set_error(KOO_ERR_INVALID, "%s: bad sizes", who);
+  if (paths == 0 || steps == 0) return KOO_OK;
+  if (!f1 || !f2 || !aux || !r1) return set_error(KOO_ERR_INVALID, "%s: null pointer", who);
+  hipLaunchKernelGGL(k_cashflows<Product>, dim3((paths + 255) / 256), dim3(256), 0,
+                     as_stream(stream), f1, f2, aux, paths, steps, r1, r2, r3);
+  return check_launch(who);
+}
+
+}  // namespace
+}  // namespace koo
+
+using namespace koo;
+
+extern "C" int cashFlowGeneratorCallerSwap(float* genFlows1, float* genFlows2, float* aux_data,
+                                           int paths, int numberSteps, float* rates1,
+                                           float* rates2, float* rates3, float* /*forwards*/,
+                                           float* /*discRatios*/, void* stream) {
+  return launch_cashflows<SwapProduct>("cashFlowGeneratorCallerSwap", genFlows1, genFlows2,
+                                       aux_data, paths, numberSteps, rates1, rates2, rates3,
+                                       stream);
+}
+
+extern "C" int cashFlowGeneratorCallerTARN(float* genFlows1, float* genFlows2, float* aux_data,
+                                           int paths, int numberSteps, float* rates1,
+                                           float* rates2, float* rates3, float* /*forwards*/,
+                                           float* /*discRatios*/, void* stream) {
+  if (!rates2) return set_error(KOO_ERR_INVALID, "cashFlowGeneratorCallerTARN: rates2 is null");
+  return launch_cashflows<TarnProduct>("cashFlowGeneratorCallerTARN", genFlows1, genFlows2,
+                                       aux_data, paths, numberSteps, rates1, rates2, rates3,
+                                       stream);
+}
+
+extern "C" int cashFlowGeneratorCallerCaplet(float* genFlows1, float* genFlows2, float* aux_data,
+                                             int paths, int numberSteps, float* rates1,
+                                             float* rates2, float* rates3, float* /*forwards*/,
+                                             float* /*discRatios*/, void* stream) {
+  return launch_cashflows<CapletProduct>("cashFlowGeneratorCallerCaplet", genFlows1, genFlows2,
+                                         aux_data, paths, numberSteps, rates1, rates2, rates3,
+                                         stream);
+}
+
+extern "C" int cashFlowDiscounting_gpu(int* firstIndex, int* secondIndex, float* theta,
+                                       float* discountRatios, float* genFlows,
+                                       float* numeraireValues, int paths, int numberSteps,
+                                       int /*useTextures*/, int /*useShared*/,
+                                       float* discountedFlows, float* summedDiscountedFlows,
+                                       void* stream) {
+  if (paths < 0 || numberSteps < 0) return set_error(KOO_ERR_INVALID, "cashFlowDiscounting_gpu: sizes");
+  if (paths == 0 || numberSteps == 0) return KOO_OK;
+  if (!firstIndex || !secondIndex || !theta || !discountRatios || !genFlows || !numeraireValues ||
+      !discountedFlows || !summedDiscountedFlows)
+    return set_error(KOO_ERR_INVALID, "cashFlowDiscounting_gpu: null pointer");
+  hipLaunchKernelGGL(k_discounting, dim3((paths + 255) / 256), dim3(256), 0, as_stream(stream),
+                     firstIndex, secondIndex, theta, discountRatios, genFlows, numeraireValues,
+                     paths, numberSteps, discountedFlows, summedDiscountedFlows);
+  return check_launch("cashFlowDiscounting_gpu");
+}
+
+extern "C" int cashFlowDiscounting_partial_gpu(int numberFlowsPerPath, int* genIndices,
+                                               int* firstIndex, int* secondIndex, float* theta,
+                                               int* targetIndices, float* discountRatios,
+                                               float* genFlows, float* numeraireValues, int paths,
+                                               int numberSteps, float* discountedFlows,
+                                               void* stream) {
+  if (paths < 0 || numberSteps < 0 || numberFlowsPerPath < 0)
+    return set_error(KOO_ERR_INVALID, "cashFlowDiscounting_partial_gpu: sizes");
+  if (paths == 0 || numberFlowsPerPath == 0) return KOO_OK;
+  if (!genIndices || !firstIndex || !secondIndex || !theta || !targetIndices || !discountRatios ||
+      !genFlows || !numeraireValues || !discountedFlows)
+    return set_error(KOO_ERR_INVALID, "cashFlowDiscounting_partial_gpu: null pointer");
+  hipLaunchKernelGGL(k_discounting_partial, dim3((paths + 255) / 256), dim3(256), 0,
+                     as_stream(stream), numberFlowsPerPath, genIndices, firstIndex, secondIndex,
+                     theta, targetIndices, discountRatios, genFlows, numeraireValues, paths,
+                     numberSteps, discountedFlows);
+  return check_launch("cashFlowDiscounting_partial_gpu");
+}
+
+extern "C" int AggregateFlows_gpu(float* aggregatedFlows, int totalPaths, int numberExerciseDates,
+                                  float* genFlows, int batchPaths, int offSetForOutput,
+                                  int numberSteps, int* precedingIndex, int start, void* stream) {
+  if (batchPaths < 0 || numberSteps < 0 || offSetForOutput < 0 || start < 0 ||
+      offSetForOutput + batchPaths > totalPaths)
+    return set_error(KOO_ERR_INVALID, "AggregateFlows_gpu: bad sizes");
+  if (batchPaths == 0 || numberSteps == 0) return KOO_OK;
+  if (!aggregatedFlows || !genFlows || !precedingIndex)
+    return set_error(KOO_ERR_INVALID, "AggregateFlows_gpu: null pointer");
+  hipLaunchKernelGGL(k_aggregate, dim3((batchPaths + 255) / 256), dim3(256), 0, as_stream(stream),
+                     aggregatedFlows, totalPaths, numberExerciseDates, genFlows, batchPaths,
+                     offSetForOutput, numberSteps, precedingIndex, start);
+  return check_launch("AggregateFlows_gpu");
+}

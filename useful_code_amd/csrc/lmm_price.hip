@@ -1,0 +1,472 @@
+// lmm_price.hip -- fused LMM pricing with pathwise adjoint Greeks, gfx950.
+//
+// Forward (k_price_fwd): one lane per path evolves the whole rate curve with
+// the predictor-corrector log-Euler step of LMM_evolver_all.cu:571-742 (spot
+// measure, displaced diffusion), prices the product on the fly and never
+// writes the rate cube: its only HBM traffic is the bridged variates (S,F,P)
+// and, when asked, one checkpoint of the log-rate state every k steps.
+//
+// Backward (k_price_bwd): one reverse sweep per path.  For each checkpoint
+// segment, last first, the k steps are recomputed from the checkpoint and
+// each step's start state and final drift sums are kept in a scratch buffer;
+// then the steps are reversed one by one, rates in descending order, with the
+// running drift sums e_f, e^_f un-accumulated and the adjoint running sums
+// E_f, E^_f accumulated in registers.  Deltas come from the adjoint of the
+// initial log-rates; vegas from sum_{s,f} Abar(s,r,f) A(s,r,f) / sigma_r,
+// whose per-rate contraction needs only quantities the reverse step already
+// has (no per-factor Abar is formed).
+//
+// The model's uniform data (pseudo-roots, taus, displacements) is read with
+// uniform addresses (scalar loads); the per-path state lives in LDS laid out
+// [rate][lane] so wavefront accesses are conflict free; block partial sums
+// are formed in FP64 and a final single pass sums them with compensation.
+#include "koo_common.h"
+
+namespace koo {
+
+namespace {
+
+constexpr int kMaxRates = 128;
+constexpr int kMaxFactors = 8;
+
+__device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ float dpp_f32_quad_xor1(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_f32_quad_xor2(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_f32_row_half_mirror(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_f32_row_mirror(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float readlane_f32(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+
+// Sum over the 64 lanes of a wavefront with DPP row operations (quad
+// butterflies, then half-row and row mirrors give each 16-lane row its sum;
+// the four row sums are read from lanes 0, 16, 32, 48).  Returns the total,
+// uniform across the wavefront.  All 64 lanes must be active.
+__device__ __forceinline__ float wave_sum_f32(float v) {
+  v += dpp_f32_quad_xor1(v);
+  v += dpp_f32_quad_xor2(v);
+  v += dpp_f32_row_half_mirror(v);
+  v += dpp_f32_row_mirror(v);
+  return (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide FP64 sum of one value per lane into part[col]; every lane of the
+// block must call it (contains barriers).
+__device__ __forceinline__ void block_sum_to(double v, double* scratch_lds, double* out) {
+  v = wave_sum_f64(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch_lds[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) t += scratch_lds[i];
+    *out = t;
+  }
+}
+
+// Caplet value at fixing, tau max(f-K,0)/(1+tau f), and its f-derivative.
+__device__ __forceinline__ float caplet_value(float f, float tau, float K, float inv) {
+  return f > K ? tau * (f - K) * inv : 0.0f;
+}
+__device__ __forceinline__ float caplet_deriv(float f, float tau, float K, float inv) {
+  return f > K ? tau * (1.0f + tau * K) * inv * inv : 0.0f;
+}
+
+struct Model {
+  const float* z;     // (S,F,P) bridged variates
+  const float* f0;    // N
+  const float* taus;  // N
+  const float* disp;  // N
+  const float* A;     // (S,N,F)
+  float strike;
+  int P, N, k;
+};
+
+// One predictor-corrector step s over rates r >= s.  `load(r)` gives the
+// start log-rate, `store(r, l)` receives the end log-rate; e / ep return the
+// final drift sums.
+template <int F, class Load, class Store>
+__device__ __forceinline__ void pc_step(const Model& m, int s, const float (&z)[F], float (&e)[F],
+                                        float (&ep)[F], Load load, Store store) {
+  const float* As = m.A + static_cast<size_t>(s) * m.N * F;
+#pragma unroll
+  for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
+  for (int r = s; r < m.N; ++r) {
+    float a[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
+    const float tau = m.taus[r], d = m.disp[r];
+    const float l = load(r);
+    const float x = fast_exp(l);
+    const float g = tau * x * fast_rcp(1.0f + tau * (x - d));
+    float mu = 0.0f, dw = 0.0f, c = 0.0f;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      e[f] += g * a[f];
+      mu += a[f] * e[f];
+      dw += a[f] * z[f];
+      c -= 0.5f * a[f] * a[f];
+    }
+    const float lh = l + mu + c + dw;
+    const float xh = fast_exp(lh);
+    const float gh = tau * xh * fast_rcp(1.0f + tau * (xh - d));
+    float muh = 0.0f;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      ep[f] += gh * a[f];
+      muh += a[f] * ep[f];
+    }
+    store(r, lh + 0.5f * (muh - mu));
+  }
+}
+
+template <int F>
+__device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[F]) {
+  const float* zs = m.z + static_cast<size_t>(s) * F * m.P + p;
+#pragma unroll
+  for (int f = 0; f < F; ++f) z[f] = zs[static_cast<size_t>(f) * m.P];
+}
+
+// ----------------------------------------------------------------------
+// Forward: price per path, block partial sums, optional checkpoints
+// ckpt[seg][r][P] (r < N: log-rate at the start of step seg*k; row N: the
+// running discount 1/B at that step).
+// ----------------------------------------------------------------------
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_price_fwd(Model m, float* __restrict__ ckpt, double* __restrict__ part) {
+  extern __shared__ float lds[];
+  __shared__ double red[4];
+  const int bd = blockDim.x;
+  const int p = blockIdx.x * bd + threadIdx.x;
+  const bool valid = p < m.P;
+  const int pc = valid ? p : m.P - 1;
+  float* L = lds + threadIdx.x;
+  const size_t P = static_cast<size_t>(m.P);
+  for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
+  float invB = 1.0f, y = 0.0f, yc = 0.0f;
+  for (int s = 0; s < m.N; ++s) {
+    if (ckpt && valid && (s % m.k) == 0) {
+      float* c = ckpt + static_cast<size_t>(s / m.k) * (m.N + 1) * P + p;
+      for (int r = s; r < m.N; ++r) c[static_cast<size_t>(r) * P] = L[r * bd];
+      c[static_cast<size_t>(m.N) * P] = invB;
+    }
+    float z[F], e[F], ep[F];
+    load_z<F>(m, s, pc, z);
+    pc_step<F>(m, s, z, e, ep, [&](int r) { return L[r * bd]; },
+               [&](int r, float v) { L[r * bd] = v; });
+    const float tau = m.taus[s];
+    const float fs = fast_exp(L[s * bd]) - m.disp[s];
+    const float inv = fast_rcp(1.0f + tau * fs);
+    // Kahan-compensated path sum of the deflated caplet values.
+    const float term = caplet_value(fs, tau, m.strike, inv) * invB - yc;
+    const float t = y + term;
+    yc = (t - y) - term;
+    y = t;
+    invB *= inv;
+  }
+  block_sum_to(valid ? static_cast<double>(y) : 0.0, red, part + blockIdx.x);
+}
+
+// ----------------------------------------------------------------------
+// Backward.  scratch slot j (j < k) holds rows [0,N): start log-rates of step
+// s0+j, row N: 1/B at that step, rows N+1..N+2F: final sums e, e^ of step
+// s0+j.  Slot 0's state rows are not used (the checkpoint is read instead).
+// ----------------------------------------------------------------------
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_price_bwd(Model m, const float* __restrict__ vols, const float* __restrict__ ckpt,
+                float* __restrict__ scratch, double* __restrict__ part) {
+  extern __shared__ float lds[];
+  __shared__ double red[4];
+  const int bd = blockDim.x;
+  const int nw = bd >> 6;
+  const int p = blockIdx.x * bd + threadIdx.x;
+  const bool valid = p < m.P;
+  const int pc = valid ? p : m.P - 1;
+  const int N = m.N;
+  const size_t P = static_cast<size_t>(m.P);
+  float* lam = lds + threadIdx.x;                            // lam[r * bd]
+  float* wv = lds + static_cast<size_t>(N) * bd + (threadIdx.x >> 6) * N;  // per-wave vega sums
+  for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
+  for (int i = threadIdx.x; i < nw * N; i += bd) lds[static_cast<size_t>(N) * bd + i] = 0.0f;
+  __syncthreads();
+
+  const size_t slot = static_cast<size_t>(N + 1 + 2 * F) * P;
+  const int nseg = (N + m.k - 1) / m.k;
+  float tail = 0.0f;
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int s0 = seg * m.k;
+    const int s1 = min(N, s0 + m.k);
+    const float* ck = ckpt + static_cast<size_t>(seg) * (N + 1) * P + pc;
+    // Recompute the segment: state (and 1/B) at the start of step s0+j+1
+    // into slot j+1, final drift sums of step s0+j into slot j.
+    for (int s = s0; s < s1; ++s) {
+      const int j = s - s0;
+      const float* src = (j == 0) ? ck : scratch + j * slot + pc;
+      float* dst = scratch + (j + 1) * slot + p;
+      float* sums = scratch + j * slot + static_cast<size_t>(N + 1) * P + p;
+      const bool keep = valid && (s + 1 < s1);
+      float z[F], e[F], ep[F];
+      float l_fix = 0.0f;
+      load_z<F>(m, s, pc, z);
+      pc_step<F>(m, s, z, e, ep, [&](int r) { return src[static_cast<size_t>(r) * P]; },
+                 [&](int r, float v) {
+                   if (r == s) l_fix = v;
+                   else if (keep) dst[static_cast<size_t>(r) * P] = v;
+                 });
+      if (valid) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          sums[static_cast<size_t>(f) * P] = e[f];
+          sums[static_cast<size_t>(F + f) * P] = ep[f];
+        }
+      }
+      if (keep) {
+        const float fs = fast_exp(l_fix) - m.disp[s];
+        dst[static_cast<size_t>(N) * P] =
+            src[static_cast<size_t>(N) * P] * fast_rcp(1.0f + m.taus[s] * fs);
+      }
+    }
+    // Reverse the segment's steps.
+    for (int s = s1 - 1; s >= s0; --s) {
+      const int j = s - s0;
+      const float* src = (j == 0) ? ck : scratch + j * slot + pc;
+      const float* sums = scratch + j * slot + static_cast<size_t>(N + 1) * P + pc;
+      const float invBs = src[static_cast<size_t>(N) * P];
+      float z[F], e[F], ep[F], E[F], Eh[F];
+      load_z<F>(m, s, pc, z);
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        e[f] = sums[static_cast<size_t>(f) * P];
+        ep[f] = sums[static_cast<size_t>(F + f) * P];
+        E[f] = Eh[f] = 0.0f;
+      }
+      const float* As = m.A + static_cast<size_t>(s) * N * F;
+      float* wrow = wv;
+      for (int r = N - 1; r >= s; --r) {
+        float a[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
+        const float tau = m.taus[r], d = m.disp[r];
+        const float omd = 1.0f - tau * d;
+        const float l = src[static_cast<size_t>(r) * P];
+        const float x = fast_exp(l);
+        const float inv1 = fast_rcp(1.0f + tau * (x - d));
+        const float g = tau * x * inv1;
+        float mu = 0.0f, muh = 0.0f, dw = 0.0f, c = 0.0f;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          mu += a[f] * e[f];
+          muh += a[f] * ep[f];
+          dw += a[f] * z[f];
+          c -= 0.5f * a[f] * a[f];
+        }
+        const float lh = l + mu + c + dw;
+        const float xh = fast_exp(lh);
+        const float inv2 = fast_rcp(1.0f + tau * (xh - d));
+        const float gh = tau * xh * inv2;
+        float lp = lam[r * bd];
+        if (r == s) {
+          // The rate fixing at the end of this step: caplet value and the
+          // numeraire it enters for later fixings.
+          const float xs = fast_exp(lh + 0.5f * (muh - mu));
+          const float fs = xs - d;
+          const float inv3 = fast_rcp(1.0f + tau * fs);
+          const float v = caplet_value(fs, tau, m.strike, inv3);
+          const float dv = caplet_deriv(fs, tau, m.strike, inv3);
+          lp += (dv * invBs - tau * inv3 * tail) * xs;
+          tail += v * invBs;
+        }
+        float ghb = 0.0f;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          Eh[f] += 0.5f * lp * a[f];
+          ghb += a[f] * Eh[f];
+        }
+        const float lhat = lp + ghb * gh * omd * inv2;
+        const float mub = lhat - 0.5f * lp;
+        float gb = 0.0f;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          E[f] += mub * a[f];
+          gb += a[f] * E[f];
+        }
+        lam[r * bd] = lhat + gb * g * omd * inv1;
+        // sum_f Abar_f a_f for this (s, r).
+        float w = 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c);
+        w = valid ? w : 0.0f;
+        const float ws = wave_sum_f32(w);
+        if ((threadIdx.x & 63) == 0) wrow[r] += ws;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          e[f] -= g * a[f];
+          ep[f] -= gh * a[f];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // Deltas: d/df0_r = lam_r / (f0_r + d_r); vegas: wave sums / sigma_r.
+  for (int r = 0; r < N; ++r) {
+    const float dl = lam[r * bd] / (m.f0[r] + m.disp[r]);
+    block_sum_to(valid ? static_cast<double>(dl) : 0.0, red,
+                 part + static_cast<size_t>(blockIdx.x) * 2 * N + r);
+  }
+  for (int r = threadIdx.x; r < N; r += bd) {
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += lds[static_cast<size_t>(N) * bd + w * N + r];
+    part[static_cast<size_t>(blockIdx.x) * 2 * N + N + r] = t / vols[r];
+  }
+}
+
+// out[c] = (sum_b part[b*ncols + c]) * scale, compensated; one block per column.
+__global__ void __launch_bounds__(256)
+    k_finalize_cols(const double* __restrict__ part, int nb, int ncols, double scale,
+                    double* __restrict__ out) {
+  __shared__ double sv[256], sc[256];
+  const int col = blockIdx.x;
+  double s = 0.0, c = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const double x = part[static_cast<size_t>(b) * ncols + col];
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  sv[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S = 0.0, C = 0.0;
+    for (int i = 0; i < static_cast<int>(blockDim.x); ++i) {
+      const double x = sv[i];
+      const double t = S + x;
+      C += (fabs(S) >= fabs(x)) ? (S - t) + x : (x - t) + S;
+      S = t;
+      C += sc[i];
+    }
+    out[col] = (S + C) * scale;
+  }
+}
+
+// Lanes per block keeping the [rate][lane] LDS state within 60 KiB.
+int fwd_threads(int N) { return N <= 60 ? 256 : (N <= 120 ? 128 : 64); }
+int bwd_threads(int N) { return N <= 56 ? 256 : (N <= 112 ? 128 : 64); }
+
+int check_model(const char* who, int paths, int rates, int factors, int k, int product) {
+  if (paths < 1 || rates < 1 || rates > kMaxRates || factors < 1 || factors > kMaxFactors || k < 1)
+    return set_error(KOO_ERR_INVALID,
+                     "%s: need paths>=1, 1<=rates<=%d, 1<=factors<=%d, checkpoint_every>=1 "
+                     "(got %d, %d, %d, %d)",
+                     who, kMaxRates, kMaxFactors, paths, rates, factors, k);
+  if (product != KOO_PRODUCT_CAPLETS)
+    return set_error(KOO_ERR_INVALID, "%s: unknown product %d", who, product);
+  return KOO_OK;
+}
+
+}  // namespace
+}  // namespace koo
+
+using namespace koo;
+
+extern "C" int lmm_price_workspace(int paths, int rates, int factors, int checkpoint_every,
+                                   size_t* checkpoint_bytes, size_t* scratch_bytes,
+                                   size_t* partial_bytes) {
+  if (int e = check_model("lmm_price_workspace", paths, rates, factors, checkpoint_every,
+                          KOO_PRODUCT_CAPLETS))
+    return e;
+  const size_t P = static_cast<size_t>(paths);
+  const int k = checkpoint_every;
+  const size_t nseg = (rates + k - 1) / k;
+  if (checkpoint_bytes) *checkpoint_bytes = sizeof(float) * nseg * (rates + 1) * P;
+  if (scratch_bytes)
+    *scratch_bytes = sizeof(float) * static_cast<size_t>(k < rates ? k : rates) *
+                     (rates + 1 + 2 * factors) * P;
+  const size_t bf = (P + fwd_threads(rates) - 1) / fwd_threads(rates);
+  const size_t bb = (P + bwd_threads(rates) - 1) / bwd_threads(rates);
+  const size_t need = bf > bb * 2 * rates ? bf : bb * 2 * rates;
+  if (partial_bytes) *partial_bytes = sizeof(double) * need;
+  return KOO_OK;
+}
+
+extern "C" int lmm_price_fwd(int paths, int rates, int factors, int checkpoint_every,
+                             const float* variates, const float* initial_rates, const float* taus,
+                             const float* displacements, const float* pseudo_roots, int product,
+                             const float* product_aux, float* checkpoints, double* partials,
+                             double* price_out, void* stream) {
+  if (int e = check_model("lmm_price_fwd", paths, rates, factors, checkpoint_every, product))
+    return e;
+  if (!variates || !initial_rates || !taus || !displacements || !pseudo_roots || !product_aux ||
+      !partials || !price_out)
+    return set_error(KOO_ERR_INVALID, "lmm_price_fwd: null pointer");
+  const float strike = product_aux[0];
+  Model m{variates, initial_rates, taus, displacements, pseudo_roots, strike, paths, rates,
+          checkpoint_every};
+  const int threads = fwd_threads(rates);
+  const int blocks = (paths + threads - 1) / threads;
+  const size_t lds = sizeof(float) * rates * threads;
+  hipStream_t st = as_stream(stream);
+  switch (factors) {
+#define KOO_CASE(F)                                                                              \
+  case F:                                                                                        \
+    hipLaunchKernelGGL(k_price_fwd<F>, dim3(blocks), dim3(threads), lds, st, m, checkpoints,    \
+                       partials);                                                               \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
+    KOO_CASE(8)
+#undef KOO_CASE
+  }
+  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, partials, blocks, 1,
+                     1.0 / static_cast<double>(paths), price_out);
+  return check_launch("lmm_price_fwd");
+}
+
+extern "C" int lmm_price_bwd(int paths, int rates, int factors, int checkpoint_every,
+                             const float* variates, const float* initial_rates, const float* taus,
+                             const float* displacements, const float* pseudo_roots,
+                             const float* vols, int product, const float* product_aux,
+                             const float* checkpoints, float* segment_scratch, double* partials,
+                             double* greeks_out, void* stream) {
+  if (int e = check_model("lmm_price_bwd", paths, rates, factors, checkpoint_every, product))
+    return e;
+  if (!variates || !initial_rates || !taus || !displacements || !pseudo_roots || !vols ||
+      !product_aux || !checkpoints || !segment_scratch || !partials || !greeks_out)
+    return set_error(KOO_ERR_INVALID, "lmm_price_bwd: null pointer");
+  const float strike = product_aux[0];
+  Model m{variates, initial_rates, taus, displacements, pseudo_roots, strike, paths, rates,
+          checkpoint_every};
+  const int threads = bwd_threads(rates);
+  const int blocks = (paths + threads - 1) / threads;
+  const size_t lds = sizeof(float) * (static_cast<size_t>(rates) * threads + (threads / 64) * rates);
+  hipStream_t st = as_stream(stream);
+  switch (factors) {
+#define KOO_CASE(F)                                                                              \
+  case F:                                                                                        \
+    hipLaunchKernelGGL(k_price_bwd<F>, dim3(blocks), dim3(threads), lds, st, m, vols,           \
+                       checkpoints, segment_scratch, partials);                                 \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
+    KOO_CASE(8)
+#undef KOO_CASE
+  }
+  hipLaunchKernelGGL(k_finalize_cols, dim3(2 * rates), dim3(256), 0, st, partials, blocks,
+                     2 * rates, 1.0 / static_cast<double>(paths), greeks_out);
+  return check_launch("lmm_price_bwd");
+}

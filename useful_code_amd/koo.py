@@ -1,0 +1,381 @@
+"""ctypes binding of libkoo_amd.so and torch-tensor wrappers.
+
+Every wrapper keeps the reference launcher's name and argument order
+(include/koo_c_abi.h cites the reference header of each), takes torch tensors
+on the GPU for device buffers, launches on torch's current stream and raises
+``KooError`` (with the library's message) on a non-zero status.  Nothing here
+computes on the CPU: if the library is missing, ``lib()`` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkoo_amd.so")
+_LIB = None
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_u = ctypes.c_uint
+_ll = ctypes.c_longlong
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+
+
+class KooError(RuntimeError):
+    pass
+
+
+# name -> (restype, argtypes); pointers are passed as integers (c_void_p).
+_SIGNATURES = {
+    "koo_last_error": (ctypes.c_char_p, []),
+    "koo_version": (ctypes.c_char_p, []),
+    # stage (a)
+    "initSobolDirectionVectors": (_i, [_i, _p]),
+    "sobolGPUInts": (_i, [_i, _i, _u, _p, _p, _p]),
+    "scramble_gpu": (_i, [_p, _p, _i, _i, _p]),
+    "inverse_normal_u32_gpu": (_i, [_p, _p, _ll, _p]),
+    "sobol_scramble_normal_gpu": (_i, [_i, _i, _u, _p, _p, _i, _p, _p]),
+    # host set-up
+    "koo_exp_corr": (_i, [_p, _i, _d, _d, _p]),
+    "koo_pseudo_sqrt": (_i, [_i, _p, _i, _i, _p]),
+    "FlatVolPseudoRootsOfCovariances": (_i, [_p, _p, _p, _i, _i, _i, _d, _d, _p]),
+    "koo_factor_pseudo_roots": (_i, [_p, _p, _i, _i, _d, _d, _p, _p]),
+    "koo_fixed_drifts": (_i, [_p, _i, _i, _i, _p]),
+    "spotDriftComputer_getDrifts": (_i, [_p, _p, _p, _p, _i, _i, _i, _p]),
+    # stage (b)
+    "bridge_allocate_diagonal": (_i, [_i, _i, _p]),
+    "brownian_bridge_gpu": (_i, [_i, _p, _p, _p, _i, _i, _p]),
+    "sobol_bridge_gpu": (_i, [_i, _u, _i, _i, _p, _p, _i, _p, _p, _p]),
+    "correlated_drift_paths_GPU": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _p]),
+    "LMM_evolver_pc_all_gpu": (_i, [_p] * 9 + [_i] * 4 + [_p] * 5 + [_i, _i, _p, _i, _i]),
+    "discount_ratios_computation_gpu": (_i, [_p, _p, _p, _p, _i, _i, _i, _p, _i, _p]),
+    "coterminal_annuity_ratios_computation_gpu": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
+    "coterminal_swap_rate_computation_gpu": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
+    "swap_rate_computation_gpu": (_i, [_p, _p, _i, _i, _i, _i, _i, _i, _p, _p]),
+    "spot_measure_numeraires_computation_gpu": (_i, [_p, _p, _i, _i, _p]),
+    "spot_measure_numeraires_computation_offset_gpu": (_i, [_p, _p, _i, _i, _i, _i, _p]),
+    "forward_rate_extraction_gpu": (_i, [_p, _p, _i, _i, _i, _p, _p]),
+    "forward_rate_extraction_selecting_step_gpu": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
+    "spot_measure_numeraires_extraction_gpu": (_i, [_p, _p, _i, _i, _i, _i, _i, _p, _p]),
+    # stage (c)
+    "cashFlowGeneratorCallerSwap": (_i, [_p] * 3 + [_i, _i] + [_p] * 5 + [_p]),
+    "cashFlowGeneratorCallerTARN": (_i, [_p] * 3 + [_i, _i] + [_p] * 5 + [_p]),
+    "cashFlowGeneratorCallerCaplet": (_i, [_p] * 3 + [_i, _i] + [_p] * 5 + [_p]),
+    "cashFlowDiscounting_gpu": (_i, [_p] * 6 + [_i, _i, _i, _i, _p, _p, _p]),
+    "cashFlowDiscounting_partial_gpu": (_i, [_i] + [_p] * 8 + [_i, _i, _p, _p]),
+    "AggregateFlows_gpu": (_i, [_p, _i, _i, _p, _i, _i, _i, _p, _i, _p]),
+    "PartialReduce_using8s_gpu": (_i, [_p, _p, _i, _i, _i, _p, _p, _p]),
+    "PartialReduce_using16s_gpu": (_i, [_p, _p, _i, _i, _i, _p, _p, _p]),
+    "reduceSinglePass": (_i, [_i, _i, _i, _p, _p, _p, _p]),
+    "koo_sum_workspace_doubles": (_ll, [_ll]),
+    "koo_sum_f32_to_f64": (_i, [_p, _ll, _p, _p, _p]),
+    "koo_sum_f64": (_i, [_p, _ll, _p, _p, _p]),
+    # fused pricing + adjoint
+    "lmm_price_workspace": (_i, [_i, _i, _i, _i, _p, _p, _p]),
+    "lmm_price_fwd": (_i, [_i, _i, _i, _i] + [_p] * 5 + [_i] + [_p] * 5),
+    "lmm_price_bwd": (_i, [_i, _i, _i, _i] + [_p] * 6 + [_i] + [_p] * 6),
+}
+
+
+def lib():
+    """The loaded libkoo_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise KooError(f"{LIB_PATH} is missing: build it with `python -m useful_code_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def check(status, name):
+    if status != 0:
+        msg = lib().koo_last_error().decode(errors="replace")
+        raise KooError(f"{name} failed ({status}): {msg}")
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def ptr(t):
+    """Device (or host numpy) address of a tensor/array; None -> NULL."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+def stream_handle(stream=None):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream
+
+
+# ------------------------------------------------------------ host set-up
+
+def _f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def sobol_directions(n_dims):
+    out = np.zeros(n_dims * 32, np.uint32)
+    call("initSobolDirectionVectors", n_dims, out.ctypes.data)
+    return out
+
+
+def bridge_allocate_diagonal(factors, steps):
+    out = np.zeros(factors * steps, np.int32)
+    call("bridge_allocate_diagonal", factors, steps, out.ctypes.data)
+    return out
+
+
+def exp_corr(times, L, beta):
+    t = _f64(times)
+    out = np.zeros((t.size, t.size))
+    call("koo_exp_corr", t.ctypes.data, t.size, L, beta, out.ctypes.data)
+    return out
+
+
+def pseudo_sqrt(cov, factors, normalise=True):
+    c = _f64(cov)
+    out = np.zeros((c.shape[0], factors))
+    call("koo_pseudo_sqrt", c.shape[0], c.ctypes.data, factors, int(normalise), out.ctypes.data)
+    return out
+
+
+def flat_vol_pseudo_roots(rate_times, evol_times, vols, factors, L, beta):
+    rt, et, v = _f64(rate_times), _f64(evol_times), _f64(vols)
+    A = np.zeros((et.size, v.size, factors))
+    call("FlatVolPseudoRootsOfCovariances", rt.ctypes.data, et.ctypes.data, v.ctypes.data, v.size,
+         et.size, factors, L, beta, A.ctypes.data)
+    return A
+
+
+def factor_pseudo_roots(rate_times, vols, factors, L, beta):
+    rt, v = _f64(rate_times), _f64(vols)
+    A = np.zeros((v.size, v.size, factors))
+    B = np.zeros((v.size, factors))
+    call("koo_factor_pseudo_roots", rt.ctypes.data, v.ctypes.data, v.size, factors, L, beta,
+         A.ctypes.data, B.ctypes.data)
+    return A, B
+
+
+def fixed_drifts(A):
+    A = _f64(A)
+    S, N, F = A.shape
+    out = np.zeros((S, N))
+    call("koo_fixed_drifts", A.ctypes.data, S, N, F, out.ctypes.data)
+    return out
+
+
+def spot_drifts(A_step, rates, taus, disp, alive=0):
+    A_step = _f64(A_step)
+    N, F = A_step.shape
+    r, t, d = _f64(rates), _f64(taus), _f64(disp)
+    out = np.zeros(N)
+    call("spotDriftComputer_getDrifts", A_step.ctypes.data, r.ctypes.data, t.ctypes.data,
+         d.ctypes.data, F, N, alive, out.ctypes.data)
+    return out
+
+
+# ------------------------------------------------------------ device ops
+
+def sobolGPUInts(n_vectors, n_dimensions, path_offset, d_directions, d_output, stream=None):
+    call("sobolGPUInts", n_vectors, n_dimensions, path_offset, ptr(d_directions), ptr(d_output),
+         stream_handle(stream))
+
+
+def scramble_gpu(device_data, scramblers, number_dim, paths, stream=None):
+    call("scramble_gpu", ptr(device_data), ptr(scramblers), number_dim, paths,
+         stream_handle(stream))
+
+
+def inverse_normal_u32_gpu(x, out, stream=None):
+    call("inverse_normal_u32_gpu", ptr(x), ptr(out), x.numel(), stream_handle(stream))
+
+
+def sobol_scramble_normal_gpu(n_vectors, n_dimensions, path_offset, d_directions, scramblers,
+                              number_scrambled, out, stream=None):
+    call("sobol_scramble_normal_gpu", n_vectors, n_dimensions, path_offset, ptr(d_directions),
+         ptr(scramblers), number_scrambled, ptr(out), stream_handle(stream))
+
+
+def brownian_bridge_gpu(steps, allocation_indices, d_in, d_out, paths, factors, stream=None):
+    call("brownian_bridge_gpu", steps, ptr(allocation_indices), ptr(d_in), ptr(d_out), paths,
+         factors, stream_handle(stream))
+
+
+def sobol_bridge_gpu(paths, path_offset, steps, factors, d_directions, scramblers,
+                     number_scrambled, allocation_indices, out, stream=None):
+    call("sobol_bridge_gpu", paths, path_offset, steps, factors, ptr(d_directions),
+         ptr(scramblers), number_scrambled, ptr(allocation_indices), ptr(out),
+         stream_handle(stream))
+
+
+def correlated_drift_paths_GPU(d_in, d_out, A, alive, drift_fixed, A_offset_per_step, factors,
+                               out_dimensions, number_paths, steps, stream=None):
+    call("correlated_drift_paths_GPU", ptr(d_in), ptr(d_out), ptr(A), ptr(alive),
+         ptr(drift_fixed), A_offset_per_step, factors, out_dimensions, number_paths, steps,
+         stream_handle(stream))
+
+
+def LMM_evolver_pc_all_gpu(initial_rates, initial_log_rates, taus, variates, pseudo_roots,
+                           initial_drifts, fixed_drifts_dev, displacements, alive, paths,
+                           factors, steps, rates, evolved_rates, evolved_log_rates, discounts,
+                           transposeData=True, doDiscounts=True, stream=None):
+    call("LMM_evolver_pc_all_gpu", ptr(initial_rates), ptr(initial_log_rates), ptr(taus),
+         ptr(variates), ptr(pseudo_roots), ptr(initial_drifts), ptr(fixed_drifts_dev),
+         ptr(displacements), ptr(alive), paths, factors, steps, rates, None, None,
+         ptr(evolved_rates), ptr(evolved_log_rates), ptr(discounts), 0, int(transposeData),
+         stream_handle(stream), 0, int(doDiscounts))
+
+
+def _host_ints(v):
+    return np.ascontiguousarray(v, dtype=np.int32)
+
+
+def discount_ratios_computation_gpu(rates_dev, taus, alive_host, paths, steps, n_rates, disc,
+                                    stream=None):
+    a = _host_ints(alive_host)
+    call("discount_ratios_computation_gpu", ptr(rates_dev), ptr(taus), a.ctypes.data, None, paths,
+         steps, n_rates, ptr(disc), 1, stream_handle(stream))
+
+
+def coterminal_annuity_ratios_computation_gpu(disc, taus, alive_host, paths, steps, n_rates, ann,
+                                              stream=None):
+    a = _host_ints(alive_host)
+    call("coterminal_annuity_ratios_computation_gpu", ptr(disc), ptr(taus), a.ctypes.data, paths,
+         steps, n_rates, ptr(ann), stream_handle(stream))
+
+
+def coterminal_swap_rate_computation_gpu(disc, ann, alive_host, paths, steps, n_rates, sr,
+                                         stream=None):
+    a = _host_ints(alive_host)
+    call("coterminal_swap_rate_computation_gpu", ptr(disc), ptr(ann), a.ctypes.data, paths, steps,
+         n_rates, ptr(sr), stream_handle(stream))
+
+
+def swap_rate_computation_gpu(disc, taus, start, end, paths, step_in, step_out, n_rates, out,
+                              stream=None):
+    call("swap_rate_computation_gpu", ptr(disc), ptr(taus), start, end, paths, step_in, step_out,
+         n_rates, ptr(out), stream_handle(stream))
+
+
+def spot_measure_numeraires_computation_gpu(disc, num, paths, n_rates, stream=None):
+    call("spot_measure_numeraires_computation_gpu", ptr(disc), ptr(num), paths, n_rates,
+         stream_handle(stream))
+
+
+def spot_measure_numeraires_computation_offset_gpu(disc, num, paths, out_paths, offset, n_rates,
+                                                   stream=None):
+    call("spot_measure_numeraires_computation_offset_gpu", ptr(disc), ptr(num), paths, out_paths,
+         offset, n_rates, stream_handle(stream))
+
+
+def forward_rate_extraction_gpu(fwd, indices_host, paths, steps, n_rates, out, stream=None):
+    ix = _host_ints(indices_host)
+    call("forward_rate_extraction_gpu", ptr(fwd), ix.ctypes.data, paths, steps, n_rates, ptr(out),
+         stream_handle(stream))
+
+
+def forward_rate_extraction_selecting_step_gpu(fwd, indices_host, steps_host, paths, n_rates, out,
+                                               stream=None):
+    ix, st = _host_ints(indices_host), _host_ints(steps_host)
+    call("forward_rate_extraction_selecting_step_gpu", ptr(fwd), ix.ctypes.data, st.ctypes.data,
+         ix.size, paths, n_rates, ptr(out), stream_handle(stream))
+
+
+def spot_measure_numeraires_extraction_gpu(all_num, some_num, paths_in, paths_out, offset,
+                                           n_rates, n_ex, ex_indices_dev, stream=None):
+    call("spot_measure_numeraires_extraction_gpu", ptr(all_num), ptr(some_num), paths_in,
+         paths_out, offset, n_rates, n_ex, ptr(ex_indices_dev), stream_handle(stream))
+
+
+def cash_flow_generator(product, f1, f2, aux, paths, steps, r1, r2=None, r3=None, stream=None):
+    name = {"swap": "cashFlowGeneratorCallerSwap", "tarn": "cashFlowGeneratorCallerTARN",
+            "caplet": "cashFlowGeneratorCallerCaplet"}[product]
+    call(name, ptr(f1), ptr(f2), ptr(aux), paths, steps, ptr(r1), ptr(r2), ptr(r3), None, None,
+         stream_handle(stream))
+
+
+def cashFlowDiscounting_gpu(first, second, theta, disc, flows, num, paths, steps, out, summed,
+                            stream=None):
+    call("cashFlowDiscounting_gpu", ptr(first), ptr(second), ptr(theta), ptr(disc), ptr(flows),
+         ptr(num), paths, steps, 0, 0, ptr(out), ptr(summed), stream_handle(stream))
+
+
+def cashFlowDiscounting_partial_gpu(n_flows, gen, first, second, theta, target, disc, flows, num,
+                                    paths, steps, out, stream=None):
+    call("cashFlowDiscounting_partial_gpu", n_flows, ptr(gen), ptr(first), ptr(second), ptr(theta),
+         ptr(target), ptr(disc), ptr(flows), ptr(num), paths, steps, ptr(out),
+         stream_handle(stream))
+
+
+def AggregateFlows_gpu(agg, total_paths, n_ex, flows, batch_paths, offset, steps, preceding,
+                       start=0, stream=None):
+    call("AggregateFlows_gpu", ptr(agg), total_paths, n_ex, ptr(flows), batch_paths, offset, steps,
+         ptr(preceding), start, stream_handle(stream))
+
+
+def partial_reduce(data, workspace, points, stop_point=1, width=8, stream=None):
+    """Returns (device address of the remaining values, their count)."""
+    res = ctypes.c_void_p()
+    rem = ctypes.c_int()
+    name = "PartialReduce_using8s_gpu" if width == 8 else "PartialReduce_using16s_gpu"
+    call(name, ptr(data), ptr(workspace), points, 0, stop_point, ctypes.addressof(res),
+         ctypes.addressof(rem), stream_handle(stream))
+    return res.value, rem.value
+
+
+def reduceSinglePass(size, d_in, wsp, final, stream=None):
+    call("reduceSinglePass", size, 0, 0, ptr(d_in), ptr(wsp), ptr(final), stream_handle(stream))
+
+
+def sum_workspace_doubles(n):
+    return int(lib().koo_sum_workspace_doubles(n))
+
+
+def sum_f32_to_f64(x, workspace, out, stream=None):
+    call("koo_sum_f32_to_f64", ptr(x), x.numel(), ptr(workspace), ptr(out), stream_handle(stream))
+
+
+def sum_f64(x, workspace, out, stream=None):
+    call("koo_sum_f64", ptr(x), x.numel(), ptr(workspace), ptr(out), stream_handle(stream))
+
+
+def lmm_price_workspace(paths, rates, factors, checkpoint_every):
+    a, b, c = _sz(), _sz(), _sz()
+    call("lmm_price_workspace", paths, rates, factors, checkpoint_every, ctypes.addressof(a),
+         ctypes.addressof(b), ctypes.addressof(c))
+    return a.value, b.value, c.value
+
+
+KOO_PRODUCT_CAPLETS = 1
+
+
+def lmm_price_fwd(paths, rates, factors, k, variates, f0, taus, disp, A, product, aux_host,
+                  checkpoints, partials, price_out, stream=None):
+    aux = np.ascontiguousarray(aux_host, np.float32)
+    call("lmm_price_fwd", paths, rates, factors, k, ptr(variates), ptr(f0), ptr(taus), ptr(disp),
+         ptr(A), product, aux.ctypes.data, ptr(checkpoints), ptr(partials), ptr(price_out),
+         stream_handle(stream))
+
+
+def lmm_price_bwd(paths, rates, factors, k, variates, f0, taus, disp, A, vols, product, aux_host,
+                  checkpoints, scratch, partials, greeks_out, stream=None):
+    aux = np.ascontiguousarray(aux_host, np.float32)
+    call("lmm_price_bwd", paths, rates, factors, k, ptr(variates), ptr(f0), ptr(taus), ptr(disp),
+         ptr(A), ptr(vols), product, aux.ctypes.data, ptr(checkpoints), ptr(scratch),
+         ptr(partials), ptr(greeks_out), stream_handle(stream))

@@ -1,0 +1,105 @@
+"""Host orchestration of the LMM pricing path on one GPU.
+
+``LMMPricer`` owns the device copies of a workload and runs
+  * ``generate``: fused Sobol -> scramble -> normal -> bridge (one kernel),
+  * ``price``: the fused forward price kernel (no rate cube written),
+  * ``price_and_greeks``: forward with checkpoints plus the reverse sweep.
+``lmm_evolution_routine`` mirrors the reference's staged pipeline
+(LMMEvolutionRoutineRawSingleKernel, LMM_evolver_full.cu:264-425) through the
+reference-named entry points.  Buffers are allocated once per pricer; calls
+only launch kernels on the current stream.
+"""
+import numpy as np
+import torch
+
+from . import koo
+
+
+def _dev(x, dtype):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype).cuda()
+
+
+class LMMPricer:
+    def __init__(self, w, device="cuda"):
+        self.w = w
+        P, N, F, S = w.paths, w.n_rates, w.factors, w.steps
+        self.dirs = _dev(koo.sobol_directions(w.dims).view(np.int32), torch.int32)
+        self.scr = _dev(np.asarray(w.scramblers, np.uint32).view(np.int32), torch.int32)
+        self.alloc = _dev(koo.bridge_allocate_diagonal(F, S), torch.int32)
+        self.f0 = _dev(w.f0, torch.float32)
+        self.taus = _dev(w.taus, torch.float32)
+        self.disp = _dev(w.disp(), torch.float32)
+        self.A = _dev(w.A, torch.float32)
+        self.vols = _dev(w.vols, torch.float32)
+        self.z = torch.empty(S * F * P, dtype=torch.float32, device=device)
+        ck, sc, pa = koo.lmm_price_workspace(P, N, F, w.checkpoint_every)
+        self.ckpt = torch.empty(ck // 4, dtype=torch.float32, device=device)
+        self.scratch = torch.empty(sc // 4, dtype=torch.float32, device=device)
+        self.partials = torch.empty(pa // 8, dtype=torch.float64, device=device)
+        self.price_out = torch.zeros(1, dtype=torch.float64, device=device)
+        self.greeks_out = torch.zeros(2 * N, dtype=torch.float64, device=device)
+        self.aux = np.array([w.strike], np.float32)
+
+    def generate(self):
+        w = self.w
+        koo.sobol_bridge_gpu(w.paths, w.path_offset, w.steps, w.factors, self.dirs, self.scr,
+                             w.dims, self.alloc, self.z)
+
+    def price_kernel(self, checkpoints=False):
+        w = self.w
+        koo.lmm_price_fwd(w.paths, w.n_rates, w.factors, w.checkpoint_every, self.z, self.f0,
+                          self.taus, self.disp, self.A, koo.KOO_PRODUCT_CAPLETS, self.aux,
+                          self.ckpt if checkpoints else None, self.partials, self.price_out)
+
+    def greeks_kernel(self):
+        w = self.w
+        koo.lmm_price_bwd(w.paths, w.n_rates, w.factors, w.checkpoint_every, self.z, self.f0,
+                          self.taus, self.disp, self.A, self.vols, koo.KOO_PRODUCT_CAPLETS,
+                          self.aux, self.ckpt, self.scratch, self.partials, self.greeks_out)
+
+    def price(self):
+        self.generate()
+        self.price_kernel(checkpoints=False)
+        return self.price_out
+
+    def price_and_greeks(self):
+        self.generate()
+        self.price_kernel(checkpoints=True)
+        self.greeks_kernel()
+        return self.price_out, self.greeks_out
+
+
+def lmm_evolution_routine(w, use_fused_generator=False):
+    """Staged pipeline of LMMEvolutionRoutineRawSingleKernel: Sobol integers
+    for D = S*F dimensions, scramble of the first S rows (as the reference
+    driver does, LMM_evolver_full.cu:324-327), inverse normal, diagonal-
+    allocation bridge, then the all-steps PC evolver with discount ratios.
+    Returns (rates, log_rates, discounts) as device tensors (S,N,P),
+    (S,N,P), (S,N+1,P)."""
+    P, N, F, S = w.paths, w.n_rates, w.factors, w.steps
+    D = S * F
+    dirs = _dev(koo.sobol_directions(D).view(np.int32), torch.int32)
+    scr = _dev(np.asarray(w.scramblers, np.uint32).view(np.int32), torch.int32)
+    normals = torch.empty(D * P, dtype=torch.float32, device="cuda")
+    if use_fused_generator:
+        koo.sobol_scramble_normal_gpu(P, D, w.path_offset, dirs, scr, S, normals)
+    else:
+        ints = torch.empty(D * P, dtype=torch.int32, device="cuda")
+        koo.sobolGPUInts(P, D, w.path_offset, dirs, ints)
+        koo.scramble_gpu(ints, scr, S, P)
+        koo.inverse_normal_u32_gpu(ints, normals)
+    alloc = _dev(koo.bridge_allocate_diagonal(F, S), torch.int32)
+    z = torch.empty(D * P, dtype=torch.float32, device="cuda")
+    koo.brownian_bridge_gpu(S, alloc, normals, z, P, F)
+    A = w.A.astype(np.float32)
+    mu0 = koo.spot_drifts(w.A[0], w.f0, w.taus, w.disp(), 0)
+    fx = koo.fixed_drifts(w.A)
+    rates = torch.empty(S * N * P, dtype=torch.float32, device="cuda")
+    logs = torch.empty_like(rates)
+    disc = torch.empty(S * (N + 1) * P, dtype=torch.float32, device="cuda")
+    koo.LMM_evolver_pc_all_gpu(_dev(w.f0, torch.float32), _dev(np.log(w.f0 + w.disp()), torch.float32),
+                               _dev(w.taus, torch.float32), z, _dev(A, torch.float32),
+                               _dev(mu0, torch.float32), _dev(fx, torch.float32),
+                               _dev(w.disp(), torch.float32), _dev(w.alive(), torch.int32), P, F,
+                               S, N, rates, logs, disc)
+    return rates.view(S, N, P), logs.view(S, N, P), disc.view(S, N + 1, P)
