@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--cpu-sample-paths", type=int, default=1 << 17)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the benchmark, copy a buffer of this many MiB twice (device to "
+                         "device): a plain copy of known size that calibrates HBM byte counters")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -199,6 +202,16 @@ def main():
                       f"the same workload with all 80 adjoint Greeks on its first "
                       f"{wc.paths} paths: {dt:.2f} s",
         }
+
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = {"read": n * 4, "write": n * 4}
 
     print(json.dumps(out), flush=True)
 

@@ -34,8 +34,10 @@ def test_swap_tarn_caplet_flows(torch_cuda):
         f1 = torch.zeros(S * P, dtype=torch.float32, device="cuda")
         f2 = torch.zeros_like(f1)
         koo.cash_flow_generator(product, f1, f2, f32(torch, aux), P, S, r1, r1, r1)
-        assert np.allclose(f1.cpu().numpy().reshape(S, P), w1, rtol=1e-6, atol=1e-8), product
-        assert np.allclose(f2.cpu().numpy().reshape(S, P), w2, rtol=1e-6, atol=1e-8), product
+        # FP32 against the FP64 gold: the TARN's final flow paid - total cancels
+        # (running FP32 coupon sum of up to S terms of ~0.01 against a cap of 0.1).
+        assert np.allclose(f1.cpu().numpy().reshape(S, P), w1, rtol=1e-6, atol=5e-8), product
+        assert np.allclose(f2.cpu().numpy().reshape(S, P), w2, rtol=1e-6, atol=5e-8), product
     aux = np.concatenate([[0.047], np.full(S, 0.5)])
     f1 = torch.zeros(S * P, dtype=torch.float32, device="cuda")
     f2 = torch.ones_like(f1)
