@@ -314,7 +314,8 @@ int koo_sum_f64(const double* in, long long n, double* workspace, double* out, v
  * checkpoints (recomputing the steps in between into segment_scratch), giving
  * greeks_out[0..N) = dV/df_i(0) and greeks_out[N..2N) = dV/dsigma_i with
  * A(s,i,f) proportional to sigma_i.  Sizes of all device buffers come from
- * lmm_price_workspace().  product_aux is a HOST array (read at call time);
+ * lmm_price_workspace().  rates <= 90, factors <= 8.  product_aux is a HOST
+ * array (read at call time);
  * every other pointer is a device pointer.  variates are (S,F,P); price_out
  * and greeks_out receive means over paths (FP64).
  * =================================================================== */

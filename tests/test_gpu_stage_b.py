@@ -43,9 +43,10 @@ def test_bridge_vs_oracle(torch_cuda, S, F, P):
     assert np.max(np.abs(got - want)) < 2e-5
 
 
-def test_fused_sobol_bridge_equals_staged(torch_cuda):
+@pytest.mark.parametrize("S,F,P,off", [(40, 5, 30000, 11), (64, 2, 5000, 1000), (20, 3, 777, 63),
+                                       (48, 1, 4096, 0), (100, 1, 300, 5)])
+def test_fused_sobol_bridge_equals_staged(torch_cuda, S, F, P, off):
     torch = torch_cuda
-    S, F, P, off = 40, 5, 30000, 11
     D = S * F
     dirs = dev(torch, koo.sobol_directions(D))
     scr = dev(torch, np.random.RandomState(3).randint(0, 2**32, D, dtype=np.uint64).astype(np.uint32))

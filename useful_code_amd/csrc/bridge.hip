@@ -113,10 +113,10 @@ __global__ void __launch_bounds__(kBridgeThreads)
               [&](int i) { return in[static_cast<size_t>(idx[i]) * paths + p]; }, out);
 }
 
-// Fused generator: the bridge variate i of factor f is the normal quantile of
-// Sobol dimension d = alloc(f, i) at Gray index p + off1, scrambled when
-// d < number_scrambled.  Direction numbers of the block's S dimensions sit in
-// LDS behind the path storage.
+// Fused generator, general steps: the bridge variate i of factor f is the
+// normal quantile of Sobol dimension d = alloc(f, i) at Gray index p + off1,
+// scrambled when d < number_scrambled.  Direction numbers of the block's S
+// dimensions sit in LDS behind the path storage.
 __global__ void __launch_bounds__(kBridgeThreads)
     k_sobol_bridge(const BridgeTables T, int steps, int factors, int paths, uint32_t off1,
                    const uint32_t* __restrict__ dirs, const uint32_t* __restrict__ scr,
@@ -136,6 +136,75 @@ __global__ void __launch_bounds__(kBridgeThreads)
                 const uint32_t s = d < number_scrambled ? scr[d] : 0u;
                 return invnorm_u32(sobol_point(v + 32 * i, n) ^ s);
               },
+              out);
+}
+
+// Fused generator, steps <= kTableSteps: no per-point bit loop.  With
+// n = G + l, G a multiple of 64 and l < 64, the Gray code splits,
+// gray(n) = gray(G) ^ gray(l), so the Sobol integer is x(n) = X(G) ^ T(l).
+// A block's lanes hold consecutive n, which span at most blockDim/64 + 1
+// multiples of 64; per block and dimension the set-up builds X(G) for those
+// (scrambler folded in; each one a 32-lane XOR reduction over the direction
+// numbers) and the 64-entry table T(l) from the first six direction numbers.
+// A point then costs two LDS reads and one XOR.
+constexpr int kTableSteps = 64;
+constexpr int kMaxGroups = 8;
+
+__global__ void __launch_bounds__(kBridgeThreads)
+    k_sobol_bridge_tab(const BridgeTables T, int steps, int factors, int paths, uint32_t off1,
+                       const uint32_t* __restrict__ dirs, const uint32_t* __restrict__ scr,
+                       int number_scrambled, const int* __restrict__ alloc,
+                       float* __restrict__ out) {
+  extern __shared__ float lds[];
+  const int bd = blockDim.x;
+  uint32_t* tlo = reinterpret_cast<uint32_t*>(lds + steps * bd);  // [steps][64]
+  uint32_t* xhi = tlo + steps * 64;                                // [steps][kMaxGroups]
+  uint32_t* vlo = xhi + steps * kMaxGroups;                        // [steps][6]
+  const int* idx = alloc + blockIdx.y * steps;
+  const uint32_t nb = static_cast<uint32_t>(blockIdx.x) * bd + off1;  // n of lane 0
+  const uint32_t g0 = nb & ~63u;
+  const int ngroups = bd / 64 + 1;
+
+  for (int w = threadIdx.x; w < steps * 6; w += bd)
+    vlo[w] = dirs[static_cast<size_t>(idx[w / 6]) * 32 + (w % 6)];
+  // X(G) for G = g0 + 64k: lane b of a half-wavefront contributes v[b] when
+  // bit b of gray(G) is set; the half XOR-reduces.  Every half runs the same
+  // number of rounds so the shuffles see all their lanes.
+  {
+    const int half = threadIdx.x >> 5, b = threadIdx.x & 31;
+    const int halves = bd >> 5, pairs = steps * ngroups;
+    for (int base = 0; base < pairs; base += halves) {
+      const int q = base + half;
+      const int qc = q < pairs ? q : pairs - 1;
+      const int i = qc / ngroups, k = qc % ngroups;
+      const uint32_t G = g0 + 64u * static_cast<uint32_t>(k);
+      const uint32_t g = G ^ (G >> 1);
+      uint32_t x = ((g >> b) & 1u) ? dirs[static_cast<size_t>(idx[i]) * 32 + b] : 0u;
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1) x ^= static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 32));
+      if (b == 0 && q < pairs) {
+        const int d = idx[i];
+        xhi[i * kMaxGroups + k] = x ^ (d < number_scrambled ? scr[d] : 0u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < steps * 64; w += bd) {
+    const uint32_t l = static_cast<uint32_t>(w & 63), g = l ^ (l >> 1);
+    const uint32_t* vv = vlo + (w >> 6) * 6;
+    uint32_t x = 0;
+#pragma unroll
+    for (int bit = 0; bit < 6; ++bit)
+      if ((g >> bit) & 1u) x ^= vv[bit];
+    tlo[w] = x;
+  }
+  __syncthreads();
+  const int p = blockIdx.x * bd + threadIdx.x;
+  if (p >= paths) return;
+  const uint32_t n = nb + static_cast<uint32_t>(threadIdx.x);
+  const uint32_t kg = (n - g0) >> 6, l = n & 63u;
+  bridge_lane(T, steps, factors, paths, p, lds,
+              [&](int i) { return invnorm_u32(xhi[i * kMaxGroups + kg] ^ tlo[i * 64 + l]); },
               out);
 }
 
@@ -202,6 +271,16 @@ extern "C" int sobol_bridge_gpu(int paths, unsigned path_offset, int steps, int 
   if (paths == 0) return KOO_OK;
   if (!d_directions || !allocation_indices || !out || (number_scrambled > 0 && !scramblers))
     return set_error(KOO_ERR_INVALID, "sobol_bridge_gpu: null pointer");
+  if (steps <= kTableSteps) {
+    const int threads = steps <= 48 ? 256 : 128;
+    const size_t lds = sizeof(float) * steps * threads +
+                       sizeof(uint32_t) * steps * (64 + kMaxGroups + 6);
+    hipLaunchKernelGGL(k_sobol_bridge_tab, dim3((paths + threads - 1) / threads, factors),
+                       dim3(threads), lds, as_stream(stream), tables_for(steps), steps, factors,
+                       paths, path_offset + 1u, d_directions, scramblers, number_scrambled,
+                       allocation_indices, out);
+    return check_launch("sobol_bridge_gpu");
+  }
   const int threads = bridge_threads(steps);
   const size_t lds = sizeof(float) * steps * threads + sizeof(uint32_t) * 32 * steps;
   hipLaunchKernelGGL(k_sobol_bridge, dim3((paths + threads - 1) / threads, factors), dim3(threads),

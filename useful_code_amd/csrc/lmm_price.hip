@@ -26,7 +26,7 @@ namespace koo {
 
 namespace {
 
-constexpr int kMaxRates = 128;
+constexpr int kMaxRates = 90;
 constexpr int kMaxFactors = 8;
 
 __device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
@@ -99,32 +99,48 @@ struct Model {
   int P, N, k;
 };
 
+// Fixed drifts c(s,r) = -0.5 sum_f A(s,r,f)^2 into LDS (read back with
+// uniform addresses); every lane of the block takes part, then a barrier.
+template <int F>
+__device__ __forceinline__ void fill_fixed_drifts(const Model& m, float* ctab) {
+  const int nn = m.N * m.N;
+  for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+    const float* a = m.A + static_cast<size_t>(i) * F;
+    float c = 0.0f;
+#pragma unroll
+    for (int f = 0; f < F; ++f) c -= 0.5f * a[f] * a[f];
+    ctab[i] = c;
+  }
+}
+
 // One predictor-corrector step s over rates r >= s.  `load(r)` gives the
-// start log-rate, `store(r, l)` receives the end log-rate; e / ep return the
-// final drift sums.
+// start log-rate (issued one rate ahead), `store(r, l)` receives the end
+// log-rate; e / ep return the final drift sums; cs = fixed drifts of step s.
 template <int F, class Load, class Store>
-__device__ __forceinline__ void pc_step(const Model& m, int s, const float (&z)[F], float (&e)[F],
-                                        float (&ep)[F], Load load, Store store) {
+__device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
+                                        const float (&z)[F], float (&e)[F], float (&ep)[F],
+                                        Load load, Store store) {
   const float* As = m.A + static_cast<size_t>(s) * m.N * F;
 #pragma unroll
   for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
+  float l_next = load(s);
   for (int r = s; r < m.N; ++r) {
+    const float l = l_next;
+    if (r + 1 < m.N) l_next = load(r + 1);
     float a[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
     const float tau = m.taus[r], d = m.disp[r];
-    const float l = load(r);
     const float x = fast_exp(l);
     const float g = tau * x * fast_rcp(1.0f + tau * (x - d));
-    float mu = 0.0f, dw = 0.0f, c = 0.0f;
+    float mu = 0.0f, dw = 0.0f;
 #pragma unroll
     for (int f = 0; f < F; ++f) {
       e[f] += g * a[f];
       mu += a[f] * e[f];
       dw += a[f] * z[f];
-      c -= 0.5f * a[f] * a[f];
     }
-    const float lh = l + mu + c + dw;
+    const float lh = l + mu + cs[r] + dw;
     const float xh = fast_exp(lh);
     const float gh = tau * xh * fast_rcp(1.0f + tau * (xh - d));
     float muh = 0.0f;
@@ -147,7 +163,8 @@ __device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[
 // ----------------------------------------------------------------------
 // Forward: price per path, block partial sums, optional checkpoints
 // ckpt[seg][r][P] (r < N: log-rate at the start of step seg*k; row N: the
-// running discount 1/B at that step).
+// running discount 1/B at that step).  LDS: state [N][lanes], then the
+// fixed-drift table [N][N].
 // ----------------------------------------------------------------------
 template <int F>
 __global__ void __launch_bounds__(256)
@@ -159,8 +176,11 @@ __global__ void __launch_bounds__(256)
   const bool valid = p < m.P;
   const int pc = valid ? p : m.P - 1;
   float* L = lds + threadIdx.x;
+  float* ctab = lds + static_cast<size_t>(m.N) * bd;
   const size_t P = static_cast<size_t>(m.P);
+  fill_fixed_drifts<F>(m, ctab);
   for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
+  __syncthreads();
   float invB = 1.0f, y = 0.0f, yc = 0.0f;
   for (int s = 0; s < m.N; ++s) {
     if (ckpt && valid && (s % m.k) == 0) {
@@ -170,7 +190,7 @@ __global__ void __launch_bounds__(256)
     }
     float z[F], e[F], ep[F];
     load_z<F>(m, s, pc, z);
-    pc_step<F>(m, s, z, e, ep, [&](int r) { return L[r * bd]; },
+    pc_step<F>(m, s, ctab + s * m.N, z, e, ep, [&](int r) { return L[r * bd]; },
                [&](int r, float v) { L[r * bd] = v; });
     const float tau = m.taus[s];
     const float fs = fast_exp(L[s * bd]) - m.disp[s];
@@ -189,7 +209,12 @@ __global__ void __launch_bounds__(256)
 // Backward.  scratch slot j (j < k) holds rows [0,N): start log-rates of step
 // s0+j, row N: 1/B at that step, rows N+1..N+2F: final sums e, e^ of step
 // s0+j.  Slot 0's state rows are not used (the checkpoint is read instead).
+// LDS: adjoints lam [N][lanes], fixed drifts [N][N], vega partial sums
+// [wave][N][8] (each wavefront folds its 64 lanes to 8 with DPP and adds
+// those into its own slots).
 // ----------------------------------------------------------------------
+constexpr int kVegaSlots = 8;
+
 template <int F>
 __global__ void __launch_bounds__(256)
     k_price_bwd(Model m, const float* __restrict__ vols, const float* __restrict__ ckpt,
@@ -198,15 +223,19 @@ __global__ void __launch_bounds__(256)
   __shared__ double red[4];
   const int bd = blockDim.x;
   const int nw = bd >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int p = blockIdx.x * bd + threadIdx.x;
   const bool valid = p < m.P;
   const int pc = valid ? p : m.P - 1;
   const int N = m.N;
   const size_t P = static_cast<size_t>(m.P);
   float* lam = lds + threadIdx.x;                            // lam[r * bd]
-  float* wv = lds + static_cast<size_t>(N) * bd + (threadIdx.x >> 6) * N;  // per-wave vega sums
+  float* ctab = lds + static_cast<size_t>(N) * bd;           // [N][N]
+  float* vslots = ctab + static_cast<size_t>(N) * N;         // [nw][N][8]
+  float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 3);
+  fill_fixed_drifts<F>(m, ctab);
   for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
-  for (int i = threadIdx.x; i < nw * N; i += bd) lds[static_cast<size_t>(N) * bd + i] = 0.0f;
+  for (int i = threadIdx.x; i < nw * N * kVegaSlots; i += bd) vslots[i] = 0.0f;
   __syncthreads();
 
   const size_t slot = static_cast<size_t>(N + 1 + 2 * F) * P;
@@ -227,7 +256,8 @@ __global__ void __launch_bounds__(256)
       float z[F], e[F], ep[F];
       float l_fix = 0.0f;
       load_z<F>(m, s, pc, z);
-      pc_step<F>(m, s, z, e, ep, [&](int r) { return src[static_cast<size_t>(r) * P]; },
+      pc_step<F>(m, s, ctab + s * N, z, e, ep,
+                 [&](int r) { return src[static_cast<size_t>(r) * P]; },
                  [&](int r, float v) {
                    if (r == s) l_fix = v;
                    else if (keep) dst[static_cast<size_t>(r) * P] = v;
@@ -250,6 +280,7 @@ __global__ void __launch_bounds__(256)
       const int j = s - s0;
       const float* src = (j == 0) ? ck : scratch + j * slot + pc;
       const float* sums = scratch + j * slot + static_cast<size_t>(N + 1) * P + pc;
+      const float* cs = ctab + s * N;
       const float invBs = src[static_cast<size_t>(N) * P];
       float z[F], e[F], ep[F], E[F], Eh[F];
       load_z<F>(m, s, pc, z);
@@ -260,30 +291,35 @@ __global__ void __launch_bounds__(256)
         E[f] = Eh[f] = 0.0f;
       }
       const float* As = m.A + static_cast<size_t>(s) * N * F;
-      float* wrow = wv;
+      float l_next = src[static_cast<size_t>(N - 1) * P];
+      float lp_next = lam[(N - 1) * bd];
       for (int r = N - 1; r >= s; --r) {
+        const float l = l_next;
+        float lp = lp_next;
+        if (r > s) {
+          l_next = src[static_cast<size_t>(r - 1) * P];
+          lp_next = lam[(r - 1) * bd];
+        }
         float a[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
         const float tau = m.taus[r], d = m.disp[r];
         const float omd = 1.0f - tau * d;
-        const float l = src[static_cast<size_t>(r) * P];
         const float x = fast_exp(l);
         const float inv1 = fast_rcp(1.0f + tau * (x - d));
         const float g = tau * x * inv1;
-        float mu = 0.0f, muh = 0.0f, dw = 0.0f, c = 0.0f;
+        float mu = 0.0f, muh = 0.0f, dw = 0.0f;
 #pragma unroll
         for (int f = 0; f < F; ++f) {
           mu += a[f] * e[f];
           muh += a[f] * ep[f];
           dw += a[f] * z[f];
-          c -= 0.5f * a[f] * a[f];
         }
+        const float c = cs[r];
         const float lh = l + mu + c + dw;
         const float xh = fast_exp(lh);
         const float inv2 = fast_rcp(1.0f + tau * (xh - d));
         const float gh = tau * xh * inv2;
-        float lp = lam[r * bd];
         if (r == s) {
           // The rate fixing at the end of this step: caplet value and the
           // numeraire it enters for later fixings.
@@ -310,11 +346,14 @@ __global__ void __launch_bounds__(256)
           gb += a[f] * E[f];
         }
         lam[r * bd] = lhat + gb * g * omd * inv1;
-        // sum_f Abar_f a_f for this (s, r).
+        // sum_f Abar_f a_f for this (s, r): the mu^, mu, dW and fixed-drift
+        // terms contract to scalars the step already has.
         float w = 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c);
         w = valid ? w : 0.0f;
-        const float ws = wave_sum_f32(w);
-        if ((threadIdx.x & 63) == 0) wrow[r] += ws;
+        w += dpp_f32_quad_xor1(w);
+        w += dpp_f32_quad_xor2(w);
+        w += dpp_f32_row_half_mirror(w);
+        if ((lane & 7) == 0) atomicAdd(myslots + r * kVegaSlots, w);
 #pragma unroll
         for (int f = 0; f < F; ++f) {
           e[f] -= g * a[f];
@@ -324,7 +363,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   __syncthreads();
-  // Deltas: d/df0_r = lam_r / (f0_r + d_r); vegas: wave sums / sigma_r.
+  // Deltas: d/df0_r = lam_r / (f0_r + d_r); vegas: slot sums / sigma_r.
   for (int r = 0; r < N; ++r) {
     const float dl = lam[r * bd] / (m.f0[r] + m.disp[r]);
     block_sum_to(valid ? static_cast<double>(dl) : 0.0, red,
@@ -332,7 +371,9 @@ __global__ void __launch_bounds__(256)
   }
   for (int r = threadIdx.x; r < N; r += bd) {
     double t = 0.0;
-    for (int w = 0; w < nw; ++w) t += lds[static_cast<size_t>(N) * bd + w * N + r];
+    for (int w = 0; w < nw; ++w)
+      for (int q = 0; q < kVegaSlots; ++q)
+        t += vslots[(static_cast<size_t>(w) * N + r) * kVegaSlots + q];
     part[static_cast<size_t>(blockIdx.x) * 2 * N + N + r] = t / vols[r];
   }
 }
@@ -366,9 +407,14 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Lanes per block keeping the [rate][lane] LDS state within 60 KiB.
-int fwd_threads(int N) { return N <= 60 ? 256 : (N <= 120 ? 128 : 64); }
-int bwd_threads(int N) { return N <= 56 ? 256 : (N <= 112 ? 128 : 64); }
+// Lanes per block keeping the dynamic LDS within 60 KiB:
+// forward N*lanes + N*N floats, backward N*lanes + N*N + (lanes/64)*N*8.
+int fwd_threads(int N) { return N <= 50 ? 256 : (N <= 75 ? 128 : 64); }
+int bwd_threads(int N) { return N <= 45 ? 256 : (N <= 71 ? 128 : 64); }
+size_t fwd_lds(int N, int t) { return sizeof(float) * (static_cast<size_t>(N) * t + N * N); }
+size_t bwd_lds(int N, int t) {
+  return sizeof(float) * (static_cast<size_t>(N) * t + N * N + (t / 64) * N * kVegaSlots);
+}
 
 int check_model(const char* who, int paths, int rates, int factors, int k, int product) {
   if (paths < 1 || rates < 1 || rates > kMaxRates || factors < 1 || factors > kMaxFactors || k < 1)
@@ -421,7 +467,7 @@ extern "C" int lmm_price_fwd(int paths, int rates, int factors, int checkpoint_e
           checkpoint_every};
   const int threads = fwd_threads(rates);
   const int blocks = (paths + threads - 1) / threads;
-  const size_t lds = sizeof(float) * rates * threads;
+  const size_t lds = fwd_lds(rates, threads);
   hipStream_t st = as_stream(stream);
   switch (factors) {
 #define KOO_CASE(F)                                                                              \
@@ -454,7 +500,7 @@ extern "C" int lmm_price_bwd(int paths, int rates, int factors, int checkpoint_e
           checkpoint_every};
   const int threads = bwd_threads(rates);
   const int blocks = (paths + threads - 1) / threads;
-  const size_t lds = sizeof(float) * (static_cast<size_t>(rates) * threads + (threads / 64) * rates);
+  const size_t lds = bwd_lds(rates, threads);
   hipStream_t st = as_stream(stream);
   switch (factors) {
 #define KOO_CASE(F)                                                                              \
