@@ -120,7 +120,7 @@ template <int F, class Load, class Store>
 __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
                                         const float (&z)[F], float (&e)[F], float (&ep)[F],
                                         Load load, Store store) {
-  const float* As = m.A + static_cast<size_t>(s) * m.N * F;
+  const float* As = m.A + s * m.N * F;  // 32-bit: keeps the address scalar
 #pragma unroll
   for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
   float l_next = load(s);
@@ -166,9 +166,18 @@ __device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[
 // running discount 1/B at that step).  LDS: state [N][lanes], then the
 // fixed-drift table [N][N].
 // ----------------------------------------------------------------------
+// The model arrays are separate __restrict__ kernel arguments (not members of
+// a by-value struct) so the compiler may read them through the scalar cache.
+#define KOO_MODEL_PARAMS                                                                  \
+  const float* __restrict__ z_, const float* __restrict__ f0_,                            \
+      const float* __restrict__ taus_, const float* __restrict__ disp_,                   \
+      const float* __restrict__ A_, float strike_, int P_, int N_, int k_
+#define KOO_MODEL_LOCAL const Model m{z_, f0_, taus_, disp_, A_, strike_, P_, N_, k_}
+
 template <int F>
 __global__ void __launch_bounds__(256)
-    k_price_fwd(Model m, float* __restrict__ ckpt, double* __restrict__ part) {
+    k_price_fwd(KOO_MODEL_PARAMS, float* __restrict__ ckpt, double* __restrict__ part) {
+  KOO_MODEL_LOCAL;
   extern __shared__ float lds[];
   __shared__ double red[4];
   const int bd = blockDim.x;
@@ -217,8 +226,9 @@ constexpr int kVegaSlots = 8;
 
 template <int F>
 __global__ void __launch_bounds__(256)
-    k_price_bwd(Model m, const float* __restrict__ vols, const float* __restrict__ ckpt,
+    k_price_bwd(KOO_MODEL_PARAMS, const float* __restrict__ vols, const float* __restrict__ ckpt,
                 float* __restrict__ scratch, double* __restrict__ part) {
+  KOO_MODEL_LOCAL;
   extern __shared__ float lds[];
   __shared__ double red[4];
   const int bd = blockDim.x;
@@ -290,7 +300,7 @@ __global__ void __launch_bounds__(256)
         ep[f] = sums[static_cast<size_t>(F + f) * P];
         E[f] = Eh[f] = 0.0f;
       }
-      const float* As = m.A + static_cast<size_t>(s) * N * F;
+      const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
       float l_next = src[static_cast<size_t>(N - 1) * P];
       float lp_next = lam[(N - 1) * bd];
       for (int r = N - 1; r >= s; --r) {
@@ -472,8 +482,8 @@ extern "C" int lmm_price_fwd(int paths, int rates, int factors, int checkpoint_e
   switch (factors) {
 #define KOO_CASE(F)                                                                              \
   case F:                                                                                        \
-    hipLaunchKernelGGL(k_price_fwd<F>, dim3(blocks), dim3(threads), lds, st, m, checkpoints,    \
-                       partials);                                                               \
+    hipLaunchKernelGGL(k_price_fwd<F>, dim3(blocks), dim3(threads), lds, st, m.z, m.f0, m.taus, \
+                       m.disp, m.A, m.strike, m.P, m.N, m.k, checkpoints, partials);            \
     break;
     KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
     KOO_CASE(8)
@@ -505,8 +515,9 @@ extern "C" int lmm_price_bwd(int paths, int rates, int factors, int checkpoint_e
   switch (factors) {
 #define KOO_CASE(F)                                                                              \
   case F:                                                                                        \
-    hipLaunchKernelGGL(k_price_bwd<F>, dim3(blocks), dim3(threads), lds, st, m, vols,           \
-                       checkpoints, segment_scratch, partials);                                 \
+    hipLaunchKernelGGL(k_price_bwd<F>, dim3(blocks), dim3(threads), lds, st, m.z, m.f0, m.taus, \
+                       m.disp, m.A, m.strike, m.P, m.N, m.k, vols, checkpoints,                 \
+                       segment_scratch, partials);                                              \
     break;
     KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
     KOO_CASE(8)
