@@ -130,6 +130,34 @@ def main():
     fwd_ms = np.mean([r[1].elapsed_time(r[2]) for r in frecs])
     price = pr.price_out.item()
 
+    # Same job with a checkpoint at every step (the forward also keeps each
+    # step's drift sums, the reverse sweep recomputes nothing).
+    k1 = {}
+    if w.checkpoint_every != 1:
+        w1 = configs.caplet_portfolio(paths=args.paths, checkpoint_every=1)
+        pr1 = pipeline.LMMPricer(w1)
+        pr1.z = pr.z  # same variates
+        for _ in range(args.warmup):
+            pr1.price_kernel(checkpoints=True)
+            pr1.greeks_kernel()
+        r1 = [(ev(), ev(), ev()) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        for e0, e1, e2 in r1:
+            e0.record()
+            pr1.price_kernel(checkpoints=True)
+            e1.record()
+            pr1.greeks_kernel()
+            e2.record()
+        torch.cuda.synchronize()
+        k1_fwd = np.mean([a.elapsed_time(b) for a, b, _ in r1])
+        k1_bwd = np.mean([b.elapsed_time(c) for _, b, c in r1])
+        k1 = {"price_fwd_checkpoints_ms": k1_fwd, "price_bwd_ms": k1_bwd,
+              "job_ms": gen_ms + k1_fwd + k1_bwd,
+              "adjoint_ratio": (gen_ms + k1_fwd + k1_bwd) / fwd_pipe_ms,
+              "greeks_equal_k8": bool(torch.allclose(pr1.greeks_out, pr.greeks_out,
+                                                     rtol=2e-5, atol=0.0))}
+        del pr1
+
     f_fwd, f_bwd = flops_per_rate_update(F)
     upd = rate_updates_per_path(N) * P
     bwd_flops = (f_fwd + f_bwd) * upd  # segment recompute + reverse step
@@ -160,6 +188,7 @@ def main():
         "price": price,
         "kernels_ms": {"sobol_bridge": gen_ms, "price_fwd_checkpoints": fwdck_ms,
                        "price_bwd": bwd_ms, "price_fwd": fwd_ms},
+        "checkpoint_every_1": k1,
         "roofline": {
             "kernel": "k_price_bwd (reverse sweep + segment recompute)",
             "bound": "valu",

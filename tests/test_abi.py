@@ -66,3 +66,5 @@ def test_workspace_sizes():
     assert ck == 4 * 5 * 41 * 2**20
     assert sc == 4 * 8 * (41 + 10) * 2**20
     assert pa >= 8 * (2**20 // 256) * 80
+    ck1, sc1, _ = koo.lmm_price_workspace(2**20, 40, 5, 1)
+    assert ck1 == 4 * 40 * (41 + 10) * 2**20 and sc1 == 4

@@ -123,10 +123,13 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
   const float* As = m.A + s * m.N * F;  // 32-bit: keeps the address scalar
 #pragma unroll
   for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
+  // start log-rates are issued two rates ahead of their use
   float l_next = load(s);
+  float l_next2 = (s + 1 < m.N) ? load(s + 1) : 0.0f;
   for (int r = s; r < m.N; ++r) {
     const float l = l_next;
-    if (r + 1 < m.N) l_next = load(r + 1);
+    l_next = l_next2;
+    if (r + 2 < m.N) l_next2 = load(r + 2);
     float a[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
@@ -191,9 +194,13 @@ __global__ void __launch_bounds__(256)
   for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
   __syncthreads();
   float invB = 1.0f, y = 0.0f, yc = 0.0f;
+  // With k == 1 every step is a checkpoint and also keeps its final drift
+  // sums (rows N+1..N+2F), so the reverse sweep needs no recomputation.
+  const int rows = m.k == 1 ? m.N + 1 + 2 * F : m.N + 1;
   for (int s = 0; s < m.N; ++s) {
-    if (ckpt && valid && (s % m.k) == 0) {
-      float* c = ckpt + static_cast<size_t>(s / m.k) * (m.N + 1) * P + p;
+    const bool save = ckpt && valid && (s % m.k) == 0;
+    float* c = save ? ckpt + static_cast<size_t>(s / m.k) * rows * P + p : nullptr;
+    if (save) {
       for (int r = s; r < m.N; ++r) c[static_cast<size_t>(r) * P] = L[r * bd];
       c[static_cast<size_t>(m.N) * P] = invB;
     }
@@ -201,6 +208,13 @@ __global__ void __launch_bounds__(256)
     load_z<F>(m, s, pc, z);
     pc_step<F>(m, s, ctab + s * m.N, z, e, ep, [&](int r) { return L[r * bd]; },
                [&](int r, float v) { L[r * bd] = v; });
+    if (save && m.k == 1) {
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        c[static_cast<size_t>(m.N + 1 + f) * P] = e[f];
+        c[static_cast<size_t>(m.N + 1 + F + f) * P] = ep[f];
+      }
+    }
     const float tau = m.taus[s];
     const float fs = fast_exp(L[s * bd]) - m.disp[s];
     const float inv = fast_rcp(1.0f + tau * fs);
@@ -254,10 +268,11 @@ __global__ void __launch_bounds__(256)
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int s0 = seg * m.k;
     const int s1 = min(N, s0 + m.k);
-    const float* ck = ckpt + static_cast<size_t>(seg) * (N + 1) * P + pc;
+    const bool every = m.k == 1;  // checkpoint holds the sums: no recompute
+    const float* ck = ckpt + static_cast<size_t>(seg) * (every ? slot : (N + 1) * P) + pc;
     // Recompute the segment: state (and 1/B) at the start of step s0+j+1
     // into slot j+1, final drift sums of step s0+j into slot j.
-    for (int s = s0; s < s1; ++s) {
+    for (int s = s0; s < (every ? s0 : s1); ++s) {
       const int j = s - s0;
       const float* src = (j == 0) ? ck : scratch + j * slot + pc;
       float* dst = scratch + (j + 1) * slot + p;
@@ -289,7 +304,7 @@ __global__ void __launch_bounds__(256)
     for (int s = s1 - 1; s >= s0; --s) {
       const int j = s - s0;
       const float* src = (j == 0) ? ck : scratch + j * slot + pc;
-      const float* sums = scratch + j * slot + static_cast<size_t>(N + 1) * P + pc;
+      const float* sums = (every ? ck : scratch + j * slot + pc) + static_cast<size_t>(N + 1) * P;
       const float* cs = ctab + s * N;
       const float invBs = src[static_cast<size_t>(N) * P];
       float z[F], e[F], ep[F], E[F], Eh[F];
@@ -301,15 +316,16 @@ __global__ void __launch_bounds__(256)
         E[f] = Eh[f] = 0.0f;
       }
       const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
+      // start log-rates two rates ahead, adjoints one rate ahead
       float l_next = src[static_cast<size_t>(N - 1) * P];
+      float l_next2 = (N - 2 >= s) ? src[static_cast<size_t>(N - 2) * P] : 0.0f;
       float lp_next = lam[(N - 1) * bd];
       for (int r = N - 1; r >= s; --r) {
         const float l = l_next;
         float lp = lp_next;
-        if (r > s) {
-          l_next = src[static_cast<size_t>(r - 1) * P];
-          lp_next = lam[(r - 1) * bd];
-        }
+        l_next = l_next2;
+        if (r - 2 >= s) l_next2 = src[static_cast<size_t>(r - 2) * P];
+        if (r > s) lp_next = lam[(r - 1) * bd];
         float a[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
@@ -451,10 +467,13 @@ extern "C" int lmm_price_workspace(int paths, int rates, int factors, int checkp
   const size_t P = static_cast<size_t>(paths);
   const int k = checkpoint_every;
   const size_t nseg = (rates + k - 1) / k;
-  if (checkpoint_bytes) *checkpoint_bytes = sizeof(float) * nseg * (rates + 1) * P;
+  const size_t rows = k == 1 ? rates + 1 + 2 * factors : rates + 1;
+  if (checkpoint_bytes) *checkpoint_bytes = sizeof(float) * nseg * rows * P;
+  // k == 1 needs no recompute scratch; one float keeps the pointer non-null.
   if (scratch_bytes)
-    *scratch_bytes = sizeof(float) * static_cast<size_t>(k < rates ? k : rates) *
-                     (rates + 1 + 2 * factors) * P;
+    *scratch_bytes = k == 1 ? sizeof(float)
+                            : sizeof(float) * static_cast<size_t>(k < rates ? k : rates) *
+                                  (rates + 1 + 2 * factors) * P;
   const size_t bf = (P + fwd_threads(rates) - 1) / fwd_threads(rates);
   const size_t bb = (P + bwd_threads(rates) - 1) / bwd_threads(rates);
   const size_t need = bf > bb * 2 * rates ? bf : bb * 2 * rates;
