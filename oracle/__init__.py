@@ -109,6 +109,15 @@ def _declare(L):
                                             _f64p]),
         "orc_caplet_pipeline": (None, [ctypes.POINTER(CapletModel), _u32p, _c_int,
                                        ctypes.POINTER(_c_dbl), _f64p, _f64p]),
+        "orc_basis_count": (_c_int, [_c_int, _c_int]),
+        "orc_basis_functions": (None, [_f64p, _c_int, _c_int, _f64p]),
+        "orc_cholesky_solve": (_c_int, [_c_int, _f64p, _f64p, _f64p]),
+        "orc_ls_first_pass": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p, _c_int,
+                                       _c_int, _c_int, _f64p]),
+        "orc_ls_second_pass": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p,
+                                        _c_int, _c_int, _c_int, _f64p]),
+        "orc_bermudan_paths": (None, [ctypes.POINTER(CapletModel), _f64p, _c_int, _i32p, _f64p,
+                                      _f64p, _f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -143,6 +152,8 @@ def _declare_ref(R):
                                              _c_int, _f64p, _f64p]),
         "ref_flat_vol_pseudo_roots": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _c_dbl,
                                              _c_dbl, _f64p]),
+        "ref_ls_multi_depth0": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p, _c_int,
+                                         _c_int, _c_int, _f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(R, name)
@@ -309,3 +320,81 @@ def caplet_pipeline(model, ints, adjoint=False):
     lib().orc_caplet_pipeline(ctypes.byref(model), ints.reshape(-1), int(adjoint),
                               ctypes.byref(price), d, v)
     return price.value, d, v
+
+
+# ---------------------------------------------------------------- stage (d)
+
+def basis_count(n_vars, use_cross=True):
+    return lib().orc_basis_count(int(n_vars), int(use_cross))
+
+
+def basis_functions(x, use_cross=True):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(basis_count(len(x), use_cross))
+    lib().orc_basis_functions(x, len(x), int(use_cross), out)
+    return out
+
+
+def cholesky_solve(A, b):
+    A = np.ascontiguousarray(A, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(len(b))
+    st = lib().orc_cholesky_solve(len(b), A.reshape(-1), b, x)
+    return x if st == 0 else None
+
+
+def _ls_args(bv, vars_each_step, cash, ev, num):
+    bv = np.ascontiguousarray(bv, np.float64)
+    E, V, P = bv.shape
+    ves = np.ascontiguousarray(vars_each_step, np.int32)
+    arrs = [np.ascontiguousarray(a, np.float64).reshape(E, P) for a in (cash, ev, num)]
+    return bv, E, V, P, ves, arrs
+
+
+def ls_first_pass(bv, vars_each_step, cash, ev, num, use_cross=True, itm_only=False):
+    """Backward induction; returns (price, coefficients (E, maxData))."""
+    bv, E, V, P, ves, (c, e, n) = _ls_args(bv, vars_each_step, cash, ev, num)
+    coeffs = np.zeros((E, basis_count(V, use_cross)))
+    price = lib().orc_ls_first_pass(E, bv.reshape(-1), ves, V, c.reshape(-1), e.reshape(-1),
+                                    n.reshape(-1), P, int(use_cross), int(itm_only),
+                                    coeffs.reshape(-1))
+    return price, coeffs
+
+
+def ls_second_pass(bv, vars_each_step, cash, ev, num, coeffs, use_cross=True, itm_only=False):
+    bv, E, V, P, ves, (c, e, n) = _ls_args(bv, vars_each_step, cash, ev, num)
+    coeffs = np.ascontiguousarray(coeffs, np.float64)
+    return lib().orc_ls_second_pass(E, bv.reshape(-1), ves, V, c.reshape(-1), e.reshape(-1),
+                                    n.reshape(-1), P, int(use_cross), int(itm_only),
+                                    coeffs.reshape(-1))
+
+
+def ref_ls_first_pass(bv, vars_each_step, cash, ev, num, use_cross=True, normalise=True):
+    """The reference's gold LS (one regression per date).  The coefficients
+    returned with ``normalise`` apply to the gold's standardised variables."""
+    bv, E, V, P, ves, (c, e, n) = _ls_args(bv, vars_each_step, cash, ev, num)
+    coeffs = np.zeros((E, basis_count(V, use_cross)))
+    price = ref().ref_ls_multi_depth0(E, bv.reshape(-1), ves, V, c.reshape(-1), e.reshape(-1),
+                                      n.reshape(-1), P, int(use_cross), int(normalise),
+                                      coeffs.reshape(-1))
+    return price, coeffs
+
+
+def bermudan_paths(model, z, ex_steps):
+    """Per exercise date: basis variables (E,2,P) = (co-terminal swap rate,
+    f_{t_e}), exercise values (E,P) and spot numeraires (E,P)."""
+    z = np.ascontiguousarray(z, np.float64)
+    ex = np.ascontiguousarray(ex_steps, np.int32)
+    E, P = len(ex), model.paths
+    bv = np.zeros((E, 2, P))
+    ev = np.zeros((E, P))
+    num = np.zeros((E, P))
+    lib().orc_bermudan_paths(ctypes.byref(model), z.reshape(-1), E, ex, bv.reshape(-1),
+                             ev.reshape(-1), num.reshape(-1))
+    return bv, ev, num
+
+
+def normals_from_ints(ints, steps, factors):
+    """Inverse normal (FP64) then the diagonal Brownian bridge: (S,F,P)."""
+    ints = np.ascontiguousarray(ints, np.uint32)
+    return bridge_transform(invnorm_u32(ints), steps, factors)

@@ -971,3 +971,226 @@ void orc_caplet_pipeline(const orc_caplet_model *m, const uint32_t *ints, int do
   free(z);
   free(u);
 }
+
+/* ======================================================================
+ * Stage (d): Longstaff-Schwartz
+ * ==================================================================== */
+
+/* Number of quadratic basis functions (quadraticPolynomialCrossGenerator,
+ * gold/LS_regression.h:166-240, or the no-cross variant 1 + 2v). */
+int orc_basis_count(int v, int use_cross) {
+  if (!use_cross) return 1 + 2 * v;
+  if (v == 0) return 1;
+  if (v == 1) return 3;
+  if (v == 2) return 6;
+  return 1 + 3 * v;
+}
+
+/* Basis values: cross variant [1, x_i..., (v==1: x^2) (v==2: x^2, xy, y^2)
+ * (v>=3: x_i x_{i+1 mod v}..., x_i^2...)] (gold/LS_regression.h:198-240);
+ * no-cross variant [1, x_0, x_0^2, x_1, x_1^2, ...] (LS_gpu.h:92-126). */
+void orc_basis_functions(const double *x, int v, int use_cross, double *out) {
+  int j = 0;
+  out[j++] = 1.0;
+  if (!use_cross) {
+    for (int i = 0; i < v; ++i) {
+      out[j++] = x[i];
+      out[j++] = x[i] * x[i];
+    }
+    return;
+  }
+  for (int i = 0; i < v; ++i) out[j++] = x[i];
+  if (v == 1) {
+    out[j++] = x[0] * x[0];
+    return;
+  }
+  if (v == 2) {
+    out[j++] = x[0] * x[0];
+    out[j++] = x[0] * x[1];
+    out[j++] = x[1] * x[1];
+    return;
+  }
+  for (int i = 0; i < v; ++i) out[j++] = x[i] * x[(i + 1) % v];
+  for (int i = 0; i < v; ++i) out[j++] = x[i] * x[i];
+}
+
+/* Cholesky solve of a symmetric positive-definite n x n system; returns 0
+ * on success, -1 if a pivot is not positive. */
+int orc_cholesky_solve(int n, const double *A, const double *b, double *x) {
+  double L[64 * 64];
+  if (n > 64) return -1;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = A[i * n + j];
+      for (int k = 0; k < j; ++k) s -= L[i * 64 + k] * L[j * 64 + k];
+      if (i == j) {
+        if (!(s > 0.0)) return -1;
+        L[i * 64 + i] = sqrt(s);
+      } else {
+        L[i * 64 + j] = s / L[j * 64 + j];
+      }
+    }
+  double y[64];
+  for (int i = 0; i < n; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= L[i * 64 + k] * y[k];
+    y[i] = s / L[i * 64 + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < n; ++k) s -= L[k * 64 + i] * x[k];
+    x[i] = s / L[i * 64 + i];
+  }
+  return 0;
+}
+
+/* Backward induction with one regression per exercise date (depth 0 of
+ * generateRegressionCoefficientsViaLSMulti_gold, gold/MultiLS_regression.cpp:
+ * 23-430, without variable normalisation): at date e the regression matrix is
+ * the mean of phi phi^T and the target the mean of phi (cash_e + Y) over the
+ * selected paths (gold/LS_regression.h:431-515), solved here by Cholesky
+ * (the gold uses SVD with a 1e-8 cut-off; equal for full-rank systems); then
+ * Y = ex_e where ex_e exceeds the estimated continuation, else Y += cash_e
+ * (updateDeflatedNextStepValuesMulti_gold), and Y is deflated to date e-1 by
+ * B_{e-1}/B_e (updateDeflationOfContinuationValues_gold).  Price = mean of
+ * Y / B_0.  With itm_only the regression uses only paths with ex_e > 0 and
+ * exercise requires ex_e > 0 (the Bermudan configuration's rule); a date
+ * whose system is singular (too few such paths) gets zero coefficients. */
+double orc_ls_first_pass(int n_ex, const double *bv, const int *vars_each_step, int max_vars,
+                         const double *cash, const double *ev, const double *num, int paths,
+                         int use_cross, int itm_only, double *coeffs) {
+  const int maxd = orc_basis_count(max_vars, use_cross);
+  double *Y = (double *)calloc(paths, sizeof(double));
+  double x[16], phi[64], M[64 * 64], t[64];
+  for (int e = n_ex - 1; e >= 0; --e) {
+    const int v = vars_each_step[e], nd = orc_basis_count(v, use_cross);
+    for (int i = 0; i < nd * nd; ++i) M[i] = 0.0;
+    for (int i = 0; i < nd; ++i) t[i] = 0.0;
+    long cnt = 0;
+    for (int p = 0; p < paths; ++p) {
+      const double exv = ev[(size_t)e * paths + p];
+      if (itm_only && !(exv > 0.0)) continue;
+      ++cnt;
+      for (int i = 0; i < v; ++i) x[i] = bv[((size_t)e * max_vars + i) * paths + p];
+      orc_basis_functions(x, v, use_cross, phi);
+      const double target = cash[(size_t)e * paths + p] + Y[p];
+      for (int i = 0; i < nd; ++i) {
+        for (int j = 0; j <= i; ++j) M[i * nd + j] += phi[i] * phi[j];
+        t[i] += phi[i] * target;
+      }
+    }
+    double *beta = coeffs + (size_t)e * maxd;
+    for (int i = 0; i < maxd; ++i) beta[i] = 0.0;
+    if (cnt > 0) {
+      for (int i = 0; i < nd; ++i) {
+        for (int j = 0; j <= i; ++j) M[i * nd + j] = M[j * nd + i] = M[i * nd + j] / cnt;
+        t[i] /= cnt;
+      }
+      if (orc_cholesky_solve(nd, M, t, beta) != 0)
+        for (int i = 0; i < nd; ++i) beta[i] = 0.0;
+    }
+    for (int p = 0; p < paths; ++p) {
+      const double exv = ev[(size_t)e * paths + p];
+      for (int i = 0; i < v; ++i) x[i] = bv[((size_t)e * max_vars + i) * paths + p];
+      orc_basis_functions(x, v, use_cross, phi);
+      double cont = 0.0;
+      for (int i = 0; i < nd; ++i) cont += phi[i] * beta[i];
+      const int allowed = !itm_only || exv > 0.0;
+      if (allowed && exv > cont)
+        Y[p] = exv;
+      else
+        Y[p] += cash[(size_t)e * paths + p];
+    }
+    if (e > 0)
+      for (int p = 0; p < paths; ++p)
+        Y[p] *= num[(size_t)(e - 1) * paths + p] / num[(size_t)e * paths + p];
+  }
+  double s = 0.0;
+  for (int p = 0; p < paths; ++p) s += Y[p] / num[p];
+  free(Y);
+  return s / paths;
+}
+
+/* Pricing under a frozen strategy: walk the dates forward, collect
+ * cash_e / B_e until the strategy exercises, then add ex_e / B_e and stop
+ * (SecondPassPriceLSAMultiUsingAggregatedFlows_gold,
+ * gold/MultiLS_regression.h:254-300, depth 0). */
+double orc_ls_second_pass(int n_ex, const double *bv, const int *vars_each_step, int max_vars,
+                          const double *cash, const double *ev, const double *num, int paths,
+                          int use_cross, int itm_only, const double *coeffs) {
+  const int maxd = orc_basis_count(max_vars, use_cross);
+  double total = 0.0;
+#pragma omp parallel for reduction(+ : total) schedule(static)
+  for (int p = 0; p < paths; ++p) {
+    double x[16], phi[64], acc = 0.0;
+    for (int e = 0; e < n_ex; ++e) {
+      const int v = vars_each_step[e], nd = orc_basis_count(v, use_cross);
+      const double exv = ev[(size_t)e * paths + p], b = num[(size_t)e * paths + p];
+      for (int i = 0; i < v; ++i) x[i] = bv[((size_t)e * max_vars + i) * paths + p];
+      orc_basis_functions(x, v, use_cross, phi);
+      double cont = 0.0;
+      for (int i = 0; i < nd; ++i) cont += phi[i] * coeffs[(size_t)e * maxd + i];
+      const int allowed = !itm_only || exv > 0.0;
+      if (allowed && cont - exv < 0.0) {
+        acc += exv / b;
+        break;
+      }
+      acc += cash[(size_t)e * paths + p] / b;
+    }
+    total += acc;
+  }
+  return total / paths;
+}
+
+/* ======================================================================
+ * Bermudan payer swaption (the Longstaff-Schwartz workload).
+ *
+ * Exercise date e is evolution step t_e (rates r >= t_e alive).  From the
+ * rates after that step: discount ratios P_{t_e} = 1,
+ * P_{j+1} = P_j / (1 + tau_j f_j) (LMM_evolver_gold.h:460-492), co-terminal
+ * annuity A = sum_{j>=t_e} tau_j P_{j+1} and swap rate SR = (1 - P_N) / A
+ * (LMM_evolver_gold.h:494-560), exercise value max((SR - K) A, 0)
+ * (earlyExerciseBermudanPayerSwaptionGold,
+ * gold/early_exercise_value_generator_gold.h:93-124), basis variables
+ * (SR, f_{t_e}) and the spot numeraire B_{t_e} = prod_{j<t_e} (1 + tau_j f_j(T_j))
+ * (LMM_evolver_gold.h:600-640).  The product pays no cash flow before
+ * exercise.
+ * ==================================================================== */
+void orc_bermudan_paths(const orc_caplet_model *m, const double *z, int n_ex, const int *ex_steps,
+                        double *bv /* (E,2,P) */, double *ev /* (E,P) */, double *num /* (E,P) */) {
+  const int N = m->n_rates, F = m->factors, P = m->paths;
+#pragma omp parallel
+  {
+    double *L = (double *)malloc(sizeof(double) * N * 2), *L2 = L + N;
+    double zs[64];
+#pragma omp for schedule(static)
+    for (int p = 0; p < P; ++p) {
+      for (int r = 0; r < N; ++r) L[r] = log(m->f0[r] + m->disp[r]);
+      double B = 1.0;
+      int e = 0;
+      for (int s = 0; s < N && e < n_ex; ++s) {
+        for (int f = 0; f < F; ++f) zs[f] = z[((size_t)s * F + f) * P + p];
+        pc_step(m, s, L, zs, L2, NULL, NULL, NULL);
+        for (int r = s; r < N; ++r) L[r] = L2[r];
+        const double fs = exp(L[s]) - m->disp[s];
+        if (s == ex_steps[e]) {
+          double disc = 1.0, ann = 0.0;
+          for (int j = s; j < N; ++j) {
+            disc /= 1.0 + m->taus[j] * (exp(L[j]) - m->disp[j]);
+            ann += m->taus[j] * disc;
+          }
+          const double sr = (1.0 - disc) / ann;
+          const double v = (sr - m->strike) * ann;
+          const size_t o = (size_t)e * P + p;
+          ev[o] = v > 0.0 ? v : 0.0;
+          num[o] = B;
+          bv[((size_t)e * 2 + 0) * P + p] = sr;
+          bv[((size_t)e * 2 + 1) * P + p] = fs;
+          ++e;
+        }
+        B *= 1.0 + m->taus[s] * fs;
+      }
+    }
+    free(L);
+  }
+}

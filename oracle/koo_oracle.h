@@ -113,6 +113,21 @@ void orc_caplet_adjoint_paths(const orc_caplet_model *m, const double *z,
 void orc_caplet_pipeline(const orc_caplet_model *m, const uint32_t *ints /* [D][P] */,
                          int do_adjoint, double *price, double *deltas, double *vegas);
 
+/* ---- stage (d): Longstaff-Schwartz -------------------------------------- */
+int orc_basis_count(int n_vars, int use_cross);
+void orc_basis_functions(const double *vars, int n_vars, int use_cross, double *out);
+int orc_cholesky_solve(int n, const double *A, const double *b, double *x);
+double orc_ls_first_pass(int n_ex, const double *basis_vars /* (E,maxVars,P) */,
+                         const int *vars_each_step, int max_vars, const double *cash /* (E,P) */,
+                         const double *ex_values /* (E,P) */, const double *numeraires /* (E,P) */,
+                         int paths, int use_cross, int itm_only, double *coeffs /* (E,maxData) */);
+double orc_ls_second_pass(int n_ex, const double *basis_vars, const int *vars_each_step,
+                          int max_vars, const double *cash, const double *ex_values,
+                          const double *numeraires, int paths, int use_cross, int itm_only,
+                          const double *coeffs);
+void orc_bermudan_paths(const orc_caplet_model *m, const double *z /* (S,F,P) */, int n_ex,
+                        const int *ex_steps, double *bv /* (E,2,P) */, double *ev /* (E,P) */,
+                        double *num /* (E,P) */);
 #ifdef __cplusplus
 }
 #endif

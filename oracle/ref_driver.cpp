@@ -12,6 +12,8 @@
 
 #include <gold/Bridge_allocator_gold.h>
 #include <gold/LMM_evolver_gold.h>
+#include <gold/MultiLS_regression.h>
+#include <gold/Regression_Selector_concrete_gold.h>
 #include <gold/cashFlowDiscounting_gold.h>
 #include <gold/cashFlowGeneration_product_gold.h>
 #include <gold/correlate_drift_gold.h>
@@ -189,6 +191,39 @@ void ref_flat_vol_pseudo_roots(const double* rate_times, const double* evol_time
   std::vector<double> rt = vec(rate_times, n_rates + 1), et = vec(evol_times, steps), v = vec(vols, n_rates);
   Cube_gold<double> c = FlatVolPseudoRootsOfCovariances(rt, et, v, factors, L, beta);
   std::memcpy(A, &c(0, 0, 0), sizeof(double) * steps * n_rates * factors);
+}
+
+// gold/MultiLS_regression.cpp:523 generateRegressionCoefficientsViaLSMultiQuadratic_gold,
+// one regression per date (maxNumberOfExtraRegressions = 0); with `normalise` the gold
+// standardises each basis variable over the regression paths first.
+double ref_ls_multi_depth0(int n_ex, const double* basis_vars, const int* vars_each_step, int max_vars,
+                           const double* cash, const double* ex_values, const double* numeraires,
+                           int paths, int use_cross, int normalise, double* coeffs_out) {
+  std::vector<double> products, targets;
+  const int maxd = use_cross ? (max_vars == 0 ? 1 : max_vars == 1 ? 3 : max_vars == 2 ? 6 : 1 + 3 * max_vars)
+                             : 1 + 2 * max_vars;
+  std::vector<double> coeffs((size_t)n_ex * 1 * maxd, 0.0), lower(n_ex, 0.0), upper(n_ex, 0.0);
+  std::vector<double> means((size_t)n_ex * max_vars, 0.0), sds((size_t)n_ex * max_vars, 1.0);
+  CubeFacade<double> coeff_cube(&coeffs[0], n_ex, 1, maxd);
+  MatrixFacade<double> lower_mat(&lower[0], n_ex, 1), upper_mat(&upper[0], n_ex, 1);
+  CubeFacade<double> means_cube(&means[0], n_ex, 1, max_vars), sds_cube(&sds[0], n_ex, 1, max_vars);
+  std::vector<double> bv(basis_vars, basis_vars + (size_t)n_ex * max_vars * paths);
+  std::vector<double> cf(cash, cash + (size_t)n_ex * paths), ev(ex_values, ex_values + (size_t)n_ex * paths);
+  std::vector<double> nm(numeraires, numeraires + (size_t)n_ex * paths);
+  CubeConstFacade<double> bv_cube(&bv[0], n_ex, max_vars, paths);
+  MatrixConstFacade<double> cf_mat(&cf[0], n_ex, paths), ev_mat(&ev[0], n_ex, paths);
+  MatrixFacade<double> nm_mat(&nm[0], n_ex, paths);
+  std::vector<int> ves(vars_each_step, vars_each_step + n_ex);
+  // Fractions that make the selector keep every path without its cut search
+  // (Regression_Selector_concrete_gold.cpp: the early return when the
+  // pre-selected count lies between the two targets): the cuts are unused at
+  // depth 0, and the search can loop forever when many net values are exactly 0.
+  RegressionSelector_gold_Fraction selector(0.5, 1.5, 2.0, 0.8);
+  double price = generateRegressionCoefficientsViaLSMultiQuadratic_gold(
+      n_ex, products, targets, coeff_cube, lower_mat, upper_mat, means_cube, sds_cube, normalise != 0, bv_cube, ves,
+      max_vars, cf_mat, ev_mat, nm_mat, paths, 0, 10, selector, use_cross != 0);
+  std::memcpy(coeffs_out, &coeffs[0], sizeof(double) * coeffs.size());
+  return price;
 }
 
 }  // extern "C"

@@ -1,0 +1,163 @@
+"""Pins the Longstaff-Schwartz part of the FP64 oracle: basis layouts, the
+normal-equation solve, the backward induction against the reference's own
+gold routine (oracle/_ref, built from the reference sources in this
+container), the Bermudan path quantities against the staged curve routines
+(themselves pinned against the gold), and limits with closed forms."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference gold library not built")
+
+
+def test_basis_function_layouts():
+    x, y, z, w = 0.3, -1.7, 2.5, 0.9
+    assert O.basis_count(0) == 1 and O.basis_count(1) == 3 and O.basis_count(2) == 6
+    assert O.basis_count(3) == 10 and O.basis_count(4) == 13
+    assert O.basis_count(3, use_cross=False) == 7
+    assert np.allclose(O.basis_functions([x]), [1, x, x * x])
+    assert np.allclose(O.basis_functions([x, y]), [1, x, y, x * x, x * y, y * y])
+    assert np.allclose(O.basis_functions([x, y, z]),
+                       [1, x, y, z, x * y, y * z, z * x, x * x, y * y, z * z])
+    # four variables: neighbour cross terms only (x0x1, x1x2, x2x3, x3x0)
+    assert np.allclose(O.basis_functions([x, y, z, w]),
+                       [1, x, y, z, w, x * y, y * z, z * w, w * x, x * x, y * y, z * z, w * w])
+    assert np.allclose(O.basis_functions([x, y], use_cross=False), [1, x, x * x, y, y * y])
+
+
+def test_cholesky_solve_matches_numpy():
+    rng = np.random.RandomState(5)
+    for n in (1, 6, 13):
+        G = rng.normal(size=(n + 3, n))
+        A = G.T @ G + 0.1 * np.eye(n)
+        b = rng.normal(size=n)
+        x = O.cholesky_solve(A, b)
+        assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    A = np.ones((3, 3))  # rank one: a zero pivot
+    assert O.cholesky_solve(A, np.ones(3)) is None
+
+
+def _small_bermudan(P=2048, N=10, F=3, seed=11, K=0.05):
+    rt = 0.5 * np.arange(1, N + 2)
+    vols = np.full(N, 0.2)
+    A = O.flat_vol_pseudo_roots(rt, rt[:N], vols, F, 0.0, 0.1)
+    rng = np.random.RandomState(seed)
+    f0 = 0.05 + rng.uniform(-0.005, 0.005, N)
+    z = rng.normal(size=(N, F, P))
+    m = O.caplet_model(f0, np.full(N, 0.5), np.zeros(N), A, vols, K, P)
+    ex = np.arange(0, N, 2, dtype=np.int32)
+    return m, z, ex, f0, A
+
+
+def test_bermudan_paths_equal_staged_curve_routines():
+    m, z, ex, f0, A = _small_bermudan(P=96)
+    N, P = m.n_rates, m.paths
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    taus = np.full(N, 0.5)
+    rates, _, _ = O.lmm_pc_evolve(f0, taus, z, A, np.zeros(N), np.arange(N, dtype=np.int32))
+    L = O.lib()
+    alive = np.arange(N, dtype=np.int32)
+    disc = np.zeros(N * (N + 1) * P)
+    L.orc_discount_ratios(np.ascontiguousarray(rates).reshape(-1), taus, alive, P, N, N, disc)
+    ann = np.zeros(N * N * P)
+    L.orc_coterminal_annuities(disc, taus, alive, P, N, N, ann)
+    sr = np.zeros(N * N * P)
+    L.orc_coterminal_swap_rates(disc, ann, alive, P, N, N, sr)
+    numer = np.zeros(N * P)
+    L.orc_spot_numeraires(disc, P, N, numer)
+    ann, sr, numer = ann.reshape(N, N, P), sr.reshape(N, N, P), numer.reshape(N, P)
+    for e, s in enumerate(ex):
+        assert np.allclose(bv[e, 0], sr[s, s], rtol=1e-12)
+        assert np.allclose(bv[e, 1], rates[s, s], rtol=1e-12)
+        assert np.allclose(num[e], numer[s], rtol=1e-12)
+        want = np.maximum((sr[s, s] - m.strike) * ann[s, s], 0.0)
+        assert np.allclose(ev[e], want, rtol=1e-11, atol=1e-16)
+
+
+@needs_ref
+@pytest.mark.parametrize("use_cross", [True, False])
+@pytest.mark.parametrize("seed,P", [(11, 2048), (12, 8192)])
+def test_ls_first_pass_matches_reference_gold(use_cross, seed, P):
+    # The gold standardises the basis variables (normalise = true) and solves
+    # by SVD; the oracle solves the raw normal equations by Cholesky.  The
+    # least-squares fit of an affine-invariant quadratic basis is the same
+    # function either way, so the exercise decisions and the price agree.
+    m, z, ex, _, _ = _small_bermudan(P=P, seed=seed)
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    E, P = ev.shape
+    ves = np.full(E, 2, np.int32)
+    ves[-1] = 1  # the last date regresses on one variable, as near maturity in the reference
+    cash = np.zeros((E, P))
+    cash[1::2] = 1e-3 * np.cos(np.arange(P))  # exercise the cash-flow terms too
+    price, _ = O.ls_first_pass(bv, ves, cash, ev, num, use_cross=use_cross)
+    rprice, _ = O.ref_ls_first_pass(bv, ves, cash, ev, num, use_cross=use_cross, normalise=True)
+    assert abs(price - rprice) <= 1e-12 * abs(rprice)
+
+
+def test_ls_first_pass_is_invariant_to_affine_rescaling_of_the_variables():
+    m, z, ex, _, _ = _small_bermudan(P=4096, seed=3)
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    E, P = ev.shape
+    ves = np.full(E, 2, np.int32)
+    cash = np.zeros((E, P))
+    a, _ = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=True)
+    b, _ = O.ls_first_pass((bv - 0.05) * 100.0, ves, cash, ev, num, itm_only=True)
+    assert abs(a - b) <= 1e-12 * a
+
+
+def test_ls_itm_only_equals_all_paths_when_every_path_is_in_the_money():
+    rng = np.random.RandomState(2)
+    E, P = 4, 500
+    bv = rng.uniform(0.02, 0.08, size=(E, 2, P))
+    ev = rng.uniform(0.001, 0.02, size=(E, P))  # strictly positive
+    num = np.cumprod(1 + 0.5 * rng.uniform(0.03, 0.06, size=(E, P)), axis=0)
+    cash = np.zeros((E, P))
+    ves = np.full(E, 2, np.int32)
+    a = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=False)
+    b = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=True)
+    assert a[0] == b[0] and np.array_equal(a[1], b[1])
+
+
+def test_ls_itm_only_regresses_on_the_in_the_money_rows():
+    # One date after the last: regression target is the deflated next value.
+    m, z, ex, _, _ = _small_bermudan(P=4096)
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    E, P = ev.shape
+    ves = np.full(E, 2, np.int32)
+    cash = np.zeros((E, P))
+    _, coeffs = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=True)
+    # Date E-2's coefficients are the least-squares fit of (ev_{E-1} B_{E-2}/B_{E-1})
+    # on the ITM rows of date E-2 (the last date always exercises when ITM).
+    e = E - 2
+    y = np.where(ev[E - 1] > 0, ev[E - 1], 0.0) * num[e] / num[E - 1]
+    itm = ev[e] > 0
+    X = np.stack([O.basis_functions(bv[e, :, p]) for p in np.nonzero(itm)[0]])
+    beta, *_ = np.linalg.lstsq(X, y[itm], rcond=None)
+    assert np.allclose(X @ coeffs[e], X @ beta, rtol=1e-8, atol=1e-12)
+
+
+def test_ls_one_date_is_the_european_price():
+    m, z, ex, _, _ = _small_bermudan(P=4096)
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    one = slice(2, 3)
+    for itm in (False, True):
+        price, coeffs = O.ls_first_pass(bv[one], [2], np.zeros((1, ev.shape[1])), ev[one],
+                                        num[one], itm_only=itm)
+        assert np.all(coeffs == 0.0)  # the regression target is identically zero
+        assert np.isclose(price, np.mean(ev[2] / num[2]), rtol=1e-13)
+
+
+def test_ls_second_pass_under_first_pass_strategy_is_close_and_bounded():
+    m, z, ex, _, _ = _small_bermudan(P=8192)
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    E, P = ev.shape
+    ves = np.full(E, 2, np.int32)
+    cash = np.zeros((E, P))
+    price1, coeffs = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=True)
+    price2 = O.ls_second_pass(bv, ves, cash, ev, num, coeffs, itm_only=True)
+    # exercising never beats perfect foresight, and beats every fixed date
+    foresight = np.mean(np.max(ev / num, axis=0))
+    best_fixed = max(np.mean(ev[e] / num[e]) for e in range(E))
+    assert best_fixed - 1e-12 <= price2 <= foresight
+    assert abs(price1 - price2) < 0.02 * price2
