@@ -336,6 +336,150 @@ int lmm_price_bwd(int paths, int rates, int factors, int checkpoint_every, const
                   const float* product_aux, const float* checkpoints, float* segment_scratch,
                   double* partials, double* greeks_out, void* stream);
 
+/* =====================================================================
+ * Stage (d): Longstaff-Schwartz.
+ *
+ * Layouts are the reference's: basis variables (E, maxVars, P), basis
+ * functions (k, P), exercise values / cash flows / numeraires (E, P), all
+ * path-minor FP32 device arrays.  Host arrays are marked.
+ * =================================================================== */
+
+/* LS_Basis_gpu.h:168-184 (basisVariableExample / basisVariableLog_gpu):
+ * three variables per exercise date from rate1..3 (E, batch) into
+ * (E, 3, total_paths) at column offset paths_previously_done; with use_logs,
+ * log(rate_i + float_data[i]) (float_data is a device array of 3 shifts).
+ * integer_data, forwards, discount_ratios, steps_and_rates and the exercise
+ * index arrays are accepted for signature parity and unused, as in the
+ * reference kernels. */
+int adjoinBasisVariablesCaller_gpu(int use_logs, const int* integer_data, const float* float_data,
+                                   const float* forwards, const float* discount_ratios,
+                                   const float* rate1, const float* rate2, const float* rate3,
+                                   int paths_per_batch, int total_paths, int paths_previously_done,
+                                   int steps_and_rates, int number_exercise_times,
+                                   const int* exercise_indices, const int* exercise_indicators,
+                                   float* basis_variables, void* stream);
+
+/* early_exercise_value_generator_gpu.h:195-210 (earlyExerciseBermudanPayerSwaption,
+ * exercise_values_examples.h:49-78): max((rate1 - K) * rate2, 0) with rate1 the
+ * co-terminal swap rate and rate2 its annuity, K = aux_float_data[0] (device). */
+int adjoinExerciseValues_Bermudan_swaption_gpu(
+    float* exercise_values, const float* aux_float_data, const int* aux_int_data,
+    const int* exercise_indices, const int* exercise_indicators, int number_exercise_times,
+    int paths_for_output, int output_offset, int batch_paths, int number_steps_and_rates,
+    const float* rates1, const float* rates2, const float* rates3, const float* forwards,
+    const float* disc_ratios, void* stream);
+
+/* LS_gpu.h:21-29: quadratic basis functions of exercise date
+ * exercise_number's variables (cross terms: 1, x_i, neighbour products,
+ * squares; plain: 1, x_0, x_0^2, ...).  At most 4 variables. */
+int expandBasisFunctions_quadratic_gpu(int total_paths, const float* basis_variables,
+                                       int exercise_number, int max_basis_variable_size,
+                                       int variable_size_this_step, int use_cross_terms,
+                                       float* basis_functions, void* stream);
+
+/* LS_gpu.h:32-39: out = shift + sum_i w_i basis_i - exercise_values
+ * (exercise_values already points at this date, as in the reference). */
+int continuation_net_value_evaluation_gpu(int paths, int exercise_number,
+                                          const float* basis_functions, int number_basis_functions,
+                                          const float* basis_weights, const float* exercise_values,
+                                          float andersen_shift, float* output, void* stream);
+
+/* LS_gpu.h:58-72: one date of the multi-regression first pass: nested
+ * regressions inside the cut bands, exercise when the net value is negative
+ * (continuation_values := exercise value), est/exercises per path. */
+int oneStepUpdateLS_multi_gpu(int paths, int exercise_number, const float* basis_functions,
+                              int number_basis_functions, int basis_functions_row_size,
+                              int max_regression_depth, const float* basis_weights,
+                              const float* lower_cuts, const float* upper_cuts,
+                              float* continuation_values, const float* deflated_cash_flows,
+                              const float* exercise_values, float andersen_shift,
+                              float* est_continuation_values, int* exercises, void* stream);
+
+/* LS_gpu.h:77-81: values *= numeraires[new] / numeraires[old]. */
+int updateDeflationOfContinuationValues_gpu(float* deflated_next_step_values,
+                                            const float* numeraire_values, int paths,
+                                            int new_step_number, int old_step_number,
+                                            void* stream);
+
+/* outerProduct_gpu.h:12-40: answer[r][c] = sum_p in1[r][p] in2[c][p]
+ * (row-major), on MFMA with FP32 accumulation and FP64 block partials.  The
+ * workspace keeps the reference's contract (>= blocks*row_size*col_size
+ * floats, 8-byte aligned); threads is ignored. */
+int reduceOuterProduct_gpu(int paths, int row_size, int col_size, int symmetric, int threads,
+                           int blocks, const float* input_data_global1,
+                           const float* input_data_global2, float* answer_global,
+                           float* workspace_data_global, void* stream);
+int reducedOuterProduct_gpu(int paths, int row_size, int col_size, int threads, int blocks,
+                            const float* input_data_global1, const float* input_data_global2,
+                            float* answer_global, float* workspace_data_global, void* stream);
+int reduceOuterProductSymmetric_gpu(int paths, int row_size, int threads, int blocks,
+                                    const float* input_data_global1, float* answer_global,
+                                    float* workspace_data_global, void* stream);
+
+/* outerProduct_gpu.h:42-67 (the cuBLAS variants): answer = alpha * in1 in2^T
+ * + beta * answer, column-major with leading dimension row_size, computed by
+ * the same MFMA kernel; the workspace (new argument) holds
+ * outer_product_workspace_floats(paths) floats. */
+size_t outer_product_workspace_floats(int paths);
+int reduceOuterProductSymmetric_cublas(int paths, int row_size, float alpha, float beta,
+                                       const float* input_data_global, float* answer_global,
+                                       float* workspace_data_global, void* stream);
+int reduceOuterProduct_cublas(int paths, int row_size, int col_size, float alpha, float beta,
+                              const float* input_data_global1, const float* input_data_global2,
+                              float* answer_global, float* workspace_data_global, void* stream);
+
+/* New: Bermudan payer swaption path quantities fused with the LMM evolution
+ * (same model as lmm_price_fwd).  exercise_steps (HOST, increasing) are
+ * evolution steps; per date e: basis_variables[e][0] = co-terminal swap rate
+ * SR(t_e), [e][1] = f_{t_e}(T_{t_e}); exercise_values[e] = max((SR - K) A, 0)
+ * with A the co-terminal annuity (values at T_{t_e}); numeraires[e] = spot
+ * numeraire B_{t_e}.  Replaces the evolve + discount ratios + annuities +
+ * swap rates + extraction + exercise value chain of LS_test.cu:680-780. */
+int bermudan_paths_gpu(int paths, int rates, int factors, const float* variates,
+                       const float* initial_rates, const float* taus, const float* displacements,
+                       const float* pseudo_roots, int n_ex, const int* exercise_steps, float strike,
+                       float* basis_variables, float* exercise_values, float* numeraires,
+                       void* stream);
+
+/* New: the fused LS first pass (backward induction; one regression per date,
+ * generateRegressionCoefficientsViaLSMultiQuadratic_gold at depth 0,
+ * gold/MultiLS_regression.cpp:20-420).  Per date the normal equations of
+ * the quadratic basis in the normalised variables x~ = (x - var_shift) *
+ * var_scale (HOST arrays (E, max_vars), NULL for 0 / 1) and of the target
+ * cash + deflated next value are accumulated on MFMA (FP32) over all paths
+ * or, with itm_only, over paths whose exercise value is positive, summed in
+ * FP64 and solved by FP64 Cholesky.  coefficients (device, E x
+ * basis_count(max_vars)) apply to x~; price_out (device double) is the
+ * first-pass price.  vars_each_step is a HOST array; cash_flows may be NULL
+ * (no flows); workspace holds ls_workspace(paths) bytes.  max_vars <= 4. */
+int ls_workspace(int paths, size_t* bytes);
+int ls_first_pass_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,
+                      int max_vars, const float* cash_flows, const float* exercise_values,
+                      const float* numeraires, int paths, int use_cross, int itm_only,
+                      const float* var_shift, const float* var_scale, void* workspace,
+                      double* coefficients, double* price_out, void* stream);
+
+/* New: price under frozen coefficients (SecondPassPriceLSAMultiUsingAggregatedFlows_gold,
+ * gold/MultiLS_regression.h:237-289, depth 0). */
+int ls_second_pass_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,
+                       int max_vars, const float* cash_flows, const float* exercise_values,
+                       const float* numeraires, int paths, int use_cross, int itm_only,
+                       const float* var_shift, const float* var_scale, const double* coefficients,
+                       void* workspace, double* price_out, void* stream);
+
+/* New: Gram matrix of [basis rows; target] over the paths whose mask is
+ * non-zero (mask and target may be NULL), (n x n) FP64 with
+ * n = n_basis + (target != NULL) <= 16; X^T X and X^T y of the LS normal
+ * equations in one MFMA contraction.  workspace: ls_workspace(paths) bytes. */
+int ls_normal_eq_gpu(int paths, int n_basis, const float* basis_functions, const float* target,
+                     const float* mask, void* workspace, double* gram_out, void* stream);
+
+/* New: FP64 Cholesky solve of A x = b (n < 16, device arrays, A row-major);
+ * info (device, may be NULL) = 0, or 1 when A is not positive definite
+ * (x = 0).  Replaces the host SVD of LinearSolverConcrete_main.cu:127-188. */
+int cholesky_solve_f64_gpu(int n, const double* A, const double* b, double* x, int* info,
+                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared():
     hdr = open(os.path.join(ROOT, "include", "koo_c_abi.h")).read()
-    return set(re.findall(r"^(?:int|long long|const char\*)\s+(\w+)\(", hdr, re.M))
+    return set(re.findall(r"^(?:int|long long|size_t|const char\*)\s+(\w+)\(", hdr, re.M))
 
 
 def test_library_exports_every_declared_symbol():
@@ -50,6 +50,14 @@ def _status(name, *args):
     ("lmm_price_fwd", (16, 200, 5, 8) + (None,) * 5 + (1,) + (None,) * 5),
     ("lmm_price_fwd", (16, 40, 5, 8) + (None,) * 5 + (7,) + (None,) * 5),
     ("inverse_normal_u32_gpu", (None, None, -3, None)),
+    ("bermudan_paths_gpu", (16, 100, 3) + (None,) * 5 + (10, None, 0.05) + (None,) * 4),
+    ("ls_first_pass_gpu", (0, None, None, 2, None, None, None, 16, 1, 1) + (None,) * 6),
+    ("ls_second_pass_gpu", (4, None, None, 7, None, None, None, 16, 1, 1) + (None,) * 6),
+    ("ls_normal_eq_gpu", (16, 16, None, None, None, None, None, None)),
+    ("cholesky_solve_f64_gpu", (16, None, None, None, None, None)),
+    ("reduceOuterProduct_gpu", (0, 6, 6, 1, 256, 256, None, None, None, None, None)),
+    ("expandBasisFunctions_quadratic_gpu", (16, None, 0, 2, 5, 1, None, None)),
+    ("oneStepUpdateLS_multi_gpu", (16, 0, None, 6, 3, 1) + (None,) * 6 + (0.0, None, None, None)),
 ])
 def test_invalid_arguments_are_rejected(call, args):
     st, msg = _status(call, *args)
@@ -68,3 +76,5 @@ def test_workspace_sizes():
     assert pa >= 8 * (2**20 // 256) * 80
     ck1, sc1, _ = koo.lmm_price_workspace(2**20, 40, 5, 1)
     assert ck1 == 4 * 40 * (41 + 10) * 2**20 and sc1 == 4
+    assert koo.ls_workspace(2**18) >= 4 * 2**18 + 8 * 256
+    assert koo.outer_product_workspace_floats(2**18) >= 2 * 256

@@ -30,6 +30,9 @@ class LMMWorkload:
     checkpoint_every: int = 8
     displacements: np.ndarray = field(default=None)
     path_offset: int = 0
+    exercise_steps: np.ndarray = field(default=None)  # Bermudan: exercise evolution steps
+    basis_shift: np.ndarray = field(default=None)  # (E, 2): LS variables are (x - shift) * scale
+    basis_scale: np.ndarray = field(default=None)
 
     @property
     def steps(self):
@@ -81,6 +84,27 @@ def payer_swap(paths=2**15, seed=2023):
     A = koo.flat_vol_pseudo_roots(rate_times, rate_times[:N], vols, F, 0.0, 0.1)
     return LMMWorkload("lmm_payer_swap_20x3", paths, N, F, tau, f0, vols, rate_times, A, 0.05,
                        "swap", _scramblers(rng, N * F))
+
+
+def bermudan_swaption(paths=2**18, seed=2025):
+    """Bermudan payer swaption by Longstaff-Schwartz: 20 forwards, tau = 0.5,
+    f0 = 0.05 + U(-0.005, 0.005), flat vol 0.2, exp(-0.1 |T_i - T_j|) reduced
+    to 3 factors by PCA per step, 2^18 Sobol paths.  Exercise into the swap to
+    T_20 at strike 0.05 on 10 dates, the resets T_0, T_2, ..., T_18 (steps
+    0, 2, ..., 18).  Basis: quadratic with cross term in the co-terminal swap
+    rate and the first alive forward (6 functions), regressed on
+    in-the-money paths; the GPU works in the variables (x - 0.05) * 100."""
+    N, F, tau = 20, 3, 0.5
+    rng = np.random.RandomState(seed)
+    f0 = 0.05 + rng.uniform(-0.005, 0.005, N)
+    rate_times = tau * np.arange(1, N + 2)
+    vols = np.full(N, 0.20)
+    A = koo.flat_vol_pseudo_roots(rate_times, rate_times[:N], vols, F, 0.0, 0.1)
+    ex = np.arange(0, N, 2, dtype=np.int32)
+    K = 0.05
+    return LMMWorkload("bermudan_payer_20x3", paths, N, F, tau, f0, vols, rate_times, A, K,
+                       "bermudan", _scramblers(rng, N * F), exercise_steps=ex,
+                       basis_shift=np.full((ex.size, 2), K), basis_scale=np.full((ex.size, 2), 100.0))
 
 
 def bridge_allocation(w):

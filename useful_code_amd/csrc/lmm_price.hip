@@ -20,66 +20,14 @@
 // uniform addresses (scalar loads); the per-path state lives in LDS laid out
 // [rate][lane] so wavefront accesses are conflict free; block partial sums
 // are formed in FP64 and a final single pass sums them with compensation.
-#include "koo_common.h"
+#include "lmm_step.h"
 
 namespace koo {
 
 namespace {
 
-constexpr int kMaxRates = 90;
-constexpr int kMaxFactors = 8;
+using namespace lmm;
 
-__device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
-__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
-__device__ __forceinline__ float dpp_f32_quad_xor1(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float dpp_f32_quad_xor2(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float dpp_f32_row_half_mirror(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float dpp_f32_row_mirror(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float readlane_f32(float x, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
-}
-
-// Sum over the 64 lanes of a wavefront with DPP row operations (quad
-// butterflies, then half-row and row mirrors give each 16-lane row its sum;
-// the four row sums are read from lanes 0, 16, 32, 48).  Returns the total,
-// uniform across the wavefront.  All 64 lanes must be active.
-__device__ __forceinline__ float wave_sum_f32(float v) {
-  v += dpp_f32_quad_xor1(v);
-  v += dpp_f32_quad_xor2(v);
-  v += dpp_f32_row_half_mirror(v);
-  v += dpp_f32_row_mirror(v);
-  return (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
-}
-
-__device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Block-wide FP64 sum of one value per lane into part[col]; every lane of the
-// block must call it (contains barriers).
-__device__ __forceinline__ void block_sum_to(double v, double* scratch_lds, double* out) {
-  v = wave_sum_f64(v);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) scratch_lds[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) t += scratch_lds[i];
-    *out = t;
-  }
-}
 
 // Caplet value at fixing, tau max(f-K,0)/(1+tau f), and its f-derivative.
 __device__ __forceinline__ float caplet_value(float f, float tau, float K, float inv) {
@@ -88,94 +36,6 @@ __device__ __forceinline__ float caplet_value(float f, float tau, float K, float
 __device__ __forceinline__ float caplet_deriv(float f, float tau, float K, float inv) {
   return f > K ? tau * (1.0f + tau * K) * inv * inv : 0.0f;
 }
-
-struct Model {
-  const float* z;     // (S,F,P) bridged variates
-  const float* f0;    // N
-  const float* taus;  // N
-  const float* disp;  // N
-  const float* A;     // (S,N,F)
-  float strike;
-  int P, N, k;
-};
-
-// Fixed drifts c(s,r) = -0.5 sum_f A(s,r,f)^2 into LDS (read back with
-// uniform addresses); every lane of the block takes part, then a barrier.
-template <int F>
-__device__ __forceinline__ void fill_fixed_drifts(const Model& m, float* ctab) {
-  const int nn = m.N * m.N;
-  for (int i = threadIdx.x; i < nn; i += blockDim.x) {
-    const float* a = m.A + static_cast<size_t>(i) * F;
-    float c = 0.0f;
-#pragma unroll
-    for (int f = 0; f < F; ++f) c -= 0.5f * a[f] * a[f];
-    ctab[i] = c;
-  }
-}
-
-// One predictor-corrector step s over rates r >= s.  `load(r)` gives the
-// start log-rate (issued one rate ahead), `store(r, l)` receives the end
-// log-rate; e / ep return the final drift sums; cs = fixed drifts of step s.
-template <int F, class Load, class Store>
-__device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
-                                        const float (&z)[F], float (&e)[F], float (&ep)[F],
-                                        Load load, Store store) {
-  const float* As = m.A + s * m.N * F;  // 32-bit: keeps the address scalar
-#pragma unroll
-  for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
-  // start log-rates are issued two rates ahead of their use
-  float l_next = load(s);
-  float l_next2 = (s + 1 < m.N) ? load(s + 1) : 0.0f;
-  for (int r = s; r < m.N; ++r) {
-    const float l = l_next;
-    l_next = l_next2;
-    if (r + 2 < m.N) l_next2 = load(r + 2);
-    float a[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
-    const float tau = m.taus[r], d = m.disp[r];
-    const float x = fast_exp(l);
-    const float g = tau * x * fast_rcp(1.0f + tau * (x - d));
-    float mu = 0.0f, dw = 0.0f;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      e[f] += g * a[f];
-      mu += a[f] * e[f];
-      dw += a[f] * z[f];
-    }
-    const float lh = l + mu + cs[r] + dw;
-    const float xh = fast_exp(lh);
-    const float gh = tau * xh * fast_rcp(1.0f + tau * (xh - d));
-    float muh = 0.0f;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      ep[f] += gh * a[f];
-      muh += a[f] * ep[f];
-    }
-    store(r, lh + 0.5f * (muh - mu));
-  }
-}
-
-template <int F>
-__device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[F]) {
-  const float* zs = m.z + static_cast<size_t>(s) * F * m.P + p;
-#pragma unroll
-  for (int f = 0; f < F; ++f) z[f] = zs[static_cast<size_t>(f) * m.P];
-}
-
-// ----------------------------------------------------------------------
-// Forward: price per path, block partial sums, optional checkpoints
-// ckpt[seg][r][P] (r < N: log-rate at the start of step seg*k; row N: the
-// running discount 1/B at that step).  LDS: state [N][lanes], then the
-// fixed-drift table [N][N].
-// ----------------------------------------------------------------------
-// The model arrays are separate __restrict__ kernel arguments (not members of
-// a by-value struct) so the compiler may read them through the scalar cache.
-#define KOO_MODEL_PARAMS                                                                  \
-  const float* __restrict__ z_, const float* __restrict__ f0_,                            \
-      const float* __restrict__ taus_, const float* __restrict__ disp_,                   \
-      const float* __restrict__ A_, float strike_, int P_, int N_, int k_
-#define KOO_MODEL_LOCAL const Model m{z_, f0_, taus_, disp_, A_, strike_, P_, N_, k_}
 
 template <int F>
 __global__ void __launch_bounds__(256)
@@ -401,35 +261,6 @@ __global__ void __launch_bounds__(256)
       for (int q = 0; q < kVegaSlots; ++q)
         t += vslots[(static_cast<size_t>(w) * N + r) * kVegaSlots + q];
     part[static_cast<size_t>(blockIdx.x) * 2 * N + N + r] = t / vols[r];
-  }
-}
-
-// out[c] = (sum_b part[b*ncols + c]) * scale, compensated; one block per column.
-__global__ void __launch_bounds__(256)
-    k_finalize_cols(const double* __restrict__ part, int nb, int ncols, double scale,
-                    double* __restrict__ out) {
-  __shared__ double sv[256], sc[256];
-  const int col = blockIdx.x;
-  double s = 0.0, c = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    const double x = part[static_cast<size_t>(b) * ncols + col];
-    const double t = s + x;
-    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
-    s = t;
-  }
-  sv[threadIdx.x] = s;
-  sc[threadIdx.x] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double S = 0.0, C = 0.0;
-    for (int i = 0; i < static_cast<int>(blockDim.x); ++i) {
-      const double x = sv[i];
-      const double t = S + x;
-      C += (fabs(S) >= fabs(x)) ? (S - t) + x : (x - t) + S;
-      S = t;
-      C += sc[i];
-    }
-    out[col] = (S + C) * scale;
   }
 }
 

@@ -1,0 +1,1094 @@
+// ls.hip -- Longstaff-Schwartz regression on gfx950.
+//
+// * k_berm_paths: one lane per path evolves the LMM (the shared
+//   predictor-corrector step) up to the last exercise date and writes, per
+//   exercise date, the basis variables (co-terminal swap rate, first alive
+//   forward), the Bermudan payer swaption exercise value and the spot
+//   numeraire.  The rate cube is never written.
+// * k_gram: sum over paths of the outer product of up to 16 path-minor rows
+//   (X^T X and X^T y in one Gram matrix), on MFMA v_mfma_f32_16x16x4f32 with
+//   FP32 accumulation inside a block and FP64 block partials.
+// * k_ls_date: one backward-induction date of the LS first pass: applies the
+//   later date's exercise decision and numeraire deflation to the running
+//   continuation values, accumulates this date's normal equations (optionally
+//   over in-the-money paths only) on MFMA, and the last block to finish sums
+//   the partials in FP64 and solves them by FP64 Cholesky.
+// * k_ls_second: the frozen-strategy pass (price under given coefficients).
+// * the reference's small LS kernels (basis variables, exercise values, basis
+//   expansion, continuation values, update, deflation).
+//
+// MFMA layout used (v_mfma_f32_16x16x4f32, one 16x16x4 block per wave): lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; result element i of lane l is
+// D[4*(l/16)+i][l%16].  For a Gram matrix A == B: row m of A holds basis
+// function m of the four paths k = 0..3.  With at most 8 rows, the 16 rows
+// hold two groups of 4 paths (rows 0-7 and 8-15), so each instruction takes
+// 8 paths and the two diagonal 8x8 blocks of D are added at the end.
+#include "lmm_step.h"
+
+namespace koo {
+namespace {
+
+using namespace lmm;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxBasisVars = 4;  // cross basis with 4 variables: 13 functions
+constexpr int kMaxGram = 16;      // rows of one Gram matrix (basis functions + target)
+constexpr int kLsThreads = 256;
+
+__device__ __forceinline__ floatx4 mfma16x16x4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float pick4(int i, float x0, float x1, float x2, float x3) {
+  return i == 0 ? x0 : (i == 1 ? x1 : (i == 2 ? x2 : x3));
+}
+
+// Quadratic basis function j of v variables.  Cross variant
+// (quadraticPolynomialCrossDevice, LS_gpu.h:138-229): 1, x_i, then for v = 1
+// x^2, for v = 2 x^2, xy, y^2, for v >= 3 x_i x_{(i+1) mod v} then x_i^2.
+// Plain variant (LS_gpu.h:92-126): 1, x_0, x_0^2, x_1, x_1^2, ...
+// Branch-free in j so lanes with different j stay converged.
+__device__ __forceinline__ float basis_value(int j, int v, bool cross, float x0, float x1, float x2,
+                                             float x3) {
+  if (j == 0) return 1.0f;
+  if (!cross) {
+    const float x = pick4((j - 1) >> 1, x0, x1, x2, x3);
+    return (j & 1) ? x : x * x;
+  }
+  if (j <= v) return pick4(j - 1, x0, x1, x2, x3);
+  if (v == 1) return x0 * x0;
+  if (v == 2) return j == 3 ? x0 * x0 : (j == 4 ? x0 * x1 : x1 * x1);
+  const int i = j - 1 - v;
+  if (i < v) {
+    const int i2 = (i + 1 == v) ? 0 : i + 1;
+    return pick4(i, x0, x1, x2, x3) * pick4(i2, x0, x1, x2, x3);
+  }
+  const float x = pick4(i - v, x0, x1, x2, x3);
+  return x * x;
+}
+
+__host__ __device__ inline int basis_count(int v, bool cross) {
+  if (!cross) return 1 + 2 * v;
+  return v == 0 ? 1 : (v == 1 ? 3 : (v == 2 ? 6 : 1 + 3 * v));
+}
+
+// Adds the two diagonal 8x8 blocks (groups == 2) or takes the 16x16 result
+// (groups == 1) of each wave's accumulator and sums the waves of the block in
+// FP64: out[i*n + j] for i, j < n.  Every lane of the block calls it.
+template <int GROUPS>
+__device__ __forceinline__ void block_gram(const floatx4& acc, int n, float* lds_d /* [nw][256] */,
+                                           double* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  float* d = lds_d + wave * 256;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
+  __syncthreads();
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t - i * n;
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      const float* dw = lds_d + w * 256;
+      s += static_cast<double>(dw[i * 16 + j]);
+      if (GROUPS == 2) s += static_cast<double>(dw[(8 + i) * 16 + 8 + j]);
+    }
+    out[t] = s;
+  }
+}
+
+// Sums part[b][t] over the blocks b < nb for t < n*n into sum[t] (LDS),
+// using all lanes of the block; contains barriers.
+__device__ __forceinline__ void sum_block_partials(const double* part, int nb, int n,
+                                                   double* red /* [256] */, double* sum) {
+  const int nn = n * n;
+  const int slices = blockDim.x / nn;
+  const int t = threadIdx.x;
+  if (t < slices * nn) {
+    const int entry = t % nn, slice = t / nn;
+    double s = 0.0;
+    for (int b = slice; b < nb; b += slices) s += part[static_cast<size_t>(b) * nn + entry];
+    red[t] = s;
+  }
+  __syncthreads();
+  if (t < nn) {
+    double s = 0.0;
+    for (int q = 0; q < slices; ++q) s += red[q * nn + t];
+    sum[t] = s;
+  }
+  __syncthreads();
+}
+
+// Cholesky solve (FP64) of the k x k system in the leading block of G
+// (leading dimension ld) with right-hand side in column k, both multiplied by
+// `inv`.  Returns false, with beta = 0, when a pivot is not positive.
+// Single lane; L (kMaxGram^2) and y (kMaxGram) are caller scratch (LDS), so
+// the kernels need no private scratch memory.
+__device__ bool cholesky_rhs(const double* G, int ld, int k, double inv, double* beta, double* L,
+                             double* y) {
+  for (int i = 0; i < k; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double s = G[i * ld + j] * inv;
+      for (int q = 0; q < j; ++q) s -= L[i * kMaxGram + q] * L[j * kMaxGram + q];
+      if (i == j) {
+        if (!(s > 0.0)) {
+          for (int q = 0; q < k; ++q) beta[q] = 0.0;
+          return false;
+        }
+        L[i * kMaxGram + i] = sqrt(s);
+      } else {
+        L[i * kMaxGram + j] = s / L[j * kMaxGram + j];
+      }
+    }
+  }
+  for (int i = 0; i < k; ++i) {
+    double s = G[i * ld + k] * inv;
+    for (int q = 0; q < i; ++q) s -= L[i * kMaxGram + q] * y[q];
+    y[i] = s / L[i * kMaxGram + i];
+  }
+  for (int i = k - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int q = i + 1; q < k; ++q) s -= L[q * kMaxGram + i] * beta[q];
+    beta[i] = s / L[i * kMaxGram + i];
+  }
+  return true;
+}
+
+// LS normal equations from a (k+1) x (k+1) Gram matrix whose row 0 is the
+// constant function (so G[0][0] is the path count) and row k the target:
+// mean-scaled as in GenerateLSRegressionMatrixSelectively_gold
+// (gold/LS_regression.h:431-519); beta = 0 when no path entered or the
+// system is not positive definite.
+__device__ void solve_normal_equations(const double* G, int k, double* beta, double* L,
+                                       double* y) {
+  const double cnt = G[0];
+  if (!(cnt > 0.0)) {
+    for (int q = 0; q < k; ++q) beta[q] = 0.0;
+    return;
+  }
+  cholesky_rhs(G, k + 1, k, 1.0 / cnt, beta, L, y);
+}
+
+// ---------------------------------------------------------------------------
+// Bermudan payer swaption path quantities.
+// ---------------------------------------------------------------------------
+struct ExMap {
+  signed char e_of_step[kMaxRates];  // exercise index of each step, -1 if none
+  int last;                          // last exercise step
+};
+
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_berm_paths(KOO_MODEL_PARAMS, const ExMap ex, float* __restrict__ bv, float* __restrict__ evo,
+                 float* __restrict__ numo) {
+  KOO_MODEL_LOCAL;
+  extern __shared__ float lds[];
+  const int bd = blockDim.x;
+  const int p = blockIdx.x * bd + threadIdx.x;
+  const bool valid = p < m.P;
+  const int pc = valid ? p : m.P - 1;
+  float* L = lds + threadIdx.x;
+  float* ctab = lds + static_cast<size_t>(m.N) * bd;
+  const size_t P = static_cast<size_t>(m.P);
+  fill_fixed_drifts<F>(m, ctab);
+  for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
+  __syncthreads();
+  float B = 1.0f;
+  for (int s = 0; s <= ex.last; ++s) {
+    float z[F], e[F], ep[F];
+    load_z<F>(m, s, pc, z);
+    pc_step<F>(m, s, ctab + s * m.N, z, e, ep, [&](int r) { return L[r * bd]; },
+               [&](int r, float v) { L[r * bd] = v; });
+    const float tau = m.taus[s];
+    const float fs = fast_exp(L[s * bd]) - m.disp[s];
+    const int ei = ex.e_of_step[s];
+    if (ei >= 0) {
+      // P_{r+1} = P_r / (1 + tau_r f_r) from P_s = 1; annuity sum tau_r P_{r+1};
+      // swap rate as the annuity-weighted forward average, which equals
+      // (1 - P_N) / annuity without its cancellation.
+      float disc = 1.0f, ann = 0.0f, flt = 0.0f;
+      for (int r = s; r < m.N; ++r) {
+        const float fr = fast_exp(L[r * bd]) - m.disp[r];
+        const float tr = m.taus[r];
+        disc *= fast_rcp(1.0f + tr * fr);
+        ann += tr * disc;
+        flt += tr * fr * disc;
+      }
+      const float sr = flt / ann;
+      const float v = (sr - m.strike) * ann;
+      if (valid) {
+        const size_t o = static_cast<size_t>(ei) * P + p;
+        bv[static_cast<size_t>(2 * ei) * P + p] = sr;
+        bv[static_cast<size_t>(2 * ei + 1) * P + p] = fs;
+        evo[o] = v > 0.0f ? v : 0.0f;
+        numo[o] = B;
+      }
+    }
+    B *= 1.0f + tau * fs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gram matrix of up to 16 path-minor rows (optionally masked), FP64 partials
+// per block into part[block][n*n].
+// ---------------------------------------------------------------------------
+struct RowSet {
+  const float* row[kMaxGram];
+};
+
+template <int GROUPS>
+__global__ void __launch_bounds__(kLsThreads)
+    k_gram(const RowSet rows, int n, const float* __restrict__ mask, int P,
+           double* __restrict__ part) {
+  __shared__ float lds_d[(kLsThreads / 64) * 256];
+  constexpr int kPathsPerOp = 4 * GROUPS;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
+  const int sub = GROUPS == 2 ? ((lane >> 3) & 1) * 4 + (lane >> 4) : (lane >> 4);
+  const float* __restrict__ src = rows.row[j < n ? j : 0];
+  floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  const int stride = gridDim.x * nw * 64;
+  for (int base = (blockIdx.x * nw + wave) * 64; base < P; base += stride) {
+#pragma unroll 4
+    for (int q = 0; q < 64 / kPathsPerOp; ++q) {
+      const int p = base + q * kPathsPerOp + sub;
+      float a = 0.0f;
+      if (j < n && p < P) {
+        a = src[p];
+        if (mask && mask[p] == 0.0f) a = 0.0f;
+      }
+      acc = mfma16x16x4(a, a, acc);
+    }
+  }
+  block_gram<GROUPS>(acc, n, lds_d, part + static_cast<size_t>(blockIdx.x) * n * n);
+}
+
+// Sums the partials of k_gram and writes the n x n result (FP64 and, when
+// asked, FP32 scaled by alpha plus beta times the old FP32 value) for the
+// (r0, c0, nr, nc) block of it, row-major with leading dimension ld.
+__global__ void __launch_bounds__(kLsThreads)
+    k_gram_finish(const double* __restrict__ part, int nb, int n, int r0, int c0, int nr, int nc,
+                  int ld, double* __restrict__ out64, float* __restrict__ out32, float alpha,
+                  float beta, int transpose) {
+  __shared__ double red[kLsThreads];
+  __shared__ double sum[kMaxGram * kMaxGram];
+  sum_block_partials(part, nb, n, red, sum);
+  for (int t = threadIdx.x; t < nr * nc; t += blockDim.x) {
+    const int i = t / nc, j = t - i * nc;
+    const double v = sum[(r0 + i) * n + c0 + j];
+    const int o = transpose ? j * ld + i : i * ld + j;
+    if (out64) out64[o] = v;
+    if (out32) out32[o] = alpha * static_cast<float>(v) + (beta != 0.0f ? beta * out32[o] : 0.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused LS first pass: one launch per exercise date, last date first.
+// ---------------------------------------------------------------------------
+struct DateArgs {
+  int e, E, P, max_vars, maxd;
+  int v, k;    // variables and basis functions at date e
+  int vn, kn;  // at date e + 1 (when e + 1 < E)
+  float sh[kMaxBasisVars], sc[kMaxBasisVars];    // x~ = (x - sh) * sc at date e
+  float shn[kMaxBasisVars], scn[kMaxBasisVars];  // at date e + 1
+  int cross, itm_only;
+};
+
+// Continuation value at a date for one path: basis of the normalised
+// variables dotted with that date's coefficients (FP64 in HBM, FP32 here).
+__device__ __forceinline__ float continuation(const float* __restrict__ bv, size_t row0, size_t P,
+                                              int p, int v, int k, bool cross, const float* sh,
+                                              const float* sc, const double* __restrict__ beta) {
+  float x[kMaxBasisVars] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < kMaxBasisVars; ++i)
+    if (i < v) x[i] = (bv[(row0 + i) * P + p] - sh[i]) * sc[i];
+  float c = 0.0f;
+  for (int j = 0; j < k; ++j)
+    c += basis_value(j, v, cross, x[0], x[1], x[2], x[3]) * static_cast<float>(beta[j]);
+  return c;
+}
+
+template <int GROUPS>
+__global__ void __launch_bounds__(kLsThreads)
+    k_ls_date(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
+              const float* __restrict__ ev, const float* __restrict__ num,
+              double* __restrict__ coeffs, float* __restrict__ Y, double* __restrict__ part,
+              int* __restrict__ ticket) {
+  __shared__ float lds_d[(kLsThreads / 64) * 256];
+  __shared__ double red[kLsThreads];
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double chol[kMaxGram * kMaxGram + 2 * kMaxGram];
+  __shared__ int is_last;
+  constexpr int kPathsPerOp = 4 * GROUPS;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const size_t P = static_cast<size_t>(a.P);
+  const bool cross = a.cross != 0, itm = a.itm_only != 0;
+  const bool later = a.e + 1 < a.E;
+  const int n = a.k + 1;  // basis functions + target
+  const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
+  const int sub = GROUPS == 2 ? ((lane >> 3) & 1) * 4 + (lane >> 4) : (lane >> 4);
+  const double* __restrict__ beta_n = coeffs + static_cast<size_t>(a.e + 1) * a.maxd;
+  floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  const int stride = gridDim.x * nw * 64;
+  for (int base = (blockIdx.x * nw + wave) * 64; base < a.P; base += stride) {
+    const int p = base + lane;
+    float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f, x3 = 0.0f, t = 0.0f, w = 0.0f;
+    if (p < a.P) {
+      float y = 0.0f;
+      if (later) {
+        // exercise decision of date e+1 and deflation to date e
+        // (updateDeflatedNextStepValuesMulti_gold, gold/MultiLS_regression.h:46-115;
+        // updateDeflationOfContinuationValues_gold, gold/LS_regression.cpp:98-115)
+        const size_t on = static_cast<size_t>(a.e + 1) * P + p;
+        y = Y[p];
+        const float cont = continuation(bv, static_cast<size_t>(a.e + 1) * a.max_vars, P, p, a.vn,
+                                        a.kn, cross, a.shn, a.scn, beta_n);
+        const float exv = ev[on];
+        const bool allowed = !itm || exv > 0.0f;
+        if (allowed && exv > cont)
+          y = exv;
+        else if (cash)
+          y += cash[on];
+        y *= num[static_cast<size_t>(a.e) * P + p] / num[on];
+      }
+      Y[p] = y;
+      const size_t oe = static_cast<size_t>(a.e) * P + p;
+      w = (!itm || ev[oe] > 0.0f) ? 1.0f : 0.0f;
+      t = (cash ? cash[oe] : 0.0f) + y;
+      const size_t r0 = static_cast<size_t>(a.e) * a.max_vars;
+      if (a.v > 0) x0 = (bv[r0 * P + p] - a.sh[0]) * a.sc[0];
+      if (a.v > 1) x1 = (bv[(r0 + 1) * P + p] - a.sh[1]) * a.sc[1];
+      if (a.v > 2) x2 = (bv[(r0 + 2) * P + p] - a.sh[2]) * a.sc[2];
+      if (a.v > 3) x3 = (bv[(r0 + 3) * P + p] - a.sh[3]) * a.sc[3];
+    }
+    // 64 paths of this wave -> 64 / (4 GROUPS) MFMAs; lane l takes row j of path `sub`.
+#pragma unroll 2
+    for (int q = 0; q < 64 / kPathsPerOp; ++q) {
+      const int srcl = q * kPathsPerOp + sub;
+      const float s0 = __shfl(x0, srcl, 64), s1 = __shfl(x1, srcl, 64);
+      const float s2 = __shfl(x2, srcl, 64), s3 = __shfl(x3, srcl, 64);
+      const float st = __shfl(t, srcl, 64), sw = __shfl(w, srcl, 64);
+      float v = j < a.k ? basis_value(j, a.v, cross, s0, s1, s2, s3) : (j == a.k ? st : 0.0f);
+      v *= sw;
+      acc = mfma16x16x4(v, v, acc);
+    }
+  }
+  block_gram<GROUPS>(acc, n, lds_d, part + static_cast<size_t>(blockIdx.x) * n * n);
+  // Last block to finish: FP64 sum of the partials and Cholesky solve.
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1) == static_cast<int>(gridDim.x) - 1;
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  sum_block_partials(part, gridDim.x, n, red, gram);
+  if (threadIdx.x == 0) {
+    double* beta = chol + kMaxGram * kMaxGram + kMaxGram;
+    solve_normal_equations(gram, a.k, beta, chol, chol + kMaxGram * kMaxGram);
+    double* out = coeffs + static_cast<size_t>(a.e) * a.maxd;
+    for (int i = 0; i < a.maxd; ++i) out[i] = i < a.k ? beta[i] : 0.0;
+    *ticket = 0;  // ready for the next date
+  }
+}
+
+// Date 0's exercise decision, then sum_p Y_p / B_0 into block partials.
+__global__ void __launch_bounds__(kLsThreads)
+    k_ls_final(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
+               const float* __restrict__ ev, const float* __restrict__ num,
+               const double* __restrict__ coeffs, const float* __restrict__ Y,
+               double* __restrict__ part) {
+  __shared__ double red[4];
+  const size_t P = static_cast<size_t>(a.P);
+  double s = 0.0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
+    float y = Y[p];
+    const float cont = continuation(bv, 0, P, p, a.v, a.k, a.cross != 0, a.sh, a.sc, coeffs);
+    const float exv = ev[p];
+    const bool allowed = !a.itm_only || exv > 0.0f;
+    if (allowed && exv > cont)
+      y = exv;
+    else if (cash)
+      y += cash[p];
+    s += static_cast<double>(y) / static_cast<double>(num[p]);
+  }
+  block_sum_to(s, red, part + blockIdx.x);
+}
+
+// Frozen-strategy pass (SecondPassPriceLSAMultiUsingAggregatedFlows_gold,
+// gold/MultiLS_regression.h:237-289 at depth 0): walk the dates, add the
+// deflated cash flow until the strategy exercises, then the deflated exercise
+// value.  The per-date variables and shifts travel as a by-value kernel
+// argument (about 3 KB, within the 4 KB argument limit).
+struct AllDates {
+  int E, P, max_vars, maxd, cross, itm_only;
+  signed char v[kMaxRates], k[kMaxRates];
+  float sh[kMaxRates][kMaxBasisVars], sc[kMaxRates][kMaxBasisVars];
+};
+
+__global__ void __launch_bounds__(kLsThreads)
+    k_ls_second(const AllDates dv, const float* __restrict__ bv,
+                const float* __restrict__ cash, const float* __restrict__ ev,
+                const float* __restrict__ num, const double* __restrict__ coeffs,
+                double* __restrict__ part) {
+  __shared__ double red[4];
+  const AllDates* d = &dv;
+  const size_t P = static_cast<size_t>(d->P);
+  double s = 0.0;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < d->P; p += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int e = 0; e < d->E; ++e) {
+      const size_t o = static_cast<size_t>(e) * P + p;
+      const float cont = continuation(bv, static_cast<size_t>(e) * d->max_vars, P, p, d->v[e],
+                                      d->k[e], d->cross != 0, d->sh[e], d->sc[e],
+                                      coeffs + static_cast<size_t>(e) * d->maxd);
+      const float exv = ev[o];
+      const bool allowed = !d->itm_only || exv > 0.0f;
+      if (allowed && cont - exv < 0.0f) {
+        acc += static_cast<double>(exv) / static_cast<double>(num[o]);
+        break;
+      }
+      if (cash) acc += static_cast<double>(cash[o]) / static_cast<double>(num[o]);
+    }
+    s += acc;
+  }
+  block_sum_to(s, red, part + blockIdx.x);
+}
+
+__global__ void k_zero_int(int* x) { *x = 0; }
+
+// ---------------------------------------------------------------------------
+// The reference's small LS kernels.
+// ---------------------------------------------------------------------------
+
+// basis_function_kernel with basisVariableExample / basisVariableLog_gpu
+// (LS_Basis_gpu.h:24-94, LS_basis_examples_gpu.h:28-147): three variables per
+// exercise date, (E, 3, totalPaths).
+__global__ void k_basis_variables(int use_log, const float* __restrict__ shifts,
+                                  const float* __restrict__ r1, const float* __restrict__ r2,
+                                  const float* __restrict__ r3, int batch, int total, int offset,
+                                  int n_ex, float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= batch) return;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  if (use_log) {
+    s0 = shifts[0];
+    s1 = shifts[1];
+    s2 = shifts[2];
+  }
+  for (int e = 0; e < n_ex; ++e) {
+    const size_t in = static_cast<size_t>(e) * batch + p;
+    float a = r1[in], b = r2[in], c = r3[in];
+    if (use_log) {
+      a = logf(a + s0);
+      b = logf(b + s1);
+      c = logf(c + s2);
+    }
+    float* o = out + static_cast<size_t>(e) * 3 * total + offset + p;
+    o[0] = a;
+    o[static_cast<size_t>(total)] = b;
+    o[2 * static_cast<size_t>(total)] = c;
+  }
+}
+
+// exercise_value_kernel with earlyExerciseBermudanPayerSwaption
+// (early_exercise_value_generator_gpu.h:38-95, exercise_values_examples.h:49-78).
+__global__ void k_exercise_values_bermudan(const float* __restrict__ aux, const float* __restrict__ r1,
+                                           const float* __restrict__ r2, int batch, int total,
+                                           int offset, int n_ex, float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= batch) return;
+  const float K = aux[0];
+  for (int e = 0; e < n_ex; ++e) {
+    const size_t in = static_cast<size_t>(e) * batch + p;
+    const float v = (r1[in] - K) * r2[in];
+    out[static_cast<size_t>(e) * total + offset + p] = v > 0.0f ? v : 0.0f;
+  }
+}
+
+// basis_function_expansion_kernel (LS_gpu.h:232-272): (v, P) -> (k, P).
+__global__ void k_expand_basis(const float* __restrict__ in, int P, int v, int cross,
+                               float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const size_t sP = static_cast<size_t>(P);
+  float x[kMaxBasisVars] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < v; ++i) x[i] = in[i * sP + p];
+  const int k = basis_count(v, cross != 0);
+  for (int j = 0; j < k; ++j)
+    out[j * sP + p] = basis_value(j, v, cross != 0, x[0], x[1], x[2], x[3]);
+}
+
+// continuation_net_value_evaluation_kernel (LS_gpu.cu:56-85).
+__global__ void k_net_continuation(int P, int k, const float* __restrict__ w,
+                                   const float* __restrict__ bf, const float* __restrict__ ev,
+                                   float shift, float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const size_t sP = static_cast<size_t>(P);
+  float c = shift;
+  for (int i = 0; i < k; ++i) c += w[i] * bf[i * sP + p];
+  out[p] = c - ev[p];
+}
+
+// exercise_updating_LS_multi_kernel (LS_gpu.cu:312-400): nested regressions
+// with cut bands, exercise when the net continuation value is negative.
+__global__ void k_update_multi(int P, int k, int row, int depth_max, const float* __restrict__ w,
+                               const float* __restrict__ lo, const float* __restrict__ hi,
+                               const float* __restrict__ bf, float* __restrict__ cont_values,
+                               const float* __restrict__ ev, float shift,
+                               float* __restrict__ est, int* __restrict__ exercises) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const size_t sP = static_cast<size_t>(P);
+  const float exv = ev[p];
+  float c = 0.0f;
+  bool exercise = false;
+  for (int depth = 0;; ++depth) {
+    const float* wd = w + static_cast<size_t>(depth) * row;
+    c = shift;
+    for (int i = 0; i < k; ++i) c += wd[i] * bf[i * sP + p];
+    const float net = c - exv;
+    if (depth == depth_max - 1 || net < lo[depth] || net > hi[depth]) {
+      exercise = net < 0.0f;
+      break;
+    }
+  }
+  if (exercise) cont_values[p] = exv;
+  exercises[p] = exercise ? 1 : -1;
+  est[p] = c;
+}
+
+// numeraire_deflating_kernel (LS_gpu.cu:246-262).
+__global__ void k_deflate(float* __restrict__ y, const float* __restrict__ num_new,
+                          const float* __restrict__ num_old, int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  y[p] *= num_new[p] / num_old[p];
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers.
+// ---------------------------------------------------------------------------
+int berm_threads(int N) { return N <= 50 ? 256 : (N <= 75 ? 128 : 64); }
+
+int ls_blocks(int paths) {
+  const int want = (paths + kLsThreads - 1) / kLsThreads;
+  const int cap = 2 * device_cu_count();
+  return want < 1 ? 1 : (want > cap ? cap : want);
+}
+
+size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+struct LsLayout {
+  size_t y, part, ticket, total;
+};
+
+LsLayout ls_layout(int paths) {
+  LsLayout l;
+  l.y = 0;
+  l.part = align256(sizeof(float) * static_cast<size_t>(paths));
+  l.ticket = l.part + align256(sizeof(double) * kMaxGram * kMaxGram * ls_blocks(paths) + 8);
+  l.total = l.ticket + 256;
+  return l;
+}
+
+int check_ls(const char* who, int n_ex, const int* ves, int max_vars, int paths, int use_cross) {
+  if (n_ex < 1 || n_ex > kMaxRates || paths < 1 || max_vars < 0 || max_vars > kMaxBasisVars ||
+      !ves)
+    return set_error(KOO_ERR_INVALID,
+                     "%s: need 1<=exercise dates<=%d, paths>=1, 0<=max_vars<=%d (got %d, %d, %d)",
+                     who, kMaxRates, kMaxBasisVars, n_ex, paths, max_vars);
+  for (int e = 0; e < n_ex; ++e)
+    if (ves[e] < 0 || ves[e] > max_vars)
+      return set_error(KOO_ERR_INVALID, "%s: vars_each_step[%d] = %d outside [0, %d]", who, e,
+                       ves[e], max_vars);
+  if (basis_count(max_vars, use_cross != 0) + 1 > kMaxGram)
+    return set_error(KOO_ERR_INVALID, "%s: %d basis functions exceed %d", who,
+                     basis_count(max_vars, use_cross != 0), kMaxGram - 1);
+  return KOO_OK;
+}
+
+void fill_norm(const float* shift, const float* scale, int e, int v, int max_vars, float* sh,
+               float* sc) {
+  for (int i = 0; i < kMaxBasisVars; ++i) {
+    sh[i] = (i < v && shift) ? shift[e * max_vars + i] : 0.0f;
+    sc[i] = (i < v && scale) ? scale[e * max_vars + i] : 1.0f;
+  }
+}
+
+// Gram matrix of `n` rows into the workspace partials, then the requested
+// block of it; returns the status.
+int gram_launch(const char* who, const float* const* rows, int n, const float* mask, int paths,
+                double* part, int nb, int r0, int c0, int nr, int nc, int ld, double* out64,
+                float* out32, float alpha, float beta, int transpose, hipStream_t st) {
+  RowSet rs{};
+  for (int i = 0; i < n; ++i) rs.row[i] = rows[i];
+  for (int i = n; i < kMaxGram; ++i) rs.row[i] = rows[0];
+  if (n <= 8)
+    hipLaunchKernelGGL(k_gram<2>, dim3(nb), dim3(kLsThreads), 0, st, rs, n, mask, paths, part);
+  else
+    hipLaunchKernelGGL(k_gram<1>, dim3(nb), dim3(kLsThreads), 0, st, rs, n, mask, paths, part);
+  hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(kLsThreads), 0, st, part, nb, n, r0, c0, nr, nc,
+                     ld, out64, out32, alpha, beta, transpose);
+  return check_launch(who);
+}
+
+}  // namespace
+}  // namespace koo
+
+using namespace koo;
+
+// ---------------------------------------------------------------------------
+// Fused Bermudan path quantities.
+// ---------------------------------------------------------------------------
+extern "C" int bermudan_paths_gpu(int paths, int rates, int factors, const float* variates,
+                                  const float* initial_rates, const float* taus,
+                                  const float* displacements, const float* pseudo_roots,
+                                  int n_ex, const int* exercise_steps, float strike,
+                                  float* basis_variables, float* exercise_values,
+                                  float* numeraires, void* stream) {
+  if (paths < 1 || rates < 1 || rates > kMaxRates || factors < 1 || factors > kMaxFactors ||
+      n_ex < 1 || n_ex > rates)
+    return set_error(KOO_ERR_INVALID,
+                     "bermudan_paths_gpu: need paths>=1, 1<=rates<=%d, 1<=factors<=%d, "
+                     "1<=exercise dates<=rates (got %d, %d, %d, %d)",
+                     kMaxRates, kMaxFactors, paths, rates, factors, n_ex);
+  if (!variates || !initial_rates || !taus || !displacements || !pseudo_roots || !exercise_steps ||
+      !basis_variables || !exercise_values || !numeraires)
+    return set_error(KOO_ERR_INVALID, "bermudan_paths_gpu: null pointer");
+  ExMap ex{};
+  for (int s = 0; s < kMaxRates; ++s) ex.e_of_step[s] = -1;
+  for (int e = 0; e < n_ex; ++e) {
+    const int s = exercise_steps[e];
+    if (s < 0 || s >= rates || (e > 0 && s <= exercise_steps[e - 1]))
+      return set_error(KOO_ERR_INVALID,
+                       "bermudan_paths_gpu: exercise steps must increase within [0, %d)", rates);
+    ex.e_of_step[s] = static_cast<signed char>(e);
+  }
+  ex.last = exercise_steps[n_ex - 1];
+  const int threads = berm_threads(rates);
+  const int blocks = (paths + threads - 1) / threads;
+  const size_t lds = sizeof(float) * (static_cast<size_t>(rates) * threads + rates * rates);
+  hipStream_t st = as_stream(stream);
+  switch (factors) {
+#define KOO_CASE(F)                                                                             \
+  case F:                                                                                       \
+    hipLaunchKernelGGL(k_berm_paths<F>, dim3(blocks), dim3(threads), lds, st, variates,         \
+                       initial_rates, taus, displacements, pseudo_roots, strike, paths, rates, 1, \
+                       ex, basis_variables, exercise_values, numeraires);                       \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
+    KOO_CASE(8)
+#undef KOO_CASE
+  }
+  return check_launch("bermudan_paths_gpu");
+}
+
+// ---------------------------------------------------------------------------
+// Fused LS first pass, frozen-strategy pass, normal equations, Cholesky.
+// ---------------------------------------------------------------------------
+extern "C" int ls_workspace(int paths, size_t* bytes) {
+  if (paths < 1) return set_error(KOO_ERR_INVALID, "ls_workspace: paths must be >= 1");
+  if (bytes) *bytes = ls_layout(paths).total;
+  return KOO_OK;
+}
+
+extern "C" int ls_first_pass_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,
+                                 int max_vars, const float* cash_flows, const float* exercise_values,
+                                 const float* numeraires, int paths, int use_cross, int itm_only,
+                                 const float* var_shift, const float* var_scale, void* workspace,
+                                 double* coefficients, double* price_out, void* stream) {
+  if (int e = check_ls("ls_first_pass_gpu", n_ex, vars_each_step, max_vars, paths, use_cross))
+    return e;
+  if (!basis_variables || !exercise_values || !numeraires || !workspace || !coefficients ||
+      !price_out)
+    return set_error(KOO_ERR_INVALID, "ls_first_pass_gpu: null pointer");
+  const LsLayout lay = ls_layout(paths);
+  char* ws = static_cast<char*>(workspace);
+  float* Y = reinterpret_cast<float*>(ws + lay.y);
+  double* part = reinterpret_cast<double*>(ws + lay.part);
+  int* ticket = reinterpret_cast<int*>(ws + lay.ticket);
+  const int nb = ls_blocks(paths);
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, ticket);
+  const bool cross = use_cross != 0;
+  DateArgs a{};
+  a.E = n_ex;
+  a.P = paths;
+  a.max_vars = max_vars;
+  a.maxd = basis_count(max_vars, cross);
+  a.cross = use_cross != 0;
+  a.itm_only = itm_only != 0;
+  for (int e = n_ex - 1; e >= 0; --e) {
+    a.e = e;
+    a.v = vars_each_step[e];
+    a.k = basis_count(a.v, cross);
+    fill_norm(var_shift, var_scale, e, a.v, max_vars, a.sh, a.sc);
+    if (e + 1 < n_ex) {
+      a.vn = vars_each_step[e + 1];
+      a.kn = basis_count(a.vn, cross);
+      fill_norm(var_shift, var_scale, e + 1, a.vn, max_vars, a.shn, a.scn);
+    } else {
+      a.vn = a.kn = 0;
+    }
+    if (a.k + 1 <= 8)
+      hipLaunchKernelGGL(k_ls_date<2>, dim3(nb), dim3(kLsThreads), 0, st, a, basis_variables,
+                         cash_flows, exercise_values, numeraires, coefficients, Y, part, ticket);
+    else
+      hipLaunchKernelGGL(k_ls_date<1>, dim3(nb), dim3(kLsThreads), 0, st, a, basis_variables,
+                         cash_flows, exercise_values, numeraires, coefficients, Y, part, ticket);
+  }
+  a.e = 0;
+  a.v = vars_each_step[0];
+  a.k = basis_count(a.v, cross);
+  fill_norm(var_shift, var_scale, 0, a.v, max_vars, a.sh, a.sc);
+  hipLaunchKernelGGL(k_ls_final, dim3(nb), dim3(kLsThreads), 0, st, a, basis_variables, cash_flows,
+                     exercise_values, numeraires, coefficients, Y, part);
+  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, part, nb, 1,
+                     1.0 / static_cast<double>(paths), price_out);
+  return check_launch("ls_first_pass_gpu");
+}
+
+extern "C" int ls_second_pass_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,
+                                  int max_vars, const float* cash_flows,
+                                  const float* exercise_values, const float* numeraires, int paths,
+                                  int use_cross, int itm_only, const float* var_shift,
+                                  const float* var_scale, const double* coefficients,
+                                  void* workspace, double* price_out, void* stream) {
+  if (int e = check_ls("ls_second_pass_gpu", n_ex, vars_each_step, max_vars, paths, use_cross))
+    return e;
+  if (!basis_variables || !exercise_values || !numeraires || !workspace || !coefficients ||
+      !price_out)
+    return set_error(KOO_ERR_INVALID, "ls_second_pass_gpu: null pointer");
+  const LsLayout lay = ls_layout(paths);
+  double* part = reinterpret_cast<double*>(static_cast<char*>(workspace) + lay.part);
+  AllDates d{};
+  d.E = n_ex;
+  d.P = paths;
+  d.max_vars = max_vars;
+  d.maxd = basis_count(max_vars, use_cross != 0);
+  d.cross = use_cross != 0;
+  d.itm_only = itm_only != 0;
+  for (int e = 0; e < n_ex; ++e) {
+    d.v[e] = static_cast<signed char>(vars_each_step[e]);
+    d.k[e] = static_cast<signed char>(basis_count(vars_each_step[e], use_cross != 0));
+    fill_norm(var_shift, var_scale, e, vars_each_step[e], max_vars, d.sh[e], d.sc[e]);
+  }
+  hipStream_t st = as_stream(stream);
+  const int nb = ls_blocks(paths);
+  hipLaunchKernelGGL(k_ls_second, dim3(nb), dim3(kLsThreads), 0, st, d, basis_variables,
+                     cash_flows, exercise_values, numeraires, coefficients, part);
+  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, part, nb, 1,
+                     1.0 / static_cast<double>(paths), price_out);
+  return check_launch("ls_second_pass_gpu");
+}
+
+extern "C" int ls_normal_eq_gpu(int paths, int n_basis, const float* basis_functions,
+                                const float* target, const float* mask, void* workspace,
+                                double* gram_out, void* stream) {
+  const int n = n_basis + (target ? 1 : 0);
+  if (paths < 1 || n_basis < 1 || n > kMaxGram)
+    return set_error(KOO_ERR_INVALID,
+                     "ls_normal_eq_gpu: need paths>=1 and 1<=rows<=%d (got %d, %d)", kMaxGram,
+                     paths, n);
+  if (!basis_functions || !workspace || !gram_out)
+    return set_error(KOO_ERR_INVALID, "ls_normal_eq_gpu: null pointer");
+  const float* rows[kMaxGram];
+  for (int i = 0; i < n_basis; ++i) rows[i] = basis_functions + static_cast<size_t>(i) * paths;
+  if (target) rows[n_basis] = target;
+  const LsLayout lay = ls_layout(paths);
+  double* part = reinterpret_cast<double*>(static_cast<char*>(workspace) + lay.part);
+  return gram_launch("ls_normal_eq_gpu", rows, n, mask, paths, part, ls_blocks(paths), 0, 0, n, n,
+                     n, gram_out, nullptr, 1.0f, 0.0f, 0, as_stream(stream));
+}
+
+namespace {
+__global__ void k_cholesky(int n, const double* __restrict__ A, const double* __restrict__ b,
+                           double* __restrict__ x, int* __restrict__ info) {
+  __shared__ double G[kMaxGram * kMaxGram];
+  __shared__ double S[kMaxGram * kMaxGram + 2 * kMaxGram];
+  const int m = n + 1;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) G[i * m + j] = A[i * n + j];
+    G[i * m + n] = b[i];
+  }
+  double* beta = S + kMaxGram * kMaxGram + kMaxGram;
+  const bool ok = cholesky_rhs(G, m, n, 1.0, beta, S, S + kMaxGram * kMaxGram);
+  for (int i = 0; i < n; ++i) x[i] = beta[i];
+  if (info) *info = ok ? 0 : 1;
+}
+}  // namespace
+
+extern "C" int cholesky_solve_f64_gpu(int n, const double* A, const double* b, double* x,
+                                      int* info, void* stream) {
+  if (n < 1 || n >= kMaxGram)
+    return set_error(KOO_ERR_INVALID, "cholesky_solve_f64_gpu: need 1<=n<%d (got %d)", kMaxGram,
+                     n);
+  if (!A || !b || !x) return set_error(KOO_ERR_INVALID, "cholesky_solve_f64_gpu: null pointer");
+  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1), 0, as_stream(stream), n, A, b, x, info);
+  return check_launch("cholesky_solve_f64_gpu");
+}
+
+// ---------------------------------------------------------------------------
+// The reference's LS launchers.
+// ---------------------------------------------------------------------------
+extern "C" int adjoinBasisVariablesCaller_gpu(int use_logs, const int* integer_data,
+                                              const float* float_data, const float* forwards,
+                                              const float* discount_ratios, const float* rate1,
+                                              const float* rate2, const float* rate3,
+                                              int paths_per_batch, int total_paths,
+                                              int paths_previously_done, int steps_and_rates,
+                                              int number_exercise_times,
+                                              const int* exercise_indices,
+                                              const int* exercise_indicators,
+                                              float* basis_variables, void* stream) {
+  (void)integer_data;
+  (void)forwards;
+  (void)discount_ratios;
+  (void)steps_and_rates;
+  (void)exercise_indices;
+  (void)exercise_indicators;
+  if (paths_per_batch < 1 || total_paths < paths_per_batch + paths_previously_done ||
+      paths_previously_done < 0 || number_exercise_times < 1)
+    return set_error(KOO_ERR_INVALID,
+                     "adjoinBasisVariablesCaller_gpu: bad sizes (batch %d, total %d, done %d, "
+                     "exercise times %d)",
+                     paths_per_batch, total_paths, paths_previously_done, number_exercise_times);
+  if (!rate1 || !rate2 || !rate3 || !basis_variables || (use_logs && !float_data))
+    return set_error(KOO_ERR_INVALID, "adjoinBasisVariablesCaller_gpu: null pointer");
+  hipLaunchKernelGGL(k_basis_variables, dim3(grid_for(paths_per_batch, 256)), dim3(256), 0,
+                     as_stream(stream), use_logs, float_data, rate1, rate2, rate3, paths_per_batch,
+                     total_paths, paths_previously_done, number_exercise_times, basis_variables);
+  return check_launch("adjoinBasisVariablesCaller_gpu");
+}
+
+extern "C" int adjoinExerciseValues_Bermudan_swaption_gpu(
+    float* exercise_values, const float* aux_float_data, const int* aux_int_data,
+    const int* exercise_indices, const int* exercise_indicators, int number_exercise_times,
+    int paths_for_output, int output_offset, int batch_paths, int number_steps_and_rates,
+    const float* rates1, const float* rates2, const float* rates3, const float* forwards,
+    const float* disc_ratios, void* stream) {
+  (void)aux_int_data;
+  (void)exercise_indices;
+  (void)exercise_indicators;
+  (void)number_steps_and_rates;
+  (void)rates3;
+  (void)forwards;
+  (void)disc_ratios;
+  if (batch_paths < 1 || output_offset < 0 || paths_for_output < batch_paths + output_offset ||
+      number_exercise_times < 1)
+    return set_error(KOO_ERR_INVALID,
+                     "adjoinExerciseValues_Bermudan_swaption_gpu: bad sizes (batch %d, output "
+                     "%d, offset %d, exercise times %d)",
+                     batch_paths, paths_for_output, output_offset, number_exercise_times);
+  if (!exercise_values || !aux_float_data || !rates1 || !rates2)
+    return set_error(KOO_ERR_INVALID, "adjoinExerciseValues_Bermudan_swaption_gpu: null pointer");
+  hipLaunchKernelGGL(k_exercise_values_bermudan, dim3(grid_for(batch_paths, 256)), dim3(256), 0,
+                     as_stream(stream), aux_float_data, rates1, rates2, batch_paths,
+                     paths_for_output, output_offset, number_exercise_times, exercise_values);
+  return check_launch("adjoinExerciseValues_Bermudan_swaption_gpu");
+}
+
+extern "C" int expandBasisFunctions_quadratic_gpu(int total_paths, const float* basis_variables,
+                                                  int exercise_number, int max_basis_variable_size,
+                                                  int variable_size_this_step, int use_cross_terms,
+                                                  float* basis_functions, void* stream) {
+  if (total_paths < 1 || exercise_number < 0 || variable_size_this_step < 0 ||
+      variable_size_this_step > max_basis_variable_size ||
+      variable_size_this_step > kMaxBasisVars)
+    return set_error(KOO_ERR_INVALID,
+                     "expandBasisFunctions_quadratic_gpu: need paths>=1, 0<=variables<=min(max "
+                     "size, %d) (got %d, %d, %d)",
+                     kMaxBasisVars, total_paths, variable_size_this_step, max_basis_variable_size);
+  if (!basis_variables || !basis_functions)
+    return set_error(KOO_ERR_INVALID, "expandBasisFunctions_quadratic_gpu: null pointer");
+  const float* in = basis_variables + static_cast<size_t>(total_paths) * max_basis_variable_size *
+                                          exercise_number;
+  hipLaunchKernelGGL(k_expand_basis, dim3(grid_for(total_paths, 256)), dim3(256), 0,
+                     as_stream(stream), in, total_paths, variable_size_this_step, use_cross_terms,
+                     basis_functions);
+  return check_launch("expandBasisFunctions_quadratic_gpu");
+}
+
+extern "C" int continuation_net_value_evaluation_gpu(int paths, int exercise_number,
+                                                     const float* basis_functions,
+                                                     int number_basis_functions,
+                                                     const float* basis_weights,
+                                                     const float* exercise_values,
+                                                     float andersen_shift, float* output,
+                                                     void* stream) {
+  (void)exercise_number;
+  if (paths < 1 || number_basis_functions < 1)
+    return set_error(KOO_ERR_INVALID, "continuation_net_value_evaluation_gpu: bad sizes");
+  if (!basis_functions || !basis_weights || !exercise_values || !output)
+    return set_error(KOO_ERR_INVALID, "continuation_net_value_evaluation_gpu: null pointer");
+  hipLaunchKernelGGL(k_net_continuation, dim3(grid_for(paths, 256)), dim3(256), 0,
+                     as_stream(stream), paths, number_basis_functions, basis_weights,
+                     basis_functions, exercise_values, andersen_shift, output);
+  return check_launch("continuation_net_value_evaluation_gpu");
+}
+
+extern "C" int oneStepUpdateLS_multi_gpu(int paths, int exercise_number,
+                                         const float* basis_functions, int number_basis_functions,
+                                         int basis_functions_row_size, int max_regression_depth,
+                                         const float* basis_weights, const float* lower_cuts,
+                                         const float* upper_cuts, float* continuation_values,
+                                         const float* deflated_cash_flows,
+                                         const float* exercise_values, float andersen_shift,
+                                         float* est_continuation_values, int* exercises,
+                                         void* stream) {
+  (void)deflated_cash_flows;
+  if (paths < 1 || exercise_number < 0 || number_basis_functions < 1 ||
+      basis_functions_row_size < number_basis_functions || max_regression_depth < 1)
+    return set_error(KOO_ERR_INVALID, "oneStepUpdateLS_multi_gpu: bad sizes");
+  if (!basis_functions || !basis_weights || !lower_cuts || !upper_cuts || !continuation_values ||
+      !exercise_values || !est_continuation_values || !exercises)
+    return set_error(KOO_ERR_INVALID, "oneStepUpdateLS_multi_gpu: null pointer");
+  const float* w = basis_weights + static_cast<size_t>(basis_functions_row_size) *
+                                       exercise_number * max_regression_depth;
+  const float* ev = exercise_values + static_cast<size_t>(paths) * exercise_number;
+  hipLaunchKernelGGL(k_update_multi, dim3(grid_for(paths, 256)), dim3(256), 0, as_stream(stream),
+                     paths, number_basis_functions, basis_functions_row_size,
+                     max_regression_depth, w, lower_cuts, upper_cuts, basis_functions,
+                     continuation_values, ev, andersen_shift, est_continuation_values, exercises);
+  return check_launch("oneStepUpdateLS_multi_gpu");
+}
+
+extern "C" int updateDeflationOfContinuationValues_gpu(float* deflated_next_step_values,
+                                                       const float* numeraire_values, int paths,
+                                                       int new_step_number, int old_step_number,
+                                                       void* stream) {
+  if (paths < 1 || new_step_number < 0 || old_step_number < 0)
+    return set_error(KOO_ERR_INVALID, "updateDeflationOfContinuationValues_gpu: bad sizes");
+  if (!deflated_next_step_values || !numeraire_values)
+    return set_error(KOO_ERR_INVALID, "updateDeflationOfContinuationValues_gpu: null pointer");
+  const size_t P = static_cast<size_t>(paths);
+  hipLaunchKernelGGL(k_deflate, dim3(grid_for(paths, 256)), dim3(256), 0, as_stream(stream),
+                     deflated_next_step_values, numeraire_values + new_step_number * P,
+                     numeraire_values + old_step_number * P, paths);
+  return check_launch("updateDeflationOfContinuationValues_gpu");
+}
+
+// ---------------------------------------------------------------------------
+// Outer products (outerProduct_gpu.h:12-92) on the MFMA Gram kernel: blocks of
+// at most 8 rows of each operand are stacked into one Gram matrix whose
+// off-diagonal block is the product.  The FP64 block partials live in the
+// caller's float workspace: the reference launchers keep the reference's
+// contract (at least blocks * row_size * col_size floats) and the number of
+// partial blocks is sized to fit it; the cuBLAS-named variants take a
+// workspace of outer_product_workspace_floats(paths) floats.
+// ---------------------------------------------------------------------------
+namespace {
+
+int outer_blocks(int paths) {
+  const int b = ls_blocks(paths);
+  return b > 256 ? 256 : b;
+}
+
+int outer_product(const char* who, int paths, int R, int C, bool symmetric, const float* in1,
+                  const float* in2, float* answer, float* workspace, size_t workspace_floats,
+                  float alpha, float beta, bool column_major, hipStream_t st) {
+  if (paths < 1 || R < 1 || C < 1)
+    return set_error(KOO_ERR_INVALID, "%s: need paths, rows, columns >= 1 (got %d, %d, %d)", who,
+                     paths, R, C);
+  if (!in1 || (!symmetric && !in2) || !answer || !workspace)
+    return set_error(KOO_ERR_INVALID, "%s: null pointer", who);
+  if (reinterpret_cast<uintptr_t>(workspace) % alignof(double) != 0)
+    return set_error(KOO_ERR_INVALID, "%s: workspace must be 8-byte aligned", who);
+  double* part = reinterpret_cast<double*>(workspace);
+  const size_t P = static_cast<size_t>(paths);
+  const float* rows[kMaxGram];
+  // answer(i, j) = sum_p in1[i][p] in2[j][p]; row-major [R][C] (reference
+  // kernels) or column-major with leading dimension R (the cuBLAS variants).
+  const int ld = column_major ? R : C;
+  for (int r0 = 0; r0 < R; r0 += 8) {
+    const int nr = R - r0 < 8 ? R - r0 : 8;
+    for (int c0 = 0; c0 < C; c0 += 8) {
+      const int nc = C - c0 < 8 ? C - c0 : 8;
+      for (int i = 0; i < nr; ++i) rows[i] = in1 + (r0 + i) * P;
+      const float* second = symmetric ? in1 : in2;
+      for (int j = 0; j < nc; ++j) rows[nr + j] = second + (c0 + j) * P;
+      float* out = answer + (column_major ? static_cast<size_t>(c0) * ld + r0
+                                          : static_cast<size_t>(r0) * ld + c0);
+      const int n = nr + nc;
+      const size_t fit = workspace_floats * sizeof(float) / (sizeof(double) * n * n);
+      const int nb = static_cast<int>(fit < static_cast<size_t>(outer_blocks(paths))
+                                          ? fit
+                                          : static_cast<size_t>(outer_blocks(paths)));
+      if (nb < 1)
+        return set_error(KOO_ERR_INVALID, "%s: workspace of %zu floats holds no %dx%d partial",
+                         who, workspace_floats, n, n);
+      if (int e = gram_launch(who, rows, n, nullptr, paths, part, nb, 0, nr, nr, nc, ld,
+                              nullptr, out, alpha, beta, column_major ? 1 : 0, st))
+        return e;
+    }
+  }
+  return KOO_OK;
+}
+
+}  // namespace
+
+extern "C" size_t outer_product_workspace_floats(int paths) {
+  return sizeof(double) / sizeof(float) * static_cast<size_t>(kMaxGram) * kMaxGram *
+         outer_blocks(paths < 1 ? 1 : paths);
+}
+
+namespace {
+size_t reference_workspace(int blocks, int R, int C) {
+  return static_cast<size_t>(blocks > 0 ? blocks : 1) * (R > 0 ? R : 1) * (C > 0 ? C : 1);
+}
+}  // namespace
+
+extern "C" int reduceOuterProduct_gpu(int paths, int row_size, int col_size, int symmetric,
+                                      int threads, int blocks, const float* input_data_global1,
+                                      const float* input_data_global2, float* answer_global,
+                                      float* workspace_data_global, void* stream) {
+  (void)threads;
+  const int C = symmetric ? row_size : col_size;
+  return outer_product("reduceOuterProduct_gpu", paths, row_size, C, symmetric != 0,
+                       input_data_global1, input_data_global2, answer_global,
+                       workspace_data_global, reference_workspace(blocks, row_size, C), 1.0f,
+                       0.0f, false, as_stream(stream));
+}
+
+extern "C" int reducedOuterProduct_gpu(int paths, int row_size, int col_size, int threads,
+                                       int blocks, const float* input_data_global1,
+                                       const float* input_data_global2, float* answer_global,
+                                       float* workspace_data_global, void* stream) {
+  (void)threads;
+  return outer_product("reducedOuterProduct_gpu", paths, row_size, col_size, false,
+                       input_data_global1, input_data_global2, answer_global,
+                       workspace_data_global, reference_workspace(blocks, row_size, col_size),
+                       1.0f, 0.0f, false, as_stream(stream));
+}
+
+extern "C" int reduceOuterProductSymmetric_gpu(int paths, int row_size, int threads, int blocks,
+                                               const float* input_data_global1,
+                                               float* answer_global, float* workspace_data_global,
+                                               void* stream) {
+  (void)threads;
+  return outer_product("reduceOuterProductSymmetric_gpu", paths, row_size, row_size, true,
+                       input_data_global1, nullptr, answer_global, workspace_data_global,
+                       reference_workspace(blocks, row_size, row_size), 1.0f, 0.0f, false,
+                       as_stream(stream));
+}
+
+extern "C" int reduceOuterProductSymmetric_cublas(int paths, int row_size, float alpha, float beta,
+                                                  const float* input_data_global,
+                                                  float* answer_global,
+                                                  float* workspace_data_global, void* stream) {
+  return outer_product("reduceOuterProductSymmetric_cublas", paths, row_size, row_size, true,
+                       input_data_global, nullptr, answer_global, workspace_data_global,
+                       outer_product_workspace_floats(paths), alpha, beta, true,
+                       as_stream(stream));
+}
+
+extern "C" int reduceOuterProduct_cublas(int paths, int row_size, int col_size, float alpha,
+                                         float beta, const float* input_data_global1,
+                                         const float* input_data_global2, float* answer_global,
+                                         float* workspace_data_global, void* stream) {
+  return outer_product("reduceOuterProduct_cublas", paths, row_size, col_size, false,
+                       input_data_global1, input_data_global2, answer_global,
+                       workspace_data_global, outer_product_workspace_floats(paths), alpha, beta,
+                       true, as_stream(stream));
+}

@@ -21,6 +21,7 @@ _u = ctypes.c_uint
 _ll = ctypes.c_longlong
 _d = ctypes.c_double
 _sz = ctypes.c_size_t
+_f = ctypes.c_float
 
 
 class KooError(RuntimeError):
@@ -76,6 +77,25 @@ _SIGNATURES = {
     "lmm_price_workspace": (_i, [_i, _i, _i, _i, _p, _p, _p]),
     "lmm_price_fwd": (_i, [_i, _i, _i, _i] + [_p] * 5 + [_i] + [_p] * 5),
     "lmm_price_bwd": (_i, [_i, _i, _i, _i] + [_p] * 6 + [_i] + [_p] * 6),
+    # stage (d)
+    "adjoinBasisVariablesCaller_gpu": (_i, [_i] + [_p] * 7 + [_i] * 5 + [_p] * 4),
+    "adjoinExerciseValues_Bermudan_swaption_gpu": (_i, [_p] * 5 + [_i] * 5 + [_p] * 6),
+    "expandBasisFunctions_quadratic_gpu": (_i, [_i, _p, _i, _i, _i, _i, _p, _p]),
+    "continuation_net_value_evaluation_gpu": (_i, [_i, _i, _p, _i, _p, _p, _f, _p, _p]),
+    "oneStepUpdateLS_multi_gpu": (_i, [_i, _i, _p, _i, _i, _i] + [_p] * 6 + [_f, _p, _p, _p]),
+    "updateDeflationOfContinuationValues_gpu": (_i, [_p, _p, _i, _i, _i, _p]),
+    "reduceOuterProduct_gpu": (_i, [_i, _i, _i, _i, _i, _i] + [_p] * 5),
+    "reducedOuterProduct_gpu": (_i, [_i, _i, _i, _i, _i] + [_p] * 5),
+    "reduceOuterProductSymmetric_gpu": (_i, [_i, _i, _i, _i] + [_p] * 4),
+    "outer_product_workspace_floats": (_sz, [_i]),
+    "reduceOuterProductSymmetric_cublas": (_i, [_i, _i, _f, _f] + [_p] * 4),
+    "reduceOuterProduct_cublas": (_i, [_i, _i, _i, _f, _f] + [_p] * 5),
+    "bermudan_paths_gpu": (_i, [_i, _i, _i] + [_p] * 5 + [_i, _p, _f] + [_p] * 4),
+    "ls_workspace": (_i, [_i, _p]),
+    "ls_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
+    "ls_second_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
+    "ls_normal_eq_gpu": (_i, [_i, _i] + [_p] * 6),
+    "cholesky_solve_f64_gpu": (_i, [_i] + [_p] * 5),
 }
 
 
@@ -379,3 +399,113 @@ def lmm_price_bwd(paths, rates, factors, k, variates, f0, taus, disp, A, vols, p
     call("lmm_price_bwd", paths, rates, factors, k, ptr(variates), ptr(f0), ptr(taus), ptr(disp),
          ptr(A), ptr(vols), product, aux.ctypes.data, ptr(checkpoints), ptr(scratch),
          ptr(partials), ptr(greeks_out), stream_handle(stream))
+
+
+# ---------------------------------------------------------------- stage (d)
+
+def _host_f32(x):
+    return None if x is None else np.ascontiguousarray(x, np.float32)
+
+
+def bermudan_paths_gpu(paths, rates, factors, variates, f0, taus, disp, A, exercise_steps, strike,
+                       basis_variables, exercise_values, numeraires, stream=None):
+    ex = _host_ints(exercise_steps)
+    call("bermudan_paths_gpu", paths, rates, factors, ptr(variates), ptr(f0), ptr(taus),
+         ptr(disp), ptr(A), len(ex), ex.ctypes.data, float(strike), ptr(basis_variables),
+         ptr(exercise_values), ptr(numeraires), stream_handle(stream))
+
+
+def ls_workspace(paths):
+    n = _sz()
+    call("ls_workspace", paths, ctypes.addressof(n))
+    return n.value
+
+
+def ls_first_pass_gpu(n_ex, basis_variables, vars_each_step, max_vars, cash, ev, num, paths,
+                      use_cross, itm_only, var_shift, var_scale, workspace, coefficients,
+                      price_out, stream=None):
+    ves = _host_ints(vars_each_step)
+    sh, sc = _host_f32(var_shift), _host_f32(var_scale)
+    call("ls_first_pass_gpu", n_ex, ptr(basis_variables), ves.ctypes.data, max_vars, ptr(cash),
+         ptr(ev), ptr(num), paths, int(use_cross), int(itm_only), ptr(sh), ptr(sc),
+         ptr(workspace), ptr(coefficients), ptr(price_out), stream_handle(stream))
+
+
+def ls_second_pass_gpu(n_ex, basis_variables, vars_each_step, max_vars, cash, ev, num, paths,
+                       use_cross, itm_only, var_shift, var_scale, coefficients, workspace,
+                       price_out, stream=None):
+    ves = _host_ints(vars_each_step)
+    sh, sc = _host_f32(var_shift), _host_f32(var_scale)
+    call("ls_second_pass_gpu", n_ex, ptr(basis_variables), ves.ctypes.data, max_vars, ptr(cash),
+         ptr(ev), ptr(num), paths, int(use_cross), int(itm_only), ptr(sh), ptr(sc),
+         ptr(coefficients), ptr(workspace), ptr(price_out), stream_handle(stream))
+
+
+def ls_normal_eq_gpu(paths, n_basis, basis_functions, target, mask, workspace, gram_out,
+                     stream=None):
+    call("ls_normal_eq_gpu", paths, n_basis, ptr(basis_functions), ptr(target), ptr(mask),
+         ptr(workspace), ptr(gram_out), stream_handle(stream))
+
+
+def cholesky_solve_f64_gpu(n, A, b, x, info=None, stream=None):
+    call("cholesky_solve_f64_gpu", n, ptr(A), ptr(b), ptr(x), ptr(info), stream_handle(stream))
+
+
+def adjoinBasisVariablesCaller_gpu(use_logs, float_data, rate1, rate2, rate3, paths_per_batch,
+                                   total_paths, paths_previously_done, n_ex, basis_variables,
+                                   stream=None):
+    call("adjoinBasisVariablesCaller_gpu", int(use_logs), None, ptr(float_data), None, None,
+         ptr(rate1), ptr(rate2), ptr(rate3), paths_per_batch, total_paths, paths_previously_done,
+         0, n_ex, None, None, ptr(basis_variables), stream_handle(stream))
+
+
+def adjoinExerciseValues_Bermudan_swaption_gpu(exercise_values, aux_float_data, n_ex,
+                                               paths_for_output, output_offset, batch_paths,
+                                               rates1, rates2, stream=None):
+    call("adjoinExerciseValues_Bermudan_swaption_gpu", ptr(exercise_values), ptr(aux_float_data),
+         None, None, None, n_ex, paths_for_output, output_offset, batch_paths, 0, ptr(rates1),
+         ptr(rates2), None, None, None, stream_handle(stream))
+
+
+def expandBasisFunctions_quadratic_gpu(total_paths, basis_variables, exercise_number, max_size,
+                                       n_vars, use_cross, basis_functions, stream=None):
+    call("expandBasisFunctions_quadratic_gpu", total_paths, ptr(basis_variables), exercise_number,
+         max_size, n_vars, int(use_cross), ptr(basis_functions), stream_handle(stream))
+
+
+def continuation_net_value_evaluation_gpu(paths, exercise_number, basis_functions, n_basis,
+                                          weights, exercise_values, shift, out, stream=None):
+    call("continuation_net_value_evaluation_gpu", paths, exercise_number, ptr(basis_functions),
+         n_basis, ptr(weights), ptr(exercise_values), float(shift), ptr(out),
+         stream_handle(stream))
+
+
+def oneStepUpdateLS_multi_gpu(paths, exercise_number, basis_functions, n_basis, row_size,
+                              max_depth, weights, lower_cuts, upper_cuts, continuation_values,
+                              cash, exercise_values, shift, est, exercises, stream=None):
+    call("oneStepUpdateLS_multi_gpu", paths, exercise_number, ptr(basis_functions), n_basis,
+         row_size, max_depth, ptr(weights), ptr(lower_cuts), ptr(upper_cuts),
+         ptr(continuation_values), ptr(cash), ptr(exercise_values), float(shift), ptr(est),
+         ptr(exercises), stream_handle(stream))
+
+
+def updateDeflationOfContinuationValues_gpu(values, numeraires, paths, new_step, old_step,
+                                            stream=None):
+    call("updateDeflationOfContinuationValues_gpu", ptr(values), ptr(numeraires), paths, new_step,
+         old_step, stream_handle(stream))
+
+
+def reduceOuterProduct_gpu(paths, row_size, col_size, symmetric, threads, blocks, in1, in2,
+                           answer, workspace, stream=None):
+    call("reduceOuterProduct_gpu", paths, row_size, col_size, int(symmetric), threads, blocks,
+         ptr(in1), ptr(in2), ptr(answer), ptr(workspace), stream_handle(stream))
+
+
+def reduceOuterProductSymmetric_cublas(paths, row_size, alpha, beta, inp, answer, workspace,
+                                       stream=None):
+    call("reduceOuterProductSymmetric_cublas", paths, row_size, float(alpha), float(beta),
+         ptr(inp), ptr(answer), ptr(workspace), stream_handle(stream))
+
+
+def outer_product_workspace_floats(paths):
+    return lib().outer_product_workspace_floats(paths)

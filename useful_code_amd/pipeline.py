@@ -4,6 +4,9 @@
   * ``generate``: fused Sobol -> scramble -> normal -> bridge (one kernel),
   * ``price``: the fused forward price kernel (no rate cube written),
   * ``price_and_greeks``: forward with checkpoints plus the reverse sweep.
+``BermudanPricer`` runs the Longstaff-Schwartz path (fused evolution to the
+exercise quantities, backward induction with MFMA normal equations and FP64
+Cholesky, and the frozen-strategy pass).
 ``lmm_evolution_routine`` mirrors the reference's staged pipeline
 (LMMEvolutionRoutineRawSingleKernel, LMM_evolver_full.cu:264-425) through the
 reference-named entry points.  Buffers are allocated once per pricer; calls
@@ -67,6 +70,71 @@ class LMMPricer:
         self.price_kernel(checkpoints=True)
         self.greeks_kernel()
         return self.price_out, self.greeks_out
+
+
+class BermudanPricer:
+    """Bermudan swaption by Longstaff-Schwartz on one GPU.
+
+    ``generate(offset)``: variates of Sobol points offset+1 ...; ``paths_kernel``:
+    basis variables, exercise values and numeraires per exercise date;
+    ``regress``: first pass (coefficients and in-sample price); ``second_pass``:
+    price under the frozen coefficients on the current paths."""
+
+    def __init__(self, w, device="cuda"):
+        self.w = w
+        P, N, F, S = w.paths, w.n_rates, w.factors, w.steps
+        E = len(w.exercise_steps)
+        self.E = E
+        self.dirs = _dev(koo.sobol_directions(w.dims).view(np.int32), torch.int32)
+        self.scr = _dev(np.asarray(w.scramblers, np.uint32).view(np.int32), torch.int32)
+        self.alloc = _dev(koo.bridge_allocate_diagonal(F, S), torch.int32)
+        self.f0 = _dev(w.f0, torch.float32)
+        self.taus = _dev(w.taus, torch.float32)
+        self.disp = _dev(w.disp(), torch.float32)
+        self.A = _dev(w.A, torch.float32)
+        self.z = torch.empty(S * F * P, dtype=torch.float32, device=device)
+        self.bv = torch.empty(E * 2 * P, dtype=torch.float32, device=device)
+        self.ev = torch.empty(E * P, dtype=torch.float32, device=device)
+        self.num = torch.empty(E * P, dtype=torch.float32, device=device)
+        self.ws = torch.empty(koo.ls_workspace(P), dtype=torch.uint8, device=device)
+        self.coeffs = torch.zeros(E * 6, dtype=torch.float64, device=device)
+        self.price1 = torch.zeros(1, dtype=torch.float64, device=device)
+        self.price2 = torch.zeros(1, dtype=torch.float64, device=device)
+        self.ves = np.full(E, 2, np.int32)
+        self.shift = np.asarray(w.basis_shift, np.float32)
+        self.scale = np.asarray(w.basis_scale, np.float32)
+
+    def generate(self, offset=None):
+        w = self.w
+        koo.sobol_bridge_gpu(w.paths, w.path_offset if offset is None else offset, w.steps,
+                             w.factors, self.dirs, self.scr, w.dims, self.alloc, self.z)
+
+    def paths_kernel(self):
+        w = self.w
+        koo.bermudan_paths_gpu(w.paths, w.n_rates, w.factors, self.z, self.f0, self.taus,
+                               self.disp, self.A, w.exercise_steps, w.strike, self.bv, self.ev,
+                               self.num)
+
+    def regress(self):
+        w = self.w
+        koo.ls_first_pass_gpu(self.E, self.bv, self.ves, 2, None, self.ev, self.num, w.paths, True,
+                              True, self.shift, self.scale, self.ws, self.coeffs, self.price1)
+
+    def second_pass(self):
+        w = self.w
+        koo.ls_second_pass_gpu(self.E, self.bv, self.ves, 2, None, self.ev, self.num, w.paths, True,
+                               True, self.shift, self.scale, self.coeffs, self.ws, self.price2)
+
+    def price(self):
+        """First-pass (in-sample) price, then the frozen-strategy price on the
+        next block of Sobol points; returns both as device tensors."""
+        self.generate()
+        self.paths_kernel()
+        self.regress()
+        self.generate(self.w.path_offset + self.w.paths)
+        self.paths_kernel()
+        self.second_pass()
+        return self.price1, self.price2
 
 
 def lmm_evolution_routine(w, use_fused_generator=False):
