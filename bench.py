@@ -12,7 +12,11 @@ adjoint ratio (job time / forward-only time), price and Greek errors against
 the FP64 oracle on the same Sobol points (on a 2^14-path prefix of the same
 workload), a roofline entry for the dominant kernel (the reverse sweep,
 VALU-bound, algorithmic flops / HIP-event duration), and the FP64 oracle timed
-on the host cores as a CPU baseline.
+on the host cores as a CPU baseline.  The ``bermudan_ls`` object reports the
+Longstaff-Schwartz workload (Bermudan payer swaption, 20 rates, 3 factors,
+2^18 paths, 10 exercise dates): stage times, its own roofline entry for the
+per-date regression kernel (HBM-bound) and its price error against the FP64
+oracle at full size.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -29,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LMM path-steps/s; adjoint all-Greeks time ÷ price time; error vs FP64"
 FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 
 
 def flops_per_rate_update(F):
@@ -41,6 +46,96 @@ def rate_updates_per_path(N):
     return N * (N + 1) // 2  # sum over steps s of alive rates N - s
 
 
+# Bytes one k_ls_date launch must move per path (FP32): this date's two basis
+# variables, exercise value and numeraire, the later date's two variables,
+# exercise value and numeraire for its exercise decision, the continuation
+# value read and written.
+LS_DATE_BYTES_PER_PATH = 4 * (2 + 1 + 1 + 2 + 1 + 1 + 1 + 1)
+
+
+def bermudan_section(args, torch, O, configs, pipeline, ev):
+    """Config-3 Longstaff-Schwartz timings, roofline and error."""
+    w = configs.bermudan_swaption(paths=args.bermudan_paths)
+    P, S, E = w.paths, w.steps, len(w.exercise_steps)
+    pr = pipeline.BermudanPricer(w)
+
+    def job(r=None):
+        if r is not None:
+            r[0].record()
+        pr.generate()
+        if r is not None:
+            r[1].record()
+        pr.paths_kernel()
+        if r is not None:
+            r[2].record()
+        pr.regress()
+        if r is not None:
+            r[3].record()
+        pr.second_pass()
+        if r is not None:
+            r[4].record()
+
+    for _ in range(args.warmup):
+        job()
+    recs = [tuple(ev() for _ in range(5)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in recs:
+        job(r)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    gen = np.mean([r[0].elapsed_time(r[1]) for r in recs])
+    paths = np.mean([r[1].elapsed_time(r[2]) for r in recs])
+    regress = np.mean([r[2].elapsed_time(r[3]) for r in recs])
+    second = np.mean([r[3].elapsed_time(r[4]) for r in recs])
+    price1, price2 = pr.price1.item(), pr.price2.item()
+    itm = float((pr.ev.view(E, P) > 0).float().mean().item())
+    # The whole first pass (E date kernels, E one-block solves, the final
+    # pass): algorithmic bytes of the E date kernels over its event time.
+    stage_bytes = E * LS_DATE_BYTES_PER_PATH * P
+    gbs = stage_bytes / (regress * 1e-3) / 1e9
+    k = 6
+    mfma_flops = E * 2.0 * itm * P * (k + 1) ** 2
+    res = {
+        "workload": "bermudan_payer_20x3: 2^18 Sobol paths, 20 rates = 20 steps, 3 factors, "
+                    "10 exercise dates (steps 0,2,..,18), ITM-only LS on 6 quadratic functions "
+                    "of (co-terminal swap rate, first alive forward)",
+        "paths": P,
+        "path_steps_per_s": P * S / ((gen + paths + regress) * 1e-3),
+        "job_ms": wall_ms,
+        "kernels_ms": {"sobol_bridge": gen, "bermudan_paths": paths, "ls_first_pass": regress,
+                       "ls_second_pass": second},
+        "price_first_pass": price1,
+        "price_second_pass_same_paths": price2,
+        "itm_fraction_mean": itm,
+        "roofline": {
+            "kernel": "LS first pass: per date k_ls_date (decision update, MFMA normal "
+                      "equations) + k_ls_solve (FP64 sum, Cholesky), then k_ls_final",
+            "bound": "hbm",
+            "achieved": gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": stage_bytes,
+            "mfma_useful_tflops": mfma_flops / (regress * 1e-3) / 1e12,
+        },
+    }
+    if not args.no_error_check:
+        ints = O.scramble(O.sobol_ints(P, w.dims, 0), w.scramblers, w.dims)
+        z = O.bridge_transform(O.invnorm_u32(ints), S, w.factors)
+        m = O.caplet_model(w.f0, w.taus, w.disp(), w.A, w.vols, w.strike, P)
+        t = time.perf_counter()
+        bv, evo, num = O.bermudan_paths(m, z, w.exercise_steps)
+        p64, _ = O.ls_first_pass(bv, np.full(E, 2, np.int32), np.zeros_like(evo), evo, num,
+                                 itm_only=True)
+        dt = time.perf_counter() - t
+        res["price_abs_err_vs_fp64"] = abs(price1 - p64)
+        res["price_fp64"] = p64
+        res["cpu_fp64_paths_and_ls_s"] = dt
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +146,10 @@ def main():
     ap.add_argument("--cpu-sample-paths", type=int, default=1 << 17)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
+    ap.add_argument("--no-bermudan", action="store_true")
+    ap.add_argument("--no-k1", action="store_true",
+                    help="skip the checkpoint-every-step comparison")
+    ap.add_argument("--bermudan-paths", type=int, default=1 << 18)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the benchmark, copy a buffer of this many MiB twice (device to "
                          "device): a plain copy of known size that calibrates HBM byte counters")
@@ -133,7 +232,7 @@ def main():
     # Same job with a checkpoint at every step (the forward also keeps each
     # step's drift sums, the reverse sweep recomputes nothing).
     k1 = {}
-    if w.checkpoint_every != 1:
+    if w.checkpoint_every != 1 and not args.no_k1:
         w1 = configs.caplet_portfolio(paths=args.paths, checkpoint_every=1)
         pr1 = pipeline.LMMPricer(w1)
         pr1.z = pr.z  # same variates
@@ -231,6 +330,9 @@ def main():
                       f"the same workload with all 80 adjoint Greeks on its first "
                       f"{wc.paths} paths: {dt:.2f} s",
         }
+
+    if not args.no_bermudan:
+        out["bermudan_ls"] = bermudan_section(args, torch, O, configs, pipeline, ev)
 
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4

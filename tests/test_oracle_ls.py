@@ -161,3 +161,18 @@ def test_ls_second_pass_under_first_pass_strategy_is_close_and_bounded():
     best_fixed = max(np.mean(ev[e] / num[e]) for e in range(E))
     assert best_fixed - 1e-12 <= price2 <= foresight
     assert abs(price1 - price2) < 0.02 * price2
+
+
+def test_ls_second_pass_on_the_regression_paths_equals_the_first_pass():
+    # With no cash flows the first pass's continuation value at date 0 is the
+    # deflated exercise value at the first date its own decisions exercise,
+    # which is exactly what the frozen-strategy pass collects on the same paths.
+    m, z, ex, _, _ = _small_bermudan(P=4096, seed=7)
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    E, P = ev.shape
+    ves = np.full(E, 2, np.int32)
+    cash = np.zeros((E, P))
+    for itm in (False, True):
+        p1, coeffs = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=itm)
+        p2 = O.ls_second_pass(bv, ves, cash, ev, num, coeffs, itm_only=itm)
+        assert abs(p1 - p2) <= 1e-14 * p1

@@ -142,11 +142,12 @@ __global__ void __launch_bounds__(kBridgeThreads)
 // Fused generator, steps <= kTableSteps: no per-point bit loop.  With
 // n = G + l, G a multiple of 64 and l < 64, the Gray code splits,
 // gray(n) = gray(G) ^ gray(l), so the Sobol integer is x(n) = X(G) ^ T(l).
-// A block's lanes hold consecutive n, which span at most blockDim/64 + 1
-// multiples of 64; per block and dimension the set-up builds X(G) for those
-// (scrambler folded in; each one a 32-lane XOR reduction over the direction
-// numbers) and the 64-entry table T(l) from the first six direction numbers.
-// A point then costs two LDS reads and one XOR.
+// A chunk of blockDim consecutive paths spans at most blockDim/64 + 1
+// multiples of 64.  Each block keeps the direction numbers of its S
+// dimensions and the 64-entry tables T(l) in LDS for its whole life and walks
+// chunks of paths; per chunk it builds X(G) for those groups (scrambler
+// folded in; each one a 32-lane XOR reduction over direction numbers read
+// from LDS).  A point then costs two LDS reads and one XOR.
 constexpr int kTableSteps = 64;
 constexpr int kMaxGroups = 8;
 
@@ -157,55 +158,60 @@ __global__ void __launch_bounds__(kBridgeThreads)
                        float* __restrict__ out) {
   extern __shared__ float lds[];
   const int bd = blockDim.x;
-  uint32_t* tlo = reinterpret_cast<uint32_t*>(lds + steps * bd);  // [steps][64]
-  uint32_t* xhi = tlo + steps * 64;                                // [steps][kMaxGroups]
-  uint32_t* vlo = xhi + steps * kMaxGroups;                        // [steps][6]
+  uint32_t* vall = reinterpret_cast<uint32_t*>(lds + steps * bd);  // [steps][32]
+  uint32_t* tlo = vall + steps * 32;                                 // [steps][64]
+  uint32_t* xhi = tlo + steps * 64;                                  // [steps][kMaxGroups]
+  uint32_t* sct = xhi + steps * kMaxGroups;                          // [steps]
   const int* idx = alloc + blockIdx.y * steps;
-  const uint32_t nb = static_cast<uint32_t>(blockIdx.x) * bd + off1;  // n of lane 0
-  const uint32_t g0 = nb & ~63u;
-  const int ngroups = bd / 64 + 1;
-
-  for (int w = threadIdx.x; w < steps * 6; w += bd)
-    vlo[w] = dirs[static_cast<size_t>(idx[w / 6]) * 32 + (w % 6)];
-  // X(G) for G = g0 + 64k: lane b of a half-wavefront contributes v[b] when
-  // bit b of gray(G) is set; the half XOR-reduces.  Every half runs the same
-  // number of rounds so the shuffles see all their lanes.
-  {
-    const int half = threadIdx.x >> 5, b = threadIdx.x & 31;
-    const int halves = bd >> 5, pairs = steps * ngroups;
-    for (int base = 0; base < pairs; base += halves) {
-      const int q = base + half;
-      const int qc = q < pairs ? q : pairs - 1;
-      const int i = qc / ngroups, k = qc % ngroups;
-      const uint32_t G = g0 + 64u * static_cast<uint32_t>(k);
-      const uint32_t g = G ^ (G >> 1);
-      uint32_t x = ((g >> b) & 1u) ? dirs[static_cast<size_t>(idx[i]) * 32 + b] : 0u;
-#pragma unroll
-      for (int o = 16; o > 0; o >>= 1) x ^= static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 32));
-      if (b == 0 && q < pairs) {
-        const int d = idx[i];
-        xhi[i * kMaxGroups + k] = x ^ (d < number_scrambled ? scr[d] : 0u);
-      }
-    }
+  for (int w = threadIdx.x; w < steps * 32; w += bd)
+    vall[w] = dirs[static_cast<size_t>(idx[w >> 5]) * 32 + (w & 31)];
+  for (int w = threadIdx.x; w < steps; w += bd) {
+    const int d = idx[w];
+    sct[w] = d < number_scrambled ? scr[d] : 0u;
   }
   __syncthreads();
   for (int w = threadIdx.x; w < steps * 64; w += bd) {
     const uint32_t l = static_cast<uint32_t>(w & 63), g = l ^ (l >> 1);
-    const uint32_t* vv = vlo + (w >> 6) * 6;
+    const uint32_t* vv = vall + (w >> 6) * 32;
     uint32_t x = 0;
 #pragma unroll
     for (int bit = 0; bit < 6; ++bit)
       if ((g >> bit) & 1u) x ^= vv[bit];
     tlo[w] = x;
   }
-  __syncthreads();
-  const int p = blockIdx.x * bd + threadIdx.x;
-  if (p >= paths) return;
-  const uint32_t n = nb + static_cast<uint32_t>(threadIdx.x);
-  const uint32_t kg = (n - g0) >> 6, l = n & 63u;
-  bridge_lane(T, steps, factors, paths, p, lds,
-              [&](int i) { return invnorm_u32(xhi[i * kMaxGroups + kg] ^ tlo[i * 64 + l]); },
-              out);
+  const int ngroups = bd / 64 + 1;
+  const int half = threadIdx.x >> 5, b = threadIdx.x & 31;
+  const int halves = bd >> 5, pairs = steps * ngroups;
+  const int nchunks = (paths + bd - 1) / bd;
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const uint32_t nb = static_cast<uint32_t>(c) * bd + off1;  // n of lane 0
+    const uint32_t g0 = nb & ~63u;
+    __syncthreads();  // the previous chunk's lanes are done with xhi
+    // X(G) for G = g0 + 64k: lane b of a half-wavefront contributes v[b] when
+    // bit b of gray(G) is set; the half XOR-reduces.  Every half runs the
+    // same number of rounds so the shuffles see all their lanes.
+    for (int base = 0; base < pairs; base += halves) {
+      const int q = base + half;
+      const int qc = q < pairs ? q : pairs - 1;
+      const int i = qc / ngroups, k = qc % ngroups;
+      const uint32_t G = g0 + 64u * static_cast<uint32_t>(k);
+      const uint32_t g = G ^ (G >> 1);
+      uint32_t x = ((g >> b) & 1u) ? vall[i * 32 + b] : 0u;
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1)
+        x ^= static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 32));
+      if (b == 0 && q < pairs) xhi[i * kMaxGroups + k] = x ^ sct[i];
+    }
+    __syncthreads();
+    const int p = c * bd + threadIdx.x;
+    if (p < paths) {
+      const uint32_t n = nb + static_cast<uint32_t>(threadIdx.x);
+      const uint32_t kg = (n - g0) >> 6, l = n & 63u;
+      bridge_lane(T, steps, factors, paths, p, lds,
+                  [&](int i) { return invnorm_u32(xhi[i * kMaxGroups + kg] ^ tlo[i * 64 + l]); },
+                  out);
+    }
+  }
 }
 
 }  // namespace
@@ -272,10 +278,18 @@ extern "C" int sobol_bridge_gpu(int paths, unsigned path_offset, int steps, int 
   if (!d_directions || !allocation_indices || !out || (number_scrambled > 0 && !scramblers))
     return set_error(KOO_ERR_INVALID, "sobol_bridge_gpu: null pointer");
   if (steps <= kTableSteps) {
-    const int threads = steps <= 48 ? 256 : 128;
+    // LDS: path storage steps x threads floats plus 105 words per step
+    // (direction numbers, T(l), X(G), scrambler) within 64 KiB.
+    const int threads = steps <= 40 ? 256 : 128;
     const size_t lds = sizeof(float) * steps * threads +
-                       sizeof(uint32_t) * steps * (64 + kMaxGroups + 6);
-    hipLaunchKernelGGL(k_sobol_bridge_tab, dim3((paths + threads - 1) / threads, factors),
+                       sizeof(uint32_t) * steps * (32 + 64 + kMaxGroups + 1);
+    // Enough blocks to fill the chip (about four per CU over all factors);
+    // each walks several chunks of paths with its tables kept in LDS.
+    const int chunks = (paths + threads - 1) / threads;
+    int per_factor = (4 * device_cu_count() + factors - 1) / factors;
+    if (per_factor > chunks) per_factor = chunks;
+    if (per_factor < 1) per_factor = 1;
+    hipLaunchKernelGGL(k_sobol_bridge_tab, dim3(per_factor, factors),
                        dim3(threads), lds, as_stream(stream), tables_for(steps), steps, factors,
                        paths, path_offset + 1u, d_directions, scramblers, number_scrambled,
                        allocation_indices, out);

@@ -97,6 +97,10 @@ __global__ void __launch_bounds__(256)
 // those into its own slots).
 // ----------------------------------------------------------------------
 constexpr int kVegaSlots = 8;
+// Depth of the queue of start log-rates read from checkpoints or scratch in
+// the backward kernel (HBM latency over the per-rate work of the resident
+// wavefronts).
+constexpr int kStatePrefetch = 8;
 
 template <int F>
 __global__ void __launch_bounds__(256)
@@ -141,7 +145,7 @@ __global__ void __launch_bounds__(256)
       float z[F], e[F], ep[F];
       float l_fix = 0.0f;
       load_z<F>(m, s, pc, z);
-      pc_step<F>(m, s, ctab + s * N, z, e, ep,
+      pc_step<F, kStatePrefetch>(m, s, ctab + s * N, z, e, ep,
                  [&](int r) { return src[static_cast<size_t>(r) * P]; },
                  [&](int r, float v) {
                    if (r == s) l_fix = v;
@@ -177,14 +181,20 @@ __global__ void __launch_bounds__(256)
       }
       const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
       // start log-rates two rates ahead, adjoints one rate ahead
-      float l_next = src[static_cast<size_t>(N - 1) * P];
-      float l_next2 = (N - 2 >= s) ? src[static_cast<size_t>(N - 2) * P] : 0.0f;
+      // start log-rates (HBM or L2) issued kStatePrefetch rates ahead,
+      // adjoints (LDS) one rate ahead
+      float q[kStatePrefetch];
+#pragma unroll
+      for (int i = 0; i < kStatePrefetch; ++i)
+        q[i] = (N - 1 - i >= s) ? src[static_cast<size_t>(N - 1 - i) * P] : 0.0f;
       float lp_next = lam[(N - 1) * bd];
       for (int r = N - 1; r >= s; --r) {
-        const float l = l_next;
+        const float l = q[0];
         float lp = lp_next;
-        l_next = l_next2;
-        if (r - 2 >= s) l_next2 = src[static_cast<size_t>(r - 2) * P];
+#pragma unroll
+        for (int i = 0; i + 1 < kStatePrefetch; ++i) q[i] = q[i + 1];
+        q[kStatePrefetch - 1] =
+            (r - kStatePrefetch >= s) ? src[static_cast<size_t>(r - kStatePrefetch) * P] : 0.0f;
         if (r > s) lp_next = lam[(r - 1) * bd];
         float a[F];
 #pragma unroll

@@ -45,6 +45,13 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
   return (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
 }
 
+// Error-free sum: s + e == a + b exactly (Knuth's TwoSum).
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -91,22 +98,25 @@ __device__ __forceinline__ void fill_fixed_drifts(const Model& m, float* ctab) {
 }
 
 // One predictor-corrector step s over rates r >= s.  `load(r)` gives the
-// start log-rate (issued one rate ahead), `store(r, l)` receives the end
-// log-rate; e / ep return the final drift sums; cs = fixed drifts of step s.
-template <int F, class Load, class Store>
+// start log-rate, issued PF rates ahead of its use (PF = 2 for LDS sources; a
+// deeper queue hides HBM latency when the state comes from global memory),
+// `store(r, l)` receives the end log-rate; e / ep return the final drift
+// sums; cs = fixed drifts of step s.
+template <int F, int PF = 2, class Load, class Store>
 __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
                                         const float (&z)[F], float (&e)[F], float (&ep)[F],
                                         Load load, Store store) {
   const float* As = m.A + s * m.N * F;  // 32-bit: keeps the address scalar
 #pragma unroll
   for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
-  // start log-rates are issued two rates ahead of their use
-  float l_next = load(s);
-  float l_next2 = (s + 1 < m.N) ? load(s + 1) : 0.0f;
+  float q[PF];
+#pragma unroll
+  for (int i = 0; i < PF; ++i) q[i] = (s + i < m.N) ? load(s + i) : 0.0f;
   for (int r = s; r < m.N; ++r) {
-    const float l = l_next;
-    l_next = l_next2;
-    if (r + 2 < m.N) l_next2 = load(r + 2);
+    const float l = q[0];
+#pragma unroll
+    for (int i = 0; i + 1 < PF; ++i) q[i] = q[i + 1];
+    q[PF - 1] = (r + PF < m.N) ? load(r + PF) : 0.0f;
     float a[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
@@ -152,8 +162,10 @@ __device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[
 __global__ void __launch_bounds__(256)
     k_finalize_cols(const double* __restrict__ part, int nb, int ncols, double scale,
                     double* __restrict__ out) {
-  __shared__ double sv[256], sc[256];
+  __shared__ double sv[16], sc[16];
   const int col = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // per lane: Neumaier over a strided slice of the blocks
   double s = 0.0, c = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     const double x = part[static_cast<size_t>(b) * ncols + col];
@@ -161,17 +173,27 @@ __global__ void __launch_bounds__(256)
     c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
     s = t;
   }
-  sv[threadIdx.x] = s;
-  sc[threadIdx.x] = c;
+  // wavefront butterfly with error-free additions (TwoSum), then the waves
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double s2 = __shfl_xor(s, o, 64), c2 = __shfl_xor(c, o, 64);
+    double t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c += c2 + e;
+  }
+  if (lane == 0) {
+    sv[wave] = s;
+    sc[wave] = c;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     double S = 0.0, C = 0.0;
-    for (int i = 0; i < static_cast<int>(blockDim.x); ++i) {
-      const double x = sv[i];
-      const double t = S + x;
-      C += (fabs(S) >= fabs(x)) ? (S - t) + x : (x - t) + S;
+    for (int i = 0; i < nw; ++i) {
+      double t, e;
+      two_sum(S, sv[i], t, e);
       S = t;
-      C += sc[i];
+      C += sc[i] + e;
     }
     out[col] = (S + C) * scale;
   }

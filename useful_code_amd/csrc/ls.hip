@@ -10,9 +10,12 @@
 //   FP32 accumulation inside a block and FP64 block partials.
 // * k_ls_date: one backward-induction date of the LS first pass: applies the
 //   later date's exercise decision and numeraire deflation to the running
-//   continuation values, accumulates this date's normal equations (optionally
-//   over in-the-money paths only) on MFMA, and the last block to finish sums
-//   the partials in FP64 and solves them by FP64 Cholesky.
+//   continuation values and accumulates this date's normal equations
+//   (optionally over in-the-money paths only) on MFMA into FP64 block
+//   partials.  The next launch (the earlier date, or k_ls_final) starts by
+//   summing those partials in FP64 and solving them by FP64 Cholesky in every
+//   block (identical inputs and order, so identical coefficients): one launch
+//   per date, no atomics or grid-wide fences.
 // * k_ls_second: the frozen-strategy pass (price under given coefficients).
 // * the reference's small LS kernels (basis variables, exercise values, basis
 //   expansion, continuation values, update, deflation).
@@ -34,7 +37,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxBasisVars = 4;  // cross basis with 4 variables: 13 functions
 constexpr int kMaxGram = 16;      // rows of one Gram matrix (basis functions + target)
-constexpr int kLsThreads = 256;
+constexpr int kLsThreads = 256;     // lane-per-path kernels
+constexpr int kGramThreads = 1024;  // MFMA contraction kernels (16 wavefronts)
 
 __device__ __forceinline__ floatx4 mfma16x16x4(float a, float b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -75,45 +79,62 @@ __host__ __device__ inline int basis_count(int v, bool cross) {
 
 // Adds the two diagonal 8x8 blocks (groups == 2) or takes the 16x16 result
 // (groups == 1) of each wave's accumulator and sums the waves of the block in
-// FP64: out[i*n + j] for i, j < n.  Every lane of the block calls it.
+// FP64 into the block's column of the entry-major partials:
+// part[(i*n + j) * gridDim.x + blockIdx.x] for i, j < n.  Every lane of the
+// block calls it.
 template <int GROUPS>
 __device__ __forceinline__ void block_gram(const floatx4& acc, int n, float* lds_d /* [nw][256] */,
-                                           double* out) {
+                                           double* part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   float* d = lds_d + wave * 256;
 #pragma unroll
   for (int i = 0; i < 4; ++i) d[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
   __syncthreads();
-  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-    const int i = t / n, j = t - i * n;
+  // 16 lanes per entry (4 entries per wavefront), lane q of the group adds
+  // the tiles of waves q, q+16, ...; then a 16-lane butterfly.
+  const int q = lane & 15;
+  for (int t = wave * 4 + (lane >> 4); t - (lane >> 4) < n * n; t += nw * 4) {
     double s = 0.0;
-    for (int w = 0; w < nw; ++w) {
-      const float* dw = lds_d + w * 256;
-      s += static_cast<double>(dw[i * 16 + j]);
-      if (GROUPS == 2) s += static_cast<double>(dw[(8 + i) * 16 + 8 + j]);
+    if (t < n * n) {
+      const int i = t / n, j = t - i * n;
+      for (int w = q; w < nw; w += 16) {
+        const float* dw = lds_d + w * 256;
+        s += static_cast<double>(dw[i * 16 + j]);
+        if (GROUPS == 2) s += static_cast<double>(dw[(8 + i) * 16 + 8 + j]);
+      }
     }
-    out[t] = s;
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (q == 0 && t < n * n) part[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = s;
   }
 }
 
-// Sums part[b][t] over the blocks b < nb for t < n*n into sum[t] (LDS),
-// using all lanes of the block; contains barriers.
-__device__ __forceinline__ void sum_block_partials(const double* part, int nb, int n,
-                                                   double* red /* [256] */, double* sum) {
-  const int nn = n * n;
-  const int slices = blockDim.x / nn;
-  const int t = threadIdx.x;
-  if (t < slices * nn) {
-    const int entry = t % nn, slice = t / nn;
-    double s = 0.0;
-    for (int b = slice; b < nb; b += slices) s += part[static_cast<size_t>(b) * nn + entry];
-    red[t] = s;
-  }
-  __syncthreads();
-  if (t < nn) {
-    double s = 0.0;
-    for (int q = 0; q < slices; ++q) s += red[q * nn + t];
-    sum[t] = s;
+// Sums the entry-major partials part[t * nb + b] over b < nb into sum[t]
+// (LDS) for t < nn: each wavefront takes 4 entries at a time and each lane 4
+// blocks of each (all 16 loads in flight, coalesced), then wavefront sums.
+// Contains a barrier.
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nb, int nn,
+                                                double* sum) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int t0 = wave * 4; t0 < nn; t0 += nw * 4) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+      double v[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int t = t0 + u, b = b0 + lane + 64 * r;
+          v[u][r] = (t < nn && b < nb) ? part[static_cast<size_t>(t) * nb + b] : 0.0;
+        }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] += (v[u][0] + v[u][1]) + (v[u][2] + v[u][3]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double tot = wave_sum_f64(s[u]);
+      if (lane == 0 && t0 + u < nn) sum[t0 + u] = tot;
+    }
   }
   __syncthreads();
 }
@@ -153,6 +174,51 @@ __device__ bool cholesky_rhs(const double* G, int ld, int k, double inv, double*
   return true;
 }
 
+// The same solve with the factor in registers (K known at compile time, all
+// loops unrolled): G in LDS, beta out.  Returns false (beta = 0) when a
+// pivot is not positive.
+template <int K>
+__device__ bool cholesky_rhs_reg(const double* G, int ld, double inv, double* beta) {
+  double L[K][K], y[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = G[i * ld + j] * inv;
+#pragma unroll
+      for (int q = 0; q < j; ++q) s -= L[i][q] * L[j][q];
+      if (i == j) {
+        if (!(s > 0.0)) {
+#pragma unroll
+          for (int r = 0; r < K; ++r) beta[r] = 0.0;
+          return false;
+        }
+        L[i][i] = sqrt(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    double s = G[i * ld + K] * inv;
+#pragma unroll
+    for (int q = 0; q < i; ++q) s -= L[i][q] * y[q];
+    y[i] = s / L[i][i];
+  }
+  double x[K];
+#pragma unroll
+  for (int i = K - 1; i >= 0; --i) {
+    double s = y[i];
+#pragma unroll
+    for (int q = i + 1; q < K; ++q) s -= L[q][i] * x[q];
+    x[i] = s / L[i][i];
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i) beta[i] = x[i];
+  return true;
+}
+
 // LS normal equations from a (k+1) x (k+1) Gram matrix whose row 0 is the
 // constant function (so G[0][0] is the path count) and row k the target:
 // mean-scaled as in GenerateLSRegressionMatrixSelectively_gold
@@ -165,7 +231,33 @@ __device__ void solve_normal_equations(const double* G, int k, double* beta, dou
     for (int q = 0; q < k; ++q) beta[q] = 0.0;
     return;
   }
-  cholesky_rhs(G, k + 1, k, 1.0 / cnt, beta, L, y);
+  const double inv = 1.0 / cnt;
+  const int ld = k + 1;
+  switch (k) {  // the basis sizes of 0..4 variables, cross or plain
+    case 1: cholesky_rhs_reg<1>(G, ld, inv, beta); return;
+    case 3: cholesky_rhs_reg<3>(G, ld, inv, beta); return;
+    case 5: cholesky_rhs_reg<5>(G, ld, inv, beta); return;
+    case 6: cholesky_rhs_reg<6>(G, ld, inv, beta); return;
+    case 7: cholesky_rhs_reg<7>(G, ld, inv, beta); return;
+    default: cholesky_rhs(G, ld, k, inv, beta, L, y); return;
+  }
+}
+
+// Every block of a date kernel solves the later date's normal equations
+// itself (same partials, same order, so the same coefficients): the sum of
+// the entry-major partials into LDS, the FP64 solve by lane 0, the
+// coefficients left in beta (LDS).  Block 0 also stores them.  Contains
+// barriers.
+__device__ __forceinline__ void solve_date(const double* __restrict__ part, int nb, int k,
+                                           double* gram, double* chol, double* beta,
+                                           double* __restrict__ coeffs_row, int maxd) {
+  reduce_partials(part, nb, (k + 1) * (k + 1), gram);
+  if (threadIdx.x == 0) {
+    solve_normal_equations(gram, k, beta, chol, chol + kMaxGram * kMaxGram);
+    if (blockIdx.x == 0)
+      for (int i = 0; i < maxd; ++i) coeffs_row[i] = i < k ? beta[i] : 0.0;
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -236,10 +328,10 @@ struct RowSet {
 };
 
 template <int GROUPS>
-__global__ void __launch_bounds__(kLsThreads)
+__global__ void __launch_bounds__(kGramThreads)
     k_gram(const RowSet rows, int n, const float* __restrict__ mask, int P,
            double* __restrict__ part) {
-  __shared__ float lds_d[(kLsThreads / 64) * 256];
+  __shared__ float lds_d[(kGramThreads / 64) * 256];
   constexpr int kPathsPerOp = 4 * GROUPS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
@@ -259,19 +351,18 @@ __global__ void __launch_bounds__(kLsThreads)
       acc = mfma16x16x4(a, a, acc);
     }
   }
-  block_gram<GROUPS>(acc, n, lds_d, part + static_cast<size_t>(blockIdx.x) * n * n);
+  block_gram<GROUPS>(acc, n, lds_d, part);
 }
 
 // Sums the partials of k_gram and writes the n x n result (FP64 and, when
 // asked, FP32 scaled by alpha plus beta times the old FP32 value) for the
 // (r0, c0, nr, nc) block of it, row-major with leading dimension ld.
-__global__ void __launch_bounds__(kLsThreads)
+__global__ void __launch_bounds__(kGramThreads)
     k_gram_finish(const double* __restrict__ part, int nb, int n, int r0, int c0, int nr, int nc,
                   int ld, double* __restrict__ out64, float* __restrict__ out32, float alpha,
                   float beta, int transpose) {
-  __shared__ double red[kLsThreads];
   __shared__ double sum[kMaxGram * kMaxGram];
-  sum_block_partials(part, nb, n, red, sum);
+  reduce_partials(part, nb, n * n, sum);
   for (int t = threadIdx.x; t < nr * nc; t += blockDim.x) {
     const int i = t / nc, j = t - i * nc;
     const double v = sum[(r0 + i) * n + c0 + j];
@@ -309,16 +400,15 @@ __device__ __forceinline__ float continuation(const float* __restrict__ bv, size
 }
 
 template <int GROUPS>
-__global__ void __launch_bounds__(kLsThreads)
+__global__ void __launch_bounds__(kGramThreads)
     k_ls_date(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
               const float* __restrict__ ev, const float* __restrict__ num,
-              double* __restrict__ coeffs, float* __restrict__ Y, double* __restrict__ part,
-              int* __restrict__ ticket) {
-  __shared__ float lds_d[(kLsThreads / 64) * 256];
-  __shared__ double red[kLsThreads];
+              double* __restrict__ coeffs, float* __restrict__ Y,
+              const double* __restrict__ part_later, double* __restrict__ part) {
+  __shared__ float lds_d[(kGramThreads / 64) * 256];
   __shared__ double gram[kMaxGram * kMaxGram];
-  __shared__ double chol[kMaxGram * kMaxGram + 2 * kMaxGram];
-  __shared__ int is_last;
+  __shared__ double chol[kMaxGram * kMaxGram + kMaxGram];
+  __shared__ double beta_n[kMaxGram];
   constexpr int kPathsPerOp = 4 * GROUPS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const size_t P = static_cast<size_t>(a.P);
@@ -327,7 +417,9 @@ __global__ void __launch_bounds__(kLsThreads)
   const int n = a.k + 1;  // basis functions + target
   const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
   const int sub = GROUPS == 2 ? ((lane >> 3) & 1) * 4 + (lane >> 4) : (lane >> 4);
-  const double* __restrict__ beta_n = coeffs + static_cast<size_t>(a.e + 1) * a.maxd;
+  if (later)  // coefficients of date e+1 from its partials
+    solve_date(part_later, gridDim.x, a.kn, gram, chol, beta_n,
+               coeffs + static_cast<size_t>(a.e + 1) * a.maxd, a.maxd);
   floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
   const int stride = gridDim.x * nw * 64;
   for (int base = (blockIdx.x * nw + wave) * 64; base < a.P; base += stride) {
@@ -366,43 +458,37 @@ __global__ void __launch_bounds__(kLsThreads)
     for (int q = 0; q < 64 / kPathsPerOp; ++q) {
       const int srcl = q * kPathsPerOp + sub;
       const float s0 = __shfl(x0, srcl, 64), s1 = __shfl(x1, srcl, 64);
-      const float s2 = __shfl(x2, srcl, 64), s3 = __shfl(x3, srcl, 64);
+      float s2 = 0.0f, s3 = 0.0f;
+      if (a.v > 2) {  // uniform
+        s2 = __shfl(x2, srcl, 64);
+        s3 = __shfl(x3, srcl, 64);
+      }
       const float st = __shfl(t, srcl, 64), sw = __shfl(w, srcl, 64);
       float v = j < a.k ? basis_value(j, a.v, cross, s0, s1, s2, s3) : (j == a.k ? st : 0.0f);
       v *= sw;
       acc = mfma16x16x4(v, v, acc);
     }
   }
-  block_gram<GROUPS>(acc, n, lds_d, part + static_cast<size_t>(blockIdx.x) * n * n);
-  // Last block to finish: FP64 sum of the partials and Cholesky solve.
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1) == static_cast<int>(gridDim.x) - 1;
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  sum_block_partials(part, gridDim.x, n, red, gram);
-  if (threadIdx.x == 0) {
-    double* beta = chol + kMaxGram * kMaxGram + kMaxGram;
-    solve_normal_equations(gram, a.k, beta, chol, chol + kMaxGram * kMaxGram);
-    double* out = coeffs + static_cast<size_t>(a.e) * a.maxd;
-    for (int i = 0; i < a.maxd; ++i) out[i] = i < a.k ? beta[i] : 0.0;
-    *ticket = 0;  // ready for the next date
-  }
+  block_gram<GROUPS>(acc, n, lds_d, part);
 }
 
-// Date 0's exercise decision, then sum_p Y_p / B_0 into block partials.
-__global__ void __launch_bounds__(kLsThreads)
+// Date 0's coefficients and exercise decision, then sum_p Y_p / B_0 into
+// block partials.
+__global__ void __launch_bounds__(kGramThreads)
     k_ls_final(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
                const float* __restrict__ ev, const float* __restrict__ num,
-               const double* __restrict__ coeffs, const float* __restrict__ Y,
-               double* __restrict__ part) {
-  __shared__ double red[4];
+               double* __restrict__ coeffs, const float* __restrict__ Y,
+               const double* __restrict__ part_date0, int nb_date0, double* __restrict__ part) {
+  __shared__ double red[kGramThreads / 64];
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double chol[kMaxGram * kMaxGram + kMaxGram];
+  __shared__ double beta0[kMaxGram];
+  solve_date(part_date0, nb_date0, a.k, gram, chol, beta0, coeffs, a.maxd);
   const size_t P = static_cast<size_t>(a.P);
   double s = 0.0;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
     float y = Y[p];
-    const float cont = continuation(bv, 0, P, p, a.v, a.k, a.cross != 0, a.sh, a.sc, coeffs);
+    const float cont = continuation(bv, 0, P, p, a.v, a.k, a.cross != 0, a.sh, a.sc, beta0);
     const float exv = ev[p];
     const bool allowed = !a.itm_only || exv > 0.0f;
     if (allowed && exv > cont)
@@ -453,8 +539,6 @@ __global__ void __launch_bounds__(kLsThreads)
   }
   block_sum_to(s, red, part + blockIdx.x);
 }
-
-__global__ void k_zero_int(int* x) { *x = 0; }
 
 // ---------------------------------------------------------------------------
 // The reference's small LS kernels.
@@ -577,18 +661,31 @@ int ls_blocks(int paths) {
   return want < 1 ? 1 : (want > cap ? cap : want);
 }
 
+// Blocks of the 1024-lane MFMA kernels: one per CU at most, so the
+// per-date partials stay at most CUs x 256 doubles.
+int gram_blocks(int paths) {
+  const int want = (paths + kGramThreads - 1) / kGramThreads;
+  const int cap = device_cu_count();
+  return want < 1 ? 1 : (want > cap ? cap : want);
+}
+
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
+// Workspace: running continuation values (P floats), then two partial
+// buffers used alternately by consecutive dates (date e reads date e+1's
+// partials while writing its own), each kMaxGram^2 doubles per block.
 struct LsLayout {
-  size_t y, part, ticket, total;
+  size_t y, part, part2, total;
 };
 
 LsLayout ls_layout(int paths) {
   LsLayout l;
   l.y = 0;
   l.part = align256(sizeof(float) * static_cast<size_t>(paths));
-  l.ticket = l.part + align256(sizeof(double) * kMaxGram * kMaxGram * ls_blocks(paths) + 8);
-  l.total = l.ticket + 256;
+  const int nb = ls_blocks(paths) > gram_blocks(paths) ? ls_blocks(paths) : gram_blocks(paths);
+  const size_t half = align256(sizeof(double) * kMaxGram * kMaxGram * nb);
+  l.part2 = l.part + half;
+  l.total = l.part2 + half;
   return l;
 }
 
@@ -625,10 +722,10 @@ int gram_launch(const char* who, const float* const* rows, int n, const float* m
   for (int i = 0; i < n; ++i) rs.row[i] = rows[i];
   for (int i = n; i < kMaxGram; ++i) rs.row[i] = rows[0];
   if (n <= 8)
-    hipLaunchKernelGGL(k_gram<2>, dim3(nb), dim3(kLsThreads), 0, st, rs, n, mask, paths, part);
+    hipLaunchKernelGGL(k_gram<2>, dim3(nb), dim3(kGramThreads), 0, st, rs, n, mask, paths, part);
   else
-    hipLaunchKernelGGL(k_gram<1>, dim3(nb), dim3(kLsThreads), 0, st, rs, n, mask, paths, part);
-  hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(kLsThreads), 0, st, part, nb, n, r0, c0, nr, nc,
+    hipLaunchKernelGGL(k_gram<1>, dim3(nb), dim3(kGramThreads), 0, st, rs, n, mask, paths, part);
+  hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(kGramThreads), 0, st, part, nb, n, r0, c0, nr, nc,
                      ld, out64, out32, alpha, beta, transpose);
   return check_launch(who);
 }
@@ -706,11 +803,10 @@ extern "C" int ls_first_pass_gpu(int n_ex, const float* basis_variables, const i
   const LsLayout lay = ls_layout(paths);
   char* ws = static_cast<char*>(workspace);
   float* Y = reinterpret_cast<float*>(ws + lay.y);
-  double* part = reinterpret_cast<double*>(ws + lay.part);
-  int* ticket = reinterpret_cast<int*>(ws + lay.ticket);
-  const int nb = ls_blocks(paths);
+  double* parts[2] = {reinterpret_cast<double*>(ws + lay.part),
+                      reinterpret_cast<double*>(ws + lay.part2)};
+  const int nb = ls_blocks(paths), ng = gram_blocks(paths);
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, ticket);
   const bool cross = use_cross != 0;
   DateArgs a{};
   a.E = n_ex;
@@ -731,20 +827,24 @@ extern "C" int ls_first_pass_gpu(int n_ex, const float* basis_variables, const i
     } else {
       a.vn = a.kn = 0;
     }
+    double* out = parts[e & 1];
+    const double* later = parts[(e + 1) & 1];
     if (a.k + 1 <= 8)
-      hipLaunchKernelGGL(k_ls_date<2>, dim3(nb), dim3(kLsThreads), 0, st, a, basis_variables,
-                         cash_flows, exercise_values, numeraires, coefficients, Y, part, ticket);
+      hipLaunchKernelGGL(k_ls_date<2>, dim3(ng), dim3(kGramThreads), 0, st, a, basis_variables,
+                         cash_flows, exercise_values, numeraires, coefficients, Y, later, out);
     else
-      hipLaunchKernelGGL(k_ls_date<1>, dim3(nb), dim3(kLsThreads), 0, st, a, basis_variables,
-                         cash_flows, exercise_values, numeraires, coefficients, Y, part, ticket);
+      hipLaunchKernelGGL(k_ls_date<1>, dim3(ng), dim3(kGramThreads), 0, st, a, basis_variables,
+                         cash_flows, exercise_values, numeraires, coefficients, Y, later, out);
   }
   a.e = 0;
   a.v = vars_each_step[0];
   a.k = basis_count(a.v, cross);
   fill_norm(var_shift, var_scale, 0, a.v, max_vars, a.sh, a.sc);
-  hipLaunchKernelGGL(k_ls_final, dim3(nb), dim3(kLsThreads), 0, st, a, basis_variables, cash_flows,
-                     exercise_values, numeraires, coefficients, Y, part);
-  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, part, nb, 1,
+  // date 0's partials are in parts[0]; the price partials go to parts[1]
+  hipLaunchKernelGGL(k_ls_final, dim3(ng), dim3(kGramThreads), 0, st, a, basis_variables,
+                     cash_flows, exercise_values, numeraires, coefficients, Y, parts[0], ng,
+                     parts[1]);
+  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, parts[1], ng, 1,
                      1.0 / static_cast<double>(paths), price_out);
   return check_launch("ls_first_pass_gpu");
 }
@@ -798,7 +898,7 @@ extern "C" int ls_normal_eq_gpu(int paths, int n_basis, const float* basis_funct
   if (target) rows[n_basis] = target;
   const LsLayout lay = ls_layout(paths);
   double* part = reinterpret_cast<double*>(static_cast<char*>(workspace) + lay.part);
-  return gram_launch("ls_normal_eq_gpu", rows, n, mask, paths, part, ls_blocks(paths), 0, 0, n, n,
+  return gram_launch("ls_normal_eq_gpu", rows, n, mask, paths, part, gram_blocks(paths), 0, 0, n, n,
                      n, gram_out, nullptr, 1.0f, 0.0f, 0, as_stream(stream));
 }
 
@@ -980,10 +1080,7 @@ extern "C" int updateDeflationOfContinuationValues_gpu(float* deflated_next_step
 // ---------------------------------------------------------------------------
 namespace {
 
-int outer_blocks(int paths) {
-  const int b = ls_blocks(paths);
-  return b > 256 ? 256 : b;
-}
+int outer_blocks(int paths) { return gram_blocks(paths); }
 
 int outer_product(const char* who, int paths, int R, int C, bool symmetric, const float* in1,
                   const float* in2, float* answer, float* workspace, size_t workspace_floats,
