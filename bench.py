@@ -136,6 +136,155 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
     return res
 
 
+def basket_section(args, torch, O, configs, pipeline, ev):
+    """Multi-asset Black-Scholes basket Asian (32 assets, 64 bridge steps,
+    2^20 paths): price + 32 deltas + 32 vegas per job; roofline of the fused
+    path kernel (HBM-bound: it reads the variates once)."""
+    w = configs.basket_asian(paths=args.basket_paths)
+    P, A, S = w.paths, w.assets, w.steps
+    pr = pipeline.BasketPricer(w)
+    for _ in range(args.warmup):
+        pr.price_and_greeks()
+    recs = [(ev(), ev(), ev()) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b, c in recs:
+        a.record()
+        pr.generate()
+        b.record()
+        pr.greeks_kernel()
+        c.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    gen = float(np.mean([a.elapsed_time(b) for a, b, _ in recs]))
+    kern = float(np.mean([b.elapsed_time(c) for _, b, c in recs]))
+    out = pr.out.cpu().numpy()
+    zbytes = 4.0 * S * A * P
+    gbs = zbytes / (kern * 1e-3) / 1e9
+    tri = A * (A + 1) // 2  # multiply-adds of the triangular correlation per path-step
+    mfma_tflops = 2.0 * tri * S * P / (kern * 1e-3) / 1e12
+    res = {
+        "workload": "basket_asian_32x64: 32 assets, 64 Sobol-bridge steps (2048 dims), 2^20 paths, "
+                    "arithmetic-average basket Asian call K=100, price + 32 deltas + 32 vegas",
+        "paths": P,
+        "path_steps_per_s": P * S / ((gen + kern) * 1e-3),
+        "job_ms": wall,
+        "kernels_ms": {"sobol_bridge": gen, "basket_asian": kern},
+        "price": float(out[0]),
+        "roofline": {
+            "kernel": "k_basket_asian<2> (MFMA correlation, log-Euler, exp, adjoint sums)",
+            "bound": "hbm",
+            "achieved": gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": zbytes,
+            "mfma_useful_tflops": mfma_tflops,
+        },
+    }
+    if not args.no_error_check:
+        ws = configs.basket_asian(paths=1 << 14)
+        ps = pipeline.BasketPricer(ws)
+        g = ps.price_and_greeks().cpu().numpy()
+        ints = O.scramble(O.sobol_ints(ws.paths, ws.dims, 0), ws.scramblers, ws.dims)
+        z = O.bridge_transform(O.invnorm_u32(ints), ws.steps, ws.assets)
+        m = O.basket_model(np.log(ws.spots), ws.vols, ws.chol, ws.weights, ws.rate, ws.dt,
+                           ws.strike, ws.steps, ws.paths)
+        pp, d, v = O.basket_asian_paths(m, z)
+        g64 = np.concatenate([d, v]) / ws.paths
+        res["price_rel_err_vs_fp64"] = abs(g[0] / pp.mean() - 1)
+        res["greeks_max_rel_err_vs_fp64"] = float(np.max(np.abs(g[1:] - g64) / np.abs(g64)))
+        res["error_sample"] = "first 2^14 paths of the workload's Sobol sequence"
+    del pr
+    torch.cuda.empty_cache()
+    return res
+
+
+def stage_section(args, torch, configs, pipeline, ev):
+    """The reference-ABI stage kernels at the config-2 shape (P = 2^20,
+    N = S = 40, F = 5, D = 200): achieved HBM GB/s from algorithmic bytes over
+    HIP-event time (mean of 3 launches after one warm-up), against 8 TB/s."""
+    from useful_code_amd import koo
+    w = configs.caplet_portfolio(paths=args.paths)
+    P, N, F, S = w.paths, w.n_rates, w.factors, w.steps
+    D = S * F
+    dev = lambda x, dt: torch.as_tensor(np.ascontiguousarray(x)).to(dt).cuda()  # noqa: E731
+    dirs = dev(koo.sobol_directions(D).view(np.int32), torch.int32)
+    scr = dev(np.asarray(w.scramblers, np.uint32).view(np.int32), torch.int32)
+    alloc = dev(koo.bridge_allocate_diagonal(F, S), torch.int32)
+    ints = torch.empty(D * P, dtype=torch.int32, device="cuda")
+    normals = torch.empty(D * P, dtype=torch.float32, device="cuda")
+    z = torch.empty(D * P, dtype=torch.float32, device="cuda")
+    rates = torch.empty(S * N * P, dtype=torch.float32, device="cuda")
+    logs = torch.empty_like(rates)
+    disc = torch.empty(S * (N + 1) * P, dtype=torch.float32, device="cuda")
+    num = torch.empty(S * P, dtype=torch.float32, device="cuda")
+    flows = torch.empty(S * P, dtype=torch.float32, device="cuda")
+    flows2 = torch.empty(S * P, dtype=torch.float32, device="cuda")
+    dflows = torch.empty(S * P, dtype=torch.float32, device="cuda")
+    summed = torch.empty(P, dtype=torch.float32, device="cuda")
+    r1 = torch.empty(S * P, dtype=torch.float32, device="cuda")
+    A = dev(w.A, torch.float32)
+    f0 = dev(w.f0, torch.float32)
+    lf0 = dev(np.log(w.f0 + w.disp()), torch.float32)
+    taus = dev(w.taus, torch.float32)
+    dsp = dev(w.disp(), torch.float32)
+    mu0 = dev(koo.spot_drifts(w.A[0], w.f0, w.taus, w.disp(), 0), torch.float32)
+    fx = dev(koo.fixed_drifts(w.A), torch.float32)
+    alive = dev(w.alive(), torch.int32)
+    first = dev(np.arange(1, S + 1), torch.int32)
+    theta = dev(np.zeros(S), torch.float32)
+    aux = dev(np.concatenate([[w.strike], np.full(S, w.tau)]), torch.float32)
+    ws = torch.empty(koo.sum_workspace_doubles(S * P), dtype=torch.float64, device="cuda")
+    out64 = torch.empty(1, dtype=torch.float64, device="cuda")
+    U = N * (N + 1) // 2  # alive rate updates per path
+    four = 4.0
+    stages = [
+        ("sobolGPUInts", lambda: koo.sobolGPUInts(P, D, 0, dirs, ints), four * D * P),
+        ("scramble_gpu (all D rows)", lambda: koo.scramble_gpu(ints, scr, D, P), 2 * four * D * P),
+        ("inverse_normal_u32_gpu", lambda: koo.inverse_normal_u32_gpu(ints, normals),
+         2 * four * D * P),
+        ("brownian_bridge_gpu", lambda: koo.brownian_bridge_gpu(S, alloc, normals, z, P, F),
+         2 * four * D * P),
+        ("sobol_bridge_gpu (fused a+bridge)",
+         lambda: koo.sobol_bridge_gpu(P, 0, S, F, dirs, scr, D, alloc, z), four * D * P),
+        ("LMM_evolver_pc_all_gpu (rates, logs, discounts)",
+         lambda: koo.LMM_evolver_pc_all_gpu(f0, lf0, taus, z, A, mu0, fx, dsp, alive, P, F, S, N,
+                                            rates, logs, disc),
+         four * (D * P + 2 * U * P + (U + S) * P)),
+        ("spot_measure_numeraires_computation_gpu",
+         lambda: koo.spot_measure_numeraires_computation_gpu(disc, num, P, N),
+         four * (2 * S * P + S * P)),
+        ("forward_rate_extraction_gpu",
+         lambda: koo.forward_rate_extraction_gpu(rates, np.arange(S), P, S, N, r1),
+         2 * four * S * P),
+        ("cashFlowGeneratorCallerCaplet",
+         lambda: koo.cash_flow_generator("caplet", flows, flows2, aux, P, S, r1),
+         3 * four * S * P),
+        ("cashFlowDiscounting_gpu",
+         lambda: koo.cashFlowDiscounting_gpu(first, first, theta, disc, flows, num, P, S, dflows,
+                                             summed),
+         four * (5 * S * P + P)),
+        ("koo_sum_f32_to_f64 (S*P values)", lambda: koo.sum_f32_to_f64(dflows, ws, out64),
+         four * S * P),
+    ]
+    res = {}
+    for name, fn, nbytes in stages:
+        fn()
+        recs = [(ev(), ev()) for _ in range(3)]
+        for a, b in recs:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in recs]))
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GB_per_s": gbs,
+                     "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +296,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--no-bermudan", action="store_true")
+    ap.add_argument("--no-basket", action="store_true")
+    ap.add_argument("--basket-paths", type=int, default=1 << 20)
+    ap.add_argument("--no-stages", action="store_true",
+                    help="skip the per-stage HBM measurements of the reference-ABI kernels")
     ap.add_argument("--no-k1", action="store_true",
                     help="skip the checkpoint-every-step comparison")
     ap.add_argument("--bermudan-paths", type=int, default=1 << 18)
@@ -333,6 +486,14 @@ def main():
 
     if not args.no_bermudan:
         out["bermudan_ls"] = bermudan_section(args, torch, O, configs, pipeline, ev)
+
+    if not args.no_basket:
+        out["basket_asian"] = basket_section(args, torch, O, configs, pipeline, ev)
+
+    if not args.no_stages:
+        del pr
+        torch.cuda.empty_cache()
+        out["stage_kernels_hbm"] = stage_section(args, torch, configs, pipeline, ev)
 
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4

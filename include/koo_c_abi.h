@@ -74,6 +74,12 @@ int sobol_scramble_normal_gpu(int n_vectors, int n_dimensions, unsigned path_off
  * Replaces GenerateExpLongCorrMatrix, gold/volstructs_gold.cpp:157. */
 int koo_exp_corr(const double* times, int n, double L, double beta, double* corr);
 
+/* Lower Cholesky factor (row-major, L L^T = corr) of a positive-definite
+ * n x n matrix.  New: the full-rank correlation root of the multi-asset
+ * Black-Scholes path (the reference feeds an arbitrary A to
+ * correlated_paths_multistep_GPU, Correlate_gpu.h:10-19). */
+int koo_cholesky_lower(int n, const double* corr, double* L);
+
 /* Rank-`factors` PCA root of a symmetric n x n matrix (largest eigenvalues),
  * rows rescaled to keep the diagonal when normalise != 0; out is n x factors.
  * Replaces pseudoSqrtSchur, gold/math/pseudoSquareRoot.cpp:76. */
@@ -479,6 +485,47 @@ int ls_normal_eq_gpu(int paths, int n_basis, const float* basis_functions, const
  * (x = 0).  Replaces the host SVD of LinearSolverConcrete_main.cu:127-188. */
 int cholesky_solve_f64_gpu(int n, const double* A, const double* b, double* x, int* info,
                            void* stream);
+
+/* =====================================================================
+ * Multi-asset Black-Scholes paths (basket products).
+ * =================================================================== */
+
+/* Correlate_gpu.h:10-19: out(s, r, p) = sum_f A(s, r, f) in(s, p, f) for
+ * r >= alive[s] (device array); in is (steps, paths, factors), out
+ * (steps, out_dimensions, paths), A advances A_offset_per_step floats per step
+ * (0: one matrix for all steps). */
+int correlated_paths_multistep_GPU(const float* input_data_device, float* output_data_device,
+                                   const float* A_device, const int* alive_device,
+                                   int A_offset_per_step, int factors, int out_dimensions,
+                                   int number_paths, int steps, void* stream);
+
+/* multid_path_gen_BS_gpu.h:10-16: out(s, r, p) = exp(logRates[r] +
+ * sum_{u<=s} increments(u, r, p)), (steps, rates, paths). */
+int multi_dim_BS_path_generator_kernel(const float* logRates_device,
+                                       const float* correlatedBrownianIncrements_device,
+                                       float* outputPaths_device, int paths, int rates, int steps,
+                                       void* stream);
+
+/* weighted_average_gpu.h:16-19: out(s, p) = sum_j weights[j] in(s, j, p);
+ * weights is a device array. */
+int basketWeightings_gpu(const float* inputPaths, float* outputPaths, const float* weights,
+                         int paths, int numberOfStocks, int numberSteps, void* stream);
+
+/* New: arithmetic-average basket Asian call with pathwise adjoint Greeks in one
+ * pass.  x_a(t_{s+1}) = x_a(t_s) + (rate - vol_a^2/2) dt + (C z_s)_a with
+ * C = scaled_chol (assets x assets, row-major, C[a][f] = vol_a sqrt(dt)
+ * L[a][f], L the lower Cholesky factor of the correlation, zero above the
+ * diagonal) and z the bridged variates (steps, assets, paths); payoff
+ * e^{-rate T} (sum_{s=1..S} sum_a w_a e^{x_a(t_s)} / S - strike)^+.
+ * out (device, FP64): [price, dV/dS_a(0) (assets), dV/dvol_a (assets)] means
+ * over paths, the vegas with L held fixed.  All arrays are device arrays;
+ * paths a multiple of 4, variates 16-byte aligned, assets <= 64; partials
+ * hold basket_asian_workspace() bytes. */
+int basket_asian_workspace(int paths, int assets, size_t* partial_bytes);
+int basket_asian_gpu(int paths, int assets, int steps, const float* variates,
+                     const float* log_spots, const float* vols, const float* scaled_chol,
+                     const float* weights, float rate, float dt, float strike, double* partials,
+                     double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,13 @@ def ref():
     return _REF
 
 
+class BasketModel(ctypes.Structure):
+    _fields_ = [("assets", _c_int), ("steps", _c_int), ("paths", _c_int),
+                ("x0", ctypes.c_void_p), ("vols", ctypes.c_void_p), ("chol", ctypes.c_void_p),
+                ("weights", ctypes.c_void_p), ("rate", _c_dbl), ("dt", _c_dbl),
+                ("strike", _c_dbl)]
+
+
 class CapletModel(ctypes.Structure):
     _fields_ = [("n_rates", _c_int), ("factors", _c_int), ("paths", _c_int),
                 ("f0", ctypes.c_void_p), ("taus", ctypes.c_void_p), ("disp", ctypes.c_void_p),
@@ -118,6 +125,14 @@ def _declare(L):
                                         _c_int, _c_int, _c_int, _f64p]),
         "orc_bermudan_paths": (None, [ctypes.POINTER(CapletModel), _f64p, _c_int, _i32p, _f64p,
                                       _f64p, _f64p]),
+        "orc_cholesky_lower": (_c_int, [_c_int, _f64p, _f64p]),
+        "orc_correlated_paths": (None, [_f64p, _f64p, _f64p, _i32p, _c_int, _c_int, _c_int,
+                                        _c_int, _c_int]),
+        "orc_multid_bs_paths": (None, [_f64p, _f64p, ctypes.c_void_p, _f64p, _c_int, _c_int,
+                                       _c_int]),
+        "orc_basket_weightings": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int]),
+        "orc_basket_asian_paths": (None, [ctypes.POINTER(BasketModel), _f64p, _f64p, _f64p,
+                                          _f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -154,6 +169,9 @@ def _declare_ref(R):
                                              _c_dbl, _f64p]),
         "ref_ls_multi_depth0": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p, _c_int,
                                          _c_int, _c_int, _f64p]),
+        "ref_correlated_paths": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _c_int]),
+        "ref_multid_bs_paths": (None, [_f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _c_int]),
+        "ref_basket_weightings": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(R, name)
@@ -398,3 +416,61 @@ def normals_from_ints(ints, steps, factors):
     """Inverse normal (FP64) then the diagonal Brownian bridge: (S,F,P)."""
     ints = np.ascontiguousarray(ints, np.uint32)
     return bridge_transform(invnorm_u32(ints), steps, factors)
+
+
+# ---------------------------------------------------------------- multi-d Black-Scholes
+
+def cholesky_lower(C):
+    C = np.ascontiguousarray(C, np.float64)
+    L = np.zeros_like(C)
+    st = lib().orc_cholesky_lower(C.shape[0], C.reshape(-1), L.reshape(-1))
+    return L if st == 0 else None
+
+
+def correlated_paths(z_spf, A, alive=None):
+    """z (S,P,F), A (N,F) for every step -> (S,N,P)."""
+    z = np.ascontiguousarray(z_spf, np.float64)
+    S, P, F = z.shape
+    A = np.ascontiguousarray(A, np.float64)
+    N = A.shape[0]
+    al = np.zeros(S, np.int32) if alive is None else np.ascontiguousarray(alive, np.int32)
+    out = np.zeros((S, N, P))
+    lib().orc_correlated_paths(z.reshape(-1), out.reshape(-1), A.reshape(-1), al, 0, F, N, P, S)
+    return out
+
+
+def multid_bs_paths(log0, incr, drifts=None):
+    incr = np.ascontiguousarray(incr, np.float64)
+    S, N, P = incr.shape
+    d = None if drifts is None else np.ascontiguousarray(drifts, np.float64)
+    out = np.zeros_like(incr)
+    lib().orc_multid_bs_paths(np.ascontiguousarray(log0, np.float64), incr.reshape(-1),
+                              None if d is None else d.ctypes.data, out.reshape(-1), P, N, S)
+    return out
+
+
+def basket_weightings(paths_snp, w):
+    x = np.ascontiguousarray(paths_snp, np.float64)
+    S, N, P = x.shape
+    out = np.zeros((S, P))
+    lib().orc_basket_weightings(x.reshape(-1), out.reshape(-1), np.ascontiguousarray(w, np.float64),
+                                P, N, S)
+    return out
+
+
+def basket_model(x0, vols, chol, weights, rate, dt, strike, steps, paths):
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (x0, vols, chol, weights)]
+    m = BasketModel(len(arrs[0]), steps, paths, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                    arrs[2].ctypes.data, arrs[3].ctypes.data, rate, dt, strike)
+    m._keep = arrs
+    return m
+
+
+def basket_asian_paths(model, z):
+    """z (S,A,P) -> (price per path, delta sums (A), vega sums (A))."""
+    z = np.ascontiguousarray(z, np.float64)
+    pp = np.zeros(model.paths)
+    d = np.zeros(model.assets)
+    v = np.zeros(model.assets)
+    lib().orc_basket_asian_paths(ctypes.byref(model), z.reshape(-1), pp, d, v)
+    return pp, d, v

@@ -1194,3 +1194,138 @@ void orc_bermudan_paths(const orc_caplet_model *m, const double *z, int n_ex, co
     free(L);
   }
 }
+
+/* ======================================================================
+ * Multi-dimensional Black-Scholes (basket Asian workload).
+ * ==================================================================== */
+
+/* Lower Cholesky factor L (row-major, L L^T = C); -1 if C is not positive
+ * definite. */
+int orc_cholesky_lower(int n, const double *C, double *L) {
+  for (int i = 0; i < n * n; ++i) L[i] = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = C[i * n + j];
+      for (int k = 0; k < j; ++k) s -= L[i * n + k] * L[j * n + k];
+      if (i == j) {
+        if (!(s > 0.0)) return -1;
+        L[i * n + i] = sqrt(s);
+      } else {
+        L[i * n + j] = s / L[j * n + j];
+      }
+    }
+  return 0;
+}
+
+/* correlation_kernel_texture_multistep (Correlate_gpu.cu:93-114, device part
+ * :26-90): out(s,r,p) = sum_f A(s,r,f) in(s,p,f) for r >= alive(s); the gold
+ * twin (gold/Correlation_gold.h:22-43) starts its path loop at 1. */
+void orc_correlated_paths(const double *in /* (S,P,F) */, double *out /* (S,N,P) */,
+                          const double *A, const int *alive, int a_offset_per_step, int factors,
+                          int out_dims, int paths, int steps) {
+  for (int s = 0; s < steps; ++s) {
+    const double *As = A + (size_t)s * a_offset_per_step;
+    for (int p = 0; p < paths; ++p) {
+      const double *z = in + ((size_t)s * paths + p) * factors;
+      for (int r = alive[s]; r < out_dims; ++r) {
+        double t = 0.0;
+        for (int f = 0; f < factors; ++f) t += As[r * factors + f] * z[f];
+        out[((size_t)s * out_dims + r) * paths + p] = t;
+      }
+    }
+  }
+}
+
+/* multi_dim_BS_path_generator_nonlog_gold (LINUX/gold/MultiD_Path_Gen_BS_gold.h:
+ * 44-101): cumulative log increments plus fixed drifts (S,N) (NULL: none, as
+ * in the GPU kernel multid_path_gen_BS_gpu.cu:14-64) from the initial log
+ * values, then exp. */
+void orc_multid_bs_paths(const double *log0, const double *incr /* (S,N,P) */,
+                         const double *drifts, double *out, int paths, int rates, int steps) {
+  for (int p = 0; p < paths; ++p)
+    for (int r = 0; r < rates; ++r) {
+      double x = log0[r];
+      for (int s = 0; s < steps; ++s) {
+        const size_t o = ((size_t)s * rates + r) * paths + p;
+        x += incr[o] + (drifts ? drifts[s * rates + r] : 0.0);
+        out[o] = exp(x);
+      }
+    }
+}
+
+/* basketWeightings_gold (gold/weighted_average_gold.h:19-51). */
+void orc_basket_weightings(const double *in /* (S,N,P) */, double *out /* (S,P) */,
+                           const double *w, int paths, int stocks, int steps) {
+  for (int p = 0; p < paths; ++p)
+    for (int s = 0; s < steps; ++s) {
+      double t = 0.0;
+      for (int i = 0; i < stocks; ++i) t += w[i] * in[((size_t)s * stocks + i) * paths + p];
+      out[(size_t)s * paths + p] = t;
+    }
+}
+
+/* Arithmetic-average basket Asian call, no reference counterpart.  Per path:
+ * x_a(t_{s+1}) = x_a(t_s) + (r - sigma_a^2/2) dt + sigma_a sqrt(dt) (L z_s)_a,
+ * Abar = (1/S) sum_{s=1..S} sum_a w_a exp(x_a(t_s)), value e^{-rT} (Abar-K)^+.
+ * Pathwise adjoint of the value: with d_a = sum_s S_a(t_s) and
+ * v_a = sum_s S_a(t_s) dx_a(t_s)/dsigma_a,
+ * dx_a(t_s)/dsigma_a = (x_a(t_s) - x_a(0) - (r + sigma_a^2/2) t_s) / sigma_a,
+ * delta_a = e^{-rT} 1{Abar>K} w_a d_a / (S S_a(0)) and
+ * vega_a = e^{-rT} 1{Abar>K} w_a v_a / S (L held fixed). */
+static double basket_path(const orc_basket_model *m, const double *z, int p, double *x,
+                          double *d, double *v, double *dl, double *vg) {
+  const int A = m->assets, S = m->steps, P = m->paths;
+  const double sq = sqrt(m->dt);
+  for (int a = 0; a < A; ++a) {
+    x[a] = m->x0[a];
+    d[a] = v[a] = 0.0;
+  }
+  for (int s = 0; s < S; ++s) {
+    const double t = (s + 1) * m->dt;
+    for (int a = 0; a < A; ++a) {
+      double c = 0.0;
+      for (int f = 0; f <= a; ++f) c += m->chol[a * A + f] * z[((size_t)s * A + f) * P + p];
+      const double sg = m->vols[a];
+      x[a] += (m->rate - 0.5 * sg * sg) * m->dt + sg * sq * c;
+      const double st = exp(x[a]);
+      d[a] += st;
+      v[a] += st * (x[a] - m->x0[a] - (m->rate + 0.5 * sg * sg) * t) / sg;
+    }
+  }
+  double avg = 0.0;
+  for (int a = 0; a < A; ++a) avg += m->weights[a] * d[a];
+  avg /= S;
+  const double disc = exp(-m->rate * S * m->dt);
+  const int itm = avg > m->strike;
+  for (int a = 0; a < A; ++a) {
+    dl[a] = itm ? disc * m->weights[a] * d[a] / (S * exp(m->x0[a])) : 0.0;
+    vg[a] = itm ? disc * m->weights[a] * v[a] / S : 0.0;
+  }
+  return itm ? disc * (avg - m->strike) : 0.0;
+}
+
+void orc_basket_asian_paths(const orc_basket_model *m, const double *z, double *price_per_path,
+                            double *delta_sum, double *vega_sum) {
+  const int A = m->assets;
+  for (int a = 0; a < A; ++a) delta_sum[a] = vega_sum[a] = 0.0;
+#pragma omp parallel
+  {
+    double *buf = (double *)malloc(sizeof(double) * 5 * A);
+    double *dsum = (double *)calloc(2 * A, sizeof(double));
+#pragma omp for schedule(static)
+    for (int p = 0; p < m->paths; ++p) {
+      price_per_path[p] = basket_path(m, z, p, buf, buf + A, buf + 2 * A, buf + 3 * A, buf + 4 * A);
+      for (int a = 0; a < A; ++a) {
+        dsum[a] += buf[3 * A + a];
+        dsum[A + a] += buf[4 * A + a];
+      }
+    }
+#pragma omp critical
+    for (int a = 0; a < A; ++a) {
+      delta_sum[a] += dsum[a];
+      vega_sum[a] += dsum[A + a];
+    }
+    free(buf);
+    free(dsum);
+  }
+}

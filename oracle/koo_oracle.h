@@ -128,6 +128,28 @@ double orc_ls_second_pass(int n_ex, const double *basis_vars, const int *vars_ea
 void orc_bermudan_paths(const orc_caplet_model *m, const double *z /* (S,F,P) */, int n_ex,
                         const int *ex_steps, double *bv /* (E,2,P) */, double *ev /* (E,P) */,
                         double *num /* (E,P) */);
+
+/* ---- multi-dimensional Black-Scholes: basket Asian ------------------- */
+typedef struct {
+  int assets, steps, paths;
+  const double *x0;      /* log spots */
+  const double *vols;    /* per asset */
+  const double *chol;    /* A x A lower Cholesky factor of the correlation */
+  const double *weights; /* basket weights */
+  double rate, dt, strike;
+} orc_basket_model;
+
+int orc_cholesky_lower(int n, const double *C, double *L);
+void orc_correlated_paths(const double *in /* (S,P,F) */, double *out /* (S,N,P) */,
+                          const double *A, const int *alive, int a_offset_per_step, int factors,
+                          int out_dims, int paths, int steps);
+void orc_multid_bs_paths(const double *log0, const double *incr /* (S,N,P) */,
+                         const double *drifts /* (S,N) or NULL */, double *out /* (S,N,P) */,
+                         int paths, int rates, int steps);
+void orc_basket_weightings(const double *in /* (S,N,P) */, double *out /* (S,P) */,
+                           const double *w, int paths, int stocks, int steps);
+void orc_basket_asian_paths(const orc_basket_model *m, const double *z /* (S,A,P) */,
+                            double *price_per_path, double *delta_sum, double *vega_sum);
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,9 @@
 #include <vector>
 
 #include <gold/Bridge_allocator_gold.h>
+#include <gold/Correlation_gold.h>
 #include <gold/LMM_evolver_gold.h>
+#include <gold/MultiD_Path_Gen_BS_gold.h>
 #include <gold/MultiLS_regression.h>
 #include <gold/Regression_Selector_concrete_gold.h>
 #include <gold/cashFlowDiscounting_gold.h>
@@ -21,6 +23,7 @@
 #include <gold/scramble_gold.h>
 #include <gold/sobol_gold.h>
 #include <gold/volstructs_gold.h>
+#include <gold/weighted_average_gold.h>
 
 template <class T>
 static std::vector<T> vec(const T* p, size_t n) { return std::vector<T>(p, p + n); }
@@ -224,6 +227,37 @@ double ref_ls_multi_depth0(int n_ex, const double* basis_vars, const int* vars_e
       max_vars, cf_mat, ev_mat, nm_mat, paths, 0, 10, selector, use_cross != 0);
   std::memcpy(coeffs_out, &coeffs[0], sizeof(double) * coeffs.size());
   return price;
+}
+
+// gold/Correlation_gold.h:22-43 correlated_paths_gold<double>, every step
+// (the gold loop starts at path 1; path 0 of the output is left at zero).
+void ref_correlated_paths(const double* in, double* out, const double* A, int factors, int out_dims,
+                          int paths, int steps) {
+  std::vector<double> i = vec(in, (size_t)steps * paths * factors);
+  std::vector<double> o((size_t)steps * out_dims * paths, 0.0);
+  std::vector<double> a = vec(A, (size_t)out_dims * factors);
+  for (int s = 0; s < steps; ++s) correlated_paths_gold<double>(i, o, a, factors, out_dims, paths, s);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// gold/MultiD_Path_Gen_BS_gold.h (LINUX copy):85-101 multi_dim_BS_path_generator_nonlog_gold<double>.
+void ref_multid_bs_paths(const double* log0, const double* incr, const double* drifts, double* out,
+                         int paths, int rates, int steps) {
+  std::vector<double> l = vec(log0, rates), c = vec(incr, (size_t)steps * rates * paths), o;
+  std::vector<double> d = vec(drifts, (size_t)steps * rates);
+  multi_dim_BS_path_generator_nonlog_gold<double>(l, c, d, o, paths, rates, steps);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// gold/weighted_average_gold.h:19-51 basketWeightings_gold<double>.
+void ref_basket_weightings(const double* in, double* out, const double* w, int paths, int stocks,
+                           int steps) {
+  std::vector<double> i = vec(in, (size_t)steps * stocks * paths), o((size_t)steps * paths);
+  std::vector<double> wv = vec(w, stocks);
+  CubeConstFacade<double> ic(&i[0], steps, stocks, paths);
+  MatrixFacade<double> om(&o[0], steps, paths);
+  basketWeightings_gold<double>(ic, om, wv, paths, stocks, steps);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
 }
 
 }  // extern "C"

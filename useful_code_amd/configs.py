@@ -107,5 +107,59 @@ def bermudan_swaption(paths=2**18, seed=2025):
                        basis_shift=np.full((ex.size, 2), K), basis_scale=np.full((ex.size, 2), 100.0))
 
 
+@dataclass
+class BasketWorkload:
+    """Multi-asset Black-Scholes basket; variates are (steps, assets, paths)."""
+    name: str
+    paths: int
+    assets: int
+    steps: int
+    maturity: float
+    rate: float
+    spots: np.ndarray
+    vols: np.ndarray
+    corr: np.ndarray
+    chol: np.ndarray  # lower Cholesky factor of corr
+    weights: np.ndarray
+    strike: float
+    scramblers: np.ndarray
+    path_offset: int = 0
+
+    @property
+    def dt(self):
+        return self.maturity / self.steps
+
+    @property
+    def factors(self):
+        return self.assets
+
+    @property
+    def dims(self):
+        return self.steps * self.assets
+
+    def scaled_chol(self):
+        """C[a][f] = vol_a sqrt(dt) L[a][f]."""
+        return self.vols[:, None] * np.sqrt(self.dt) * self.chol
+
+
+def basket_asian(paths=2**20, seed=2027, assets=32, steps=64):
+    """Multi-dimensional Black-Scholes: 32 assets, spots 100 exp(N(0, 0.1)),
+    vols U(0.15, 0.35), rate 0.03, correlation 0.3 constant plus a seeded
+    rank-2 perturbation (unit diagonal), Cholesky-factored; 64 Sobol-bridge
+    steps over one year, 2^20 paths; arithmetic-average basket Asian call at
+    K = 100 with equal weights; 32 deltas and 32 vegas by adjoint."""
+    rng = np.random.RandomState(seed)
+    spots = 100.0 * np.exp(rng.normal(0.0, 0.1, assets))
+    vols = rng.uniform(0.15, 0.35, assets)
+    V = rng.normal(0.0, 0.2, size=(assets, 2))
+    C = 0.3 * np.ones((assets, assets)) + 0.7 * np.eye(assets) + V @ V.T
+    d = np.sqrt(np.diag(C))
+    corr = C / np.outer(d, d)
+    chol = koo.cholesky_lower(corr)
+    return BasketWorkload("basket_asian_32x64", paths, assets, steps, 1.0, 0.03, spots, vols, corr,
+                          chol, np.full(assets, 1.0 / assets), 100.0,
+                          _scramblers(rng, steps * assets))
+
+
 def bridge_allocation(w):
     return koo.bridge_allocate_diagonal(w.factors, w.steps)

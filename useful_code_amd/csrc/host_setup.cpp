@@ -107,6 +107,25 @@ extern "C" int koo_exp_corr(const double* times, int n, double L, double beta, d
   return KOO_OK;
 }
 
+extern "C" int koo_cholesky_lower(int n, const double* corr, double* L) {
+  if (n < 1 || !corr || !L) return set_host_error(KOO_ERR_INVALID, "koo_cholesky_lower: bad arguments");
+  for (int i = 0; i < n * n; ++i) L[i] = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = corr[i * n + j];
+      for (int k = 0; k < j; ++k) s -= L[i * n + k] * L[j * n + k];
+      if (i == j) {
+        if (!(s > 0.0))
+          return set_host_error(KOO_ERR_INVALID,
+                                "koo_cholesky_lower: matrix not positive definite at row %d", i);
+        L[i * n + i] = std::sqrt(s);
+      } else {
+        L[i * n + j] = s / L[j * n + j];
+      }
+    }
+  return KOO_OK;
+}
+
 extern "C" int koo_pseudo_sqrt(int n, const double* cov, int factors, int normalise, double* out) {
   if (n < 1 || factors < 1 || factors > n || !cov || !out)
     return set_host_error(KOO_ERR_INVALID, "koo_pseudo_sqrt: bad arguments");

@@ -96,6 +96,13 @@ _SIGNATURES = {
     "ls_second_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
     "ls_normal_eq_gpu": (_i, [_i, _i] + [_p] * 6),
     "cholesky_solve_f64_gpu": (_i, [_i] + [_p] * 5),
+    # multi-asset Black-Scholes
+    "koo_cholesky_lower": (_i, [_i, _p, _p]),
+    "correlated_paths_multistep_GPU": (_i, [_p] * 4 + [_i] * 5 + [_p]),
+    "multi_dim_BS_path_generator_kernel": (_i, [_p] * 3 + [_i] * 3 + [_p]),
+    "basketWeightings_gpu": (_i, [_p] * 3 + [_i] * 3 + [_p]),
+    "basket_asian_workspace": (_i, [_i, _i, _p]),
+    "basket_asian_gpu": (_i, [_i, _i, _i] + [_p] * 5 + [_f, _f, _f, _p, _p, _p]),
 }
 
 
@@ -509,3 +516,41 @@ def reduceOuterProductSymmetric_cublas(paths, row_size, alpha, beta, inp, answer
 
 def outer_product_workspace_floats(paths):
     return lib().outer_product_workspace_floats(paths)
+
+
+# ---------------------------------------------------------------- multi-asset Black-Scholes
+
+def cholesky_lower(corr):
+    c = _f64(corr)
+    L = np.zeros_like(c)
+    call("koo_cholesky_lower", c.shape[0], c.ctypes.data, L.ctypes.data)
+    return L
+
+
+def correlated_paths_multistep_GPU(d_in, d_out, A, alive, A_offset_per_step, factors,
+                                   out_dimensions, number_paths, steps, stream=None):
+    call("correlated_paths_multistep_GPU", ptr(d_in), ptr(d_out), ptr(A), ptr(alive),
+         A_offset_per_step, factors, out_dimensions, number_paths, steps, stream_handle(stream))
+
+
+def multi_dim_BS_path_generator_kernel(log0, incr, out, paths, rates, steps, stream=None):
+    call("multi_dim_BS_path_generator_kernel", ptr(log0), ptr(incr), ptr(out), paths, rates, steps,
+         stream_handle(stream))
+
+
+def basketWeightings_gpu(inp, out, weights, paths, stocks, steps, stream=None):
+    call("basketWeightings_gpu", ptr(inp), ptr(out), ptr(weights), paths, stocks, steps,
+         stream_handle(stream))
+
+
+def basket_asian_workspace(paths, assets):
+    n = _sz()
+    call("basket_asian_workspace", paths, assets, ctypes.addressof(n))
+    return n.value
+
+
+def basket_asian_gpu(paths, assets, steps, variates, log_spots, vols, scaled_chol, weights, rate,
+                     dt, strike, partials, out, stream=None):
+    call("basket_asian_gpu", paths, assets, steps, ptr(variates), ptr(log_spots), ptr(vols),
+         ptr(scaled_chol), ptr(weights), float(rate), float(dt), float(strike), ptr(partials),
+         ptr(out), stream_handle(stream))

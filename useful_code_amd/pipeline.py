@@ -137,6 +137,43 @@ class BermudanPricer:
         return self.price1, self.price2
 
 
+class BasketPricer:
+    """Basket Asian on one GPU: the fused generator (Sobol -> scramble ->
+    normal -> bridge) into (steps, assets, paths) variates, then the fused
+    path/payoff/adjoint kernel.  ``price_and_greeks`` returns the device
+    vector [price, deltas (A), vegas (A)]."""
+
+    def __init__(self, w, device="cuda"):
+        self.w = w
+        P, A, S = w.paths, w.assets, w.steps
+        self.dirs = _dev(koo.sobol_directions(w.dims).view(np.int32), torch.int32)
+        self.scr = _dev(np.asarray(w.scramblers, np.uint32).view(np.int32), torch.int32)
+        self.alloc = _dev(koo.bridge_allocate_diagonal(A, S), torch.int32)
+        self.z = torch.empty(S * A * P, dtype=torch.float32, device=device)
+        self.x0 = _dev(np.log(w.spots), torch.float32)
+        self.vols = _dev(w.vols, torch.float32)
+        self.lt = _dev(w.scaled_chol(), torch.float32)
+        self.weights = _dev(w.weights, torch.float32)
+        self.partials = torch.empty(koo.basket_asian_workspace(P, A) // 8, dtype=torch.float64,
+                                    device=device)
+        self.out = torch.zeros(1 + 2 * A, dtype=torch.float64, device=device)
+
+    def generate(self):
+        w = self.w
+        koo.sobol_bridge_gpu(w.paths, w.path_offset, w.steps, w.assets, self.dirs, self.scr,
+                             w.dims, self.alloc, self.z)
+
+    def greeks_kernel(self):
+        w = self.w
+        koo.basket_asian_gpu(w.paths, w.assets, w.steps, self.z, self.x0, self.vols, self.lt,
+                             self.weights, w.rate, w.dt, w.strike, self.partials, self.out)
+
+    def price_and_greeks(self):
+        self.generate()
+        self.greeks_kernel()
+        return self.out
+
+
 def lmm_evolution_routine(w, use_fused_generator=False):
     """Staged pipeline of LMMEvolutionRoutineRawSingleKernel: Sobol integers
     for D = S*F dimensions, scramble of the first S rows (as the reference
