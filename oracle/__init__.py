@@ -57,6 +57,11 @@ def ref():
     return _REF
 
 
+class FrozenStrategy(ctypes.Structure):
+    _fields_ = [("n_ex", _c_int), ("ex_steps", ctypes.c_void_p), ("coeffs", ctypes.c_void_p),
+                ("shift", ctypes.c_void_p), ("scale", ctypes.c_void_p), ("itm_only", _c_int)]
+
+
 class BasketModel(ctypes.Structure):
     _fields_ = [("assets", _c_int), ("steps", _c_int), ("paths", _c_int),
                 ("x0", ctypes.c_void_p), ("vols", ctypes.c_void_p), ("chol", ctypes.c_void_p),
@@ -126,6 +131,9 @@ def _declare(L):
         "orc_bermudan_paths": (None, [ctypes.POINTER(CapletModel), _f64p, _c_int, _i32p, _f64p,
                                       _f64p, _f64p]),
         "orc_cholesky_lower": (_c_int, [_c_int, _f64p, _f64p]),
+        "orc_bermudan_frozen_paths": (None, [ctypes.POINTER(CapletModel),
+                                             ctypes.POINTER(FrozenStrategy), _f64p, _c_int, _f64p,
+                                             ctypes.c_void_p, _f64p, _f64p]),
         "orc_correlated_paths": (None, [_f64p, _f64p, _f64p, _i32p, _c_int, _c_int, _c_int,
                                         _c_int, _c_int]),
         "orc_multid_bs_paths": (None, [_f64p, _f64p, ctypes.c_void_p, _f64p, _c_int, _c_int,
@@ -474,3 +482,26 @@ def basket_asian_paths(model, z):
     v = np.zeros(model.assets)
     lib().orc_basket_asian_paths(ctypes.byref(model), z.reshape(-1), pp, d, v)
     return pp, d, v
+
+
+# ---------------------------------------------------------------- frozen Bermudan
+
+def frozen_strategy(ex_steps, coeffs, shift, scale, itm_only=True):
+    arrs = [np.ascontiguousarray(ex_steps, np.int32)] + \
+        [np.ascontiguousarray(a, np.float64) for a in (coeffs, shift, scale)]
+    fs = FrozenStrategy(len(arrs[0]), arrs[0].ctypes.data, arrs[1].ctypes.data,
+                        arrs[2].ctypes.data, arrs[3].ctypes.data, int(itm_only))
+    fs._keep = arrs
+    return fs
+
+
+def bermudan_frozen_paths(model, strategy, z, adjoint=True):
+    """-> (value per path, exercise step per path (-1: never), delta sums, vega sums)."""
+    z = np.ascontiguousarray(z, np.float64)
+    vals = np.zeros(model.paths)
+    ex = np.zeros(model.paths, np.int32)
+    d = np.zeros(model.n_rates)
+    v = np.zeros(model.n_rates)
+    lib().orc_bermudan_frozen_paths(ctypes.byref(model), ctypes.byref(strategy), z.reshape(-1),
+                                    int(adjoint), vals, ex.ctypes.data, d, v)
+    return vals, ex, d, v

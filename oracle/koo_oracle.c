@@ -1329,3 +1329,152 @@ void orc_basket_asian_paths(const orc_basket_model *m, const double *z, double *
     free(dsum);
   }
 }
+
+/* ======================================================================
+ * Bermudan swaption under a frozen exercise strategy: price and pathwise
+ * adjoint Greeks (no reference counterpart; the strategy is the LS one of
+ * the first pass, frozen as in the reference's second pass,
+ * cashFlowGen_earlyExProduct_gpu.h:28-172).
+ *
+ * The path exercises at the first exercise date e with ev_e > 0 (when
+ * itm_only) and ev_e > cont_e, cont_e = beta_e . phi((x - shift) * scale),
+ * phi the quadratic basis with cross term in x = (SR, f_t).  Its value is
+ * V = ev_tau / B_tau, ev = 1 - P_N - K A at T_t (t the exercise step),
+ * P_{j+1} = P_j / (1 + tau_j f_j), A = sum_j tau_j P_{j+1}.  With the decision
+ * held fixed: dV/df_j(T_t) = g_j (P_N + K Q_j) / B_t, g_j = tau_j / (1 + tau_j f_j),
+ * Q_j = sum_{m > j} tau_{m-1} P_m, and dV/df_s(T_s) = -V g_s for the fixings
+ * s < t inside B_t; then the PC steps t..0 are reversed as for the caplets.
+ * ==================================================================== */
+static double bermudan_path_adjoint(const orc_caplet_model *m, const orc_frozen_strategy *fs,
+                                    const double *z, int p, double *states, double *lam,
+                                    double *delta, double *vw, step_rate *rec, double *erec,
+                                    double *eprec, int do_adjoint, int *tau_out) {
+  const int N = m->n_rates, F = m->factors, P = m->paths;
+  double *Bs = states + (size_t)(N + 1) * N;
+  for (int r = 0; r < N; ++r) states[r] = log(m->f0[r] + m->disp[r]);
+  double B = 1.0, V = 0.0, zs[64];
+  int t = -1, e = 0;
+  const int last = fs->ex_steps[fs->n_ex - 1];
+  for (int s = 0; s <= last && t < 0; ++s) {
+    for (int f = 0; f < F; ++f) zs[f] = z[((size_t)s * F + f) * P + p];
+    double *Ls = states + (size_t)s * N, *Ln = states + (size_t)(s + 1) * N;
+    for (int r = 0; r < N; ++r) Ln[r] = Ls[r];
+    pc_step(m, s, Ls, zs, Ln, NULL, NULL, NULL);
+    Bs[s] = B;
+    const double fsx = exp(Ln[s]) - m->disp[s];
+    if (s == fs->ex_steps[e]) {
+      double disc = 1.0, ann = 0.0;
+      for (int j = s; j < N; ++j) {
+        disc /= 1.0 + m->taus[j] * (exp(Ln[j]) - m->disp[j]);
+        ann += m->taus[j] * disc;
+      }
+      const double sr = (1.0 - disc) / ann;
+      const double ev = (sr - m->strike) * ann;
+      const double x[2] = {(sr - fs->shift[2 * e]) * fs->scale[2 * e],
+                           (fsx - fs->shift[2 * e + 1]) * fs->scale[2 * e + 1]};
+      double phi[6];
+      orc_basis_functions(x, 2, 1, phi);
+      double cont = 0.0;
+      for (int i = 0; i < 6; ++i) cont += phi[i] * fs->coeffs[6 * e + i];
+      const double evp = ev > 0.0 ? ev : 0.0;
+      const int allowed = !fs->itm_only || evp > 0.0;
+      if (allowed && cont - evp < 0.0) {
+        t = s;
+        V = evp / B;
+      }
+      ++e;
+    }
+    B *= 1.0 + m->taus[s] * fsx;
+  }
+  if (tau_out) *tau_out = t;
+  if (!do_adjoint || t < 0 || !(V > 0.0)) return V;
+  /* payoff adjoint on the log-rates at the end of step t */
+  for (int r = 0; r < N; ++r) lam[r] = 0.0;
+  {
+    const double *Le = states + (size_t)(t + 1) * N;
+    double Pm[128];
+    Pm[t] = 1.0;
+    for (int j = t; j < N; ++j) Pm[j + 1] = Pm[j] / (1.0 + m->taus[j] * (exp(Le[j]) - m->disp[j]));
+    double Q = 0.0;
+    for (int j = N - 1; j >= t; --j) {
+      const double fj = exp(Le[j]) - m->disp[j];
+      const double g = m->taus[j] / (1.0 + m->taus[j] * fj);
+      Q += m->taus[j] * Pm[j + 1]; /* Q_j = sum_{m > j} tau_{m-1} P_m */
+      lam[j] = g * (Pm[N] + m->strike * Q) / Bs[t] * exp(Le[j]);
+    }
+  }
+  for (int s = t; s >= 0; --s) {
+    for (int f = 0; f < F; ++f) zs[f] = z[((size_t)s * F + f) * P + p];
+    const double *Ls = states + (size_t)s * N;
+    double Lend[64];
+    pc_step(m, s, Ls, zs, Lend, rec, erec, eprec);
+    if (s < t) { /* the fixing f_s(T_s) inside B_t */
+      const double xs = exp(Lend[s]), fsx = xs - m->disp[s], tau = m->taus[s];
+      lam[s] += -tau / (1.0 + tau * fsx) * V * xs;
+    }
+    const double *As = m->A + (size_t)s * N * F;
+    double Eh[64], E[64];
+    for (int f = 0; f < F; ++f) Eh[f] = E[f] = 0.0;
+    for (int r = N - 1; r >= s; --r) {
+      const double *Ar = As + r * F;
+      const double d = m->disp[r];
+      const double lp = lam[r];
+      const step_rate q = rec[r];
+      double gph = 0.0;
+      for (int f = 0; f < F; ++f) {
+        Eh[f] += 0.5 * lp * Ar[f];
+        gph += Ar[f] * Eh[f];
+      }
+      const double fh = q.xp - d;
+      const double lh = lp + gph * q.gp * (1.0 - m->taus[r] * d) / (1.0 + m->taus[r] * fh);
+      const double mub = lh - 0.5 * lp;
+      double gb = 0.0;
+      for (int f = 0; f < F; ++f) {
+        E[f] += mub * Ar[f];
+        gb += Ar[f] * E[f];
+      }
+      const double fr = q.x - d;
+      const double lnew = lh + gb * q.g * (1.0 - m->taus[r] * d) / (1.0 + m->taus[r] * fr);
+      double acc = 0.0;
+      for (int f = 0; f < F; ++f) {
+        const double abar = 0.5 * lp * eprec[r * F + f] + q.gp * Eh[f] + mub * erec[r * F + f] +
+                            q.g * E[f] + lh * zs[f] - lh * Ar[f];
+        acc += abar * Ar[f];
+      }
+      vw[r] += acc;
+      lam[r] = lnew;
+    }
+  }
+  for (int r = 0; r < N; ++r) delta[r] += lam[r] / (m->f0[r] + m->disp[r]);
+  return V;
+}
+
+void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strategy *fs,
+                               const double *z, int do_adjoint, double *value_per_path,
+                               int *exercise_step, double *delta_sum, double *vega_sum) {
+  const int N = m->n_rates, F = m->factors;
+  for (int r = 0; r < N; ++r) delta_sum[r] = vega_sum[r] = 0.0;
+#pragma omp parallel
+  {
+    double *states = (double *)malloc(sizeof(double) * ((size_t)(N + 1) * N + N));
+    double *lam = (double *)malloc(sizeof(double) * N);
+    double *dl = (double *)calloc(N, sizeof(double)), *vw = (double *)calloc(N, sizeof(double));
+    step_rate *rec = (step_rate *)malloc(sizeof(step_rate) * N);
+    double *erec = (double *)malloc(sizeof(double) * N * F * 2), *eprec = erec + N * F;
+#pragma omp for schedule(static)
+    for (int p = 0; p < m->paths; ++p)
+      value_per_path[p] = bermudan_path_adjoint(m, fs, z, p, states, lam, dl, vw, rec, erec, eprec,
+                                                do_adjoint, exercise_step ? exercise_step + p : NULL);
+#pragma omp critical
+    for (int r = 0; r < N; ++r) {
+      delta_sum[r] += dl[r];
+      vega_sum[r] += vw[r] / m->sigma[r];
+    }
+    free(states);
+    free(lam);
+    free(dl);
+    free(vw);
+    free(rec);
+    free(erec);
+  }
+}

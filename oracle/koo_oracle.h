@@ -129,6 +129,21 @@ void orc_bermudan_paths(const orc_caplet_model *m, const double *z /* (S,F,P) */
                         const int *ex_steps, double *bv /* (E,2,P) */, double *ev /* (E,P) */,
                         double *num /* (E,P) */);
 
+/* ---- Bermudan swaption under a frozen strategy: price and adjoint ---- */
+typedef struct {
+  int n_ex;
+  const int *ex_steps;
+  const double *coeffs; /* (E, 6): quadratic cross basis in the normalised variables */
+  const double *shift;  /* (E, 2) */
+  const double *scale;  /* (E, 2) */
+  int itm_only;
+} orc_frozen_strategy;
+
+void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strategy *fs,
+                               const double *z /* (S,F,P) */, int do_adjoint,
+                               double *value_per_path, int *exercise_step /* or NULL */,
+                               double *delta_sum /* N */, double *vega_sum /* N */);
+
 /* ---- multi-dimensional Black-Scholes: basket Asian ------------------- */
 typedef struct {
   int assets, steps, paths;

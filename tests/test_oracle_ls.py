@@ -176,3 +176,60 @@ def test_ls_second_pass_on_the_regression_paths_equals_the_first_pass():
         p1, coeffs = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=itm)
         p2 = O.ls_second_pass(bv, ves, cash, ev, num, coeffs, itm_only=itm)
         assert abs(p1 - p2) <= 1e-14 * p1
+
+
+def _frozen_setup(P=2048, seed=21):
+    m, z, ex, f0, A = _small_bermudan(P=P, seed=seed)
+    bv, ev, num = O.bermudan_paths(m, z, ex)
+    E = len(ex)
+    shift, scale = np.full((E, 2), 0.05), np.full((E, 2), 100.0)
+    bvn = (bv - shift[:, :, None]) * scale[:, :, None]
+    _, coeffs = O.ls_first_pass(bvn, np.full(E, 2, np.int32), np.zeros_like(ev), ev, num,
+                                itm_only=True)
+    return m, z, ex, f0, A, bv, ev, num, coeffs, shift, scale
+
+
+def test_frozen_bermudan_value_equals_second_pass():
+    m, z, ex, f0, A, bv, ev, num, coeffs, shift, scale = _frozen_setup()
+    E, P = ev.shape
+    fs = O.frozen_strategy(ex, coeffs, shift, scale)
+    vals, exs, _, _ = O.bermudan_frozen_paths(m, fs, z, adjoint=False)
+    bvn = (bv - shift[:, :, None]) * scale[:, :, None]
+    want = O.ls_second_pass(bvn, np.full(E, 2, np.int32), np.zeros_like(ev), ev, num, coeffs,
+                            itm_only=True)
+    assert abs(vals.mean() - want) <= 1e-13 * want
+    assert set(np.unique(exs)) <= set([-1] + list(ex))
+
+
+def test_frozen_bermudan_adjoint_matches_central_fd_with_frozen_decisions():
+    m, z, ex, f0, A, bv, ev, num, coeffs, shift, scale = _frozen_setup(P=512, seed=22)
+    N, P = f0.size, z.shape[2]
+    taus, disp, vols = np.full(N, 0.5), np.zeros(N), np.full(N, 0.2)
+    fs = O.frozen_strategy(ex, coeffs, shift, scale)
+    vals, ex0, d, v = O.bermudan_frozen_paths(m, fs, z)
+    h = 1e-7
+    for i in range(N):
+        res = []
+        for sgn in (1, -1):
+            fb = f0.copy()
+            fb[i] += sgn * h
+            mb = O.caplet_model(fb, taus, disp, A, vols, m.strike, P)
+            vb, exb, _, _ = O.bermudan_frozen_paths(mb, fs, z, adjoint=False)
+            assert np.array_equal(exb, ex0)  # no decision moves at this bump size
+            res.append(vb.sum())
+        fd = (res[0] - res[1]) / (2 * h)
+        assert abs(fd - d[i]) <= 1e-6 * np.max(np.abs(d)) + 1e-9
+    # vegas: A(s, r, f) scales with sigma_r (flat-vol pseudo-roots are linear in the vols)
+    for i in range(N):
+        res = []
+        for sgn in (1, -1):
+            Ab = A.copy()
+            Ab[:, i, :] *= (0.2 + sgn * h) / 0.2
+            vb_ = vols.copy()
+            vb_[i] += sgn * h
+            mb = O.caplet_model(f0, taus, disp, Ab, vb_, m.strike, P)
+            vb, exb, _, _ = O.bermudan_frozen_paths(mb, fs, z, adjoint=False)
+            assert np.array_equal(exb, ex0)
+            res.append(vb.sum())
+        fv = (res[0] - res[1]) / (2 * h)
+        assert abs(fv - v[i]) <= 1e-6 * np.max(np.abs(v)) + 1e-9
