@@ -136,6 +136,52 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
     return res
 
 
+def swap_section(args, torch, O, configs, pipeline, ev):
+    """Plain-LMM payer swap (20 rates, 3 factors, 2^15 paths) through the
+    staged reference-named kernels, with the reference's own CPU pipeline
+    (oracle/_ref, one core) timed beside it when it is present."""
+    from useful_code_amd import koo
+    w = configs.payer_swap()
+    P, S = w.paths, w.steps
+    pr = pipeline.SwapPricer(w)
+    for _ in range(args.warmup):
+        pr.price()
+    recs = [(ev(), ev()) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    for a, b in recs:
+        a.record()
+        pr.price()
+        b.record()
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in recs]))
+    price = pr.price_out.item()
+    model_free = float(np.sum(w.tau * (w.f0 - w.strike) * np.cumprod(1 / (1 + w.tau * w.f0))))
+    res = {
+        "workload": "lmm_payer_swap_20x3: 2^15 scrambled Sobol paths, 20 rates = 20 steps, "
+                    "3 factors, payer swap K=0.05, staged pipeline of reference-named kernels",
+        "paths": P,
+        "gpu_ms": ms,
+        "path_steps_per_s": P * S / (ms * 1e-3),
+        "price": price,
+        "model_free_value": model_free,
+    }
+    if O.ref_available():
+        pw = int(np.ceil(np.log2(S)))
+        scr = np.random.RandomState(7).randint(0, 2**32, size=(1 << pw) * w.factors,
+                                               dtype=np.uint64).astype(np.uint32)
+        t = time.perf_counter()
+        rp = O.ref_swap_pipeline(P, w.f0, w.taus, w.A, koo.fixed_drifts(w.A), w.strike, scr, pw)
+        dt = time.perf_counter() - t
+        res["cpu_reference"] = {
+            "value": P * S / dt, "unit": "path-steps/s", "cores": 1, "kind": "reference",
+            "seconds": dt, "price": rp,
+            "sample": "the reference's gold LMMEvolutionRoutineGold + curve, swap-flow and "
+                      "discounting routines (compiled from the reference sources into "
+                      "oracle/_ref), the same workload with its own Sobol/bridge choices",
+        }
+    return res
+
+
 def basket_section(args, torch, O, configs, pipeline, ev):
     """Multi-asset Black-Scholes basket Asian (32 assets, 64 bridge steps,
     2^20 paths): price + 32 deltas + 32 vegas per job; roofline of the fused
@@ -297,6 +343,7 @@ def main():
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--no-bermudan", action="store_true")
     ap.add_argument("--no-basket", action="store_true")
+    ap.add_argument("--no-swap", action="store_true")
     ap.add_argument("--basket-paths", type=int, default=1 << 20)
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the per-stage HBM measurements of the reference-ABI kernels")
@@ -486,6 +533,9 @@ def main():
 
     if not args.no_bermudan:
         out["bermudan_ls"] = bermudan_section(args, torch, O, configs, pipeline, ev)
+
+    if not args.no_swap:
+        out["payer_swap_config1"] = swap_section(args, torch, O, configs, pipeline, ev)
 
     if not args.no_basket:
         out["basket_asian"] = basket_section(args, torch, O, configs, pipeline, ev)

@@ -527,6 +527,41 @@ int basket_asian_gpu(int paths, int assets, int steps, const float* variates,
                      const float* weights, float rate, float dt, float strike, double* partials,
                      double* out, void* stream);
 
+/* =====================================================================
+ * Bermudan payer swaption under a frozen exercise strategy: price and
+ * pathwise adjoint Greeks (new; the frozen strategy is the reference's
+ * second pass, cashFlowGen_earlyExProduct_gpu.h:28-172, and the adjoint is
+ * the reference's open task doc/todo.txt:10).
+ *
+ * Same model and variates as lmm_price_fwd.  The path exercises at the first
+ * exercise step whose exercise value ev = (SR - K) A is positive (when
+ * itm_only) and exceeds cont = beta_e . phi(x~), phi the quadratic basis with
+ * cross term of x~ = ((SR - shift[e][0]) scale[e][0], (f_t - shift[e][1])
+ * scale[e][1]) -- the coefficients ls_first_pass_gpu returns.  price_out is
+ * the mean of ev / B at exercise; greeks_out[0..N) = dV/df_i(0),
+ * [N..2N) = dV/dsigma_i (A proportional to sigma_i), with the exercise
+ * decisions held fixed.  exercise_steps (increasing, <= 40 dates),
+ * coefficients (E x 6), var_shift and var_scale (E x 2, NULL for 0 / 1) are
+ * HOST arrays; buffers come from bermudan_adjoint_workspace().  The backward
+ * reads the path buffers the forward wrote.
+ * =================================================================== */
+int bermudan_adjoint_workspace(int paths, int rates, int factors, int checkpoint_every,
+                               size_t* checkpoint_bytes, size_t* scratch_bytes,
+                               size_t* path_bytes, size_t* partial_bytes);
+int bermudan_frozen_fwd(int paths, int rates, int factors, int checkpoint_every,
+                        const float* variates, const float* initial_rates, const float* taus,
+                        const float* displacements, const float* pseudo_roots, float strike,
+                        int n_ex, const int* exercise_steps, const double* coefficients,
+                        const double* var_shift, const double* var_scale, int itm_only,
+                        float* checkpoints, float* path_buffers, double* partials,
+                        double* price_out, void* stream);
+int bermudan_frozen_bwd(int paths, int rates, int factors, int checkpoint_every,
+                        const float* variates, const float* initial_rates, const float* taus,
+                        const float* displacements, const float* pseudo_roots, const float* vols,
+                        float strike, const float* checkpoints, const float* path_buffers,
+                        float* segment_scratch, double* partials, double* greeks_out,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -180,6 +180,8 @@ def _declare_ref(R):
         "ref_correlated_paths": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _c_int]),
         "ref_multid_bs_paths": (None, [_f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _c_int]),
         "ref_basket_weightings": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int]),
+        "ref_swap_pipeline": (_c_dbl, [_c_int, _c_int, _c_int, _c_int, _u32p, _f64p, _f64p, _f64p,
+                                       _f64p, _f64p, _c_dbl]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(R, name)
@@ -505,3 +507,18 @@ def bermudan_frozen_paths(model, strategy, z, adjoint=True):
     lib().orc_bermudan_frozen_paths(ctypes.byref(model), ctypes.byref(strategy), z.reshape(-1),
                                     int(adjoint), vals, ex.ctypes.data, d, v)
     return vals, ex, d, v
+
+
+def ref_swap_pipeline(paths, f0, taus, A, fixed_drifts, strike, scramblers, power_of_two,
+                      disp=None):
+    """The reference's own CPU pipeline for the plain-LMM payer swap
+    (LMMEvolutionRoutineGold + gold curve, flow and discounting routines);
+    returns its price.  scramblers: 2^power_of_two * factors words."""
+    A = np.ascontiguousarray(A, np.float64)
+    S, N, F = A.shape
+    d = np.zeros(N) if disp is None else np.ascontiguousarray(disp, np.float64)
+    return ref().ref_swap_pipeline(paths, N, F, power_of_two,
+                                   np.ascontiguousarray(scramblers, np.uint32), A.reshape(-1),
+                                   np.ascontiguousarray(fixed_drifts, np.float64).reshape(-1), d,
+                                   np.ascontiguousarray(f0, np.float64),
+                                   np.ascontiguousarray(taus, np.float64), float(strike))

@@ -12,6 +12,7 @@
 
 #include <gold/Bridge_allocator_gold.h>
 #include <gold/Correlation_gold.h>
+#include <gold/LMM_evolver_full_gold.h>
 #include <gold/LMM_evolver_gold.h>
 #include <gold/MultiD_Path_Gen_BS_gold.h>
 #include <gold/MultiLS_regression.h>
@@ -258,6 +259,49 @@ void ref_basket_weightings(const double* in, double* out, const double* w, int p
   MatrixFacade<double> om(&o[0], steps, paths);
   basketWeightings_gold<double>(ic, om, wv, paths, stocks, steps);
   std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// The reference's own CPU pipeline for the plain-LMM payer swap:
+// gold/LMM_evolver_full_gold.h:140-230 LMMEvolutionRoutineGold<double> (Sobol,
+// scramble, inverse normal, power-of-two bridge, correlate + drift, PC
+// evolver), then discount ratios, spot numeraires, extraction of the rate
+// fixing at each step, swap flows and their discounting (the gold routines
+// wrapped above).  Returns the mean discounted value; the timing of this call
+// is the reference CPU baseline of the workload.
+double ref_swap_pipeline(int paths, int rates, int factors, int power_of_two,
+                         const unsigned* scramblers, const double* A, const double* fixed_drifts,
+                         const double* disp, const double* f0, const double* taus, double strike) {
+  const int S = rates;
+  std::vector<unsigned> scr(scramblers, scramblers + (1 << power_of_two) * factors);
+  std::vector<double> a = vec(A, (size_t)S * rates * factors), fd = vec(fixed_drifts, (size_t)S * rates);
+  std::vector<double> d = vec(disp, rates), f = vec(f0, rates), t = vec(taus, rates), lf(rates);
+  for (int r = 0; r < rates; ++r) lf[r] = std::log(f[r] + d[r]);
+  std::vector<double> er, el;
+  LMMEvolutionRoutineGold<double>(paths, 0, rates, factors, S, power_of_two, scr, a, fd, d, f, lf, t,
+                                  er, el);
+  std::vector<int> alive(S);
+  for (int s = 0; s < S; ++s) alive[s] = s;
+  std::vector<double> disc((size_t)S * (rates + 1) * paths, 0.0), num((size_t)S * paths, 0.0);
+  discount_ratios_computation_gold<double>(er, t, alive, paths, S, rates, disc, 0);
+  spot_measure_numeraires_computation_offset_gold<double>(disc, paths, paths, 0, rates, num);
+  std::vector<int> idx(S);
+  for (int s = 0; s < S; ++s) idx[s] = s;
+  std::vector<double> r1((size_t)S * paths, 0.0);
+  forward_rate_extraction_gold<double>(er, idx, paths, S, rates, r1, 0);
+  std::vector<double> aux(2 + 2 * S);
+  aux[0] = strike;
+  aux[1] = 1.0;
+  for (int s = 0; s < S; ++s) aux[2 + s] = aux[2 + S + s] = t[s];
+  std::vector<double> f1((size_t)S * paths), f2((size_t)S * paths), dummy(1, 0.0);
+  cashFlowGeneratorCallerSwap_gold<double>(f1, f2, aux, paths, S, r1, r1, r1, dummy, dummy);
+  std::vector<int> first(S), second(S);
+  std::vector<double> th(S, 0.0), o, s1, s2;
+  for (int s = 0; s < S; ++s) first[s] = second[s] = s + 1;
+  cashFlowDiscounting_gold<double>(first, second, th, disc, f1, num, paths, S, o, s1, 0);
+  cashFlowDiscounting_gold<double>(first, second, th, disc, f2, num, paths, S, o, s2, 0);
+  double tot = 0.0;
+  for (int p = 0; p < paths; ++p) tot += s1[p] + s2[p];
+  return tot / paths;
 }
 
 }  // extern "C"

@@ -348,3 +348,28 @@ def test_continuation_update_and_deflation(torch_cuda):
     y = dev(torch, cont0)
     koo.updateDeflationOfContinuationValues_gpu(y, dev(torch, num), P, 1, 3)
     assert np.allclose(y.cpu().numpy(), cont0 * num[1] / num[3], rtol=1e-6)
+
+
+# ---------------------------------------------------------------- frozen-strategy adjoint
+
+@pytest.mark.parametrize("k", [4, 1, 20])
+def test_frozen_bermudan_price_and_greeks_vs_fp64_oracle(torch_cuda, berm, k):
+    w, pr, z, bv, ev, num = berm
+    torch = torch_cuda
+    pr.regress()
+    price, greeks = pr.greeks(checkpoint_every=k)
+    torch.cuda.synchronize()
+    coeffs = pr.coeffs.cpu().numpy().reshape(pr.E, 6)
+    N, P = w.n_rates, w.paths
+    fs = O.frozen_strategy(w.exercise_steps, coeffs, w.basis_shift, w.basis_scale)
+    vals, ex_o, d, v = O.bermudan_frozen_paths(_oracle_model(w), fs, z)
+    tau_gpu = pr.adj[3].view(torch.int32)[N * P:(N + 1) * P].cpu().numpy()
+    flips = np.mean(tau_gpu != ex_o)
+    g = greeks.cpu().numpy()
+    g64 = np.concatenate([d, v]) / P
+    err = np.max(np.abs(g - g64)) / np.max(np.abs(g64))
+    print(f"k={k}: frozen price gpu={price.item():.10f} fp64={vals.mean():.10f}; "
+          f"decision mismatches {flips:.2e}; greeks max rel err {err:.2e}")
+    assert flips < 1e-3
+    assert abs(price.item() - vals.mean()) < 1e-5
+    assert err < 2e-3

@@ -159,3 +159,37 @@ def test_full_sums(torch_cuda, n):
     final = torch.zeros(1, dtype=torch.float32, device="cuda")
     koo.reduceSinglePass(n, f32(torch, x), wsp.view(torch.float32), final)
     assert abs(final.item() - exact) <= 1e-6 * max(1.0, abs(exact))
+
+
+def test_config1_payer_swap_end_to_end(torch_cuda):
+    """The plain-LMM payer swap through the staged reference-named kernels
+    against the FP64 oracle chain on the same Sobol integers (1e-6 absolute on
+    a price of about 1e-2) and against the model-free value
+    sum_i tau (f_i(0) - K) P(T_0, T_{i+1}) (Sobol error plus the
+    predictor-corrector bias, 1e-4 absolute)."""
+    from useful_code_amd import configs, pipeline
+    w = configs.payer_swap()
+    S, N, F, P = w.steps, w.n_rates, w.factors, w.paths
+    got = pipeline.SwapPricer(w).price().item()
+    ints = O.scramble(O.sobol_ints(P, S * F, 0), w.scramblers, S)
+    z = O.bridge_transform(O.invnorm_u32(ints), S, F)
+    rates, _, disc = O.lmm_pc_evolve(w.f0, w.taus, z, w.A, w.disp(), w.alive())
+    L = O.lib()
+    num = np.zeros(S * P)
+    L.orc_spot_numeraires(np.ascontiguousarray(disc).reshape(-1), P, N, num)
+    r1 = np.ascontiguousarray(np.stack([rates[s, s] for s in range(S)])).reshape(-1)
+    aux = np.concatenate([[w.strike, 1.0], w.taus, w.taus])
+    f1, f2 = np.zeros(S * P), np.zeros(S * P)
+    L.orc_swap_flows(f1, f2, aux, P, S, r1)
+    first = np.arange(1, S + 1, dtype=np.int32)
+    tot = 0.0
+    for fl in (f1, f2):
+        d, sm = np.zeros(S * P), np.zeros(P)
+        L.orc_cash_flow_discounting(first, first, np.zeros(S), np.ascontiguousarray(disc).reshape(-1),
+                                    fl, num, P, S, d, sm)
+        tot += sm.sum()
+    want = tot / P
+    model_free = np.sum(0.5 * (w.f0 - w.strike) * np.cumprod(1 / (1 + 0.5 * w.f0)))
+    print(f"config 1 swap: gpu={got:.10f} fp64={want:.10f} model-free={model_free:.10f}")
+    assert abs(got - want) < 1e-6
+    assert abs(got - model_free) < 1e-4

@@ -1,0 +1,380 @@
+// berm_adjoint.hip -- Bermudan payer swaption under a frozen exercise
+// strategy: price and pathwise adjoint Greeks, gfx950.
+//
+// Forward (k_berm_fwd): one lane per path evolves the curve (the shared PC
+// step, state in LDS), checkpoints the state every k steps, and at each
+// exercise date applies the frozen Longstaff-Schwartz rule (exercise when the
+// exercise value is positive, if asked, and exceeds the continuation value of
+// the given coefficients).  At exercise it records the step, the deflated
+// value and the payoff's adjoint on the end-of-step log-rates,
+// d(ev/B)/dl_r = g_r (P_N + K Q_r) / B e^{l_r}, g_r = tau_r / (1 + tau_r f_r),
+// Q_r = sum_{m>r} tau_{m-1} P_m (two passes over the alive rates).  A
+// wavefront stops once all its paths have exercised.
+//
+// Backward (k_berm_bwd): the caplet backward's segment recompute and reverse
+// step (lmm_step.h), run per wavefront only up to its last exercise step; at a
+// path's exercise step the recorded payoff adjoint seeds lambda, and each
+// earlier fixing f_s(T_s) inside the numeraire adds -V g_s e^{l_s}.
+#include "lmm_step.h"
+
+namespace koo {
+namespace {
+
+using namespace lmm;
+
+constexpr int kMaxBermDates = 40;
+constexpr int kVegaSlots = 8;
+constexpr int kStatePrefetch = 8;
+
+struct BermArgs {
+  int E, itm_only, last;
+  signed char e_of_step[kMaxRates];  // exercise index of each step, -1 if none
+  float coeff[kMaxBermDates][6];     // quadratic cross basis in the normalised variables
+  float sh[kMaxBermDates][2], sc[kMaxBermDates][2];
+};
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_berm_fwd(KOO_MODEL_PARAMS, const BermArgs b, float* __restrict__ ckpt,
+               float* __restrict__ lamt, int* __restrict__ tau_out, float* __restrict__ vout,
+               double* __restrict__ part) {
+  KOO_MODEL_LOCAL;
+  extern __shared__ float lds[];
+  __shared__ double red[4];
+  const int bd = blockDim.x;
+  const int p = blockIdx.x * bd + threadIdx.x;
+  const bool valid = p < m.P;
+  const int pc = valid ? p : m.P - 1;
+  float* L = lds + threadIdx.x;
+  float* ctab = lds + static_cast<size_t>(m.N) * bd;
+  const size_t P = static_cast<size_t>(m.P);
+  fill_fixed_drifts<F>(m, ctab);
+  for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
+  __syncthreads();
+  float invB = 1.0f, V = 0.0f;
+  int tau = -1;
+  bool done = !valid;
+  const int rows = m.k == 1 ? m.N + 1 + 2 * F : m.N + 1;
+  for (int s = 0; s <= b.last; ++s) {
+    if (__all(done)) break;  // the wavefront's paths have all exercised
+    const bool save = ckpt && valid && (s % m.k) == 0;
+    float* c = save ? ckpt + static_cast<size_t>(s / m.k) * rows * P + p : nullptr;
+    if (save) {
+      for (int r = s; r < m.N; ++r) c[static_cast<size_t>(r) * P] = L[r * bd];
+      c[static_cast<size_t>(m.N) * P] = invB;
+    }
+    float z[F], e[F], ep[F];
+    load_z<F>(m, s, pc, z);
+    pc_step<F>(m, s, ctab + s * m.N, z, e, ep, [&](int r) { return L[r * bd]; },
+               [&](int r, float v) { L[r * bd] = v; });
+    if (save && m.k == 1) {
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        c[static_cast<size_t>(m.N + 1 + f) * P] = e[f];
+        c[static_cast<size_t>(m.N + 1 + F + f) * P] = ep[f];
+      }
+    }
+    const float fs = fast_exp(L[s * bd]) - m.disp[s];
+    const int ei = b.e_of_step[s];
+    if (ei >= 0 && !done) {
+      float disc = 1.0f, ann = 0.0f, flt = 0.0f;
+      for (int r = s; r < m.N; ++r) {
+        const float fr = fast_exp(L[r * bd]) - m.disp[r];
+        const float tr = m.taus[r];
+        disc *= fast_rcp(1.0f + tr * fr);
+        ann += tr * disc;
+        flt += tr * fr * disc;
+      }
+      const float sr = flt / ann;
+      const float ev = (sr - m.strike) * ann;
+      const float x0 = (sr - b.sh[ei][0]) * b.sc[ei][0];
+      const float x1 = (fs - b.sh[ei][1]) * b.sc[ei][1];
+      const float* cf = b.coeff[ei];
+      const float cont = cf[0] + cf[1] * x0 + cf[2] * x1 + cf[3] * x0 * x0 + cf[4] * x0 * x1 +
+                         cf[5] * x1 * x1;
+      const float evp = ev > 0.0f ? ev : 0.0f;
+      const bool allowed = !b.itm_only || evp > 0.0f;
+      if (allowed && cont - evp < 0.0f) {
+        done = true;
+        tau = s;
+        V = evp * invB;
+        // payoff adjoint on the end-of-step log-rates r >= s
+        float Pr = disc, Q = 0.0f;  // P_{r+1}, sum_{m>r} tau_{m-1} P_m
+        for (int r = m.N - 1; r >= s; --r) {
+          const float x = fast_exp(L[r * bd]);
+          const float fr = x - m.disp[r];
+          const float tr = m.taus[r];
+          const float g = tr * fast_rcp(1.0f + tr * fr);
+          Q += tr * Pr;
+          lamt[static_cast<size_t>(r) * P + p] = ev > 0.0f ? g * (disc + m.strike * Q) * invB * x : 0.0f;
+          Pr *= 1.0f + tr * fr;
+        }
+      }
+    }
+    invB *= fast_rcp(1.0f + m.taus[s] * fs);
+  }
+  if (valid) {
+    tau_out[p] = tau;
+    vout[p] = V;
+  }
+  block_sum_to(valid ? static_cast<double>(V) : 0.0, red, part + blockIdx.x);
+}
+
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_berm_bwd(KOO_MODEL_PARAMS, const float* __restrict__ vols, const float* __restrict__ ckpt,
+               const float* __restrict__ lamt, const int* __restrict__ tau_in,
+               const float* __restrict__ vin, float* __restrict__ scratch,
+               double* __restrict__ part) {
+  KOO_MODEL_LOCAL;
+  extern __shared__ float lds[];
+  __shared__ double red[4];
+  const int bd = blockDim.x;
+  const int nw = bd >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p = blockIdx.x * bd + threadIdx.x;
+  const bool valid = p < m.P;
+  const int pc = valid ? p : m.P - 1;
+  const int N = m.N;
+  const size_t P = static_cast<size_t>(m.P);
+  float* lam = lds + threadIdx.x;
+  float* ctab = lds + static_cast<size_t>(N) * bd;
+  float* vslots = ctab + static_cast<size_t>(N) * N;
+  float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 3);
+  fill_fixed_drifts<F>(m, ctab);
+  for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
+  for (int i = threadIdx.x; i < nw * N * kVegaSlots; i += bd) vslots[i] = 0.0f;
+  __syncthreads();
+
+  const int tau = valid ? tau_in[p] : -1;
+  const float V = valid ? vin[p] : 0.0f;
+  const int s_top = wave_max_int(tau);  // the wavefront's last exercise step
+  const size_t slot = static_cast<size_t>(N + 1 + 2 * F) * P;
+  const bool every = m.k == 1;
+  for (int seg = s_top < 0 ? -1 : s_top / m.k; seg >= 0; --seg) {
+    const int s0 = seg * m.k;
+    const int s1 = min(s_top + 1, s0 + m.k);
+    const float* ck = ckpt + static_cast<size_t>(seg) * (every ? slot : (N + 1) * P) + pc;
+    for (int s = s0; s < (every ? s0 : s1); ++s) {
+      const int j = s - s0;
+      const float* src = (j == 0) ? ck : scratch + j * slot + pc;
+      float* dst = scratch + (j + 1) * slot + p;
+      float* sums = scratch + j * slot + static_cast<size_t>(N + 1) * P + p;
+      const bool keep = valid && (s + 1 < s1);
+      float z[F], e[F], ep[F];
+      float l_fix = 0.0f;
+      load_z<F>(m, s, pc, z);
+      pc_step<F, kStatePrefetch>(m, s, ctab + s * N, z, e, ep,
+                                 [&](int r) { return src[static_cast<size_t>(r) * P]; },
+                                 [&](int r, float v) {
+                                   if (r == s) l_fix = v;
+                                   else if (keep) dst[static_cast<size_t>(r) * P] = v;
+                                 });
+      if (valid) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          sums[static_cast<size_t>(f) * P] = e[f];
+          sums[static_cast<size_t>(F + f) * P] = ep[f];
+        }
+      }
+      if (keep) {
+        const float fsx = fast_exp(l_fix) - m.disp[s];
+        dst[static_cast<size_t>(N) * P] =
+            src[static_cast<size_t>(N) * P] * fast_rcp(1.0f + m.taus[s] * fsx);
+      }
+    }
+    for (int s = s1 - 1; s >= s0; --s) {
+      const int j = s - s0;
+      const float* src = (j == 0) ? ck : scratch + j * slot + pc;
+      const float* sums = (every ? ck : scratch + j * slot + pc) + static_cast<size_t>(N + 1) * P;
+      if (s == tau)  // seed lambda with the payoff adjoint recorded at exercise
+        for (int r = s; r < N; ++r) lam[r * bd] = lamt[static_cast<size_t>(r) * P + p];
+      float z[F], e[F], ep[F];
+      load_z<F>(m, s, pc, z);
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        e[f] = sums[static_cast<size_t>(f) * P];
+        ep[f] = sums[static_cast<size_t>(F + f) * P];
+      }
+      reverse_step<F, kStatePrefetch>(
+          m, s, ctab + s * N, [&](int r) { return src[static_cast<size_t>(r) * P]; }, lam, bd, z,
+          e, ep,
+          [&](float& lp, float xs, float fs, float inv3) {
+            (void)fs;
+            if (s < tau) lp -= m.taus[s] * inv3 * V * xs;  // fixing inside B_tau
+          },
+          [&](int r, float w) {
+            w = valid ? w : 0.0f;
+            w += dpp_f32_quad_xor1(w);
+            w += dpp_f32_quad_xor2(w);
+            w += dpp_f32_row_half_mirror(w);
+            if ((lane & 7) == 0) atomicAdd(myslots + r * kVegaSlots, w);
+          });
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < N; ++r) {
+    const float dl = lam[r * bd] / (m.f0[r] + m.disp[r]);
+    block_sum_to(valid ? static_cast<double>(dl) : 0.0, red,
+                 part + static_cast<size_t>(blockIdx.x) * 2 * N + r);
+  }
+  for (int r = threadIdx.x; r < N; r += bd) {
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w)
+      for (int q = 0; q < kVegaSlots; ++q)
+        t += vslots[(static_cast<size_t>(w) * N + r) * kVegaSlots + q];
+    part[static_cast<size_t>(blockIdx.x) * 2 * N + N + r] = t / vols[r];
+  }
+}
+
+int fwd_threads(int N) { return N <= 50 ? 256 : (N <= 75 ? 128 : 64); }
+int bwd_threads(int N) { return N <= 45 ? 256 : (N <= 71 ? 128 : 64); }
+size_t fwd_lds(int N, int t) { return sizeof(float) * (static_cast<size_t>(N) * t + N * N); }
+size_t bwd_lds(int N, int t) {
+  return sizeof(float) * (static_cast<size_t>(N) * t + N * N + (t / 64) * N * kVegaSlots);
+}
+
+int check_berm(const char* who, int paths, int rates, int factors, int k) {
+  if (paths < 1 || rates < 1 || rates > kMaxRates || factors < 1 || factors > kMaxFactors || k < 1)
+    return set_error(KOO_ERR_INVALID,
+                     "%s: need paths>=1, 1<=rates<=%d, 1<=factors<=%d, checkpoint_every>=1 "
+                     "(got %d, %d, %d, %d)",
+                     who, kMaxRates, kMaxFactors, paths, rates, factors, k);
+  return KOO_OK;
+}
+
+// Host strategy -> kernel argument.
+int make_berm_args(const char* who, int rates, int n_ex, const int* ex_steps, const double* coeffs,
+                   const double* shift, const double* scale, int itm_only, BermArgs* b) {
+  if (n_ex < 1 || n_ex > kMaxBermDates || !ex_steps || !coeffs)
+    return set_error(KOO_ERR_INVALID, "%s: need 1<=exercise dates<=%d and coefficients", who,
+                     kMaxBermDates);
+  *b = BermArgs{};
+  for (int s = 0; s < kMaxRates; ++s) b->e_of_step[s] = -1;
+  for (int e = 0; e < n_ex; ++e) {
+    const int s = ex_steps[e];
+    if (s < 0 || s >= rates || (e > 0 && s <= ex_steps[e - 1]))
+      return set_error(KOO_ERR_INVALID, "%s: exercise steps must increase within [0, %d)", who,
+                       rates);
+    b->e_of_step[s] = static_cast<signed char>(e);
+    for (int i = 0; i < 6; ++i) b->coeff[e][i] = static_cast<float>(coeffs[6 * e + i]);
+    for (int i = 0; i < 2; ++i) {
+      b->sh[e][i] = shift ? static_cast<float>(shift[2 * e + i]) : 0.0f;
+      b->sc[e][i] = scale ? static_cast<float>(scale[2 * e + i]) : 1.0f;
+    }
+  }
+  b->E = n_ex;
+  b->itm_only = itm_only != 0;
+  b->last = ex_steps[n_ex - 1];
+  return KOO_OK;
+}
+
+}  // namespace
+}  // namespace koo
+
+using namespace koo;
+
+extern "C" int bermudan_adjoint_workspace(int paths, int rates, int factors, int checkpoint_every,
+                                          size_t* checkpoint_bytes, size_t* scratch_bytes,
+                                          size_t* path_bytes, size_t* partial_bytes) {
+  if (int e = check_berm("bermudan_adjoint_workspace", paths, rates, factors, checkpoint_every))
+    return e;
+  const size_t P = static_cast<size_t>(paths);
+  const int k = checkpoint_every;
+  const size_t nseg = (rates + k - 1) / k;
+  const size_t rows = k == 1 ? rates + 1 + 2 * factors : rates + 1;
+  if (checkpoint_bytes) *checkpoint_bytes = sizeof(float) * nseg * rows * P;
+  if (scratch_bytes)
+    *scratch_bytes = k == 1 ? sizeof(float)
+                            : sizeof(float) * static_cast<size_t>(k < rates ? k : rates) *
+                                  (rates + 1 + 2 * factors) * P;
+  // payoff adjoints (rates x P floats), exercise step (P ints), value (P floats)
+  if (path_bytes) *path_bytes = sizeof(float) * (static_cast<size_t>(rates) + 2) * P;
+  const size_t bf = (P + fwd_threads(rates) - 1) / fwd_threads(rates);
+  const size_t bb = (P + bwd_threads(rates) - 1) / bwd_threads(rates);
+  const size_t need = bf > bb * 2 * rates ? bf : bb * 2 * rates;
+  if (partial_bytes) *partial_bytes = sizeof(double) * need;
+  return KOO_OK;
+}
+
+extern "C" int bermudan_frozen_fwd(int paths, int rates, int factors, int checkpoint_every,
+                                   const float* variates, const float* initial_rates,
+                                   const float* taus, const float* displacements,
+                                   const float* pseudo_roots, float strike, int n_ex,
+                                   const int* exercise_steps, const double* coefficients,
+                                   const double* var_shift, const double* var_scale, int itm_only,
+                                   float* checkpoints, float* path_buffers, double* partials,
+                                   double* price_out, void* stream) {
+  if (int e = check_berm("bermudan_frozen_fwd", paths, rates, factors, checkpoint_every)) return e;
+  if (!variates || !initial_rates || !taus || !displacements || !pseudo_roots || !checkpoints ||
+      !path_buffers || !partials || !price_out)
+    return set_error(KOO_ERR_INVALID, "bermudan_frozen_fwd: null pointer");
+  BermArgs b;
+  if (int e = make_berm_args("bermudan_frozen_fwd", rates, n_ex, exercise_steps, coefficients,
+                             var_shift, var_scale, itm_only, &b))
+    return e;
+  const size_t P = static_cast<size_t>(paths);
+  float* lamt = path_buffers;
+  int* tau = reinterpret_cast<int*>(path_buffers + static_cast<size_t>(rates) * P);
+  float* vout = path_buffers + static_cast<size_t>(rates + 1) * P;
+  const int threads = fwd_threads(rates);
+  const int blocks = (paths + threads - 1) / threads;
+  const size_t lds = fwd_lds(rates, threads);
+  hipStream_t st = as_stream(stream);
+  switch (factors) {
+#define KOO_CASE(F)                                                                            \
+  case F:                                                                                      \
+    hipLaunchKernelGGL(k_berm_fwd<F>, dim3(blocks), dim3(threads), lds, st, variates,          \
+                       initial_rates, taus, displacements, pseudo_roots, strike, paths, rates, \
+                       checkpoint_every, b, checkpoints, lamt, tau, vout, partials);           \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
+    KOO_CASE(8)
+#undef KOO_CASE
+  }
+  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, partials, blocks, 1,
+                     1.0 / static_cast<double>(paths), price_out);
+  return check_launch("bermudan_frozen_fwd");
+}
+
+extern "C" int bermudan_frozen_bwd(int paths, int rates, int factors, int checkpoint_every,
+                                   const float* variates, const float* initial_rates,
+                                   const float* taus, const float* displacements,
+                                   const float* pseudo_roots, const float* vols, float strike,
+                                   const float* checkpoints, const float* path_buffers,
+                                   float* segment_scratch, double* partials, double* greeks_out,
+                                   void* stream) {
+  if (int e = check_berm("bermudan_frozen_bwd", paths, rates, factors, checkpoint_every)) return e;
+  if (!variates || !initial_rates || !taus || !displacements || !pseudo_roots || !vols ||
+      !checkpoints || !path_buffers || !segment_scratch || !partials || !greeks_out)
+    return set_error(KOO_ERR_INVALID, "bermudan_frozen_bwd: null pointer");
+  const size_t P = static_cast<size_t>(paths);
+  const float* lamt = path_buffers;
+  const int* tau = reinterpret_cast<const int*>(path_buffers + static_cast<size_t>(rates) * P);
+  const float* vin = path_buffers + static_cast<size_t>(rates + 1) * P;
+  const int threads = bwd_threads(rates);
+  const int blocks = (paths + threads - 1) / threads;
+  const size_t lds = bwd_lds(rates, threads);
+  hipStream_t st = as_stream(stream);
+  switch (factors) {
+#define KOO_CASE(F)                                                                            \
+  case F:                                                                                      \
+    hipLaunchKernelGGL(k_berm_bwd<F>, dim3(blocks), dim3(threads), lds, st, variates,          \
+                       initial_rates, taus, displacements, pseudo_roots, strike, paths, rates, \
+                       checkpoint_every, vols, checkpoints, lamt, tau, vin, segment_scratch,   \
+                       partials);                                                              \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
+    KOO_CASE(8)
+#undef KOO_CASE
+  }
+  hipLaunchKernelGGL(k_finalize_cols, dim3(2 * rates), dim3(256), 0, st, partials, blocks,
+                     2 * rates, 1.0 / static_cast<double>(paths), greeks_out);
+  return check_launch("bermudan_frozen_bwd");
+}

@@ -171,91 +171,31 @@ __global__ void __launch_bounds__(256)
       const float* sums = (every ? ck : scratch + j * slot + pc) + static_cast<size_t>(N + 1) * P;
       const float* cs = ctab + s * N;
       const float invBs = src[static_cast<size_t>(N) * P];
-      float z[F], e[F], ep[F], E[F], Eh[F];
+      float z[F], e[F], ep[F];
       load_z<F>(m, s, pc, z);
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         e[f] = sums[static_cast<size_t>(f) * P];
         ep[f] = sums[static_cast<size_t>(F + f) * P];
-        E[f] = Eh[f] = 0.0f;
       }
-      const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
-      // start log-rates two rates ahead, adjoints one rate ahead
-      // start log-rates (HBM or L2) issued kStatePrefetch rates ahead,
-      // adjoints (LDS) one rate ahead
-      float q[kStatePrefetch];
-#pragma unroll
-      for (int i = 0; i < kStatePrefetch; ++i)
-        q[i] = (N - 1 - i >= s) ? src[static_cast<size_t>(N - 1 - i) * P] : 0.0f;
-      float lp_next = lam[(N - 1) * bd];
-      for (int r = N - 1; r >= s; --r) {
-        const float l = q[0];
-        float lp = lp_next;
-#pragma unroll
-        for (int i = 0; i + 1 < kStatePrefetch; ++i) q[i] = q[i + 1];
-        q[kStatePrefetch - 1] =
-            (r - kStatePrefetch >= s) ? src[static_cast<size_t>(r - kStatePrefetch) * P] : 0.0f;
-        if (r > s) lp_next = lam[(r - 1) * bd];
-        float a[F];
-#pragma unroll
-        for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
-        const float tau = m.taus[r], d = m.disp[r];
-        const float omd = 1.0f - tau * d;
-        const float x = fast_exp(l);
-        const float inv1 = fast_rcp(1.0f + tau * (x - d));
-        const float g = tau * x * inv1;
-        float mu = 0.0f, muh = 0.0f, dw = 0.0f;
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          mu += a[f] * e[f];
-          muh += a[f] * ep[f];
-          dw += a[f] * z[f];
-        }
-        const float c = cs[r];
-        const float lh = l + mu + c + dw;
-        const float xh = fast_exp(lh);
-        const float inv2 = fast_rcp(1.0f + tau * (xh - d));
-        const float gh = tau * xh * inv2;
-        if (r == s) {
-          // The rate fixing at the end of this step: caplet value and the
-          // numeraire it enters for later fixings.
-          const float xs = fast_exp(lh + 0.5f * (muh - mu));
-          const float fs = xs - d;
-          const float inv3 = fast_rcp(1.0f + tau * fs);
-          const float v = caplet_value(fs, tau, m.strike, inv3);
-          const float dv = caplet_deriv(fs, tau, m.strike, inv3);
-          lp += (dv * invBs - tau * inv3 * tail) * xs;
-          tail += v * invBs;
-        }
-        float ghb = 0.0f;
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          Eh[f] += 0.5f * lp * a[f];
-          ghb += a[f] * Eh[f];
-        }
-        const float lhat = lp + ghb * gh * omd * inv2;
-        const float mub = lhat - 0.5f * lp;
-        float gb = 0.0f;
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          E[f] += mub * a[f];
-          gb += a[f] * E[f];
-        }
-        lam[r * bd] = lhat + gb * g * omd * inv1;
-        // sum_f Abar_f a_f for this (s, r): the mu^, mu, dW and fixed-drift
-        // terms contract to scalars the step already has.
-        float w = 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c);
-        w = valid ? w : 0.0f;
-        w += dpp_f32_quad_xor1(w);
-        w += dpp_f32_quad_xor2(w);
-        w += dpp_f32_row_half_mirror(w);
-        if ((lane & 7) == 0) atomicAdd(myslots + r * kVegaSlots, w);
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          e[f] -= g * a[f];
-          ep[f] -= gh * a[f];
-        }
-      }
+      reverse_step<F, kStatePrefetch>(
+          m, s, cs, [&](int r) { return src[static_cast<size_t>(r) * P]; }, lam, bd, z, e, ep,
+          [&](float& lp, float xs, float fs, float inv3) {
+            // The rate fixing at the end of this step: caplet value and the
+            // numeraire it enters for later fixings.
+            const float tau = m.taus[s];
+            const float v = caplet_value(fs, tau, m.strike, inv3);
+            const float dv = caplet_deriv(fs, tau, m.strike, inv3);
+            lp += (dv * invBs - tau * inv3 * tail) * xs;
+            tail += v * invBs;
+          },
+          [&](int r, float w) {
+            w = valid ? w : 0.0f;
+            w += dpp_f32_quad_xor1(w);
+            w += dpp_f32_quad_xor2(w);
+            w += dpp_f32_row_half_mirror(w);
+            if ((lane & 7) == 0) atomicAdd(myslots + r * kVegaSlots, w);
+          });
     }
   }
   __syncthreads();

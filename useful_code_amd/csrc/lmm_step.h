@@ -143,6 +143,86 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
   }
 }
 
+// Reverse of PC step s over the rates N-1..s.  On entry lam[r * bd] (LDS)
+// holds the adjoint of the end-of-step log-rate r, on exit that of the
+// start-of-step one.  e / ep are the step's final drift sums (un-accumulated
+// here), z its variates, cs its fixed drifts; `src(r)` gives the start
+// log-rate r (issued PF rates ahead).  `fixing(lp, x, f, 1/(1+tau f))` adds
+// the product's adjoint of the rate fixing at the end of the step (r == s) to
+// lp; `vega(r, w)` receives w = sum_f Abar(s,r,f) A(s,r,f) for the step, which
+// contracts to scalars the reverse already has (the mu^, mu, dW and
+// fixed-drift terms).  E, E^ are the adjoint running sums.
+template <int F, int PF, class Src, class Fixing, class Vega>
+__device__ __forceinline__ void reverse_step(const Model& m, int s, const float* cs, Src src,
+                                             float* lam, int bd, const float (&z)[F],
+                                             float (&e)[F], float (&ep)[F], Fixing fixing,
+                                             Vega vega) {
+  const int N = m.N;
+  const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
+  float E[F], Eh[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) E[f] = Eh[f] = 0.0f;
+  // start log-rates issued PF rates ahead, adjoints (LDS) one rate ahead
+  float q[PF];
+#pragma unroll
+  for (int i = 0; i < PF; ++i) q[i] = (N - 1 - i >= s) ? src(N - 1 - i) : 0.0f;
+  float lp_next = lam[(N - 1) * bd];
+  for (int r = N - 1; r >= s; --r) {
+    const float l = q[0];
+    float lp = lp_next;
+#pragma unroll
+    for (int i = 0; i + 1 < PF; ++i) q[i] = q[i + 1];
+    q[PF - 1] = (r - PF >= s) ? src(r - PF) : 0.0f;
+    if (r > s) lp_next = lam[(r - 1) * bd];
+    float a[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
+    const float tau = m.taus[r], d = m.disp[r];
+    const float omd = 1.0f - tau * d;
+    const float x = fast_exp(l);
+    const float inv1 = fast_rcp(1.0f + tau * (x - d));
+    const float g = tau * x * inv1;
+    float mu = 0.0f, muh = 0.0f, dw = 0.0f;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      mu += a[f] * e[f];
+      muh += a[f] * ep[f];
+      dw += a[f] * z[f];
+    }
+    const float c = cs[r];
+    const float lh = l + mu + c + dw;
+    const float xh = fast_exp(lh);
+    const float inv2 = fast_rcp(1.0f + tau * (xh - d));
+    const float gh = tau * xh * inv2;
+    if (r == s) {
+      const float xs = fast_exp(lh + 0.5f * (muh - mu));
+      const float fs = xs - d;
+      fixing(lp, xs, fs, fast_rcp(1.0f + tau * fs));
+    }
+    float ghb = 0.0f;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      Eh[f] += 0.5f * lp * a[f];
+      ghb += a[f] * Eh[f];
+    }
+    const float lhat = lp + ghb * gh * omd * inv2;
+    const float mub = lhat - 0.5f * lp;
+    float gb = 0.0f;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      E[f] += mub * a[f];
+      gb += a[f] * E[f];
+    }
+    lam[r * bd] = lhat + gb * g * omd * inv1;
+    vega(r, 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c));
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      e[f] -= g * a[f];
+      ep[f] -= gh * a[f];
+    }
+  }
+}
+
 template <int F>
 __device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[F]) {
   const float* zs = m.z + static_cast<size_t>(s) * F * m.P + p;

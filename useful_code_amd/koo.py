@@ -103,6 +103,11 @@ _SIGNATURES = {
     "basketWeightings_gpu": (_i, [_p] * 3 + [_i] * 3 + [_p]),
     "basket_asian_workspace": (_i, [_i, _i, _p]),
     "basket_asian_gpu": (_i, [_i, _i, _i] + [_p] * 5 + [_f, _f, _f, _p, _p, _p]),
+    # frozen-strategy Bermudan
+    "bermudan_adjoint_workspace": (_i, [_i, _i, _i, _i, _p, _p, _p, _p]),
+    "bermudan_frozen_fwd": (_i, [_i, _i, _i, _i] + [_p] * 5 + [_f, _i] + [_p] * 4 + [_i] +
+                            [_p] * 5),
+    "bermudan_frozen_bwd": (_i, [_i, _i, _i, _i] + [_p] * 6 + [_f] + [_p] * 6),
 }
 
 
@@ -554,3 +559,32 @@ def basket_asian_gpu(paths, assets, steps, variates, log_spots, vols, scaled_cho
     call("basket_asian_gpu", paths, assets, steps, ptr(variates), ptr(log_spots), ptr(vols),
          ptr(scaled_chol), ptr(weights), float(rate), float(dt), float(strike), ptr(partials),
          ptr(out), stream_handle(stream))
+
+
+# ---------------------------------------------------------------- frozen Bermudan
+
+def bermudan_adjoint_workspace(paths, rates, factors, checkpoint_every):
+    a, b, c, d = _sz(), _sz(), _sz(), _sz()
+    call("bermudan_adjoint_workspace", paths, rates, factors, checkpoint_every,
+         ctypes.addressof(a), ctypes.addressof(b), ctypes.addressof(c), ctypes.addressof(d))
+    return a.value, b.value, c.value, d.value
+
+
+def bermudan_frozen_fwd(paths, rates, factors, k, variates, f0, taus, disp, A, strike,
+                        exercise_steps, coeffs_host, shift_host, scale_host, itm_only, checkpoints,
+                        path_buffers, partials, price_out, stream=None):
+    ex = _host_ints(exercise_steps)
+    c = _f64(coeffs_host)
+    sh = None if shift_host is None else _f64(shift_host)
+    sc = None if scale_host is None else _f64(scale_host)
+    call("bermudan_frozen_fwd", paths, rates, factors, k, ptr(variates), ptr(f0), ptr(taus),
+         ptr(disp), ptr(A), float(strike), len(ex), ex.ctypes.data, c.ctypes.data, ptr(sh),
+         ptr(sc), int(itm_only), ptr(checkpoints), ptr(path_buffers), ptr(partials),
+         ptr(price_out), stream_handle(stream))
+
+
+def bermudan_frozen_bwd(paths, rates, factors, k, variates, f0, taus, disp, A, vols, strike,
+                        checkpoints, path_buffers, scratch, partials, greeks_out, stream=None):
+    call("bermudan_frozen_bwd", paths, rates, factors, k, ptr(variates), ptr(f0), ptr(taus),
+         ptr(disp), ptr(A), ptr(vols), float(strike), ptr(checkpoints), ptr(path_buffers),
+         ptr(scratch), ptr(partials), ptr(greeks_out), stream_handle(stream))

@@ -125,6 +125,31 @@ class BermudanPricer:
         koo.ls_second_pass_gpu(self.E, self.bv, self.ves, 2, None, self.ev, self.num, w.paths, True,
                                True, self.shift, self.scale, self.coeffs, self.ws, self.price2)
 
+    def greeks(self, checkpoint_every=4):
+        """Frozen-strategy price and pathwise adjoint Greeks (N deltas, N vegas)
+        on the current variates, with the coefficients of the last regress()."""
+        w = self.w
+        P, N, F = w.paths, w.n_rates, w.factors
+        if not hasattr(self, "adj") or self.adj[0] != checkpoint_every:
+            ck, sc, pb, pa = koo.bermudan_adjoint_workspace(P, N, F, checkpoint_every)
+            dev = self.z.device
+            self.adj = (checkpoint_every,
+                        torch.empty(ck // 4, dtype=torch.float32, device=dev),
+                        torch.empty(sc // 4, dtype=torch.float32, device=dev),
+                        torch.empty(pb // 4, dtype=torch.float32, device=dev),
+                        torch.empty(pa // 8, dtype=torch.float64, device=dev))
+            self.vols = _dev(w.vols, torch.float32)
+            self.frozen_price = torch.zeros(1, dtype=torch.float64, device=dev)
+            self.frozen_greeks = torch.zeros(2 * N, dtype=torch.float64, device=dev)
+        k, ck, sc, pb, pa = self.adj
+        coeffs = self.coeffs.cpu().numpy()
+        koo.bermudan_frozen_fwd(P, N, F, k, self.z, self.f0, self.taus, self.disp, self.A,
+                                w.strike, w.exercise_steps, coeffs, self.shift, self.scale, True,
+                                ck, pb, pa, self.frozen_price)
+        koo.bermudan_frozen_bwd(P, N, F, k, self.z, self.f0, self.taus, self.disp, self.A,
+                                self.vols, w.strike, ck, pb, sc, pa, self.frozen_greeks)
+        return self.frozen_price, self.frozen_greeks
+
     def price(self):
         """First-pass (in-sample) price, then the frozen-strategy price on the
         next block of Sobol points; returns both as device tensors."""
@@ -172,6 +197,66 @@ class BasketPricer:
         self.generate()
         self.greeks_kernel()
         return self.out
+
+
+class SwapPricer:
+    """Plain-LMM payer swap through the reference-named stage kernels, the
+    reference's staged pipeline (LMMEvolutionRoutineRawSingleKernel and its
+    product chain, LMM_evolver_full.cu:264-425, Market_model_pricer.cu): Sobol
+    integers, scramble of the first S rows, inverse normal, bridge, all-steps
+    PC evolver with discount ratios, spot numeraires, extraction of each
+    step's fixing, swap flows, discounting of both flow legs, FP64 sum."""
+
+    def __init__(self, w, device="cuda"):
+        self.w = w
+        P, N, F, S = w.paths, w.n_rates, w.factors, w.steps
+        D = S * F
+        f32 = lambda n: torch.empty(n, dtype=torch.float32, device=device)  # noqa: E731
+        self.dirs = _dev(koo.sobol_directions(D).view(np.int32), torch.int32)
+        self.scr = _dev(np.asarray(w.scramblers, np.uint32).view(np.int32), torch.int32)
+        self.alloc = _dev(koo.bridge_allocate_diagonal(F, S), torch.int32)
+        self.ints = torch.empty(D * P, dtype=torch.int32, device=device)
+        self.normals, self.z = f32(D * P), f32(D * P)
+        self.rates, self.logs = f32(S * N * P), f32(S * N * P)
+        self.disc, self.num, self.r1 = f32(S * (N + 1) * P), f32(S * P), f32(S * P)
+        self.f1, self.f2, self.d1, self.d2 = f32(S * P), f32(S * P), f32(S * P), f32(S * P)
+        self.s1, self.s2 = f32(P), f32(P)
+        self.f0 = _dev(w.f0, torch.float32)
+        self.lf0 = _dev(np.log(w.f0 + w.disp()), torch.float32)
+        self.taus = _dev(w.taus, torch.float32)
+        self.A = _dev(w.A, torch.float32)
+        self.mu0 = _dev(koo.spot_drifts(w.A[0], w.f0, w.taus, w.disp(), 0), torch.float32)
+        self.fx = _dev(koo.fixed_drifts(w.A), torch.float32)
+        self.dsp = _dev(w.disp(), torch.float32)
+        self.alive = _dev(w.alive(), torch.int32)
+        self.first = _dev(np.arange(1, S + 1), torch.int32)
+        self.theta = _dev(np.zeros(S), torch.float32)
+        self.aux = _dev(np.concatenate([[w.strike, 1.0], w.taus, w.taus]), torch.float32)
+        self.ws = torch.empty(koo.sum_workspace_doubles(P), dtype=torch.float64, device=device)
+        self.price_out = torch.zeros(1, dtype=torch.float64, device=device)
+
+    def price(self):
+        w = self.w
+        P, N, F, S = w.paths, w.n_rates, w.factors, w.steps
+        koo.sobolGPUInts(P, S * F, w.path_offset, self.dirs, self.ints)
+        koo.scramble_gpu(self.ints, self.scr, S, P)
+        koo.inverse_normal_u32_gpu(self.ints, self.normals)
+        koo.brownian_bridge_gpu(S, self.alloc, self.normals, self.z, P, F)
+        koo.LMM_evolver_pc_all_gpu(self.f0, self.lf0, self.taus, self.z, self.A, self.mu0, self.fx,
+                                   self.dsp, self.alive, P, F, S, N, self.rates, self.logs,
+                                   self.disc)
+        koo.spot_measure_numeraires_computation_gpu(self.disc, self.num, P, N)
+        koo.forward_rate_extraction_gpu(self.rates, np.arange(S), P, S, N, self.r1)
+        koo.cash_flow_generator("swap", self.f1, self.f2, self.aux, P, S, self.r1, self.r1,
+                                self.r1)
+        koo.cashFlowDiscounting_gpu(self.first, self.first, self.theta, self.disc, self.f1,
+                                    self.num, P, S, self.d1, self.s1)
+        koo.cashFlowDiscounting_gpu(self.first, self.first, self.theta, self.disc, self.f2,
+                                    self.num, P, S, self.d2, self.s2)
+        self.s1.add_(self.s2)
+        koo.sum_f32_to_f64(self.s1, self.ws, self.price_out)
+        self.price_out.div_(P)
+        return self.price_out
 
 
 def lmm_evolution_routine(w, use_fused_generator=False):
