@@ -136,6 +136,52 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
     return res
 
 
+def portfolio_section(args, torch, configs, pipeline, ev):
+    """Config 5: 64 Bermudan swaptions (10-40 rates, 4-20 exercise dates,
+    1-5 factors, 2^17 paths each): per product generator, path quantities,
+    LS first pass and frozen-strategy adjoint, all captured into one HIP graph
+    over four streams with the autotuner's checkpoint intervals and stream
+    schedule; one step = one graph replay (price + 2N Greeks for every product)."""
+    from useful_code_amd import autotune
+    prods = configs.bermudan_portfolio(paths=args.portfolio_paths)
+    pf = pipeline.PortfolioPricer(prods, checkpoint_every=4, n_streams=4)
+    t = time.perf_counter()
+    tuning = autotune.tune_portfolio(pf)
+    tune_s = time.perf_counter() - t
+    pf.apply_tuning(tuning)
+    # eager reference timing (launch-bound on the host)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pf.run(capture=False)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t) * 1e3
+    pf.run(capture=True)  # warm-up + capture + first replay
+    torch.cuda.synchronize()
+    recs = [(ev(), ev()) for _ in range(args.steps)]
+    for a, b in recs:
+        a.record()
+        pf.run(capture=True)
+        b.record()
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in recs]))
+    res = pf.results()
+    path_steps = sum(w.paths * w.steps for w in prods)
+    return {
+        "workload": "bermudan_portfolio_64: 64 Bermudan payer swaptions, 10-40 rates, 4-20 "
+                    "exercise dates, K 0.03-0.07, 1-5 factors, 2^17 Sobol paths each; LS first pass "
+                    "(ITM, MFMA normal equations, FP64 Cholesky) + frozen-strategy adjoint deltas "
+                    "and vegas per product",
+        "graph_replay_ms": ms,
+        "eager_ms": eager_ms,
+        "path_steps_per_s": path_steps / (ms * 1e-3),
+        "products": len(prods),
+        "tuning_s": tune_s,
+        "checkpoint_every": tuning["k"],
+        "stream_load_ms": tuning["stream_load_ms"],
+        "frozen_price_sum": float(sum(r[1] for r in res)),
+    }
+
+
 def swap_section(args, torch, O, configs, pipeline, ev):
     """Plain-LMM payer swap (20 rates, 3 factors, 2^15 paths) through the
     staged reference-named kernels, with the reference's own CPU pipeline
@@ -344,6 +390,8 @@ def main():
     ap.add_argument("--no-bermudan", action="store_true")
     ap.add_argument("--no-basket", action="store_true")
     ap.add_argument("--no-swap", action="store_true")
+    ap.add_argument("--no-portfolio", action="store_true")
+    ap.add_argument("--portfolio-paths", type=int, default=1 << 17)
     ap.add_argument("--basket-paths", type=int, default=1 << 20)
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the per-stage HBM measurements of the reference-ABI kernels")
@@ -533,6 +581,10 @@ def main():
 
     if not args.no_bermudan:
         out["bermudan_ls"] = bermudan_section(args, torch, O, configs, pipeline, ev)
+
+    if not args.no_portfolio:
+        out["bermudan_portfolio"] = portfolio_section(args, torch, configs, pipeline, ev)
+        torch.cuda.empty_cache()
 
     if not args.no_swap:
         out["payer_swap_config1"] = swap_section(args, torch, O, configs, pipeline, ev)

@@ -540,10 +540,12 @@ int basket_asian_gpu(int paths, int assets, int steps, const float* variates,
  * scale[e][1]) -- the coefficients ls_first_pass_gpu returns.  price_out is
  * the mean of ev / B at exercise; greeks_out[0..N) = dV/df_i(0),
  * [N..2N) = dV/dsigma_i (A proportional to sigma_i), with the exercise
- * decisions held fixed.  exercise_steps (increasing, <= 40 dates),
- * coefficients (E x 6), var_shift and var_scale (E x 2, NULL for 0 / 1) are
- * HOST arrays; buffers come from bermudan_adjoint_workspace().  The backward
- * reads the path buffers the forward wrote.
+ * decisions held fixed.  coefficients is a DEVICE array (E x 6 doubles, as
+ * ls_first_pass_gpu writes them); exercise_steps (increasing, <= 40 dates),
+ * var_shift and var_scale (E x 2, NULL for 0 / 1) are HOST arrays; buffers
+ * come from bermudan_adjoint_workspace().  The backward reads the path
+ * buffers the forward wrote.  Nothing synchronises, so the whole first pass
+ * plus adjoint can be captured in a graph.
  * =================================================================== */
 int bermudan_adjoint_workspace(int paths, int rates, int factors, int checkpoint_every,
                                size_t* checkpoint_bytes, size_t* scratch_bytes,

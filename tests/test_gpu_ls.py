@@ -373,3 +373,41 @@ def test_frozen_bermudan_price_and_greeks_vs_fp64_oracle(torch_cuda, berm, k):
     assert flips < 1e-3
     assert abs(price.item() - vals.mean()) < 1e-5
     assert err < 2e-3
+
+
+# ---------------------------------------------------------------- batched portfolio
+
+def test_portfolio_graph_replays_equal_eager_and_oracle(torch_cuda):
+    torch = torch_cuda
+    from useful_code_amd import autotune
+    prods = configs.bermudan_portfolio(n_products=6, paths=1 << 13, seed=77)
+    pf = pipeline.PortfolioPricer(prods, checkpoint_every=4, n_streams=3)
+    pf.run(capture=False)
+    torch.cuda.synchronize()
+    eager = pf.results()
+    for rep in range(3):
+        pf.run(capture=True)
+        torch.cuda.synchronize()
+        got = pf.results()
+        for (a1, a2, ag), (b1, b2, bg) in zip(eager, got):
+            assert a1 == b1 and a2 == b2 and np.array_equal(ag, bg), rep
+        # eager work between replays
+        pf.pricers[0].generate()
+        torch.cuda.synchronize()
+    tuning = autotune.tune_portfolio(pf)
+    assert len(tuning["k"]) == 6 and set(tuning["stream"]) <= {0, 1, 2}
+    pf.apply_tuning(tuning)
+    pf.run(capture=True)
+    torch.cuda.synchronize()
+    for j, (w, pr, (p1, p2, g)) in enumerate(zip(prods, pf.pricers, pf.results())):
+        E, P, N = pr.E, w.paths, w.n_rates
+        z = pr.z.cpu().numpy().astype(np.float64).reshape(w.steps, w.factors, P)
+        coeffs = pr.coeffs.cpu().numpy().reshape(E, 6)
+        fs = O.frozen_strategy(w.exercise_steps, coeffs, w.basis_shift, w.basis_scale)
+        vals, _, d, v = O.bermudan_frozen_paths(_oracle_model(w), fs, z)
+        g64 = np.concatenate([d, v]) / P
+        err = np.max(np.abs(g - g64)) / np.max(np.abs(g64))
+        print(f"product {j} {w.name} k={pf.k[j]}: frozen {p2:.8f} vs fp64 {vals.mean():.8f}, "
+              f"greeks rel err {err:.2e}")
+        assert abs(p2 - vals.mean()) < 1e-5
+        assert err < 5e-3

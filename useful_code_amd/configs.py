@@ -107,6 +107,31 @@ def bermudan_swaption(paths=2**18, seed=2025):
                        basis_shift=np.full((ex.size, 2), K), basis_scale=np.full((ex.size, 2), 100.0))
 
 
+def bermudan_portfolio(n_products=64, paths=2**17, seed=2028):
+    """Batched callable portfolio: Bermudan payer swaptions with 10-40 rates
+    (uniform from seed), 4-20 exercise dates (evenly spread over the resets up
+    to the second-to-last, fewer when the deal is short), strikes U(0.03, 0.07),
+    1-5 factors, flat vols U(0.15, 0.25), forwards 0.05 + U(-0.005, 0.005),
+    2^17 Sobol paths each.  LS variables are normalised as (x - 0.05) * 100."""
+    rng = np.random.RandomState(seed)
+    prods = []
+    for j in range(n_products):
+        N = int(rng.randint(10, 41))
+        E = int(min(rng.randint(4, 21), N - 1))
+        K = float(rng.uniform(0.03, 0.07))
+        F = int(min(rng.randint(1, 6), N))
+        f0 = 0.05 + rng.uniform(-0.005, 0.005, N)
+        vols = np.full(N, rng.uniform(0.15, 0.25))
+        rate_times = 0.5 * np.arange(1, N + 2)
+        A = koo.flat_vol_pseudo_roots(rate_times, rate_times[:N], vols, F, 0.0, 0.1)
+        ex = np.unique(np.round(np.linspace(0, N - 2, E)).astype(np.int32))
+        prods.append(LMMWorkload(f"bermudan_{j}_{N}x{F}_e{ex.size}", paths, N, F, 0.5, f0, vols,
+                                 rate_times, A, K, "bermudan", _scramblers(rng, N * F),
+                                 exercise_steps=ex, basis_shift=np.full((ex.size, 2), 0.05),
+                                 basis_scale=np.full((ex.size, 2), 100.0)))
+    return prods
+
+
 @dataclass
 class BasketWorkload:
     """Multi-asset Black-Scholes basket; variates are (steps, assets, paths)."""

@@ -29,7 +29,6 @@ constexpr int kStatePrefetch = 8;
 struct BermArgs {
   int E, itm_only, last;
   signed char e_of_step[kMaxRates];  // exercise index of each step, -1 if none
-  float coeff[kMaxBermDates][6];     // quadratic cross basis in the normalised variables
   float sh[kMaxBermDates][2], sc[kMaxBermDates][2];
 };
 
@@ -41,7 +40,8 @@ __device__ __forceinline__ int wave_max_int(int v) {
 
 template <int F>
 __global__ void __launch_bounds__(256)
-    k_berm_fwd(KOO_MODEL_PARAMS, const BermArgs b, float* __restrict__ ckpt,
+    k_berm_fwd(KOO_MODEL_PARAMS, const BermArgs b, const double* __restrict__ coeffs,
+               float* __restrict__ ckpt,
                float* __restrict__ lamt, int* __restrict__ tau_out, float* __restrict__ vout,
                double* __restrict__ part) {
   KOO_MODEL_LOCAL;
@@ -95,9 +95,10 @@ __global__ void __launch_bounds__(256)
       const float ev = (sr - m.strike) * ann;
       const float x0 = (sr - b.sh[ei][0]) * b.sc[ei][0];
       const float x1 = (fs - b.sh[ei][1]) * b.sc[ei][1];
-      const float* cf = b.coeff[ei];
-      const float cont = cf[0] + cf[1] * x0 + cf[2] * x1 + cf[3] * x0 * x0 + cf[4] * x0 * x1 +
-                         cf[5] * x1 * x1;
+      const double* cf = coeffs + 6 * ei;  // uniform: scalar loads
+      const float cont = static_cast<float>(cf[0]) + static_cast<float>(cf[1]) * x0 +
+                         static_cast<float>(cf[2]) * x1 + static_cast<float>(cf[3]) * x0 * x0 +
+                         static_cast<float>(cf[4]) * x0 * x1 + static_cast<float>(cf[5]) * x1 * x1;
       const float evp = ev > 0.0f ? ev : 0.0f;
       const bool allowed = !b.itm_only || evp > 0.0f;
       if (allowed && cont - evp < 0.0f) {
@@ -250,11 +251,10 @@ int check_berm(const char* who, int paths, int rates, int factors, int k) {
 }
 
 // Host strategy -> kernel argument.
-int make_berm_args(const char* who, int rates, int n_ex, const int* ex_steps, const double* coeffs,
+int make_berm_args(const char* who, int rates, int n_ex, const int* ex_steps,
                    const double* shift, const double* scale, int itm_only, BermArgs* b) {
-  if (n_ex < 1 || n_ex > kMaxBermDates || !ex_steps || !coeffs)
-    return set_error(KOO_ERR_INVALID, "%s: need 1<=exercise dates<=%d and coefficients", who,
-                     kMaxBermDates);
+  if (n_ex < 1 || n_ex > kMaxBermDates || !ex_steps)
+    return set_error(KOO_ERR_INVALID, "%s: need 1<=exercise dates<=%d", who, kMaxBermDates);
   *b = BermArgs{};
   for (int s = 0; s < kMaxRates; ++s) b->e_of_step[s] = -1;
   for (int e = 0; e < n_ex; ++e) {
@@ -263,7 +263,6 @@ int make_berm_args(const char* who, int rates, int n_ex, const int* ex_steps, co
       return set_error(KOO_ERR_INVALID, "%s: exercise steps must increase within [0, %d)", who,
                        rates);
     b->e_of_step[s] = static_cast<signed char>(e);
-    for (int i = 0; i < 6; ++i) b->coeff[e][i] = static_cast<float>(coeffs[6 * e + i]);
     for (int i = 0; i < 2; ++i) {
       b->sh[e][i] = shift ? static_cast<float>(shift[2 * e + i]) : 0.0f;
       b->sc[e][i] = scale ? static_cast<float>(scale[2 * e + i]) : 1.0f;
@@ -312,12 +311,12 @@ extern "C" int bermudan_frozen_fwd(int paths, int rates, int factors, int checkp
                                    float* checkpoints, float* path_buffers, double* partials,
                                    double* price_out, void* stream) {
   if (int e = check_berm("bermudan_frozen_fwd", paths, rates, factors, checkpoint_every)) return e;
-  if (!variates || !initial_rates || !taus || !displacements || !pseudo_roots || !checkpoints ||
-      !path_buffers || !partials || !price_out)
+  if (!variates || !initial_rates || !taus || !displacements || !pseudo_roots || !coefficients ||
+      !checkpoints || !path_buffers || !partials || !price_out)
     return set_error(KOO_ERR_INVALID, "bermudan_frozen_fwd: null pointer");
   BermArgs b;
-  if (int e = make_berm_args("bermudan_frozen_fwd", rates, n_ex, exercise_steps, coefficients,
-                             var_shift, var_scale, itm_only, &b))
+  if (int e = make_berm_args("bermudan_frozen_fwd", rates, n_ex, exercise_steps, var_shift,
+                             var_scale, itm_only, &b))
     return e;
   const size_t P = static_cast<size_t>(paths);
   float* lamt = path_buffers;
@@ -332,7 +331,8 @@ extern "C" int bermudan_frozen_fwd(int paths, int rates, int factors, int checkp
   case F:                                                                                      \
     hipLaunchKernelGGL(k_berm_fwd<F>, dim3(blocks), dim3(threads), lds, st, variates,          \
                        initial_rates, taus, displacements, pseudo_roots, strike, paths, rates, \
-                       checkpoint_every, b, checkpoints, lamt, tau, vout, partials);           \
+                       checkpoint_every, b, coefficients, checkpoints, lamt, tau, vout,        \
+                       partials);                                                              \
     break;
     KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
     KOO_CASE(8)

@@ -571,14 +571,13 @@ def bermudan_adjoint_workspace(paths, rates, factors, checkpoint_every):
 
 
 def bermudan_frozen_fwd(paths, rates, factors, k, variates, f0, taus, disp, A, strike,
-                        exercise_steps, coeffs_host, shift_host, scale_host, itm_only, checkpoints,
+                        exercise_steps, coeffs_dev, shift_host, scale_host, itm_only, checkpoints,
                         path_buffers, partials, price_out, stream=None):
     ex = _host_ints(exercise_steps)
-    c = _f64(coeffs_host)
     sh = None if shift_host is None else _f64(shift_host)
     sc = None if scale_host is None else _f64(scale_host)
     call("bermudan_frozen_fwd", paths, rates, factors, k, ptr(variates), ptr(f0), ptr(taus),
-         ptr(disp), ptr(A), float(strike), len(ex), ex.ctypes.data, c.ctypes.data, ptr(sh),
+         ptr(disp), ptr(A), float(strike), len(ex), ex.ctypes.data, ptr(coeffs_dev), ptr(sh),
          ptr(sc), int(itm_only), ptr(checkpoints), ptr(path_buffers), ptr(partials),
          ptr(price_out), stream_handle(stream))
 

@@ -142,9 +142,8 @@ class BermudanPricer:
             self.frozen_price = torch.zeros(1, dtype=torch.float64, device=dev)
             self.frozen_greeks = torch.zeros(2 * N, dtype=torch.float64, device=dev)
         k, ck, sc, pb, pa = self.adj
-        coeffs = self.coeffs.cpu().numpy()
         koo.bermudan_frozen_fwd(P, N, F, k, self.z, self.f0, self.taus, self.disp, self.A,
-                                w.strike, w.exercise_steps, coeffs, self.shift, self.scale, True,
+                                w.strike, w.exercise_steps, self.coeffs, self.shift, self.scale, True,
                                 ck, pb, pa, self.frozen_price)
         koo.bermudan_frozen_bwd(P, N, F, k, self.z, self.f0, self.taus, self.disp, self.A,
                                 self.vols, w.strike, ck, pb, sc, pa, self.frozen_greeks)
@@ -197,6 +196,66 @@ class BasketPricer:
         self.generate()
         self.greeks_kernel()
         return self.out
+
+
+class PortfolioPricer:
+    """A portfolio of Bermudan swaptions priced with Longstaff-Schwartz and
+    frozen-strategy adjoint Greeks, every product's whole sequence (generator,
+    path quantities, first pass, frozen forward and reverse sweep) issued on
+    one of ``n_streams`` streams and, with ``capture``, recorded once into a
+    single HIP graph that ``run`` replays.  ``checkpoint_every`` maps a
+    product index to its checkpoint interval (the autotuner's choice) or is
+    one interval for all."""
+
+    def __init__(self, products, checkpoint_every=4, n_streams=4, device="cuda"):
+        self.products = products
+        self.pricers = [BermudanPricer(w, device=device) for w in products]
+        self.k = [checkpoint_every(j) if callable(checkpoint_every) else checkpoint_every
+                  for j in range(len(products))]
+        self.streams = [torch.cuda.Stream(device=device) for _ in range(n_streams)]
+        self.stream_of = [j % n_streams for j in range(len(products))]
+        self.graph = None
+
+    def apply_tuning(self, tuning):
+        """Checkpoint interval and stream per product from autotune.tune_portfolio."""
+        self.k = list(tuning["k"])
+        self.stream_of = list(tuning["stream"])
+        self.graph = None
+
+    def _issue(self):
+        main = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(main)
+        order = sorted(range(len(self.pricers)), key=lambda j: self.stream_of[j])
+        for j in order:
+            pr = self.pricers[j]
+            with torch.cuda.stream(self.streams[self.stream_of[j]]):
+                pr.generate()
+                pr.paths_kernel()
+                pr.regress()
+                pr.greeks(checkpoint_every=self.k[j])
+        for s in self.streams:
+            main.wait_stream(s)
+
+    def run(self, capture=True):
+        """Prices the portfolio; with ``capture`` the first call records the
+        graph (after one eager warm-up that allocates every buffer) and every
+        call replays it."""
+        if not capture:
+            self._issue()
+            return
+        if self.graph is None:
+            self._issue()  # warm-up: allocations, tables, lazy module loads
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self._issue()
+        self.graph.replay()
+
+    def results(self):
+        """[(first-pass price, frozen-strategy price, greeks (2N))] per product."""
+        return [(pr.price1.item(), pr.frozen_price.item(), pr.frozen_greeks.cpu().numpy())
+                for pr in self.pricers]
 
 
 class SwapPricer:
