@@ -45,6 +45,8 @@ def _status(name, *args):
 @pytest.mark.parametrize("call,args", [
     ("brownian_bridge_gpu", (129, None, None, None, 16, 1, None)),
     ("sobol_bridge_gpu", (16, 0, 0, 2, None, None, 0, None, None, None)),
+    # Gray-code indices would pass 2^32 - 1 (rejected before any launch)
+    ("sobol_bridge_gpu", (100, 0xFFFFFFF0, 4, 1, 8, 8, 0, 8, 8, None)),
     ("sobolGPUInts", (-1, 3, 0, None, None, None)),
     ("LMM_evolver_pc_all_gpu", (None,) * 9 + (16, 9, 4, 4) + (None,) * 5 + (0, 1, None, 0, 1)),
     ("lmm_price_fwd", (16, 200, 5, 8) + (None,) * 5 + (1,) + (None,) * 5),

@@ -8,8 +8,9 @@
   relative of their largest magnitude; against FP64 central finite
   differences with common random numbers (1e-4 relative); asset counts that
   fill one to four 16-asset MFMA tiles, partly padded.
-* The config-4 workload (32 assets, 64 steps): GPU price and Greeks within the
-  same tolerances (and each Greek within 1e-3 of itself) of the oracle that generates its own FP64 normals from the
+* The config-4 workload (32 assets, 64 steps): GPU price within 2e-5 and the
+  Greeks within 4e-4 of the largest (a few strike crossings between FP32 and
+  FP64 paths) and 1e-3 of themselves, against the oracle that generates its own FP64 normals from the
   same scrambled Sobol integers (2048 dimensions), on the first 2^14 paths."""
 import numpy as np
 import pytest
@@ -130,6 +131,9 @@ def test_config4_basket_vs_full_fp64_oracle(torch_cuda):
           f"greeks max err {err_g:.2e} of the largest, elementwise {err_e:.2e}")
     assert abs(out[0] / price - 1) < 2e-5
     # FP32 normals and paths against FP64 ones move a few paths across the
-    # strike; the smallest vegas take that as a larger relative error
-    assert err_g < 1e-4
+    # strike.  At 2^14 paths one such path moves a vega by its pathwise vega
+    # (w_a S_a dx_a/dsigma_a, order 1) / 2^14, about 1e-4 of the largest
+    # Greek (a vega near 0.5), so the bound here allows a few such paths; the
+    # same-variates tests above hold the kernel itself to 1e-4.
+    assert err_g < 4e-4
     assert err_e < 1e-3

@@ -44,7 +44,9 @@ def test_bridge_vs_oracle(torch_cuda, S, F, P):
 
 
 @pytest.mark.parametrize("S,F,P,off", [(40, 5, 30000, 11), (64, 2, 5000, 1000), (20, 3, 777, 63),
-                                       (48, 1, 4096, 0), (100, 1, 300, 5)])
+                                       (48, 1, 4096, 0), (100, 1, 300, 5), (1, 1, 100, 0),
+                                       (7, 4, 2049, 64), (40, 5, 20000, (1 << 28) + 12345),
+                                       (33, 2, 3000, (1 << 31) - 7)])
 def test_fused_sobol_bridge_equals_staged(torch_cuda, S, F, P, off):
     torch = torch_cuda
     D = S * F

@@ -2,13 +2,16 @@
 // unit grid) with successive differencing, and the fused
 // Sobol -> scramble -> normal -> bridge generator, for gfx950.
 //
-// One lane owns one (path, factor) pair; the S bridge points of that pair
-// live in LDS laid out [step][lane] so every LDS access of a wavefront is
-// bank-conflict free, and every global load/store is a contiguous 256-byte
-// row of the path-minor layouts ([D][P] in, (S,F,P) out).  The construction
-// tables travel in the kernel arguments (scalar loads), so a call allocates
+// One lane owns one (path, factor) pair and every global load/store is a
+// contiguous 256-byte row of the path-minor layouts ([D][P] in, (S,F,P)
+// out).  For S <= 64 steps the construction order is a compile-time constant
+// and the S bridge points of a lane live in VGPRs; longer bridges keep them
+// in LDS laid out [step][lane] (bank-conflict free).  The construction
+// weights travel in the kernel arguments (scalar loads), so a call allocates
 // nothing.
+#include <array>
 #include <mutex>
+#include <utility>
 #include <unordered_map>
 #include <vector>
 
@@ -139,80 +142,173 @@ __global__ void __launch_bounds__(kBridgeThreads)
               out);
 }
 
-// Fused generator, steps <= kTableSteps: no per-point bit loop.  With
-// n = G + l, G a multiple of 64 and l < 64, the Gray code splits,
-// gray(n) = gray(G) ^ gray(l), so the Sobol integer is x(n) = X(G) ^ T(l).
-// A chunk of blockDim consecutive paths spans at most blockDim/64 + 1
-// multiples of 64.  Each block keeps the direction numbers of its S
-// dimensions and the 64-entry tables T(l) in LDS for its whole life and walks
-// chunks of paths; per chunk it builds X(G) for those groups (scrambler
-// folded in; each one a 32-lane XOR reduction over direction numbers read
-// from LDS).  A point then costs two LDS reads and one XOR.
-constexpr int kTableSteps = 64;
-constexpr int kMaxGroups = 8;
+// ---------------------------------------------------------------------------
+// Register-resident bridge for S <= kRegSteps.  The construction order
+// depends on S only, so for a compile-time S every index is a constant and
+// the S bridge points of a lane stay in VGPRs (no LDS path storage, which
+// held occupancy to one or two waves per SIMD).  The weights stay in the
+// kernel arguments (uniform, scalar loads).
+// ---------------------------------------------------------------------------
+constexpr int kRegSteps = 64;
 
+// The index walk of make_tables (brownian_bridge.cu:639-700) as a constant
+// expression.
+template <int S>
+struct BridgeOrder {
+  int16_t bridge[S] = {}, left[S] = {}, right[S] = {};
+  constexpr BridgeOrder() {
+    int map[S] = {};
+    map[S - 1] = 1;
+    bridge[0] = S - 1;
+    for (int j = 0, i = 1; i < S; ++i) {
+      while (map[j]) ++j;
+      int k = j;
+      while (!map[k]) ++k;
+      const int l = j + ((k - 1 - j) >> 1);
+      map[l] = i;
+      bridge[i] = static_cast<int16_t>(l);
+      left[i] = static_cast<int16_t>(j);
+      right[i] = static_cast<int16_t>(k);
+      j = k + 1;
+      if (j >= S) j = 0;
+    }
+  }
+};
+
+template <int S, int I, class Z>
+__device__ __forceinline__ void bridge_point(float (&w)[S], const BridgeTables& T, Z& z) {
+  constexpr BridgeOrder<S> M{};
+  if constexpr (I == 0) {
+    w[S - 1] = T.sd[0] * z(0);
+  } else {
+    constexpr int j = M.left[I], k = M.right[I], l = M.bridge[I];
+    float v = T.rw[I] * w[k];
+    if constexpr (j != 0) v = T.lw[I] * w[j - 1] + v;
+    w[l] = v + T.sd[I] * z(I);
+  }
+}
+
+template <int S, class Z, int... I>
+__device__ __forceinline__ void bridge_points(float (&w)[S], const BridgeTables& T, Z& z,
+                                              std::integer_sequence<int, I...>) {
+  (bridge_point<S, I>(w, T, z), ...);
+}
+
+// Builds the lane's path from z(i) (bridge variate i, i a constant) and
+// stores its increments to (S,F,P) at path p when `live`.
+template <int S, class Z>
+__device__ __forceinline__ void bridge_store(const BridgeTables& T, Z z, int factors, int f,
+                                             int paths, int p, bool live,
+                                             float* __restrict__ out) {
+  float w[S];
+  bridge_points<S>(w, T, z, std::make_integer_sequence<int, S>{});
+  float prev = 0.0f;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    if (live) out[(static_cast<size_t>(i) * factors + f) * paths + p] = w[i] - prev;
+    prev = w[i];
+  }
+}
+
+template <int S>
 __global__ void __launch_bounds__(kBridgeThreads)
-    k_sobol_bridge_tab(const BridgeTables T, int steps, int factors, int paths, uint32_t off1,
+    k_bridge_reg(const BridgeTables T, int factors, int paths, const int* __restrict__ alloc,
+                 const float* __restrict__ in, float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const int* idx = alloc + blockIdx.y * S;
+  bridge_store<S>(T, [&](int i) { return in[static_cast<size_t>(idx[i]) * paths + p]; },
+                  factors, blockIdx.y, paths, p, true, out);
+}
+
+// Fused generator on the register bridge.  Each wavefront walks a contiguous
+// run of 64-path chunks of one factor.  With n = G + l (G a multiple of 64,
+// l < 64) the Gray code splits, gray(n) = gray(G) ^ gray(l), so the
+// scrambled Sobol integer is X(G) ^ T(l) with X(G) = scrambler ^ XOR of the
+// direction numbers over the set bits of gray(G).  A lane's l is the same in
+// every chunk; T(l) sits in LDS.  X(G) lives one dimension per lane: from
+// G = 64m to 64(m+1) gray flips bits 5 and 6 + ctz(m+1), so advancing costs
+// two XORs; a chunk (lanes at n = 64c + off1 + lane) spans G and G + 64, so
+// both are formed and read out by readlane.
+template <int S>
+__global__ void __launch_bounds__(kBridgeThreads)
+    k_sobol_bridge_reg(const BridgeTables T, int factors, int paths, uint32_t off1,
                        const uint32_t* __restrict__ dirs, const uint32_t* __restrict__ scr,
                        int number_scrambled, const int* __restrict__ alloc,
                        float* __restrict__ out) {
-  extern __shared__ float lds[];
-  const int bd = blockDim.x;
-  uint32_t* vall = reinterpret_cast<uint32_t*>(lds + steps * bd);  // [steps][32]
-  uint32_t* tlo = vall + steps * 32;                                 // [steps][64]
-  uint32_t* xhi = tlo + steps * 64;                                  // [steps][kMaxGroups]
-  uint32_t* sct = xhi + steps * kMaxGroups;                          // [steps]
-  const int* idx = alloc + blockIdx.y * steps;
-  for (int w = threadIdx.x; w < steps * 32; w += bd)
-    vall[w] = dirs[static_cast<size_t>(idx[w >> 5]) * 32 + (w & 31)];
-  for (int w = threadIdx.x; w < steps; w += bd) {
-    const int d = idx[w];
-    sct[w] = d < number_scrambled ? scr[d] : 0u;
-  }
+  __shared__ uint32_t vall[S * 32];  // direction numbers [bridge variate][bit]
+  __shared__ uint32_t tlo[S * 64];   // T(l) [bridge variate][l]
+  const int f = blockIdx.y;
+  const int* idx = alloc + f * S;
+  for (int q = threadIdx.x; q < S * 32; q += blockDim.x)
+    vall[q] = dirs[static_cast<size_t>(idx[q >> 5]) * 32 + (q & 31)];
   __syncthreads();
-  for (int w = threadIdx.x; w < steps * 64; w += bd) {
-    const uint32_t l = static_cast<uint32_t>(w & 63), g = l ^ (l >> 1);
-    const uint32_t* vv = vall + (w >> 6) * 32;
+  for (int q = threadIdx.x; q < S * 64; q += blockDim.x) {
+    const uint32_t l = static_cast<uint32_t>(q & 63), g = l ^ (l >> 1);
+    const uint32_t* vv = vall + (q >> 6) * 32;
     uint32_t x = 0;
 #pragma unroll
     for (int bit = 0; bit < 6; ++bit)
       if ((g >> bit) & 1u) x ^= vv[bit];
-    tlo[w] = x;
+    tlo[q] = x;
   }
-  const int ngroups = bd / 64 + 1;
-  const int half = threadIdx.x >> 5, b = threadIdx.x & 31;
-  const int halves = bd >> 5, pairs = steps * ngroups;
-  const int nchunks = (paths + bd - 1) / bd;
-  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const uint32_t nb = static_cast<uint32_t>(c) * bd + off1;  // n of lane 0
-    const uint32_t g0 = nb & ~63u;
-    __syncthreads();  // the previous chunk's lanes are done with xhi
-    // X(G) for G = g0 + 64k: lane b of a half-wavefront contributes v[b] when
-    // bit b of gray(G) is set; the half XOR-reduces.  Every half runs the
-    // same number of rounds so the shuffles see all their lanes.
-    for (int base = 0; base < pairs; base += halves) {
-      const int q = base + half;
-      const int qc = q < pairs ? q : pairs - 1;
-      const int i = qc / ngroups, k = qc % ngroups;
-      const uint32_t G = g0 + 64u * static_cast<uint32_t>(k);
-      const uint32_t g = G ^ (G >> 1);
-      uint32_t x = ((g >> b) & 1u) ? vall[i * 32 + b] : 0u;
-#pragma unroll
-      for (int o = 16; o > 0; o >>= 1)
-        x ^= static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o, 32));
-      if (b == 0 && q < pairs) xhi[i * kMaxGroups + k] = x ^ sct[i];
-    }
-    __syncthreads();
-    const int p = c * bd + threadIdx.x;
-    if (p < paths) {
-      const uint32_t n = nb + static_cast<uint32_t>(threadIdx.x);
-      const uint32_t kg = (n - g0) >> 6, l = n & 63u;
-      bridge_lane(T, steps, factors, paths, p, lds,
-                  [&](int i) { return invnorm_u32(xhi[i * kMaxGroups + kg] ^ tlo[i * 64 + l]); },
-                  out);
-    }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+  const int64_t wv = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t chunks = (static_cast<int64_t>(paths) + 63) >> 6;
+  const int c0 = static_cast<int>(wv * chunks / waves);
+  const int c1 = static_cast<int>((wv + 1) * chunks / waves);
+  if (c0 >= c1) return;
+
+  // this lane's dimension for the X(G) bookkeeping
+  const int dl = lane < S ? lane : S - 1;
+  const uint32_t* vd = vall + dl * 32;
+  const uint32_t r = off1 & 63u;
+  const bool upper = r + static_cast<uint32_t>(lane) >= 64u;  // lane sits in G + 64
+  const uint32_t l = (r + static_cast<uint32_t>(lane)) & 63u;
+  uint32_t m = static_cast<uint32_t>(c0) + (off1 >> 6);  // G = 64 m for chunk c0
+  uint32_t xc;
+  {
+    const int d = idx[dl];
+    xc = d < number_scrambled ? scr[d] : 0u;
+    const uint32_t G = m << 6, g = G ^ (G >> 1);
+    for (int b = 5; b < 32; ++b)
+      if ((g >> b) & 1u) xc ^= vd[b];
+  }
+  const uint32_t v5 = vd[5];
+  for (int c = c0; c < c1; ++c, ++m) {
+    const int t = 6 + __builtin_ctz(m + 1u);
+    const uint32_t xn = xc ^ v5 ^ (t < 32 ? vd[t] : 0u);
+    const int p = c * 64 + lane;
+    auto z = [&](int i) {
+      const uint32_t xa = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(xc), i));
+      const uint32_t xb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(xn), i));
+      return invnorm_u32((upper ? xb : xa) ^ tlo[i * 64 + l]);
+    };
+    // every lane computes (the readlanes see whole-wave values); only the
+    // lanes inside the path range store
+    bridge_store<S>(T, z, factors, f, paths, p, p < paths, out);
+    xc = xn;
   }
 }
+
+// Launch tables for every S in [1, kRegSteps].
+using RegBridgeFn = void (*)(const BridgeTables, int, int, const int*, const float*, float*);
+using RegGenFn = void (*)(const BridgeTables, int, int, uint32_t, const uint32_t*,
+                          const uint32_t*, int, const int*, float*);
+
+template <int... I>
+constexpr auto reg_bridge_table(std::integer_sequence<int, I...>) {
+  return std::array<RegBridgeFn, sizeof...(I)>{&k_bridge_reg<I + 1>...};
+}
+template <int... I>
+constexpr auto reg_gen_table(std::integer_sequence<int, I...>) {
+  return std::array<RegGenFn, sizeof...(I)>{&k_sobol_bridge_reg<I + 1>...};
+}
+const auto kRegBridge = reg_bridge_table(std::make_integer_sequence<int, kRegSteps>{});
+const auto kRegGen = reg_gen_table(std::make_integer_sequence<int, kRegSteps>{});
 
 }  // namespace
 }  // namespace koo
@@ -261,6 +357,13 @@ extern "C" int brownian_bridge_gpu(int steps, const int* allocation_indices, con
   if (paths == 0) return KOO_OK;
   if (!allocation_indices || !in || !out)
     return set_error(KOO_ERR_INVALID, "brownian_bridge_gpu: null pointer");
+  if (steps <= kRegSteps) {
+    hipLaunchKernelGGL(kRegBridge[steps - 1], dim3((paths + kBridgeThreads - 1) / kBridgeThreads,
+                                                   factors),
+                       dim3(kBridgeThreads), 0, as_stream(stream), tables_for(steps), factors, paths,
+                       allocation_indices, in, out);
+    return check_launch("brownian_bridge_gpu");
+  }
   const int threads = bridge_threads(steps);
   const size_t lds = sizeof(float) * steps * threads;
   hipLaunchKernelGGL(k_bridge, dim3((paths + threads - 1) / threads, factors), dim3(threads), lds,
@@ -277,20 +380,35 @@ extern "C" int sobol_bridge_gpu(int paths, unsigned path_offset, int steps, int 
   if (paths == 0) return KOO_OK;
   if (!d_directions || !allocation_indices || !out || (number_scrambled > 0 && !scramblers))
     return set_error(KOO_ERR_INVALID, "sobol_bridge_gpu: null pointer");
-  if (steps <= kTableSteps) {
-    // LDS: path storage steps x threads floats plus 105 words per step
-    // (direction numbers, T(l), X(G), scrambler) within 64 KiB.
-    const int threads = steps <= 40 ? 256 : 128;
-    const size_t lds = sizeof(float) * steps * threads +
-                       sizeof(uint32_t) * steps * (32 + 64 + kMaxGroups + 1);
-    // Enough blocks to fill the chip (about four per CU over all factors);
-    // each walks several chunks of paths with its tables kept in LDS.
-    const int chunks = (paths + threads - 1) / threads;
-    int per_factor = (4 * device_cu_count() + factors - 1) / factors;
-    if (per_factor > chunks) per_factor = chunks;
+  if (static_cast<uint64_t>(path_offset) + 1u + static_cast<uint64_t>(paths) > (1ull << 32))
+    return set_error(KOO_ERR_INVALID, "sobol_bridge_gpu: path_offset + paths exceeds 2^32 - 1");
+  if (steps <= kRegSteps) {
+    // One block set that is resident at once (blocks per CU from the
+    // occupancy calculator), split evenly over the factors; each wavefront
+    // then walks its own run of chunks, so there is no tail of late blocks.
+    static std::array<int, kRegSteps> resident{};
+    static std::mutex mu;
+    int per_cu;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      if (!resident[steps - 1]) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void*>(kRegGen[steps - 1]), kBridgeThreads, 0) !=
+                hipSuccess ||
+            nb < 1)
+          nb = 1;
+        resident[steps - 1] = nb;
+      }
+      per_cu = resident[steps - 1];
+    }
+    const int64_t chunks = (static_cast<int64_t>(paths) + 63) / 64;
+    int64_t per_factor = (static_cast<int64_t>(per_cu) * device_cu_count()) / factors;
+    const int64_t most = (chunks + kBridgeThreads / 64 - 1) / (kBridgeThreads / 64);
+    if (per_factor > most) per_factor = most;
     if (per_factor < 1) per_factor = 1;
-    hipLaunchKernelGGL(k_sobol_bridge_tab, dim3(per_factor, factors),
-                       dim3(threads), lds, as_stream(stream), tables_for(steps), steps, factors,
+    hipLaunchKernelGGL(kRegGen[steps - 1], dim3(static_cast<unsigned>(per_factor), factors),
+                       dim3(kBridgeThreads), 0, as_stream(stream), tables_for(steps), factors,
                        paths, path_offset + 1u, d_directions, scramblers, number_scrambled,
                        allocation_indices, out);
     return check_launch("sobol_bridge_gpu");

@@ -52,10 +52,19 @@ __device__ __forceinline__ float invnorm_u32(uint32_t x) {
   const bool upper = x >= 0x80000000u;
   const uint32_t t = upper ? (0u - x) : x;
   const float tf = (t == 0u) ? 0.5f : static_cast<float>(t);
-  const float z = -logf(tf * 4.656612873077392578125e-10f);  // w = t * 2^-31
+  // w = t * 2^-31 lies in [2^-32, 1]: always a normal number, so the
+  // hardware log2 (v_log_f32) needs none of logf's range handling;
+  // z = -ln(w) = -ln2 * log2(w).
+  const float w = tf * 4.656612873077392578125e-10f;
+  const float z = __builtin_amdgcn_logf(w) * -0.693147180559945309417f;
   const float p = P1 + z * (P2 + z * (P3 + z * (P4 + (P5 + P6 * z) * z)));
   const float q = 1.0f + z * (Q2 + z * (Q3 + z * (Q4 + (Q5 + Q6 * z) * z)));
-  const float r = z * p / q;
+  // z p / q by the hardware reciprocal and one Newton correction of the
+  // quotient (q >= 1, so nothing overflows or underflows)
+  const float n = z * p;
+  const float rq = __builtin_amdgcn_rcpf(q);
+  const float r0 = n * rq;
+  const float r = fmaf(fmaf(-q, r0, n), rq, r0);
   return upper ? r : -r;
 }
 

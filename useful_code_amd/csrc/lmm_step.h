@@ -102,44 +102,55 @@ __device__ __forceinline__ void fill_fixed_drifts(const Model& m, float* ctab) {
 // deeper queue hides HBM latency when the state comes from global memory),
 // `store(r, l)` receives the end log-rate; e / ep return the final drift
 // sums; cs = fixed drifts of step s.
+//
+// The rate loop is unrolled by PF and slot u of the queue always serves rates
+// with (r - s) % PF == u, so no queue register is ever copied: copying a
+// register whose load is still in flight waits for that load, which would
+// turn the queue back into one load in flight.  Tail loads are clamped to
+// the last rate (in bounds, never used).
 template <int F, int PF = 2, class Load, class Store>
 __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
                                         const float (&z)[F], float (&e)[F], float (&ep)[F],
                                         Load load, Store store) {
   const float* As = m.A + s * m.N * F;  // 32-bit: keeps the address scalar
+  const int cnt = m.N - s;
 #pragma unroll
   for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
   float q[PF];
 #pragma unroll
-  for (int i = 0; i < PF; ++i) q[i] = (s + i < m.N) ? load(s + i) : 0.0f;
-  for (int r = s; r < m.N; ++r) {
-    const float l = q[0];
+  for (int i = 0; i < PF; ++i) q[i] = load(s + (i < cnt ? i : cnt - 1));
+  for (int b = 0; b < cnt; b += PF) {
 #pragma unroll
-    for (int i = 0; i + 1 < PF; ++i) q[i] = q[i + 1];
-    q[PF - 1] = (r + PF < m.N) ? load(r + PF) : 0.0f;
-    float a[F];
+    for (int u = 0; u < PF; ++u) {
+      if (b + u >= cnt) break;
+      const int r = s + b + u;
+      const float l = q[u];
+      float a[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
-    const float tau = m.taus[r], d = m.disp[r];
-    const float x = fast_exp(l);
-    const float g = tau * x * fast_rcp(1.0f + tau * (x - d));
-    float mu = 0.0f, dw = 0.0f;
+      for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
+      const float tau = m.taus[r], d = m.disp[r];
+      const float x = fast_exp(l);
+      const float g = tau * x * fast_rcp(1.0f + tau * (x - d));
+      float mu = 0.0f, dw = 0.0f;
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      e[f] += g * a[f];
-      mu += a[f] * e[f];
-      dw += a[f] * z[f];
+      for (int f = 0; f < F; ++f) {
+        e[f] += g * a[f];
+        mu += a[f] * e[f];
+        dw += a[f] * z[f];
+      }
+      const float lh = l + mu + cs[r] + dw;
+      const float xh = fast_exp(lh);
+      const float gh = tau * xh * fast_rcp(1.0f + tau * (xh - d));
+      float muh = 0.0f;
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        ep[f] += gh * a[f];
+        muh += a[f] * ep[f];
+      }
+      store(r, lh + 0.5f * (muh - mu));
+      // refill after l's last use, so the load can land in l's register
+      q[u] = load(s + (b + u + PF < cnt ? b + u + PF : cnt - 1));
     }
-    const float lh = l + mu + cs[r] + dw;
-    const float xh = fast_exp(lh);
-    const float gh = tau * xh * fast_rcp(1.0f + tau * (xh - d));
-    float muh = 0.0f;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      ep[f] += gh * a[f];
-      muh += a[f] * ep[f];
-    }
-    store(r, lh + 0.5f * (muh - mu));
   }
 }
 
@@ -159,66 +170,71 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
                                              Vega vega) {
   const int N = m.N;
   const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
+  const int cnt = N - s;
   float E[F], Eh[F];
 #pragma unroll
   for (int f = 0; f < F; ++f) E[f] = Eh[f] = 0.0f;
-  // start log-rates issued PF rates ahead, adjoints (LDS) one rate ahead
+  // start log-rates issued PF rates ahead into fixed slots (rate loop
+  // unrolled by PF, as in pc_step), adjoints (LDS) one rate ahead
   float q[PF];
 #pragma unroll
-  for (int i = 0; i < PF; ++i) q[i] = (N - 1 - i >= s) ? src(N - 1 - i) : 0.0f;
+  for (int i = 0; i < PF; ++i) q[i] = src(N - 1 - (i < cnt ? i : cnt - 1));
   float lp_next = lam[(N - 1) * bd];
-  for (int r = N - 1; r >= s; --r) {
-    const float l = q[0];
-    float lp = lp_next;
+  for (int b = 0; b < cnt; b += PF) {
 #pragma unroll
-    for (int i = 0; i + 1 < PF; ++i) q[i] = q[i + 1];
-    q[PF - 1] = (r - PF >= s) ? src(r - PF) : 0.0f;
-    if (r > s) lp_next = lam[(r - 1) * bd];
-    float a[F];
+    for (int u = 0; u < PF; ++u) {
+      if (b + u >= cnt) break;
+      const int r = N - 1 - b - u;
+      const float l = q[u];
+      float lp = lp_next;
+      if (r > s) lp_next = lam[(r - 1) * bd];
+      float a[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
-    const float tau = m.taus[r], d = m.disp[r];
-    const float omd = 1.0f - tau * d;
-    const float x = fast_exp(l);
-    const float inv1 = fast_rcp(1.0f + tau * (x - d));
-    const float g = tau * x * inv1;
-    float mu = 0.0f, muh = 0.0f, dw = 0.0f;
+      for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
+      const float tau = m.taus[r], d = m.disp[r];
+      const float omd = 1.0f - tau * d;
+      const float x = fast_exp(l);
+      const float inv1 = fast_rcp(1.0f + tau * (x - d));
+      const float g = tau * x * inv1;
+      float mu = 0.0f, muh = 0.0f, dw = 0.0f;
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      mu += a[f] * e[f];
-      muh += a[f] * ep[f];
-      dw += a[f] * z[f];
-    }
-    const float c = cs[r];
-    const float lh = l + mu + c + dw;
-    const float xh = fast_exp(lh);
-    const float inv2 = fast_rcp(1.0f + tau * (xh - d));
-    const float gh = tau * xh * inv2;
-    if (r == s) {
-      const float xs = fast_exp(lh + 0.5f * (muh - mu));
-      const float fs = xs - d;
-      fixing(lp, xs, fs, fast_rcp(1.0f + tau * fs));
-    }
-    float ghb = 0.0f;
+      for (int f = 0; f < F; ++f) {
+        mu += a[f] * e[f];
+        muh += a[f] * ep[f];
+        dw += a[f] * z[f];
+      }
+      const float c = cs[r];
+      const float lh = l + mu + c + dw;
+      const float xh = fast_exp(lh);
+      const float inv2 = fast_rcp(1.0f + tau * (xh - d));
+      const float gh = tau * xh * inv2;
+      if (r == s) {
+        const float xs = fast_exp(lh + 0.5f * (muh - mu));
+        const float fs = xs - d;
+        fixing(lp, xs, fs, fast_rcp(1.0f + tau * fs));
+      }
+      float ghb = 0.0f;
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      Eh[f] += 0.5f * lp * a[f];
-      ghb += a[f] * Eh[f];
-    }
-    const float lhat = lp + ghb * gh * omd * inv2;
-    const float mub = lhat - 0.5f * lp;
-    float gb = 0.0f;
+      for (int f = 0; f < F; ++f) {
+        Eh[f] += 0.5f * lp * a[f];
+        ghb += a[f] * Eh[f];
+      }
+      const float lhat = lp + ghb * gh * omd * inv2;
+      const float mub = lhat - 0.5f * lp;
+      float gb = 0.0f;
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      E[f] += mub * a[f];
-      gb += a[f] * E[f];
-    }
-    lam[r * bd] = lhat + gb * g * omd * inv1;
-    vega(r, 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c));
+      for (int f = 0; f < F; ++f) {
+        E[f] += mub * a[f];
+        gb += a[f] * E[f];
+      }
+      lam[r * bd] = lhat + gb * g * omd * inv1;
+      vega(r, 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c));
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      e[f] -= g * a[f];
-      ep[f] -= gh * a[f];
+      for (int f = 0; f < F; ++f) {
+        e[f] -= g * a[f];
+        ep[f] -= gh * a[f];
+      }
+      q[u] = src(N - 1 - (b + u + PF < cnt ? b + u + PF : cnt - 1));
     }
   }
 }

@@ -805,7 +805,7 @@ extern "C" int ls_first_pass_gpu(int n_ex, const float* basis_variables, const i
   float* Y = reinterpret_cast<float*>(ws + lay.y);
   double* parts[2] = {reinterpret_cast<double*>(ws + lay.part),
                       reinterpret_cast<double*>(ws + lay.part2)};
-  const int nb = ls_blocks(paths), ng = gram_blocks(paths);
+  const int ng = gram_blocks(paths);
   hipStream_t st = as_stream(stream);
   const bool cross = use_cross != 0;
   DateArgs a{};
