@@ -53,6 +53,31 @@ def rate_updates_per_path(N):
 LS_DATE_BYTES_PER_PATH = 4 * (2 + 1 + 1 + 2 + 1 + 1 + 1 + 1)
 
 
+def profiled_traffic(kernel, grid):
+    """HBM bytes per dispatch of `kernel` at `grid` (total threads) from the
+    rocprofv3 counter summaries committed under profiles/ (their
+    `traffic_json:` lines: FETCH_SIZE + WRITE_SIZE scaled by a plain-copy
+    calibration), with the summary's file name; (None, None) when absent."""
+    import glob
+    for fn in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(fn) as f:
+            for line in f:
+                if not line.startswith("traffic_json: "):
+                    continue
+                rec = json.loads(line[len("traffic_json: "):])
+                if rec.get("kernel") == kernel and rec.get("grid") == grid and rec.get("hbm_bytes"):
+                    return float(rec["hbm_bytes"]), os.path.relpath(fn, ROOT)
+    return None, None
+
+
+def with_traffic(roof, kernel, grid):
+    t, src = profiled_traffic(kernel, grid)
+    roof["traffic"] = t
+    if src:
+        roof["traffic_source"] = src
+    return roof
+
+
 def bermudan_section(args, torch, O, configs, pipeline, ev):
     """Config-3 Longstaff-Schwartz timings, roofline and error."""
     w = configs.bermudan_swaption(paths=args.bermudan_paths)
@@ -108,9 +133,10 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
         "price_first_pass": price1,
         "price_second_pass_same_paths": price2,
         "itm_fraction_mean": itm,
-        "roofline": {
-            "kernel": "LS first pass: per date k_ls_date (decision update, MFMA normal "
-                      "equations) + k_ls_solve (FP64 sum, Cholesky), then k_ls_final",
+        "roofline": with_traffic({
+            "kernel": "LS first pass (ls_first_pass_gpu): per date k_ls_date (the later "
+                      "date's FP64 sum and Cholesky solve, decision update, MFMA normal "
+                      "equations), then k_ls_final",
             "bound": "hbm",
             "achieved": gbs,
             "peak": HBM_PEAK_GBS,
@@ -119,7 +145,7 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
             "traffic": None,
             "algorithmic_bytes": stage_bytes,
             "mfma_useful_tflops": mfma_flops / (regress * 1e-3) / 1e12,
-        },
+        }, "ls_first_pass_gpu", P),
     }
     if not args.no_error_check:
         ints = O.scramble(O.sobol_ints(P, w.dims, 0), w.scramblers, w.dims)
@@ -263,7 +289,7 @@ def basket_section(args, torch, O, configs, pipeline, ev):
         "job_ms": wall,
         "kernels_ms": {"sobol_bridge": gen, "basket_asian": kern},
         "price": float(out[0]),
-        "roofline": {
+        "roofline": with_traffic({
             "kernel": "k_basket_asian<2> (MFMA correlation, log-Euler, exp, adjoint sums)",
             "bound": "hbm",
             "achieved": gbs,
@@ -273,7 +299,7 @@ def basket_section(args, torch, O, configs, pipeline, ev):
             "traffic": None,
             "algorithmic_bytes": zbytes,
             "mfma_useful_tflops": mfma_tflops,
-        },
+        }, "k_basket_asian<2>", P // 4),
     }
     if not args.no_error_check:
         ws = configs.basket_asian(paths=1 << 14)
@@ -384,7 +410,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--paths", type=int, default=1 << 20)
     ap.add_argument("--checkpoint-every", type=int, default=8)
-    ap.add_argument("--cpu-sample-paths", type=int, default=1 << 17)
+    ap.add_argument("--cpu-sample-paths", type=int, default=1 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--no-bermudan", action="store_true")
@@ -536,15 +562,15 @@ def main():
         "kernels_ms": {"sobol_bridge": gen_ms, "price_fwd_checkpoints": fwdck_ms,
                        "price_bwd": bwd_ms, "price_fwd": fwd_ms},
         "checkpoint_every_1": k1,
-        "roofline": {
-            "kernel": "k_price_bwd (reverse sweep + segment recompute)",
+        "roofline": with_traffic({
+            "kernel": f"k_price_bwd<{F}> (reverse sweep + segment recompute)",
             "bound": "valu",
             "achieved": achieved_tflops,
             "peak": FP32_VECTOR_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_VECTOR_PEAK_TFLOPS,
             "traffic": None,
-        },
+        }, f"k_price_bwd<{F}>", P),
     }
 
     if not args.no_error_check:
@@ -574,9 +600,10 @@ def main():
             "unit": "path-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"FP64 oracle (oracle/koo_oracle.c, OpenMP over paths, gcc -O2) pricing "
-                      f"the same workload with all 80 adjoint Greeks on its first "
-                      f"{wc.paths} paths: {dt:.2f} s",
+            "sample": f"FP64 oracle (oracle/koo_oracle.c, OpenMP over paths, gcc -O3): "
+                      f"quantiles, bridge, evolution, caplets and all 80 adjoint Greeks of the "
+                      f"same workload on {wc.paths} paths (the Sobol integers generated "
+                      f"beforehand): {dt:.2f} s",
         }
 
     if not args.no_bermudan:

@@ -136,6 +136,24 @@ int sobol_bridge_gpu(int paths, unsigned path_offset, int steps, int factors,
                      int number_scrambled, const int* allocation_indices, float* out,
                      void* stream);
 
+/* Bridge_gpu.h:11 (brownianBridgeGPU_constant_memory): the legacy
+ * power-of-two bridge (BrownianBridge<float>, gold/Bridge_gold.h:168-252) on
+ * n_vectors vectors of 2^n_poweroftwo points, [points][vectors] in and out
+ * (differenced).  n_poweroftwo <= 8; the construction scalars travel in the
+ * kernel arguments (no constant-memory state, so calls are reentrant). */
+int brownianBridgeGPU_constant_memory(int n_vectors, int n_poweroftwo, const float* input,
+                                      float* output, void* stream);
+
+/* Bridge_multi_gpu.h:12-20 (brownianBridgeMultiGPUReorder): Sobol dimensions
+ * [2^n F][P] to the (points, paths, factors) arrangement the power-of-two
+ * bridge then transforms as points x (paths * factors) vectors (the
+ * reference's MultiDBridge, Bridge_main.cu:111-173).  allocator: 0 row,
+ * 1 column, 2 triangular (BrownianBridgeMultiDim<float>::ordering); useTextures
+ * is ignored. */
+int brownianBridgeMultiGPUReorder(const float* input, float* output, int n_poweroftwo,
+                                  int factors, int number_paths, int allocator, int useTextures,
+                                  void* stream);
+
 /* out(s,r,p) = sum_f A(s,r,f) in(s,p,f) + drift_fixed(s,r) for r >= alive[s].
  * Replaces correlated_drift_paths_GPU, Correlate_drift_gpu.h:12. */
 int correlated_drift_paths_GPU(float* input_data_device, float* output_data_device,
@@ -160,6 +178,31 @@ int LMM_evolver_pc_all_gpu(float* initial_rates_device, float* initial_log_rates
                            float* evolved_log_rates_device, float* discounts_device,
                            int sharedMemForLogs, int transposeData, void* stream,
                            int numberThreads, int doDiscounts);
+
+/* The per-step evolvers on correlated increments (S,N,P) = A z + fixed
+ * drift (the output of correlated_drift_paths_GPU): predictor-corrector
+ * (replaces LMM_evolver_pc_gpu, LMM_evolver_gpu.h:16-32) and log-Euler with
+ * the log of the displaced rate capped at log 10 (replaces
+ * LMM_evolver_euler_gpu, LMM_evolver_gpu.h:35-50), both in one launch over
+ * all steps.  alive_indices_host (host, `steps` entries) replaces the
+ * reference's std::vector aliveIndices; step 0 evolves every rate from the
+ * initial drifts, later steps the rates r >= alive[s] (the other entries of
+ * the output cubes are not written).  e buffers and useShared are accepted
+ * and ignored. */
+int LMM_evolver_pc_gpu(float* initial_rates_device, float* initial_log_rates_device,
+                       float* taus_device, float* correlated_increments_device,
+                       float* pseudo_roots_device, float* initial_drifts_device,
+                       float* displacements_device, const int* alive_indices_host, int paths,
+                       int factors, int steps, int rates, float* e_buffer_device,
+                       float* e_buffer_pred_device, float* evolved_rates_device,
+                       float* evolved_log_rates_device, int useShared, void* stream);
+int LMM_evolver_euler_gpu(float* initial_rates_device, float* initial_log_rates_device,
+                          float* taus_device, float* correlated_increments_device,
+                          float* pseudo_roots_device, float* initial_drifts_device,
+                          float* displacements_device, const int* alive_indices_host, int paths,
+                          int factors, int steps, int rates, float* e_buffer_device,
+                          float* evolved_rates_device, float* evolved_log_rates_device,
+                          void* stream);
 
 /* P(s,alive) = 1, P(s,r+1) = P(s,r)/(1 + tau_r f(s,r)).  alive_host has
  * `steps` entries (host copy of the reference's aliveIndices vector),
@@ -433,6 +476,21 @@ int reduceOuterProductSymmetric_cublas(int paths, int row_size, float alpha, flo
 int reduceOuterProduct_cublas(int paths, int row_size, int col_size, float alpha, float beta,
                               const float* input_data_global1, const float* input_data_global2,
                               float* answer_global, float* workspace_data_global, void* stream);
+
+/* outerProduct_gpu.h:94-115: the per-path products whose path sums are
+ * the outer products.  PointwiseProduct: out[p] = in1[p] in2[p];
+ * SymmetricUpperTriangular: in (rows, paths) -> out (rows, rows, paths) with
+ * out[r][c][p] = in[r][p] in[c][p] for c >= r (the lower triangle is not
+ * written); MultipleFirst: out[r][p] = in1[r][p] in2[p].  blocks and
+ * threads are accepted for signature parity and ignored. */
+int PointwiseProduct_gpu(int blocks, int threads, int paths, const float* in1_global,
+                         const float* in2_global, float* out_global, void* stream);
+int PointwiseProductSymmetricUpperTriangular_gpu(int blocks, int threads, int paths, int rows,
+                                                 const float* in_global, float* out_global,
+                                                 void* stream);
+int PointwiseProductMultipleFirst_gpu(int blocks, int threads, int paths, const float* in1_global,
+                                      const float* in2_global, float* out_global, int rows,
+                                      void* stream);
 
 /* New: Bermudan payer swaption path quantities fused with the LMM evolution
  * (same model as lmm_price_fwd).  exercise_steps (HOST, increasing) are

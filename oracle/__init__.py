@@ -87,12 +87,16 @@ def _declare(L):
         "orc_bridge_allocate_diagonal": (None, [_c_int, _c_int, _i32p]),
         "orc_bridge_tables": (None, [_c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p]),
         "orc_bridge_transform": (None, [_c_int, _c_int, _c_int, _i32p, _f64p, _f64p]),
+        "orc_bridge_pow2": (None, [_c_int, _c_int, _f64p, _f64p]),
+        "orc_bridge_reorder": (None, [_f64p, _f64p, _c_int, _c_int, _c_int, _c_int]),
         "orc_correlate_drift": (None, [_f64p, _f64p, _f64p, _i32p, _f64p, _c_int, _c_int,
                                        _c_int, _c_int, _c_int]),
         "orc_spot_drifts": (None, [_f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _f64p]),
         "orc_lmm_pc_evolve": (None, [_f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                      _i32p, _c_int, _c_int, _c_int, _c_int, _c_int,
                                      _f64p, _f64p, _f64p]),
+        "orc_lmm_steps": (None, [_f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _i32p, _c_int, _c_int,
+                                 _c_int, _c_int, _c_int, _f64p, _f64p]),
         "orc_discount_ratios": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
         "orc_coterminal_annuities": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
         "orc_coterminal_swap_rates": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
@@ -155,11 +159,15 @@ def _declare_ref(R):
         "ref_scramble": (None, [_u32p, _u32p, _u32p, _c_int, _c_int]),
         "ref_invnorm_sb": (ctypes.c_float, [_c_u32]),
         "ref_bridge_allocate_diagonal": (None, [_c_int, _c_int, _i32p]),
+        "ref_bridge_pow2": (None, [_c_int, _c_int, _f64p, _f64p]),
+        "ref_bridge_multi": (None, [_c_int, _c_int, _c_int, _c_int, _f64p, _f64p]),
         "ref_correlate_drift": (None, [_f64p, _f64p, _f64p, _c_int, _f64p, _c_int, _c_int,
                                        _c_int, _c_int]),
         "ref_spot_drifts": (None, [_f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _f64p]),
         "ref_lmm_pc": (None, [_f64p, _f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _c_int,
                               _f64p, _f64p]),
+        "ref_lmm_euler": (None, [_f64p, _f64p, _f64p, _f64p, _f64p, _c_int, _c_int, _c_int,
+                                 _c_int, _f64p, _f64p]),
         "ref_discount_ratios": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
         "ref_annuities": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
         "ref_cot_swap_rates": (None, [_f64p, _f64p, _i32p, _c_int, _c_int, _c_int, _f64p]),
@@ -313,6 +321,46 @@ def lmm_pc_evolve(f0, taus, variates, A, disp, alive, transposed=True, initial_d
                             fx.reshape(-1), disp, alive, P, F, S, N, int(transposed),
                             rates, logs, disc)
     return rates.reshape(S, N, P), logs.reshape(S, N, P), disc.reshape(S, N + 1, P)
+
+
+def bridge_pow2(log2, z):
+    """Legacy power-of-two bridge of z [2^log2][n] (differenced)."""
+    z = np.ascontiguousarray(z, np.float64)
+    out = np.zeros_like(z)
+    lib().orc_bridge_pow2(int(log2), z.shape[1], z.reshape(-1), out.reshape(-1))
+    return out
+
+
+def bridge_reorder(z, log2, factors, ordering):
+    """[D][P] -> (S, P, F) arrangement of the multi-factor power-of-two bridge."""
+    z = np.ascontiguousarray(z, np.float64)
+    P = z.shape[1]
+    out = np.zeros(((1 << log2), P, factors))
+    lib().orc_bridge_reorder(z.reshape(-1), out.reshape(-1), int(log2), int(factors), P,
+                             int(ordering))
+    return out
+
+
+def lmm_steps(f0, taus, incr, A, disp, alive, euler=False, initial_drifts=None):
+    """The multi-kernel PC (or Euler) evolver on correlated increments
+    incr (S,N,P) = A z + fixed drift; returns rates and log-rates (S,N,P)
+    (entries of dead rates, r < alive[s] for s >= 1, stay zero)."""
+    f0 = np.ascontiguousarray(f0, np.float64)
+    taus = np.ascontiguousarray(taus, np.float64)
+    disp = np.ascontiguousarray(disp, np.float64)
+    A = np.ascontiguousarray(A, np.float64)
+    S, N, F = A.shape
+    incr = np.ascontiguousarray(incr, np.float64)
+    P = incr.shape[2]
+    alive = np.ascontiguousarray(alive, np.int32)
+    logs0 = np.log(f0 + disp)
+    mu0 = spot_drifts(A[0], f0, taus, disp, 0) if initial_drifts is None else \
+        np.ascontiguousarray(initial_drifts, np.float64)
+    rates = np.zeros(S * N * P)
+    logs = np.zeros(S * N * P)
+    lib().orc_lmm_steps(logs0, taus, incr.reshape(-1), A.reshape(-1), mu0, disp, alive, P, F, S, N,
+                        int(euler), rates, logs)
+    return rates.reshape(S, N, P), logs.reshape(S, N, P)
 
 
 def caplet_model(f0, taus, disp, A, sigma, strike, paths):

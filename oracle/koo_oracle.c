@@ -249,6 +249,81 @@ void orc_bridge_transform(int steps, int factors, int paths, const int *indices,
   free(w);
 }
 
+/* The legacy power-of-two bridge on [S][vectors], S = 2^log2, differenced
+ * output.  gold/Bridge_gold.h:168-252 (BrownianBridge<D>): last point
+ * 2^(n/2) z_0; one-sided points 2^(n-i-1) - 1 = right/2 + sqrt(2^(n-i-1)/2) z;
+ * then per level i = 2..n the midpoints j 2^(n-i) - 1 (j odd >= 3) =
+ * (left + right)/2 + sqrt(2^(n-i)/2) z; point k takes the variate whose
+ * number is its place in that construction order. */
+void orc_bridge_pow2(int log2, int vectors, const double *in, double *out) {
+  const int S = 1 << log2;
+  int *var = (int *)malloc(sizeof(int) * S);
+  int c = 0;
+  for (int i = 0; i <= log2; ++i) {
+    const int m = 1 << (log2 - i);
+    for (int j = 1; j <= (1 << i); j += 2) var[j * m - 1] = c++;
+  }
+#pragma omp parallel
+  {
+    double *w = (double *)malloc(sizeof(double) * S);
+#pragma omp for schedule(static)
+    for (int p = 0; p < vectors; ++p) {
+#define ORC_Z(k) in[(size_t)var[k] * vectors + p]
+      w[S - 1] = ORC_Z(S - 1) * pow(2.0, 0.5 * log2);
+      for (int i = 0; i < log2; ++i) {
+        const int k = (1 << (log2 - i - 1)) - 1, right = (1 << (log2 - i)) - 1;
+        w[k] = 0.5 * w[right] + sqrt(0.5 * pow(2.0, log2 - i - 1.0)) * ORC_Z(k);
+      }
+      for (int i = 2; i <= log2; ++i) {
+        const int m = 1 << (log2 - i);
+        for (int j = 3; j < (1 << i); j += 2) {
+          const int k = j * m - 1;
+          w[k] = 0.5 * w[k + m] + 0.5 * w[k - m] + sqrt(m * 0.5) * ORC_Z(k);
+        }
+      }
+#undef ORC_Z
+      for (int k = S - 1; k >= 1; --k) w[k] -= w[k - 1];
+      for (int k = 0; k < S; ++k) out[(size_t)k * vectors + p] = w[k];
+    }
+    free(w);
+  }
+  free(var);
+}
+
+/* Sobol dimension i of path p to (variate v(i), factor f(i)) of an
+ * (S, P, F) array, the orderings of BrownianBridgeMultiDim
+ * (gold/Bridge_gold.h:291-331): 0 row (v = i / F, f = i % F), 1 column
+ * (v = i % S, f = i / S), 2 triangular (anti-diagonal walk).
+ * Bridge_multi_gpu.cu:67-112. */
+void orc_bridge_reorder(const double *in, double *out, int log2, int factors, int paths,
+                        int ordering) {
+  const int S = 1 << log2, N = S * factors;
+  int crow = 0, ccol = 0;
+  for (int i = 0; i < N; ++i) {
+    int v, f;
+    if (ordering == 1) {
+      v = i % S;
+      f = i / S;
+    } else if (ordering == 0) {
+      v = i / factors;
+      f = i % factors;
+    } else {
+      v = crow;
+      f = ccol;
+      if (ccol > 0 && crow < S - 1) {
+        --ccol;
+        ++crow;
+      } else {
+        const int rpc = crow + ccol + 1;
+        ccol = rpc < factors - 1 ? rpc : factors - 1;
+        crow = rpc - ccol;
+      }
+    }
+    for (int p = 0; p < paths; ++p)
+      out[((size_t)v * paths + p) * factors + f] = in[(size_t)i * paths + p];
+  }
+}
+
 /* y(s,r,p) = sum_f A(s,r,f) z(s,p,f) + mu_fixed(s,r) for r >= alive(s).
  * correlate_drift_gpu.cu:142-220 (correlation_drift_kernel); gold twin
  * gold/correlate_drift_gold.h:22-61 (which fills every r). */
@@ -345,6 +420,66 @@ void orc_lmm_pc_evolve(const double *initial_rates, const double *initial_log_ra
         rates_out[((size_t)s * rates + r) * SP + p] = rate;
         df /= (1.0 + rate * tau);
         discounts_out[((size_t)s * (rates + 1) + r + 1) * SP + p] = df;
+      }
+    }
+  }
+}
+
+/* The multi-kernel evolvers on correlated increments incr (S,N,P) = A z +
+ * fixed drift: LMM_evolver_pc_gpu (LMM_evolver_gpu.cu:366-605, one launch
+ * per step) and LMM_evolver_euler_gpu (:835-1010).  Step 0 takes the initial
+ * log-rates and drifts for every rate; step s >= 1 rebuilds the drift over
+ * r >= alive[s] from the previous step's rates, g = (f tau + d tau) /
+ * (1 + f tau), e_f += g a_f, mu = sum_f a_f e_f (the gold computes the same
+ * sums from rate 0 with zero loadings for dead rates,
+ * gold/LMM_evolver_gold.h:235-389).  PC: predictor with the predicted rates'
+ * drift, log-rate += (mu^ - mu)/2.  Euler: the log of the displaced rate is
+ * capped at log 10 (LMM_evolver_gpu.cu:20). */
+void orc_lmm_steps(const double *initial_log_rates, const double *taus, const double *incr,
+                   const double *A, const double *initial_drifts, const double *displacements,
+                   const int *alive, int paths, int factors, int steps, int rates, int euler,
+                   double *rates_out, double *log_rates_out) {
+  const size_t SP = (size_t)paths;
+  const double ceiling = log(10.0); /* the gold's ceiling; the GPU code's is its float rounding */
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < paths; ++p) {
+    double e[64], ep[64];
+    for (int s = 0; s < steps; ++s) {
+      const int a = s == 0 ? 0 : alive[s];
+      const double *As = A + (size_t)s * rates * factors;
+      for (int f = 0; f < factors; ++f) e[f] = ep[f] = 0.0;
+      for (int r = a; r < rates; ++r) {
+        const double *Ar = As + r * factors;
+        const double tau = taus[r], d = displacements[r];
+        double mu, lf;
+        if (s == 0) {
+          mu = initial_drifts[r];
+          lf = initial_log_rates[r];
+        } else {
+          const double fr = rates_out[((size_t)(s - 1) * rates + r) * SP + p];
+          const double g = (fr * tau + d * tau) / (1.0 + fr * tau);
+          mu = 0.0;
+          for (int f = 0; f < factors; ++f) {
+            e[f] += g * Ar[f];
+            mu += Ar[f] * e[f];
+          }
+          lf = log_rates_out[((size_t)(s - 1) * rates + r) * SP + p];
+        }
+        lf += mu + incr[((size_t)s * rates + r) * SP + p];
+        if (euler) {
+          if (lf > ceiling) lf = ceiling;
+        } else {
+          const double xp = exp(lf);
+          const double gp = xp * tau / (1.0 + (xp - d) * tau);
+          double mup = 0.0;
+          for (int f = 0; f < factors; ++f) {
+            ep[f] += gp * Ar[f];
+            mup += Ar[f] * ep[f];
+          }
+          lf += 0.5 * (mup - mu);
+        }
+        log_rates_out[((size_t)s * rates + r) * SP + p] = lf;
+        rates_out[((size_t)s * rates + r) * SP + p] = exp(lf) - d;
       }
     }
   }

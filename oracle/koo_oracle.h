@@ -41,6 +41,9 @@ void orc_bridge_tables(int steps, int *bridge_index, int *left_index, int *right
                        double *left_weight, double *right_weight, double *std_dev);
 void orc_bridge_transform(int steps, int factors, int paths, const int *indices,
                           const double *in /* [D][P] */, double *out /* (S,F,P) */);
+void orc_bridge_pow2(int log2, int vectors, const double *in /* [S][n] */, double *out);
+void orc_bridge_reorder(const double *in /* [D][P] */, double *out /* (S,P,F) */, int log2,
+                        int factors, int paths, int ordering);
 void orc_correlate_drift(const double *in /* (S,P,F) */, double *out /* (S,N,P) */,
                          const double *A, const int *alive, const double *drift_fixed,
                          int a_offset_per_step, int factors, int out_dims, int paths, int steps);
@@ -53,6 +56,10 @@ void orc_lmm_pc_evolve(const double *initial_rates, const double *initial_log_ra
                        const double *displacements, const int *alive, int paths, int factors,
                        int steps, int rates, int transposed_variates,
                        double *rates_out, double *log_rates_out, double *discounts_out);
+void orc_lmm_steps(const double *initial_log_rates, const double *taus, const double *incr,
+                   const double *A, const double *initial_drifts, const double *displacements,
+                   const int *alive, int paths, int factors, int steps, int rates, int euler,
+                   double *rates_out, double *log_rates_out);
 void orc_discount_ratios(const double *rates, const double *taus, const int *alive, int paths,
                          int steps, int n_rates, double *discounts);
 void orc_coterminal_annuities(const double *discounts, const double *taus, const int *alive,

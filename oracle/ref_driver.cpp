@@ -11,6 +11,7 @@
 #include <vector>
 
 #include <gold/Bridge_allocator_gold.h>
+#include <gold/Bridge_gold.h>
 #include <gold/Correlation_gold.h>
 #include <gold/LMM_evolver_full_gold.h>
 #include <gold/LMM_evolver_gold.h>
@@ -56,6 +57,35 @@ void ref_bridge_allocate_diagonal(int factors, int steps, int* idx) {
   std::memcpy(idx, v.data(), v.size() * sizeof(int));
 }
 
+// gold/Bridge_gold.h:168-252 BrownianBridge<double>::GenerateBridge on each
+// vector of [S][n]
+void ref_bridge_pow2(int log2, int vectors, const double* in, double* out) {
+  BrownianBridge<double> bb(log2);
+  const int S = bb.dimensions();
+  std::vector<double> v(S), o(S);
+  for (int p = 0; p < vectors; ++p) {
+    for (int k = 0; k < S; ++k) v[k] = in[(size_t)k * vectors + p];
+    bb.GenerateBridge(&v[0], &o[0]);
+    for (int k = 0; k < S; ++k) out[(size_t)k * vectors + p] = o[k];
+  }
+}
+
+// gold/Bridge_gold.h:291-400 BrownianBridgeMultiDim<double>::GenerateBridge
+// per path ([F][S] per path in the gold), written as (S, P, F)
+void ref_bridge_multi(int log2, int factors, int ordering, int paths, const double* in,
+                      double* out) {
+  BrownianBridgeMultiDim<double> bb(log2, factors,
+                                    static_cast<BrownianBridgeMultiDim<double>::ordering>(ordering));
+  const int S = 1 << log2, N = S * factors;
+  std::vector<double> v(N), o(N);
+  for (int p = 0; p < paths; ++p) {
+    for (int i = 0; i < N; ++i) v[i] = in[(size_t)i * paths + p];
+    bb.GenerateBridge(&v[0], &o[0]);
+    for (int f = 0; f < factors; ++f)
+      for (int s = 0; s < S; ++s) out[((size_t)s * paths + p) * factors + f] = o[f * S + s];
+  }
+}
+
 // gold/correlate_drift_gold.h:22 correlate_drift_paths_gold<double>
 void ref_correlate_drift(const double* in, double* out, const double* A, int a_off, const double* drift,
                          int factors, int out_dims, int paths, int steps) {
@@ -86,6 +116,19 @@ void ref_lmm_pc(const double* f0, const double* taus, const double* incr, const 
   std::vector<double> a = vec(A, (size_t)steps * rates * factors);
   std::vector<double> ro, lo;
   LMM_evolver_pc_gold<double>(i, t, c, a, d, paths, factors, steps, rates, ro, lo, 100.0);
+  std::memcpy(rates_out, ro.data(), ro.size() * sizeof(double));
+  std::memcpy(logs_out, lo.data(), lo.size() * sizeof(double));
+}
+
+// gold/LMM_evolver_gold.h:236 LMM_evolver_Euler_gold<double>
+void ref_lmm_euler(const double* f0, const double* taus, const double* incr, const double* A,
+                   const double* disp, int paths, int factors, int steps, int rates,
+                   double* rates_out, double* logs_out) {
+  std::vector<double> i = vec(f0, rates), t = vec(taus, rates), d = vec(disp, rates);
+  std::vector<double> c = vec(incr, (size_t)steps * rates * paths);
+  std::vector<double> a = vec(A, (size_t)steps * rates * factors);
+  std::vector<double> ro, lo;
+  LMM_evolver_Euler_gold<double>(i, t, c, a, d, paths, factors, steps, rates, ro, lo);
   std::memcpy(rates_out, ro.data(), ro.size() * sizeof(double));
   std::memcpy(logs_out, lo.data(), lo.size() * sizeof(double));
 }

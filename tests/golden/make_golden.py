@@ -91,6 +91,45 @@ def lmm(R):
                 fidx=fidx, fx=fx.reshape(S, P), swr=swr.reshape(S, P), mu0=mu0)
 
 
+def lmm_euler(R, g):
+    """The gold Euler evolver on the same correlated increments, and on
+    increments scaled by 25 so that the log-rate ceiling (log 10) is hit."""
+    N, F, P = int(g["N"]), int(g["F"]), int(g["P"])
+    S = N
+    out = {}
+    for tag, scale in (("", 1.0), ("_stressed", 25.0)):
+        incr = (scale * g["incr"]).reshape(-1).copy()
+        rates = np.zeros(S * N * P)
+        logs = np.zeros(S * N * P)
+        R.ref_lmm_euler(g["f0"], g["taus"], incr, g["A"].reshape(-1).copy(), g["disp"], P, F, S, N,
+                        rates, logs)
+        out["incr" + tag] = incr.reshape(S, N, P)
+        out["rates" + tag] = rates.reshape(S, N, P)
+        out["logs" + tag] = logs.reshape(S, N, P)
+    return out
+
+
+def bridge_pow2(R):
+    """The gold power-of-two bridge (one factor) and the gold multi-factor
+    bridge (reorder + bridge) on seeded normals."""
+    out = {}
+    for log2 in (0, 1, 3, 6, 8):
+        n = 37 if log2 < 8 else 7
+        z = np.random.RandomState(log2).normal(size=(1 << log2, n))
+        o = np.zeros(z.size)
+        R.ref_bridge_pow2(log2, n, z.reshape(-1).copy(), o)
+        out[f"z_{log2}"], out[f"out_{log2}"] = z, o.reshape(z.shape)
+    for ordering in (0, 1, 2):
+        for log2, F in ((3, 3), (5, 2), (4, 1)):
+            S, P = 1 << log2, 23
+            z = np.random.RandomState(ordering + 10 * log2).normal(size=(S * F, P))
+            o = np.zeros(S * P * F)
+            R.ref_bridge_multi(log2, F, ordering, P, z.reshape(-1).copy(), o)
+            key = f"{ordering}_{log2}_{F}"
+            out["mz_" + key], out["mout_" + key] = z, o.reshape(S, P, F)
+    return out
+
+
 def cashflows(R, g):
     N, P = g["N"], g["P"]
     S = N
@@ -137,6 +176,8 @@ def main():
     np.savez_compressed(os.path.join(HERE, "invnorm_ref.npz"), **invnorm(R))
     g = lmm(R)
     np.savez_compressed(os.path.join(HERE, "lmm_gold_small.npz"), **g)
+    np.savez_compressed(os.path.join(HERE, "lmm_euler_gold_small.npz"), **lmm_euler(R, g))
+    np.savez_compressed(os.path.join(HERE, "bridge_pow2_ref.npz"), **bridge_pow2(R))
     np.savez_compressed(os.path.join(HERE, "cashflow_ref.npz"), **cashflows(R, g))
     np.savez_compressed(os.path.join(HERE, "pseudo_roots_ref.npz"), **pseudo_roots(R))
     print("golden fixtures written to", HERE)

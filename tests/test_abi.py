@@ -48,6 +48,11 @@ def _status(name, *args):
     # Gray-code indices would pass 2^32 - 1 (rejected before any launch)
     ("sobol_bridge_gpu", (100, 0xFFFFFFF0, 4, 1, 8, 8, 0, 8, 8, None)),
     ("sobolGPUInts", (-1, 3, 0, None, None, None)),
+    ("brownianBridgeGPU_constant_memory", (16, 9, None, None, None)),
+    ("brownianBridgeMultiGPUReorder", (None, None, 3, 2, 16, 3, 0, None)),
+    ("LMM_evolver_pc_gpu", (None,) * 8 + (16, 9, 4, 4) + (None,) * 4 + (0, None)),
+    ("LMM_evolver_euler_gpu", (None,) * 7 + (np.zeros(4, np.int32).ctypes.data, 16, 2, 4, 200)
+     + (None,) * 4),
     ("LMM_evolver_pc_all_gpu", (None,) * 9 + (16, 9, 4, 4) + (None,) * 5 + (0, 1, None, 0, 1)),
     ("lmm_price_fwd", (16, 200, 5, 8) + (None,) * 5 + (1,) + (None,) * 5),
     ("lmm_price_fwd", (16, 40, 5, 8) + (None,) * 5 + (7,) + (None,) * 5),
@@ -59,6 +64,9 @@ def _status(name, *args):
     ("cholesky_solve_f64_gpu", (16, None, None, None, None, None)),
     ("reduceOuterProduct_gpu", (0, 6, 6, 1, 256, 256, None, None, None, None, None)),
     ("expandBasisFunctions_quadratic_gpu", (16, None, 0, 2, 5, 1, None, None)),
+    ("PointwiseProduct_gpu", (1, 1, 16, None, None, None, None)),
+    ("PointwiseProductSymmetricUpperTriangular_gpu", (1, 1, 16, -1, None, None, None)),
+    ("PointwiseProductMultipleFirst_gpu", (1, 1, 16, None, None, None, 3, None)),
     ("oneStepUpdateLS_multi_gpu", (16, 0, None, 6, 3, 1) + (None,) * 6 + (0.0, None, None, None)),
 ])
 def test_invalid_arguments_are_rejected(call, args):

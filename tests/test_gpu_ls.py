@@ -13,7 +13,8 @@
   coefficients to 1e-6.
 * The config-3 workload end to end: GPU price within 1e-5 of the oracle that
   generates its own FP64 normals from the same Sobol integers.
-* The reference-named small kernels against their stated formulas."""
+* The reference-named small kernels against their stated formulas; the
+  pointwise products bit-for-bit against FP32 numpy products."""
 import numpy as np
 import pytest
 
@@ -92,6 +93,38 @@ def test_reference_sized_workspace_is_enough(torch_cuda):
     koo.reduceOuterProduct_gpu(P, R, R, True, blocks, blocks, dev(torch, a), None, ans, ws)
     want = a.astype(np.float64) @ a.astype(np.float64).T
     assert np.allclose(ans.cpu().numpy().reshape(R, R), want, rtol=1e-5, atol=1e-5 * P)
+
+
+@pytest.mark.parametrize("P,off", [(4096, 0), (4099, 0), (4096, 1), (1, 0)])
+def test_pointwise_products_are_exact(torch_cuda, P, off):
+    """FP32 products are exact roundings of the FP64 ones: bit-for-bit with
+    numpy's FP32 multiply, on the float4 path (P % 4 == 0, aligned) and the
+    scalar path (ragged P or a row starting one float in)."""
+    torch = torch_cuda
+    R = 5
+    rng = np.random.RandomState(P + off)
+    a = rng.normal(size=(R, P)).astype(np.float32)
+    b = rng.normal(size=P).astype(np.float32)
+    da = torch.zeros(R * P + off, dtype=torch.float32, device="cuda")
+    da[off:] = dev(torch, a.reshape(-1))
+    db = torch.zeros(P + off, dtype=torch.float32, device="cuda")
+    db[off:] = dev(torch, b)
+    out = torch.zeros(P + off, dtype=torch.float32, device="cuda")
+    koo.PointwiseProduct_gpu(64, 256, P, da[off:], db[off:], out[off:])
+    assert np.array_equal(out[off:].cpu().numpy(), a[0] * b)
+    outm = torch.zeros(R * P + off, dtype=torch.float32, device="cuda")
+    koo.PointwiseProductMultipleFirst_gpu(64, 256, P, da[off:], db[off:], outm[off:], R)
+    assert np.array_equal(outm[off:].cpu().numpy().reshape(R, P), a * b[None, :])
+    sentinel = -7.0
+    outs = torch.full((R * R * P + off,), sentinel, dtype=torch.float32, device="cuda")
+    koo.PointwiseProductSymmetricUpperTriangular_gpu(64, 256, P, R, da[off:], outs[off:])
+    got = outs[off:].cpu().numpy().reshape(R, R, P)
+    for r in range(R):
+        for c in range(R):
+            if c >= r:
+                assert np.array_equal(got[r, c], a[r] * a[c])
+            else:
+                assert np.all(got[r, c] == sentinel)
 
 
 def test_cholesky_solve_gpu(torch_cuda):

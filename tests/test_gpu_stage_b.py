@@ -43,6 +43,42 @@ def test_bridge_vs_oracle(torch_cuda, S, F, P):
     assert np.max(np.abs(got - want)) < 2e-5
 
 
+@pytest.mark.parametrize("log2,n", [(0, 100), (3, 5000), (6, 4097), (7, 1000), (8, 300)])
+def test_pow2_bridge_vs_oracle(torch_cuda, log2, n):
+    """The legacy power-of-two bridge (register path up to 64 points, LDS
+    beyond) against the FP64 oracle on the same FP32 normals (the reference
+    test's L1 2e-7 scale: mean relative L1 5e-7, max 2e-5)."""
+    torch = torch_cuda
+    S = 1 << log2
+    z = np.random.RandomState(log2).normal(size=(S, n)).astype(np.float32)
+    out = torch.zeros(S * n, dtype=torch.float32, device="cuda")
+    print(f"pow2 bridge log2={log2} n={n}", flush=True)
+    koo.brownianBridgeGPU_constant_memory(n, log2, dev(torch, z), out)
+    got = out.cpu().numpy().reshape(S, n)
+    want = O.bridge_pow2(log2, z.astype(np.float64))
+    assert mean_rel_l1(got, want) < 5e-7
+    assert np.max(np.abs(got - want)) < 2e-5
+
+
+@pytest.mark.parametrize("ordering", [0, 1, 2])
+def test_multi_factor_pow2_bridge_vs_oracle(torch_cuda, ordering):
+    """Reorder then bridge over points x (paths * factors) vectors, as the
+    reference's MultiDBridge does; the reorder is exact."""
+    torch = torch_cuda
+    log2, F, P = 5, 3, 2000
+    S = 1 << log2
+    z = np.random.RandomState(ordering).normal(size=(S * F, P)).astype(np.float32)
+    re = torch.zeros(S * P * F, dtype=torch.float32, device="cuda")
+    koo.brownianBridgeMultiGPUReorder(dev(torch, z), re, log2, F, P, ordering)
+    want_re = O.bridge_reorder(z.astype(np.float64), log2, F, ordering)
+    assert np.array_equal(re.cpu().numpy().reshape(S, P, F), want_re.astype(np.float32))
+    out = torch.zeros_like(re)
+    koo.brownianBridgeGPU_constant_memory(P * F, log2, re, out)
+    want = O.bridge_pow2(log2, want_re.reshape(S, P * F)).reshape(S, P, F)
+    got = out.cpu().numpy().reshape(S, P, F)
+    assert mean_rel_l1(got, want) < 5e-7
+
+
 @pytest.mark.parametrize("S,F,P,off", [(40, 5, 30000, 11), (64, 2, 5000, 1000), (20, 3, 777, 63),
                                        (48, 1, 4096, 0), (100, 1, 300, 5), (1, 1, 100, 0),
                                        (7, 4, 2049, 64), (40, 5, 20000, (1 << 28) + 12345),
@@ -142,6 +178,48 @@ def test_lmm_evolver_vs_oracle(torch_cuda, which, P, transposed):
                        np.concatenate([l_want[s, s:].ravel() for s in range(S)])) < 2e-6
     assert mean_rel_l1(np.concatenate([d_got[s, s:].ravel() for s in range(S)]),
                        np.concatenate([d_want[s, s:].ravel() for s in range(S)])) < 2e-6
+
+
+@pytest.mark.parametrize("which,P,euler,scale", [("caplets", 4096, False, 1.0),
+                                                 ("swap", 3001, False, 1.0),
+                                                 ("swap", 2048, True, 1.0),
+                                                 ("swap", 1000, True, 25.0)])
+def test_per_step_evolvers_vs_oracle(torch_cuda, which, P, euler, scale):
+    """LMM_evolver_pc_gpu / LMM_evolver_euler_gpu on FP32 correlated
+    increments against the FP64 oracle on the same increments (the reference
+    tests' mean relative L1 2e-6, LMM_evolver_test.cu:45); scale 25 drives
+    Euler log-rates into the log 10 ceiling."""
+    torch = torch_cuda
+    w = configs.caplet_portfolio(paths=P) if which == "caplets" else configs.payer_swap(paths=P)
+    S, N, F, z = _evolver_case(w, P, transposed=False)
+    f0, taus, disp = w.f0, w.taus, w.disp()
+    incr = np.zeros(S * N * P)
+    O.lib().orc_correlate_drift(np.ascontiguousarray(z).reshape(-1), incr,
+                                np.ascontiguousarray(w.A).reshape(-1), np.zeros(S, np.int32),
+                                np.ascontiguousarray(koo.fixed_drifts(w.A)).reshape(-1), N * F, F,
+                                N, P, S)
+    incr32 = (scale * incr.reshape(S, N, P)).astype(np.float32)
+    mu0 = koo.spot_drifts(w.A[0], f0, taus, disp)
+    rates = torch.zeros(S * N * P, dtype=torch.float32, device="cuda")
+    logs = torch.zeros_like(rates)
+    fn = koo.LMM_evolver_euler_gpu if euler else koo.LMM_evolver_pc_gpu
+    print(f"per-step evolver {which} P={P} euler={euler} scale={scale}", flush=True)
+    fn(dev(torch, f0.astype(np.float32)), dev(torch, np.log(f0 + disp).astype(np.float32)),
+       dev(torch, taus.astype(np.float32)), dev(torch, incr32),
+       dev(torch, w.A.astype(np.float32)), dev(torch, mu0.astype(np.float32)),
+       dev(torch, disp.astype(np.float32)), w.alive(), P, F, S, N, rates, logs)
+    r_want, l_want = O.lmm_steps(f0, taus, incr32.astype(np.float64), w.A, disp, w.alive(),
+                                 euler=euler, initial_drifts=mu0)
+    r_got = rates.cpu().numpy().reshape(S, N, P)
+    l_got = logs.cpu().numpy().reshape(S, N, P)
+    alive = w.alive()
+    sel = lambda x: np.concatenate([x[s, (0 if s == 0 else alive[s]):].ravel()  # noqa: E731
+                                    for s in range(S)])
+    assert mean_rel_l1(sel(r_got), sel(r_want)) < 2e-6
+    assert mean_rel_l1(sel(l_got), sel(l_want)) < 2e-6
+    if euler and scale > 1:
+        assert np.sum(sel(l_want) >= np.log(10.0) - 1e-12) > 100
+        assert np.max(sel(l_got)) <= np.float32(np.log(10.0))
 
 
 def test_curve_kernels_vs_oracle(torch_cuda):

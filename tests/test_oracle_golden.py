@@ -312,3 +312,85 @@ def test_caplet_price_near_black76():
     df = np.cumprod(1 / (1 + 0.5 * f0))  # P(T0, T_{i+1}) / P(0, T0)
     black = np.sum(df * 0.5 * (f0 * norm.cdf(d1) - K * norm.cdf(d2)))
     assert abs(price / black - 1) < 5e-3
+
+
+def test_multikernel_pc_evolver_matches_reference_gold():
+    """The per-step PC evolver on correlated increments (A8) against the gold
+    PC evolver's fixture on the same increments (alive rates)."""
+    g = load("lmm_gold_small.npz")
+    N = int(g["N"])
+    rates, logs = O.lmm_steps(g["f0"], g["taus"], g["incr"], g["A"], g["disp"],
+                              np.arange(N, dtype=np.int32))
+    for s in range(N):
+        assert np.allclose(rates[s, s:], g["rates"][s, s:], rtol=1e-12, atol=1e-15)
+        assert np.allclose(logs[s, s:], g["logs"][s, s:], rtol=1e-12, atol=1e-15)
+
+
+def test_multikernel_pc_equals_all_steps_evolver_on_its_increments():
+    """Correlating the variates first (A6) and evolving the increments (A8)
+    is the all-steps evolver (A7) on the variates."""
+    g = load("lmm_gold_small.npz")
+    N = int(g["N"])
+    alive = np.arange(N, dtype=np.int32)
+    r7, l7, _ = O.lmm_pc_evolve(g["f0"], g["taus"], g["z"], g["A"], g["disp"], alive,
+                                transposed=False)
+    r8, l8 = O.lmm_steps(g["f0"], g["taus"], g["incr"], g["A"], g["disp"], alive)
+    for s in range(N):
+        assert np.allclose(r8[s, s:], r7[s, s:], rtol=1e-13, atol=1e-16)
+        assert np.allclose(l8[s, s:], l7[s, s:], rtol=1e-13, atol=1e-16)
+
+
+@pytest.mark.parametrize("tag", ["", "_stressed"])
+def test_euler_evolver_matches_reference_gold(tag):
+    """Euler on the fixture's increments and on increments scaled by 25, where
+    the log-rate ceiling (log 10) binds.  The gold Euler takes the initial
+    log-rates with logf and the rates with expf even in double
+    (gold/LMM_evolver_gold.h:279,300), so it agrees with the FP64 oracle to
+    float precision: log-rates within 5e-7, rates within 5e-6 of f + d (a
+    dropped drift term would move the log-rates by ~1e-3)."""
+    g = load("lmm_gold_small.npz")
+    e = load("lmm_euler_gold_small.npz")
+    N = int(g["N"])
+    rates, logs = O.lmm_steps(g["f0"], g["taus"], e["incr" + tag], g["A"], g["disp"],
+                              np.arange(N, dtype=np.int32), euler=True)
+    for s in range(N):
+        want_r, want_l = e["rates" + tag][s, s:], e["logs" + tag][s, s:]
+        assert np.max(np.abs(logs[s, s:] - want_l)) < 5e-7
+        assert np.max(np.abs(rates[s, s:] - want_r) / (want_r + g["disp"][s:, None])) < 5e-6
+    if tag:
+        assert np.sum(np.isclose(logs, np.log(10.0), rtol=0, atol=1e-15)) > 10
+
+
+@pytest.mark.parametrize("log2", [0, 1, 3, 6, 8])
+def test_pow2_bridge_matches_reference_gold(log2):
+    g = load("bridge_pow2_ref.npz")
+    got = O.bridge_pow2(log2, g[f"z_{log2}"])
+    assert np.allclose(got, g[f"out_{log2}"], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("log2", [1, 4, 6])
+def test_pow2_bridge_is_a_brownian_motion(log2):
+    """The bridge is linear, z -> M z; exact Brownian increments on the unit
+    grid need M M^T = I (each increment unit variance, all independent)."""
+    S = 1 << log2
+    M = O.bridge_pow2(log2, np.eye(S))
+    assert np.allclose(M @ M.T, np.eye(S), atol=1e-12)
+    # the terminal value (sum of the increments) is 2^(n/2) z_0
+    z = np.random.RandomState(1).normal(size=(S, 5))
+    assert np.allclose(O.bridge_pow2(log2, z).sum(axis=0), np.sqrt(S) * z[0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("ordering", [0, 1, 2])
+@pytest.mark.parametrize("log2,F", [(3, 3), (5, 2), (4, 1)])
+def test_multi_factor_pow2_bridge_matches_reference_gold(ordering, log2, F):
+    """Reorder to (S, P, F), then the one-dimensional bridge over S x (P F)
+    vectors (the reference's MultiDBridge) against the gold multi-factor
+    bridge path by path; the reorder is a permutation."""
+    g = load("bridge_pow2_ref.npz")
+    key = f"{ordering}_{log2}_{F}"
+    z = g["mz_" + key]
+    S, P = 1 << log2, z.shape[1]
+    re = O.bridge_reorder(z, log2, F, ordering)
+    assert np.array_equal(np.sort(re.reshape(-1)), np.sort(z.reshape(-1)))
+    got = O.bridge_pow2(log2, re.reshape(S, P * F)).reshape(S, P, F)
+    assert np.allclose(got, g["mout_" + key], rtol=1e-13, atol=1e-14)

@@ -10,6 +10,7 @@
 // weights travel in the kernel arguments (scalar loads), so a call allocates
 // nothing.
 #include <array>
+#include <cmath>
 #include <mutex>
 #include <utility>
 #include <unordered_map>
@@ -310,6 +311,169 @@ constexpr auto reg_gen_table(std::integer_sequence<int, I...>) {
 const auto kRegBridge = reg_bridge_table(std::make_integer_sequence<int, kRegSteps>{});
 const auto kRegGen = reg_gen_table(std::make_integer_sequence<int, kRegSteps>{});
 
+// ---------------------------------------------------------------------------
+// Power-of-two bridge (the reference's legacy BrownianBridge<D>,
+// gold/Bridge_gold.h:168-252, GPU Bridge_gpu.cu:58-195): S = 2^n points;
+// the last point first (scale 2^(n/2)), then the one-sided points
+// 2^(n-i-1) - 1 from their right neighbour, then level by level the
+// midpoints between two built points; point k uses variate var(k), the
+// order in which the construction visits it.  Input and output are
+// [S][vectors] (vector-minor), the output differenced.
+// ---------------------------------------------------------------------------
+constexpr int kPow2MaxLog2 = 8;
+
+struct Pow2Tables {
+  float first;                   // 2^(n/2)
+  float one[kPow2MaxLog2];       // one-sided step i: sqrt(2^(n-i-1) / 2)
+  float mid[kPow2MaxLog2 + 1];   // two-sided level i: sqrt(2^(n-i) / 2)
+};
+
+// Point -> variate of the construction (gold/Bridge_gold.h:237-251).
+template <int LOG2>
+struct Pow2Order {
+  int16_t var[1 << LOG2] = {};
+  constexpr Pow2Order() {
+    int c = 0;
+    for (int i = 0; i <= LOG2; ++i) {
+      const int m = 1 << (LOG2 - i);
+      for (int j = 1; j <= (1 << i); j += 2) var[j * m - 1] = static_cast<int16_t>(c++);
+    }
+  }
+};
+
+__device__ __forceinline__ int pow2_var(int LOG2, int k) {
+  // position of point k in the construction: k + 1 = j 2^(n-i) with j odd
+  const int t = __builtin_ctz(static_cast<unsigned>(k + 1));  // n - i
+  const int i = LOG2 - t;
+  const int j = (k + 1) >> t;
+  return (i == 0) ? 0 : (1 << (i - 1)) + (j >> 1);
+}
+
+template <int LOG2>
+__global__ void __launch_bounds__(kBridgeThreads)
+    k_bridge_pow2_reg(const Pow2Tables T, int vectors, const float* __restrict__ in,
+                      float* __restrict__ out) {
+  constexpr int S = 1 << LOG2;
+  constexpr Pow2Order<LOG2> M{};
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= vectors) return;
+  const size_t n = static_cast<size_t>(vectors);
+  auto z = [&](int k) { return in[static_cast<size_t>(M.var[k]) * n + p]; };
+  float w[S];
+  w[S - 1] = z(S - 1) * T.first;
+#pragma unroll
+  for (int i = 0; i < LOG2; ++i) {
+    const int k = (1 << (LOG2 - i - 1)) - 1, right = (1 << (LOG2 - i)) - 1;
+    w[k] = 0.5f * w[right] + T.one[i] * z(k);
+  }
+#pragma unroll
+  for (int i = 2; i <= LOG2; ++i) {
+    const int m = 1 << (LOG2 - i);
+#pragma unroll
+    for (int j = 3; j < (1 << i); j += 2) {
+      const int k = j * m - 1;
+      w[k] = 0.5f * w[k + m] + 0.5f * w[k - m] + T.mid[i] * z(k);
+    }
+  }
+  out[p] = w[0];
+#pragma unroll
+  for (int k = 1; k < S; ++k) out[static_cast<size_t>(k) * n + p] = w[k] - w[k - 1];
+}
+
+// Same construction with the points in LDS [point][lane] (S = 128, 256).
+__global__ void __launch_bounds__(64)
+    k_bridge_pow2_lds(const Pow2Tables T, int log2, int vectors, const float* __restrict__ in,
+                      float* __restrict__ out) {
+  extern __shared__ float lds[];
+  const int S = 1 << log2;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= vectors) return;
+  const int bd = blockDim.x;
+  float* w = lds + threadIdx.x;  // w[k * bd]
+  const size_t n = static_cast<size_t>(vectors);
+  auto z = [&](int k) { return in[static_cast<size_t>(pow2_var(log2, k)) * n + p]; };
+  w[(S - 1) * bd] = z(S - 1) * T.first;
+  for (int i = 0; i < log2; ++i) {
+    const int k = (1 << (log2 - i - 1)) - 1, right = (1 << (log2 - i)) - 1;
+    w[k * bd] = 0.5f * w[right * bd] + T.one[i] * z(k);
+  }
+  for (int i = 2; i <= log2; ++i) {
+    const int m = 1 << (log2 - i);
+    for (int j = 3; j < (1 << i); j += 2) {
+      const int k = j * m - 1;
+      w[k * bd] = 0.5f * w[(k + m) * bd] + 0.5f * w[(k - m) * bd] + T.mid[i] * z(k);
+    }
+  }
+  float prev = 0.0f;
+  for (int k = 0; k < S; ++k) {
+    const float v = w[k * bd];
+    out[static_cast<size_t>(k) * n + p] = v - prev;
+    prev = v;
+  }
+}
+
+// Multi-factor reordering (Bridge_multi_gpu.cu:67-112 with the orderings of
+// BrownianBridgeMultiDim, gold/Bridge_gold.h:291-331): Sobol dimension i of
+// path p goes to (bridge variate v(i), factor f(i)) of the (S, P, F) array
+// the power-of-two bridge then transforms as S x (P F) vectors.  The
+// ordering is walked in scalar registers (the same for every lane), so no
+// table is needed.  ordering: 0 row, 1 column, 2 triangular.
+__global__ void __launch_bounds__(256)
+    k_bridge_reorder(const float* __restrict__ in, float* __restrict__ out, int points,
+                     int factors, int paths, int ordering) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const size_t P = static_cast<size_t>(paths);
+  const int N = points * factors;
+  int crow = 0, ccol = 0;
+  for (int i = 0; i < N; ++i) {
+    int v, f;
+    if (ordering == 1) {
+      v = i % points;
+      f = i / points;
+    } else if (ordering == 0) {
+      v = i / factors;
+      f = i % factors;
+    } else {
+      v = crow;
+      f = ccol;
+      if (ccol > 0 && crow < points - 1) {
+        --ccol;
+        ++crow;
+      } else {
+        const int rpc = crow + ccol + 1;
+        ccol = rpc < factors - 1 ? rpc : factors - 1;
+        crow = rpc - ccol;
+      }
+    }
+    out[(static_cast<size_t>(v) * P + p) * factors + f] = in[static_cast<size_t>(i) * P + p];
+  }
+}
+
+using Pow2Fn = void (*)(const Pow2Tables, int, const float*, float*);
+template <int... I>
+constexpr auto pow2_table(std::integer_sequence<int, I...>) {
+  return std::array<Pow2Fn, sizeof...(I)>{&k_bridge_pow2_reg<I>...};
+}
+constexpr int kPow2RegMaxLog2 = 6;
+const auto kPow2Reg = pow2_table(std::make_integer_sequence<int, kPow2RegMaxLog2 + 1>{});
+
+// The reference's float tables (gold/Bridge_gold.h:168-200): scalars are
+// formed in double and rounded to float.
+Pow2Tables make_pow2_tables(int log2) {
+  Pow2Tables T{};
+  T.first = static_cast<float>(std::pow(2.0, 0.5 * log2));
+  for (int i = 0; i < log2; ++i) {
+    const float t = static_cast<float>(std::pow(2.0, log2 - i - 1.0));
+    T.one[i] = static_cast<float>(std::sqrt(0.5 * t));
+  }
+  for (int i = 2; i <= log2; ++i) {
+    const int m = 1 << (log2 - i);
+    T.mid[i] = static_cast<float>(std::sqrt(m * 0.5));
+  }
+  return T;
+}
+
 }  // namespace
 }  // namespace koo
 
@@ -420,4 +584,54 @@ extern "C" int sobol_bridge_gpu(int paths, unsigned path_offset, int steps, int 
                      factors, paths, path_offset + 1u, d_directions, scramblers,
                      number_scrambled, allocation_indices, out);
   return check_launch("sobol_bridge_gpu");
+}
+
+// Bridge_gpu.h:11 (brownianBridgeGPU_constant_memory): power-of-two bridge of
+// n_vectors vectors of 2^n_poweroftwo points, [points][vectors] in and out.
+extern "C" int brownianBridgeGPU_constant_memory(int n_vectors, int n_poweroftwo,
+                                                 const float* input, float* output,
+                                                 void* stream) {
+  if (n_vectors < 0 || n_poweroftwo < 0 || n_poweroftwo > kPow2MaxLog2)
+    return set_error(KOO_ERR_INVALID,
+                     "brownianBridgeGPU_constant_memory: n_vectors %d, n_poweroftwo %d outside "
+                     "[0,%d]",
+                     n_vectors, n_poweroftwo, kPow2MaxLog2);
+  if (n_vectors == 0) return KOO_OK;
+  if (!input || !output)
+    return set_error(KOO_ERR_INVALID, "brownianBridgeGPU_constant_memory: null pointer");
+  const Pow2Tables T = make_pow2_tables(n_poweroftwo);
+  hipStream_t st = as_stream(stream);
+  if (n_poweroftwo <= kPow2RegMaxLog2) {
+    hipLaunchKernelGGL(kPow2Reg[n_poweroftwo],
+                       dim3((n_vectors + kBridgeThreads - 1) / kBridgeThreads),
+                       dim3(kBridgeThreads), 0, st, T, n_vectors, input, output);
+  } else {
+    const int threads = 64;
+    const size_t lds = sizeof(float) * (static_cast<size_t>(1) << n_poweroftwo) * threads;
+    hipLaunchKernelGGL(k_bridge_pow2_lds, dim3((n_vectors + threads - 1) / threads),
+                       dim3(threads), lds, st, T, n_poweroftwo, n_vectors, input, output);
+  }
+  return check_launch("brownianBridgeGPU_constant_memory");
+}
+
+// Bridge_multi_gpu.h:12-20 (brownianBridgeMultiGPUReorder): [D][P] Sobol
+// dimensions to the (S, P, F) arrangement of bridge variates and factors,
+// D = 2^n_poweroftwo * factors.  allocator: 0 row, 1 column, 2 triangular
+// (BrownianBridgeMultiDim<float>::ordering).  useTextures is ignored.
+extern "C" int brownianBridgeMultiGPUReorder(const float* input, float* output, int n_poweroftwo,
+                                             int factors, int number_paths, int allocator,
+                                             int /*useTextures*/, void* stream) {
+  if (n_poweroftwo < 0 || n_poweroftwo > kPow2MaxLog2 || factors < 1 || number_paths < 0 ||
+      allocator < 0 || allocator > 2)
+    return set_error(KOO_ERR_INVALID,
+                     "brownianBridgeMultiGPUReorder: bad arguments (n %d, factors %d, paths %d, "
+                     "allocator %d)",
+                     n_poweroftwo, factors, number_paths, allocator);
+  if (number_paths == 0) return KOO_OK;
+  if (!input || !output)
+    return set_error(KOO_ERR_INVALID, "brownianBridgeMultiGPUReorder: null pointer");
+  hipLaunchKernelGGL(k_bridge_reorder, dim3((number_paths + 255) / 256), dim3(256), 0,
+                     as_stream(stream), input, output, 1 << n_poweroftwo, factors, number_paths,
+                     allocator);
+  return check_launch("brownianBridgeMultiGPUReorder");
 }

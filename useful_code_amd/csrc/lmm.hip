@@ -149,6 +149,88 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ----------------------------------------------------------------------
+// LMM_evolver_pc_gpu / LMM_evolver_euler_gpu on correlated increments
+// incr (S,N,P) = A z + fixed drift.  The reference launches one kernel per
+// step and reads the previous step's rate and log-rate cubes back
+// (LMM_evolver_gpu.cu:366-833, 835-1180); here one launch walks every step
+// with the curve in LDS ([rate][lane]) and only writes the cubes.  Step 0
+// takes the host initial drifts for every rate; step s >= 1 rebuilds the
+// drift over r >= alive[s] from the previous step's rates,
+// g = (f tau + d tau) / (1 + f tau).  PC adds the predictor-corrector half
+// step; Euler caps the log of the displaced rate at log 10
+// (LMM_evolver_gpu.cu:20).
+// ----------------------------------------------------------------------
+constexpr float kLogCeiling = 2.302585092994045684f;  // log 10
+
+template <int F, bool PC>
+__global__ void __launch_bounds__(256)
+    k_lmm_steps(const float* __restrict__ f0, const float* __restrict__ logf0,
+                const float* __restrict__ taus, const float* __restrict__ incr,
+                const float* __restrict__ A, const float* __restrict__ mu0,
+                const float* __restrict__ disp, const IndexVec alive, int paths, int steps,
+                int rates, float* __restrict__ rates_out, float* __restrict__ logs_out) {
+  extern __shared__ float lds[];
+  const int bd = blockDim.x;
+  float* L = lds + threadIdx.x;               // L[r * bd]
+  float* R = lds + rates * bd + threadIdx.x;  // R[r * bd]
+  const int p = blockIdx.x * bd + threadIdx.x;
+  if (p >= paths) return;
+  const size_t P = static_cast<size_t>(paths);
+  for (int r = 0; r < rates; ++r) {
+    L[r * bd] = logf0[r];
+    R[r * bd] = f0[r];
+  }
+  for (int s = 0; s < steps; ++s) {
+    const int a = s == 0 ? 0 : alive.v[s];
+    float e[F], ep[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
+    const float* As = A + static_cast<size_t>(s) * rates * F;
+    const float* in = incr + static_cast<size_t>(s) * rates * P + p;
+    for (int r = a; r < rates; ++r) {
+      const float tau = taus[r], d = disp[r];
+      float ar[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) ar[f] = As[r * F + f];
+      float mu;
+      if (s == 0) {
+        mu = mu0[r];
+      } else {
+        const float fr = R[r * bd];
+        const float frt = fr * tau;
+        const float g = (frt + d * tau) / (1.0f + frt);
+        mu = 0.0f;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          e[f] += g * ar[f];
+          mu += e[f] * ar[f];
+        }
+      }
+      float lf = L[r * bd] + (mu + in[static_cast<size_t>(r) * P]);
+      if (PC) {
+        const float xp = expf(lf);
+        const float gp = xp * tau / (1.0f + (xp - d) * tau);
+        float mup = 0.0f;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          ep[f] += gp * ar[f];
+          mup += ar[f] * ep[f];
+        }
+        lf += 0.5f * (mup - mu);
+      } else if (lf > kLogCeiling) {
+        lf = kLogCeiling;
+      }
+      const float rate = expf(lf) - d;
+      L[r * bd] = lf;
+      R[r * bd] = rate;
+      const size_t o = (static_cast<size_t>(s) * rates + r) * P + p;
+      logs_out[o] = lf;
+      rates_out[o] = rate;
+    }
+  }
+}
+
 // P(s,alive) = 1, P(s,r+1) = P(s,r)/(1+tau_r f(s,r))  (LMM_evolver_gpu.cu:1188-1293).
 __global__ void k_discount_ratios(const float* __restrict__ rates, const float* __restrict__ taus,
                                   const IndexVec alive, int paths, int steps, int n_rates,
@@ -288,6 +370,17 @@ extern "C" int correlated_drift_paths_GPU(float* in, float* out, float* A, int* 
   return check_launch("correlated_drift_paths_GPU");
 }
 
+static int check_alive(const char* who, const int* alive_host, int steps, int n_rates) {
+  if (steps < 0 || steps > kMaxSteps)
+    return set_error(KOO_ERR_INVALID, "%s: steps %d outside [0,%d]", who, steps, kMaxSteps);
+  if (!alive_host && steps > 0) return set_error(KOO_ERR_INVALID, "%s: null alive indices", who);
+  for (int s = 0; s < steps; ++s)
+    if (alive_host[s] < 0 || alive_host[s] > n_rates)
+      return set_error(KOO_ERR_INVALID, "%s: alive[%d] = %d outside [0,%d]", who, s, alive_host[s],
+                       n_rates);
+  return KOO_OK;
+}
+
 extern "C" int LMM_evolver_pc_all_gpu(float* initial_rates, float* initial_log_rates,
                                       float* taus, float* variates, float* pseudo_roots,
                                       float* initial_drifts, float* fixed_drifts,
@@ -328,16 +421,71 @@ extern "C" int LMM_evolver_pc_all_gpu(float* initial_rates, float* initial_log_r
   return check_launch("LMM_evolver_pc_all_gpu");
 }
 
-static int check_alive(const char* who, const int* alive_host, int steps, int n_rates) {
-  if (steps < 0 || steps > kMaxSteps)
-    return set_error(KOO_ERR_INVALID, "%s: steps %d outside [0,%d]", who, steps, kMaxSteps);
-  if (!alive_host && steps > 0) return set_error(KOO_ERR_INVALID, "%s: null alive indices", who);
-  for (int s = 0; s < steps; ++s)
-    if (alive_host[s] < 0 || alive_host[s] > n_rates)
-      return set_error(KOO_ERR_INVALID, "%s: alive[%d] = %d outside [0,%d]", who, s, alive_host[s],
-                       n_rates);
-  return KOO_OK;
+static int lmm_steps_launch(const char* who, bool pc, float* initial_rates,
+                            float* initial_log_rates, float* taus, float* increments,
+                            float* pseudo_roots, float* initial_drifts, float* displacements,
+                            const int* alive_host, int paths, int factors, int steps, int rates,
+                            float* evolved_rates, float* evolved_log_rates, void* stream) {
+  if (factors < 1 || factors > kMaxFactors || rates < 1 || rates > kMaxRates || paths < 0)
+    return set_error(KOO_ERR_INVALID, "%s: need 1<=factors<=%d, 1<=rates<=%d (got %d, %d)", who,
+                     kMaxFactors, kMaxRates, factors, rates);
+  if (int e = check_alive(who, alive_host, steps, rates)) return e;
+  if (paths == 0 || steps == 0) return KOO_OK;
+  if (!initial_rates || !initial_log_rates || !taus || !increments || !pseudo_roots ||
+      !initial_drifts || !displacements || !evolved_rates || !evolved_log_rates)
+    return set_error(KOO_ERR_INVALID, "%s: null pointer", who);
+  const int threads = lanes_for_state(2 * rates);
+  const size_t lds = sizeof(float) * 2 * rates * threads;
+  const IndexVec alive = to_index_vec(alive_host, steps);
+  dim3 g = path_grid(paths, threads);
+  hipStream_t st = as_stream(stream);
+  switch (factors) {
+#define KOO_CASE(F)                                                                            \
+  case F:                                                                                      \
+    if (pc)                                                                                    \
+      hipLaunchKernelGGL((k_lmm_steps<F, true>), g, dim3(threads), lds, st, initial_rates,     \
+                         initial_log_rates, taus, increments, pseudo_roots, initial_drifts,    \
+                         displacements, alive, paths, steps, rates, evolved_rates,             \
+                         evolved_log_rates);                                                   \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_lmm_steps<F, false>), g, dim3(threads), lds, st, initial_rates,    \
+                         initial_log_rates, taus, increments, pseudo_roots, initial_drifts,    \
+                         displacements, alive, paths, steps, rates, evolved_rates,             \
+                         evolved_log_rates);                                                   \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)
+    KOO_CASE(8)
+#undef KOO_CASE
+  }
+  return check_launch(who);
 }
+
+extern "C" int LMM_evolver_pc_gpu(float* initial_rates, float* initial_log_rates, float* taus,
+                                  float* correlated_increments, float* pseudo_roots,
+                                  float* initial_drifts, float* displacements,
+                                  const int* alive_indices_host, int paths, int factors, int steps,
+                                  int rates, float* /*e_buffer*/, float* /*e_buffer_pred*/,
+                                  float* evolved_rates, float* evolved_log_rates,
+                                  int /*useShared*/, void* stream) {
+  return lmm_steps_launch("LMM_evolver_pc_gpu", true, initial_rates, initial_log_rates, taus,
+                          correlated_increments, pseudo_roots, initial_drifts, displacements,
+                          alive_indices_host, paths, factors, steps, rates, evolved_rates,
+                          evolved_log_rates, stream);
+}
+
+extern "C" int LMM_evolver_euler_gpu(float* initial_rates, float* initial_log_rates, float* taus,
+                                     float* correlated_increments, float* pseudo_roots,
+                                     float* initial_drifts, float* displacements,
+                                     const int* alive_indices_host, int paths, int factors,
+                                     int steps, int rates, float* /*e_buffer*/,
+                                     float* evolved_rates, float* evolved_log_rates,
+                                     void* stream) {
+  return lmm_steps_launch("LMM_evolver_euler_gpu", false, initial_rates, initial_log_rates, taus,
+                          correlated_increments, pseudo_roots, initial_drifts, displacements,
+                          alive_indices_host, paths, factors, steps, rates, evolved_rates,
+                          evolved_log_rates, stream);
+}
+
 
 extern "C" int discount_ratios_computation_gpu(float* evolved_rates, float* taus,
                                                const int* alive_host, int* /*alive_device*/,

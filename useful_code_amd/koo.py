@@ -49,8 +49,12 @@ _SIGNATURES = {
     "bridge_allocate_diagonal": (_i, [_i, _i, _p]),
     "brownian_bridge_gpu": (_i, [_i, _p, _p, _p, _i, _i, _p]),
     "sobol_bridge_gpu": (_i, [_i, _u, _i, _i, _p, _p, _i, _p, _p, _p]),
+    "brownianBridgeGPU_constant_memory": (_i, [_i, _i, _p, _p, _p]),
+    "brownianBridgeMultiGPUReorder": (_i, [_p, _p, _i, _i, _i, _i, _i, _p]),
     "correlated_drift_paths_GPU": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _p]),
     "LMM_evolver_pc_all_gpu": (_i, [_p] * 9 + [_i] * 4 + [_p] * 5 + [_i, _i, _p, _i, _i]),
+    "LMM_evolver_pc_gpu": (_i, [_p] * 8 + [_i] * 4 + [_p] * 4 + [_i, _p]),
+    "LMM_evolver_euler_gpu": (_i, [_p] * 8 + [_i] * 4 + [_p] * 4),
     "discount_ratios_computation_gpu": (_i, [_p, _p, _p, _p, _i, _i, _i, _p, _i, _p]),
     "coterminal_annuity_ratios_computation_gpu": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
     "coterminal_swap_rate_computation_gpu": (_i, [_p, _p, _p, _i, _i, _i, _p, _p]),
@@ -90,6 +94,9 @@ _SIGNATURES = {
     "outer_product_workspace_floats": (_sz, [_i]),
     "reduceOuterProductSymmetric_cublas": (_i, [_i, _i, _f, _f] + [_p] * 4),
     "reduceOuterProduct_cublas": (_i, [_i, _i, _i, _f, _f] + [_p] * 5),
+    "PointwiseProduct_gpu": (_i, [_i, _i, _i, _p, _p, _p, _p]),
+    "PointwiseProductSymmetricUpperTriangular_gpu": (_i, [_i, _i, _i, _i, _p, _p, _p]),
+    "PointwiseProductMultipleFirst_gpu": (_i, [_i, _i, _i, _p, _p, _p, _i, _p]),
     "bermudan_paths_gpu": (_i, [_i, _i, _i] + [_p] * 5 + [_i, _p, _f] + [_p] * 4),
     "ls_workspace": (_i, [_i, _p]),
     "ls_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
@@ -256,6 +263,17 @@ def sobol_bridge_gpu(paths, path_offset, steps, factors, d_directions, scrambler
          stream_handle(stream))
 
 
+def brownianBridgeGPU_constant_memory(n_vectors, n_poweroftwo, inp, out, stream=None):
+    call("brownianBridgeGPU_constant_memory", n_vectors, n_poweroftwo, ptr(inp), ptr(out),
+         stream_handle(stream))
+
+
+def brownianBridgeMultiGPUReorder(inp, out, n_poweroftwo, factors, number_paths, allocator,
+                                  stream=None):
+    call("brownianBridgeMultiGPUReorder", ptr(inp), ptr(out), n_poweroftwo, factors,
+         number_paths, int(allocator), 0, stream_handle(stream))
+
+
 def correlated_drift_paths_GPU(d_in, d_out, A, alive, drift_fixed, A_offset_per_step, factors,
                                out_dimensions, number_paths, steps, stream=None):
     call("correlated_drift_paths_GPU", ptr(d_in), ptr(d_out), ptr(A), ptr(alive),
@@ -276,6 +294,26 @@ def LMM_evolver_pc_all_gpu(initial_rates, initial_log_rates, taus, variates, pse
 
 def _host_ints(v):
     return np.ascontiguousarray(v, dtype=np.int32)
+
+
+def LMM_evolver_pc_gpu(initial_rates, initial_log_rates, taus, increments, pseudo_roots,
+                       initial_drifts, displacements, alive_host, paths, factors, steps, rates,
+                       evolved_rates, evolved_log_rates, stream=None):
+    a = _host_ints(alive_host)
+    call("LMM_evolver_pc_gpu", ptr(initial_rates), ptr(initial_log_rates), ptr(taus),
+         ptr(increments), ptr(pseudo_roots), ptr(initial_drifts), ptr(displacements),
+         a.ctypes.data, paths, factors, steps, rates, None, None, ptr(evolved_rates),
+         ptr(evolved_log_rates), 0, stream_handle(stream))
+
+
+def LMM_evolver_euler_gpu(initial_rates, initial_log_rates, taus, increments, pseudo_roots,
+                          initial_drifts, displacements, alive_host, paths, factors, steps, rates,
+                          evolved_rates, evolved_log_rates, stream=None):
+    a = _host_ints(alive_host)
+    call("LMM_evolver_euler_gpu", ptr(initial_rates), ptr(initial_log_rates), ptr(taus),
+         ptr(increments), ptr(pseudo_roots), ptr(initial_drifts), ptr(displacements),
+         a.ctypes.data, paths, factors, steps, rates, None, ptr(evolved_rates),
+         ptr(evolved_log_rates), stream_handle(stream))
 
 
 def discount_ratios_computation_gpu(rates_dev, taus, alive_host, paths, steps, n_rates, disc,
@@ -521,6 +559,22 @@ def reduceOuterProductSymmetric_cublas(paths, row_size, alpha, beta, inp, answer
 
 def outer_product_workspace_floats(paths):
     return lib().outer_product_workspace_floats(paths)
+
+
+def PointwiseProduct_gpu(blocks, threads, paths, in1, in2, out, stream=None):
+    call("PointwiseProduct_gpu", blocks, threads, paths, ptr(in1), ptr(in2), ptr(out),
+         stream_handle(stream))
+
+
+def PointwiseProductSymmetricUpperTriangular_gpu(blocks, threads, paths, rows, inp, out,
+                                                 stream=None):
+    call("PointwiseProductSymmetricUpperTriangular_gpu", blocks, threads, paths, rows, ptr(inp),
+         ptr(out), stream_handle(stream))
+
+
+def PointwiseProductMultipleFirst_gpu(blocks, threads, paths, in1, in2, out, rows, stream=None):
+    call("PointwiseProductMultipleFirst_gpu", blocks, threads, paths, ptr(in1), ptr(in2), ptr(out),
+         rows, stream_handle(stream))
 
 
 # ---------------------------------------------------------------- multi-asset Black-Scholes
