@@ -622,6 +622,56 @@ int bermudan_frozen_bwd(int paths, int rates, int factors, int checkpoint_every,
                         float* segment_scratch, double* partials, double* greeks_out,
                         void* stream);
 
+/* =====================================================================
+ * Portfolios of Bermudan swaptions (new): the frozen forward and the
+ * adjoint backward of every product in one launch each.  A product is
+ * described as for bermudan_frozen_fwd/_bwd (its own buffers from
+ * bermudan_adjoint_workspace; partials shared by its forward and backward);
+ * the batch takes factors <= 5 and rates <= 45.  prepare() writes the
+ * descriptor table (bermudan_portfolio_descriptor_bytes(n) bytes of device
+ * memory) with a synchronous copy, ordering the products longest first,
+ * and fills the plan; fwd()/bwd() then launch without synchronising or
+ * allocating, so the whole portfolio can be captured in one graph.  Each
+ * product's price_out (1 double) and greeks_out (2 x rates doubles) are as
+ * in the single-product calls.  Replaces the reference's per-product loop
+ * over cashFlowGeneratorEE_kernel second passes
+ * (cashFlowGen_earlyExProduct_gpu.h:28-172), which has no adjoint.
+ * =================================================================== */
+typedef struct koo_berm_product {
+  int paths, rates, factors, checkpoint_every;
+  const float* variates;      /* (S,F,P) */
+  const float* initial_rates; /* N */
+  const float* taus;          /* N */
+  const float* displacements; /* N */
+  const float* pseudo_roots;  /* (S,N,F) */
+  const float* vols;          /* N */
+  float strike;
+  int n_ex;
+  const int* exercise_steps;  /* host, increasing */
+  const double* coefficients; /* device, E x 6 */
+  const double* var_shift;    /* host, E x 2, or NULL */
+  const double* var_scale;    /* host, E x 2, or NULL */
+  int itm_only;
+  float* checkpoints;
+  float* path_buffers;
+  float* segment_scratch;
+  double* partials;
+  double* price_out;  /* device */
+  double* greeks_out; /* device */
+} koo_berm_product;
+
+typedef struct koo_portfolio_plan {
+  int products, blocks, max_rates;
+} koo_portfolio_plan;
+
+size_t bermudan_portfolio_descriptor_bytes(int n_products);
+int bermudan_portfolio_prepare(int n_products, const koo_berm_product* products,
+                               void* descriptors_device, koo_portfolio_plan* plan);
+int bermudan_portfolio_fwd(const koo_portfolio_plan* plan, const void* descriptors_device,
+                           void* stream);
+int bermudan_portfolio_bwd(const koo_portfolio_plan* plan, const void* descriptors_device,
+                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,9 @@
 // step (lmm_step.h), run per wavefront only up to its last exercise step; at a
 // path's exercise step the recorded payoff adjoint seeds lambda, and each
 // earlier fixing f_s(T_s) inside the numeraire adds -V g_s e^{l_s}.
+#include <algorithm>
+#include <vector>
+
 #include "lmm_step.h"
 
 namespace koo {
@@ -38,17 +41,18 @@ __device__ __forceinline__ int wave_max_int(int v) {
   return v;
 }
 
+// One block of the forward: paths [block * blockDim, ...) of product m.
 template <int F>
-__global__ void __launch_bounds__(256)
-    k_berm_fwd(KOO_MODEL_PARAMS, const BermArgs b, const double* __restrict__ coeffs,
-               float* __restrict__ ckpt,
-               float* __restrict__ lamt, int* __restrict__ tau_out, float* __restrict__ vout,
-               double* __restrict__ part) {
-  KOO_MODEL_LOCAL;
+__device__ __forceinline__ void berm_fwd_block(const Model& m, const BermArgs& b,
+                                               const double* __restrict__ coeffs,
+                                               float* __restrict__ ckpt, float* __restrict__ lamt,
+                                               int* __restrict__ tau_out,
+                                               float* __restrict__ vout,
+                                               double* __restrict__ part, int block) {
   extern __shared__ float lds[];
   __shared__ double red[4];
   const int bd = blockDim.x;
-  const int p = blockIdx.x * bd + threadIdx.x;
+  const int p = block * bd + threadIdx.x;
   const bool valid = p < m.P;
   const int pc = valid ? p : m.P - 1;
   float* L = lds + threadIdx.x;
@@ -124,22 +128,33 @@ __global__ void __launch_bounds__(256)
     tau_out[p] = tau;
     vout[p] = V;
   }
-  block_sum_to(valid ? static_cast<double>(V) : 0.0, red, part + blockIdx.x);
+  block_sum_to(valid ? static_cast<double>(V) : 0.0, red, part + block);
 }
 
 template <int F>
 __global__ void __launch_bounds__(256)
-    k_berm_bwd(KOO_MODEL_PARAMS, const float* __restrict__ vols, const float* __restrict__ ckpt,
-               const float* __restrict__ lamt, const int* __restrict__ tau_in,
-               const float* __restrict__ vin, float* __restrict__ scratch,
-               double* __restrict__ part) {
+    k_berm_fwd(KOO_MODEL_PARAMS, const BermArgs b, const double* __restrict__ coeffs,
+               float* __restrict__ ckpt, float* __restrict__ lamt, int* __restrict__ tau_out,
+               float* __restrict__ vout, double* __restrict__ part) {
   KOO_MODEL_LOCAL;
+  berm_fwd_block<F>(m, b, coeffs, ckpt, lamt, tau_out, vout, part, blockIdx.x);
+}
+
+// One block of the backward (paths [block * blockDim, ...)).
+template <int F>
+__device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __restrict__ vols,
+                                               const float* __restrict__ ckpt,
+                                               const float* __restrict__ lamt,
+                                               const int* __restrict__ tau_in,
+                                               const float* __restrict__ vin,
+                                               float* __restrict__ scratch,
+                                               double* __restrict__ part, int block) {
   extern __shared__ float lds[];
   __shared__ double red[4];
   const int bd = blockDim.x;
   const int nw = bd >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int p = blockIdx.x * bd + threadIdx.x;
+  const int p = block * bd + threadIdx.x;
   const bool valid = p < m.P;
   const int pc = valid ? p : m.P - 1;
   const int N = m.N;
@@ -223,14 +238,126 @@ __global__ void __launch_bounds__(256)
   for (int r = 0; r < N; ++r) {
     const float dl = lam[r * bd] / (m.f0[r] + m.disp[r]);
     block_sum_to(valid ? static_cast<double>(dl) : 0.0, red,
-                 part + static_cast<size_t>(blockIdx.x) * 2 * N + r);
+                 part + static_cast<size_t>(block) * 2 * N + r);
   }
   for (int r = threadIdx.x; r < N; r += bd) {
     double t = 0.0;
     for (int w = 0; w < nw; ++w)
       for (int q = 0; q < kVegaSlots; ++q)
         t += vslots[(static_cast<size_t>(w) * N + r) * kVegaSlots + q];
-    part[static_cast<size_t>(blockIdx.x) * 2 * N + N + r] = t / vols[r];
+    part[static_cast<size_t>(block) * 2 * N + N + r] = t / vols[r];
+  }
+}
+
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_berm_bwd(KOO_MODEL_PARAMS, const float* __restrict__ vols, const float* __restrict__ ckpt,
+               const float* __restrict__ lamt, const int* __restrict__ tau_in,
+               const float* __restrict__ vin, float* __restrict__ scratch,
+               double* __restrict__ part) {
+  KOO_MODEL_LOCAL;
+  berm_bwd_block<F>(m, vols, ckpt, lamt, tau_in, vin, scratch, part, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched portfolio: one launch runs one stage (forward or backward) for
+// every product of a portfolio.  The products' descriptors sit in device
+// memory; blocks are numbered product after product (the host orders the
+// products by cost, longest first, so short products fill the tail), and
+// each block finds its product by a binary search over the first-block
+// offsets (uniform per block: scalar loads) and dispatches on its factor
+// count.  Every product runs 256-lane blocks; LDS is sized for the largest.
+// ---------------------------------------------------------------------------
+// Device code reads the descriptor's pointers as global-memory pointers
+// (address space 1), so accesses through them are global, not flat,
+// instructions (flat ones also count against the LDS wait counter, which
+// would serialise the state prefetch against the LDS traffic).  The host
+// fills the same layout with plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KOO_GLOBAL __attribute__((address_space(1)))
+#else
+#define KOO_GLOBAL
+#endif
+typedef const float KOO_GLOBAL g_cfloat;
+typedef const double KOO_GLOBAL g_cdouble;
+typedef float KOO_GLOBAL g_float;
+typedef double KOO_GLOBAL g_double;
+typedef int KOO_GLOBAL g_int;
+struct BermItem {
+  g_cfloat *z, *f0, *taus, *disp, *A, *vols;
+  g_cdouble* coeffs;
+  g_float *ckpt, *lamt, *vout, *scratch;
+  g_int* tau;
+  g_double *part_fwd, *part_bwd, *price_out, *greeks_out;
+  float strike;
+  int P, N, k, F, block0, blocks;
+  BermArgs b;
+};
+
+__device__ __forceinline__ const BermItem& find_item(const BermItem* items, int n, int block) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= block) lo = mid;
+    else hi = mid - 1;
+  }
+  return items[lo];
+}
+
+// Address space 1 -> generic (the compiler keeps track that it is global).
+template <class T>
+__device__ __forceinline__ T* gmem(T KOO_GLOBAL* p) {
+  return (T*)p;
+}
+
+__device__ __forceinline__ Model item_model(const BermItem& it) {
+  return Model{gmem(it.z), gmem(it.f0), gmem(it.taus), gmem(it.disp), gmem(it.A), it.strike,
+               it.P, it.N, it.k};
+}
+
+__global__ void __launch_bounds__(256)
+    k_berm_fwd_batch(const BermItem* __restrict__ items, int n) {
+  const BermItem& it = find_item(items, n, blockIdx.x);
+  const Model m = item_model(it);
+  const int blk = blockIdx.x - it.block0;
+  switch (it.F) {
+#define KOO_CASE(F)                                                                            \
+  case F:                                                                                      \
+    berm_fwd_block<F>(m, it.b, gmem(it.coeffs), gmem(it.ckpt), gmem(it.lamt), gmem(it.tau),     \
+                      gmem(it.vout), gmem(it.part_fwd), blk);                                  \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5)
+#undef KOO_CASE
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_berm_bwd_batch(const BermItem* __restrict__ items, int n) {
+  const BermItem& it = find_item(items, n, blockIdx.x);
+  const Model m = item_model(it);
+  const int blk = blockIdx.x - it.block0;
+  switch (it.F) {
+#define KOO_CASE(F)                                                                            \
+  case F:                                                                                      \
+    berm_bwd_block<F>(m, gmem(it.vols), gmem(it.ckpt), gmem(it.lamt), gmem(it.tau),            \
+                      gmem(it.vout), gmem(it.scratch), gmem(it.part_bwd), blk);               \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5)
+#undef KOO_CASE
+  }
+}
+
+// Per product: price (column 0 of the forward partials) and the 2N Greeks.
+// grid (2 * max N + 1, products); block x = 0 is the price.
+__global__ void __launch_bounds__(256)
+    k_berm_finalize_batch(const BermItem* __restrict__ items, int fwd) {
+  const BermItem& it = items[blockIdx.y];
+  const double scale = 1.0 / static_cast<double>(it.P);
+  if (fwd) {
+    if (blockIdx.x == 0)
+      finalize_col(gmem(it.part_fwd), it.blocks, 1, 0, scale, gmem(it.price_out));
+  } else if (static_cast<int>(blockIdx.x) < 2 * it.N) {
+    finalize_col(gmem(it.part_bwd), it.blocks, 2 * it.N, blockIdx.x, scale, gmem(it.greeks_out));
   }
 }
 
@@ -377,4 +504,112 @@ extern "C" int bermudan_frozen_bwd(int paths, int rates, int factors, int checkp
   hipLaunchKernelGGL(k_finalize_cols, dim3(2 * rates), dim3(256), 0, st, partials, blocks,
                      2 * rates, 1.0 / static_cast<double>(paths), greeks_out);
   return check_launch("bermudan_frozen_bwd");
+}
+
+// ---------------------------------------------------------------------------
+// Batched portfolio entry points.
+// ---------------------------------------------------------------------------
+extern "C" size_t bermudan_portfolio_descriptor_bytes(int n_products) {
+  return n_products > 0 ? sizeof(BermItem) * static_cast<size_t>(n_products) : 0;
+}
+
+extern "C" int bermudan_portfolio_prepare(int n_products, const koo_berm_product* products,
+                                          void* descriptors_device, koo_portfolio_plan* plan) {
+  const char* who = "bermudan_portfolio_prepare";
+  if (n_products < 1 || !products || !descriptors_device || !plan)
+    return set_error(KOO_ERR_INVALID, "%s: bad arguments", who);
+  std::vector<BermItem> items(n_products);
+  std::vector<double> cost(n_products);
+  int max_n = 0;
+  for (int i = 0; i < n_products; ++i) {
+    const koo_berm_product& q = products[i];
+    if (int e = check_berm(who, q.paths, q.rates, q.factors, q.checkpoint_every)) return e;
+    if (q.factors > 5 || q.rates > 45)
+      return set_error(KOO_ERR_INVALID,
+                       "%s: product %d: batched kernels take factors <= 5 and rates <= 45", who,
+                       i);
+    if (!q.variates || !q.initial_rates || !q.taus || !q.displacements || !q.pseudo_roots ||
+        !q.vols || !q.coefficients || !q.checkpoints || !q.path_buffers ||
+        !q.segment_scratch || !q.partials || !q.price_out || !q.greeks_out)
+      return set_error(KOO_ERR_INVALID, "%s: product %d: null pointer", who, i);
+    BermItem& it = items[i];
+    it = BermItem{};
+    if (int e = make_berm_args(who, q.rates, q.n_ex, q.exercise_steps, q.var_shift, q.var_scale,
+                               q.itm_only, &it.b))
+      return e;
+    const size_t P = static_cast<size_t>(q.paths);
+    it.z = (g_cfloat*)q.variates;
+    it.f0 = (g_cfloat*)q.initial_rates;
+    it.taus = (g_cfloat*)q.taus;
+    it.disp = (g_cfloat*)q.displacements;
+    it.A = (g_cfloat*)q.pseudo_roots;
+    it.vols = (g_cfloat*)q.vols;
+    it.coeffs = (g_cdouble*)q.coefficients;
+    it.ckpt = (g_float*)q.checkpoints;
+    it.lamt = (g_float*)q.path_buffers;
+    it.tau = (g_int*)(q.path_buffers + static_cast<size_t>(q.rates) * P);
+    it.vout = (g_float*)(q.path_buffers + static_cast<size_t>(q.rates + 1) * P);
+    it.scratch = (g_float*)q.segment_scratch;
+    it.part_fwd = it.part_bwd = (g_double*)q.partials;
+    it.price_out = (g_double*)q.price_out;
+    it.greeks_out = (g_double*)q.greeks_out;
+    it.strike = q.strike;
+    it.P = q.paths;
+    it.N = q.rates;
+    it.k = q.checkpoint_every;
+    it.F = q.factors;
+    it.blocks = (q.paths + 255) / 256;
+    // work ~ paths x rate updates up to the last exercise step
+    double u = 0.0;
+    for (int s = 0; s <= it.b.last; ++s) u += q.rates - s;
+    cost[i] = u * q.paths * (q.factors + 4);
+    if (q.rates > max_n) max_n = q.rates;
+  }
+  std::vector<int> order(n_products);
+  for (int i = 0; i < n_products; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<BermItem> sorted(n_products);
+  int64_t block0 = 0;
+  for (int i = 0; i < n_products; ++i) {
+    sorted[i] = items[order[i]];
+    sorted[i].block0 = static_cast<int>(block0);
+    block0 += sorted[i].blocks;
+  }
+  if (block0 > 0x7fffffff) return set_error(KOO_ERR_INVALID, "%s: too many blocks", who);
+  if (hipMemcpy(descriptors_device, sorted.data(), sizeof(BermItem) * n_products,
+                hipMemcpyHostToDevice) != hipSuccess)
+    return set_error(KOO_ERR_HIP, "%s: descriptor copy failed", who);
+  plan->products = n_products;
+  plan->blocks = static_cast<int>(block0);
+  plan->max_rates = max_n;
+  return KOO_OK;
+}
+
+static int portfolio_launch(const char* who, bool fwd, const koo_portfolio_plan* plan,
+                            const void* descriptors_device, void* stream) {
+  if (!plan || !descriptors_device || plan->products < 1 || plan->blocks < 1 ||
+      plan->max_rates < 1 || plan->max_rates > 45)
+    return set_error(KOO_ERR_INVALID, "%s: bad plan", who);
+  const BermItem* items = static_cast<const BermItem*>(descriptors_device);
+  const int N = plan->max_rates;
+  hipStream_t st = as_stream(stream);
+  if (fwd)
+    hipLaunchKernelGGL(k_berm_fwd_batch, dim3(plan->blocks), dim3(256), fwd_lds(N, 256), st,
+                       items, plan->products);
+  else
+    hipLaunchKernelGGL(k_berm_bwd_batch, dim3(plan->blocks), dim3(256), bwd_lds(N, 256), st,
+                       items, plan->products);
+  hipLaunchKernelGGL(k_berm_finalize_batch, dim3(fwd ? 1 : 2 * N, plan->products), dim3(256), 0,
+                     st, items, fwd ? 1 : 0);
+  return check_launch(who);
+}
+
+extern "C" int bermudan_portfolio_fwd(const koo_portfolio_plan* plan,
+                                      const void* descriptors_device, void* stream) {
+  return portfolio_launch("bermudan_portfolio_fwd", true, plan, descriptors_device, stream);
+}
+
+extern "C" int bermudan_portfolio_bwd(const koo_portfolio_plan* plan,
+                                      const void* descriptors_device, void* stream) {
+  return portfolio_launch("bermudan_portfolio_bwd", false, plan, descriptors_device, stream);
 }

@@ -254,12 +254,11 @@ __device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[
       const float* __restrict__ A_, float strike_, int P_, int N_, int k_
 #define KOO_MODEL_LOCAL const Model m{z_, f0_, taus_, disp_, A_, strike_, P_, N_, k_}
 
-// out[c] = (sum_b part[b*ncols + c]) * scale, compensated; one block per column.
-__global__ void __launch_bounds__(256)
-    k_finalize_cols(const double* __restrict__ part, int nb, int ncols, double scale,
-                    double* __restrict__ out) {
+// out[col] = (sum_b part[b*ncols + col]) * scale, compensated; called by a
+// whole block (k_finalize_cols runs one block per column).
+__device__ __forceinline__ void finalize_col(const double* __restrict__ part, int nb, int ncols,
+                                             int col, double scale, double* __restrict__ out) {
   __shared__ double sv[16], sc[16];
-  const int col = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   // per lane: Neumaier over a strided slice of the blocks
   double s = 0.0, c = 0.0;
@@ -293,6 +292,12 @@ __global__ void __launch_bounds__(256)
     }
     out[col] = (S + C) * scale;
   }
+}
+
+__global__ void __launch_bounds__(256)
+    k_finalize_cols(const double* __restrict__ part, int nb, int ncols, double scale,
+                    double* __restrict__ out) {
+  finalize_col(part, nb, ncols, blockIdx.x, scale, out);
 }
 
 }  // namespace

@@ -28,6 +28,21 @@ class KooError(RuntimeError):
     pass
 
 
+class BermProduct(ctypes.Structure):
+    """koo_berm_product (include/koo_c_abi.h)."""
+    _fields_ = [("paths", _i), ("rates", _i), ("factors", _i), ("checkpoint_every", _i),
+                ("variates", _p), ("initial_rates", _p), ("taus", _p), ("displacements", _p),
+                ("pseudo_roots", _p), ("vols", _p), ("strike", _f), ("n_ex", _i),
+                ("exercise_steps", _p), ("coefficients", _p), ("var_shift", _p),
+                ("var_scale", _p), ("itm_only", _i), ("checkpoints", _p), ("path_buffers", _p),
+                ("segment_scratch", _p), ("partials", _p), ("price_out", _p), ("greeks_out", _p)]
+
+
+class PortfolioPlan(ctypes.Structure):
+    """koo_portfolio_plan (include/koo_c_abi.h)."""
+    _fields_ = [("products", _i), ("blocks", _i), ("max_rates", _i)]
+
+
 # name -> (restype, argtypes); pointers are passed as integers (c_void_p).
 _SIGNATURES = {
     "koo_last_error": (ctypes.c_char_p, []),
@@ -98,6 +113,10 @@ _SIGNATURES = {
     "PointwiseProductSymmetricUpperTriangular_gpu": (_i, [_i, _i, _i, _i, _p, _p, _p]),
     "PointwiseProductMultipleFirst_gpu": (_i, [_i, _i, _i, _p, _p, _p, _i, _p]),
     "bermudan_paths_gpu": (_i, [_i, _i, _i] + [_p] * 5 + [_i, _p, _f] + [_p] * 4),
+    "bermudan_portfolio_descriptor_bytes": (_sz, [_i]),
+    "bermudan_portfolio_prepare": (_i, [_i, _p, _p, _p]),
+    "bermudan_portfolio_fwd": (_i, [_p, _p, _p]),
+    "bermudan_portfolio_bwd": (_i, [_p, _p, _p]),
     "ls_workspace": (_i, [_i, _p]),
     "ls_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
     "ls_second_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
@@ -634,6 +653,29 @@ def bermudan_frozen_fwd(paths, rates, factors, k, variates, f0, taus, disp, A, s
          ptr(disp), ptr(A), float(strike), len(ex), ex.ctypes.data, ptr(coeffs_dev), ptr(sh),
          ptr(sc), int(itm_only), ptr(checkpoints), ptr(path_buffers), ptr(partials),
          ptr(price_out), stream_handle(stream))
+
+
+def bermudan_portfolio_prepare(products, descriptors):
+    """products: list of BermProduct; descriptors: a device buffer of
+    bermudan_portfolio_descriptor_bytes(len(products)) bytes.  Returns the
+    plan (host)."""
+    arr = (BermProduct * len(products))(*products)
+    plan = PortfolioPlan()
+    call("bermudan_portfolio_prepare", len(products), ctypes.addressof(arr), ptr(descriptors),
+         ctypes.addressof(plan))
+    return plan
+
+
+def bermudan_portfolio_descriptor_bytes(n):
+    return lib().bermudan_portfolio_descriptor_bytes(n)
+
+
+def bermudan_portfolio_fwd(plan, descriptors, stream=None):
+    call("bermudan_portfolio_fwd", ctypes.addressof(plan), ptr(descriptors), stream_handle(stream))
+
+
+def bermudan_portfolio_bwd(plan, descriptors, stream=None):
+    call("bermudan_portfolio_bwd", ctypes.addressof(plan), ptr(descriptors), stream_handle(stream))
 
 
 def bermudan_frozen_bwd(paths, rates, factors, k, variates, f0, taus, disp, A, vols, strike,

@@ -125,9 +125,9 @@ class BermudanPricer:
         koo.ls_second_pass_gpu(self.E, self.bv, self.ves, 2, None, self.ev, self.num, w.paths, True,
                                True, self.shift, self.scale, self.coeffs, self.ws, self.price2)
 
-    def greeks(self, checkpoint_every=4):
-        """Frozen-strategy price and pathwise adjoint Greeks (N deltas, N vegas)
-        on the current variates, with the coefficients of the last regress()."""
+    def adjoint_buffers(self, checkpoint_every):
+        """(k, checkpoints, segment scratch, path buffers, partials) of the
+        frozen adjoint at checkpoint interval k, allocated on first use."""
         w = self.w
         P, N, F = w.paths, w.n_rates, w.factors
         if not hasattr(self, "adj") or self.adj[0] != checkpoint_every:
@@ -141,7 +141,30 @@ class BermudanPricer:
             self.vols = _dev(w.vols, torch.float32)
             self.frozen_price = torch.zeros(1, dtype=torch.float64, device=dev)
             self.frozen_greeks = torch.zeros(2 * N, dtype=torch.float64, device=dev)
-        k, ck, sc, pb, pa = self.adj
+        return self.adj
+
+    def batch_product(self, checkpoint_every):
+        """This product's koo.BermProduct for the batched portfolio kernels
+        (buffers as in greeks(); host arrays kept alive on the pricer)."""
+        w = self.w
+        k, ck, sc, pb, pa = self.adjoint_buffers(checkpoint_every)
+        self._ex = np.ascontiguousarray(w.exercise_steps, np.int32)
+        self._sh = np.ascontiguousarray(self.shift, np.float64)
+        self._sc = np.ascontiguousarray(self.scale, np.float64)
+        return koo.BermProduct(
+            w.paths, w.n_rates, w.factors, k, self.z.data_ptr(), self.f0.data_ptr(),
+            self.taus.data_ptr(), self.disp.data_ptr(), self.A.data_ptr(), self.vols.data_ptr(),
+            float(w.strike), len(self._ex), self._ex.ctypes.data, self.coeffs.data_ptr(),
+            self._sh.ctypes.data, self._sc.ctypes.data, 1, ck.data_ptr(), pb.data_ptr(),
+            sc.data_ptr(), pa.data_ptr(), self.frozen_price.data_ptr(),
+            self.frozen_greeks.data_ptr())
+
+    def greeks(self, checkpoint_every=4):
+        """Frozen-strategy price and pathwise adjoint Greeks (N deltas, N vegas)
+        on the current variates, with the coefficients of the last regress()."""
+        w = self.w
+        P, N, F = w.paths, w.n_rates, w.factors
+        k, ck, sc, pb, pa = self.adjoint_buffers(checkpoint_every)
         koo.bermudan_frozen_fwd(P, N, F, k, self.z, self.f0, self.taus, self.disp, self.A,
                                 w.strike, w.exercise_steps, self.coeffs, self.shift, self.scale, True,
                                 ck, pb, pa, self.frozen_price)
@@ -207,7 +230,7 @@ class PortfolioPricer:
     product index to its checkpoint interval (the autotuner's choice) or is
     one interval for all."""
 
-    def __init__(self, products, checkpoint_every=4, n_streams=4, device="cuda"):
+    def __init__(self, products, checkpoint_every=4, n_streams=4, device="cuda", batched=True):
         self.products = products
         self.pricers = [BermudanPricer(w, device=device) for w in products]
         self.k = [checkpoint_every(j) if callable(checkpoint_every) else checkpoint_every
@@ -215,14 +238,29 @@ class PortfolioPricer:
         self.streams = [torch.cuda.Stream(device=device) for _ in range(n_streams)]
         self.stream_of = [j % n_streams for j in range(len(products))]
         self.graph = None
+        self.device = device
+        # batched: every product's frozen forward in one launch, then every
+        # backward in one launch (bermudan_portfolio_fwd/_bwd); otherwise the
+        # per-product calls on the streams
+        self.batched = batched and all(w.factors <= 5 and w.n_rates <= 45 for w in products)
+        self.plan = None
 
     def apply_tuning(self, tuning):
         """Checkpoint interval and stream per product from autotune.tune_portfolio."""
         self.k = list(tuning["k"])
         self.stream_of = list(tuning["stream"])
         self.graph = None
+        self.plan = None
+
+    def _prepare_batch(self):
+        items = [pr.batch_product(self.k[j]) for j, pr in enumerate(self.pricers)]
+        self.descriptors = torch.empty(koo.bermudan_portfolio_descriptor_bytes(len(items)),
+                                       dtype=torch.uint8, device=self.device)
+        self.plan = koo.bermudan_portfolio_prepare(items, self.descriptors)
 
     def _issue(self):
+        if self.batched and self.plan is None:
+            self._prepare_batch()
         main = torch.cuda.current_stream()
         for s in self.streams:
             s.wait_stream(main)
@@ -233,9 +271,13 @@ class PortfolioPricer:
                 pr.generate()
                 pr.paths_kernel()
                 pr.regress()
-                pr.greeks(checkpoint_every=self.k[j])
+                if not self.batched:
+                    pr.greeks(checkpoint_every=self.k[j])
         for s in self.streams:
             main.wait_stream(s)
+        if self.batched:
+            koo.bermudan_portfolio_fwd(self.plan, self.descriptors)
+            koo.bermudan_portfolio_bwd(self.plan, self.descriptors)
 
     def run(self, capture=True):
         """Prices the portfolio; with ``capture`` the first call records the
