@@ -660,8 +660,14 @@ typedef struct koo_berm_product {
   double* greeks_out; /* device */
 } koo_berm_product;
 
+/* Products are grouped by rate count into up to 4 classes (<= 16, 24, 32
+ * and more rates), one launch each, so a block reserves LDS for its class's
+ * largest curve rather than the portfolio's. */
 typedef struct koo_portfolio_plan {
-  int products, blocks, max_rates;
+  int products, blocks, max_rates; /* the 256-lane kernels */
+  int ls_blocks, max_ex;           /* the 1024-lane LS kernels (LS batch only) */
+  int classes;
+  int class_first_block[4], class_blocks[4], class_max_rates[4];
 } koo_portfolio_plan;
 
 size_t bermudan_portfolio_descriptor_bytes(int n_products);
@@ -671,6 +677,43 @@ int bermudan_portfolio_fwd(const koo_portfolio_plan* plan, const void* descripto
                            void* stream);
 int bermudan_portfolio_bwd(const koo_portfolio_plan* plan, const void* descriptors_device,
                            void* stream);
+
+/* Portfolio path quantities and LS first passes (new): bermudan_paths_gpu
+ * for every product in one launch, then the backward induction of every
+ * product date by date, one launch per date rank (round r handles date
+ * E_q - 1 - r of product q; a product finishes with its date-0 decision in
+ * round E_q), then the first-pass prices.  Per product as in
+ * bermudan_paths_gpu + ls_first_pass_gpu with two basis variables per date
+ * (co-terminal swap rate, first forward), the quadratic basis with cross
+ * term, in-the-money regression when itm_only; paths x rates within the
+ * batch limits (factors <= 5, rates <= 50, dates <= 40).  Coefficients and
+ * prices are bit-identical to the single-product calls. */
+typedef struct koo_ls_product {
+  int paths, rates, factors;
+  const float* variates;      /* (S,F,P) */
+  const float* initial_rates; /* N */
+  const float* taus;          /* N */
+  const float* displacements; /* N */
+  const float* pseudo_roots;  /* (S,N,F) */
+  float strike;
+  int n_ex;
+  const int* exercise_steps; /* host, increasing */
+  const float* var_shift;    /* host, E x 2, or NULL */
+  const float* var_scale;    /* host, E x 2, or NULL */
+  int itm_only;
+  float* basis_variables; /* (E,2,P) out */
+  float* exercise_values; /* (E,P) out */
+  float* numeraires;      /* (E,P) out */
+  void* workspace;        /* ls_workspace(paths) bytes */
+  double* coefficients;   /* device, E x 6 out */
+  double* price_out;      /* device, first-pass price */
+} koo_ls_product;
+
+size_t bermudan_ls_portfolio_descriptor_bytes(int n_products);
+int bermudan_ls_portfolio_prepare(int n_products, const koo_ls_product* products,
+                                  void* descriptors_device, koo_portfolio_plan* plan);
+int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan, const void* descriptors_device,
+                              void* stream);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,8 @@ def _status(name, *args):
     ("PointwiseProductSymmetricUpperTriangular_gpu", (1, 1, 16, -1, None, None, None)),
     ("PointwiseProductMultipleFirst_gpu", (1, 1, 16, None, None, None, 3, None)),
     ("bermudan_portfolio_prepare", (0, None, None, None)),
+    ("bermudan_ls_portfolio_prepare", (0, None, None, None)),
+    ("bermudan_ls_portfolio_run", (None, None, None)),
     ("bermudan_portfolio_fwd", (None, None, None)),
     ("bermudan_portfolio_bwd", (None, None, None)),
     ("oneStepUpdateLS_multi_gpu", (16, 0, None, 6, 3, 1) + (None,) * 6 + (0.0, None, None, None)),

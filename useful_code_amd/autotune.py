@@ -1,13 +1,20 @@
-"""Autotuner for the batched Bermudan portfolio (measured on the device).
+"""Autotuner for the Bermudan portfolio (measured on the device).
 
-For every product it times, with HIP events, the frozen-strategy forward and
-reverse sweep at each candidate checkpoint interval k (1, 2, 4, 8, 16, bounded
-by a checkpoint-memory budget) and keeps the fastest; then it times the
-product's whole sequence at that k and schedules the products over the
-streams by longest-processing-time-first (each product goes to the stream
-with the least accumulated time).  The result -- k and stream per product and
-the measured times -- is a plain dict that PortfolioPricer takes and that can
-be stored as JSON next to the results it produced.
+Per-product mode: for every product it times, with HIP events, the
+frozen-strategy forward and reverse sweep at each candidate checkpoint
+interval k (1, 2, 4, 8, 16, bounded by a checkpoint-memory budget) and keeps
+the fastest; then it times the product's whole sequence at that k and
+schedules the products over the streams by longest-processing-time-first
+(each product goes to the stream with the least accumulated time).
+
+Batched mode: the candidate intervals are timed on the batched forward and
+backward launches of the whole portfolio (the products share the device, so
+the sum is what counts), and the generators are scheduled over the streams
+by LPT.
+
+The result -- k and stream per product and the measured times -- is a plain
+dict that PortfolioPricer takes and that can be stored as JSON next to the
+results it produced.
 """
 import json
 
@@ -37,10 +44,55 @@ def checkpoint_bytes(w, k):
     return ck + sc + pb
 
 
+def _lpt(ms, n_streams):
+    load = [0.0] * n_streams
+    stream = [0] * len(ms)
+    for j in sorted(range(len(ms)), key=lambda j: -ms[j]):
+        s = int(np.argmin(load))
+        stream[j] = s
+        load[s] += ms[j]
+    return stream, load
+
+
+def tune_batched(pricer, memory_budget_bytes=64 << 30, n_streams=None):
+    """Batched portfolio: one checkpoint interval for every product, the one
+    whose batched forward + backward launches are fastest (the products share
+    the device, so their sum is what counts); generators scheduled over the
+    streams by LPT on their measured times."""
+    n_streams = n_streams or len(pricer.streams)
+    for pr in pricer.pricers:
+        pr.generate()
+    pricer.k = [1] * len(pricer.pricers)
+    pricer.plan = None
+    pricer._issue()  # path quantities and regressions, so the strategy is realistic
+    torch.cuda.synchronize()
+    k_ms = {}
+    for k in CANDIDATE_K:
+        if any(k > pr.w.n_rates for pr in pricer.pricers):
+            continue
+        if sum(checkpoint_bytes(pr.w, k) for pr in pricer.pricers) > memory_budget_bytes:
+            continue
+        pricer.k = [k] * len(pricer.pricers)
+        pricer._prepare_batch()
+        k_ms[k] = _time(lambda: (koo.bermudan_portfolio_fwd(pricer.plan, pricer.descriptors),
+                                 koo.bermudan_portfolio_bwd(pricer.plan, pricer.descriptors)))
+        for pr in pricer.pricers:  # release this interval's buffers
+            del pr.adj
+        torch.cuda.empty_cache()
+    best = min(k_ms, key=k_ms.get)
+    ms = [_time(pr.generate, reps=2) for pr in pricer.pricers]
+    stream, load = _lpt(ms, n_streams)
+    return {"k": [best] * len(pricer.pricers), "stream": stream, "ms": ms,
+            "k_ms": [{str(a): b for a, b in k_ms.items()}] * len(pricer.pricers),
+            "stream_load_ms": load, "batched": True}
+
+
 def tune_portfolio(pricer, memory_budget_bytes=64 << 30, n_streams=None):
     """pricer: a PortfolioPricer (its buffers are reused).  Returns
-    {"k": [...], "stream": [...], "ms": [...], "k_ms": [{k: ms}, ...]}."""
-    from .pipeline import BermudanPricer  # noqa: F401  (type of pricer.pricers)
+    {"k": [...], "stream": [...], "ms": [...], "k_ms": [{k: ms}, ...]}.
+    A batched pricer is tuned by tune_batched."""
+    if getattr(pricer, "batched", False):
+        return tune_batched(pricer, memory_budget_bytes, n_streams)
     n_streams = n_streams or len(pricer.streams)
     ks, k_ms = [], []
     budget_each = memory_budget_bytes / max(1, len(pricer.pricers))
@@ -71,12 +123,7 @@ def tune_portfolio(pricer, memory_budget_bytes=64 << 30, n_streams=None):
             pr.regress()
             pr.greeks(checkpoint_every=k)
         ms.append(_time(seq, reps=2))
-    load = [0.0] * n_streams
-    stream = [0] * len(ms)
-    for j in sorted(range(len(ms)), key=lambda j: -ms[j]):
-        s = int(np.argmin(load))
-        stream[j] = s
-        load[s] += ms[j]
+    stream, load = _lpt(ms, n_streams)
     return {"k": ks, "stream": stream, "ms": ms, "k_ms": [{str(a): b for a, b in t.items()}
                                                           for t in k_ms],
             "stream_load_ms": load}

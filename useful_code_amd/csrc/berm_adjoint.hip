@@ -262,27 +262,14 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Batched portfolio: one launch runs one stage (forward or backward) for
 // every product of a portfolio.  The products' descriptors sit in device
-// memory; blocks are numbered product after product (the host orders the
-// products by cost, longest first, so short products fill the tail), and
-// each block finds its product by a binary search over the first-block
-// offsets (uniform per block: scalar loads) and dispatches on its factor
-// count.  Every product runs 256-lane blocks; LDS is sized for the largest.
+// memory; blocks are numbered product after product (the host groups the
+// products by rate-count class, one launch per class with LDS sized for the
+// class's largest curve, and orders each class by cost, longest first, so
+// short products fill the tail), and each block finds its product by a
+// binary search over the first-block offsets (uniform per block: scalar
+// loads) and dispatches on its factor count.  Every product runs 256-lane
+// blocks.
 // ---------------------------------------------------------------------------
-// Device code reads the descriptor's pointers as global-memory pointers
-// (address space 1), so accesses through them are global, not flat,
-// instructions (flat ones also count against the LDS wait counter, which
-// would serialise the state prefetch against the LDS traffic).  The host
-// fills the same layout with plain pointers.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define KOO_GLOBAL __attribute__((address_space(1)))
-#else
-#define KOO_GLOBAL
-#endif
-typedef const float KOO_GLOBAL g_cfloat;
-typedef const double KOO_GLOBAL g_cdouble;
-typedef float KOO_GLOBAL g_float;
-typedef double KOO_GLOBAL g_double;
-typedef int KOO_GLOBAL g_int;
 struct BermItem {
   g_cfloat *z, *f0, *taus, *disp, *A, *vols;
   g_cdouble* coeffs;
@@ -295,19 +282,7 @@ struct BermItem {
 };
 
 __device__ __forceinline__ const BermItem& find_item(const BermItem* items, int n, int block) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].block0 <= block) lo = mid;
-    else hi = mid - 1;
-  }
-  return items[lo];
-}
-
-// Address space 1 -> generic (the compiler keeps track that it is global).
-template <class T>
-__device__ __forceinline__ T* gmem(T KOO_GLOBAL* p) {
-  return (T*)p;
+  return items[find_block_owner(items, n, block, &BermItem::block0)];
 }
 
 __device__ __forceinline__ Model item_model(const BermItem& it) {
@@ -316,10 +291,11 @@ __device__ __forceinline__ Model item_model(const BermItem& it) {
 }
 
 __global__ void __launch_bounds__(256)
-    k_berm_fwd_batch(const BermItem* __restrict__ items, int n) {
-  const BermItem& it = find_item(items, n, blockIdx.x);
+    k_berm_fwd_batch(const BermItem* __restrict__ items, int n, int block_offset) {
+  const int block = blockIdx.x + block_offset;
+  const BermItem& it = find_item(items, n, block);
   const Model m = item_model(it);
-  const int blk = blockIdx.x - it.block0;
+  const int blk = block - it.block0;
   switch (it.F) {
 #define KOO_CASE(F)                                                                            \
   case F:                                                                                      \
@@ -332,10 +308,11 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void __launch_bounds__(256)
-    k_berm_bwd_batch(const BermItem* __restrict__ items, int n) {
-  const BermItem& it = find_item(items, n, blockIdx.x);
+    k_berm_bwd_batch(const BermItem* __restrict__ items, int n, int block_offset) {
+  const int block = blockIdx.x + block_offset;
+  const BermItem& it = find_item(items, n, block);
   const Model m = item_model(it);
-  const int blk = blockIdx.x - it.block0;
+  const int blk = block - it.block0;
   switch (it.F) {
 #define KOO_CASE(F)                                                                            \
   case F:                                                                                      \
@@ -567,12 +544,25 @@ extern "C" int bermudan_portfolio_prepare(int n_products, const koo_berm_product
   }
   std::vector<int> order(n_products);
   for (int i = 0; i < n_products; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  // rate class first (one launch per class), then most expensive first
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int ca = rate_class(items[a].N), cb = rate_class(items[b].N);
+    return ca != cb ? ca < cb : cost[a] > cost[b];
+  });
   std::vector<BermItem> sorted(n_products);
   int64_t block0 = 0;
+  *plan = koo_portfolio_plan{};
   for (int i = 0; i < n_products; ++i) {
     sorted[i] = items[order[i]];
     sorted[i].block0 = static_cast<int>(block0);
+    const int c = rate_class(sorted[i].N);
+    if (plan->classes == 0 || rate_class(sorted[i - 1].N) != c) {
+      plan->class_first_block[plan->classes] = static_cast<int>(block0);
+      ++plan->classes;
+    }
+    plan->class_blocks[plan->classes - 1] += sorted[i].blocks;
+    if (sorted[i].N > plan->class_max_rates[plan->classes - 1])
+      plan->class_max_rates[plan->classes - 1] = sorted[i].N;
     block0 += sorted[i].blocks;
   }
   if (block0 > 0x7fffffff) return set_error(KOO_ERR_INVALID, "%s: too many blocks", who);
@@ -588,17 +578,23 @@ extern "C" int bermudan_portfolio_prepare(int n_products, const koo_berm_product
 static int portfolio_launch(const char* who, bool fwd, const koo_portfolio_plan* plan,
                             const void* descriptors_device, void* stream) {
   if (!plan || !descriptors_device || plan->products < 1 || plan->blocks < 1 ||
-      plan->max_rates < 1 || plan->max_rates > 45)
+      plan->max_rates < 1 || plan->max_rates > 45 || plan->classes < 1 || plan->classes > 4)
     return set_error(KOO_ERR_INVALID, "%s: bad plan", who);
   const BermItem* items = static_cast<const BermItem*>(descriptors_device);
   const int N = plan->max_rates;
   hipStream_t st = as_stream(stream);
-  if (fwd)
-    hipLaunchKernelGGL(k_berm_fwd_batch, dim3(plan->blocks), dim3(256), fwd_lds(N, 256), st,
-                       items, plan->products);
-  else
-    hipLaunchKernelGGL(k_berm_bwd_batch, dim3(plan->blocks), dim3(256), bwd_lds(N, 256), st,
-                       items, plan->products);
+  for (int c = 0; c < plan->classes; ++c) {
+    const int nc = plan->class_max_rates[c], b0 = plan->class_first_block[c];
+    const int nb = plan->class_blocks[c];
+    if (nc < 1 || nc > N || nb < 1 || b0 < 0 || b0 + nb > plan->blocks)
+      return set_error(KOO_ERR_INVALID, "%s: bad plan class %d", who, c);
+    if (fwd)
+      hipLaunchKernelGGL(k_berm_fwd_batch, dim3(nb), dim3(256), fwd_lds(nc, 256), st, items,
+                         plan->products, b0);
+    else
+      hipLaunchKernelGGL(k_berm_bwd_batch, dim3(nb), dim3(256), bwd_lds(nc, 256), st, items,
+                         plan->products, b0);
+  }
   hipLaunchKernelGGL(k_berm_finalize_batch, dim3(fwd ? 1 : 2 * N, plan->products), dim3(256), 0,
                      st, items, fwd ? 1 : 0);
   return check_launch(who);

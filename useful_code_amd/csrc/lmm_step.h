@@ -254,6 +254,46 @@ __device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[
       const float* __restrict__ A_, float strike_, int P_, int N_, int k_
 #define KOO_MODEL_LOCAL const Model m{z_, f0_, taus_, disp_, A_, strike_, P_, N_, k_}
 
+// Batched kernels read the pointers of their device descriptor tables as
+// global-memory pointers (address space 1), so accesses through them are
+// global, not flat, instructions (flat ones also count against the LDS wait
+// counter, which would serialise the state prefetch against the LDS
+// traffic).  The host fills the same layout with plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KOO_GLOBAL __attribute__((address_space(1)))
+#else
+#define KOO_GLOBAL
+#endif
+typedef const float KOO_GLOBAL g_cfloat;
+typedef const double KOO_GLOBAL g_cdouble;
+typedef float KOO_GLOBAL g_float;
+typedef double KOO_GLOBAL g_double;
+typedef int KOO_GLOBAL g_int;
+// Address space 1 -> generic (the compiler keeps track that it is global).
+template <class T>
+__device__ __forceinline__ T* gmem(T KOO_GLOBAL* p) {
+  return (T*)p;
+}
+
+// Batched launches number their blocks item after item; the item owning
+// `block` is the last one whose first block (the member `first`) is <= block
+// (binary search, uniform per block).
+template <class Item>
+__device__ __forceinline__ int find_block_owner(const Item* items, int n, int block,
+                                                int Item::*first) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].*first <= block) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Rate-count class of a product in a batched portfolio (launches are per
+// class so LDS is reserved for the class's largest curve).
+inline int rate_class(int rates) { return rates <= 16 ? 0 : (rates <= 24 ? 1 : (rates <= 32 ? 2 : 3)); }
+
 // out[col] = (sum_b part[b*ncols + col]) * scale, compensated; called by a
 // whole block (k_finalize_cols runs one block per column).
 __device__ __forceinline__ void finalize_col(const double* __restrict__ part, int nb, int ncols,

@@ -26,6 +26,9 @@
 // function m of the four paths k = 0..3.  With at most 8 rows, the 16 rows
 // hold two groups of 4 paths (rows 0-7 and 8-15), so each instruction takes
 // 8 paths and the two diagonal 8x8 blocks of D are added at the end.
+#include <algorithm>
+#include <vector>
+
 #include "lmm_step.h"
 
 namespace koo {
@@ -80,11 +83,11 @@ __host__ __device__ inline int basis_count(int v, bool cross) {
 // Adds the two diagonal 8x8 blocks (groups == 2) or takes the 16x16 result
 // (groups == 1) of each wave's accumulator and sums the waves of the block in
 // FP64 into the block's column of the entry-major partials:
-// part[(i*n + j) * gridDim.x + blockIdx.x] for i, j < n.  Every lane of the
-// block calls it.
+// part[(i*n + j) * nblocks + block] for i, j < n.  Every lane of the block
+// calls it.
 template <int GROUPS>
 __device__ __forceinline__ void block_gram(const floatx4& acc, int n, float* lds_d /* [nw][256] */,
-                                           double* part) {
+                                           double* part, int block, int nblocks) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   float* d = lds_d + wave * 256;
 #pragma unroll
@@ -105,7 +108,7 @@ __device__ __forceinline__ void block_gram(const floatx4& acc, int n, float* lds
     }
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (q == 0 && t < n * n) part[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = s;
+    if (q == 0 && t < n * n) part[static_cast<size_t>(t) * nblocks + block] = s;
   }
 }
 
@@ -250,11 +253,11 @@ __device__ void solve_normal_equations(const double* G, int k, double* beta, dou
 // barriers.
 __device__ __forceinline__ void solve_date(const double* __restrict__ part, int nb, int k,
                                            double* gram, double* chol, double* beta,
-                                           double* __restrict__ coeffs_row, int maxd) {
+                                           double* __restrict__ coeffs_row, int maxd, int block) {
   reduce_partials(part, nb, (k + 1) * (k + 1), gram);
   if (threadIdx.x == 0) {
     solve_normal_equations(gram, k, beta, chol, chol + kMaxGram * kMaxGram);
-    if (blockIdx.x == 0)
+    if (block == 0)
       for (int i = 0; i < maxd; ++i) coeffs_row[i] = i < k ? beta[i] : 0.0;
   }
   __syncthreads();
@@ -269,13 +272,12 @@ struct ExMap {
 };
 
 template <int F>
-__global__ void __launch_bounds__(256)
-    k_berm_paths(KOO_MODEL_PARAMS, const ExMap ex, float* __restrict__ bv, float* __restrict__ evo,
-                 float* __restrict__ numo) {
-  KOO_MODEL_LOCAL;
+__device__ __forceinline__ void berm_paths_block(const Model& m, const ExMap& ex,
+                                                 float* __restrict__ bv, float* __restrict__ evo,
+                                                 float* __restrict__ numo, int block) {
   extern __shared__ float lds[];
   const int bd = blockDim.x;
-  const int p = blockIdx.x * bd + threadIdx.x;
+  const int p = block * bd + threadIdx.x;
   const bool valid = p < m.P;
   const int pc = valid ? p : m.P - 1;
   float* L = lds + threadIdx.x;
@@ -319,6 +321,14 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_berm_paths(KOO_MODEL_PARAMS, const ExMap ex, float* __restrict__ bv, float* __restrict__ evo,
+                 float* __restrict__ numo) {
+  KOO_MODEL_LOCAL;
+  berm_paths_block<F>(m, ex, bv, evo, numo, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Gram matrix of up to 16 path-minor rows (optionally masked), FP64 partials
 // per block into part[block][n*n].
@@ -351,7 +361,7 @@ __global__ void __launch_bounds__(kGramThreads)
       acc = mfma16x16x4(a, a, acc);
     }
   }
-  block_gram<GROUPS>(acc, n, lds_d, part);
+  block_gram<GROUPS>(acc, n, lds_d, part, blockIdx.x, gridDim.x);
 }
 
 // Sums the partials of k_gram and writes the n x n result (FP64 and, when
@@ -399,12 +409,15 @@ __device__ __forceinline__ float continuation(const float* __restrict__ bv, size
   return c;
 }
 
+// One block (of nblocks) of one exercise date.
 template <int GROUPS>
-__global__ void __launch_bounds__(kGramThreads)
-    k_ls_date(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
-              const float* __restrict__ ev, const float* __restrict__ num,
-              double* __restrict__ coeffs, float* __restrict__ Y,
-              const double* __restrict__ part_later, double* __restrict__ part) {
+__device__ __forceinline__ void ls_date_block(const DateArgs& a, const float* __restrict__ bv,
+                                              const float* __restrict__ cash,
+                                              const float* __restrict__ ev,
+                                              const float* __restrict__ num,
+                                              double* __restrict__ coeffs, float* __restrict__ Y,
+                                              const double* __restrict__ part_later,
+                                              double* __restrict__ part, int block, int nblocks) {
   __shared__ float lds_d[(kGramThreads / 64) * 256];
   __shared__ double gram[kMaxGram * kMaxGram];
   __shared__ double chol[kMaxGram * kMaxGram + kMaxGram];
@@ -418,11 +431,11 @@ __global__ void __launch_bounds__(kGramThreads)
   const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
   const int sub = GROUPS == 2 ? ((lane >> 3) & 1) * 4 + (lane >> 4) : (lane >> 4);
   if (later)  // coefficients of date e+1 from its partials
-    solve_date(part_later, gridDim.x, a.kn, gram, chol, beta_n,
-               coeffs + static_cast<size_t>(a.e + 1) * a.maxd, a.maxd);
+    solve_date(part_later, nblocks, a.kn, gram, chol, beta_n,
+               coeffs + static_cast<size_t>(a.e + 1) * a.maxd, a.maxd, block);
   floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  const int stride = gridDim.x * nw * 64;
-  for (int base = (blockIdx.x * nw + wave) * 64; base < a.P; base += stride) {
+  const int stride = nblocks * nw * 64;
+  for (int base = (block * nw + wave) * 64; base < a.P; base += stride) {
     const int p = base + lane;
     float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f, x3 = 0.0f, t = 0.0f, w = 0.0f;
     if (p < a.P) {
@@ -469,24 +482,38 @@ __global__ void __launch_bounds__(kGramThreads)
       acc = mfma16x16x4(v, v, acc);
     }
   }
-  block_gram<GROUPS>(acc, n, lds_d, part);
+  block_gram<GROUPS>(acc, n, lds_d, part, block, nblocks);
+}
+
+template <int GROUPS>
+__global__ void __launch_bounds__(kGramThreads)
+    k_ls_date(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
+              const float* __restrict__ ev, const float* __restrict__ num,
+              double* __restrict__ coeffs, float* __restrict__ Y,
+              const double* __restrict__ part_later, double* __restrict__ part) {
+  ls_date_block<GROUPS>(a, bv, cash, ev, num, coeffs, Y, part_later, part, blockIdx.x, gridDim.x);
 }
 
 // Date 0's coefficients and exercise decision, then sum_p Y_p / B_0 into
-// block partials.
-__global__ void __launch_bounds__(kGramThreads)
-    k_ls_final(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
-               const float* __restrict__ ev, const float* __restrict__ num,
-               double* __restrict__ coeffs, const float* __restrict__ Y,
-               const double* __restrict__ part_date0, int nb_date0, double* __restrict__ part) {
+// block partials (one block of nblocks; date 0's partials come from
+// nb_date0 blocks).
+__device__ __forceinline__ void ls_final_block(const DateArgs& a, const float* __restrict__ bv,
+                                               const float* __restrict__ cash,
+                                               const float* __restrict__ ev,
+                                               const float* __restrict__ num,
+                                               double* __restrict__ coeffs,
+                                               const float* __restrict__ Y,
+                                               const double* __restrict__ part_date0,
+                                               int nb_date0, double* __restrict__ part, int block,
+                                               int nblocks) {
   __shared__ double red[kGramThreads / 64];
   __shared__ double gram[kMaxGram * kMaxGram];
   __shared__ double chol[kMaxGram * kMaxGram + kMaxGram];
   __shared__ double beta0[kMaxGram];
-  solve_date(part_date0, nb_date0, a.k, gram, chol, beta0, coeffs, a.maxd);
+  solve_date(part_date0, nb_date0, a.k, gram, chol, beta0, coeffs, a.maxd, block);
   const size_t P = static_cast<size_t>(a.P);
   double s = 0.0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
+  for (int p = block * blockDim.x + threadIdx.x; p < a.P; p += nblocks * blockDim.x) {
     float y = Y[p];
     const float cont = continuation(bv, 0, P, p, a.v, a.k, a.cross != 0, a.sh, a.sc, beta0);
     const float exv = ev[p];
@@ -497,7 +524,16 @@ __global__ void __launch_bounds__(kGramThreads)
       y += cash[p];
     s += static_cast<double>(y) / static_cast<double>(num[p]);
   }
-  block_sum_to(s, red, part + blockIdx.x);
+  block_sum_to(s, red, part + block);
+}
+
+__global__ void __launch_bounds__(kGramThreads)
+    k_ls_final(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
+               const float* __restrict__ ev, const float* __restrict__ num,
+               double* __restrict__ coeffs, const float* __restrict__ Y,
+               const double* __restrict__ part_date0, int nb_date0, double* __restrict__ part) {
+  ls_final_block(a, bv, cash, ev, num, coeffs, Y, part_date0, nb_date0, part, blockIdx.x,
+                 gridDim.x);
 }
 
 // Frozen-strategy pass (SecondPassPriceLSAMultiUsingAggregatedFlows_gold,
@@ -728,6 +764,99 @@ int gram_launch(const char* who, const float* const* rows, int n, const float* m
   hipLaunchKernelGGL(k_gram_finish, dim3(1), dim3(kGramThreads), 0, st, part, nb, n, r0, c0, nr, nc,
                      ld, out64, out32, alpha, beta, transpose);
   return check_launch(who);
+}
+
+// ---------------------------------------------------------------------------
+// Batched portfolio: Bermudan path quantities of every product in one
+// launch, then one launch per date rank for the LS first passes (round r
+// runs date E - 1 - r of each product, or its date-0 decision when r == E).
+// Blocks are numbered product after product, in both kernels; each block
+// finds its product by binary search over the first-block offsets.
+// ---------------------------------------------------------------------------
+constexpr int kLsBatchMaxDates = 40;
+
+struct LsItem {
+  g_cfloat *z, *f0, *taus, *disp, *A;
+  g_float *bv, *ev, *num, *Y;
+  g_double *part0, *part1, *coeffs, *price_out;
+  float strike;
+  int P, N, F, E, itm_only;
+  int paths_block0, paths_blocks, ls_block0, ls_blocks;
+  ExMap ex;
+  float sh[kLsBatchMaxDates][2], sc[kLsBatchMaxDates][2];
+};
+
+__global__ void __launch_bounds__(256)
+    k_berm_paths_batch(const LsItem* __restrict__ items, int n, int block_offset) {
+  const int block = blockIdx.x + block_offset;
+  const LsItem& it = items[find_block_owner(items, n, block, &LsItem::paths_block0)];
+  const Model m{gmem(it.z), gmem(it.f0), gmem(it.taus), gmem(it.disp), gmem(it.A), it.strike,
+                it.P, it.N, 1};
+  const int blk = block - it.paths_block0;
+  switch (it.F) {
+#define KOO_CASE(F)                                                                            \
+  case F:                                                                                      \
+    berm_paths_block<F>(m, it.ex, gmem(it.bv), gmem(it.ev), gmem(it.num), blk);                \
+    break;
+    KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5)
+#undef KOO_CASE
+  }
+}
+
+// The date arguments of date e of a product (two variables per date, the
+// cross basis: 6 functions).
+__device__ __forceinline__ DateArgs item_date(const LsItem& it, int e) {
+  DateArgs a{};
+  a.e = e;
+  a.E = it.E;
+  a.P = it.P;
+  a.max_vars = 2;
+  a.maxd = 6;
+  a.v = 2;
+  a.k = 6;
+  a.cross = 1;
+  a.itm_only = it.itm_only;
+  for (int i = 0; i < kMaxBasisVars; ++i) {
+    a.sh[i] = i < 2 ? it.sh[e][i] : 0.0f;
+    a.sc[i] = i < 2 ? it.sc[e][i] : 1.0f;
+  }
+  if (e + 1 < it.E) {
+    a.vn = 2;
+    a.kn = 6;
+    for (int i = 0; i < kMaxBasisVars; ++i) {
+      a.shn[i] = i < 2 ? it.sh[e + 1][i] : 0.0f;
+      a.scn[i] = i < 2 ? it.sc[e + 1][i] : 1.0f;
+    }
+  }
+  return a;
+}
+
+__global__ void __launch_bounds__(kGramThreads)
+    k_ls_round(const LsItem* __restrict__ items, int n, int round) {
+  const LsItem& it = items[find_block_owner(items, n, blockIdx.x, &LsItem::ls_block0)];
+  const int e = it.E - 1 - round;
+  if (e < -1) return;  // this product finished in an earlier round
+  const int blk = blockIdx.x - it.ls_block0;
+  // the two partial buffers alternate between consecutive dates
+  g_double* even = it.part0;
+  g_double* odd = it.part1;
+  if (e >= 0) {
+    const DateArgs a = item_date(it, e);
+    ls_date_block<2>(a, gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), gmem(it.coeffs),
+                     gmem(it.Y), gmem((e & 1) ? even : odd), gmem((e & 1) ? odd : even), blk,
+                     it.ls_blocks);
+  } else {
+    const DateArgs a = item_date(it, 0);
+    ls_final_block(a, gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), gmem(it.coeffs),
+                   gmem(it.Y), gmem(even), it.ls_blocks, gmem(odd), blk, it.ls_blocks);
+  }
+}
+
+// First-pass price of each product (one block per product).
+__global__ void __launch_bounds__(256) k_ls_price_batch(const LsItem* __restrict__ items) {
+  const LsItem& it = items[blockIdx.x];
+  finalize_col(gmem(it.part1), it.ls_blocks, 1, 0, 1.0 / static_cast<double>(it.P),
+               gmem(it.price_out));
 }
 
 }  // namespace
@@ -1188,4 +1317,145 @@ extern "C" int reduceOuterProduct_cublas(int paths, int row_size, int col_size, 
                        input_data_global1, input_data_global2, answer_global,
                        workspace_data_global, outer_product_workspace_floats(paths), alpha, beta,
                        true, as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Batched portfolio: path quantities and LS first passes.
+// ---------------------------------------------------------------------------
+extern "C" size_t bermudan_ls_portfolio_descriptor_bytes(int n_products) {
+  return n_products > 0 ? sizeof(LsItem) * static_cast<size_t>(n_products) : 0;
+}
+
+extern "C" int bermudan_ls_portfolio_prepare(int n_products, const koo_ls_product* products,
+                                             void* descriptors_device, koo_portfolio_plan* plan) {
+  const char* who = "bermudan_ls_portfolio_prepare";
+  if (n_products < 1 || !products || !descriptors_device || !plan)
+    return set_error(KOO_ERR_INVALID, "%s: bad arguments", who);
+  std::vector<LsItem> items(n_products);
+  std::vector<double> cost(n_products);
+  int max_n = 0, max_e = 0;
+  for (int i = 0; i < n_products; ++i) {
+    const koo_ls_product& q = products[i];
+    if (q.paths < 1 || q.rates < 1 || q.rates > 50 || q.factors < 1 || q.factors > 5 ||
+        q.n_ex < 1 || q.n_ex > kLsBatchMaxDates || q.n_ex > q.rates)
+      return set_error(KOO_ERR_INVALID,
+                       "%s: product %d: need paths>=1, 1<=rates<=50, 1<=factors<=5, "
+                       "1<=dates<=min(rates,%d)",
+                       who, i, kLsBatchMaxDates);
+    if (!q.variates || !q.initial_rates || !q.taus || !q.displacements || !q.pseudo_roots ||
+        !q.exercise_steps || !q.basis_variables || !q.exercise_values || !q.numeraires ||
+        !q.workspace || !q.coefficients || !q.price_out)
+      return set_error(KOO_ERR_INVALID, "%s: product %d: null pointer", who, i);
+    LsItem& it = items[i];
+    it = LsItem{};
+    for (int s = 0; s < kMaxRates; ++s) it.ex.e_of_step[s] = -1;
+    for (int e = 0; e < q.n_ex; ++e) {
+      const int s = q.exercise_steps[e];
+      if (s < 0 || s >= q.rates || (e > 0 && s <= q.exercise_steps[e - 1]))
+        return set_error(KOO_ERR_INVALID, "%s: product %d: exercise steps must increase", who, i);
+      it.ex.e_of_step[s] = static_cast<signed char>(e);
+      for (int v = 0; v < 2; ++v) {
+        it.sh[e][v] = q.var_shift ? q.var_shift[2 * e + v] : 0.0f;
+        it.sc[e][v] = q.var_scale ? q.var_scale[2 * e + v] : 1.0f;
+      }
+    }
+    it.ex.last = q.exercise_steps[q.n_ex - 1];
+    const LsLayout lay = ls_layout(q.paths);
+    char* ws = static_cast<char*>(q.workspace);
+    it.z = (g_cfloat*)q.variates;
+    it.f0 = (g_cfloat*)q.initial_rates;
+    it.taus = (g_cfloat*)q.taus;
+    it.disp = (g_cfloat*)q.displacements;
+    it.A = (g_cfloat*)q.pseudo_roots;
+    it.bv = (g_float*)q.basis_variables;
+    it.ev = (g_float*)q.exercise_values;
+    it.num = (g_float*)q.numeraires;
+    it.Y = (g_float*)(ws + lay.y);
+    it.part0 = (g_double*)(ws + lay.part);
+    it.part1 = (g_double*)(ws + lay.part2);
+    it.coeffs = (g_double*)q.coefficients;
+    it.price_out = (g_double*)q.price_out;
+    it.strike = q.strike;
+    it.P = q.paths;
+    it.N = q.rates;
+    it.F = q.factors;
+    it.E = q.n_ex;
+    it.itm_only = q.itm_only != 0;
+    it.paths_blocks = (q.paths + 255) / 256;
+    // LS blocks per product: about two 1024-lane blocks per CU over the whole
+    // portfolio (each block also sums the later date's partials and solves
+    // its normal equations, so fewer, fuller blocks amortise that)
+    {
+      const int share = (2 * device_cu_count() + n_products - 1) / n_products;
+      const int g = gram_blocks(q.paths);
+      it.ls_blocks = share < 1 ? 1 : (share < g ? share : g);
+    }
+    double u = 0.0;
+    for (int s = 0; s <= it.ex.last; ++s) u += q.rates - s;
+    cost[i] = u * q.paths * (q.factors + 4);
+    if (q.rates > max_n) max_n = q.rates;
+    if (q.n_ex > max_e) max_e = q.n_ex;
+  }
+  // path kernel: rate class first (one launch per class), then most
+  // expensive first; LS blocks in the same order
+  std::vector<int> order(n_products);
+  for (int i = 0; i < n_products; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int ca = rate_class(items[a].N), cb = rate_class(items[b].N);
+    return ca != cb ? ca < cb : cost[a] > cost[b];
+  });
+  std::vector<LsItem> sorted(n_products);
+  int64_t pb = 0, lb = 0;
+  *plan = koo_portfolio_plan{};
+  for (int i = 0; i < n_products; ++i) {
+    sorted[i] = items[order[i]];
+    sorted[i].paths_block0 = static_cast<int>(pb);
+    sorted[i].ls_block0 = static_cast<int>(lb);
+    const int c = rate_class(sorted[i].N);
+    if (plan->classes == 0 || rate_class(sorted[i - 1].N) != c) {
+      plan->class_first_block[plan->classes] = static_cast<int>(pb);
+      ++plan->classes;
+    }
+    plan->class_blocks[plan->classes - 1] += sorted[i].paths_blocks;
+    if (sorted[i].N > plan->class_max_rates[plan->classes - 1])
+      plan->class_max_rates[plan->classes - 1] = sorted[i].N;
+    pb += sorted[i].paths_blocks;
+    lb += sorted[i].ls_blocks;
+  }
+  if (pb > 0x7fffffff || lb > 0x7fffffff)
+    return set_error(KOO_ERR_INVALID, "%s: too many blocks", who);
+  if (hipMemcpy(descriptors_device, sorted.data(), sizeof(LsItem) * n_products,
+                hipMemcpyHostToDevice) != hipSuccess)
+    return set_error(KOO_ERR_HIP, "%s: descriptor copy failed", who);
+  plan->products = n_products;
+  plan->blocks = static_cast<int>(pb);
+  plan->max_rates = max_n;
+  plan->ls_blocks = static_cast<int>(lb);
+  plan->max_ex = max_e;
+  return KOO_OK;
+}
+
+extern "C" int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan,
+                                         const void* descriptors_device, void* stream) {
+  const char* who = "bermudan_ls_portfolio_run";
+  if (!plan || !descriptors_device || plan->products < 1 || plan->blocks < 1 ||
+      plan->ls_blocks < 1 || plan->max_rates < 1 || plan->max_rates > 50 || plan->max_ex < 1 ||
+      plan->max_ex > kLsBatchMaxDates || plan->classes < 1 || plan->classes > 4)
+    return set_error(KOO_ERR_INVALID, "%s: bad plan", who);
+  const LsItem* items = static_cast<const LsItem*>(descriptors_device);
+  hipStream_t st = as_stream(stream);
+  for (int c = 0; c < plan->classes; ++c) {
+    const int nc = plan->class_max_rates[c], b0 = plan->class_first_block[c];
+    const int nb = plan->class_blocks[c];
+    if (nc < 1 || nc > plan->max_rates || nb < 1 || b0 < 0 || b0 + nb > plan->blocks)
+      return set_error(KOO_ERR_INVALID, "%s: bad plan class %d", who, c);
+    const size_t lds = sizeof(float) * (static_cast<size_t>(nc) * 256 + nc * nc);
+    hipLaunchKernelGGL(k_berm_paths_batch, dim3(nb), dim3(256), lds, st, items, plan->products,
+                       b0);
+  }
+  for (int r = 0; r <= plan->max_ex; ++r)
+    hipLaunchKernelGGL(k_ls_round, dim3(plan->ls_blocks), dim3(kGramThreads), 0, st, items,
+                       plan->products, r);
+  hipLaunchKernelGGL(k_ls_price_batch, dim3(plan->products), dim3(256), 0, st, items);
+  return check_launch(who);
 }

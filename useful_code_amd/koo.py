@@ -40,7 +40,19 @@ class BermProduct(ctypes.Structure):
 
 class PortfolioPlan(ctypes.Structure):
     """koo_portfolio_plan (include/koo_c_abi.h)."""
-    _fields_ = [("products", _i), ("blocks", _i), ("max_rates", _i)]
+    _fields_ = [("products", _i), ("blocks", _i), ("max_rates", _i), ("ls_blocks", _i),
+                ("max_ex", _i), ("classes", _i), ("class_first_block", _i * 4),
+                ("class_blocks", _i * 4), ("class_max_rates", _i * 4)]
+
+
+class LsProduct(ctypes.Structure):
+    """koo_ls_product (include/koo_c_abi.h)."""
+    _fields_ = [("paths", _i), ("rates", _i), ("factors", _i), ("variates", _p),
+                ("initial_rates", _p), ("taus", _p), ("displacements", _p), ("pseudo_roots", _p),
+                ("strike", _f), ("n_ex", _i), ("exercise_steps", _p), ("var_shift", _p),
+                ("var_scale", _p), ("itm_only", _i), ("basis_variables", _p),
+                ("exercise_values", _p), ("numeraires", _p), ("workspace", _p),
+                ("coefficients", _p), ("price_out", _p)]
 
 
 # name -> (restype, argtypes); pointers are passed as integers (c_void_p).
@@ -114,6 +126,9 @@ _SIGNATURES = {
     "PointwiseProductMultipleFirst_gpu": (_i, [_i, _i, _i, _p, _p, _p, _i, _p]),
     "bermudan_paths_gpu": (_i, [_i, _i, _i] + [_p] * 5 + [_i, _p, _f] + [_p] * 4),
     "bermudan_portfolio_descriptor_bytes": (_sz, [_i]),
+    "bermudan_ls_portfolio_descriptor_bytes": (_sz, [_i]),
+    "bermudan_ls_portfolio_prepare": (_i, [_i, _p, _p, _p]),
+    "bermudan_ls_portfolio_run": (_i, [_p, _p, _p]),
     "bermudan_portfolio_prepare": (_i, [_i, _p, _p, _p]),
     "bermudan_portfolio_fwd": (_i, [_p, _p, _p]),
     "bermudan_portfolio_bwd": (_i, [_p, _p, _p]),
@@ -664,6 +679,25 @@ def bermudan_portfolio_prepare(products, descriptors):
     call("bermudan_portfolio_prepare", len(products), ctypes.addressof(arr), ptr(descriptors),
          ctypes.addressof(plan))
     return plan
+
+
+def bermudan_ls_portfolio_prepare(products, descriptors):
+    """products: list of LsProduct; descriptors: a device buffer of
+    bermudan_ls_portfolio_descriptor_bytes(len(products)) bytes."""
+    arr = (LsProduct * len(products))(*products)
+    plan = PortfolioPlan()
+    call("bermudan_ls_portfolio_prepare", len(products), ctypes.addressof(arr), ptr(descriptors),
+         ctypes.addressof(plan))
+    return plan
+
+
+def bermudan_ls_portfolio_descriptor_bytes(n):
+    return lib().bermudan_ls_portfolio_descriptor_bytes(n)
+
+
+def bermudan_ls_portfolio_run(plan, descriptors, stream=None):
+    call("bermudan_ls_portfolio_run", ctypes.addressof(plan), ptr(descriptors),
+         stream_handle(stream))
 
 
 def bermudan_portfolio_descriptor_bytes(n):

@@ -143,6 +143,21 @@ class BermudanPricer:
             self.frozen_greeks = torch.zeros(2 * N, dtype=torch.float64, device=dev)
         return self.adj
 
+    def ls_batch_product(self):
+        """This product's koo.LsProduct (path quantities and LS first pass,
+        as paths_kernel() + regress())."""
+        w = self.w
+        self._ex_ls = np.ascontiguousarray(w.exercise_steps, np.int32)
+        self._sh_ls = np.ascontiguousarray(self.shift, np.float32)
+        self._sc_ls = np.ascontiguousarray(self.scale, np.float32)
+        return koo.LsProduct(
+            w.paths, w.n_rates, w.factors, self.z.data_ptr(), self.f0.data_ptr(),
+            self.taus.data_ptr(), self.disp.data_ptr(), self.A.data_ptr(), float(w.strike),
+            len(self._ex_ls), self._ex_ls.ctypes.data, self._sh_ls.ctypes.data,
+            self._sc_ls.ctypes.data, 1, self.bv.data_ptr(), self.ev.data_ptr(),
+            self.num.data_ptr(), self.ws.data_ptr(), self.coeffs.data_ptr(),
+            self.price1.data_ptr())
+
     def batch_product(self, checkpoint_every):
         """This product's koo.BermProduct for the batched portfolio kernels
         (buffers as in greeks(); host arrays kept alive on the pricer)."""
@@ -239,9 +254,10 @@ class PortfolioPricer:
         self.stream_of = [j % n_streams for j in range(len(products))]
         self.graph = None
         self.device = device
-        # batched: every product's frozen forward in one launch, then every
-        # backward in one launch (bermudan_portfolio_fwd/_bwd); otherwise the
-        # per-product calls on the streams
+        # batched: the generators on the streams, then the path quantities and
+        # LS first passes of every product (one launch for the paths, one per
+        # date rank), every frozen forward in one launch and every backward in
+        # one launch; otherwise each product's sequence on its stream
         self.batched = batched and all(w.factors <= 5 and w.n_rates <= 45 for w in products)
         self.plan = None
 
@@ -253,8 +269,13 @@ class PortfolioPricer:
         self.plan = None
 
     def _prepare_batch(self):
+        n = len(self.pricers)
+        ls_items = [pr.ls_batch_product() for pr in self.pricers]
+        self.ls_descriptors = torch.empty(koo.bermudan_ls_portfolio_descriptor_bytes(n),
+                                          dtype=torch.uint8, device=self.device)
+        self.ls_plan = koo.bermudan_ls_portfolio_prepare(ls_items, self.ls_descriptors)
         items = [pr.batch_product(self.k[j]) for j, pr in enumerate(self.pricers)]
-        self.descriptors = torch.empty(koo.bermudan_portfolio_descriptor_bytes(len(items)),
+        self.descriptors = torch.empty(koo.bermudan_portfolio_descriptor_bytes(n),
                                        dtype=torch.uint8, device=self.device)
         self.plan = koo.bermudan_portfolio_prepare(items, self.descriptors)
 
@@ -269,13 +290,14 @@ class PortfolioPricer:
             pr = self.pricers[j]
             with torch.cuda.stream(self.streams[self.stream_of[j]]):
                 pr.generate()
-                pr.paths_kernel()
-                pr.regress()
                 if not self.batched:
+                    pr.paths_kernel()
+                    pr.regress()
                     pr.greeks(checkpoint_every=self.k[j])
         for s in self.streams:
             main.wait_stream(s)
         if self.batched:
+            koo.bermudan_ls_portfolio_run(self.ls_plan, self.ls_descriptors)
             koo.bermudan_portfolio_fwd(self.plan, self.descriptors)
             koo.bermudan_portfolio_bwd(self.plan, self.descriptors)
 
