@@ -164,10 +164,12 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
 
 def portfolio_section(args, torch, configs, pipeline, ev):
     """Config 5: 64 Bermudan swaptions (10-40 rates, 4-20 exercise dates,
-    1-5 factors, 2^17 paths each): per product generator, path quantities,
-    LS first pass and frozen-strategy adjoint, all captured into one HIP graph
-    over four streams with the autotuner's checkpoint intervals and stream
-    schedule; one step = one graph replay (price + 2N Greeks for every product)."""
+    1-5 factors, 2^17 paths each): the generators over four streams, then the
+    batched launches -- path quantities (one per rate class), LS first passes
+    (one per date rank), frozen forward and adjoint backward (one per rate
+    class each) -- all captured into one HIP graph with the autotuner's
+    checkpoint interval and stream schedule; one step = one graph replay
+    (first-pass price, frozen price and 2N Greeks for every product)."""
     from useful_code_amd import autotune
     prods = configs.bermudan_portfolio(paths=args.portfolio_paths)
     pf = pipeline.PortfolioPricer(prods, checkpoint_every=4, n_streams=4)
@@ -196,7 +198,7 @@ def portfolio_section(args, torch, configs, pipeline, ev):
         "workload": "bermudan_portfolio_64: 64 Bermudan payer swaptions, 10-40 rates, 4-20 "
                     "exercise dates, K 0.03-0.07, 1-5 factors, 2^17 Sobol paths each; LS first pass "
                     "(ITM, MFMA normal equations, FP64 Cholesky) + frozen-strategy adjoint deltas "
-                    "and vegas per product",
+                    "and vegas per product; batched launches in one graph",
         "graph_replay_ms": ms,
         "eager_ms": eager_ms,
         "path_steps_per_s": path_steps / (ms * 1e-3),
@@ -205,6 +207,7 @@ def portfolio_section(args, torch, configs, pipeline, ev):
         "checkpoint_every": tuning["k"],
         "stream_load_ms": tuning["stream_load_ms"],
         "frozen_price_sum": float(sum(r[1] for r in res)),
+        "batched": bool(pf.batched),
     }
 
 

@@ -297,6 +297,30 @@ int cashFlowGeneratorCallerCaplet(float* genFlows1, float* genFlows2, float* aux
                                   int numberSteps, float* rates1, float* rates2, float* rates3,
                                   float* forwards, float* discRatios, void* stream);
 
+/* cashFlowGeneratorEE_gpu<Swap, earlyExerciseNull,
+ * LSAMultiExerciseStrategyQuadratic(Cross)_gpu> (cashFlowGen_earlyExProduct_gpu.h:
+ * 28-172, 323-497, 756-1049; the instantiations of LSmulti_test.cu:2185-2236 and
+ * bermudanIRDPricer.cu:1391-1440): the cancellable swap's second pass under a
+ * Longstaff-Schwartz multi-regression strategy.  use_cross_terms picks the
+ * cross strategy.  The strategy arrays are the reference's packed layout
+ * (outputDataVectors): int data = depth, max variables, row size, variables
+ * per date [E]; float data = Andersen shifts [E], variable shifts and
+ * divisors [E][depth][maxVars], lower and upper cuts [E][depth], weights
+ * [E][depth][row].  isExerciseDate is one byte per step (the reference's
+ * bool array); rates (steps, paths); strategyVariables (E, maxSize, paths);
+ * estContinuationValues (steps, paths) is written at exercise dates;
+ * exerciseIndices[p] = steps processed.  The exercise-value tables, forwards
+ * and discount ratios are unused (earlyExerciseNull), as in the reference. */
+int cashFlowGeneratorEE_cancellable_swap_gpu(
+    int use_cross_terms, float* genFlows1, float* genFlows2, const float* productData,
+    const float* exerciseValueDataFloat, const int* exerciseValueDataInt,
+    const float* exerciseStrategyDataFloat, int exerciseStrategyDataFloat_size,
+    const int* exerciseStrategyDataInt, int exerciseStrategyDataInt_size, int paths,
+    int numberSteps, int numberExerciseDates, const unsigned char* isExerciseDate,
+    const float* rates1, const float* rates2, const float* rates3, const float* strategyVariables,
+    int strategyVariables_maxSize, const float* forwards, const float* discRatios,
+    float* estContinuationValues, int* exerciseIndices, void* stream);
+
 /* Discount flows generated at step s, paid between rate times
  * firstIndex[s] and secondIndex[s] (log-linear weight theta[s]), to the
  * numeraire: discountedFlows(s,p) and summedDiscountedFlows(p).  The two

@@ -88,6 +88,13 @@ def _declare(L):
         "orc_bridge_tables": (None, [_c_int, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p]),
         "orc_bridge_transform": (None, [_c_int, _c_int, _c_int, _i32p, _f64p, _f64p]),
         "orc_bridge_pow2": (None, [_c_int, _c_int, _f64p, _f64p]),
+        "orc_ls_multi_first_pass": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p,
+                                             _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_dbl,
+                                             _c_dbl, _c_dbl, _f64p, _f64p, _f64p, _f64p, _f64p]),
+        "orc_select_fraction": (ctypes.c_long, [_f64p, _i32p, _i32p, _c_int, ctypes.c_long,
+                                                _c_dbl, _c_dbl, _c_dbl, _c_dbl, _f64p, _f64p]),
+        "orc_cashflows_ee": (None, [_c_int, _f64p, _f64p, _f64p, _f64p, _i32p, _c_int, _c_int,
+                                    _c_int, ctypes.c_void_p, _f64p, _f64p, _c_int, _f64p, _i32p]),
         "orc_bridge_reorder": (None, [_f64p, _f64p, _c_int, _c_int, _c_int, _c_int]),
         "orc_correlate_drift": (None, [_f64p, _f64p, _f64p, _i32p, _f64p, _c_int, _c_int,
                                        _c_int, _c_int, _c_int]),
@@ -160,6 +167,11 @@ def _declare_ref(R):
         "ref_invnorm_sb": (ctypes.c_float, [_c_u32]),
         "ref_bridge_allocate_diagonal": (None, [_c_int, _c_int, _i32p]),
         "ref_bridge_pow2": (None, [_c_int, _c_int, _f64p, _f64p]),
+        "ref_ls_multi": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p, _c_int,
+                                  _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _f64p,
+                                  _f64p, _f64p]),
+        "ref_cashflows_ee": (None, [_c_int, _f64p, _f64p, _f64p, _c_int, _f64p, _i32p, _c_int,
+                                    _c_int, _c_int, ctypes.c_void_p, _f64p, _f64p, _c_int]),
         "ref_bridge_multi": (None, [_c_int, _c_int, _c_int, _c_int, _f64p, _f64p]),
         "ref_correlate_drift": (None, [_f64p, _f64p, _f64p, _c_int, _f64p, _c_int, _c_int,
                                        _c_int, _c_int]),
@@ -321,6 +333,76 @@ def lmm_pc_evolve(f0, taus, variates, A, disp, alive, transposed=True, initial_d
                             fx.reshape(-1), disp, alive, P, F, S, N, int(transposed),
                             rates, logs, disc)
     return rates.reshape(S, N, P), logs.reshape(S, N, P), disc.reshape(S, N + 1, P)
+
+
+def ls_multi_first_pass(bv, vars_each_step, cash, ev, num, use_cross, max_regressions,
+                        lower_path_cutoff, lower_frac, upper_frac, sd_guess, multiplier):
+    """Multi-regression first pass with the fraction selector (normalised
+    variables).  bv (E, maxVars, P); returns price, coefficients
+    (E, R, maxd), lower and upper cuts (E, R), means and sds (E, R, maxVars)."""
+    bv = np.ascontiguousarray(bv, np.float64)
+    E, mv, P = bv.shape
+    R = int(max_regressions)
+    maxd = basis_count(mv, use_cross)
+    co = np.zeros(E * R * maxd)
+    lo, up = np.zeros(E * R), np.zeros(E * R)
+    me, sd = np.zeros(E * R * mv), np.zeros(E * R * mv)
+    f = lambda a: np.ascontiguousarray(a, np.float64).reshape(-1)  # noqa: E731
+    price = lib().orc_ls_multi_first_pass(
+        E, bv.reshape(-1), np.ascontiguousarray(vars_each_step, np.int32), mv, f(cash), f(ev),
+        f(num), P, int(use_cross), R, int(lower_path_cutoff), float(lower_frac), float(upper_frac),
+        float(sd_guess), float(multiplier), co, lo, up, me, sd)
+    return (price, co.reshape(E, R, maxd), lo.reshape(E, R), up.reshape(E, R),
+            me.reshape(E, R, mv), sd.reshape(E, R, mv))
+
+
+def select_fraction(net, pre, lower_frac, upper_frac, sd_guess, multiplier):
+    """The fraction selector on net values over the pre-selected paths;
+    returns (selected mask, count, lower cut, upper cut)."""
+    net = np.ascontiguousarray(net, np.float64)
+    pre = np.ascontiguousarray(pre, np.int32)
+    sel = np.zeros(len(net), np.int32)
+    lo, up = np.zeros(1), np.zeros(1)
+    c = lib().orc_select_fraction(net, pre, sel, len(net), int(pre.sum()), float(lower_frac),
+                                  float(upper_frac), float(sd_guess), float(multiplier), lo, up)
+    return sel, int(c), float(lo[0]), float(up[0])
+
+
+def ls_multi_strategy_tables(shifts, var_shift, var_div, lower, upper, weights, vars_per_date,
+                             cross):
+    """The reference's packed LS multi-regression strategy arrays
+    (LSAMultiExerciseStrategyQuadratic(Cross)_gpu::outputDataVectors,
+    cashFlowGen_earlyExProduct_gpu.h:775-830): float data = Andersen shifts
+    [E], variable shifts [E][depth][maxv], divisors [E][depth][maxv], lower
+    cuts [E][depth], upper cuts [E][depth], weights [E][depth][row]; int data
+    = depth, maxv, row, variables per date [E]."""
+    E, depth, maxv = np.shape(var_shift)
+    v = int(max(vars_per_date))
+    assert maxv == v, "the variable stride of the packed tables is the largest variable count"
+    row = (3 * v + 1 if v > 2 else (6 if v == 2 else 3)) if cross else 2 * v + 1
+    fd = np.concatenate([np.ravel(shifts), np.ravel(var_shift), np.ravel(var_div),
+                         np.ravel(lower), np.ravel(upper), np.ravel(weights)]).astype(np.float64)
+    assert np.shape(weights) == (E, depth, row)
+    iv = np.concatenate([[depth, maxv, row], vars_per_date]).astype(np.int32)
+    return fd, iv
+
+
+def cashflows_ee(cross, aux, fd, iv, is_ex, r1, sv):
+    """Cancellable swap second pass; r1 (S,P), sv (E, maxSize, P); returns
+    flows1, flows2 (S,P), continuation estimates (S,P) (exercise dates only)
+    and the steps processed per path."""
+    r1 = np.ascontiguousarray(r1, np.float64)
+    S, P = r1.shape
+    sv = np.ascontiguousarray(sv, np.float64)
+    E, smax, _ = sv.shape
+    f1, f2, est = np.zeros((S, P)), np.zeros((S, P)), np.zeros((S, P))
+    idx = np.zeros(P, np.int32)
+    ex = np.ascontiguousarray(is_ex, np.uint8)
+    lib().orc_cashflows_ee(int(cross), f1.reshape(-1), f2.reshape(-1),
+                           np.ascontiguousarray(aux, np.float64), np.ascontiguousarray(fd),
+                           np.ascontiguousarray(iv, np.int32), P, S, E, ex.ctypes.data,
+                           r1.reshape(-1), sv.reshape(-1), smax, est.reshape(-1), idx)
+    return f1, f2, est, idx
 
 
 def bridge_pow2(log2, z):

@@ -324,6 +324,93 @@ void orc_bridge_reorder(const double *in, double *out, int log2, int factors, in
   }
 }
 
+/* Cancellable swap under the LS multi-regression strategy, the second pass
+ * cashFlowGeneratorEE<Swap, earlyExerciseNull, LSAMultiExerciseStrategy>
+ * (gold/cashFlowGeneration_earlyEx_product.h:26-86 and its strategy
+ * :405-640; GPU cashFlowGen_earlyExProduct_gpu.h:28-172, 756-1049).  The
+ * strategy tables are the reference's packed GPU arrays (see the product
+ * header); at depth d the estimate is the Andersen shift plus the regression
+ * of (x_i - shift)/divisor; before the last depth a net value below the
+ * date's lower cut exercises and above its upper cut continues, otherwise
+ * the next depth refines; at the last depth exercise iff estimate < 0
+ * (the exercise value of a cancellation).  Flows: swap flows
+ * (aux = [K, payReceive, acc1[S], acc2[S]]) until cancellation or the end;
+ * ex_index[p] = steps processed. */
+static double orc_ls_multi_value(int cross, const double *fd, const int *id, int E, int ex,
+                                 const double *sv, int sv_max, size_t P, int p, double ev,
+                                 int *exercise) {
+  const int depth_max = id[0], maxv = id[1], row = id[2], nv = id[3 + ex];
+  const double *shifts = fd + E + (size_t)ex * depth_max * maxv;
+  const double *divs = shifts + (size_t)E * depth_max * maxv;
+  const double *lower = fd + E + 2 * (size_t)E * depth_max * maxv + (size_t)ex * depth_max;
+  const double *upper = lower + (size_t)E * depth_max;
+  const double *w0 = fd + E + 2 * (size_t)E * depth_max * maxv + 2 * (size_t)E * depth_max +
+                     (size_t)ex * depth_max * row;
+  const double *vars = sv + (size_t)ex * P * sv_max + p;
+  double x[16];
+  for (int depth = 0;;) {
+    const double *w = w0 + (size_t)depth * row;
+    for (int i = 0; i < nv; ++i) x[i] = (vars[(size_t)i * P] - shifts[maxv * depth + i]) / divs[maxv * depth + i];
+    double value = fd[ex] + *w++;
+    if (!cross || nv == 1) {
+      for (int i = 0; i < nv; ++i) {
+        value += (*w++) * x[i];
+        value += (*w++) * x[i] * x[i];
+      }
+    } else if (nv == 2) {
+      value += w[0] * x[0] + w[1] * x[1] + w[2] * x[0] * x[0] + w[3] * x[0] * x[1] +
+               w[4] * x[1] * x[1];
+    } else {
+      for (int i = 0; i < nv; ++i) value += (*w++) * x[i];
+      for (int i = 0; i < nv; ++i) value += (*w++) * x[i] * x[(i + 1) % nv];
+      for (int i = 0; i < nv; ++i) value += (*w++) * x[i] * x[i];
+    }
+    ++depth;
+    if (depth == depth_max) {
+      *exercise = value < ev;
+      return value;
+    }
+    if (value - ev < lower[depth - 1]) {
+      *exercise = 1;
+      return lower[depth - 1];
+    }
+    if (value - ev > upper[depth - 1]) {
+      *exercise = 0;
+      return upper[depth - 1];
+    }
+  }
+}
+
+void orc_cashflows_ee(int cross, double *f1, double *f2, const double *aux, const double *fd,
+                      const int *id, int paths, int steps, int n_ex, const unsigned char *is_ex,
+                      const double *r1, const double *sv, int sv_max, double *est,
+                      int *ex_index) {
+  const size_t P = (size_t)paths;
+  for (int p = 0; p < paths; ++p) {
+    int i = 0, ex = 0, done = 0, index = 0;
+    for (; i < steps && !done; ++i) {
+      const size_t o = (size_t)i * P + p;
+      double a = 0.0, b = 0.0;
+      int exercise = 0;
+      if (is_ex[i]) {
+        est[o] = orc_ls_multi_value(cross, fd, id, n_ex, ex, sv, sv_max, P, p, 0.0, &exercise);
+        if (exercise) done = 1;
+        ++ex;
+      }
+      if (!exercise) {
+        b = aux[2 + steps + index] * r1[o] * aux[1];
+        a = -aux[0] * aux[2 + index] * aux[1];
+        ++index;
+        done = index == steps;
+      }
+      f1[o] = a;
+      f2[o] = b;
+    }
+    ex_index[p] = i;
+    for (; i < steps; ++i) f1[(size_t)i * P + p] = f2[(size_t)i * P + p] = 0.0;
+  }
+}
+
 /* y(s,r,p) = sum_f A(s,r,f) z(s,p,f) + mu_fixed(s,r) for r >= alive(s).
  * correlate_drift_gpu.cu:142-220 (correlation_drift_kernel); gold twin
  * gold/correlate_drift_gold.h:22-61 (which fills every r). */
@@ -1243,6 +1330,197 @@ double orc_ls_first_pass(int n_ex, const double *bv, const int *vars_each_step, 
   double s = 0.0;
   for (int p = 0; p < paths; ++p) s += Y[p] / num[p];
   free(Y);
+  return s / paths;
+}
+
+/* RegressionSelector_gold_Fraction::Select (Regression_Selector_concrete_gold.cpp:
+ * 13-183, tolerance 1e-6): a symmetric band [-c, c] on the net values of the
+ * pre-selected paths holding between lower_frac and upper_frac of them,
+ * found from c = sd_guess * sd by growing (c /= multiplier), shrinking
+ * (c *= multiplier) and bisection; marks the paths inside, returns their
+ * count and the cuts. */
+static long orc_count_band(const double *x, const int *pre, int n, double lo, double hi) {
+  long c = 0;
+  for (int p = 0; p < n; ++p)
+    if (x[p] >= lo && x[p] <= hi && pre[p]) ++c;
+  return c;
+}
+
+long orc_select_fraction(const double *net, const int *pre, int *sel, int paths, long n_pre,
+                         double lower_frac, double upper_frac, double sd_guess, double multiplier,
+                         double *lower_cut, double *upper_cut) {
+  const long lowerT = (long)(n_pre * lower_frac), upperT = (long)(n_pre * upper_frac);
+  double cut;
+  if (n_pre > lowerT && n_pre < upperT) {
+    cut = 10000.0;
+  } else {
+    double sum = 0.0, sumsq = 0.0;
+    for (int p = 0; p < paths; ++p)
+      if (pre[p]) {
+        sum += net[p];
+        sumsq += net[p] * net[p];
+      }
+    const double mean = sum / n_pre, sd = sqrt(sumsq / n_pre - mean * mean);
+    double guess = sd * sd_guess;
+    long count = orc_count_band(net, pre, paths, -guess, guess);
+    while (count < lowerT) {
+      guess /= multiplier;
+      count = orc_count_band(net, pre, paths, -guess, guess);
+    }
+    if (count < upperT) {
+      cut = guess;
+    } else {
+      double low = guess * multiplier;
+      count = orc_count_band(net, pre, paths, -low, low);
+      while (count > upperT) {
+        low *= multiplier;
+        count = orc_count_band(net, pre, paths, -low, low);
+      }
+      if (count > lowerT) {
+        cut = low;
+      } else {
+        double high = guess;
+        guess = 0.5 * (low + high);
+        count = orc_count_band(net, pre, paths, -guess, guess);
+        while ((count < lowerT || count > upperT) && (high - low) > 1e-6) {
+          if (count < lowerT)
+            low = guess;
+          else
+            high = guess;
+          guess = 0.5 * (low + high);
+          count = orc_count_band(net, pre, paths, -guess, guess);
+        }
+        cut = guess;
+      }
+    }
+  }
+  *lower_cut = -cut;
+  *upper_cut = cut;
+  long c = 0;
+  for (int p = 0; p < paths; ++p) {
+    sel[p] = net[p] >= -cut && net[p] <= cut && pre[p];
+    c += sel[p];
+  }
+  return c;
+}
+
+/* The multi-regression first pass with normalised variables,
+ * generateRegressionCoefficientsViaLSMulti_gold (gold/MultiLS_regression.cpp:
+ * 20-420, normalise = true; GPU LS_main_cula.h:565-838).  Per exercise date
+ * (last first): regression of cash + deflated next value on the quadratic
+ * basis of the variables standardised by their mean and standard deviation
+ * over the regression paths (depth 0: every path); net continuation values;
+ * the selector's band (RegressionSelector_gold_Fraction) picks the paths of
+ * the next regression, up to max_regressions, while more than
+ * lower_path_cutoff paths remain (cuts 0 otherwise).  Then each path walks
+ * the depths while its net value lies strictly inside the previous depth's
+ * band (updateDeflatedNextStepValuesMulti_gold, gold/MultiLS_regression.h:
+ * 47-116) and exercises when the exercise value exceeds the estimate.
+ * Outputs (E x max_regressions x maxd coefficients, E x max_regressions
+ * cuts, E x max_regressions x max_vars means and sds) are zero where a depth
+ * was not reached.  Returns the first-pass price. */
+double orc_ls_multi_first_pass(int n_ex, const double *bv, const int *vars_each_step,
+                               int max_vars, const double *cash, const double *ev,
+                               const double *num, int paths, int use_cross, int max_regressions,
+                               int lower_path_cutoff, double lower_frac, double upper_frac,
+                               double sd_guess, double multiplier, double *coeffs, double *lower,
+                               double *upper, double *means, double *sds) {
+  const int maxd = orc_basis_count(max_vars, use_cross), R = max_regressions;
+  double *Y = (double *)calloc(paths, sizeof(double));
+  double *net = (double *)calloc(paths, sizeof(double));
+  int *pre = (int *)malloc(sizeof(int) * paths), *sel = (int *)calloc(paths, sizeof(int));
+  double x[16], phi[64], M[64 * 64], t[64];
+  for (size_t i = 0; i < (size_t)n_ex * R * maxd; ++i) coeffs[i] = 0.0;
+  for (size_t i = 0; i < (size_t)n_ex * R; ++i) lower[i] = upper[i] = 0.0;
+  for (size_t i = 0; i < (size_t)n_ex * R * max_vars; ++i) means[i] = sds[i] = 0.0;
+  for (int e = n_ex - 1; e >= 0; --e) {
+    const int v = vars_each_step[e], nd = orc_basis_count(v, use_cross);
+    for (int p = 0; p < paths; ++p) pre[p] = 1;
+    long n_pre = paths;
+    for (int depth = 0; depth < R; ++depth) {
+      double *m = means + ((size_t)e * R + depth) * max_vars;
+      double *sd = sds + ((size_t)e * R + depth) * max_vars;
+      for (int i = 0; i < v; ++i) {
+        double a = 0.0, b = 0.0;
+        for (int p = 0; p < paths; ++p)
+          if (pre[p]) {
+            const double xv = bv[((size_t)e * max_vars + i) * paths + p];
+            a += xv;
+            b += xv * xv;
+          }
+        a /= n_pre;
+        b /= n_pre;
+        m[i] = a;
+        sd[i] = sqrt(b - a * a);
+      }
+      for (int i = 0; i < nd * nd; ++i) M[i] = 0.0;
+      for (int i = 0; i < nd; ++i) t[i] = 0.0;
+      for (int p = 0; p < paths; ++p) {
+        if (!pre[p]) continue;
+        for (int i = 0; i < v; ++i) x[i] = (bv[((size_t)e * max_vars + i) * paths + p] - m[i]) / sd[i];
+        orc_basis_functions(x, v, use_cross, phi);
+        const double target = cash[(size_t)e * paths + p] + Y[p];
+        for (int i = 0; i < nd; ++i) {
+          for (int j = 0; j <= i; ++j) M[i * nd + j] += phi[i] * phi[j];
+          t[i] += target * phi[i];
+        }
+      }
+      for (int i = 0; i < nd; ++i) {
+        M[i * nd + i] /= n_pre;
+        t[i] /= n_pre;
+        for (int j = 0; j < i; ++j) M[j * nd + i] = M[i * nd + j] = M[i * nd + j] / n_pre;
+      }
+      double *beta = coeffs + ((size_t)e * R + depth) * maxd;
+      if (orc_cholesky_solve(nd, M, t, beta) != 0)
+        for (int i = 0; i < nd; ++i) beta[i] = 0.0;
+      for (int p = 0; p < paths; ++p) {
+        if (!pre[p]) continue;
+        for (int i = 0; i < v; ++i) x[i] = (bv[((size_t)e * max_vars + i) * paths + p] - m[i]) / sd[i];
+        orc_basis_functions(x, v, use_cross, phi);
+        double est = 0.0;
+        for (int i = 0; i < nd; ++i) est += phi[i] * beta[i];
+        net[p] = est - ev[(size_t)e * paths + p];
+      }
+      double lo, hi;
+      const long after = orc_select_fraction(net, pre, sel, paths, n_pre, lower_frac, upper_frac,
+                                             sd_guess, multiplier, &lo, &hi);
+      if (after > lower_path_cutoff) {
+        lower[(size_t)e * R + depth] = lo;
+        upper[(size_t)e * R + depth] = hi;
+      }
+      if (after <= lower_path_cutoff) break;
+      for (int p = 0; p < paths; ++p) pre[p] = sel[p];
+      n_pre = after;
+    }
+    for (int p = 0; p < paths; ++p) {
+      const double exv = ev[(size_t)e * paths + p];
+      double est = 0.0, nv = 0.0;
+      for (int depth = 0; depth == 0 || (depth < R && nv > lower[(size_t)e * R + depth - 1] &&
+                                                    nv < upper[(size_t)e * R + depth - 1]);
+           ++depth) {
+        const double *m = means + ((size_t)e * R + depth) * max_vars;
+        const double *sd = sds + ((size_t)e * R + depth) * max_vars;
+        for (int i = 0; i < v; ++i) x[i] = (bv[((size_t)e * max_vars + i) * paths + p] - m[i]) / sd[i];
+        orc_basis_functions(x, v, use_cross, phi);
+        est = 0.0;
+        for (int i = 0; i < nd; ++i) est += phi[i] * coeffs[((size_t)e * R + depth) * maxd + i];
+        nv = est - exv;
+      }
+      if (exv > est)
+        Y[p] = exv;
+      else
+        Y[p] += cash[(size_t)e * paths + p];
+    }
+    if (e > 0)
+      for (int p = 0; p < paths; ++p)
+        Y[p] *= num[(size_t)(e - 1) * paths + p] / num[(size_t)e * paths + p];
+  }
+  double s = 0.0;
+  for (int p = 0; p < paths; ++p) s += Y[p] / num[p];
+  free(Y);
+  free(net);
+  free(pre);
+  free(sel);
   return s / paths;
 }
 

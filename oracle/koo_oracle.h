@@ -44,6 +44,10 @@ void orc_bridge_transform(int steps, int factors, int paths, const int *indices,
 void orc_bridge_pow2(int log2, int vectors, const double *in /* [S][n] */, double *out);
 void orc_bridge_reorder(const double *in /* [D][P] */, double *out /* (S,P,F) */, int log2,
                         int factors, int paths, int ordering);
+void orc_cashflows_ee(int cross, double *f1, double *f2, const double *aux, const double *fd,
+                      const int *id, int paths, int steps, int n_ex, const unsigned char *is_ex,
+                      const double *r1, const double *sv, int sv_max, double *est,
+                      int *ex_index);
 void orc_correlate_drift(const double *in /* (S,P,F) */, double *out /* (S,N,P) */,
                          const double *A, const int *alive, const double *drift_fixed,
                          int a_offset_per_step, int factors, int out_dims, int paths, int steps);
@@ -128,6 +132,15 @@ double orc_ls_first_pass(int n_ex, const double *basis_vars /* (E,maxVars,P) */,
                          const int *vars_each_step, int max_vars, const double *cash /* (E,P) */,
                          const double *ex_values /* (E,P) */, const double *numeraires /* (E,P) */,
                          int paths, int use_cross, int itm_only, double *coeffs /* (E,maxData) */);
+long orc_select_fraction(const double *net, const int *pre, int *sel, int paths, long n_pre,
+                         double lower_frac, double upper_frac, double sd_guess, double multiplier,
+                         double *lower_cut, double *upper_cut);
+double orc_ls_multi_first_pass(int n_ex, const double *bv, const int *vars_each_step,
+                               int max_vars, const double *cash, const double *ev,
+                               const double *num, int paths, int use_cross, int max_regressions,
+                               int lower_path_cutoff, double lower_frac, double upper_frac,
+                               double sd_guess, double multiplier, double *coeffs, double *lower,
+                               double *upper, double *means, double *sds);
 double orc_ls_second_pass(int n_ex, const double *basis_vars, const int *vars_each_step,
                           int max_vars, const double *cash, const double *ex_values,
                           const double *numeraires, int paths, int use_cross, int itm_only,

@@ -19,6 +19,9 @@
 #include <gold/MultiLS_regression.h>
 #include <gold/Regression_Selector_concrete_gold.h>
 #include <gold/cashFlowDiscounting_gold.h>
+#include <gold/cashFlowGeneration_earlyEx_product.h>
+#include <gold/early_exercise_value_generator_gold.h>
+#include <gold/LS_regression.h>
 #include <gold/cashFlowGeneration_product_gold.h>
 #include <gold/correlate_drift_gold.h>
 #include <gold/math/Normals_gold.h>
@@ -84,6 +87,58 @@ void ref_bridge_multi(int log2, int factors, int ordering, int paths, const doub
     for (int f = 0; f < factors; ++f)
       for (int s = 0; s < S; ++s) out[((size_t)s * paths + p) * factors + f] = o[f * S + s];
   }
+}
+
+// gold/cashFlowGeneration_earlyEx_product.h:26-86 cashFlowGeneratorEE_gold with
+// Swap_gold, earlyExerciseNullGold and LSAMultiExerciseStrategy over the
+// quadratic (cross) generators, built from the GPU strategy's packed arrays.
+}  // extern "C"
+template <class Gen>
+static void ref_ee_impl(double* f1, double* f2, const double* aux, int aux_n, const double* fd,
+                        const int* id, int paths, int steps, int E, const unsigned char* is_ex,
+                        const double* r1, const double* sv, int sv_max) {
+  const int depth = id[0], maxv = id[1], row = id[2];
+  std::vector<int> vps(id + 3, id + 3 + E);
+  std::vector<double> shifts(fd, fd + E);
+  const double* msh = fd + E;
+  const double* mdv = msh + (size_t)E * depth * maxv;
+  const double* lo = mdv + (size_t)E * depth * maxv;
+  const double* up = lo + (size_t)E * depth;
+  const double* wts = up + (size_t)E * depth;
+  std::vector<double> weights(wts, wts + (size_t)E * depth * row);
+  std::vector<double> means(msh, msh + (size_t)E * depth * maxv), sds(mdv, mdv + (size_t)E * depth * maxv);
+  std::vector<double> lower(lo, lo + (size_t)E * depth), upper(up, up + (size_t)E * depth);
+  MatrixConstFacade<double> L(&lower[0], E, depth), U(&upper[0], E, depth);
+  CubeConstFacade<double> M(&means[0], E, depth, maxv), D(&sds[0], E, depth, maxv);
+  std::vector<Gen> producers;
+  for (int e = 0; e < E; ++e) producers.push_back(Gen(vps[e]));
+  LSAMultiExerciseStrategy<Gen, double> strategy(E, vps, shifts, weights, depth, L, U, M, D,
+                                                 producers, maxv);
+  std::vector<double> a(aux, aux + aux_n);
+  Swap_gold<double> swap(a, steps);
+  earlyExerciseNullGold<double> null_value;
+  std::vector<bool> ex(steps);
+  for (int i = 0; i < steps; ++i) ex[i] = is_ex[i] != 0;
+  std::vector<double> rates(r1, r1 + (size_t)steps * paths), dummy(1, 0.0);
+  std::vector<double> svv(sv, sv + (size_t)E * sv_max * paths);
+  CubeConstFacade<double> svc(&svv[0], E, sv_max, paths);
+  std::vector<double> g1, g2;
+  cashFlowGeneratorEE_gold(g1, g2, swap, null_value, strategy, paths, steps, ex, rates, rates,
+                           rates, svc, dummy, dummy);
+  std::memcpy(f1, g1.data(), g1.size() * sizeof(double));
+  std::memcpy(f2, g2.data(), g2.size() * sizeof(double));
+}
+extern "C" {
+
+void ref_cashflows_ee(int cross, double* f1, double* f2, const double* aux, int aux_n,
+                      const double* fd, const int* id, int paths, int steps, int E,
+                      const unsigned char* is_ex, const double* r1, const double* sv, int sv_max) {
+  if (cross)
+    ref_ee_impl<quadraticPolynomialCrossGenerator>(f1, f2, aux, aux_n, fd, id, paths, steps, E,
+                                                   is_ex, r1, sv, sv_max);
+  else
+    ref_ee_impl<quadraticPolynomialGenerator>(f1, f2, aux, aux_n, fd, id, paths, steps, E, is_ex,
+                                              r1, sv, sv_max);
 }
 
 // gold/correlate_drift_gold.h:22 correlate_drift_paths_gold<double>
@@ -270,6 +325,40 @@ double ref_ls_multi_depth0(int n_ex, const double* basis_vars, const int* vars_e
       n_ex, products, targets, coeff_cube, lower_mat, upper_mat, means_cube, sds_cube, normalise != 0, bv_cube, ves,
       max_vars, cf_mat, ev_mat, nm_mat, paths, 0, 10, selector, use_cross != 0);
   std::memcpy(coeffs_out, &coeffs[0], sizeof(double) * coeffs.size());
+  return price;
+}
+
+// The same with extra regressions and the fraction selector
+// (gold/MultiLS_regression.cpp:20-420, normalise = true).
+double ref_ls_multi(int n_ex, const double* basis_vars, const int* vars_each_step, int max_vars,
+                    const double* cash, const double* ex_values, const double* numeraires, int paths,
+                    int use_cross, int max_regressions, int lower_path_cutoff, double lower_frac,
+                    double upper_frac, double sd_guess, double multiplier, double* coeffs_out,
+                    double* lower_out, double* upper_out) {
+  std::vector<double> products, targets;
+  const int maxd = use_cross ? (max_vars == 0 ? 1 : max_vars == 1 ? 3 : max_vars == 2 ? 6 : 1 + 3 * max_vars)
+                             : 1 + 2 * max_vars;
+  const int R = max_regressions;
+  std::vector<double> coeffs((size_t)n_ex * R * maxd, 0.0), lower((size_t)n_ex * R, 0.0),
+      upper((size_t)n_ex * R, 0.0);
+  std::vector<double> means((size_t)n_ex * R * max_vars, 0.0), sds((size_t)n_ex * R * max_vars, 1.0);
+  CubeFacade<double> coeff_cube(&coeffs[0], n_ex, R, maxd);
+  MatrixFacade<double> lower_mat(&lower[0], n_ex, R), upper_mat(&upper[0], n_ex, R);
+  CubeFacade<double> means_cube(&means[0], n_ex, R, max_vars), sds_cube(&sds[0], n_ex, R, max_vars);
+  std::vector<double> bv(basis_vars, basis_vars + (size_t)n_ex * max_vars * paths);
+  std::vector<double> cf(cash, cash + (size_t)n_ex * paths), ev(ex_values, ex_values + (size_t)n_ex * paths);
+  std::vector<double> nm(numeraires, numeraires + (size_t)n_ex * paths);
+  CubeConstFacade<double> bv_cube(&bv[0], n_ex, max_vars, paths);
+  MatrixConstFacade<double> cf_mat(&cf[0], n_ex, paths), ev_mat(&ev[0], n_ex, paths);
+  MatrixFacade<double> nm_mat(&nm[0], n_ex, paths);
+  std::vector<int> ves(vars_each_step, vars_each_step + n_ex);
+  RegressionSelector_gold_Fraction selector(lower_frac, upper_frac, sd_guess, multiplier);
+  double price = generateRegressionCoefficientsViaLSMultiQuadratic_gold(
+      n_ex, products, targets, coeff_cube, lower_mat, upper_mat, means_cube, sds_cube, true, bv_cube, ves,
+      max_vars, cf_mat, ev_mat, nm_mat, paths, R - 1, lower_path_cutoff, selector, use_cross != 0);
+  std::memcpy(coeffs_out, &coeffs[0], sizeof(double) * coeffs.size());
+  std::memcpy(lower_out, &lower[0], sizeof(double) * lower.size());
+  std::memcpy(upper_out, &upper[0], sizeof(double) * upper.size());
   return price;
 }
 

@@ -130,6 +130,100 @@ def bridge_pow2(R):
     return out
 
 
+def ee_case(seed, cross, depth, S=12, E=5, P=40):
+    """Seeded inputs of the cancellable-swap second pass (symmetric cut bands,
+    one to three variables per date)."""
+    rng = np.random.RandomState(seed)
+    vpd = rng.randint(1, 4, size=E).astype(np.int32)
+    v = maxv = int(vpd.max())  # the packed tables' variable stride is the largest count
+    row = (3 * v + 1 if v > 2 else (6 if v == 2 else 3)) if cross else 2 * v + 1
+    ex_steps = np.sort(rng.choice(np.arange(1, S - 1), E, replace=False))
+    is_ex = np.zeros(S, np.uint8)
+    is_ex[ex_steps] = 1
+    shifts = rng.normal(0, 0.001, E)
+    var_shift = rng.normal(0.05, 0.005, (E, depth, maxv))
+    var_div = rng.uniform(0.005, 0.02, (E, depth, maxv))
+    cut = rng.uniform(0.002, 0.01, (E, depth))
+    weights = rng.normal(0, 0.01, (E, depth, row))
+    weights[:, :, 0] += 0.004  # a positive mean continuation value: some paths run on
+    r1 = 0.05 + 0.01 * rng.normal(size=(S, P))
+    sv = 0.05 + 0.01 * rng.normal(size=(E, maxv, P))
+    acc = np.full(S, 0.5)
+    aux = np.concatenate([[0.05, 1.0], acc, acc])
+    return dict(vpd=vpd, is_ex=is_ex, shifts=shifts, var_shift=var_shift, var_div=var_div,
+                lower=-cut, upper=cut, weights=weights, r1=r1, sv=sv, aux=aux)
+
+
+def cashflows_ee(R):
+    out = {}
+    for cross in (0, 1):
+        for depth in (1, 2, 3):
+            c = ee_case(10 * depth + cross, cross, depth)
+            fd, iv = oracle.ls_multi_strategy_tables(c["shifts"], c["var_shift"], c["var_div"],
+                                                     c["lower"], c["upper"], c["weights"],
+                                                     c["vpd"], cross)
+            S, P = c["r1"].shape
+            E, smax, _ = c["sv"].shape
+            f1, f2 = np.zeros(S * P), np.zeros(S * P)
+            R.ref_cashflows_ee(cross, f1, f2, c["aux"], len(c["aux"]), fd, iv, P, S, E,
+                               c["is_ex"].ctypes.data, c["r1"].reshape(-1).copy(),
+                               c["sv"].reshape(-1).copy(), smax)
+            key = f"{cross}_{depth}"
+            out["f1_" + key], out["f2_" + key] = f1.reshape(S, P), f2.reshape(S, P)
+            for name, val in c.items():  # the inputs
+                out[f"in_{name}_{key}"] = val
+    return out
+
+
+def ls_multi_case(seed, E=6, P=3000, max_vars=2, last_vars=1):
+    """Seeded LS data (basis variables, cash flows, exercise values,
+    numeraires) with cash flows on every date so no net value is exactly 0.
+    Every date regresses on max_vars variables except the last, which has
+    last_vars (LSmulti_test.cu:1239-1241 gives the final dates fewer).  The
+    gold rebuilds its basis generator only when a date's count differs from
+    the LAST date's (MultiLS_regression.cpp:72,108 never updates the old
+    count), so a count that returns to the last date's value after a change
+    would keep the stale generator; these sequences never do that, and on
+    them the gold regresses every date on its own count as the GPU code does
+    (LS_main.h:231-232).  The generator is numpy's legacy RandomState, whose
+    streams are fixed, so the tests rebuild the inputs from the seed."""
+    rng = np.random.RandomState(seed)
+    ves = np.full(E, max_vars, np.int32)
+    ves[-1] = last_vars
+    bv = 0.05 + 0.01 * rng.normal(size=(E, max_vars, P))
+    ev = np.maximum((bv[:, 0, :] - 0.05) * 4.0 + 0.002 * rng.normal(size=(E, P)), 0.0)
+    cash = 0.001 * rng.normal(size=(E, P)) + 0.0005
+    num = np.cumprod(1.0 + 0.02 + 0.002 * rng.normal(size=(E, P)), axis=0)
+    return dict(ves=ves, bv=bv, ev=ev, cash=cash, num=num)
+
+
+SELECTOR = (0.25, 0.26, 2.5, 0.8)  # lower, upper fraction, sd guess, multiplier (LSmulti_test.cu:270-273)
+# (cross, depth) -> ls_multi_case arguments (seed, E, P, max_vars, last_vars)
+LS_MULTI_CASES = {(0, 1): (7, 6, 3000, 2, 1), (0, 2): (14, 6, 3000, 2, 2),
+                  (0, 3): (21, 6, 3000, 3, 1), (1, 1): (8, 6, 3000, 2, 1),
+                  (1, 2): (15, 6, 3000, 3, 3), (1, 3): (22, 6, 3000, 2, 1)}
+
+
+def ls_multi(R):
+    out = {}
+    for cross in (0, 1):
+        for depth in (1, 2, 3):
+            c = ls_multi_case(*LS_MULTI_CASES[(cross, depth)])
+            E, mv, P = c["bv"].shape
+            maxd = (1 + 3 * mv if mv > 2 else (6 if mv == 2 else 3)) if cross else 1 + 2 * mv
+            co, lo, up = np.zeros(E * depth * maxd), np.zeros(E * depth), np.zeros(E * depth)
+            price = R.ref_ls_multi(E, c["bv"].reshape(-1).copy(), c["ves"], mv,
+                                   c["cash"].reshape(-1).copy(), c["ev"].reshape(-1).copy(),
+                                   c["num"].reshape(-1).copy(), P, cross, depth, 10, *SELECTOR,
+                                   co, lo, up)
+            key = f"{cross}_{depth}"
+            out["price_" + key] = np.array(price)
+            out["coeffs_" + key] = co.reshape(E, depth, maxd)
+            out["lower_" + key] = lo.reshape(E, depth)
+            out["upper_" + key] = up.reshape(E, depth)
+    return out
+
+
 def cashflows(R, g):
     N, P = g["N"], g["P"]
     S = N
@@ -178,6 +272,8 @@ def main():
     np.savez_compressed(os.path.join(HERE, "lmm_gold_small.npz"), **g)
     np.savez_compressed(os.path.join(HERE, "lmm_euler_gold_small.npz"), **lmm_euler(R, g))
     np.savez_compressed(os.path.join(HERE, "bridge_pow2_ref.npz"), **bridge_pow2(R))
+    np.savez_compressed(os.path.join(HERE, "cashflows_ee_ref.npz"), **cashflows_ee(R))
+    np.savez_compressed(os.path.join(HERE, "ls_multi_ref.npz"), **ls_multi(R))
     np.savez_compressed(os.path.join(HERE, "cashflow_ref.npz"), **cashflows(R, g))
     np.savez_compressed(os.path.join(HERE, "pseudo_roots_ref.npz"), **pseudo_roots(R))
     print("golden fixtures written to", HERE)

@@ -67,6 +67,8 @@ def _status(name, *args):
     ("PointwiseProduct_gpu", (1, 1, 16, None, None, None, None)),
     ("PointwiseProductSymmetricUpperTriangular_gpu", (1, 1, 16, -1, None, None, None)),
     ("PointwiseProductMultipleFirst_gpu", (1, 1, 16, None, None, None, 3, None)),
+    ("cashFlowGeneratorEE_cancellable_swap_gpu", (1,) + (None,) * 6 + (1, None, 2, 16, 10, 5)
+     + (None,) * 5 + (1,) + (None,) * 5),
     ("bermudan_portfolio_prepare", (0, None, None, None)),
     ("bermudan_ls_portfolio_prepare", (0, None, None, None)),
     ("bermudan_ls_portfolio_run", (None, None, None)),

@@ -193,3 +193,56 @@ def test_config1_payer_swap_end_to_end(torch_cuda):
     print(f"config 1 swap: gpu={got:.10f} fp64={want:.10f} model-free={model_free:.10f}")
     assert abs(got - want) < 1e-6
     assert abs(got - model_free) < 1e-4
+
+
+@pytest.mark.parametrize("cross", [0, 1])
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_cancellable_swap_second_pass_vs_oracle(torch_cuda, cross, depth):
+    """The reference's cancellable-swap second pass on FP32 inputs against
+    the oracle (pinned to the gold) on the same values: the cancellation step
+    agrees on every path but those whose net continuation value sits within
+    FP32 rounding of a cut or of zero (under 1 in 1000); on the others flows
+    agree to FP32 rounding and the continuation estimates to 1e-5."""
+    torch = torch_cuda
+    rng = np.random.RandomState(100 * depth + cross)
+    S, E, P = 24, 8, 20000
+    vpd = rng.randint(1, 4, size=E).astype(np.int32)
+    v = int(vpd.max())
+    row = (3 * v + 1 if v > 2 else (6 if v == 2 else 3)) if cross else 2 * v + 1
+    ex_steps = np.sort(rng.choice(np.arange(1, S - 1), E, replace=False))
+    is_ex = np.zeros(S, np.uint8)
+    is_ex[ex_steps] = 1
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa: E731
+    shifts = f32(rng.normal(0, 0.001, E))
+    var_shift = f32(rng.normal(0.05, 0.005, (E, depth, v)))
+    var_div = f32(rng.uniform(0.005, 0.02, (E, depth, v)))
+    cut = f32(rng.uniform(0.002, 0.01, (E, depth)))
+    weights = rng.normal(0, 0.01, (E, depth, row))
+    weights[:, :, 0] += 0.004
+    weights = f32(weights)
+    r1 = f32(0.05 + 0.01 * rng.normal(size=(S, P)))
+    sv = f32(0.05 + 0.01 * rng.normal(size=(E, v, P)))
+    aux = f32(np.concatenate([[0.05, 1.0], np.full(S, 0.5), np.full(S, 0.5)]))
+    fd, iv = O.ls_multi_strategy_tables(shifts, var_shift, var_div, -cut, cut, weights, vpd, cross)
+    w1, w2, west, widx = O.cashflows_ee(cross, aux, fd, iv, is_ex, r1, sv)
+    d = lambda a, t=torch.float32: torch.as_tensor(np.ascontiguousarray(a)).to(t).cuda()  # noqa: E731
+    f1 = torch.zeros(S * P, dtype=torch.float32, device="cuda")
+    f2, est = torch.zeros_like(f1), torch.zeros_like(f1)
+    idx = torch.zeros(P, dtype=torch.int32, device="cuda")
+    print(f"cancellable swap cross={cross} depth={depth}", flush=True)
+    koo.cashFlowGeneratorEE_cancellable_swap_gpu(
+        cross, f1, f2, d(aux), d(fd), d(iv, torch.int32), P, S, E, d(is_ex, torch.uint8), d(r1),
+        None, None, d(sv), v, est, idx)
+    g1 = f1.cpu().numpy().reshape(S, P)
+    g2 = f2.cpu().numpy().reshape(S, P)
+    gest = est.cpu().numpy().reshape(S, P)
+    gidx = idx.cpu().numpy()
+    same = gidx == widx
+    print(f"cancellation step differs on {np.sum(~same)} of {P} paths")
+    assert np.mean(~same) < 1e-3
+    assert len(np.unique(widx)) > 1  # paths cancel at different dates (or run to the end)
+    assert np.allclose(g1[:, same], w1[:, same], rtol=1e-6, atol=1e-9)
+    assert np.allclose(g2[:, same], w2[:, same], rtol=1e-6, atol=1e-9)
+    for i in ex_steps:
+        m = same & (widx > i)
+        assert np.allclose(gest[i, m], west[i, m], rtol=1e-5, atol=1e-7)

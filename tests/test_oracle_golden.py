@@ -394,3 +394,84 @@ def test_multi_factor_pow2_bridge_matches_reference_gold(ordering, log2, F):
     assert np.array_equal(np.sort(re.reshape(-1)), np.sort(z.reshape(-1)))
     got = O.bridge_pow2(log2, re.reshape(S, P * F)).reshape(S, P, F)
     assert np.allclose(got, g["mout_" + key], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("cross", [0, 1])
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_cancellable_swap_second_pass_matches_reference_gold(cross, depth):
+    """Cancellable swap under the LS multi-regression strategy (one to three
+    regressions per date with symmetric cut bands, one to three variables per
+    date, quadratic basis with and without cross terms) against the gold
+    cashFlowGeneratorEE_gold + LSAMultiExerciseStrategy built from the same
+    packed tables."""
+    g = load("cashflows_ee_ref.npz")
+    key = f"{cross}_{depth}"
+    c = {name: g[f"in_{name}_{key}"] for name in ("vpd", "is_ex", "shifts", "var_shift", "var_div",
+                                                   "lower", "upper", "weights", "r1", "sv", "aux")}
+    fd, iv = O.ls_multi_strategy_tables(c["shifts"], c["var_shift"], c["var_div"], c["lower"],
+                                        c["upper"], c["weights"], c["vpd"], cross)
+    f1, f2, est, idx = O.cashflows_ee(cross, c["aux"], fd, iv, c["is_ex"], c["r1"], c["sv"])
+    assert np.allclose(f1, g["f1_" + key], rtol=0, atol=1e-15)
+    assert np.allclose(f2, g["f2_" + key], rtol=0, atol=1e-15)
+    S = c["r1"].shape[0]
+    # flows stop after the step processed last; a path that never cancels runs to the end
+    for p in range(len(idx)):
+        assert np.all(f1[idx[p]:, p] == 0) and np.all(f2[idx[p]:, p] == 0)
+    assert 1 <= idx.min() and idx.max() <= S
+
+
+def _make_golden():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(G, "make_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.parametrize("cross", [0, 1])
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_ls_multi_first_pass_matches_reference_gold(cross, depth):
+    """Multi-regression LS first pass (one to three nested regressions per
+    date, fraction selector with LSmulti_test.cu's parameters, normalised
+    quadratic basis with and without cross terms) against the gold
+    generateRegressionCoefficientsViaLSMulti_gold on the same seeded data:
+    price, every coefficient and every cut band."""
+    M = _make_golden()
+    g = load("ls_multi_ref.npz")
+    c = M.ls_multi_case(*M.LS_MULTI_CASES[(cross, depth)])
+    price, co, lo, up, me, sd = O.ls_multi_first_pass(c["bv"], c["ves"], c["cash"], c["ev"],
+                                                      c["num"], cross, depth, 10, *M.SELECTOR)
+    key = f"{cross}_{depth}"
+    assert price == pytest.approx(float(g["price_" + key]), rel=1e-12)
+    assert np.allclose(co, g["coeffs_" + key], rtol=0, atol=1e-12)
+    assert np.allclose(lo, g["lower_" + key], rtol=0, atol=1e-12)
+    assert np.allclose(up, g["upper_" + key], rtol=0, atol=1e-12)
+    if depth > 1:  # the nested regressions ran: narrower bands than the first date's
+        assert np.any(up[:, 1] != 0)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fraction_selector_band_holds_the_requested_fraction(seed):
+    """The selector's symmetric band keeps between lower_frac and upper_frac
+    of the pre-selected paths (continuous data, so bisection reaches the
+    window), and marks exactly the pre-selected paths inside it."""
+    rng = np.random.default_rng(seed)
+    n = 5000
+    net = rng.normal(0.01 * seed, 0.02, n)
+    pre = (rng.random(n) < 0.7).astype(np.int32)
+    sel, count, lo, up = O.select_fraction(net, pre, 0.25, 0.26, 2.5, 0.8)
+    n_pre = int(pre.sum())
+    assert lo == -up and up > 0
+    assert np.array_equal(sel.astype(bool), pre.astype(bool) & (net >= lo) & (net <= up))
+    assert count == sel.sum()
+    assert int(n_pre * 0.25) <= count <= int(n_pre * 0.26)
+
+
+def test_fraction_selector_keeps_everything_when_already_in_window():
+    """A pre-selected set already between the two fractions of itself is
+    impossible for fractions below one, so a window above one keeps every
+    pre-selected path (the cut is the reference's 10000 sentinel)."""
+    net = np.linspace(-1, 1, 101)
+    pre = np.ones(101, np.int32)
+    sel, count, lo, up = O.select_fraction(net, pre, 0.5, 1.5, 2.5, 0.8)
+    assert up == 10000.0 and count == 101

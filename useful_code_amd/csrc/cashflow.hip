@@ -157,6 +157,127 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Cancellable swap under a Longstaff-Schwartz multi-regression strategy: the
+// reference's second pass cashFlowGeneratorEE_kernel<Swap, earlyExerciseNull,
+// LSAMultiExerciseStrategyQuadratic(Cross)_gpu> (cashFlowGen_earlyExProduct_gpu.h:
+// 28-172, strategies :756-1049 and the quadratic one after it).  Per path and
+// step: at an exercise date the strategy's estimated continuation value (an
+// Andersen shift plus the regression of the date's normalised variables,
+// refined through up to `depth` regressions while the net value stays inside
+// the date's cut band) against the exercise value (zero: cancellation);
+// exercise ends the path's flows, otherwise the swap pays.  The strategy
+// tables are the reference's packed arrays; every lane of a wavefront is at
+// the same exercise date and depth, so they are read with uniform addresses
+// (scalar loads) instead of being staged in shared memory.
+//   int data: depth, max variables, row size, variables per date [E]
+//   float data: Andersen shifts [E], variable shifts [E][depth][maxv],
+//     divisors [E][depth][maxv], lower cuts [E][depth], upper cuts [E][depth],
+//     weights [E][depth][row size]
+// ---------------------------------------------------------------------------
+template <bool CROSS>
+__device__ __forceinline__ bool ls_multi_strategy(const float* __restrict__ fd,
+                                                  const int* __restrict__ id, int E, int ex,
+                                                  const float* __restrict__ sv, int sv_max,
+                                                  size_t P, int p, float ex_value, float& value) {
+  const int depth_max = id[0], maxv = id[1], row = id[2], nv = id[3 + ex];
+  const float* shifts = fd + E + static_cast<size_t>(ex) * depth_max * maxv;
+  const float* divs = shifts + static_cast<size_t>(E) * depth_max * maxv;
+  const float* lower = fd + E + 2 * static_cast<size_t>(E) * depth_max * maxv +
+                       static_cast<size_t>(ex) * depth_max;
+  const float* upper = lower + static_cast<size_t>(E) * depth_max;
+  const float* weights0 = fd + E + 2 * static_cast<size_t>(E) * depth_max * maxv +
+                          2 * static_cast<size_t>(E) * depth_max +
+                          static_cast<size_t>(ex) * depth_max * row;
+  const float* vars = sv + static_cast<size_t>(ex) * P * sv_max + p;
+  for (int depth = 0;;) {
+    value = fd[ex];
+    const float* w = weights0 + depth * row;
+    const float* sh = shifts + maxv * depth;
+    const float* dv = divs + maxv * depth;
+    value += *w++;
+    auto var = [&](int i) { return (vars[static_cast<size_t>(i) * P] - sh[i]) / dv[i]; };
+    if (!CROSS) {
+      for (int i = 0; i < nv; ++i) {
+        const float v = var(i);
+        value += (*w++) * v;
+        value += (*w++) * v * v;
+      }
+    } else if (nv == 1) {
+      const float v = var(0);
+      value += (*w++) * v;
+      value += (*w++) * v * v;
+    } else if (nv == 2) {
+      const float x = var(0), y = var(1);
+      value += (*w++) * x;
+      value += (*w++) * y;
+      value += (*w++) * x * x;
+      value += (*w++) * x * y;
+      value += (*w++) * y * y;
+    } else {
+      for (int i = 0; i < nv; ++i) value += (*w++) * var(i);
+      for (int i = 0; i < nv; ++i) value += (*w++) * var(i) * var((i + 1) % nv);
+      for (int i = 0; i < nv; ++i) {
+        const float v = var(i);
+        value += (*w++) * v * v;
+      }
+    }
+    ++depth;
+    if (depth == depth_max) break;
+    if (value - ex_value < lower[depth - 1]) {
+      value = lower[depth - 1];
+      return true;
+    }
+    if (value - ex_value > upper[depth - 1]) {
+      value = upper[depth - 1];
+      return false;
+    }
+  }
+  return value < ex_value;
+}
+
+template <bool CROSS>
+__global__ void __launch_bounds__(256)
+    k_cashflows_ee(float* __restrict__ f1, float* __restrict__ f2, const float* __restrict__ aux,
+                   const float* __restrict__ fd, const int* __restrict__ id, int paths,
+                   int steps, int n_ex, const unsigned char* __restrict__ is_ex,
+                   const float* __restrict__ r1, const float* __restrict__ r2,
+                   const float* __restrict__ r3, const float* __restrict__ sv, int sv_max,
+                   float* __restrict__ est, int* __restrict__ ex_index) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const size_t P = static_cast<size_t>(paths);
+  SwapProduct swap(aux, steps);
+  bool done = false;
+  int i = 0, ex = 0;
+  for (; i < steps && !done; ++i) {
+    const size_t o = static_cast<size_t>(i) * P + p;
+    float a = 0.0f, b = 0.0f;
+    const float x1 = r1[o], x2 = r2 ? r2[o] : 0.0f, x3 = r3 ? r3[o] : 0.0f;
+    bool exercise = false;
+    if (is_ex[i]) {
+      const float ev = 0.0f;  // earlyExerciseNull: cancellation pays nothing
+      float cont = -1.0e20f;
+      exercise = ls_multi_strategy<CROSS>(fd, id, n_ex, ex, sv, sv_max, P, p, ev, cont);
+      est[o] = cont;
+      if (exercise) {
+        a = ev;
+        done = true;
+      }
+      ++ex;
+    }
+    if (!exercise) done = swap.flows(a, b, x1, x2, x3);
+    f1[o] = a;
+    f2[o] = b;
+  }
+  ex_index[p] = i;
+  for (; i < steps; ++i) {
+    const size_t o = static_cast<size_t>(i) * P + p;
+    f1[o] = 0.0f;
+    f2[o] = 0.0f;
+  }
+}
+
 template <class Product>
 int launch_cashflows(const char* who, float* f1, float* f2, float* aux, int paths, int steps,
                      float* r1, float* r2, float* r3, void* stream) {
@@ -250,4 +371,47 @@ extern "C" int AggregateFlows_gpu(float* aggregatedFlows, int totalPaths, int nu
                      aggregatedFlows, totalPaths, numberExerciseDates, genFlows, batchPaths,
                      offSetForOutput, numberSteps, precedingIndex, start);
   return check_launch("AggregateFlows_gpu");
+}
+
+// cashFlowGeneratorEE_gpu<Swap, earlyExerciseNull, LSAMultiExerciseStrategyQuadratic(Cross)_gpu>
+// (cashFlowGen_earlyExProduct_gpu.h:323-497; cashFlowGen_earlyEx_product_main.h,
+// the instantiations of LSmulti_test.cu:2185-2236 and bermudanIRDPricer.cu:1391-1440):
+// the cancellable swap's second pass.  use_cross_terms selects the cross
+// strategy; the exercise-value tables (earlyExerciseNull) are unused, as in
+// the reference; isExerciseDate is one byte per step (the reference's bool
+// array).
+extern "C" int cashFlowGeneratorEE_cancellable_swap_gpu(
+    int use_cross_terms, float* genFlows1, float* genFlows2, const float* productData,
+    const float* /*exerciseValueDataFloat*/, const int* /*exerciseValueDataInt*/,
+    const float* exerciseStrategyDataFloat, int exerciseStrategyDataFloat_size,
+    const int* exerciseStrategyDataInt, int exerciseStrategyDataInt_size, int paths,
+    int numberSteps, int numberExerciseDates, const unsigned char* isExerciseDate,
+    const float* rates1, const float* rates2, const float* rates3, const float* strategyVariables,
+    int strategyVariables_maxSize, const float* /*forwards*/, const float* /*discRatios*/,
+    float* estContinuationValues, int* exerciseIndices, void* stream) {
+  const char* who = "cashFlowGeneratorEE_cancellable_swap_gpu";
+  if (paths < 0 || numberSteps < 0 || numberExerciseDates < 1 ||
+      exerciseStrategyDataInt_size < 3 + numberExerciseDates || exerciseStrategyDataFloat_size < 1 ||
+      strategyVariables_maxSize < 1)
+    return set_error(KOO_ERR_INVALID, "%s: bad sizes", who);
+  if (paths == 0 || numberSteps == 0) return KOO_OK;
+  if (!genFlows1 || !genFlows2 || !productData || !exerciseStrategyDataFloat ||
+      !exerciseStrategyDataInt || !isExerciseDate || !rates1 || !strategyVariables ||
+      !estContinuationValues || !exerciseIndices)
+    return set_error(KOO_ERR_INVALID, "%s: null pointer", who);
+  hipStream_t st = as_stream(stream);
+  const dim3 g((paths + 255) / 256);
+  if (use_cross_terms)
+    hipLaunchKernelGGL(k_cashflows_ee<true>, g, dim3(256), 0, st, genFlows1, genFlows2,
+                       productData, exerciseStrategyDataFloat, exerciseStrategyDataInt, paths,
+                       numberSteps, numberExerciseDates, isExerciseDate, rates1, rates2, rates3,
+                       strategyVariables, strategyVariables_maxSize, estContinuationValues,
+                       exerciseIndices);
+  else
+    hipLaunchKernelGGL(k_cashflows_ee<false>, g, dim3(256), 0, st, genFlows1, genFlows2,
+                       productData, exerciseStrategyDataFloat, exerciseStrategyDataInt, paths,
+                       numberSteps, numberExerciseDates, isExerciseDate, rates1, rates2, rates3,
+                       strategyVariables, strategyVariables_maxSize, estContinuationValues,
+                       exerciseIndices);
+  return check_launch(who);
 }

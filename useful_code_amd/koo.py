@@ -125,6 +125,8 @@ _SIGNATURES = {
     "PointwiseProductSymmetricUpperTriangular_gpu": (_i, [_i, _i, _i, _i, _p, _p, _p]),
     "PointwiseProductMultipleFirst_gpu": (_i, [_i, _i, _i, _p, _p, _p, _i, _p]),
     "bermudan_paths_gpu": (_i, [_i, _i, _i] + [_p] * 5 + [_i, _p, _f] + [_p] * 4),
+    "cashFlowGeneratorEE_cancellable_swap_gpu": (_i, [_i] + [_p] * 6 + [_i, _p, _i, _i, _i, _i]
+                                                 + [_p] * 5 + [_i] + [_p] * 5),
     "bermudan_portfolio_descriptor_bytes": (_sz, [_i]),
     "bermudan_ls_portfolio_descriptor_bytes": (_sz, [_i]),
     "bermudan_ls_portfolio_prepare": (_i, [_i, _p, _p, _p]),
@@ -668,6 +670,15 @@ def bermudan_frozen_fwd(paths, rates, factors, k, variates, f0, taus, disp, A, s
          ptr(disp), ptr(A), float(strike), len(ex), ex.ctypes.data, ptr(coeffs_dev), ptr(sh),
          ptr(sc), int(itm_only), ptr(checkpoints), ptr(path_buffers), ptr(partials),
          ptr(price_out), stream_handle(stream))
+
+
+def cashFlowGeneratorEE_cancellable_swap_gpu(cross, f1, f2, aux, strat_float, strat_int, paths,
+                                             steps, n_ex, is_ex, r1, r2, r3, strat_vars,
+                                             strat_max_size, est, ex_index, stream=None):
+    call("cashFlowGeneratorEE_cancellable_swap_gpu", int(cross), ptr(f1), ptr(f2), ptr(aux), None,
+         None, ptr(strat_float), strat_float.numel(), ptr(strat_int), strat_int.numel(), paths,
+         steps, n_ex, ptr(is_ex), ptr(r1), ptr(r2), ptr(r3), ptr(strat_vars), strat_max_size,
+         None, None, ptr(est), ptr(ex_index), stream_handle(stream))
 
 
 def bermudan_portfolio_prepare(products, descriptors):
