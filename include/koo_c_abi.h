@@ -547,6 +547,43 @@ int ls_first_pass_gpu(int n_ex, const float* basis_variables, const int* vars_ea
                       const float* var_shift, const float* var_scale, void* workspace,
                       double* coefficients, double* price_out, void* stream);
 
+/* generateMultiRegressionCoefficientsViaLS_gpu_flexisolve (LS_main_cula.h:
+ * 565-838; the caller in the reference's test is
+ * generateRegressionCoefficientsViaLSMultiquadratic_flexi_main,
+ * LSmulti_test.cu:1690-1712) with RegressionSelectorFraction(lower_frac,
+ * upper_frac, sd_guess, multiplier) (selector 0, RegressionSelectorConcrete.cu:
+ * 102-233, as its gold RegressionSelector_gold_Fraction) or
+ * RegressionSelectorStandardDeviations(sd_guess) (selector 1, :19-90; the
+ * band is open and sd_guess standard deviations wide; lower_frac, upper_frac
+ * and multiplier are then unused): the multi-regression LS first pass, as the gold
+ * generateRegressionCoefficientsViaLSMultiQuadratic_gold computes it
+ * (gold/MultiLS_regression.cpp:20-420).  Per exercise date, up to
+ * max_regressions nested regressions: the first over every path, each later
+ * one over the paths whose net continuation value lies in the band the
+ * selector chose on the previous one ([-c, c] holding between lower_frac and
+ * upper_frac of them, or (-c, c) with c = sd_guess standard deviations),
+ * while more than lower_path_cutoff paths remain.
+ * Variables are standardised per regression (normalise != 0) or raw; the
+ * normal equations are solved like the gold's SVD with tolerance 1e-8.
+ * FP64 throughout.  Inputs as ls_first_pass_gpu (basis_variables
+ * (E, max_vars, P), cash_flows / exercise_values / numeraires (E, P), all
+ * device FP32; vars_each_step HOST).  Outputs (device double, zero where a
+ * regression was not reached): coefficients (E, max_regressions,
+ * basis_count(max_vars)), lower_cuts / upper_cuts (E, max_regressions),
+ * means / sds (E, max_regressions, max_vars; may be NULL), price_out (the
+ * first-pass price).  workspace: ls_multi_workspace(paths) bytes, 256-byte
+ * aligned.  The reference returns the price and fills float coefficients;
+ * here everything is a device output, and the pass never synchronises. */
+int ls_multi_workspace(int paths, size_t* bytes);
+int ls_multi_first_pass_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,
+                            int max_vars, const float* cash_flows, const float* exercise_values,
+                            const float* numeraires, int paths, int use_cross, int normalise,
+                            int max_regressions, int lower_path_cutoff, int selector,
+                            double lower_frac, double upper_frac, double sd_guess,
+                            double multiplier, void* workspace, double* coefficients,
+                            double* lower_cuts, double* upper_cuts, double* means, double* sds,
+                            double* price_out, void* stream);
+
 /* New: price under frozen coefficients (SecondPassPriceLSAMultiUsingAggregatedFlows_gold,
  * gold/MultiLS_regression.h:237-289, depth 0). */
 int ls_second_pass_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,

@@ -89,8 +89,12 @@ def _declare(L):
         "orc_bridge_transform": (None, [_c_int, _c_int, _c_int, _i32p, _f64p, _f64p]),
         "orc_bridge_pow2": (None, [_c_int, _c_int, _f64p, _f64p]),
         "orc_ls_multi_first_pass": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p,
-                                             _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_dbl,
+                                             _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                             _c_dbl, _c_dbl,
                                              _c_dbl, _c_dbl, _f64p, _f64p, _f64p, _f64p, _f64p]),
+        "orc_sym_solve_truncated": (_c_int, [_c_int, _f64p, _f64p, _c_dbl, _f64p]),
+        "orc_select_sds": (ctypes.c_long, [_f64p, _i32p, _i32p, _c_int, ctypes.c_long, _c_dbl,
+                                           _f64p, _f64p]),
         "orc_select_fraction": (ctypes.c_long, [_f64p, _i32p, _i32p, _c_int, ctypes.c_long,
                                                 _c_dbl, _c_dbl, _c_dbl, _c_dbl, _f64p, _f64p]),
         "orc_cashflows_ee": (None, [_c_int, _f64p, _f64p, _f64p, _f64p, _i32p, _c_int, _c_int,
@@ -168,7 +172,7 @@ def _declare_ref(R):
         "ref_bridge_allocate_diagonal": (None, [_c_int, _c_int, _i32p]),
         "ref_bridge_pow2": (None, [_c_int, _c_int, _f64p, _f64p]),
         "ref_ls_multi": (_c_dbl, [_c_int, _f64p, _i32p, _c_int, _f64p, _f64p, _f64p, _c_int,
-                                  _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _f64p,
+                                  _c_int, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _f64p,
                                   _f64p, _f64p]),
         "ref_cashflows_ee": (None, [_c_int, _f64p, _f64p, _f64p, _c_int, _f64p, _i32p, _c_int,
                                     _c_int, _c_int, ctypes.c_void_p, _f64p, _f64p, _c_int]),
@@ -336,10 +340,13 @@ def lmm_pc_evolve(f0, taus, variates, A, disp, alive, transposed=True, initial_d
 
 
 def ls_multi_first_pass(bv, vars_each_step, cash, ev, num, use_cross, max_regressions,
-                        lower_path_cutoff, lower_frac, upper_frac, sd_guess, multiplier):
-    """Multi-regression first pass with the fraction selector (normalised
-    variables).  bv (E, maxVars, P); returns price, coefficients
-    (E, R, maxd), lower and upper cuts (E, R), means and sds (E, R, maxVars)."""
+                        lower_path_cutoff, lower_frac, upper_frac, sd_guess, multiplier,
+                        normalise=True, selector=0):
+    """Multi-regression first pass with the fraction selector (selector 0)
+    or the standard-deviation selector (1: sd_guess standard deviations),
+    variables normalised per regression or raw.  bv (E, maxVars, P); returns price,
+    coefficients (E, R, maxd), lower and upper cuts (E, R), means and sds
+    (E, R, maxVars)."""
     bv = np.ascontiguousarray(bv, np.float64)
     E, mv, P = bv.shape
     R = int(max_regressions)
@@ -350,8 +357,9 @@ def ls_multi_first_pass(bv, vars_each_step, cash, ev, num, use_cross, max_regres
     f = lambda a: np.ascontiguousarray(a, np.float64).reshape(-1)  # noqa: E731
     price = lib().orc_ls_multi_first_pass(
         E, bv.reshape(-1), np.ascontiguousarray(vars_each_step, np.int32), mv, f(cash), f(ev),
-        f(num), P, int(use_cross), R, int(lower_path_cutoff), float(lower_frac), float(upper_frac),
-        float(sd_guess), float(multiplier), co, lo, up, me, sd)
+        f(num), P, int(use_cross), int(normalise), R, int(lower_path_cutoff), int(selector),
+        float(lower_frac), float(upper_frac), float(sd_guess), float(multiplier), co, lo, up, me,
+        sd)
     return (price, co.reshape(E, R, maxd), lo.reshape(E, R), up.reshape(E, R),
             me.reshape(E, R, mv), sd.reshape(E, R, mv))
 
@@ -365,6 +373,16 @@ def select_fraction(net, pre, lower_frac, upper_frac, sd_guess, multiplier):
     lo, up = np.zeros(1), np.zeros(1)
     c = lib().orc_select_fraction(net, pre, sel, len(net), int(pre.sum()), float(lower_frac),
                                   float(upper_frac), float(sd_guess), float(multiplier), lo, up)
+    return sel, int(c), float(lo[0]), float(up[0])
+
+
+def select_sds(net, pre, n_sds):
+    """The standard-deviation selector; returns (mask, count, lower, upper)."""
+    net = np.ascontiguousarray(net, np.float64)
+    pre = np.ascontiguousarray(pre, np.int32)
+    sel = np.zeros(len(net), np.int32)
+    lo, up = np.zeros(1), np.zeros(1)
+    c = lib().orc_select_sds(net, pre, sel, len(net), int(pre.sum()), float(n_sds), lo, up)
     return sel, int(c), float(lo[0]), float(up[0])
 
 
@@ -499,6 +517,15 @@ def cholesky_solve(A, b):
     x = np.zeros(len(b))
     st = lib().orc_cholesky_solve(len(b), A.reshape(-1), b, x)
     return x if st == 0 else None
+
+
+def sym_solve_truncated(A, b, tol):
+    """Symmetric solve with eigenvalues below tol * the largest dropped."""
+    A = np.ascontiguousarray(A, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(len(b))
+    lib().orc_sym_solve_truncated(len(b), A.reshape(-1), b, float(tol), x)
+    return x
 
 
 def _ls_args(bv, vars_each_step, cash, ev, num):

@@ -1266,6 +1266,70 @@ int orc_cholesky_solve(int n, const double *A, const double *b, double *x) {
   return 0;
 }
 
+/* Solve of a symmetric n x n system by its eigendecomposition with small
+ * eigenvalues dropped: x = sum over |l_i| >= tol * max|l| of
+ * (v_i . b / l_i) v_i.  For a symmetric matrix this is the truncated SVD
+ * solve of SVDSquareMatrix::SVDSolve (gold/math/svd_gold.cpp:560-596: keep
+ * singular value s_i when s_i / s_0 >= tolerance), since the singular values
+ * are the |l_i| and the singular vectors the v_i up to sign.  The
+ * eigendecomposition is cyclic Jacobi (Golub and Van Loan, Matrix
+ * Computations, 4th ed., 8.5.2, with the stable rotation of 8.5.1) run until
+ * the off-diagonal mass is below 1e-30 of the total.  Returns 0. */
+int orc_sym_solve_truncated(int n, const double *A, const double *b, double tol, double *x) {
+  double a[64 * 64], v[64 * 64];
+  if (n > 64 || n < 1) return -1;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      a[i * 64 + j] = A[i * n + j];
+      v[i * 64 + j] = i == j ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double q = a[i * 64 + j] * a[i * 64 + j];
+        tot += q;
+        if (i != j) off += q;
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[p * 64 + q];
+        if (apq == 0.0) continue;
+        const double tau = (a[q * 64 + q] - a[p * 64 + p]) / (2.0 * apq);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+        for (int k = 0; k < n; ++k) {  /* A <- A J (columns p, q) */
+          const double akp = a[k * 64 + p], akq = a[k * 64 + q];
+          a[k * 64 + p] = c * akp - sn * akq;
+          a[k * 64 + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  /* A <- J^T A (rows p, q) */
+          const double apk = a[p * 64 + k], aqk = a[q * 64 + k];
+          a[p * 64 + k] = c * apk - sn * aqk;
+          a[q * 64 + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {  /* V <- V J */
+          const double vkp = v[k * 64 + p], vkq = v[k * 64 + q];
+          v[k * 64 + p] = c * vkp - sn * vkq;
+          v[k * 64 + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  double lmax = 0.0;
+  for (int i = 0; i < n; ++i) lmax = fmax(lmax, fabs(a[i * 64 + i]));
+  for (int i = 0; i < n; ++i) x[i] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double l = a[i * 64 + i];
+    if (!(lmax > 0.0) || fabs(l) < tol * lmax) continue;
+    double e = 0.0;
+    for (int k = 0; k < n; ++k) e += v[k * 64 + i] * b[k];
+    e /= l;
+    for (int k = 0; k < n; ++k) x[k] += e * v[k * 64 + i];
+  }
+  return 0;
+}
+
 /* Backward induction with one regression per exercise date (depth 0 of
  * generateRegressionCoefficientsViaLSMulti_gold, gold/MultiLS_regression.cpp:
  * 23-430, without variable normalisation): at date e the regression matrix is
@@ -1338,7 +1402,15 @@ double orc_ls_first_pass(int n_ex, const double *bv, const int *vars_each_step, 
  * pre-selected paths holding between lower_frac and upper_frac of them,
  * found from c = sd_guess * sd by growing (c /= multiplier), shrinking
  * (c *= multiplier) and bisection; marks the paths inside, returns their
- * count and the cuts. */
+ * count and the cuts.  Each of the three loops stops after
+ * ORC_SELECT_MAX_STEPS steps (the gold's loops are unbounded and never end
+ * when the net values have no spread, e.g. a zero standard deviation keeps
+ * c = 0 under the growth step); where the gold ends, it ends within far
+ * fewer steps, so the cap changes nothing there. */
+#define ORC_SELECT_MAX_STEPS 1000
+/* the gold's SVD tolerance for the LS normal equations
+ * (gold/MultiLS_regression.cpp:172) */
+#define ORC_SVD_TOLERANCE 1e-8
 static long orc_count_band(const double *x, const int *pre, int n, double lo, double hi) {
   long c = 0;
   for (int p = 0; p < n; ++p)
@@ -1363,7 +1435,8 @@ long orc_select_fraction(const double *net, const int *pre, int *sel, int paths,
     const double mean = sum / n_pre, sd = sqrt(sumsq / n_pre - mean * mean);
     double guess = sd * sd_guess;
     long count = orc_count_band(net, pre, paths, -guess, guess);
-    while (count < lowerT) {
+    int it = 0;
+    while (count < lowerT && it++ < ORC_SELECT_MAX_STEPS) {
       guess /= multiplier;
       count = orc_count_band(net, pre, paths, -guess, guess);
     }
@@ -1372,7 +1445,8 @@ long orc_select_fraction(const double *net, const int *pre, int *sel, int paths,
     } else {
       double low = guess * multiplier;
       count = orc_count_band(net, pre, paths, -low, low);
-      while (count > upperT) {
+      it = 0;
+      while (count > upperT && it++ < ORC_SELECT_MAX_STEPS) {
         low *= multiplier;
         count = orc_count_band(net, pre, paths, -low, low);
       }
@@ -1382,7 +1456,9 @@ long orc_select_fraction(const double *net, const int *pre, int *sel, int paths,
         double high = guess;
         guess = 0.5 * (low + high);
         count = orc_count_band(net, pre, paths, -guess, guess);
-        while ((count < lowerT || count > upperT) && (high - low) > 1e-6) {
+        it = 0;
+        while ((count < lowerT || count > upperT) && (high - low) > 1e-6 &&
+               it++ < ORC_SELECT_MAX_STEPS) {
           if (count < lowerT)
             low = guess;
           else
@@ -1404,14 +1480,40 @@ long orc_select_fraction(const double *net, const int *pre, int *sel, int paths,
   return c;
 }
 
-/* The multi-regression first pass with normalised variables,
- * generateRegressionCoefficientsViaLSMulti_gold (gold/MultiLS_regression.cpp:
- * 20-420, normalise = true; GPU LS_main_cula.h:565-838).  Per exercise date
- * (last first): regression of cash + deflated next value on the quadratic
- * basis of the variables standardised by their mean and standard deviation
- * over the regression paths (depth 0: every path); net continuation values;
- * the selector's band (RegressionSelector_gold_Fraction) picks the paths of
- * the next regression, up to max_regressions, while more than
+/* RegressionSelectorStandardDeviations::Select_gold (RegressionSelectorConcrete.cu:
+ * 50-90; the GPU form :19-46): cut = n_sds * sd of the net values over the
+ * pre-selected paths (sd = sqrt(E[x^2] - mean^2)), keeping the paths strictly
+ * inside (-cut, cut).  The reference rounds the cut to FP32; here it stays
+ * FP64 like every other quantity of the pass.  Returns the count. */
+long orc_select_sds(const double *net, const int *pre, int *sel, int paths, long n_pre,
+                    double n_sds, double *lower_cut, double *upper_cut) {
+  double sum = 0.0, sumsq = 0.0;
+  for (int p = 0; p < paths; ++p)
+    if (pre[p]) {
+      sum += net[p];
+      sumsq += net[p] * net[p];
+    }
+  const double mean = sum / n_pre, sd = sqrt(sumsq / n_pre - mean * mean);
+  const double cut = n_sds * sd;
+  *lower_cut = -cut;
+  *upper_cut = cut;
+  long c = 0;
+  for (int p = 0; p < paths; ++p) {
+    sel[p] = pre[p] && -cut < net[p] && net[p] < cut;
+    c += sel[p];
+  }
+  return c;
+}
+
+/* The multi-regression first pass, generateRegressionCoefficientsViaLSMulti_gold
+ * (gold/MultiLS_regression.cpp:20-420; GPU LS_main_cula.h:565-838).  Per
+ * exercise date (last first): regression of cash + deflated next value on the
+ * quadratic basis of the variables -- standardised by their mean and standard
+ * deviation over the regression paths (depth 0: every path) when normalise is
+ * set, raw otherwise (means 0, sds 1) -- ; net continuation values;
+ * the selector's band (selector 0: RegressionSelector_gold_Fraction; 1:
+ * RegressionSelectorStandardDeviations with sd_guess standard deviations)
+ * picks the paths of the next regression, up to max_regressions, while more than
  * lower_path_cutoff paths remain (cuts 0 otherwise).  Then each path walks
  * the depths while its net value lies strictly inside the previous depth's
  * band (updateDeflatedNextStepValuesMulti_gold, gold/MultiLS_regression.h:
@@ -1421,8 +1523,9 @@ long orc_select_fraction(const double *net, const int *pre, int *sel, int paths,
  * was not reached.  Returns the first-pass price. */
 double orc_ls_multi_first_pass(int n_ex, const double *bv, const int *vars_each_step,
                                int max_vars, const double *cash, const double *ev,
-                               const double *num, int paths, int use_cross, int max_regressions,
-                               int lower_path_cutoff, double lower_frac, double upper_frac,
+                               const double *num, int paths, int use_cross, int normalise,
+                               int max_regressions, int lower_path_cutoff, int selector,
+                               double lower_frac, double upper_frac,
                                double sd_guess, double multiplier, double *coeffs, double *lower,
                                double *upper, double *means, double *sds) {
   const int maxd = orc_basis_count(max_vars, use_cross), R = max_regressions;
@@ -1441,6 +1544,11 @@ double orc_ls_multi_first_pass(int n_ex, const double *bv, const int *vars_each_
       double *m = means + ((size_t)e * R + depth) * max_vars;
       double *sd = sds + ((size_t)e * R + depth) * max_vars;
       for (int i = 0; i < v; ++i) {
+        if (!normalise) {
+          m[i] = 0.0;
+          sd[i] = 1.0;
+          continue;
+        }
         double a = 0.0, b = 0.0;
         for (int p = 0; p < paths; ++p)
           if (pre[p]) {
@@ -1471,8 +1579,7 @@ double orc_ls_multi_first_pass(int n_ex, const double *bv, const int *vars_each_
         for (int j = 0; j < i; ++j) M[j * nd + i] = M[i * nd + j] = M[i * nd + j] / n_pre;
       }
       double *beta = coeffs + ((size_t)e * R + depth) * maxd;
-      if (orc_cholesky_solve(nd, M, t, beta) != 0)
-        for (int i = 0; i < nd; ++i) beta[i] = 0.0;
+      orc_sym_solve_truncated(nd, M, t, ORC_SVD_TOLERANCE, beta);
       for (int p = 0; p < paths; ++p) {
         if (!pre[p]) continue;
         for (int i = 0; i < v; ++i) x[i] = (bv[((size_t)e * max_vars + i) * paths + p] - m[i]) / sd[i];
@@ -1482,8 +1589,10 @@ double orc_ls_multi_first_pass(int n_ex, const double *bv, const int *vars_each_
         net[p] = est - ev[(size_t)e * paths + p];
       }
       double lo, hi;
-      const long after = orc_select_fraction(net, pre, sel, paths, n_pre, lower_frac, upper_frac,
-                                             sd_guess, multiplier, &lo, &hi);
+      const long after =
+          selector == 1 ? orc_select_sds(net, pre, sel, paths, n_pre, sd_guess, &lo, &hi)
+                        : orc_select_fraction(net, pre, sel, paths, n_pre, lower_frac, upper_frac,
+                                              sd_guess, multiplier, &lo, &hi);
       if (after > lower_path_cutoff) {
         lower[(size_t)e * R + depth] = lo;
         upper[(size_t)e * R + depth] = hi;

@@ -128,6 +128,7 @@ void orc_caplet_pipeline(const orc_caplet_model *m, const uint32_t *ints /* [D][
 int orc_basis_count(int n_vars, int use_cross);
 void orc_basis_functions(const double *vars, int n_vars, int use_cross, double *out);
 int orc_cholesky_solve(int n, const double *A, const double *b, double *x);
+int orc_sym_solve_truncated(int n, const double *A, const double *b, double tol, double *x);
 double orc_ls_first_pass(int n_ex, const double *basis_vars /* (E,maxVars,P) */,
                          const int *vars_each_step, int max_vars, const double *cash /* (E,P) */,
                          const double *ex_values /* (E,P) */, const double *numeraires /* (E,P) */,
@@ -135,10 +136,13 @@ double orc_ls_first_pass(int n_ex, const double *basis_vars /* (E,maxVars,P) */,
 long orc_select_fraction(const double *net, const int *pre, int *sel, int paths, long n_pre,
                          double lower_frac, double upper_frac, double sd_guess, double multiplier,
                          double *lower_cut, double *upper_cut);
+long orc_select_sds(const double *net, const int *pre, int *sel, int paths, long n_pre,
+                    double n_sds, double *lower_cut, double *upper_cut);
 double orc_ls_multi_first_pass(int n_ex, const double *bv, const int *vars_each_step,
                                int max_vars, const double *cash, const double *ev,
-                               const double *num, int paths, int use_cross, int max_regressions,
-                               int lower_path_cutoff, double lower_frac, double upper_frac,
+                               const double *num, int paths, int use_cross, int normalise,
+                               int max_regressions, int lower_path_cutoff, int selector,
+                               double lower_frac, double upper_frac,
                                double sd_guess, double multiplier, double *coeffs, double *lower,
                                double *upper, double *means, double *sds);
 double orc_ls_second_pass(int n_ex, const double *basis_vars, const int *vars_each_step,

@@ -329,10 +329,10 @@ double ref_ls_multi_depth0(int n_ex, const double* basis_vars, const int* vars_e
 }
 
 // The same with extra regressions and the fraction selector
-// (gold/MultiLS_regression.cpp:20-420, normalise = true).
+// (gold/MultiLS_regression.cpp:20-420), variables normalised or not.
 double ref_ls_multi(int n_ex, const double* basis_vars, const int* vars_each_step, int max_vars,
                     const double* cash, const double* ex_values, const double* numeraires, int paths,
-                    int use_cross, int max_regressions, int lower_path_cutoff, double lower_frac,
+                    int use_cross, int normalise, int max_regressions, int lower_path_cutoff, double lower_frac,
                     double upper_frac, double sd_guess, double multiplier, double* coeffs_out,
                     double* lower_out, double* upper_out) {
   std::vector<double> products, targets;
@@ -354,8 +354,8 @@ double ref_ls_multi(int n_ex, const double* basis_vars, const int* vars_each_ste
   std::vector<int> ves(vars_each_step, vars_each_step + n_ex);
   RegressionSelector_gold_Fraction selector(lower_frac, upper_frac, sd_guess, multiplier);
   double price = generateRegressionCoefficientsViaLSMultiQuadratic_gold(
-      n_ex, products, targets, coeff_cube, lower_mat, upper_mat, means_cube, sds_cube, true, bv_cube, ves,
-      max_vars, cf_mat, ev_mat, nm_mat, paths, R - 1, lower_path_cutoff, selector, use_cross != 0);
+      n_ex, products, targets, coeff_cube, lower_mat, upper_mat, means_cube, sds_cube, normalise != 0, bv_cube,
+      ves, max_vars, cf_mat, ev_mat, nm_mat, paths, R - 1, lower_path_cutoff, selector, use_cross != 0);
   std::memcpy(coeffs_out, &coeffs[0], sizeof(double) * coeffs.size());
   std::memcpy(lower_out, &lower[0], sizeof(double) * lower.size());
   std::memcpy(upper_out, &upper[0], sizeof(double) * upper.size());

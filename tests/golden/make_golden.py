@@ -206,17 +206,17 @@ LS_MULTI_CASES = {(0, 1): (7, 6, 3000, 2, 1), (0, 2): (14, 6, 3000, 2, 2),
 
 def ls_multi(R):
     out = {}
-    for cross in (0, 1):
-        for depth in (1, 2, 3):
-            c = ls_multi_case(*LS_MULTI_CASES[(cross, depth)])
+    for (cross, depth), args in LS_MULTI_CASES.items():
+        for norm in (0, 1):
+            c = ls_multi_case(*args)
             E, mv, P = c["bv"].shape
             maxd = (1 + 3 * mv if mv > 2 else (6 if mv == 2 else 3)) if cross else 1 + 2 * mv
             co, lo, up = np.zeros(E * depth * maxd), np.zeros(E * depth), np.zeros(E * depth)
             price = R.ref_ls_multi(E, c["bv"].reshape(-1).copy(), c["ves"], mv,
                                    c["cash"].reshape(-1).copy(), c["ev"].reshape(-1).copy(),
-                                   c["num"].reshape(-1).copy(), P, cross, depth, 10, *SELECTOR,
-                                   co, lo, up)
-            key = f"{cross}_{depth}"
+                                   c["num"].reshape(-1).copy(), P, cross, norm, depth, 10,
+                                   *SELECTOR, co, lo, up)
+            key = f"{cross}_{depth}_{norm}"
             out["price_" + key] = np.array(price)
             out["coeffs_" + key] = co.reshape(E, depth, maxd)
             out["lower_" + key] = lo.reshape(E, depth)

@@ -75,10 +75,32 @@ def _status(name, *args):
     ("bermudan_portfolio_fwd", (None, None, None)),
     ("bermudan_portfolio_bwd", (None, None, None)),
     ("oneStepUpdateLS_multi_gpu", (16, 0, None, 6, 3, 1) + (None,) * 6 + (0.0, None, None, None)),
+    ("ls_multi_first_pass_gpu", (0, None, None, 2, None, None, None, 16, 1, 1, 2, 10, 0, 0.25,
+                                 0.26, 2.5, 0.8) + (None,) * 8),
 ])
 def test_invalid_arguments_are_rejected(call, args):
     st, msg = _status(call, *args)
     assert st == -1 and call.split("_")[0] in msg
+
+
+@pytest.mark.parametrize("field,value", [
+    ("max_regressions", 9), ("max_regressions", 0), ("lower_path_cutoff", -1),
+    ("multiplier", 1.0), ("multiplier", 0.0), ("upper_frac", 0.2), ("sd_guess", 0.0),
+    ("max_vars", 5), ("paths", 0), ("workspace", None), ("selector", 2),
+])
+def test_multi_ls_rejects_bad_selector_and_sizes(field, value):
+    """ls_multi_first_pass_gpu checks the selector parameters (a multiplier
+    outside (0, 1) would make the band search diverge), the depth, the sizes
+    and the pointers before any device work."""
+    ves = np.array([2, 2, 1], np.int32)
+    args = dict(n_ex=3, bv=8, ves=ves.ctypes.data, max_vars=2, cash=8, ev=8, num=8, paths=64,
+                cross=1, normalise=1, max_regressions=3, lower_path_cutoff=10, selector=0,
+                lower_frac=0.25,
+                upper_frac=0.26, sd_guess=2.5, multiplier=0.8, workspace=256, coeffs=8, lo=8,
+                hi=8, means=None, sds=None, price=8, stream=None)
+    args[field] = value
+    st, msg = _status("ls_multi_first_pass_gpu", *args.values())
+    assert st == -1 and "ls_multi_first_pass_gpu" in msg
 
 
 def test_empty_work_is_a_no_op():

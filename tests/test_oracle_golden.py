@@ -428,26 +428,53 @@ def _make_golden():
     return mod
 
 
+@pytest.mark.parametrize("norm", [0, 1])
 @pytest.mark.parametrize("cross", [0, 1])
 @pytest.mark.parametrize("depth", [1, 2, 3])
-def test_ls_multi_first_pass_matches_reference_gold(cross, depth):
+def test_ls_multi_first_pass_matches_reference_gold(cross, depth, norm):
     """Multi-regression LS first pass (one to three nested regressions per
-    date, fraction selector with LSmulti_test.cu's parameters, normalised
-    quadratic basis with and without cross terms) against the gold
+    date, fraction selector with LSmulti_test.cu's parameters, quadratic basis
+    with and without cross terms, variables normalised per regression or raw
+    as in LSmulti_test.cu:1541) against the gold
     generateRegressionCoefficientsViaLSMulti_gold on the same seeded data:
-    price, every coefficient and every cut band."""
+    price, every coefficient and every cut band.  The raw basis is so badly
+    conditioned that the gold's SVD tolerance drops eigen-directions, so
+    the coefficients are compared relative to their size."""
     M = _make_golden()
     g = load("ls_multi_ref.npz")
     c = M.ls_multi_case(*M.LS_MULTI_CASES[(cross, depth)])
     price, co, lo, up, me, sd = O.ls_multi_first_pass(c["bv"], c["ves"], c["cash"], c["ev"],
-                                                      c["num"], cross, depth, 10, *M.SELECTOR)
-    key = f"{cross}_{depth}"
+                                                      c["num"], cross, depth, 10, *M.SELECTOR,
+                                                      normalise=norm)
+    key = f"{cross}_{depth}_{norm}"
+    gc = g["coeffs_" + key]
     assert price == pytest.approx(float(g["price_" + key]), rel=1e-12)
-    assert np.allclose(co, g["coeffs_" + key], rtol=0, atol=1e-12)
+    assert np.abs(co - gc).max() <= (1e-12 if norm else 1e-7) * max(1.0, np.abs(gc).max())
     assert np.allclose(lo, g["lower_" + key], rtol=0, atol=1e-12)
     assert np.allclose(up, g["upper_" + key], rtol=0, atol=1e-12)
-    if depth > 1:  # the nested regressions ran: narrower bands than the first date's
+    if not norm:
+        assert np.all(me == 0) and set(np.unique(sd)) <= {0.0, 1.0}
+    if depth > 1:  # the nested regressions ran
         assert np.any(up[:, 1] != 0)
+
+
+def test_truncated_symmetric_solve_matches_numpy_eigh():
+    """The truncated solve against numpy's eigendecomposition on matrices
+    with eigenvalues spread over 12 decades, and against a plain solve when
+    nothing is dropped."""
+    rng = np.random.default_rng(0)
+    for n in (3, 6, 13):
+        Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        lam = np.logspace(0, -12, n)
+        A = (Q * lam) @ Q.T
+        b = rng.normal(size=n)
+        x = O.sym_solve_truncated(A, b, 1e-8)
+        w, V = np.linalg.eigh(A)
+        keep = np.abs(w) >= 1e-8 * np.abs(w).max()
+        ref = V[:, keep] @ ((V[:, keep].T @ b) / w[keep])
+        assert np.abs(x - ref).max() <= 1e-8 * np.abs(ref).max()
+        B = (Q * np.linspace(1, 2, n)) @ Q.T
+        assert np.allclose(O.sym_solve_truncated(B, b, 1e-8), np.linalg.solve(B, b), atol=1e-13)
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
@@ -465,6 +492,53 @@ def test_fraction_selector_band_holds_the_requested_fraction(seed):
     assert np.array_equal(sel.astype(bool), pre.astype(bool) & (net >= lo) & (net <= up))
     assert count == sel.sum()
     assert int(n_pre * 0.25) <= count <= int(n_pre * 0.26)
+
+
+@pytest.mark.parametrize("n_sds", [0.3, 1.0, 2.5])
+def test_sd_selector_is_a_strict_band_of_n_standard_deviations(n_sds):
+    """The standard-deviation selector against its closed form: the band is
+    n_sds population standard deviations of the pre-selected net values
+    (numpy's std), open at both ends; planting values exactly on the cut
+    shows the ends are excluded."""
+    rng = np.random.default_rng(3)
+    net = rng.normal(0.001, 0.02, 4000)
+    pre = (rng.random(4000) < 0.8).astype(np.int32)
+    cut = n_sds * np.std(net[pre == 1])
+    _, _, lo0, up0 = O.select_sds(net, pre, n_sds)
+    net[:3] = [up0, lo0, 0.0]
+    pre[:3] = 1
+    sel, count, lo, up = O.select_sds(net, pre, n_sds)
+    assert up == pytest.approx(n_sds * np.std(net[pre == 1]), rel=1e-12) and lo == -up
+    assert up0 == pytest.approx(cut, rel=1e-12)
+    inside = (pre == 1) & (np.abs(net) < up)
+    assert np.array_equal(sel.astype(bool), inside) and count == inside.sum()
+    assert sel[2] == 1 and (sel[0] == 0 or net[0] < up) and (sel[1] == 0 or net[1] > lo)
+
+
+def test_multi_first_pass_with_sd_selector_regresses_inside_the_bands():
+    """Two nested regressions with the standard-deviation selector: the
+    second regression's coefficients equal a direct FP64 least-squares fit
+    over the paths inside the first band (normalised by those paths' means
+    and sds), date by date on the last date (no later decision involved)."""
+    M = _make_golden()
+    c = M.ls_multi_case(11, E=3, P=3000, max_vars=2, last_vars=2)
+    price, co, lo, up, me, sd = O.ls_multi_first_pass(c["bv"], c["ves"], c["cash"], c["ev"],
+                                                      c["num"], 1, 2, 10, 0.25, 0.26, 0.5, 0.8,
+                                                      normalise=1, selector=1)
+    e = 2
+    x = c["bv"][e]
+    phi = lambda z: np.stack([np.ones_like(z[0]), z[0], z[1], z[0] ** 2, z[0] * z[1],  # noqa: E731
+                              z[1] ** 2])
+    z0 = (x - me[e, 0][:, None]) / sd[e, 0][:, None]
+    net = co[e, 0] @ phi(z0) - c["ev"][e]
+    band = np.abs(net) < up[e, 0]
+    assert up[e, 0] == pytest.approx(0.5 * np.std(net), rel=1e-10)
+    xs = x[:, band]
+    m, s = xs.mean(axis=1), xs.std(axis=1)
+    assert np.allclose(me[e, 1], m, rtol=1e-12) and np.allclose(sd[e, 1], s, rtol=1e-9)
+    A = phi((xs - m[:, None]) / s[:, None]).T
+    beta, *_ = np.linalg.lstsq(A, c["cash"][e][band], rcond=None)
+    assert np.allclose(co[e, 1], beta, rtol=1e-8, atol=1e-12)
 
 
 def test_fraction_selector_keeps_everything_when_already_in_window():

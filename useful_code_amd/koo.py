@@ -136,6 +136,9 @@ _SIGNATURES = {
     "bermudan_portfolio_bwd": (_i, [_p, _p, _p]),
     "ls_workspace": (_i, [_i, _p]),
     "ls_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
+    "ls_multi_workspace": (_i, [_i, _p]),
+    "ls_multi_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _d, _d,
+                                     _d, _d] + [_p] * 8),
     "ls_second_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
     "ls_normal_eq_gpu": (_i, [_i, _i] + [_p] * 6),
     "cholesky_solve_f64_gpu": (_i, [_i] + [_p] * 5),
@@ -515,6 +518,25 @@ def ls_first_pass_gpu(n_ex, basis_variables, vars_each_step, max_vars, cash, ev,
     call("ls_first_pass_gpu", n_ex, ptr(basis_variables), ves.ctypes.data, max_vars, ptr(cash),
          ptr(ev), ptr(num), paths, int(use_cross), int(itm_only), ptr(sh), ptr(sc),
          ptr(workspace), ptr(coefficients), ptr(price_out), stream_handle(stream))
+
+
+def ls_multi_workspace(paths):
+    n = ctypes.c_size_t(0)
+    call("ls_multi_workspace", paths, ctypes.addressof(n))
+    return n.value
+
+
+def ls_multi_first_pass_gpu(n_ex, basis_variables, vars_each_step, max_vars, cash, ev, num, paths,
+                            use_cross, normalise, max_regressions, lower_path_cutoff, lower_frac,
+                            upper_frac, sd_guess, multiplier, workspace, coefficients, lower_cuts,
+                            upper_cuts, means, sds, price_out, stream=None, selector=0):
+    ves = _host_ints(vars_each_step)
+    call("ls_multi_first_pass_gpu", n_ex, ptr(basis_variables), ves.ctypes.data, max_vars,
+         ptr(cash), ptr(ev), ptr(num), paths, int(use_cross), int(normalise),
+         int(max_regressions), int(lower_path_cutoff), int(selector), float(lower_frac),
+         float(upper_frac), float(sd_guess), float(multiplier), ptr(workspace),
+         ptr(coefficients), ptr(lower_cuts), ptr(upper_cuts), ptr(means), ptr(sds),
+         ptr(price_out), stream_handle(stream))
 
 
 def ls_second_pass_gpu(n_ex, basis_variables, vars_each_step, max_vars, cash, ev, num, paths,
