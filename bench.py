@@ -162,6 +162,61 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
     return res
 
 
+def multi_ls_section(args, torch, O, configs, pipeline, ev):
+    """The reference's multi-regression LS first pass (nested re-regressions
+    with the fraction selector, per-regression normalisation) on the config-3
+    Bermudan path quantities, against the FP64 oracle on the same inputs."""
+    from useful_code_amd import koo
+    w = configs.bermudan_swaption(paths=args.bermudan_paths)
+    P, E = w.paths, len(w.exercise_steps)
+    R, cutoff, sel = 3, 10, (0.25, 0.26, 2.5, 0.8)  # LSmulti_test.cu:270-273
+    pr = pipeline.BermudanPricer(w)
+    pr.generate()
+    pr.paths_kernel()
+    maxd = O.basis_count(2, True)
+    ws = torch.empty(koo.ls_multi_workspace(P), dtype=torch.uint8, device="cuda")
+    co = torch.zeros(E * R * maxd, dtype=torch.float64, device="cuda")
+    lo = torch.zeros(E * R, dtype=torch.float64, device="cuda")
+    up = torch.zeros_like(lo)
+    price = torch.zeros(1, dtype=torch.float64, device="cuda")
+    cash = torch.zeros(E * P, dtype=torch.float32, device="cuda")
+
+    def run():
+        koo.ls_multi_first_pass_gpu(E, pr.bv, pr.ves, 2, cash, pr.ev, pr.num, P, 1, 1, R, cutoff,
+                                    *sel, ws, co, lo, up, None, None, price)
+
+    for _ in range(args.warmup):
+        run()
+    recs = [(ev(), ev()) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    for a, b in recs:
+        a.record()
+        run()
+        b.record()
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in recs]))
+    res = {
+        "workload": "bermudan_payer_20x3 path quantities (2^18 paths, 10 exercise dates, 2 "
+                    "variables, cross basis): up to 3 nested regressions per date, "
+                    "RegressionSelectorFraction(0.25, 0.26, 2.5, 0.8), normalised, FP64",
+        "paths": P,
+        "ms_per_pass": ms,
+        "price_first_pass": price.item(),
+        "nested_bands": int((up.view(E, R)[:, :2] > 0).sum().item()),
+    }
+    if not args.no_error_check:
+        bv = pr.bv.view(E, 2, P).cpu().double().numpy()
+        evo = pr.ev.view(E, P).cpu().double().numpy()
+        num = pr.num.view(E, P).cpu().double().numpy()
+        t = time.perf_counter()
+        p64 = O.ls_multi_first_pass(bv, pr.ves, np.zeros_like(evo), evo, num, 1, R, cutoff, *sel,
+                                    normalise=1)[0]
+        res["cpu_fp64_oracle_s"] = time.perf_counter() - t
+        res["price_fp64"] = p64
+        res["price_rel_err_vs_fp64"] = abs(price.item() - p64) / abs(p64)
+    return res
+
+
 def portfolio_section(args, torch, configs, pipeline, ev):
     """Config 5: 64 Bermudan swaptions (10-40 rates, 4-20 exercise dates,
     1-5 factors, 2^17 paths each): the generators over four streams, then the
@@ -420,6 +475,7 @@ def main():
     ap.add_argument("--no-basket", action="store_true")
     ap.add_argument("--no-swap", action="store_true")
     ap.add_argument("--no-portfolio", action="store_true")
+    ap.add_argument("--no-multi-ls", action="store_true")
     ap.add_argument("--portfolio-paths", type=int, default=1 << 17)
     ap.add_argument("--basket-paths", type=int, default=1 << 20)
     ap.add_argument("--no-stages", action="store_true",
@@ -611,6 +667,10 @@ def main():
 
     if not args.no_bermudan:
         out["bermudan_ls"] = bermudan_section(args, torch, O, configs, pipeline, ev)
+
+    if not args.no_multi_ls:
+        out["bermudan_multi_ls"] = multi_ls_section(args, torch, O, configs, pipeline, ev)
+        torch.cuda.empty_cache()
 
     if not args.no_portfolio:
         out["bermudan_portfolio"] = portfolio_section(args, torch, configs, pipeline, ev)
