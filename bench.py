@@ -113,6 +113,16 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
     paths = np.mean([r[1].elapsed_time(r[2]) for r in recs])
     regress = np.mean([r[2].elapsed_time(r[3]) for r in recs])
     second = np.mean([r[3].elapsed_time(r[4]) for r in recs])
+    # the same job recorded once as a HIP graph and replayed
+    pr.job(capture=True)
+    grecs = [(ev(), ev()) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    for a, b in grecs:
+        a.record()
+        pr.job(capture=True)
+        b.record()
+    torch.cuda.synchronize()
+    graph_ms = float(np.mean([a.elapsed_time(b) for a, b in grecs]))
     price1, price2 = pr.price1.item(), pr.price2.item()
     itm = float((pr.ev.view(E, P) > 0).float().mean().item())
     # The whole first pass (E date kernels, E one-block solves, the final
@@ -128,6 +138,7 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
         "paths": P,
         "path_steps_per_s": P * S / ((gen + paths + regress) * 1e-3),
         "job_ms": wall_ms,
+        "graph_job_ms": graph_ms,
         "kernels_ms": {"sobol_bridge": gen, "bermudan_paths": paths, "ls_first_pass": regress,
                        "ls_second_pass": second},
         "price_first_pass": price1,

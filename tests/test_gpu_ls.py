@@ -442,6 +442,32 @@ def test_batched_portfolio_equals_per_product_calls(torch_cuda):
         assert abs(pr.price1.item() - p1) <= 1e-12 * abs(p1), j
 
 
+def test_bermudan_job_graph_replays_equal_eager(torch_cuda):
+    """The config-3 job recorded as one HIP graph: every replay, with eager
+    work and clobbered outputs between replays, equals the eager job bit for
+    bit (coefficients, first- and second-pass prices, path quantities)."""
+    torch = torch_cuda
+    w = configs.bermudan_swaption(paths=1 << 15)
+    pr = pipeline.BermudanPricer(w)
+    pr.job(capture=False)
+    torch.cuda.synchronize()
+    want = [t.clone() for t in (pr.coeffs, pr.price1, pr.price2, pr.bv, pr.ev, pr.num)]
+    other = pipeline.BermudanPricer(configs.bermudan_swaption(paths=1 << 12, seed=9))
+    for rep in range(4):
+        for t in (pr.coeffs, pr.price1, pr.price2):
+            t.fill_(float("nan"))
+        pr.bv.zero_()
+        pr.ws.fill_(7)
+        other.job(capture=False)  # eager work of the program between replays
+        torch.cuda.synchronize()
+        pr.job(capture=True)
+        torch.cuda.synchronize()
+        got = (pr.coeffs, pr.price1, pr.price2, pr.bv, pr.ev, pr.num)
+        for a, b in zip(want, got):
+            assert torch.equal(a, b), rep
+    print(f"graph replays equal eager: price {pr.price1.item():.8f} / {pr.price2.item():.8f}")
+
+
 def test_portfolio_graph_replays_equal_eager_and_oracle(torch_cuda):
     torch = torch_cuda
     from useful_code_amd import autotune
