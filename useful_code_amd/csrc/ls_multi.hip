@@ -22,18 +22,20 @@
 //                   values of the members as 64-bit sort keys |net| (FP64
 //                   bits; non-members get a sentinel above every key) and
 //                   FP64 partial sums of net and net^2.
-//   radix sort      rocPRIM radix_sort_keys over the keys.
-//   k_multi_search  one block: the selector's band search (grow, shrink,
-//                   bisect on a symmetric band) with exact counts read off
-//                   the sorted keys (a 4096-entry sample index in LDS, then
-//                   one segment of keys), the cuts, and the next depth's
-//                   member count and activity.
+//   k_multi_ladder  a histogram of the member keys against the ladder of cuts
+//                   the selector's search can ask for (one pass, LDS bins).
+//   k_multi_select  one block: the selector's band search (grow, shrink,
+//                   bisect on a symmetric band) on the exact ladder counts;
+//                   the cuts and the next depth's member count, or the
+//                   interval of a bisection.
+//   k_multi_tree    a histogram of the keys inside the bisection interval
+//                   against every guess of the next ten bisection steps.
+//   k_multi_bisect  one block: the bisection on those counts (two stages of
+//                   tree + bisection cover twenty steps).
 // The whole pass computes in FP64 from the FP32 inputs, so the band search
 // takes the same steps as the FP64 gold: counts are exact and the net
 // values agree to rounding.  Depths past the last active one return at once;
 // the host launches every depth, so no result is read back during the pass.
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include "ls_common.h"
 
 namespace koo {
@@ -44,7 +46,6 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kMultiThreads = 1024;  // multi-block kernels (16 wavefronts)
 constexpr int kSearchThreads = 256;
 constexpr int kMaxDepth = 8;         // regressions per exercise date
-constexpr int kIndexMax = 4096;      // sample index of the sorted keys (LDS)
 constexpr int kSelectMaxSteps = 1000;
 constexpr double kSvdTolerance = 1e-8;  // gold/MultiLS_regression.cpp:172
 constexpr unsigned long long kKeySentinel = 0x7FFFFFFFFFFFFFFFull;
@@ -482,152 +483,442 @@ __global__ void __launch_bounds__(kMultiThreads)
 }
 
 // ---------------------------------------------------------------------------
-// The selector's band search on the sorted keys.
+// The selector's band search with exact counts, without ordering the keys.
+//
+// The gold's search asks for counts of members with |net| <= g at a ladder
+// of cuts fixed by the net values' standard deviation -- the growth steps
+// G_j = G_0 / m^j, the shrink steps G_0 m^(i+1) and the first shrink step
+// G_j m after j growth steps -- and then bisects between two of them.
+// k_multi_ladder histograms every member key against the whole ladder (one
+// pass), k_multi_select replays the search on those exact counts, and when
+// it ends in a bisection, k_multi_tree histograms the keys against every
+// guess of the first bisection steps and k_multi_bisect replays those.  A search that runs
+// past the ladder (kLadder steps either way) continues in k_multi_select with
+// block-wide counts over all keys, so every count the gold would take is
+// taken exactly.
 // ---------------------------------------------------------------------------
+constexpr int kLadder = 64;                   // growth / shrink steps held
+constexpr int kLadderSlots = 256;             // >= 3 kLadder - 1
+constexpr unsigned long long kEmpty = ~0ull;  // a cut that counts no path
 
-// Number of member keys <= gb (sorted keys, members first): the sample index
-// idx[j] = keys[j * s] (j < m) locates the segment, one wave counts it.
-// Called by the 64 lanes of one wave; uniform result.
-__device__ __forceinline__ long long count_le(const unsigned long long* __restrict__ keys,
-                                              long long n_pre, const unsigned long long* idx,
-                                              int m, long long s, unsigned long long gb) {
-  const int lane = threadIdx.x & 63;
-  if (n_pre == 0) return 0;
-  const int nch = (m + 63) / 64;
-  const int last = lane * 64 + 63 < m ? lane * 64 + 63 : m - 1;
-  const int c = __popcll(__ballot(lane < nch && idx[last] <= gb));
-  int pos;
-  if (c == nch) {
-    pos = m;
+// Bisection hand-over from k_multi_select to k_multi_tree / k_multi_bisect.
+struct SelectState {
+  double low, high;
+  long long count_low;
+  int bisect;  // 1: the search continues by bisection in (low, high]
+  int it;      // the gold's bisection step counter so far
+};
+
+__device__ __forceinline__ unsigned long long key_of(double g) {
+  return static_cast<unsigned long long>(__double_as_longlong(g + 0.0));
+}
+
+// Net-value mean and standard deviation as the gold forms them.
+__device__ __forceinline__ double net_sd(const double* sums, long long n_pre) {
+  const double np = static_cast<double>(n_pre);
+  const double mean = sums[0] / np;
+  return sqrt(sums[1] / np - mean * mean);
+}
+
+// The ladder of raw cut values v[0..n) and their key bounds tb (a member
+// counts at cut t when key <= tb[t]; kEmpty counts none).  Layout: fraction
+// selector with the pre-selected count inside the window: v[0] = 10000;
+// fraction selector otherwise: v[j] = G_j (j < L), v[L + i] = G_0 m^(i+1)
+// (i < L, by repeated multiplication as the gold's shrink loop), v[2L + j -
+// 1] = G_j m (1 <= j < L); standard-deviation selector: v[0] = sd_guess sd
+// with the open bound.  Whole block; returns n.
+__device__ int ladder_build(const MultiArgs& a, long long n_pre, const double* sums, double* v,
+                            unsigned long long* tb) {
+  const int t = threadIdx.x, L = kLadder;
+  const double sd = net_sd(sums, n_pre);
+  const double np = static_cast<double>(n_pre);
+  const long long lowerT = static_cast<long long>(np * a.lower_frac);
+  const long long upperT = static_cast<long long>(np * a.upper_frac);
+  int n;
+  if (a.selector == 1) {
+    n = 1;
+    if (t == 0) {
+      v[0] = a.sd_guess * sd;
+      tb[0] = v[0] > 0.0 ? static_cast<unsigned long long>(__double_as_longlong(v[0])) - 1ull
+                         : kEmpty;
+    }
+  } else if (n_pre > lowerT && n_pre < upperT) {
+    n = 1;
+    if (t == 0) {
+      v[0] = 10000.0;
+      tb[0] = key_of(10000.0);
+    }
   } else {
-    const int j = c * 64 + lane;
-    pos = c * 64 + __popcll(__ballot(j < m && idx[j] <= gb));
+    n = 3 * L - 1;
+    if (t == 0) {
+      double g = sd * a.sd_guess;
+      v[0] = g;
+      for (int j = 1; j < L; ++j) {
+        g /= a.multiplier;
+        v[j] = g;
+      }
+    } else if (t == 64) {  // the kernels run at least 128 lanes
+      double lo = (sd * a.sd_guess) * a.multiplier;
+      v[L] = lo;
+      for (int i = 1; i < L; ++i) {
+        lo *= a.multiplier;
+        v[L + i] = lo;
+      }
+    }
+    __syncthreads();
+    for (int j = 1 + t; j < L; j += blockDim.x) v[2 * L + j - 1] = v[j] * a.multiplier;
+    __syncthreads();
+    for (int u = t; u < n; u += blockDim.x) tb[u] = v[u] >= 0.0 ? key_of(v[u]) : kEmpty;
   }
-  if (pos == 0) return 0;
-  const long long start = static_cast<long long>(pos - 1) * s;
-  const long long end = start + s < n_pre ? start + s : n_pre;
-  long long cnt = start;
-  for (long long i0 = start; i0 < end; i0 += 64) {
-    const long long i = i0 + lane;
-    cnt += __popcll(__ballot(i < end && keys[i] <= gb));
+  __syncthreads();
+  return n;
+}
+
+// Stable ranks of the key bounds: ts (ascending) and rank[t] = position of
+// tb[t] in ts.  Whole block.
+__device__ __forceinline__ void ladder_sort(int n, const unsigned long long* tb,
+                                            unsigned long long* ts, int* rank) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const unsigned long long x = tb[t];
+    int r = 0;
+    for (int u = 0; u < n; ++u) r += (tb[u] < x || (tb[u] == x && u < t)) ? 1 : 0;
+    ts[r] = x;
+    if (rank) rank[t] = r;
   }
-  return cnt;
+  __syncthreads();
 }
 
-// Members with |net| < g.
-__device__ __forceinline__ long long count_open(const unsigned long long* __restrict__ keys,
-                                                long long n_pre, const unsigned long long* idx,
-                                                int m, long long s, double g) {
-  if (!(g > 0.0)) return 0;
-  return count_le(keys, n_pre, idx, m, s,
-                  static_cast<unsigned long long>(__double_as_longlong(g)) - 1ull);
+// First k < n with ts[k] >= key (n when none).
+__device__ __forceinline__ int ladder_bin(const unsigned long long* ts, int n,
+                                          unsigned long long key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ts[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
 }
 
-// Members with |net| <= g (the gold's -g <= net <= g).
-__device__ __forceinline__ long long count_band(const unsigned long long* __restrict__ keys,
-                                                long long n_pre, const unsigned long long* idx,
-                                                int m, long long s, double g) {
-  if (!(g >= 0.0)) return 0;
-  return count_le(keys, n_pre, idx, m, s,
-                  static_cast<unsigned long long>(__double_as_longlong(g + 0.0)));
-}
-
-__global__ void __launch_bounds__(kSearchThreads)
-    k_multi_search(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
-                   const double* __restrict__ net_part, int nb, DepthState* __restrict__ state,
-                   double* __restrict__ lower, double* __restrict__ upper) {
-  __shared__ unsigned long long idx[kIndexMax];
+// Histogram of the member keys over the ladder: hist_part[bin * nb + block]
+// (FP64, exact) counts the members whose first bound >= key is bin.
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_ladder(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+                   const double* __restrict__ net_part, int nb_net,
+                   const DepthState* __restrict__ state, double* __restrict__ hist_part) {
   __shared__ double sums[2];
+  __shared__ double v[kLadderSlots];
+  __shared__ unsigned long long tb[kLadderSlots], ts[kLadderSlots];
+  __shared__ int hist[kLadderSlots];
+  if (!state[depth].active) return;
+  reduce_partials(net_part, nb_net, 2, sums);
+  const int n = ladder_build(a, state[depth].n_pre, sums, v, tb);
+  ladder_sort(n, tb, ts, nullptr);
+  for (int t = threadIdx.x; t < n; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
+    const unsigned long long key = keys[p];
+    if (key != kKeySentinel) {
+      const int b = ladder_bin(ts, n, key);
+      if (b < n) atomicAdd(&hist[b], 1);
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < n; t += blockDim.x)
+    hist_part[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = static_cast<double>(hist[t]);
+}
+
+// Block-wide count of the keys[0..n) that are <= gb and not the sentinel.
+// Whole block (barriers); uniform result.
+__device__ long long block_count(const unsigned long long* __restrict__ keys, long long n,
+                                 unsigned long long gb, long long* red /* [16] */) {
+  long long c = 0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned long long k = keys[i];
+    c += (k != kKeySentinel && k <= gb) ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  long long tot = 0;
+  for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) tot += red[w];
+  __syncthreads();
+  return tot;
+}
+
+// Writes the regression's cuts (when more than the cutoff remain) and the
+// next depth's members.  Single thread.
+__device__ __forceinline__ void finish_select(const MultiArgs& a, int depth, double cut,
+                                              long long after, DepthState* state, double* lower,
+                                              double* upper) {
+  if (after > a.cutoff) {
+    lower[static_cast<size_t>(a.e) * a.R + depth] = -cut;
+    upper[static_cast<size_t>(a.e) * a.R + depth] = cut;
+  }
+  if (depth + 1 < a.R) {
+    state[depth + 1].active = after > a.cutoff ? 1 : 0;
+    state[depth + 1].n_pre = after;
+    if (a.selector == 1)
+      state[depth + 1].cutbits =
+          cut > 0.0 ? static_cast<unsigned long long>(__double_as_longlong(cut)) - 1ull : 0ull;
+    else
+      state[depth + 1].cutbits = cut >= 0.0 ? key_of(cut) : 0ull;
+  }
+}
+
+// The band search on the ladder counts (RegressionSelector_gold_Fraction::Select,
+// Regression_Selector_concrete_gold.cpp:13-183; RegressionSelectorStandardDeviations,
+// RegressionSelectorConcrete.cu:19-90).  One block; every thread runs the same
+// search on LDS values so the block-wide fallback counts stay uniform.
+__global__ void __launch_bounds__(kSearchThreads)
+    k_multi_select(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+                   const double* __restrict__ net_part, int nb,
+                   const double* __restrict__ hist_part, int nh, DepthState* __restrict__ state,
+                   SelectState* __restrict__ sel, double* __restrict__ lower,
+                   double* __restrict__ upper) {
+  __shared__ double sums[2];
+  __shared__ double v[kLadderSlots], cnt[kLadderSlots];
+  __shared__ unsigned long long tb[kLadderSlots], ts[kLadderSlots];
+  __shared__ int rank[kLadderSlots];
+  __shared__ long long red[16];
   if (!state[depth].active) {
-    if (threadIdx.x == 0 && depth + 1 < a.R) state[depth + 1].active = 0;
+    if (threadIdx.x == 0) {
+      sel->bisect = 0;
+      if (depth + 1 < a.R) state[depth + 1].active = 0;
+    }
     return;
   }
   const long long n_pre = state[depth].n_pre;
   reduce_partials(net_part, nb, 2, sums);
-  const long long s = n_pre > kIndexMax ? (n_pre + kIndexMax - 1) / kIndexMax : 1;
-  const int m = static_cast<int>((n_pre + s - 1) / s);
-  for (int j = threadIdx.x; j < m; j += blockDim.x) idx[j] = keys[j * s];
+  const int n = ladder_build(a, n_pre, sums, v, tb);
+  ladder_sort(n, tb, ts, rank);
+  reduce_partials(hist_part, nh, n, cnt);  // cnt[k]: members in bin k
+  if (threadIdx.x == 0)
+    for (int k = 1; k < n; ++k) cnt[k] += cnt[k - 1];
   __syncthreads();
-  if (threadIdx.x >= 64) return;
-  if (a.selector == 1) {
-    // RegressionSelectorStandardDeviations (RegressionSelectorConcrete.cu:19-90):
-    // the open band of sd_guess standard deviations
-    const double mean = sums[0] / static_cast<double>(n_pre);
-    const double sd = sqrt(sums[1] / static_cast<double>(n_pre) - mean * mean);
-    const double cut = a.sd_guess * sd;
-    const long long after = count_open(keys, n_pre, idx, m, s, cut);
-    if (threadIdx.x == 0) {
-      if (after > a.cutoff) {
-        lower[static_cast<size_t>(a.e) * a.R + depth] = -cut;
-        upper[static_cast<size_t>(a.e) * a.R + depth] = cut;
-      }
-      if (depth + 1 < a.R) {
-        state[depth + 1].active = after > a.cutoff ? 1 : 0;
-        state[depth + 1].n_pre = after;
-        state[depth + 1].cutbits =
-            cut > 0.0 ? static_cast<unsigned long long>(__double_as_longlong(cut)) - 1ull : 0ull;
-      }
-    }
-    return;
-  }
-  // RegressionSelector_gold_Fraction::Select (Regression_Selector_concrete_gold.cpp:13-183)
-  const long long lowerT = static_cast<long long>(static_cast<double>(n_pre) * a.lower_frac);
-  const long long upperT = static_cast<long long>(static_cast<double>(n_pre) * a.upper_frac);
+  // exact count at ladder value t
+  auto C = [&](int t) -> long long {
+    return tb[t] == kEmpty ? 0 : static_cast<long long>(cnt[rank[t]]);
+  };
+  auto brute = [&](double g) -> long long {
+    return g >= 0.0 ? block_count(keys, a.P, key_of(g), red) : 0;
+  };
+  const double np = static_cast<double>(n_pre);
+  const long long lowerT = static_cast<long long>(np * a.lower_frac);
+  const long long upperT = static_cast<long long>(np * a.upper_frac);
+  const double m = a.multiplier;
+  const int L = kLadder;
   double cut;
-  if (n_pre > lowerT && n_pre < upperT) {
-    cut = 10000.0;
+  long long after;
+  int bisect = 0;
+  double low = 0.0, high = 0.0;
+  if (a.selector == 1 || (n_pre > lowerT && n_pre < upperT)) {
+    cut = v[0];
+    after = C(0);
   } else {
-    const double mean = sums[0] / static_cast<double>(n_pre);
-    const double sd = sqrt(sums[1] / static_cast<double>(n_pre) - mean * mean);
-    double guess = sd * a.sd_guess;
-    long long count = count_band(keys, n_pre, idx, m, s, guess);
-    int it = 0;
+    double guess = v[0];
+    long long count = C(0);
+    int it = 0, j = 0;
     while (count < lowerT && it++ < kSelectMaxSteps) {
-      guess /= a.multiplier;
-      count = count_band(keys, n_pre, idx, m, s, guess);
+      ++j;
+      if (j < L) {
+        guess = v[j];
+        count = C(j);
+      } else {
+        guess /= m;
+        count = brute(guess);
+      }
     }
     if (count < upperT) {
       cut = guess;
+      after = count;
     } else {
-      double low = guess * a.multiplier;
-      count = count_band(keys, n_pre, idx, m, s, low);
+      if (j == 0) {
+        low = v[L];
+        count = C(L);
+      } else if (j < L) {
+        low = v[2 * L + j - 1];
+        count = C(2 * L + j - 1);
+      } else {
+        low = guess * m;
+        count = brute(low);
+      }
       it = 0;
+      int i = 0;
       while (count > upperT && it++ < kSelectMaxSteps) {
-        low *= a.multiplier;
-        count = count_band(keys, n_pre, idx, m, s, low);
+        ++i;
+        if (j == 0 && i < L) {
+          low = v[L + i];
+          count = C(L + i);
+        } else {
+          low *= m;
+          count = brute(low);
+        }
       }
       if (count > lowerT) {
         cut = low;
+        after = count;
       } else {
-        double high = guess;
-        guess = 0.5 * (low + high);
-        count = count_band(keys, n_pre, idx, m, s, guess);
-        it = 0;
-        while ((count < lowerT || count > upperT) && (high - low) > 1e-6 &&
-               it++ < kSelectMaxSteps) {
-          if (count < lowerT)
-            low = guess;
-          else
-            high = guess;
-          guess = 0.5 * (low + high);
-          count = count_band(keys, n_pre, idx, m, s, guess);
-        }
-        cut = guess;
+        bisect = 1;
+        high = guess;
+        cut = 0.0;
+        after = count;  // count at low, handed to the bisection
       }
     }
   }
-  const long long after = count_band(keys, n_pre, idx, m, s, cut);
   if (threadIdx.x == 0) {
-    if (after > a.cutoff) {
-      lower[static_cast<size_t>(a.e) * a.R + depth] = -cut;
-      upper[static_cast<size_t>(a.e) * a.R + depth] = cut;
+    sel->bisect = bisect;
+    if (bisect) {
+      sel->low = low;
+      sel->high = high;
+      sel->count_low = after;
+      sel->it = 0;
+    } else {
+      finish_select(a, depth, cut, after, state, lower, upper);
     }
-    if (depth + 1 < a.R) {
-      state[depth + 1].active = after > a.cutoff ? 1 : 0;
-      state[depth + 1].n_pre = after;
-      state[depth + 1].cutbits =
-          cut >= 0.0 ? static_cast<unsigned long long>(__double_as_longlong(cut + 0.0)) : 0ull;
+  }
+}
+
+// The bisection tree: node 1 is the first guess 0.5 (low + high); node i
+// (heap order) splits its interval at its guess, its left child 2i takes the
+// lower half (the gold sets high = guess), its right child 2i + 1 the upper
+// half (low = guess).  The guesses of a depth-kTreeDepth tree cover every
+// guess the first kTreeDepth bisection steps can take, with the same
+// arithmetic as the gold's loop; in-order position of node i at level k is
+// (2 (i - 2^k) + 1) 2^(D-1-k) - 1, and the guesses ascend in that order.
+constexpr int kTreeDepth = 10;
+constexpr int kTreeNodes = (1 << kTreeDepth) - 1;
+constexpr int kBisectStages = 2;  // bisection steps held: kBisectStages * kTreeDepth
+constexpr int kHistBlocks = 256;  // blocks of k_multi_ladder and k_multi_tree (at most)
+
+__device__ __forceinline__ int tree_rank(int i) {
+  const int k = 31 - __builtin_clz(i);  // level
+  return (2 * (i - (1 << k)) + 1) * (1 << (kTreeDepth - 1 - k)) - 1;
+}
+
+// Per block: cumulative counts, in in-order position, of the member keys in
+// (low, high] that are <= each tree guess: cum_part[rank * nb + block].
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_tree(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+                 const DepthState* __restrict__ state, const SelectState* __restrict__ sel,
+                 int* __restrict__ cum_part) {
+  __shared__ double lo[kTreeNodes + 1], hi[kTreeNodes + 1];
+  __shared__ unsigned long long ts[kTreeNodes];
+  __shared__ int hist[kTreeNodes + 1];
+  if (!state[depth].active || !sel->bisect) return;
+  const double low = sel->low, high = sel->high;
+  for (int k = 0; k < kTreeDepth; ++k) {
+    for (int i = (1 << k) + threadIdx.x; i < (2 << k); i += blockDim.x) {
+      double l = low, h = high;
+      if (i > 1) {
+        const int q = i >> 1;
+        const double g = 0.5 * (lo[q] + hi[q]);
+        if (i & 1) {
+          l = g;
+          h = hi[q];
+        } else {
+          l = lo[q];
+          h = g;
+        }
+      }
+      lo[i] = l;
+      hi[i] = h;
+      ts[tree_rank(i)] = key_of(0.5 * (l + h));
     }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < kTreeNodes; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  const unsigned long long lb = key_of(low), hb = key_of(high);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
+    const unsigned long long key = keys[p];
+    if (key != kKeySentinel && key > lb && key <= hb) {
+      const int b = ladder_bin(ts, kTreeNodes, key);
+      if (b < kTreeNodes) atomicAdd(&hist[b], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)  // inclusive prefix over the in-order bins
+    for (int t = 1; t < kTreeNodes; ++t) hist[t] += hist[t - 1];
+  __syncthreads();
+  for (int t = threadIdx.x; t < kTreeNodes; t += blockDim.x)
+    cum_part[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = hist[t];
+}
+
+// Block-wide sum of cum_part[rank * nb + b] over the blocks.
+__device__ long long tree_count(const int* __restrict__ cum_part, int nb, int rank,
+                                long long* red) {
+  long long c = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    c += cum_part[static_cast<size_t>(rank) * nb + b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  long long tot = 0;
+  for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) tot += red[w];
+  __syncthreads();
+  return tot;
+}
+
+// The gold's bisection loop on the tree counts.  Count at a guess g in
+// [low, high] = count at low + keys in (low, g].  A stage that runs past its
+// tree hands (low, high, count at low, step counter) to the next stage
+// (k_multi_tree + k_multi_bisect again); the last stage continues with
+// block-wide counts over all keys.
+__global__ void __launch_bounds__(kSearchThreads)
+    k_multi_bisect(const MultiArgs a, int depth, DepthState* __restrict__ state,
+                   SelectState* __restrict__ sel, const unsigned long long* __restrict__ keys,
+                   const int* __restrict__ cum_part, int nb, double* __restrict__ lower,
+                   double* __restrict__ upper, int last_stage) {
+  __shared__ long long red[16];
+  if (!state[depth].active || !sel->bisect) return;
+  const long long n_pre = state[depth].n_pre;
+  const double np = static_cast<double>(n_pre);
+  const long long lowerT = static_cast<long long>(np * a.lower_frac);
+  const long long upperT = static_cast<long long>(np * a.upper_frac);
+  long long c_low = sel->count_low;
+  double low = sel->low, high = sel->high;
+  int it = sel->it;
+  int node = 1;
+  const long long c0 = c_low;  // the tree counts keys in (low, high] of this stage
+  auto count_at = [&](double g) -> long long {
+    return node <= kTreeNodes ? c0 + tree_count(cum_part, nb, tree_rank(node), red)
+                              : (g >= 0.0 ? block_count(keys, a.P, key_of(g), red) : 0);
+  };
+  double guess = 0.5 * (low + high);
+  long long count = count_at(guess);
+  while ((count < lowerT || count > upperT) && (high - low) > 1e-6 && it++ < kSelectMaxSteps) {
+    const bool up = count < lowerT;
+    if (up) {
+      low = guess;
+      c_low = count;
+    } else {
+      high = guess;
+    }
+    if (node <= kTreeNodes) node = 2 * node + (up ? 1 : 0);  // past the tree: stays past it
+    if (node > kTreeNodes && !last_stage) {
+      __syncthreads();
+      if (threadIdx.x == 0) {  // hand over; the next stage's root is 0.5 (low + high)
+        sel->low = low;
+        sel->high = high;
+        sel->count_low = c_low;
+        sel->it = it;
+      }
+      return;
+    }
+    guess = 0.5 * (low + high);
+    count = count_at(guess);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sel->bisect = 0;
+    finish_select(a, depth, guess, count, state, lower, upper);
   }
 }
 
@@ -667,18 +958,17 @@ int multi_blocks(int paths) {
   return want < 1 ? 1 : (want > cap ? cap : want);
 }
 
-size_t al256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
-
-size_t sort_temp_bytes(int paths) {
-  size_t bytes = 0;
-  (void)rocprim::radix_sort_keys(nullptr, bytes, static_cast<const unsigned long long*>(nullptr),
-                           static_cast<unsigned long long*>(nullptr),
-                           static_cast<unsigned int>(paths), 0, 63);
-  return bytes;
+// Blocks of the ladder and tree histograms: few, so the one-block search
+// sums few partials per count (the pass itself moves 8 B per path).
+int hist_blocks(int paths) {
+  const int nb = multi_blocks(paths);
+  return nb < kHistBlocks ? nb : kHistBlocks;
 }
 
+size_t al256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
 struct MultiLayout {
-  size_t Y, mem, keys, sorted, stats, gram, net, price, norm, state, sort_tmp, sort_bytes, total;
+  size_t Y, mem, keys, cum, stats, gram, net, hist, price, norm, state, sel, total;
 };
 
 MultiLayout multi_layout(int paths) {
@@ -692,23 +982,24 @@ MultiLayout multi_layout(int paths) {
   o += al256(P);
   l.keys = o;
   o += al256(8 * P);
-  l.sorted = o;
-  o += al256(8 * P);
+  l.cum = o;
+  o += al256(4 * static_cast<size_t>(kTreeNodes) * nb);
   l.stats = o;
   o += al256(8 * 2 * kMaxBasisVars * nb);
   l.gram = o;
   o += al256(8 * kMaxGram * kMaxGram * nb);
   l.net = o;
   o += al256(8 * 2 * nb);
+  l.hist = o;
+  o += al256(8 * kLadderSlots * nb);
   l.price = o;
   o += al256(8 * nb);
   l.norm = o;
   o += al256(8 * static_cast<size_t>(kMaxRates) * kMaxDepth * 2 * kMaxBasisVars);
   l.state = o;
   o += al256(sizeof(DepthState) * kMaxDepth);
-  l.sort_tmp = o;
-  l.sort_bytes = sort_temp_bytes(paths);
-  o += al256(l.sort_bytes);
+  l.sel = o;
+  o += al256(sizeof(SelectState));
   l.total = o;
   return l;
 }
@@ -773,16 +1064,18 @@ extern "C" int ls_multi_first_pass_gpu(
   double* Y = reinterpret_cast<double*>(ws + lay.Y);
   unsigned char* mem = reinterpret_cast<unsigned char*>(ws + lay.mem);
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(ws + lay.keys);
-  unsigned long long* sorted = reinterpret_cast<unsigned long long*>(ws + lay.sorted);
+  int* cum = reinterpret_cast<int*>(ws + lay.cum);
   double* stats = reinterpret_cast<double*>(ws + lay.stats);
   double* gram = reinterpret_cast<double*>(ws + lay.gram);
   double* net = reinterpret_cast<double*>(ws + lay.net);
+  double* hist = reinterpret_cast<double*>(ws + lay.hist);
   double* price_part = reinterpret_cast<double*>(ws + lay.price);
   double* norm = reinterpret_cast<double*>(ws + lay.norm);
   DepthState* state = reinterpret_cast<DepthState*>(ws + lay.state);
-  void* sort_tmp = ws + lay.sort_tmp;
+  SelectState* sel = reinterpret_cast<SelectState*>(ws + lay.sel);
   const int R = max_regressions;
   const int nb = multi_blocks(paths);
+  const int nh = hist_blocks(paths);
   hipStream_t st = as_stream(stream);
 
   // outputs start at zero (depths never reached stay zero)
@@ -836,13 +1129,17 @@ extern "C" int ls_multi_first_pass_gpu(
       hipLaunchKernelGGL(k_multi_net, dim3(nb), dim3(kMultiThreads), 0, st, a, d,
                          basis_variables, exercise_values, mem, state, norm, gram, coefficients,
                          keys, net);
-      size_t sort_bytes = lay.sort_bytes;
-      const hipError_t se = rocprim::radix_sort_keys(sort_tmp, sort_bytes, keys, sorted,
-                                                     static_cast<unsigned int>(paths), 0, 63, st);
-      if (se != hipSuccess)
-        return set_error(KOO_ERR_HIP, "%s: radix sort failed: %s", who, hipGetErrorString(se));
-      hipLaunchKernelGGL(k_multi_search, dim3(1), dim3(kSearchThreads), 0, st, a, d, sorted, net,
-                         nb, state, lower_cuts, upper_cuts);
+      hipLaunchKernelGGL(k_multi_ladder, dim3(nh), dim3(kMultiThreads), 0, st, a, d, keys, net, nb,
+                         state, hist);
+      hipLaunchKernelGGL(k_multi_select, dim3(1), dim3(kSearchThreads), 0, st, a, d, keys, net,
+                         nb, hist, nh, state, sel, lower_cuts, upper_cuts);
+      for (int stage = 0; stage < kBisectStages; ++stage) {
+        hipLaunchKernelGGL(k_multi_tree, dim3(nh), dim3(kMultiThreads), 0, st, a, d, keys, state,
+                           sel, cum);
+        hipLaunchKernelGGL(k_multi_bisect, dim3(1), dim3(kSearchThreads), 0, st, a, d, state, sel,
+                           keys, cum, nh, lower_cuts, upper_cuts,
+                           stage + 1 == kBisectStages ? 1 : 0);
+      }
     }
   }
   a.e = 0;
