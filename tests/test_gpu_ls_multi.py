@@ -153,6 +153,27 @@ def test_multi_first_pass_sd_selector(torch_cuda, n_sds):
     assert np.all(got[3][:, :2] > 0)
 
 
+def test_multi_first_pass_searches_past_the_ladder_and_trees(torch_cuda):
+    """Net values that drive the band search past what its histograms hold,
+    so the block-wide fallback counts decide: on one date 1% of the paths
+    exercise for 1e6, so the standard deviation is about 1e5 while most net
+    values are below 0.1; the shrink runs about 74 steps (past the 64-step
+    ladder) and the bisection from [0.01, 2.5e5] about 31 (past the two
+    ten-step trees).  (The growth side passes the ladder only when
+    |mean| / sd > 4e6, where the gold's one-pass variance E[x^2] - mean^2
+    has lost most of its digits -- at |mean| / sd = 1e8 the FP64 oracle's sd
+    is NaN -- so no parity test can pin that branch.)"""
+    c = f32(M.ls_multi_case(17, E=5, P=20000, max_vars=2, last_vars=1))
+    rng = np.random.RandomState(5)
+    c["ev"][1, rng.choice(20000, 200, replace=False)] = 1.0e6
+    for cross in (0, 1):
+        got = run_gpu(torch_cuda, c, cross, 2, 1)
+        want = O.ls_multi_first_pass(c["bv"], c["ves"], c["cash"], c["ev"], c["num"], cross, 2,
+                                     10, *M.SELECTOR, normalise=1)
+        check(got, want, 1)
+        assert 0 < got[3][1, 0] < 1.0  # the search ended far below its start
+
+
 def test_multi_first_pass_is_deterministic(torch_cuda):
     c = f32(M.ls_multi_case(7, E=6, P=20000, max_vars=2, last_vars=1))
     a = run_gpu(torch_cuda, c, 1, 3, 1)
