@@ -1,7 +1,7 @@
 // ls_multi.hip -- the multi-regression Longstaff-Schwartz first pass
 // (generateMultiRegressionCoefficientsViaLS_gpu_flexisolve, LS_main_cula.h:
-// 565-838, with RegressionSelectorFraction, RegressionSelectorConcrete.cu:
-// 102-233; the gold it is checked against is
+// 565-838, with RegressionSelectorFraction or RegressionSelectorStandardDeviations,
+// RegressionSelectorConcrete.cu:19-233; the gold it is checked against is
 // generateRegressionCoefficientsViaLSMultiQuadratic_gold,
 // gold/MultiLS_regression.cpp:20-420).
 //
@@ -19,9 +19,9 @@
 //                   solve with tolerance 1e-8: Cholesky when a trace bound
 //                   shows no eigenvalue can fall under the tolerance, else a
 //                   truncated Jacobi eigen-solve), then net continuation
-//                   values of the members as 64-bit sort keys |net| (FP64
-//                   bits; non-members get a sentinel above every key) and
-//                   FP64 partial sums of net and net^2.
+//                   values of the members as 64-bit keys |net| (the FP64
+//                   bits, which order like the values; non-members get a
+//                   sentinel above every key) and FP64 sums of net and net^2.
 //   k_multi_ladder  a histogram of the member keys against the ladder of cuts
 //                   the selector's search can ask for (one pass, LDS bins).
 //   k_multi_select  one block: the selector's band search (grow, shrink,
