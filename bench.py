@@ -271,6 +271,8 @@ def portfolio_section(args, torch, configs, pipeline, ev):
         "products": len(prods),
         "tuning_s": tune_s,
         "checkpoint_every": tuning["k"],
+        "ls_blocks_per_product": tuning.get("ls_blocks"),
+        "ls_blocks_per_cu_ms": tuning.get("ls_blocks_per_cu_ms"),
         "stream_load_ms": tuning["stream_load_ms"],
         "frozen_price_sum": float(sum(r[1] for r in res)),
         "batched": bool(pf.batched),

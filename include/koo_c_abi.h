@@ -768,6 +768,9 @@ typedef struct koo_ls_product {
   void* workspace;        /* ls_workspace(paths) bytes */
   double* coefficients;   /* device, E x 6 out */
   double* price_out;      /* device, first-pass price */
+  int ls_blocks;          /* 1024-lane blocks of each LS date launch for this product,
+                             at most one per 1024 paths and per CU; 0: the default share
+                             (about two blocks per CU over the portfolio) */
 } koo_ls_product;
 
 size_t bermudan_ls_portfolio_descriptor_bytes(int n_products);

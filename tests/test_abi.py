@@ -103,6 +103,31 @@ def test_multi_ls_rejects_bad_selector_and_sizes(field, value):
     assert st == -1 and "ls_multi_first_pass_gpu" in msg
 
 
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """The ctypes mirrors of the descriptor structs have the C header's
+    sizes and member offsets (gcc on the header itself)."""
+    structs = {"koo_berm_product": koo.BermProduct, "koo_portfolio_plan": koo.PortfolioPlan,
+               "koo_ls_product": koo.LsProduct}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "koo_c_abi.h"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out.splitlines()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
 def test_empty_work_is_a_no_op():
     assert koo.lib().sobolGPUInts(0, 4, 0, None, None, None) == 0
     assert koo.lib().scramble_gpu(None, None, 0, 0, None) == 0

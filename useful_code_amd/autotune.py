@@ -9,10 +9,12 @@ schedules the products over the streams by longest-processing-time-first
 
 Batched mode: the candidate intervals are timed on the batched forward and
 backward launches of the whole portfolio (the products share the device, so
-the sum is what counts), and the generators are scheduled over the streams
-by LPT.
+the sum is what counts); the LS tile -- 1024-lane blocks per product in the
+batched regression launches, a share of 1, 2, 4 or 8 blocks per CU over the
+portfolio -- is timed on the batched path-quantity and regression launches;
+and the generators are scheduled over the streams by LPT.
 
-The result -- k and stream per product and the measured times -- is a plain
+The result -- k, stream and LS tile per product and the measured times -- is a plain
 dict that PortfolioPricer takes and that can be stored as JSON next to the
 results it produced.
 """
@@ -24,6 +26,7 @@ import torch
 from . import koo
 
 CANDIDATE_K = (1, 2, 4, 8, 16)
+CANDIDATE_LS_BLOCKS_PER_CU = (1, 2, 4, 8)
 
 
 def _time(fn, reps=3):
@@ -80,10 +83,26 @@ def tune_batched(pricer, memory_budget_bytes=64 << 30, n_streams=None):
             del pr.adj
         torch.cuda.empty_cache()
     best = min(k_ms, key=k_ms.get)
+    # LS tile: 1024-lane blocks per product of the batched LS date launches,
+    # a share of the CUs over the portfolio (2 per CU is the library default)
+    n = len(pricer.pricers)
+    cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+    ls_ms = {}
+    for per_cu in CANDIDATE_LS_BLOCKS_PER_CU:
+        pricer.ls_blocks = [max(1, -(-per_cu * cus // n))] * n
+        pricer.k = [best] * n
+        pricer._prepare_batch()
+        ls_ms[per_cu] = _time(lambda: koo.bermudan_ls_portfolio_run(pricer.ls_plan,
+                                                                    pricer.ls_descriptors))
+    best_ls = min(ls_ms, key=ls_ms.get)
+    ls_blocks = [max(1, -(-best_ls * cus // n))] * n
+    pricer.ls_blocks = ls_blocks
+    pricer.plan = None
     ms = [_time(pr.generate, reps=2) for pr in pricer.pricers]
     stream, load = _lpt(ms, n_streams)
-    return {"k": [best] * len(pricer.pricers), "stream": stream, "ms": ms,
-            "k_ms": [{str(a): b for a, b in k_ms.items()}] * len(pricer.pricers),
+    return {"k": [best] * n, "stream": stream, "ms": ms,
+            "k_ms": [{str(a): b for a, b in k_ms.items()}] * n,
+            "ls_blocks": ls_blocks, "ls_blocks_per_cu_ms": {str(a): b for a, b in ls_ms.items()},
             "stream_load_ms": load, "batched": True}
 
 

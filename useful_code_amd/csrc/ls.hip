@@ -1152,7 +1152,8 @@ extern "C" int bermudan_ls_portfolio_prepare(int n_products, const koo_ls_produc
     // portfolio (each block also sums the later date's partials and solves
     // its normal equations, so fewer, fuller blocks amortise that)
     {
-      const int share = (2 * device_cu_count() + n_products - 1) / n_products;
+      const int share = q.ls_blocks > 0 ? q.ls_blocks
+                                        : (2 * device_cu_count() + n_products - 1) / n_products;
       const int g = gram_blocks(q.paths);
       it.ls_blocks = share < 1 ? 1 : (share < g ? share : g);
     }

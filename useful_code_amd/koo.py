@@ -52,7 +52,7 @@ class LsProduct(ctypes.Structure):
                 ("strike", _f), ("n_ex", _i), ("exercise_steps", _p), ("var_shift", _p),
                 ("var_scale", _p), ("itm_only", _i), ("basis_variables", _p),
                 ("exercise_values", _p), ("numeraires", _p), ("workspace", _p),
-                ("coefficients", _p), ("price_out", _p)]
+                ("coefficients", _p), ("price_out", _p), ("ls_blocks", _i)]
 
 
 # name -> (restype, argtypes); pointers are passed as integers (c_void_p).

@@ -162,9 +162,9 @@ class BermudanPricer:
             self.frozen_greeks = torch.zeros(2 * N, dtype=torch.float64, device=dev)
         return self.adj
 
-    def ls_batch_product(self):
+    def ls_batch_product(self, ls_blocks=0):
         """This product's koo.LsProduct (path quantities and LS first pass,
-        as paths_kernel() + regress())."""
+        as paths_kernel() + regress()); ls_blocks as koo_ls_product."""
         w = self.w
         self._ex_ls = np.ascontiguousarray(w.exercise_steps, np.int32)
         self._sh_ls = np.ascontiguousarray(self.shift, np.float32)
@@ -175,7 +175,7 @@ class BermudanPricer:
             len(self._ex_ls), self._ex_ls.ctypes.data, self._sh_ls.ctypes.data,
             self._sc_ls.ctypes.data, 1, self.bv.data_ptr(), self.ev.data_ptr(),
             self.num.data_ptr(), self.ws.data_ptr(), self.coeffs.data_ptr(),
-            self.price1.data_ptr())
+            self.price1.data_ptr(), int(ls_blocks))
 
     def batch_product(self, checkpoint_every):
         """This product's koo.BermProduct for the batched portfolio kernels
@@ -278,18 +278,21 @@ class PortfolioPricer:
         # date rank), every frozen forward in one launch and every backward in
         # one launch; otherwise each product's sequence on its stream
         self.batched = batched and all(w.factors <= 5 and w.n_rates <= 45 for w in products)
+        self.ls_blocks = [0] * len(products)  # LS tile per product (0: the library's share)
         self.plan = None
 
     def apply_tuning(self, tuning):
-        """Checkpoint interval and stream per product from autotune.tune_portfolio."""
+        """Checkpoint interval, stream and LS tile per product from
+        autotune.tune_portfolio."""
         self.k = list(tuning["k"])
         self.stream_of = list(tuning["stream"])
+        self.ls_blocks = list(tuning.get("ls_blocks", [0] * len(self.pricers)))
         self.graph = None
         self.plan = None
 
     def _prepare_batch(self):
         n = len(self.pricers)
-        ls_items = [pr.ls_batch_product() for pr in self.pricers]
+        ls_items = [pr.ls_batch_product(self.ls_blocks[j]) for j, pr in enumerate(self.pricers)]
         self.ls_descriptors = torch.empty(koo.bermudan_ls_portfolio_descriptor_bytes(n),
                                           dtype=torch.uint8, device=self.device)
         self.ls_plan = koo.bermudan_ls_portfolio_prepare(ls_items, self.ls_descriptors)
