@@ -90,8 +90,23 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// (r1)^(1 - theta) (r2)^theta, the log-linear interpolation of the discount
+// factor between two rate times.  theta is uniform over a launch's lanes:
+// the end points (0: payment on a rate time, the usual case) are exact, in
+// between 2^((1 - theta) log2 r1 + theta log2 r2) with the hardware log2 /
+// exp2 (v_log_f32 / v_exp_f32, about 1 ulp each; the ratios are positive
+// normal numbers) instead of two powf calls.
+__device__ __forceinline__ float interp_df(float r1, float r2, float th) {
+  if (th == 0.0f) return r1;
+  if (th == 1.0f) return r2;
+  return __builtin_amdgcn_exp2f((1.0f - th) * __builtin_amdgcn_logf(r1) +
+                                th * __builtin_amdgcn_logf(r2));
+}
+
 // df = (P1/P_s)^(1-theta) (P2/P_s)^theta / B_s plus per-path sum
-// (cashFlowDiscounting_gpu.cu:170-258).
+// (cashFlowDiscounting_gpu.cu:170-258).  The step loop is unrolled so the
+// gathers of several steps are in flight; quotients by the hardware
+// reciprocal (about 1 ulp).
 __global__ void __launch_bounds__(256)
     k_discounting(const int* __restrict__ first, const int* __restrict__ second,
                   const float* __restrict__ theta, const float* __restrict__ disc,
@@ -101,15 +116,15 @@ __global__ void __launch_bounds__(256)
   if (p >= paths) return;
   const size_t P = static_cast<size_t>(paths);
   float total = 0.0f;
+#pragma unroll 4
   for (int s = 0; s < steps; ++s) {
     const float* dcol = disc + static_cast<size_t>(s) * (steps + 1) * P + p;
-    const float now = dcol[static_cast<size_t>(s) * P];
-    const float r1 = dcol[static_cast<size_t>(first[s]) * P] / now;
-    const float r2 = dcol[static_cast<size_t>(second[s]) * P] / now;
-    const float th = theta[s];
-    const float rdf = powf(r1, 1.0f - th) * powf(r2, th);
+    const float inv_now = __builtin_amdgcn_rcpf(dcol[static_cast<size_t>(s) * P]);
+    const float r1 = dcol[static_cast<size_t>(first[s]) * P] * inv_now;
+    const float r2 = dcol[static_cast<size_t>(second[s]) * P] * inv_now;
+    const float rdf = interp_df(r1, r2, theta[s]);
     const size_t o = static_cast<size_t>(s) * P + p;
-    const float fl = flows[o] * (rdf / num[o]);
+    const float fl = flows[o] * (rdf * __builtin_amdgcn_rcpf(num[o]));
     total += fl;
     out[o] = fl;
   }
@@ -127,16 +142,17 @@ __global__ void __launch_bounds__(256)
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= paths) return;
   const size_t P = static_cast<size_t>(paths);
+#pragma unroll 4
   for (int k = 0; k < n_flows; ++k) {
     const int i = gen[k];
     const float* dcol = disc + static_cast<size_t>(i) * (steps + 1) * P + p;
-    const float now = dcol[static_cast<size_t>(i) * P];
-    const float r1 = dcol[static_cast<size_t>(first[k]) * P] / now;
-    const float r2 = dcol[static_cast<size_t>(second[k]) * P] / now;
-    const float th = theta[k];
-    const float rdf = powf(r1, 1.0f - th) * powf(r2, th);
-    float fl = flows[static_cast<size_t>(k) * P + p] * (rdf / now);
-    fl *= num[static_cast<size_t>(target[k]) * P + p] / num[static_cast<size_t>(i) * P + p];
+    const float inv_now = __builtin_amdgcn_rcpf(dcol[static_cast<size_t>(i) * P]);
+    const float r1 = dcol[static_cast<size_t>(first[k]) * P] * inv_now;
+    const float r2 = dcol[static_cast<size_t>(second[k]) * P] * inv_now;
+    const float rdf = interp_df(r1, r2, theta[k]);
+    float fl = flows[static_cast<size_t>(k) * P + p] * (rdf * inv_now);
+    fl *= num[static_cast<size_t>(target[k]) * P + p] *
+          __builtin_amdgcn_rcpf(num[static_cast<size_t>(i) * P + p]);
     out[static_cast<size_t>(k) * P + p] = fl;
   }
 }

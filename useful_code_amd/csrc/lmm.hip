@@ -10,10 +10,14 @@
 // traffic is the variates read and the rate / log-rate / discount cubes
 // written.
 #include "koo_common.h"
+#include "lmm_step.h"
 
 namespace koo {
 
 namespace {
+
+using lmm::fast_exp;
+using lmm::fast_rcp;
 
 constexpr int kMaxSteps = 512;
 constexpr int kMaxRates = 128;
@@ -68,7 +72,10 @@ __global__ void __launch_bounds__(256)
 // LMM_evolver_pc_all_gpu: predictor-corrector log-Euler over all steps,
 // discount ratios fused (LMM_evolver_all.cu:506-747).  Step 0 takes the host
 // initial drifts; later steps recompute the drifts from the previous step's
-// rates, which stay in LDS ([rate][lane], log-rates then rates).
+// rates: the log-rates stay in LDS ([rate][lane]) and f + d = exp(log-rate)
+// is recomputed (one v_exp_f32 instead of a second LDS curve, so twice the
+// lanes fit per CU).  exp and 1/x are the hardware forms (v_exp_f32,
+// v_rcp_f32, about 1 ulp), as in the fused pricing kernels.
 // ----------------------------------------------------------------------
 template <int F>
 __global__ void __launch_bounds__(256)
@@ -81,15 +88,12 @@ __global__ void __launch_bounds__(256)
                  float* __restrict__ disc_out, int do_disc) {
   extern __shared__ float lds[];
   const int bd = blockDim.x;
-  float* L = lds + threadIdx.x;               // L[r * bd]
-  float* R = lds + rates * bd + threadIdx.x;  // R[r * bd]
+  float* L = lds + threadIdx.x;  // L[r * bd]: log(f + d) of the curve
   const int p = blockIdx.x * bd + threadIdx.x;
   if (p >= paths) return;
   const size_t P = static_cast<size_t>(paths);
-  for (int r = alive[0]; r < rates; ++r) {
-    L[r * bd] = logf0[r];
-    R[r * bd] = f0[r];
-  }
+  (void)f0;  // step 0 takes the host drifts; later steps rebuild f + d = exp(L)
+  for (int r = alive[0]; r < rates; ++r) L[r * bd] = logf0[r];
   for (int s = 0; s < steps; ++s) {
     const int a = alive[s];
     float z[F], e[F], ep[F];
@@ -113,8 +117,8 @@ __global__ void __launch_bounds__(256)
       if (s == 0) {
         mu = mu0[r];
       } else {
-        const float fr = R[r * bd];
-        const float g = (fr + d) * tau / (1.0f + fr * tau);
+        const float x = fast_exp(L[r * bd]);  // f + d at the start of the step
+        const float g = tau * x * fast_rcp(1.0f + tau * (x - d));
         mu = 0.0f;
 #pragma unroll
         for (int f = 0; f < F; ++f) {
@@ -126,8 +130,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int f = 0; f < F; ++f) dW += ar[f] * z[f];
       float lf = L[r * bd] + mu + dW + fixed[s * rates + r];
-      const float xp = expf(lf);
-      const float gp = xp * tau / (1.0f + (xp - d) * tau);
+      const float xp = fast_exp(lf);
+      const float gp = tau * xp * fast_rcp(1.0f + tau * (xp - d));
       float mup = 0.0f;
 #pragma unroll
       for (int f = 0; f < F; ++f) {
@@ -135,14 +139,13 @@ __global__ void __launch_bounds__(256)
         mup += ar[f] * ep[f];
       }
       lf += 0.5f * (mup - mu);
-      const float rate = expf(lf) - d;
+      const float rate = fast_exp(lf) - d;
       L[r * bd] = lf;
-      R[r * bd] = rate;
       const size_t o = (static_cast<size_t>(s) * rates + r) * P + p;
       logs_out[o] = lf;
       rates_out[o] = rate;
       if (do_disc) {
-        df /= 1.0f + rate * tau;
+        df *= fast_rcp(1.0f + rate * tau);
         dcol[static_cast<size_t>(r + 1) * P] = df;
       }
     }
@@ -402,8 +405,8 @@ extern "C" int LMM_evolver_pc_all_gpu(float* initial_rates, float* initial_log_r
     return set_error(KOO_ERR_INVALID, "LMM_evolver_pc_all_gpu: null pointer");
   const int path_mult = transposeData ? 1 : factors;
   const int fact_mult = transposeData ? paths : 1;
-  const int threads = lanes_for_state(2 * rates);
-  const size_t lds = sizeof(float) * 2 * rates * threads;
+  const int threads = lanes_for_state(rates);
+  const size_t lds = sizeof(float) * rates * threads;
   dim3 g = path_grid(paths, threads);
   hipStream_t st = as_stream(stream);
   switch (factors) {

@@ -72,18 +72,19 @@ __global__ void __launch_bounds__(kSobolThreads)
 }
 
 // In-place XOR of the first number_dim rows; 4 words per lane when rows are
-// 16-byte aligned.
-__global__ void k_scramble_vec4(uint4* __restrict__ data, const uint32_t* __restrict__ scr,
-                                int64_t n4, int64_t paths4) {
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n4;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t s = scr[i / paths4];
-    uint4 w = data[i];
+// 16-byte aligned.  One row per blockIdx.y, so the scrambler is one scalar
+// load per block and no lane divides to find its row.
+__global__ void k_scramble_rows(uint4* __restrict__ data, const uint32_t* __restrict__ scr,
+                                int paths4) {
+  const uint32_t s = scr[blockIdx.y];
+  uint4* row = data + static_cast<size_t>(blockIdx.y) * paths4;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < paths4; i += gridDim.x * blockDim.x) {
+    uint4 w = row[i];
     w.x ^= s;
     w.y ^= s;
     w.z ^= s;
     w.w ^= s;
-    data[i] = w;
+    row[i] = w;
   }
 }
 
@@ -219,11 +220,12 @@ extern "C" int scramble_gpu(unsigned int* device_data, unsigned int* scramblers,
   if (n == 0) return KOO_OK;
   if (!device_data || !scramblers) return set_error(KOO_ERR_INVALID, "scramble_gpu: null pointer");
   const int cap = 8 * device_cu_count() * 8;
-  if (paths % 4 == 0 && (reinterpret_cast<uintptr_t>(device_data) & 15u) == 0) {
-    const int64_t n4 = n / 4;
-    hipLaunchKernelGGL(k_scramble_vec4, dim3(grid_for(n4, 256, cap)), dim3(256), 0,
-                       as_stream(stream), reinterpret_cast<uint4*>(device_data), scramblers, n4,
-                       static_cast<int64_t>(paths / 4));
+  if (paths % 4 == 0 && (reinterpret_cast<uintptr_t>(device_data) & 15u) == 0 &&
+      number_dim <= 65535) {
+    const int paths4 = paths / 4;
+    hipLaunchKernelGGL(k_scramble_rows, dim3(grid_for(paths4, 256, 65535), number_dim), dim3(256),
+                       0, as_stream(stream), reinterpret_cast<uint4*>(device_data), scramblers,
+                       paths4);
   } else {
     hipLaunchKernelGGL(k_scramble, dim3(grid_for(n, 256, cap)), dim3(256), 0, as_stream(stream),
                        device_data, scramblers, n, static_cast<int64_t>(paths));
