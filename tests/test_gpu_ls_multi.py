@@ -174,6 +174,22 @@ def test_multi_first_pass_searches_past_the_ladder_and_trees(torch_cuda):
         assert 0 < got[3][1, 0] < 1.0  # the search ended far below its start
 
 
+def test_multi_first_pass_many_exact_zero_net_values(torch_cuda):
+    """The last date with no cash flow and 60% of the paths out of the money
+    (exercise value 0): the regression target is 0, so 60% of the net values
+    are exactly 0, more than the selector's upper fraction at every cut, and
+    the gold's shrink loop runs all of its 1000 steps (the Bermudan
+    swaption's own first pass does this)."""
+    c = f32(M.ls_multi_case(19, E=4, P=20000, max_vars=2, last_vars=1))
+    rng = np.random.RandomState(3)
+    c["cash"][3, :] = 0.0
+    c["ev"][3, rng.rand(20000) < 0.6] = 0.0
+    got = run_gpu(torch_cuda, c, 1, 2, 1)
+    want = O.ls_multi_first_pass(c["bv"], c["ves"], c["cash"], c["ev"], c["num"], 1, 2, 10,
+                                 *M.SELECTOR, normalise=1)
+    check(got, want, 1)
+
+
 def test_multi_first_pass_is_deterministic(torch_cuda):
     c = f32(M.ls_multi_case(7, E=6, P=20000, max_vars=2, last_vars=1))
     a = run_gpu(torch_cuda, c, 1, 3, 1)

@@ -509,6 +509,44 @@ struct SelectState {
   int it;      // the gold's bisection step counter so far
 };
 
+// Extent of the member keys: smallest positive key, largest non-NaN key
+// (+inf included), count of zero keys and of non-NaN keys.  Every count at a
+// cut below the smallest positive key is the zero count, at or above the
+// largest key the non-NaN count, so the search's long tails (e.g. half the
+// net values exactly 0, where the gold's shrink loop runs all its 1000
+// steps) cost no pass over the keys.
+struct KeyExtent {
+  unsigned long long minpos, maxval;
+  long long nzero, nvalid;
+};
+
+constexpr unsigned long long kInfKey = 0x7FF0000000000000ull;
+
+// Sums the per-block extents ext[0..nb) into LDS *out.  Whole block.
+__device__ void reduce_extents(const KeyExtent* __restrict__ ext, int nb, KeyExtent* out) {
+  if (threadIdx.x < 64) {
+    unsigned long long mn = ~0ull, mx = 0ull;
+    long long nz = 0, nv = 0;
+    for (int b = threadIdx.x; b < nb; b += 64) {
+      const KeyExtent e = ext[b];
+      mn = e.minpos < mn ? e.minpos : mn;
+      mx = e.maxval > mx ? e.maxval : mx;
+      nz += e.nzero;
+      nv += e.nvalid;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
+      mn = a < mn ? a : mn;
+      mx = b > mx ? b : mx;
+      nz += __shfl_xor(nz, o, 64);
+      nv += __shfl_xor(nv, o, 64);
+    }
+    if (threadIdx.x == 0) *out = KeyExtent{mn, mx, nz, nv};
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ unsigned long long key_of(double g) {
   return static_cast<unsigned long long>(__double_as_longlong(g + 0.0));
 }
@@ -607,27 +645,67 @@ __device__ __forceinline__ int ladder_bin(const unsigned long long* ts, int n,
 __global__ void __launch_bounds__(kMultiThreads)
     k_multi_ladder(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
                    const double* __restrict__ net_part, int nb_net,
-                   const DepthState* __restrict__ state, double* __restrict__ hist_part) {
+                   const DepthState* __restrict__ state, double* __restrict__ hist_part,
+                   KeyExtent* __restrict__ ext_part) {
   __shared__ double sums[2];
   __shared__ double v[kLadderSlots];
   __shared__ unsigned long long tb[kLadderSlots], ts[kLadderSlots];
   __shared__ int hist[kLadderSlots];
+  __shared__ KeyExtent wext[16];
   if (!state[depth].active) return;
   reduce_partials(net_part, nb_net, 2, sums);
   const int n = ladder_build(a, state[depth].n_pre, sums, v, tb);
   ladder_sort(n, tb, ts, nullptr);
   for (int t = threadIdx.x; t < n; t += blockDim.x) hist[t] = 0;
   __syncthreads();
+  unsigned long long mn = ~0ull, mx = 0ull;
+  long long nz = 0, nv = 0;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
     const unsigned long long key = keys[p];
     if (key != kKeySentinel) {
       const int b = ladder_bin(ts, n, key);
       if (b < n) atomicAdd(&hist[b], 1);
+      if (key <= kInfKey) {  // not NaN
+        ++nv;
+        mx = key > mx ? key : mx;
+        if (key == 0ull)
+          ++nz;
+        else
+          mn = key < mn ? key : mn;
+      }
     }
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long a2 = __shfl_xor(mn, o, 64), b2 = __shfl_xor(mx, o, 64);
+    mn = a2 < mn ? a2 : mn;
+    mx = b2 > mx ? b2 : mx;
+    nz += __shfl_xor(nz, o, 64);
+    nv += __shfl_xor(nv, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) wext[threadIdx.x >> 6] = KeyExtent{mn, mx, nz, nv};
   __syncthreads();
   for (int t = threadIdx.x; t < n; t += blockDim.x)
     hist_part[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = static_cast<double>(hist[t]);
+  if (threadIdx.x == 0) {
+    KeyExtent e = wext[0];
+    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+      e.minpos = wext[w].minpos < e.minpos ? wext[w].minpos : e.minpos;
+      e.maxval = wext[w].maxval > e.maxval ? wext[w].maxval : e.maxval;
+      e.nzero += wext[w].nzero;
+      e.nvalid += wext[w].nvalid;
+    }
+    ext_part[blockIdx.x] = e;
+  }
+}
+
+// Count of member keys <= key_of(g) for g >= 0 from the extents when they
+// decide it, else -1.
+__device__ __forceinline__ long long extent_count(const KeyExtent& x, double g) {
+  const unsigned long long gb = key_of(g);
+  if (gb < x.minpos) return x.nzero;
+  if (gb >= x.maxval) return x.nvalid;
+  return -1;
 }
 
 // Block-wide count of the keys[0..n) that are <= gb and not the sentinel.
@@ -677,14 +755,15 @@ __device__ __forceinline__ void finish_select(const MultiArgs& a, int depth, dou
 __global__ void __launch_bounds__(kSearchThreads)
     k_multi_select(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
                    const double* __restrict__ net_part, int nb,
-                   const double* __restrict__ hist_part, int nh, DepthState* __restrict__ state,
-                   SelectState* __restrict__ sel, double* __restrict__ lower,
-                   double* __restrict__ upper) {
+                   const double* __restrict__ hist_part, const KeyExtent* __restrict__ ext_part,
+                   int nh, DepthState* __restrict__ state, SelectState* __restrict__ sel,
+                   double* __restrict__ lower, double* __restrict__ upper) {
   __shared__ double sums[2];
   __shared__ double v[kLadderSlots], cnt[kLadderSlots];
   __shared__ unsigned long long tb[kLadderSlots], ts[kLadderSlots];
   __shared__ int rank[kLadderSlots];
   __shared__ long long red[16];
+  __shared__ KeyExtent ext;
   if (!state[depth].active) {
     if (threadIdx.x == 0) {
       sel->bisect = 0;
@@ -697,6 +776,7 @@ __global__ void __launch_bounds__(kSearchThreads)
   const int n = ladder_build(a, n_pre, sums, v, tb);
   ladder_sort(n, tb, ts, rank);
   reduce_partials(hist_part, nh, n, cnt);  // cnt[k]: members in bin k
+  reduce_extents(ext_part, nh, &ext);
   if (threadIdx.x == 0)
     for (int k = 1; k < n; ++k) cnt[k] += cnt[k - 1];
   __syncthreads();
@@ -705,7 +785,9 @@ __global__ void __launch_bounds__(kSearchThreads)
     return tb[t] == kEmpty ? 0 : static_cast<long long>(cnt[rank[t]]);
   };
   auto brute = [&](double g) -> long long {
-    return g >= 0.0 ? block_count(keys, a.P, key_of(g), red) : 0;
+    if (!(g >= 0.0)) return 0;
+    const long long c = extent_count(ext, g);
+    return c >= 0 ? c : block_count(keys, a.P, key_of(g), red);
   };
   const double np = static_cast<double>(n_pre);
   const long long lowerT = static_cast<long long>(np * a.lower_frac);
@@ -874,10 +956,13 @@ __device__ long long tree_count(const int* __restrict__ cum_part, int nb, int ra
 __global__ void __launch_bounds__(kSearchThreads)
     k_multi_bisect(const MultiArgs a, int depth, DepthState* __restrict__ state,
                    SelectState* __restrict__ sel, const unsigned long long* __restrict__ keys,
-                   const int* __restrict__ cum_part, int nb, double* __restrict__ lower,
-                   double* __restrict__ upper, int last_stage) {
+                   const int* __restrict__ cum_part, const KeyExtent* __restrict__ ext_part,
+                   int nb, double* __restrict__ lower, double* __restrict__ upper,
+                   int last_stage) {
   __shared__ long long red[16];
+  __shared__ KeyExtent ext;
   if (!state[depth].active || !sel->bisect) return;
+  reduce_extents(ext_part, nb, &ext);
   const long long n_pre = state[depth].n_pre;
   const double np = static_cast<double>(n_pre);
   const long long lowerT = static_cast<long long>(np * a.lower_frac);
@@ -888,8 +973,10 @@ __global__ void __launch_bounds__(kSearchThreads)
   int node = 1;
   const long long c0 = c_low;  // the tree counts keys in (low, high] of this stage
   auto count_at = [&](double g) -> long long {
-    return node <= kTreeNodes ? c0 + tree_count(cum_part, nb, tree_rank(node), red)
-                              : (g >= 0.0 ? block_count(keys, a.P, key_of(g), red) : 0);
+    if (node <= kTreeNodes) return c0 + tree_count(cum_part, nb, tree_rank(node), red);
+    if (!(g >= 0.0)) return 0;
+    const long long c = extent_count(ext, g);
+    return c >= 0 ? c : block_count(keys, a.P, key_of(g), red);
   };
   double guess = 0.5 * (low + high);
   long long count = count_at(guess);
@@ -968,7 +1055,7 @@ int hist_blocks(int paths) {
 size_t al256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 struct MultiLayout {
-  size_t Y, mem, keys, cum, stats, gram, net, hist, price, norm, state, sel, total;
+  size_t Y, mem, keys, cum, stats, gram, net, hist, ext, price, norm, state, sel, total;
 };
 
 MultiLayout multi_layout(int paths) {
@@ -992,6 +1079,8 @@ MultiLayout multi_layout(int paths) {
   o += al256(8 * 2 * nb);
   l.hist = o;
   o += al256(8 * kLadderSlots * nb);
+  l.ext = o;
+  o += al256(sizeof(KeyExtent) * nb);
   l.price = o;
   o += al256(8 * nb);
   l.norm = o;
@@ -1069,6 +1158,7 @@ extern "C" int ls_multi_first_pass_gpu(
   double* gram = reinterpret_cast<double*>(ws + lay.gram);
   double* net = reinterpret_cast<double*>(ws + lay.net);
   double* hist = reinterpret_cast<double*>(ws + lay.hist);
+  KeyExtent* ext = reinterpret_cast<KeyExtent*>(ws + lay.ext);
   double* price_part = reinterpret_cast<double*>(ws + lay.price);
   double* norm = reinterpret_cast<double*>(ws + lay.norm);
   DepthState* state = reinterpret_cast<DepthState*>(ws + lay.state);
@@ -1130,14 +1220,14 @@ extern "C" int ls_multi_first_pass_gpu(
                          basis_variables, exercise_values, mem, state, norm, gram, coefficients,
                          keys, net);
       hipLaunchKernelGGL(k_multi_ladder, dim3(nh), dim3(kMultiThreads), 0, st, a, d, keys, net, nb,
-                         state, hist);
+                         state, hist, ext);
       hipLaunchKernelGGL(k_multi_select, dim3(1), dim3(kSearchThreads), 0, st, a, d, keys, net,
-                         nb, hist, nh, state, sel, lower_cuts, upper_cuts);
+                         nb, hist, ext, nh, state, sel, lower_cuts, upper_cuts);
       for (int stage = 0; stage < kBisectStages; ++stage) {
         hipLaunchKernelGGL(k_multi_tree, dim3(nh), dim3(kMultiThreads), 0, st, a, d, keys, state,
                            sel, cum);
         hipLaunchKernelGGL(k_multi_bisect, dim3(1), dim3(kSearchThreads), 0, st, a, d, state, sel,
-                           keys, cum, nh, lower_cuts, upper_cuts,
+                           keys, cum, ext, nh, lower_cuts, upper_cuts,
                            stage + 1 == kBisectStages ? 1 : 0);
       }
     }
