@@ -195,6 +195,8 @@ def _declare_ref(R):
         "ref_tarn_flows": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _f64p, _f64p]),
         "ref_cash_flow_indices_weights": (None, [_f64p, _c_int, _f64p, _c_int, _i32p, _i32p,
                                                  _f64p]),
+        "ref_cash_flow_discounting_partial": (None, [_c_int, _i32p, _i32p, _i32p, _f64p, _i32p,
+                                                     _f64p, _f64p, _f64p, _c_int, _c_int, _f64p]),
         "ref_cash_flow_discounting": (None, [_i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _c_int,
                                              _c_int, _f64p, _f64p]),
         "ref_flat_vol_pseudo_roots": (None, [_f64p, _f64p, _f64p, _c_int, _c_int, _c_int, _c_dbl,

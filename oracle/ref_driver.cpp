@@ -287,6 +287,28 @@ void ref_cash_flow_discounting(const int* first, const int* second, const double
   std::memcpy(summed, s.data(), s.size() * sizeof(double));
 }
 
+// gold/cashFlowDiscounting_gold.h:97-205 cashFlowDiscountingPartial_gold<double>:
+// flows generated at step gen[k], discounted log-linearly to the payment time
+// and by the numeraire ratio to step target[k]; n_flows <= steps.
+void ref_cash_flow_discounting_partial(int n_flows, const int* gen, const int* first,
+                                       const int* second, const double* thetas, const int* target,
+                                       const double* disc, const double* flows, const double* num,
+                                       int paths, int steps, double* out) {
+  std::vector<int> g = vec(gen, n_flows), a = vec(first, n_flows), b = vec(second, n_flows);
+  std::vector<int> t = vec(target, n_flows);
+  std::vector<double> th = vec(thetas, n_flows);
+  std::vector<double> d = vec(disc, (size_t)steps * (steps + 1) * paths);
+  // the gold sizes the flow matrices by steps (n_flows <= steps rows used)
+  std::vector<double> f((size_t)steps * paths, 0.0), nv = vec(num, (size_t)steps * paths);
+  std::memcpy(f.data(), flows, sizeof(double) * (size_t)n_flows * paths);
+  std::vector<double> o((size_t)steps * paths, 0.0);
+  CubeConstFacade<double> dc(&d[0], steps, steps + 1, paths);
+  MatrixConstFacade<double> fm(&f[0], steps, paths), nm(&nv[0], steps, paths);
+  MatrixFacade<double> om(&o[0], steps, paths);
+  cashFlowDiscountingPartial_gold<double>(g, a, b, th, t, dc, fm, nm, paths, steps, om);
+  std::memcpy(out, o.data(), sizeof(double) * (size_t)n_flows * paths);
+}
+
 // gold/volstructs_gold.cpp:236 FlatVolPseudoRootsOfCovariances (exp-long correlation)
 void ref_flat_vol_pseudo_roots(const double* rate_times, const double* evol_times, const double* vols,
                                int n_rates, int steps, int factors, double L, double beta, double* A) {

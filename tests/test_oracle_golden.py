@@ -219,6 +219,35 @@ def test_discounting_matches_reference_gold():
     assert np.allclose(summed, c["summed"], rtol=1e-14)
 
 
+def test_partial_discounting_matches_reference_gold():
+    """Partial discounting (flows generated at a step, discounted log-linearly
+    to their payment time and by the numeraire ratio to an earlier target
+    step) against the reference's own cashFlowDiscountingPartial_gold, built
+    into oracle/_ref, on the gold's discount-ratio cube and numeraires, with
+    theta at both end points and in between.  The reference GPU kernel (and
+    the restatement) also divides by the step's own ratio P(s, s), which the
+    gold does not; the ratios are normalised so that P(s, s) = 1 exactly."""
+    g = load("lmm_gold_small.npz")
+    disc, num = g["disc"], g["num"]
+    S, _, P = disc.shape
+    assert np.all(disc[np.arange(S), np.arange(S), :] == 1.0)
+    rng = np.random.RandomState(4)
+    K = 8
+    gen = rng.randint(0, S - 2, K).astype(np.int32)
+    first, second = (gen + 1).astype(np.int32), (gen + 2).astype(np.int32)
+    theta = rng.uniform(0, 1, K)
+    theta[:3] = [0.0, 1.0, 0.5]
+    target = np.array([rng.randint(0, i + 1) for i in gen], np.int32)
+    flows = rng.normal(size=(K, P))
+    d, n = np.ascontiguousarray(disc).reshape(-1), np.ascontiguousarray(num).reshape(-1)
+    got, want = np.zeros(K * P), np.zeros(K * P)
+    O.lib().orc_cash_flow_discounting_partial(K, gen, first, second, theta, target, d,
+                                              flows.reshape(-1).copy(), n, P, S, got)
+    O.ref().ref_cash_flow_discounting_partial(K, gen, first, second, theta, target, d,
+                                              flows.reshape(-1).copy(), n, P, S, want)
+    assert np.allclose(got, want, rtol=1e-14, atol=1e-15)
+
+
 def test_compensated_sum_is_exact_on_cancellation():
     x = np.array([1e16, 1.0, -1e16, 1.0] * 1000)
     assert O.lib().orc_sum_compensated(x, x.size) == 2000.0

@@ -132,7 +132,9 @@ __global__ void __launch_bounds__(256)
 }
 
 // Partial discounting to targetIndices via the numeraire
-// (cashFlowDiscounting_gpu.cu:369-440).
+// (cashFlowDiscounting_gpu.cu:369-440; like that kernel, the discount factor
+// is also divided by the step's own ratio P(s, s), which the discount-ratio
+// cubes hold as 1).
 __global__ void __launch_bounds__(256)
     k_discounting_partial(int n_flows, const int* __restrict__ gen, const int* __restrict__ first,
                           const int* __restrict__ second, const float* __restrict__ theta,
