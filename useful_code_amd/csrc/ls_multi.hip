@@ -812,6 +812,13 @@ __global__ void __launch_bounds__(kSearchThreads)
         count = C(j);
       } else {
         guess /= m;
+        if (guess >= 0.0 && key_of(guess) >= ext.maxval && ext.nvalid < lowerT) {
+          // every later guess is larger, so the count stays at nvalid: the
+          // gold's loop runs out its steps (the same divisions, no counts)
+          while (it++ < kSelectMaxSteps) guess /= m;
+          count = ext.nvalid;
+          break;
+        }
         count = brute(guess);
       }
     }
@@ -838,6 +845,13 @@ __global__ void __launch_bounds__(kSearchThreads)
           count = C(L + i);
         } else {
           low *= m;
+          if (low >= 0.0 && key_of(low) < ext.minpos && ext.nzero > upperT) {
+            // every later cut is smaller, so the count stays at the zero
+            // count: the gold's loop runs out its steps (same products)
+            while (it++ < kSelectMaxSteps) low *= m;
+            count = ext.nzero;
+            break;
+          }
           count = brute(low);
         }
       }
