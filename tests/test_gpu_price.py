@@ -126,6 +126,30 @@ def test_greeks_vs_reference_gold_bumped_prices(torch_cuda):
     print(f"max relative difference to the gold bumped-price Greeks {worst:.2e}")
 
 
+def test_full_config2_price_and_greeks_vs_fp64_oracle(torch_cuda):
+    """The headline workload at its full size (2^20 Sobol paths, 40 rates,
+    5 factors, checkpoint every 8 steps): the GPU price within 1e-5 and every
+    delta and vega within 1e-4 relative of the FP64 oracle's pathwise adjoint
+    on the same 2^20 points (OpenMP, a few seconds on the host), and the
+    Greeks with a checkpoint at every step equal to those at every 8th."""
+    w = configs.caplet_portfolio(paths=1 << 20, checkpoint_every=8)
+    pr = pipeline.LMMPricer(w)
+    p_gpu, g = pr.price_and_greeks()
+    p_gpu, g = p_gpu.item(), g.cpu().numpy()
+    ints = O.scramble(O.sobol_ints(w.paths, w.dims, 0), w.scramblers, w.dims)
+    m = O.caplet_model(w.f0, w.taus, w.disp(), w.A, w.vols, w.strike, w.paths)
+    p64, d64, v64 = O.caplet_pipeline(m, ints, adjoint=True)
+    g64 = np.concatenate([d64, v64])
+    rel = np.abs(g - g64) / np.abs(g64)
+    print(f"2^20 paths: price gpu {p_gpu:.10f} fp64 {p64:.10f}; greeks max rel {rel.max():.2e}",
+          flush=True)
+    assert abs(p_gpu - p64) < 1e-5
+    assert np.all(rel < 1e-4)
+    w.checkpoint_every = 1
+    _, g1 = pipeline.LMMPricer(w).price_and_greeks()
+    assert np.allclose(g1.cpu().numpy(), g, rtol=2e-5, atol=0.0)
+
+
 def test_ragged_path_counts(torch_cuda):
     for P in (1, 63, 65, 1000):
         w = configs.caplet_portfolio(paths=P)
