@@ -499,6 +499,10 @@ __global__ void __launch_bounds__(kMultiThreads)
 // ---------------------------------------------------------------------------
 constexpr int kLadder = 64;                   // growth / shrink steps held
 constexpr int kLadderSlots = 256;             // >= 3 kLadder - 1
+static_assert(3 * kLadder - 1 <= kLadderSlots, "ladder slots");
+// ladder_build uses lanes 0 and 64; block sums use one LDS slot per wavefront
+static_assert(kSearchThreads >= 128 && kMultiThreads >= 128, "ladder needs 128 lanes");
+static_assert(kSearchThreads <= 1024 && kMultiThreads <= 1024, "16 wavefront slots");
 constexpr unsigned long long kEmpty = ~0ull;  // a cut that counts no path
 
 // Bisection hand-over from k_multi_select to k_multi_tree / k_multi_bisect.
