@@ -161,7 +161,7 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
   const size_t P = static_cast<size_t>(m.P);
   float* lam = lds + threadIdx.x;
   float* ctab = lds + static_cast<size_t>(N) * bd;
-  float* vslots = ctab + static_cast<size_t>(N) * N;
+  float* vslots = ctab + step_table_floats(N);
   float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 3);
   fill_fixed_drifts<F>(m, ctab);
   for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
@@ -340,9 +340,9 @@ __global__ void __launch_bounds__(256)
 
 int fwd_threads(int N) { return N <= 50 ? 256 : (N <= 75 ? 128 : 64); }
 int bwd_threads(int N) { return N <= 45 ? 256 : (N <= 71 ? 128 : 64); }
-size_t fwd_lds(int N, int t) { return sizeof(float) * (static_cast<size_t>(N) * t + N * N); }
+size_t fwd_lds(int N, int t) { return sizeof(float) * (static_cast<size_t>(N) * t + step_table_floats(N)); }
 size_t bwd_lds(int N, int t) {
-  return sizeof(float) * (static_cast<size_t>(N) * t + N * N + (t / 64) * N * kVegaSlots);
+  return sizeof(float) * (static_cast<size_t>(N) * t + step_table_floats(N) + (t / 64) * N * kVegaSlots);
 }
 
 int check_berm(const char* who, int paths, int rates, int factors, int k) {

@@ -17,6 +17,14 @@ constexpr int kMaxFactors = 8;
 __device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Drift coefficient g = tau (f + d) / (1 + tau f) of a displaced rate from
+// its log l = log(f + d): g = 1 / (1 + kr exp(-l)) with kr = 1/tau - d (one
+// multiply-add between the two hardware transcendentals).  Its derivative in
+// l is g (1 - g).
+__device__ __forceinline__ float drift_coeff(float l, float kr) {
+  return fast_rcp(1.0f + kr * fast_exp(-l));
+}
+
 __device__ __forceinline__ float dpp_f32_quad_xor1(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
 }
@@ -83,12 +91,24 @@ struct Model {
   int P, N, k;
 };
 
-// Fixed drifts c(s,r) = -0.5 sum_f A(s,r,f)^2 into LDS (read back with
-// uniform addresses); every lane of the block takes part, then a barrier.
+// The step table in LDS (read back with uniform addresses): rows 0..N-1 hold
+// the fixed drifts c(s,r) = -0.5 sum_f A(s,r,f)^2, row N the drift
+// coefficients' constants kr = 1/tau_r - d_r.  The steps are given row s and
+// find row N at N - s rows past it.
+__host__ __device__ constexpr size_t step_table_floats(int N) {
+  return static_cast<size_t>(N) * N + N;
+}
+
+// Every lane of the block takes part; the caller then synchronises.
 template <int F>
 __device__ __forceinline__ void fill_fixed_drifts(const Model& m, float* ctab) {
   const int nn = m.N * m.N;
-  for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+  for (int i = threadIdx.x; i < nn + m.N; i += blockDim.x) {
+    if (i >= nn) {
+      const int r = i - nn;
+      ctab[i] = 1.0f / m.taus[r] - m.disp[r];
+      continue;
+    }
     const float* a = m.A + static_cast<size_t>(i) * F;
     float c = 0.0f;
 #pragma unroll
@@ -101,7 +121,7 @@ __device__ __forceinline__ void fill_fixed_drifts(const Model& m, float* ctab) {
 // start log-rate, issued PF rates ahead of its use (PF = 2 for LDS sources; a
 // deeper queue hides HBM latency when the state comes from global memory),
 // `store(r, l)` receives the end log-rate; e / ep return the final drift
-// sums; cs = fixed drifts of step s.
+// sums; cs = row s of the step table (fixed drifts of step s).
 //
 // The rate loop is unrolled by PF and slot u of the queue always serves rates
 // with (r - s) % PF == u, so no queue register is ever copied: copying a
@@ -113,6 +133,7 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
                                         const float (&z)[F], float (&e)[F], float (&ep)[F],
                                         Load load, Store store) {
   const float* As = m.A + s * m.N * F;  // 32-bit: keeps the address scalar
+  const float* kt = cs + (m.N - s) * m.N;
   const int cnt = m.N - s;
 #pragma unroll
   for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
@@ -128,9 +149,8 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
       float a[F];
 #pragma unroll
       for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
-      const float tau = m.taus[r], d = m.disp[r];
-      const float x = fast_exp(l);
-      const float g = tau * x * fast_rcp(1.0f + tau * (x - d));
+      const float kr = kt[r];
+      const float g = drift_coeff(l, kr);
       float mu = 0.0f, dw = 0.0f;
 #pragma unroll
       for (int f = 0; f < F; ++f) {
@@ -139,8 +159,7 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
         dw += a[f] * z[f];
       }
       const float lh = l + mu + cs[r] + dw;
-      const float xh = fast_exp(lh);
-      const float gh = tau * xh * fast_rcp(1.0f + tau * (xh - d));
+      const float gh = drift_coeff(lh, kr);
       float muh = 0.0f;
 #pragma unroll
       for (int f = 0; f < F; ++f) {
@@ -157,7 +176,7 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
 // Reverse of PC step s over the rates N-1..s.  On entry lam[r * bd] (LDS)
 // holds the adjoint of the end-of-step log-rate r, on exit that of the
 // start-of-step one.  e / ep are the step's final drift sums (un-accumulated
-// here), z its variates, cs its fixed drifts; `src(r)` gives the start
+// here), z its variates, cs row s of the step table; `src(r)` gives the start
 // log-rate r (issued PF rates ahead).  `fixing(lp, x, f, 1/(1+tau f))` adds
 // the product's adjoint of the rate fixing at the end of the step (r == s) to
 // lp; `vega(r, w)` receives w = sum_f Abar(s,r,f) A(s,r,f) for the step, which
@@ -170,6 +189,7 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
                                              Vega vega) {
   const int N = m.N;
   const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
+  const float* kt = cs + (N - s) * N;
   const int cnt = N - s;
   float E[F], Eh[F];
 #pragma unroll
@@ -191,11 +211,8 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
       float a[F];
 #pragma unroll
       for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
-      const float tau = m.taus[r], d = m.disp[r];
-      const float omd = 1.0f - tau * d;
-      const float x = fast_exp(l);
-      const float inv1 = fast_rcp(1.0f + tau * (x - d));
-      const float g = tau * x * inv1;
+      const float kr = kt[r];
+      const float g = drift_coeff(l, kr);
       float mu = 0.0f, muh = 0.0f, dw = 0.0f;
 #pragma unroll
       for (int f = 0; f < F; ++f) {
@@ -205,12 +222,11 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
       }
       const float c = cs[r];
       const float lh = l + mu + c + dw;
-      const float xh = fast_exp(lh);
-      const float inv2 = fast_rcp(1.0f + tau * (xh - d));
-      const float gh = tau * xh * inv2;
+      const float gh = drift_coeff(lh, kr);
       if (r == s) {
+        const float tau = m.taus[s];
         const float xs = fast_exp(lh + 0.5f * (muh - mu));
-        const float fs = xs - d;
+        const float fs = xs - m.disp[s];
         fixing(lp, xs, fs, fast_rcp(1.0f + tau * fs));
       }
       float ghb = 0.0f;
@@ -219,7 +235,7 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
         Eh[f] += 0.5f * lp * a[f];
         ghb += a[f] * Eh[f];
       }
-      const float lhat = lp + ghb * gh * omd * inv2;
+      const float lhat = lp + ghb * (gh - gh * gh);  // dgh/dlh = gh (1 - gh)
       const float mub = lhat - 0.5f * lp;
       float gb = 0.0f;
 #pragma unroll
@@ -227,7 +243,7 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
         E[f] += mub * a[f];
         gb += a[f] * E[f];
       }
-      lam[r * bd] = lhat + gb * g * omd * inv1;
+      lam[r * bd] = lhat + gb * (g - g * g);
       vega(r, 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c));
 #pragma unroll
       for (int f = 0; f < F; ++f) {

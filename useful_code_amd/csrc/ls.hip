@@ -660,7 +660,7 @@ extern "C" int bermudan_paths_gpu(int paths, int rates, int factors, const float
   ex.last = exercise_steps[n_ex - 1];
   const int threads = berm_threads(rates);
   const int blocks = (paths + threads - 1) / threads;
-  const size_t lds = sizeof(float) * (static_cast<size_t>(rates) * threads + rates * rates);
+  const size_t lds = sizeof(float) * (static_cast<size_t>(rates) * threads + step_table_floats(rates));
   hipStream_t st = as_stream(stream);
   switch (factors) {
 #define KOO_CASE(F)                                                                             \
@@ -1216,7 +1216,7 @@ extern "C" int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan,
     const int nb = plan->class_blocks[c];
     if (nc < 1 || nc > plan->max_rates || nb < 1 || b0 < 0 || b0 + nb > plan->blocks)
       return set_error(KOO_ERR_INVALID, "%s: bad plan class %d", who, c);
-    const size_t lds = sizeof(float) * (static_cast<size_t>(nc) * 256 + nc * nc);
+    const size_t lds = sizeof(float) * (static_cast<size_t>(nc) * 256 + step_table_floats(nc));
     hipLaunchKernelGGL(k_berm_paths_batch, dim3(nb), dim3(256), lds, st, items, plan->products,
                        b0);
   }
