@@ -58,7 +58,7 @@ __device__ __forceinline__ void berm_fwd_block(const Model& m, const BermArgs& b
   float* L = lds + threadIdx.x;
   float* ctab = lds + static_cast<size_t>(m.N) * bd;
   const size_t P = static_cast<size_t>(m.P);
-  fill_fixed_drifts<F>(m, ctab);
+  fill_step_table<F>(m, ctab);
   for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
   __syncthreads();
   float invB = 1.0f, V = 0.0f;
@@ -163,7 +163,7 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
   float* ctab = lds + static_cast<size_t>(N) * bd;
   float* vslots = ctab + step_table_floats(N);
   float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 3);
-  fill_fixed_drifts<F>(m, ctab);
+  fill_step_table<F>(m, ctab);
   for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
   for (int i = threadIdx.x; i < nw * N * kVegaSlots; i += bd) vslots[i] = 0.0f;
   __syncthreads();

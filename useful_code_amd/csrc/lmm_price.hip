@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256)
   float* L = lds + threadIdx.x;
   float* ctab = lds + static_cast<size_t>(m.N) * bd;
   const size_t P = static_cast<size_t>(m.P);
-  fill_fixed_drifts<F>(m, ctab);
+  fill_step_table<F>(m, ctab);
   for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
   __syncthreads();
   float invB = 1.0f, y = 0.0f, yc = 0.0f;
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256)
   float* ctab = lds + static_cast<size_t>(N) * bd;           // [N][N]
   float* vslots = ctab + step_table_floats(N);         // [nw][N][8]
   float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 3);
-  fill_fixed_drifts<F>(m, ctab);
+  fill_step_table<F>(m, ctab);
   for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
   for (int i = threadIdx.x; i < nw * N * kVegaSlots; i += bd) vslots[i] = 0.0f;
   __syncthreads();

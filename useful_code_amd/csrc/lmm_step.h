@@ -101,7 +101,7 @@ __host__ __device__ constexpr size_t step_table_floats(int N) {
 
 // Every lane of the block takes part; the caller then synchronises.
 template <int F>
-__device__ __forceinline__ void fill_fixed_drifts(const Model& m, float* ctab) {
+__device__ __forceinline__ void fill_step_table(const Model& m, float* ctab) {
   const int nn = m.N * m.N;
   for (int i = threadIdx.x; i < nn + m.N; i += blockDim.x) {
     if (i >= nn) {

@@ -49,7 +49,7 @@ __device__ __forceinline__ void berm_paths_block(const Model& m, const ExMap& ex
   float* L = lds + threadIdx.x;
   float* ctab = lds + static_cast<size_t>(m.N) * bd;
   const size_t P = static_cast<size_t>(m.P);
-  fill_fixed_drifts<F>(m, ctab);
+  fill_step_table<F>(m, ctab);
   for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
   __syncthreads();
   float B = 1.0f;
