@@ -77,7 +77,7 @@ class CapletModel(ctypes.Structure):
 
 def _declare(L):
     sig = {
-        "orc_sobol_directions": (_c_int, [_c_int, _u32p]),
+        "orc_sobol_directions": (_c_int, [_c_int, _u32p, _u32p, _u32p, _c_int, _u32p]),
         "orc_sobol_ints": (None, [_c_int, _c_int, _c_u32, _u32p, _u32p]),
         "orc_scramble": (None, [_u32p, _u32p, _c_int, _c_int]),
         "orc_invnorm_sb_ref_f32": (ctypes.c_float, [_c_u32]),
@@ -166,6 +166,7 @@ def _declare(L):
 def _declare_ref(R):
     sig = {
         "ref_sobol_directions": (None, [_c_int, _u32p]),
+        "ref_sobol_primitives": (None, [_c_int, _u32p, _u32p, _u32p]),
         "ref_sobol_ints_offset": (None, [_c_int, _c_int, _c_int, _u32p, _u32p]),
         "ref_scramble": (None, [_u32p, _u32p, _u32p, _c_int, _c_int]),
         "ref_invnorm_sb": (ctypes.c_float, [_c_u32]),
@@ -217,10 +218,29 @@ def _declare_ref(R):
 
 # ---------------------------------------------------------------- wrappers
 
-def sobol_directions(n_dims):
-    d = np.zeros(n_dims * 32, np.uint32)
-    if lib().orc_sobol_directions(n_dims, d) != 0:
+_PRIMITIVES = None
+
+
+def sobol_primitives():
+    """The joe-kuo-6.10200 (degree, a, m[17]) rows read out of the reference's
+    compiled gold/sobol_primitives.cpp by tests/golden/make_golden.py."""
+    global _PRIMITIVES
+    if _PRIMITIVES is None:
+        g = np.load(os.path.join(_HERE, "..", "tests", "golden", "sobol_primitives_ref.npz"))
+        _PRIMITIVES = (np.ascontiguousarray(g["degree"], np.uint32),
+                       np.ascontiguousarray(g["a"], np.uint32),
+                       np.ascontiguousarray(g["m"], np.uint32))
+    return _PRIMITIVES
+
+
+def sobol_directions(n_dims, primitives=None):
+    deg, a, m = sobol_primitives() if primitives is None else primitives
+    if not 1 <= n_dims <= len(deg):
         raise ValueError("n_dims out of table range")
+    d = np.zeros(n_dims * 32, np.uint32)
+    if lib().orc_sobol_directions(n_dims, deg, a, np.ascontiguousarray(m.reshape(-1)), m.shape[1],
+                                  d) != 0:
+        raise ValueError("bad primitive rows")
     return d
 
 

@@ -18,31 +18,29 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* The Joe-Kuo (degree, a, m[]) table: data shared with the product. */
-struct koo_sobol_primitive {
-  unsigned degree;
-  unsigned a;
-  unsigned m[18];
-};
-#include "../useful_code_amd/csrc/sobol_joe_kuo.inc"
 
 /* ======================================================================
  * Stage (a)
  * ==================================================================== */
 
-/* Bratley-Fox direction numbers, 32 per dimension, Q0.32.
+/* Bratley-Fox direction numbers, 32 per dimension, Q0.32, from the Joe-Kuo
+ * (degree, a, m[]) rows passed in (degree[d], a[d], m[d*max_m ..]).  The rows
+ * are the caller's: the tests pass the table read out of the reference's own
+ * compiled gold/sobol_primitives.cpp.
  * Follows gold/sobol_gold.cpp:118-164 (initSobolDirectionVectors). */
-int orc_sobol_directions(int n_dims, uint32_t *dirs) {
-  if (n_dims < 1 || n_dims > KOO_SOBOL_TABLE_DIMS) return -1;
+int orc_sobol_directions(int n_dims, const uint32_t *degree, const uint32_t *a_bits,
+                         const uint32_t *m, int max_m, uint32_t *dirs) {
+  if (n_dims < 1) return -1;
   for (int d = 0; d < n_dims; ++d) {
     uint32_t *v = dirs + 32 * d;
     if (d == 0) {
       for (int i = 0; i < 32; ++i) v[i] = 1u << (31 - i);
       continue;
     }
-    int deg = (int)koo_sobol_table[d].degree;
-    uint32_t a = koo_sobol_table[d].a;
-    for (int i = 0; i < deg && i < 32; ++i) v[i] = koo_sobol_table[d].m[i] << (31 - i);
+    int deg = (int)degree[d];
+    uint32_t a = a_bits[d];
+    if (deg < 1 || deg > max_m) return -1;
+    for (int i = 0; i < deg; ++i) v[i] = m[(size_t)d * max_m + i] << (31 - i);
     for (int i = deg; i < 32; ++i) {
       uint32_t x = v[i - deg] ^ (v[i - deg] >> deg);
       for (int j = 1; j < deg; ++j)

@@ -26,7 +26,9 @@ extern "C" {
 #endif
 
 /* ---- stage (a): Sobol, scramble, inverse normal ---------------------- */
-int orc_sobol_directions(int n_dims, uint32_t *dirs /* [n_dims][32] */);
+int orc_sobol_directions(int n_dims, const uint32_t *degree, const uint32_t *a_bits,
+                         const uint32_t *m /* [n_dims][max_m] */, int max_m,
+                         uint32_t *dirs /* [n_dims][32] */);
 void orc_sobol_ints(int n_vectors, int n_dims, uint32_t path_offset,
                     const uint32_t *dirs, uint32_t *out /* [n_dims][n_vectors] */);
 void orc_scramble(uint32_t *data, const uint32_t *scramblers, int number_dim, int paths);

@@ -27,6 +27,7 @@
 #include <gold/math/Normals_gold.h>
 #include <gold/scramble_gold.h>
 #include <gold/sobol_gold.h>
+#include <gold/sobol_primitives.h>
 #include <gold/volstructs_gold.h>
 #include <gold/weighted_average_gold.h>
 
@@ -37,6 +38,16 @@ extern "C" {
 
 // gold/sobol_gold.cpp:118 initSobolDirectionVectors
 void ref_sobol_directions(int n_dims, unsigned* dirs) { initSobolDirectionVectors(n_dims, dirs); }
+
+// gold/sobol_primitives.cpp:71 sobol_primitives[] (the compiled joe-kuo-6.10200
+// table): degree, a and the max_m m values of the first n rows.
+void ref_sobol_primitives(int n, unsigned* degree, unsigned* a, unsigned* m) {
+  for (int d = 0; d < n; ++d) {
+    degree[d] = sobol_primitives[d].degree;
+    a[d] = sobol_primitives[d].a;
+    for (int i = 0; i < max_m; ++i) m[(size_t)d * max_m + i] = sobol_primitives[d].m[i];
+  }
+}
 
 // gold/sobol_gold.cpp:191 sobolCPUintsOffset
 void ref_sobol_ints_offset(int n_vectors, int n_dims, int offset, unsigned* dirs, unsigned* out) {

@@ -19,11 +19,41 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 import oracle  # noqa: E402
 
 
+SOBOL_TABLE_DIMS, SOBOL_MAX_M = 10200, 17
+
+
+def sobol_dims_fold(dirs):
+    """Two 32-bit digests per dimension of a [D][32] direction-number array:
+    the sum of (2k+1)*v[k] and the XOR of v[k] rotated left by k (mod 2^32).
+    Used by the tests to pin every dimension of the table without storing
+    all 10 200 x 32 words."""
+    v = np.asarray(dirs, np.uint64).reshape(-1, 32)
+    k = np.arange(32, dtype=np.uint64)
+    s = ((v * (2 * k + 1)) & 0xFFFFFFFF).sum(axis=1) & 0xFFFFFFFF
+    rot = ((v << k) | (v >> ((32 - k) % 32))) & 0xFFFFFFFF
+    x = np.bitwise_xor.reduce(rot, axis=1)
+    return np.stack([s, x], axis=1).astype(np.uint32)
+
+
+def sobol_primitives(R):
+    n = SOBOL_TABLE_DIMS
+    deg, a = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    m = np.zeros(n * SOBOL_MAX_M, np.uint32)
+    R.ref_sobol_primitives(n, deg, a, m)
+    return dict(degree=deg, a=a, m=m.reshape(n, SOBOL_MAX_M))
+
+
 def sobol(R):
     D, P = 200, 64
-    dirs = np.zeros(210 * 32, np.uint32)
-    R.ref_sobol_directions(210, dirs)
-    out = {"dirs210": dirs}
+    dirs_all = np.zeros(SOBOL_TABLE_DIMS * 32, np.uint32)
+    R.ref_sobol_directions(SOBOL_TABLE_DIMS, dirs_all)
+    dirs = dirs_all[: 210 * 32].copy()
+    out = {"dirs210": dirs, "dirs2048": dirs_all[: 2048 * 32].copy(),
+           "dirs_fold_all": sobol_dims_fold(dirs_all)}
+    # 2048 dimensions (the 32-asset, 64-step basket) at a non-zero offset
+    o = np.zeros(2048 * 16, np.uint32)
+    R.ref_sobol_ints_offset(16, 2048, 4093, dirs_all[: 2048 * 32].copy(), o)
+    out["ints_d2048_off4093"] = o.reshape(2048, 16)
     for off in (0, 1000, 65535):
         o = np.zeros(D * P, np.uint32)
         R.ref_sobol_ints_offset(P, D, off, dirs[: D * 32].copy(), o)
@@ -263,10 +293,18 @@ def pseudo_roots(R):
     return dict(rate_times=rate_times, vols=vols, A=A.reshape(N, N, F))
 
 
-def main():
+def main(only=None):
+    """Write every fixture, or only those named (file stems) in ``only``."""
     oracle.build(ref=True)
     R = oracle.ref()
-    np.savez_compressed(os.path.join(HERE, "sobol_ref.npz"), **sobol(R))
+    want = (lambda name: True) if not only else (lambda name: name in only)
+    if want("sobol_primitives_ref"):
+        np.savez_compressed(os.path.join(HERE, "sobol_primitives_ref.npz"), **sobol_primitives(R))
+    if want("sobol_ref"):
+        np.savez_compressed(os.path.join(HERE, "sobol_ref.npz"), **sobol(R))
+    if only:
+        print("golden fixtures written to", HERE, sorted(only))
+        return
     np.savez_compressed(os.path.join(HERE, "invnorm_ref.npz"), **invnorm(R))
     g = lmm(R)
     np.savez_compressed(os.path.join(HERE, "lmm_gold_small.npz"), **g)
@@ -280,4 +318,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]) or None)

@@ -31,6 +31,21 @@ def test_sobol_ints_bit_exact(torch_cuda, P, D, off):
     assert np.array_equal(host_u32(out).reshape(D, P), O.sobol_ints(P, D, off))
 
 
+def test_sobol_ints_2048_dims_match_reference_gold(torch_cuda):
+    # The basket workload's 2048 dimensions at a non-zero offset against the
+    # integers the reference's own gold generator wrote (sobolCPUintsOffset).
+    import os
+    torch = torch_cuda
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sobol_ref.npz"))
+    D, P = 2048, 16
+    print(f"sobolGPUInts P={P} D={D} offset=4093 vs reference gold", flush=True)
+    dirs = dev_u32(torch, koo.sobol_directions(D))
+    out = torch.empty(P * D, dtype=torch.int32, device="cuda")
+    koo.sobolGPUInts(P, D, 4093, dirs, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(host_u32(out).reshape(D, P), g["ints_d2048_off4093"])
+
+
 def test_sobol_split_batches(torch_cuda):
     torch = torch_cuda
     D = 32

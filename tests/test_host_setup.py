@@ -12,9 +12,11 @@ G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def test_direction_numbers_match_reference_gold():
+    from test_oracle_golden import sobol_dims_fold
     g = np.load(os.path.join(G, "sobol_ref.npz"))
     assert np.array_equal(koo.sobol_directions(210), g["dirs210"])
-    assert np.array_equal(koo.sobol_directions(2048), O.sobol_directions(2048))
+    assert np.array_equal(koo.sobol_directions(2048), g["dirs2048"])
+    assert np.array_equal(sobol_dims_fold(koo.sobol_directions(10200)), g["dirs_fold_all"])
 
 
 def test_bridge_allocation_matches_oracle():

@@ -18,9 +18,51 @@ def load(name):
 
 # ---------------------------------------------------------------- stage (a)
 
+def sobol_dims_fold(dirs):
+    """Per-dimension digests of a [D][32] direction array, the same two
+    folds tests/golden/make_golden.py stores for the reference's table."""
+    v = np.asarray(dirs, np.uint64).reshape(-1, 32)
+    k = np.arange(32, dtype=np.uint64)
+    s = ((v * (2 * k + 1)) & 0xFFFFFFFF).sum(axis=1) & 0xFFFFFFFF
+    rot = ((v << k) | (v >> ((32 - k) % 32))) & 0xFFFFFFFF
+    return np.stack([s, np.bitwise_xor.reduce(rot, axis=1)], axis=1).astype(np.uint32)
+
+
 def test_sobol_directions_match_reference_gold():
     g = load("sobol_ref.npz")
     assert np.array_equal(O.sobol_directions(210), g["dirs210"])
+    # 2048 dimensions (the basket workload) word for word, and every one of
+    # the reference's 10 200 dimensions through its digest
+    assert np.array_equal(O.sobol_directions(2048), g["dirs2048"])
+    assert np.array_equal(sobol_dims_fold(O.sobol_directions(10200)), g["dirs_fold_all"])
+
+
+def test_sobol_dims_fold_detects_one_flipped_bit():
+    g = load("sobol_ref.npz")
+    d = g["dirs2048"].copy()
+    d[32 * 1500 + 20] ^= 1 << 7
+    f = sobol_dims_fold(d)
+    assert not np.array_equal(f[1500], g["dirs_fold_all"][1500])
+    assert np.array_equal(np.delete(f, 1500, 0), np.delete(g["dirs_fold_all"][:2048], 1500, 0))
+
+
+def test_sobol_ints_2048_dims_match_reference_gold():
+    g = load("sobol_ref.npz")
+    assert np.array_equal(O.sobol_ints(16, 2048, 4093), g["ints_d2048_off4093"])
+
+
+def test_sobol_table_is_joe_kuo_10200_not_21201():
+    # The two published Joe-Kuo sets agree up to dimension 210 and part there;
+    # SciPy ships new-joe-kuo-6.21201, the reference joe-kuo-6.10200.
+    from scipy.stats import qmc
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pts = qmc.Sobol(400, scramble=False, bits=32).random(65)
+    sp = (pts * 2.0**32).astype(np.uint64).astype(np.uint32)[1:].T
+    ours = O.sobol_ints(64, 400, 0)
+    assert np.array_equal(ours[:211], sp[:211])
+    assert not np.array_equal(ours[211:], sp[211:])
 
 
 @pytest.mark.parametrize("off", [0, 1000, 65535])

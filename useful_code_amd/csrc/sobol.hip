@@ -12,11 +12,12 @@
 
 #include "koo_common.h"
 
-// Joe-Kuo (degree, a, m[]) rows, one per dimension (host data).
+// Joe-Kuo (degree, a, m[]) rows, one per dimension (host data); degree <= 17
+// (max_m, gold/sobol_primitives.h:67).
 struct koo_sobol_primitive {
   unsigned degree;
   unsigned a;
-  unsigned m[18];
+  unsigned m[17];
 };
 #include "sobol_joe_kuo.inc"
 
