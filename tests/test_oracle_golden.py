@@ -157,6 +157,129 @@ def test_bridge_increments_are_orthonormal(steps):
     assert np.isclose(M[:, 0].sum(), np.sqrt(steps))
 
 
+def bisection_bridge(S):
+    """The Brownian bridge on t_i = i + 1 (i = 0..S-1, W(0) = 0) built by
+    level-order bisection, written from the construction's definition rather
+    than from any table code: the first variate sets the terminal point;
+    then every gap of still-unset points that exists at the start of a sweep
+    is halved once, left to right, at its lower middle point, before any gap
+    made by that sweep is touched (a FIFO queue of gaps).  A gap is the
+    half-open run [lo, hi) of unset points whose right neighbour hi is set
+    and whose left neighbour is lo - 1 (or the origin when lo = 0).
+
+    Each point's conditional law given its two neighbours at times a < t < b
+    is the Brownian-bridge law N(((b - t) W(a) + (t - a) W(b)) / (b - a),
+    (t - a)(b - t) / (b - a)), which gives the weights and sigma.
+    Returns (bridge, left, right, lw, rw, sd) with the reference's meaning of
+    left (= lo, the first unset point of the gap) and right (= hi)."""
+    from collections import deque
+    bridge, left, right = [S - 1], [0], [0]
+    lw, rw, sd = [0.0], [0.0], [np.sqrt(float(S))]
+    gaps = deque([(0, S - 1)])
+    while gaps:
+        lo, hi = gaps.popleft()
+        if lo >= hi:
+            continue
+        l = (lo + hi - 1) // 2
+        a, t, b = float(lo), float(l + 1), float(hi + 1)   # times of lo-1, l, hi
+        bridge.append(l)
+        left.append(lo)
+        right.append(hi)
+        lw.append((b - t) / (b - a))
+        rw.append((t - a) / (b - a))
+        sd.append(np.sqrt((t - a) * (b - t) / (b - a)))
+        gaps.append((lo, l))
+        gaps.append((l + 1, hi))
+    assert len(bridge) == S
+    return (np.array(bridge), np.array(left), np.array(right), np.array(lw), np.array(rw),
+            np.array(sd))
+
+
+def bisection_bridge_matrix(S):
+    """Increments = M z for the construction above, applied by hand."""
+    bridge, left, right, lw, rw, sd = bisection_bridge(S)
+    M = np.zeros((S, S))
+    W = np.zeros((S, S))           # W[point] as a row over variates
+    W[S - 1, 0] = sd[0]
+    for i in range(1, S):
+        l, lo, hi = bridge[i], left[i], right[i]
+        W[l] = rw[i] * W[hi] + (lw[i] * W[lo - 1] if lo > 0 else 0.0)
+        W[l, i] += sd[i]
+    M[0] = W[0]
+    M[1:] = W[1:] - W[:-1]
+    return M
+
+
+def test_bisection_bridge_hand_example():
+    # S = 8 on t = 1..8: the terminal point 7; the gap [0, 7) halves at 3;
+    # the sweep then halves [0, 3) at 1 and [4, 7) at 5; the last sweep fills
+    # 0, 2, 4, 6.  A depth-first order (7, 3, 1, 0, 2, 5, 4, 6) is a
+    # different bridge and must not be produced.
+    b, l, r, lw, rw, sd = bisection_bridge(8)
+    assert list(b) == [7, 3, 1, 5, 0, 2, 4, 6]
+    assert list(l[1:]) == [0, 0, 4, 0, 2, 4, 6] and list(r[1:]) == [7, 3, 7, 1, 3, 5, 7]
+    # point 3 (t = 4) between the origin and W(8): mean W(8)/2, variance 4*4/8
+    assert rw[1] == 0.5 and sd[1] == np.sqrt(2.0)
+    # point 5 (t = 6) between t = 4 and t = 8: weights 1/2, 1/2, variance 1
+    assert lw[3] == 0.5 and rw[3] == 0.5 and sd[3] == 1.0
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 5, 8, 20, 40, 63, 64])
+def test_bridge_tables_follow_the_bisection_construction(steps):
+    # The oracle's tables (a restatement of brownian_bridge.cu:639-700)
+    # against the construction derived above: same order and brackets
+    # exactly, weights and sigma to rounding.
+    bi, li, ri, lw, rw, sd = O.bridge_tables(steps)
+    b, l, r, lw0, rw0, sd0 = bisection_bridge(steps)
+    assert np.array_equal(bi, b)
+    assert np.array_equal(li[1:], l[1:]) and np.array_equal(ri[1:], r[1:])
+    assert np.allclose(lw[1:], lw0[1:], rtol=1e-15, atol=0)
+    assert np.allclose(rw, rw0, rtol=1e-15, atol=0)
+    assert np.allclose(sd, sd0, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("steps,factors", [(5, 1), (8, 1), (20, 3), (40, 5), (64, 2)])
+def test_bridge_transform_equals_bisection_matrix(steps, factors):
+    # The oracle's transform (brownian_bridge.cu:401-456 with the diagonal
+    # allocation) against the hand-applied construction: factor f's step-i
+    # variate is input row indices[f, i].
+    rng = np.random.RandomState(steps * 7 + factors)
+    P = 9
+    z = rng.standard_normal((steps * factors, P))
+    idx = O.bridge_allocate_diagonal(factors, steps).reshape(factors, steps)
+    out = O.bridge_transform(z, steps, factors)
+    M = bisection_bridge_matrix(steps)
+    for f in range(factors):
+        assert np.allclose(out[:, f, :], M @ z[idx[f]], rtol=0, atol=1e-13)
+
+
+def test_bisection_construction_would_catch_a_depth_first_order():
+    # A depth-first bridge (a plausible mis-ordering) is an equally valid
+    # Brownian bridge (M M^T = I) but a different matrix: the invariant test
+    # above cannot see it, this comparison does.
+    S = 8
+    M = bisection_bridge_matrix(S)
+    order, stack = [S - 1], [(0, S - 1)]
+    Wd = np.zeros((S, S))
+    Wd[S - 1, 0] = np.sqrt(S)
+    while stack:
+        lo, hi = stack.pop(0)
+        if lo >= hi:
+            continue
+        l = (lo + hi - 1) // 2
+        a, t, b = float(lo), float(l + 1), float(hi + 1)
+        Wd[l] = (t - a) / (b - a) * Wd[hi] + ((b - t) / (b - a) * Wd[lo - 1] if lo > 0 else 0)
+        Wd[l, len(order)] += np.sqrt((t - a) * (b - t) / (b - a))
+        order.append(l)
+        stack[:0] = [(lo, l), (l + 1, hi)]          # depth first
+    assert order == [7, 3, 1, 0, 2, 5, 4, 6]
+    Md = np.vstack([Wd[:1], Wd[1:] - Wd[:-1]])
+    assert np.allclose(Md @ Md.T, np.eye(S), atol=1e-13)
+    assert not np.allclose(Md, M, atol=1e-3)
+    out = O.bridge_transform(np.eye(S), S, 1, indices=np.arange(S, dtype=np.int32))[:, 0, :]
+    assert np.allclose(out, M, atol=1e-13) and not np.allclose(out, Md, atol=1e-3)
+
+
 def test_correlate_drift_matches_reference_gold():
     g = load("lmm_gold_small.npz")
     N, F, P = int(g["N"]), int(g["F"]), int(g["P"])
