@@ -516,6 +516,24 @@ def test_caplet_adjoint_matches_reference_gold_bumped_prices():
         assert abs(fv - v[i] / w.paths) <= 1e-8 * abs(fv), (i, fv, v[i] / w.paths)
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="reference gold library not built")
+def test_caplet_gold_bumped_fixture_matches_gold_chain():
+    # The stored gold bumped-price Greeks (read by the GPU test) are what the
+    # gold chain computes now, and agree with the oracle's adjoint.
+    from gold_chain import gold_caplet_price
+    from useful_code_amd import configs
+    g = load("caplet_gold_bumped.npz")
+    w = configs.caplet_portfolio(paths=int(g["paths"]))
+    ints = O.scramble(O.sobol_ints(w.paths, w.dims, 0), w.scramblers, w.dims)
+    z = O.bridge_transform(O.invnorm_u32(ints), w.steps, w.factors)
+    assert gold_caplet_price(w, z) == float(g["price"])
+    m = O.caplet_model(w.f0, w.taus, w.disp(), w.A, w.vols, w.strike, w.paths)
+    _, d, v = O.caplet_adjoint_paths(m, z)
+    r = g["rates"]
+    assert np.allclose(d[r] / w.paths, g["fd_delta"], rtol=1e-8, atol=0)
+    assert np.allclose(v[r] / w.paths, g["fd_vega"], rtol=1e-8, atol=0)
+
+
 def test_caplet_price_near_black76():
     # 2^14 scrambled-free Sobol paths: MC + discretisation error well below 1%.
     from scipy.stats import norm
