@@ -474,6 +474,75 @@ def stage_section(args, torch, configs, pipeline, ev):
     return res
 
 
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Cores this process may run on (its affinity mask), capped by
+    OMP_NUM_THREADS when that is set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_baseline_section(args, O, configs, S):
+    """The FP64 oracle on the host cores, same workload and Sobol points:
+    (1) one thread on a 2^16-path prefix, scaled linearly in paths to the
+    full path count (the oracle's cost per path does not depend on P);
+    (2) OpenMP over paths on every core of the affinity mask at the sample
+    size (default the full 2^20 paths)."""
+    cores = host_cores()
+
+    def run(paths, threads):
+        wc = configs.caplet_portfolio(paths=paths)
+        ints = O.scramble(O.sobol_ints(wc.paths, wc.dims, 0), wc.scramblers, wc.dims)
+        m = O.caplet_model(wc.f0, wc.taus, wc.disp(), wc.A, wc.vols, wc.strike, wc.paths)
+        O.set_threads(threads)
+        t = time.perf_counter()
+        O.caplet_pipeline(m, ints, adjoint=True)
+        return time.perf_counter() - t
+
+    p1 = min(1 << 16, args.cpu_sample_paths)
+    dt1 = run(p1, 1)
+    dtn = run(args.cpu_sample_paths, cores)
+    O.set_threads(cores)
+    return {
+        "value": args.cpu_sample_paths * S / dtn,
+        "unit": "path-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "cpu_model": host_cpu_model(),
+        "compiler": "gcc -O3 -fopenmp (oracle/Makefile)",
+        "sample": f"FP64 oracle (oracle/koo_oracle.c, OpenMP over paths on {cores} cores of the "
+                  f"affinity mask): quantiles, bridge, evolution, caplets and all 80 adjoint "
+                  f"Greeks of the same workload on {args.cpu_sample_paths} paths (the Sobol "
+                  f"integers generated beforehand, not timed): {dtn:.2f} s",
+        "single_thread": {
+            "value": p1 * S / dt1,
+            "unit": "path-steps/s",
+            "cores": 1,
+            "paths": p1,
+            "seconds": dt1,
+            "seconds_scaled_to_full_paths": dt1 * args.cpu_sample_paths / p1,
+            "scaling": f"linear in paths: {p1} timed, x{args.cpu_sample_paths // p1} for "
+                       f"{args.cpu_sample_paths}",
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -660,23 +729,7 @@ def main():
         out["error_sample"] = "first 2^14 paths of the workload's Sobol sequence"
 
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        wc = configs.caplet_portfolio(paths=args.cpu_sample_paths)
-        ints = O.scramble(O.sobol_ints(wc.paths, wc.dims, 0), wc.scramblers, wc.dims)
-        m = O.caplet_model(wc.f0, wc.taus, wc.disp(), wc.A, wc.vols, wc.strike, wc.paths)
-        t = time.perf_counter()
-        O.caplet_pipeline(m, ints, adjoint=True)
-        dt = time.perf_counter() - t
-        out["cpu_baseline"] = {
-            "value": wc.paths * S / dt,
-            "unit": "path-steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"FP64 oracle (oracle/koo_oracle.c, OpenMP over paths, gcc -O3): "
-                      f"quantiles, bridge, evolution, caplets and all 80 adjoint Greeks of the "
-                      f"same workload on {wc.paths} paths (the Sobol integers generated "
-                      f"beforehand): {dt:.2f} s",
-        }
+        out["cpu_baseline"] = cpu_baseline_section(args, O, configs, S)
 
     if not args.no_bermudan:
         out["bermudan_ls"] = bermudan_section(args, torch, O, configs, pipeline, ev)

@@ -134,6 +134,8 @@ def _declare(L):
         "orc_caplet_price_paths": (None, [ctypes.POINTER(CapletModel), _f64p, _f64p]),
         "orc_caplet_adjoint_paths": (None, [ctypes.POINTER(CapletModel), _f64p, _f64p, _f64p,
                                             _f64p]),
+        "orc_set_threads": (None, [_c_int]),
+        "orc_get_threads": (_c_int, []),
         "orc_caplet_pipeline": (None, [ctypes.POINTER(CapletModel), _u32p, _c_int,
                                        ctypes.POINTER(_c_dbl), _f64p, _f64p]),
         "orc_basis_count": (_c_int, [_c_int, _c_int]),
@@ -508,6 +510,15 @@ def caplet_adjoint_paths(model, z):
     v = np.zeros(model.n_rates)
     lib().orc_caplet_adjoint_paths(ctypes.byref(model), z.reshape(-1), pp, d, v)
     return pp, d, v
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
 
 
 def caplet_pipeline(model, ints, adjoint=False):

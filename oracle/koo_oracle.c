@@ -15,6 +15,7 @@
 #include "koo_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1998,3 +1999,8 @@ void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strat
     free(erec);
   }
 }
+
+/* Thread count of the OpenMP loops above (the CPU baseline times the oracle
+ * on one core and on every core the process may run on). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }

@@ -191,6 +191,9 @@ void orc_basket_weightings(const double *in /* (S,N,P) */, double *out /* (S,P) 
                            const double *w, int paths, int stocks, int steps);
 void orc_basket_asian_paths(const orc_basket_model *m, const double *z /* (S,A,P) */,
                             double *price_per_path, double *delta_sum, double *vega_sum);
+/* OpenMP thread count used by the parallel oracle loops (CPU baseline timing). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 #ifdef __cplusplus
 }
 #endif
