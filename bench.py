@@ -37,9 +37,21 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 
 
 def flops_per_rate_update(F):
-    """Algorithmic flops of one rate update (FMA = 2), counted from the
-    kernels' arithmetic: forward PC step 13F + 20; reverse step 21F + 40."""
-    return 13 * F + 20, 21 * F + 40
+    """Algorithmic flops of one rate update (one path, step and alive rate),
+    an FMA counted as 2 and a hardware exp or reciprocal as 1, counted from
+    the arithmetic of the shared step code (lmm_step.h):
+      forward PC step (pc_step): 5F FMA (e, mu, dW, e^, mu^ sums), 3 MUL,
+        4 ADD, 4 transcendental (exp and reciprocal of g and g^) = 10F + 11;
+      reverse step (reverse_step): 9F + 6 FMA (mu, mu^, dW recomputed, the
+        adjoint sums E^, E and their contractions, e and e^ un-accumulated,
+        the lambda, lambda^ and vega terms), 7 MUL, 7 ADD (three of them the
+        DPP fold of the vega term), 4 transcendental = 18F + 30.
+    At F = 5: 61 + 120 = 181 for the backward kernel (segment recompute plus
+    reverse).  The SQ counters of k_price_bwd<5> read 195 executed FP32 flops
+    per update (85.1 FMA, 6.6 MUL, 10.1 ADD, 8.2 transcendental): the
+    difference is per-step work spread over the rates and multiply-adds the
+    compiler forms from a MUL and an ADD."""
+    return 10 * F + 11, 18 * F + 30
 
 
 def rate_updates_per_path(N):
@@ -164,7 +176,9 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
         m = O.caplet_model(w.f0, w.taus, w.disp(), w.A, w.vols, w.strike, P)
         t = time.perf_counter()
         bv, evo, num = O.bermudan_paths(m, z, w.exercise_steps)
-        p64, _ = O.ls_first_pass(bv, np.full(E, 2, np.int32), np.zeros_like(evo), evo, num,
+        # the variables as the GPU regresses on them, (x - shift) * scale
+        bvn = (bv - w.basis_shift[:, :, None]) * w.basis_scale[:, :, None]
+        p64, _ = O.ls_first_pass(bvn, np.full(E, 2, np.int32), np.zeros_like(evo), evo, num,
                                  itm_only=True)
         dt = time.perf_counter() - t
         res["price_abs_err_vs_fp64"] = abs(price1 - p64)
@@ -429,6 +443,13 @@ def stage_section(args, torch, configs, pipeline, ev):
     out64 = torch.empty(1, dtype=torch.float64, device="cuda")
     U = N * (N + 1) // 2  # alive rate updates per path
     four = 4.0
+    # the LS normal equations of 6 basis functions and a target over P paths
+    # (the reference's reduceOuterProductSymmetric_gpu on [X; y])
+    kb = 6
+    basis = torch.empty(kb * P, dtype=torch.float32, device="cuda").normal_()
+    target = torch.empty(P, dtype=torch.float32, device="cuda").normal_()
+    lsws = torch.empty(koo.ls_workspace(P), dtype=torch.uint8, device="cuda")
+    gram = torch.empty((kb + 1) ** 2, dtype=torch.float64, device="cuda")
     stages = [
         ("sobolGPUInts", lambda: koo.sobolGPUInts(P, D, 0, dirs, ints), four * D * P),
         ("scramble_gpu (all D rows)", lambda: koo.scramble_gpu(ints, scr, D, P), 2 * four * D * P),
@@ -457,6 +478,8 @@ def stage_section(args, torch, configs, pipeline, ev):
          four * (5 * S * P + P)),
         ("koo_sum_f32_to_f64 (S*P values)", lambda: koo.sum_f32_to_f64(dflows, ws, out64),
          four * S * P),
+        ("ls_normal_eq_gpu (X^T X and X^T y, 6 basis functions + target, MFMA)",
+         lambda: koo.ls_normal_eq_gpu(P, kb, basis, target, None, lsws, gram), four * (kb + 1) * P),
     ]
     res = {}
     for name, fn, nbytes in stages:

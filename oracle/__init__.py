@@ -93,6 +93,7 @@ def _declare(L):
                                              _c_dbl, _c_dbl,
                                              _c_dbl, _c_dbl, _f64p, _f64p, _f64p, _f64p, _f64p]),
         "orc_sym_solve_truncated": (_c_int, [_c_int, _f64p, _f64p, _c_dbl, _f64p]),
+        "orc_sym_solve_cut": (_c_int, [_c_int, _f64p, _f64p, _c_dbl, _f64p]),
         "orc_select_sds": (ctypes.c_long, [_f64p, _i32p, _i32p, _c_int, ctypes.c_long, _c_dbl,
                                            _f64p, _f64p]),
         "orc_select_fraction": (ctypes.c_long, [_f64p, _i32p, _i32p, _c_int, ctypes.c_long,
@@ -151,6 +152,9 @@ def _declare(L):
         "orc_bermudan_frozen_paths": (None, [ctypes.POINTER(CapletModel),
                                              ctypes.POINTER(FrozenStrategy), _f64p, _c_int, _f64p,
                                              ctypes.c_void_p, _f64p, _f64p]),
+        "orc_bermudan_given_paths": (None, [ctypes.POINTER(CapletModel),
+                                            ctypes.POINTER(FrozenStrategy), _f64p, _i32p, _c_int,
+                                            _f64p, _f64p, _f64p]),
         "orc_correlated_paths": (None, [_f64p, _f64p, _f64p, _i32p, _c_int, _c_int, _c_int,
                                         _c_int, _c_int]),
         "orc_multid_bs_paths": (None, [_f64p, _f64p, ctypes.c_void_p, _f64p, _c_int, _c_int,
@@ -552,6 +556,16 @@ def cholesky_solve(A, b):
     return x if st == 0 else None
 
 
+def sym_solve_cut(A, b, tol):
+    """Cholesky when a trace bound shows no eigenvalue below tol times the
+    largest, else the truncated eigen-solve (the LS passes' solve)."""
+    A = np.ascontiguousarray(A, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(len(b))
+    lib().orc_sym_solve_cut(len(b), A.reshape(-1), b, float(tol), x)
+    return x
+
+
 def sym_solve_truncated(A, b, tol):
     """Symmetric solve with eigenvalues below tol * the largest dropped."""
     A = np.ascontiguousarray(A, np.float64)
@@ -697,6 +711,20 @@ def bermudan_frozen_paths(model, strategy, z, adjoint=True):
     lib().orc_bermudan_frozen_paths(ctypes.byref(model), ctypes.byref(strategy), z.reshape(-1),
                                     int(adjoint), vals, ex.ctypes.data, d, v)
     return vals, ex, d, v
+
+
+def bermudan_given_paths(model, strategy, z, given, adjoint=True):
+    """Frozen-strategy value and adjoint with each path's exercise step given
+    (-1: never) -> (value per path, delta sums, vega sums)."""
+    z = np.ascontiguousarray(z, np.float64)
+    given = np.ascontiguousarray(given, np.int32)
+    assert given.shape == (model.paths,)
+    vals = np.zeros(model.paths)
+    d = np.zeros(model.n_rates)
+    v = np.zeros(model.n_rates)
+    lib().orc_bermudan_given_paths(ctypes.byref(model), ctypes.byref(strategy), z.reshape(-1),
+                                   given, int(adjoint), vals, d, v)
+    return vals, d, v
 
 
 def ref_swap_pipeline(paths, f0, taus, A, fixed_drifts, strike, scramblers, power_of_two,

@@ -1329,18 +1329,62 @@ int orc_sym_solve_truncated(int n, const double *A, const double *b, double tol,
   return 0;
 }
 
+/* The truncated solve of orc_sym_solve_truncated (eigenvalues below tol
+ * times the largest dropped), reached through Cholesky: when A is positive
+ * definite and 1 / trace(A^-1) >= tol * trace(A) -- a lower bound of the
+ * smallest eigenvalue against an upper bound of the largest -- nothing is
+ * dropped and the truncated solve is the Cholesky solution, which is
+ * returned; otherwise the Jacobi eigen-solve with the cut.  (The GPU's LS
+ * passes decide the same way.)  Returns 0. */
+int orc_sym_solve_cut(int n, const double *A, const double *b, double tol, double *x) {
+  double L[64 * 64];
+  if (n > 64 || n < 1) return -1;
+  int ok = 1;
+  double trA = 0.0;
+  for (int i = 0; i < n && ok; ++i) {
+    trA += A[i * n + i];
+    for (int j = 0; j <= i; ++j) {
+      double s = A[i * n + j];
+      for (int k = 0; k < j; ++k) s -= L[i * 64 + k] * L[j * 64 + k];
+      if (i == j) {
+        if (!(s > 0.0)) {
+          ok = 0;
+          break;
+        }
+        L[i * 64 + i] = sqrt(s);
+      } else {
+        L[i * 64 + j] = s / L[j * 64 + j];
+      }
+    }
+  }
+  if (ok) {
+    double trInv = 0.0, z[64];
+    for (int c = 0; c < n; ++c)
+      for (int i = c; i < n; ++i) {
+        double s = i == c ? 1.0 : 0.0;
+        for (int q = c; q < i; ++q) s -= L[i * 64 + q] * z[q];
+        z[i] = s / L[i * 64 + i];
+        trInv += z[i] * z[i];
+      }
+    if (trInv * tol * trA <= 1.0) return orc_cholesky_solve(n, A, b, x);
+  }
+  return orc_sym_solve_truncated(n, A, b, tol, x);
+}
+
 /* Backward induction with one regression per exercise date (depth 0 of
  * generateRegressionCoefficientsViaLSMulti_gold, gold/MultiLS_regression.cpp:
  * 23-430, without variable normalisation): at date e the regression matrix is
  * the mean of phi phi^T and the target the mean of phi (cash_e + Y) over the
- * selected paths (gold/LS_regression.h:431-515), solved here by Cholesky
- * (the gold uses SVD with a 1e-8 cut-off; equal for full-rank systems); then
+ * selected paths (gold/LS_regression.h:431-515), solved like the gold's SVD
+ * with a 1e-8 cut-off (orc_sym_solve_cut: the Cholesky solution when a trace
+ * bound shows no eigenvalue is below the cut, which is the usual case and
+ * equals the truncated solve there; the truncated eigen-solve otherwise); then
  * Y = ex_e where ex_e exceeds the estimated continuation, else Y += cash_e
  * (updateDeflatedNextStepValuesMulti_gold), and Y is deflated to date e-1 by
  * B_{e-1}/B_e (updateDeflationOfContinuationValues_gold).  Price = mean of
  * Y / B_0.  With itm_only the regression uses only paths with ex_e > 0 and
  * exercise requires ex_e > 0 (the Bermudan configuration's rule); a date
- * whose system is singular (too few such paths) gets zero coefficients. */
+ * with no such path gets zero coefficients. */
 double orc_ls_first_pass(int n_ex, const double *bv, const int *vars_each_step, int max_vars,
                          const double *cash, const double *ev, const double *num, int paths,
                          int use_cross, int itm_only, double *coeffs) {
@@ -1371,8 +1415,7 @@ double orc_ls_first_pass(int n_ex, const double *bv, const int *vars_each_step, 
         for (int j = 0; j <= i; ++j) M[i * nd + j] = M[j * nd + i] = M[i * nd + j] / cnt;
         t[i] /= cnt;
       }
-      if (orc_cholesky_solve(nd, M, t, beta) != 0)
-        for (int i = 0; i < nd; ++i) beta[i] = 0.0;
+      orc_sym_solve_cut(nd, M, t, 1e-8, beta);
     }
     for (int p = 0; p < paths; ++p) {
       const double exv = ev[(size_t)e * paths + p];
@@ -1866,15 +1909,16 @@ void orc_basket_asian_paths(const orc_basket_model *m, const double *z, double *
  * Q_j = sum_{m > j} tau_{m-1} P_m, and dV/df_s(T_s) = -V g_s for the fixings
  * s < t inside B_t; then the PC steps t..0 are reversed as for the caplets.
  * ==================================================================== */
+#define ORC_GIVEN_EV_TOL 1e-7
 static double bermudan_path_adjoint(const orc_caplet_model *m, const orc_frozen_strategy *fs,
                                     const double *z, int p, double *states, double *lam,
                                     double *delta, double *vw, step_rate *rec, double *erec,
-                                    double *eprec, int do_adjoint, int *tau_out) {
+                                    double *eprec, int do_adjoint, int given, int *tau_out) {
   const int N = m->n_rates, F = m->factors, P = m->paths;
   double *Bs = states + (size_t)(N + 1) * N;
   for (int r = 0; r < N; ++r) states[r] = log(m->f0[r] + m->disp[r]);
   double B = 1.0, V = 0.0, zs[64];
-  int t = -1, e = 0;
+  int t = -1, e = 0, kink_itm = 0;
   const int last = fs->ex_steps[fs->n_ex - 1];
   for (int s = 0; s <= last && t < 0; ++s) {
     for (int f = 0; f < F; ++f) zs[f] = z[((size_t)s * F + f) * P + p];
@@ -1899,16 +1943,23 @@ static double bermudan_path_adjoint(const orc_caplet_model *m, const orc_frozen_
       for (int i = 0; i < 6; ++i) cont += phi[i] * fs->coeffs[6 * e + i];
       const double evp = ev > 0.0 ? ev : 0.0;
       const int allowed = !fs->itm_only || evp > 0.0;
-      if (allowed && cont - evp < 0.0) {
+      /* given >= -1: the exercise step is the caller's (-1: never) */
+      if (given >= -1 ? s == given : (allowed && cont - evp < 0.0)) {
         t = s;
         V = evp / B;
+        /* A given exercise where the FP64 value is not positive but within
+         * ORC_GIVEN_EV_TOL of zero -- below the FP32 resolution of
+         * (SR - K) A, so an FP32 caller saw it in the money -- takes the
+         * in-the-money side's derivative of the payoff's kink, as that
+         * caller's adjoint does. */
+        if (given >= -1 && !(ev > 0.0) && ev > -ORC_GIVEN_EV_TOL) kink_itm = 1;
       }
       ++e;
     }
     B *= 1.0 + m->taus[s] * fsx;
   }
   if (tau_out) *tau_out = t;
-  if (!do_adjoint || t < 0 || !(V > 0.0)) return V;
+  if (!do_adjoint || t < 0 || !(V > 0.0 || kink_itm)) return V;
   /* payoff adjoint on the log-rates at the end of step t */
   for (int r = 0; r < N; ++r) lam[r] = 0.0;
   {
@@ -1970,9 +2021,10 @@ static double bermudan_path_adjoint(const orc_caplet_model *m, const orc_frozen_
   return V;
 }
 
-void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strategy *fs,
-                               const double *z, int do_adjoint, double *value_per_path,
-                               int *exercise_step, double *delta_sum, double *vega_sum) {
+static void bermudan_paths_adjoint(const orc_caplet_model *m, const orc_frozen_strategy *fs,
+                                   const double *z, const int *given, int do_adjoint,
+                                   double *value_per_path, int *exercise_step, double *delta_sum,
+                                   double *vega_sum) {
   const int N = m->n_rates, F = m->factors;
   for (int r = 0; r < N; ++r) delta_sum[r] = vega_sum[r] = 0.0;
 #pragma omp parallel
@@ -1985,7 +2037,8 @@ void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strat
 #pragma omp for schedule(static)
     for (int p = 0; p < m->paths; ++p)
       value_per_path[p] = bermudan_path_adjoint(m, fs, z, p, states, lam, dl, vw, rec, erec, eprec,
-                                                do_adjoint, exercise_step ? exercise_step + p : NULL);
+                                                do_adjoint, given ? given[p] : -2,
+                                                exercise_step ? exercise_step + p : NULL);
 #pragma omp critical
     for (int r = 0; r < N; ++r) {
       delta_sum[r] += dl[r];
@@ -1998,6 +2051,24 @@ void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strat
     free(rec);
     free(erec);
   }
+}
+
+void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strategy *fs,
+                               const double *z, int do_adjoint, double *value_per_path,
+                               int *exercise_step, double *delta_sum, double *vega_sum) {
+  bermudan_paths_adjoint(m, fs, z, NULL, do_adjoint, value_per_path, exercise_step, delta_sum,
+                         vega_sum);
+}
+
+/* The same price and adjoint with each path's exercise step given
+ * (given[p] = exercise step, -1 = never exercised): the GPU's own decisions
+ * can be replayed in FP64, so its Greeks are compared on the same function
+ * even where an FP32 continuation value sits within rounding of the
+ * exercise value. */
+void orc_bermudan_given_paths(const orc_caplet_model *m, const orc_frozen_strategy *fs,
+                              const double *z, const int *given, int do_adjoint,
+                              double *value_per_path, double *delta_sum, double *vega_sum) {
+  bermudan_paths_adjoint(m, fs, z, given, do_adjoint, value_per_path, NULL, delta_sum, vega_sum);
 }
 
 /* Thread count of the OpenMP loops above (the CPU baseline times the oracle

@@ -131,6 +131,7 @@ int orc_basis_count(int n_vars, int use_cross);
 void orc_basis_functions(const double *vars, int n_vars, int use_cross, double *out);
 int orc_cholesky_solve(int n, const double *A, const double *b, double *x);
 int orc_sym_solve_truncated(int n, const double *A, const double *b, double tol, double *x);
+int orc_sym_solve_cut(int n, const double *A, const double *b, double tol, double *x);
 double orc_ls_first_pass(int n_ex, const double *basis_vars /* (E,maxVars,P) */,
                          const int *vars_each_step, int max_vars, const double *cash /* (E,P) */,
                          const double *ex_values /* (E,P) */, const double *numeraires /* (E,P) */,
@@ -169,6 +170,10 @@ void orc_bermudan_frozen_paths(const orc_caplet_model *m, const orc_frozen_strat
                                const double *z /* (S,F,P) */, int do_adjoint,
                                double *value_per_path, int *exercise_step /* or NULL */,
                                double *delta_sum /* N */, double *vega_sum /* N */);
+void orc_bermudan_given_paths(const orc_caplet_model *m, const orc_frozen_strategy *fs,
+                              const double *z /* (S,F,P) */, const int *given /* P */,
+                              int do_adjoint, double *value_per_path, double *delta_sum,
+                              double *vega_sum);
 
 /* ---- multi-dimensional Black-Scholes: basket Asian ------------------- */
 typedef struct {

@@ -181,17 +181,19 @@ def test_displaced_rates_and_varied_accruals_vs_fp64_adjoint(torch_cuda, k):
 
 
 def test_ragged_path_counts(torch_cuda):
+    # the oracle runs on the GPU's own variates: with one path an FP32 normal
+    # can move a caplet across its strike against the FP64 chain
     for P in (1, 63, 65, 1000):
         w = configs.caplet_portfolio(paths=P)
-        z = oracle_z(w)
-        pp, d, v = O.caplet_adjoint_paths(oracle_model(w), z)
         pr = pipeline.LMMPricer(w)
         p, g = pr.price_and_greeks()
         g = g.cpu().numpy()
+        z = pr.z.cpu().numpy().astype(np.float64).reshape(w.steps, w.factors, P)
+        pp, d, v = O.caplet_adjoint_paths(oracle_model(w), z)
         assert abs(p.item() - pp.mean()) < 1e-5
         N = w.n_rates
-        assert np.allclose(g[:N], d / P, rtol=2e-4, atol=1e-8)
-        assert np.allclose(g[N:], v / P, rtol=2e-4, atol=1e-8)
+        assert np.all(np.abs(g[:N] - d / P) <= 1e-4 * np.abs(d / P) + 1e-8), P
+        assert np.all(np.abs(g[N:] - v / P) <= 1e-4 * np.abs(v / P) + 1e-8), P
 
 
 def test_repeat_calls_are_deterministic(torch_cuda):

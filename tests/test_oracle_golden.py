@@ -717,6 +717,40 @@ def test_truncated_symmetric_solve_matches_numpy_eigh():
         assert np.allclose(O.sym_solve_truncated(B, b, 1e-8), np.linalg.solve(B, b), atol=1e-13)
 
 
+def test_cut_solve_is_cholesky_when_nothing_is_dropped_else_truncated():
+    """orc_sym_solve_cut (the LS first pass's solve): the Cholesky solution,
+    bit for bit, when the trace bound shows no eigenvalue under the cut; the
+    truncated eigen-solve (pinned against numpy above) when the cut binds,
+    including a rank-deficient matrix a plain Cholesky would fail on or blow
+    up; and a matrix whose spectrum sits just above the cut but whose trace
+    bound cannot show it gives the truncated solve, which keeps everything."""
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 6, 7):
+        Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        b = rng.normal(size=n)
+        A = (Q * np.logspace(0, -4, n)) @ Q.T
+        chol = np.zeros(n)
+        assert O.lib().orc_cholesky_solve(n, A.reshape(-1).copy(), b, chol) == 0
+        assert np.array_equal(O.sym_solve_cut(A, b, 1e-8), chol)
+        if n < 3:
+            continue
+        lam = np.logspace(0, -12, n)
+        A = (Q * lam) @ Q.T
+        assert np.array_equal(O.sym_solve_cut(A, b, 1e-8), O.sym_solve_truncated(A, b, 1e-8))
+        # exactly rank-deficient: the quadratic basis of one variable twice
+        x = rng.normal(size=500)
+        X = np.stack([np.ones_like(x), x, x, x * x, x * x, x * x], 1)[:, :n]
+        G = X.T @ X / len(x)
+        y = 1.0 + 2.0 * x + (0.5 * x * x if n > 3 else 0.0)  # in the span of the columns
+        got = O.sym_solve_cut(G, X.T @ y / len(x), 1e-8)
+        assert np.all(np.isfinite(got)) and np.abs(X @ got - y).max() < 1e-8
+        # eigenvalues down to 2e-8: nothing is cut, but 1/trace(A^-1) < 1e-8 trace(A)
+        A = (Q * np.concatenate([[1.0], np.full(n - 1, 2e-8)])) @ Q.T
+        x_cut = O.sym_solve_cut(A, b, 1e-8)
+        assert np.array_equal(x_cut, O.sym_solve_truncated(A, b, 1e-8))
+        assert np.allclose(A @ x_cut, b, rtol=0, atol=1e-6 * np.abs(b).max())
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_fraction_selector_band_holds_the_requested_fraction(seed):
     """The selector's symmetric band keeps between lower_frac and upper_frac

@@ -79,10 +79,14 @@ def test_bermudan_paths_equal_staged_curve_routines():
 @pytest.mark.parametrize("use_cross", [True, False])
 @pytest.mark.parametrize("seed,P", [(11, 2048), (12, 8192)])
 def test_ls_first_pass_matches_reference_gold(use_cross, seed, P):
-    # The gold standardises the basis variables (normalise = true) and solves
-    # by SVD; the oracle solves the raw normal equations by Cholesky.  The
-    # least-squares fit of an affine-invariant quadratic basis is the same
-    # function either way, so the exercise decisions and the price agree.
+    # Both solve by the truncated SVD (1e-8 cut) of the same mean-scaled
+    # normal equations.  On raw rate-level variables (x ~ 0.05) the quadratic
+    # basis is badly scaled and the cut drops real directions in both, so the
+    # oracle on raw variables equals the gold without normalisation; on the
+    # well-scaled (x - 0.05) * 100 the cut is inactive, the fit is the exact
+    # least-squares one, and it equals the gold that standardises its
+    # variables (normalise = true), where the cut is inactive too.  The two
+    # prices differ from each other by a few percent (the raw cut's effect).
     m, z, ex, _, _ = _small_bermudan(P=P, seed=seed)
     bv, ev, num = O.bermudan_paths(m, z, ex)
     E, P = ev.shape
@@ -91,18 +95,25 @@ def test_ls_first_pass_matches_reference_gold(use_cross, seed, P):
     cash = np.zeros((E, P))
     cash[1::2] = 1e-3 * np.cos(np.arange(P))  # exercise the cash-flow terms too
     price, _ = O.ls_first_pass(bv, ves, cash, ev, num, use_cross=use_cross)
-    rprice, _ = O.ref_ls_first_pass(bv, ves, cash, ev, num, use_cross=use_cross, normalise=True)
+    rprice, _ = O.ref_ls_first_pass(bv, ves, cash, ev, num, use_cross=use_cross, normalise=False)
     assert abs(price - rprice) <= 1e-12 * abs(rprice)
+    price_n, _ = O.ls_first_pass((bv - 0.05) * 100.0, ves, cash, ev, num, use_cross=use_cross)
+    rprice_n, _ = O.ref_ls_first_pass(bv, ves, cash, ev, num, use_cross=use_cross, normalise=True)
+    assert abs(price_n - rprice_n) <= 1e-12 * abs(rprice_n)
+    assert abs(price_n - price) > 1e-4 * price  # the raw cut binds: a real difference
 
 
 def test_ls_first_pass_is_invariant_to_affine_rescaling_of_the_variables():
+    # Where the 1e-8 cut drops nothing (well-scaled variables) the fit is the
+    # exact least-squares one, which an affine change of each variable leaves
+    # unchanged (the quadratic basis with cross terms spans the same functions).
     m, z, ex, _, _ = _small_bermudan(P=4096, seed=3)
     bv, ev, num = O.bermudan_paths(m, z, ex)
     E, P = ev.shape
     ves = np.full(E, 2, np.int32)
     cash = np.zeros((E, P))
-    a, _ = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=True)
-    b, _ = O.ls_first_pass((bv - 0.05) * 100.0, ves, cash, ev, num, itm_only=True)
+    a, _ = O.ls_first_pass((bv - 0.05) * 100.0, ves, cash, ev, num, itm_only=True)
+    b, _ = O.ls_first_pass((bv - 0.045) * 70.0 + 0.3, ves, cash, ev, num, itm_only=True)
     assert abs(a - b) <= 1e-12 * a
 
 
@@ -126,13 +137,14 @@ def test_ls_itm_only_regresses_on_the_in_the_money_rows():
     E, P = ev.shape
     ves = np.full(E, 2, np.int32)
     cash = np.zeros((E, P))
-    _, coeffs = O.ls_first_pass(bv, ves, cash, ev, num, itm_only=True)
+    bvn = (bv - 0.05) * 100.0  # well scaled: the 1e-8 cut drops nothing
+    _, coeffs = O.ls_first_pass(bvn, ves, cash, ev, num, itm_only=True)
     # Date E-2's coefficients are the least-squares fit of (ev_{E-1} B_{E-2}/B_{E-1})
     # on the ITM rows of date E-2 (the last date always exercises when ITM).
     e = E - 2
     y = np.where(ev[E - 1] > 0, ev[E - 1], 0.0) * num[e] / num[E - 1]
     itm = ev[e] > 0
-    X = np.stack([O.basis_functions(bv[e, :, p]) for p in np.nonzero(itm)[0]])
+    X = np.stack([O.basis_functions(bvn[e, :, p]) for p in np.nonzero(itm)[0]])
     beta, *_ = np.linalg.lstsq(X, y[itm], rcond=None)
     assert np.allclose(X @ coeffs[e], X @ beta, rtol=1e-8, atol=1e-12)
 
@@ -233,3 +245,58 @@ def test_frozen_bermudan_adjoint_matches_central_fd_with_frozen_decisions():
             res.append(vb.sum())
         fv = (res[0] - res[1]) / (2 * h)
         assert abs(fv - v[i]) <= 1e-6 * np.max(np.abs(v)) + 1e-9
+
+
+def test_bermudan_given_steps_replays_the_strategy_and_fixed_dates():
+    """Exercise steps given per path: the strategy's own steps reproduce the
+    frozen pass exactly (values and Greeks); 'never' prices zero; one date
+    for every path prices that date's exercise value deflated by the
+    numeraire, max(ev, 0)/B, path by path."""
+    m, z, ex, f0, A, bv, ev, num, coeffs, shift, scale = _frozen_setup(P=1024, seed=23)
+    E, P = ev.shape
+    fs = O.frozen_strategy(ex, coeffs, shift, scale)
+    vals, exs, d, v = O.bermudan_frozen_paths(m, fs, z)
+    gv, gd, gvv = O.bermudan_given_paths(m, fs, z, exs)
+    assert np.array_equal(gv, vals)
+    # per-thread partial sums are combined in thread order: equal to rounding
+    assert np.allclose(gd, d, rtol=1e-13, atol=0) and np.allclose(gvv, v, rtol=1e-13, atol=0)
+    nv, nd, nvv = O.bermudan_given_paths(m, fs, z, np.full(P, -1, np.int32))
+    assert not nv.any() and not nd.any() and not nvv.any()
+    for e in (0, E // 2, E - 1):
+        fv, _, _ = O.bermudan_given_paths(m, fs, z, np.full(P, ex[e], np.int32), adjoint=False)
+        assert np.allclose(fv, np.maximum(ev[e], 0) / num[e], rtol=1e-12, atol=1e-15)
+    # a moved decision changes exactly that path's value
+    j = int(np.argmax(exs >= 0))
+    moved = exs.copy()
+    moved[j] = -1
+    mv, _, _ = O.bermudan_given_paths(m, fs, z, moved, adjoint=False)
+    assert mv[j] == 0.0 and vals[j] > 0.0
+    assert np.array_equal(np.delete(mv, j), np.delete(vals, j))
+
+
+def test_bermudan_given_exercise_at_the_payoff_kink_takes_the_in_the_money_derivative():
+    """A path given as exercised where its FP64 exercise value is a hair below
+    zero (within 1e-7, under the FP32 resolution of (SR - K) A) gets the
+    in-the-money side's payoff derivative, like an FP32 caller that saw it in
+    the money; clearly out of the money it gets none."""
+    m, z, ex, f0, A = _small_bermudan(P=1, seed=31)
+    N = f0.size
+    taus, disp, vols = np.full(N, 0.5), np.zeros(N), np.full(N, 0.2)
+    E = len(ex)
+    fs = O.frozen_strategy(ex, np.zeros((E, 6)), np.full((E, 2), 0.05), np.full((E, 2), 100.0))
+    low = 0.0
+    ml = O.caplet_model(f0, taus, disp, A, vols, low, 1)
+    bv, ev, _ = O.bermudan_paths(ml, z, ex)
+    sr = bv[0, 0, 0]
+    ann = ev[0, 0] / (sr - low)  # the annuity at the first date
+    out = {}
+    for tag, gap in (("itm", -5e-8), ("kink", 5e-8), ("otm", 1e-5)):
+        mk = O.caplet_model(f0, taus, disp, A, vols, sr + gap / ann, 1)
+        out[tag] = O.bermudan_given_paths(mk, fs, z, np.array([ex[0]], np.int32))
+    v_itm, d_itm, g_itm = out["itm"]
+    v_kink, d_kink, g_kink = out["kink"]
+    assert v_itm[0] > 0.0 and v_kink[0] == 0.0
+    assert np.any(d_itm != 0.0)
+    assert np.allclose(d_kink, d_itm, rtol=1e-5, atol=0) and np.allclose(g_kink, g_itm, rtol=1e-5)
+    v_otm, d_otm, g_otm = out["otm"]
+    assert v_otm[0] == 0.0 and not d_otm.any() and not g_otm.any()

@@ -66,8 +66,20 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   return v;
 }
 
+// Agent-scope (device-wide) coherent store and load of a double: the store
+// writes through the XCD's L2, the load misses it, so values exchanged
+// between blocks of one running kernel need no cache write-back or
+// invalidation (used by the persistent kernels for their block partials).
+__device__ __forceinline__ void store_agent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Block-wide FP64 sum of one value per lane into part[col]; every lane of the
-// block must call it (contains barriers).
+// block must call it (contains barriers).  Agent: the store is agent-coherent.
+template <bool Agent = false>
 __device__ __forceinline__ void block_sum_to(double v, double* scratch_lds, double* out) {
   v = wave_sum_f64(v);
   const int w = threadIdx.x >> 6;
@@ -77,7 +89,10 @@ __device__ __forceinline__ void block_sum_to(double v, double* scratch_lds, doub
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) t += scratch_lds[i];
-    *out = t;
+    if (Agent)
+      store_agent(out, t);
+    else
+      *out = t;
   }
 }
 
@@ -311,7 +326,9 @@ __device__ __forceinline__ int find_block_owner(const Item* items, int n, int bl
 inline int rate_class(int rates) { return rates <= 16 ? 0 : (rates <= 24 ? 1 : (rates <= 32 ? 2 : 3)); }
 
 // out[col] = (sum_b part[b*ncols + col]) * scale, compensated; called by a
-// whole block (k_finalize_cols runs one block per column).
+// whole block (k_finalize_cols runs one block per column).  Agent: the
+// partials are read with agent-coherent loads.
+template <bool Agent = false>
 __device__ __forceinline__ void finalize_col(const double* __restrict__ part, int nb, int ncols,
                                              int col, double scale, double* __restrict__ out) {
   __shared__ double sv[16], sc[16];
@@ -319,7 +336,8 @@ __device__ __forceinline__ void finalize_col(const double* __restrict__ part, in
   // per lane: Neumaier over a strided slice of the blocks
   double s = 0.0, c = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    const double x = part[static_cast<size_t>(b) * ncols + col];
+    const double x = Agent ? load_agent(part + static_cast<size_t>(b) * ncols + col)
+                           : part[static_cast<size_t>(b) * ncols + col];
     const double t = s + x;
     c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
     s = t;

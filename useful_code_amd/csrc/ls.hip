@@ -20,6 +20,7 @@
 // * the reference's small LS kernels (basis variables, exercise values, basis
 //   expansion, continuation values, update, deflation).
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "ls_common.h"
@@ -175,19 +176,20 @@ __device__ __forceinline__ float continuation(const float* __restrict__ bv, size
   return c;
 }
 
-// One block (of nblocks) of one exercise date.
-template <int GROUPS>
-__device__ __forceinline__ void ls_date_block(const DateArgs& a, const float* __restrict__ bv,
-                                              const float* __restrict__ cash,
-                                              const float* __restrict__ ev,
-                                              const float* __restrict__ num,
-                                              double* __restrict__ coeffs, float* __restrict__ Y,
-                                              const double* __restrict__ part_later,
-                                              double* __restrict__ part, int block, int nblocks) {
-  __shared__ float lds_d[(kGramThreads / 64) * 256];
-  __shared__ double gram[kMaxGram * kMaxGram];
-  __shared__ double chol[kMaxGram * kMaxGram + kMaxGram];
-  __shared__ double beta_n[kMaxGram];
+// The path work of one block (of nblocks) of one exercise date, given the
+// later date's coefficients in beta_n (LDS; unused at the last date): the
+// later date's exercise decision and deflation of the running values, then
+// this date's normal equations on MFMA into the block's partials.  Agent: the
+// partials are written with agent-coherent stores (the persistent kernels,
+// whose blocks exchange them without a kernel boundary).
+template <int GROUPS, bool Agent>
+__device__ __forceinline__ void ls_date_work(const DateArgs& a, const float* __restrict__ bv,
+                                             const float* __restrict__ cash,
+                                             const float* __restrict__ ev,
+                                             const float* __restrict__ num,
+                                             const double* beta_n, float* __restrict__ Y,
+                                             double* __restrict__ part, int block, int nblocks) {
+  __shared__ double lds_d[(kGramThreads / 64) * 256];
   constexpr int kPathsPerOp = 4 * GROUPS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const size_t P = static_cast<size_t>(a.P);
@@ -196,10 +198,8 @@ __device__ __forceinline__ void ls_date_block(const DateArgs& a, const float* __
   const int n = a.k + 1;  // basis functions + target
   const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
   const int sub = GROUPS == 2 ? ((lane >> 3) & 1) * 4 + (lane >> 4) : (lane >> 4);
-  if (later)  // coefficients of date e+1 from its partials
-    solve_date(part_later, nblocks, a.kn, gram, chol, beta_n,
-               coeffs + static_cast<size_t>(a.e + 1) * a.maxd, a.maxd, block);
-  floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  // each MFMA's 4-path sums (FP32) accumulated in FP64
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int stride = nblocks * nw * 64;
   for (int base = (block * nw + wave) * 64; base < a.P; base += stride) {
     const int p = base + lane;
@@ -245,10 +245,32 @@ __device__ __forceinline__ void ls_date_block(const DateArgs& a, const float* __
       const float st = __shfl(t, srcl, 64), sw = __shfl(w, srcl, 64);
       float v = j < a.k ? basis_value(j, a.v, cross, s0, s1, s2, s3) : (j == a.k ? st : 0.0f);
       v *= sw;
-      acc = mfma16x16x4(v, v, acc);
+      const floatx4 d = mfma16x16x4(v, v, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] += static_cast<double>(d[i]);
     }
   }
-  block_gram<GROUPS>(acc, n, lds_d, part, block, nblocks);
+  block_gram_d<GROUPS, Agent>(acc, n, lds_d, part, block, nblocks);
+}
+
+// One block (of nblocks) of one exercise date: the later date's coefficients
+// from its partials (every block solves them itself: same partials, same
+// order, so the same coefficients; block 0 stores them), then the path work.
+template <int GROUPS>
+__device__ __forceinline__ void ls_date_block(const DateArgs& a, const float* __restrict__ bv,
+                                              const float* __restrict__ cash,
+                                              const float* __restrict__ ev,
+                                              const float* __restrict__ num,
+                                              double* __restrict__ coeffs, float* __restrict__ Y,
+                                              const double* __restrict__ part_later,
+                                              double* __restrict__ part, int block, int nblocks) {
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double chol[kSolveScratch];
+  __shared__ double beta_n[kMaxGram];
+  if (a.e + 1 < a.E)  // coefficients of date e+1 from its partials
+    solve_date(part_later, nblocks, a.kn, gram, chol, beta_n,
+               coeffs + static_cast<size_t>(a.e + 1) * a.maxd, a.maxd, block);
+  ls_date_work<GROUPS, false>(a, bv, cash, ev, num, beta_n, Y, part, block, nblocks);
 }
 
 template <int GROUPS>
@@ -260,23 +282,17 @@ __global__ void __launch_bounds__(kGramThreads)
   ls_date_block<GROUPS>(a, bv, cash, ev, num, coeffs, Y, part_later, part, blockIdx.x, gridDim.x);
 }
 
-// Date 0's coefficients and exercise decision, then sum_p Y_p / B_0 into
-// block partials (one block of nblocks; date 0's partials come from
-// nb_date0 blocks).
-__device__ __forceinline__ void ls_final_block(const DateArgs& a, const float* __restrict__ bv,
-                                               const float* __restrict__ cash,
-                                               const float* __restrict__ ev,
-                                               const float* __restrict__ num,
-                                               double* __restrict__ coeffs,
-                                               const float* __restrict__ Y,
-                                               const double* __restrict__ part_date0,
-                                               int nb_date0, double* __restrict__ part, int block,
-                                               int nblocks) {
+// Date 0's exercise decision under its coefficients beta0 (LDS), then
+// sum_p Y_p / B_0 into the block's price partial (one block of nblocks).
+// Agent as for ls_date_work.
+template <bool Agent>
+__device__ __forceinline__ void ls_final_work(const DateArgs& a, const float* __restrict__ bv,
+                                              const float* __restrict__ cash,
+                                              const float* __restrict__ ev,
+                                              const float* __restrict__ num, const double* beta0,
+                                              const float* __restrict__ Y,
+                                              double* __restrict__ part, int block, int nblocks) {
   __shared__ double red[kGramThreads / 64];
-  __shared__ double gram[kMaxGram * kMaxGram];
-  __shared__ double chol[kMaxGram * kMaxGram + kMaxGram];
-  __shared__ double beta0[kMaxGram];
-  solve_date(part_date0, nb_date0, a.k, gram, chol, beta0, coeffs, a.maxd, block);
   const size_t P = static_cast<size_t>(a.P);
   double s = 0.0;
   for (int p = block * blockDim.x + threadIdx.x; p < a.P; p += nblocks * blockDim.x) {
@@ -290,7 +306,114 @@ __device__ __forceinline__ void ls_final_block(const DateArgs& a, const float* _
       y += cash[p];
     s += static_cast<double>(y) / static_cast<double>(num[p]);
   }
-  block_sum_to(s, red, part + block);
+  block_sum_to<Agent>(s, red, part + block);
+}
+
+// Date 0's coefficients from its partials (nb_date0 blocks; every block
+// solves, block 0 stores them), then the final decision and price partial.
+__device__ __forceinline__ void ls_final_block(const DateArgs& a, const float* __restrict__ bv,
+                                               const float* __restrict__ cash,
+                                               const float* __restrict__ ev,
+                                               const float* __restrict__ num,
+                                               double* __restrict__ coeffs,
+                                               const float* __restrict__ Y,
+                                               const double* __restrict__ part_date0,
+                                               int nb_date0, double* __restrict__ part, int block,
+                                               int nblocks) {
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double chol[kSolveScratch];
+  __shared__ double beta0[kMaxGram];
+  solve_date(part_date0, nb_date0, a.k, gram, chol, beta0, coeffs, a.maxd, block);
+  ls_final_work<false>(a, bv, cash, ev, num, beta0, Y, part, block, nblocks);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent first passes.  One block solves each date's normal equations
+// (from the partials of all the date's blocks) and publishes the
+// coefficients; the blocks of the next date wait for them, do their path
+// work and count themselves done.  All exchanges between blocks go through
+// agent-coherent stores and loads (the partials, the coefficients, two
+// counter words), each wave waiting for its stores to be performed before
+// its block signals, so no cache write-back or invalidation is needed.  The
+// running values stay with the block that owns their paths.  Every wait polls
+// a bounded number of times (kBarrierPolls); a wait that gives up makes the
+// kernel stop with its error word set and NaN prices.
+// ---------------------------------------------------------------------------
+
+// Thread 0 waits until *word >= target; the outcome is returned on every lane.
+__device__ __forceinline__ bool wait_at_least(const unsigned* word, unsigned target, int* flag_lds) {
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    unsigned polls = 0;
+    while (__hip_atomic_load(const_cast<unsigned*>(word), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(4);
+      if (++polls > kBarrierPolls) {
+        ok = 0;
+        break;
+      }
+    }
+    *flag_lds = ok;
+  }
+  __syncthreads();
+  const bool ok = *flag_lds != 0;
+  __syncthreads();
+  return ok;
+}
+
+// Whether *word >= target now (thread 0 reads, every lane gets the answer).
+__device__ __forceinline__ bool reached(const unsigned* word, unsigned target, int* flag_lds) {
+  if (threadIdx.x == 0)
+    *flag_lds = __hip_atomic_load(const_cast<unsigned*>(word), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT) >= target;
+  __syncthreads();
+  const bool ok = *flag_lds != 0;
+  __syncthreads();
+  return ok;
+}
+
+// After the block's agent-coherent stores: each wave waits for its own stores
+// to be performed (s_waitcnt vmcnt(0)), then the block barrier, then one lane
+// adds v to *word.
+__device__ __forceinline__ void signal_add(unsigned* word, unsigned v) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(word, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// signal_add(word, 1) that also tells the block (every lane) whether its add
+// was the one that brought *word to target, i.e. it arrived last.
+__device__ __forceinline__ bool arrive_last(unsigned* word, unsigned target, int* flag_lds) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    *flag_lds = __hip_atomic_fetch_add(word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
+                target;
+  __syncthreads();
+  const bool last = *flag_lds != 0;
+  __syncthreads();
+  return last;
+}
+
+// One block: the normal equations of a date from its nb partials (agent
+// loads), solved, the coefficients row (maxd entries, zero past k) stored
+// with agent stores.
+__device__ __forceinline__ void ls_solve_publish(const double* part, int nb, int k, int maxd,
+                                                 double* coeffs_row) {
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double scratch[kSolveScratch];
+  __shared__ double beta[kMaxGram];
+  reduce_partials<true>(part, nb, (k + 1) * (k + 1), gram);
+  if (threadIdx.x == 0) {
+    solve_normal_equations(gram, k, beta, scratch);
+    for (int i = 0; i < maxd; ++i) store_agent(coeffs_row + i, i < k ? beta[i] : 0.0);
+  }
+}
+
+// The published coefficients row (agent loads) into LDS.
+__device__ __forceinline__ void ls_load_coeffs(const double* coeffs_row, int k, double* beta_lds) {
+  if (threadIdx.x < k) beta_lds[threadIdx.x] = load_agent(coeffs_row + threadIdx.x);
+  __syncthreads();
 }
 
 __global__ void __launch_bounds__(kGramThreads)
@@ -312,6 +435,84 @@ struct AllDates {
   signed char v[kMaxRates], k[kMaxRates];
   float sh[kMaxRates][kMaxBasisVars], sc[kMaxRates][kMaxBasisVars];
 };
+
+// The date arguments of date e from the all-dates table.
+__device__ __forceinline__ DateArgs all_dates_date(const AllDates& d, int e) {
+  DateArgs a{};
+  a.e = e;
+  a.E = d.E;
+  a.P = d.P;
+  a.max_vars = d.max_vars;
+  a.maxd = d.maxd;
+  a.cross = d.cross;
+  a.itm_only = d.itm_only;
+  a.v = d.v[e];
+  a.k = d.k[e];
+  for (int i = 0; i < kMaxBasisVars; ++i) {
+    a.sh[i] = d.sh[e][i];
+    a.sc[i] = d.sc[e][i];
+  }
+  if (e + 1 < d.E) {
+    a.vn = d.v[e + 1];
+    a.kn = d.k[e + 1];
+    for (int i = 0; i < kMaxBasisVars; ++i) {
+      a.shn[i] = d.sh[e + 1][i];
+      a.scn[i] = d.sc[e + 1][i];
+    }
+  }
+  return a;
+}
+
+// The whole LS first pass in one persistent launch.  Round r handles date
+// e = E - 1 - r (r = E: date 0's decision and the price): every block waits
+// for the later date's coefficients (sync[1] counts those published), does
+// its path work for date e into its partials and counts itself done
+// (sync[0]); the block that arrives last solves date e from all the partials
+// and publishes its coefficients -- or, after the last round, sums the price.
+// Each date's work and solve are those of the per-date launches (same
+// blocks, same order), so the results are identical.  Launched with at most
+// as many blocks as can be resident.  If a wait gives up, the price is NaN
+// and sync[2] = 1.
+__global__ void __launch_bounds__(kGramThreads)
+    k_ls_first_pass(const AllDates dv, const float* __restrict__ bv,
+                    const float* __restrict__ cash, const float* __restrict__ ev,
+                    const float* __restrict__ num, double* __restrict__ coeffs,
+                    float* __restrict__ Y, double* __restrict__ part0,
+                    double* __restrict__ part1, unsigned* __restrict__ sync,
+                    double* __restrict__ price_out) {
+  __shared__ int flag;
+  __shared__ double beta[kMaxGram];
+  const AllDates& d = dv;
+  const int nb = gridDim.x;
+  for (int r = 0; r <= d.E; ++r) {
+    const int e = d.E - 1 - r;  // -1: the final decision
+    if (r > 0) {
+      if (!wait_at_least(sync + 1, static_cast<unsigned>(r), &flag)) goto fail;
+      ls_load_coeffs(coeffs + static_cast<size_t>(e + 1) * d.maxd, d.k[e + 1], beta);
+    }
+    double* out = (e & 1) ? part1 : part0;  // e = -1: the price partials go to part1
+    if (e >= 0) {
+      // every date has k + 1 <= 8 rows (the host launches this kernel only then)
+      ls_date_work<2, true>(all_dates_date(d, e), bv, cash, ev, num, beta, Y, out, blockIdx.x, nb);
+    } else {
+      ls_final_work<true>(all_dates_date(d, 0), bv, cash, ev, num, beta, Y, out, blockIdx.x, nb);
+    }
+    if (arrive_last(sync, static_cast<unsigned>(r + 1) * nb, &flag)) {
+      if (e >= 0) {
+        ls_solve_publish(out, nb, d.k[e], d.maxd, coeffs + static_cast<size_t>(e) * d.maxd);
+        signal_add(sync + 1, 1u);
+      } else {
+        finalize_col<true>(out, nb, 1, 0, 1.0 / static_cast<double>(d.P), price_out);
+      }
+    }
+  }
+  return;
+fail:
+  if (threadIdx.x == 0) {
+    sync[2] = 1u;
+    *price_out = __builtin_nan("");
+  }
+}
 
 __global__ void __launch_bounds__(kLsThreads)
     k_ls_second(const AllDates dv, const float* __restrict__ bv,
@@ -476,8 +677,10 @@ size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 // Workspace: running continuation values (P floats), then two partial
 // buffers used alternately by consecutive dates (date e reads date e+1's
 // partials while writing its own), each kMaxGram^2 doubles per block.
+// Then the persistent first pass's three 32-bit sync words (blocks done,
+// solves published, error).
 struct LsLayout {
-  size_t y, part, part2, total;
+  size_t y, part, part2, bar, total;
 };
 
 LsLayout ls_layout(int paths) {
@@ -487,8 +690,23 @@ LsLayout ls_layout(int paths) {
   const int nb = ls_blocks(paths) > gram_blocks(paths) ? ls_blocks(paths) : gram_blocks(paths);
   const size_t half = align256(sizeof(double) * kMaxGram * kMaxGram * nb);
   l.part2 = l.part + half;
-  l.total = l.part2 + half;
+  l.bar = l.part2 + half;
+  l.total = l.bar + 256;
   return l;
+}
+
+// Blocks of the persistent first pass: gram_blocks(paths), provided that
+// many 1024-lane blocks can be resident at once; 0 when the occupancy query
+// fails or allows none (the caller then launches one kernel per date).
+int ls_persistent_blocks(int paths) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ls_first_pass, kGramThreads, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    return 0;
+  const int want = gram_blocks(paths);
+  const int cap = per_cu * device_cu_count();
+  return want < cap ? want : cap;
 }
 
 int check_ls(const char* who, int n_ex, const int* ves, int max_vars, int paths, int use_cross) {
@@ -597,12 +815,13 @@ __device__ __forceinline__ DateArgs item_date(const LsItem& it, int e) {
   return a;
 }
 
-__global__ void __launch_bounds__(kGramThreads)
-    k_ls_round(const LsItem* __restrict__ items, int n, int round) {
-  const LsItem& it = items[find_block_owner(items, n, blockIdx.x, &LsItem::ls_block0)];
+// Block `block` (numbered product after product) of round `round`.
+__device__ __forceinline__ void ls_round_block(const LsItem* __restrict__ items, int n, int round,
+                                               int block) {
+  const LsItem& it = items[find_block_owner(items, n, block, &LsItem::ls_block0)];
   const int e = it.E - 1 - round;
   if (e < -1) return;  // this product finished in an earlier round
-  const int blk = blockIdx.x - it.ls_block0;
+  const int blk = block - it.ls_block0;
   // the two partial buffers alternate between consecutive dates
   g_double* even = it.part0;
   g_double* odd = it.part1;
@@ -618,11 +837,104 @@ __global__ void __launch_bounds__(kGramThreads)
   }
 }
 
+__global__ void __launch_bounds__(kGramThreads)
+    k_ls_round(const LsItem* __restrict__ items, int n, int round) {
+  ls_round_block(items, n, round, blockIdx.x);
+}
+
 // First-pass price of each product (one block per product).
 __global__ void __launch_bounds__(256) k_ls_price_batch(const LsItem* __restrict__ items) {
   const LsItem& it = items[blockIdx.x];
   finalize_col(gmem(it.part1), it.ls_blocks, 1, 0, 1.0 / static_cast<double>(it.P),
                gmem(it.price_out));
+}
+
+// LS block b (of product q) of round r in the persistent portfolio kernel:
+// the later date's published coefficients (r > 0) into beta (LDS), the
+// date's path work (or, past date 0, the final decision and price partial),
+// then q's round count; the block completing it solves and publishes q's
+// date (or sums q's price).
+__device__ __forceinline__ void ls_round_work(const LsItem* __restrict__ items, int q, int r,
+                                              int b, double* beta, unsigned* __restrict__ sync,
+                                              int* flag) {
+  const LsItem& it = items[q];
+  const int e = it.E - 1 - r;
+  if (r > 0) ls_load_coeffs(gmem(it.coeffs) + static_cast<size_t>(e + 1) * 6, 6, beta);
+  const int blk = b - it.ls_block0;
+  g_double* out = (e & 1) ? it.part1 : it.part0;  // e = -1: the price partials go to part1
+  if (e >= 0)
+    ls_date_work<2, true>(item_date(it, e), gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), beta,
+                          gmem(it.Y), gmem(out), blk, it.ls_blocks);
+  else
+    ls_final_work<true>(item_date(it, 0), gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), beta,
+                        gmem(it.Y), gmem(out), blk, it.ls_blocks);
+  if (arrive_last(sync + 2 * q, static_cast<unsigned>(r + 1) * it.ls_blocks, flag)) {
+    if (e >= 0) {
+      ls_solve_publish(gmem(out), it.ls_blocks, 6, 6, gmem(it.coeffs) + static_cast<size_t>(e) * 6);
+      signal_add(sync + 2 * q + 1, 1u);
+    } else {
+      finalize_col<true>(gmem(out), it.ls_blocks, 1, 0, 1.0 / static_cast<double>(it.P),
+                         gmem(it.price_out));
+    }
+  }
+}
+
+// Every LS first pass of the portfolio in one persistent launch.  The
+// resident blocks share out the products' LS blocks (block b takes LS blocks
+// b, b + gridDim.x, ...).  In round r an LS block of product q waits for q's
+// later-date coefficients (q's published-solve count sync[2q+1] reaching r),
+// does its path work for q's date E_q - 1 - r and counts itself done
+// (sync[2q]); the LS block that completes q's round solves q's date and
+// publishes the coefficients -- or, after q's last round, sums q's price.  A
+// block takes its LS blocks of a round in whichever order their products
+// become ready, so products wait only for themselves; the block that would
+// publish never waits first, so every wait ends.  Same per-block work and
+// solves as the per-round launches, so the same results.  If a wait gives
+// up: NaN prices and sync[2n] = 1.
+__global__ void __launch_bounds__(kGramThreads)
+    k_ls_portfolio(const LsItem* __restrict__ items, int n, int total, int max_ex,
+                   unsigned* __restrict__ sync) {
+  __shared__ int flag;
+  __shared__ double beta[kMaxGram];
+  const int g = gridDim.x;
+  const int mine = (total - static_cast<int>(blockIdx.x) + g - 1) / g;
+  for (int r = 0; r <= max_ex; ++r) {
+    // the block's LS blocks of this round (at most 32 are tracked: each as
+    // soon as its product is ready; with more than 32, in order)
+    unsigned todo = mine >= 32 ? 0xFFFFFFFFu : ((1u << mine) - 1u);
+    unsigned polls = 0;
+    for (int i = 0; mine > 32 && i < mine; ++i) {
+      const int b = blockIdx.x + i * g;
+      const int q = find_block_owner(items, n, b, &LsItem::ls_block0);
+      if (items[q].E - 1 - r < -1) continue;
+      if (r > 0 && !wait_at_least(sync + 2 * q + 1, static_cast<unsigned>(r), &flag)) goto fail;
+      ls_round_work(items, q, r, b, beta, sync, &flag);
+    }
+    while (mine <= 32 && todo) {
+      bool progressed = false;
+      for (int i = 0; i < mine; ++i) {
+        if (!((todo >> i) & 1u)) continue;
+        const int b = blockIdx.x + i * g;
+        const int q = find_block_owner(items, n, b, &LsItem::ls_block0);
+        if (items[q].E - 1 - r >= -1) {  // else finished in an earlier round
+          if (r > 0 && !reached(sync + 2 * q + 1, static_cast<unsigned>(r), &flag)) continue;
+          ls_round_work(items, q, r, b, beta, sync, &flag);
+        }
+        todo &= ~(1u << i);
+        progressed = true;
+      }
+      if (!progressed) {
+        __builtin_amdgcn_s_sleep(4);
+        if (++polls > kBarrierPolls) goto fail;
+      }
+    }
+  }
+  return;
+fail:
+  if (threadIdx.x == 0) {
+    sync[2 * n] = 1u;
+    for (int q = 0; q < n; ++q) *gmem(items[q].price_out) = __builtin_nan("");
+  }
 }
 
 }  // namespace
@@ -703,6 +1015,34 @@ extern "C" int ls_first_pass_gpu(int n_ex, const float* basis_variables, const i
   const int ng = gram_blocks(paths);
   hipStream_t st = as_stream(stream);
   const bool cross = use_cross != 0;
+  // One persistent launch over all dates (after zeroing its sync words) when
+  // every date's basis has at most 7 functions, unless the blocks cannot all
+  // be resident or KOO_LS_LAUNCH_PER_DATE=1 asks for one launch per date
+  // (the same per-date code and partials).
+  const char* per_date = getenv("KOO_LS_LAUNCH_PER_DATE");
+  bool small = true;  // the persistent kernel takes bases of at most 7 functions
+  for (int e = 0; e < n_ex; ++e) small = small && basis_count(vars_each_step[e], cross) + 1 <= 8;
+  const int np = (!small || (per_date && per_date[0] == '1')) ? 0 : ls_persistent_blocks(paths);
+  if (np > 0) {
+    AllDates d{};
+    d.E = n_ex;
+    d.P = paths;
+    d.max_vars = max_vars;
+    d.maxd = basis_count(max_vars, cross);
+    d.cross = cross;
+    d.itm_only = itm_only != 0;
+    for (int e = 0; e < n_ex; ++e) {
+      d.v[e] = static_cast<signed char>(vars_each_step[e]);
+      d.k[e] = static_cast<signed char>(basis_count(vars_each_step[e], cross));
+      fill_norm(var_shift, var_scale, e, vars_each_step[e], max_vars, d.sh[e], d.sc[e]);
+    }
+    unsigned* bar = reinterpret_cast<unsigned*>(ws + lay.bar);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, bar, 3);
+    hipLaunchKernelGGL(k_ls_first_pass, dim3(np), dim3(kGramThreads), 0, st, d, basis_variables,
+                       cash_flows, exercise_values, numeraires, coefficients, Y, parts[0],
+                       parts[1], bar, price_out);
+    return check_launch("ls_first_pass_gpu");
+  }
   DateArgs a{};
   a.E = n_ex;
   a.P = paths;
@@ -1088,8 +1428,18 @@ extern "C" int reduceOuterProduct_cublas(int paths, int row_size, int col_size, 
 // ---------------------------------------------------------------------------
 // Batched portfolio: path quantities and LS first passes.
 // ---------------------------------------------------------------------------
+// The descriptor table, then (at the next 256-byte boundary) the persistent
+// LS kernel's sync words: two per product (LS blocks done, solves published)
+// and an error word.
+size_t ls_portfolio_bar_offset(int n_products) {
+  return align256(sizeof(LsItem) * static_cast<size_t>(n_products));
+}
+int ls_portfolio_sync_words(int n_products) { return 2 * n_products + 1; }
+
 extern "C" size_t bermudan_ls_portfolio_descriptor_bytes(int n_products) {
-  return n_products > 0 ? sizeof(LsItem) * static_cast<size_t>(n_products) : 0;
+  return n_products > 0 ? ls_portfolio_bar_offset(n_products) +
+                              align256(sizeof(unsigned) * ls_portfolio_sync_words(n_products))
+                        : 0;
 }
 
 extern "C" int bermudan_ls_portfolio_prepare(int n_products, const koo_ls_product* products,
@@ -1219,6 +1569,26 @@ extern "C" int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan,
     const size_t lds = sizeof(float) * (static_cast<size_t>(nc) * 256 + step_table_floats(nc));
     hipLaunchKernelGGL(k_berm_paths_batch, dim3(nb), dim3(256), lds, st, items, plan->products,
                        b0);
+  }
+  // The LS first passes: one persistent launch (after zeroing its barrier
+  // words) with as many blocks as can be resident, or, when none can or
+  // KOO_LS_LAUNCH_PER_DATE=1, one launch per round and one for the prices.
+  int per_cu = 0;
+  const char* per_date = getenv("KOO_LS_LAUNCH_PER_DATE");
+  if (!(per_date && per_date[0] == '1') &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ls_portfolio, kGramThreads, 0) ==
+          hipSuccess &&
+      per_cu > 0) {
+    const int cap = per_cu * device_cu_count();
+    const int grid = plan->ls_blocks < cap ? plan->ls_blocks : cap;
+    unsigned* bar = reinterpret_cast<unsigned*>(static_cast<char*>(const_cast<void*>(
+                                                    descriptors_device)) +
+                                                ls_portfolio_bar_offset(plan->products));
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, bar,
+                       ls_portfolio_sync_words(plan->products));
+    hipLaunchKernelGGL(k_ls_portfolio, dim3(grid), dim3(kGramThreads), 0, st, items,
+                       plan->products, plan->ls_blocks, plan->max_ex, bar);
+    return check_launch(who);
   }
   for (int r = 0; r <= plan->max_ex; ++r)
     hipLaunchKernelGGL(k_ls_round, dim3(plan->ls_blocks), dim3(kGramThreads), 0, st, items,

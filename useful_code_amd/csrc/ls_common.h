@@ -96,10 +96,46 @@ __device__ __forceinline__ void block_gram(const floatx4& acc, int n, float* lds
   }
 }
 
+// block_gram for per-lane FP64 accumulators (the LS first pass adds each
+// MFMA's 4-path FP32 result into FP64, so its Gram carries no FP32 rounding
+// beyond that of the products and 4-term sums): the same layout and
+// partials, the waves' tiles staged in FP64.
+template <int GROUPS, bool Agent = false>
+__device__ __forceinline__ void block_gram_d(const double (&acc)[4], int n,
+                                             double* lds_d /* [nw][256] */, double* part,
+                                             int block, int nblocks) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* d = lds_d + wave * 256;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
+  __syncthreads();
+  const int q = lane & 15;
+  for (int t = wave * 4 + (lane >> 4); t - (lane >> 4) < n * n; t += nw * 4) {
+    double s = 0.0;
+    if (t < n * n) {
+      const int i = t / n, j = t - i * n;
+      for (int w = q; w < nw; w += 16) {
+        const double* dw = lds_d + w * 256;
+        s += dw[i * 16 + j];
+        if (GROUPS == 2) s += dw[(8 + i) * 16 + 8 + j];
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (q == 0 && t < n * n) {
+      if (Agent)
+        store_agent(part + static_cast<size_t>(t) * nblocks + block, s);
+      else
+        part[static_cast<size_t>(t) * nblocks + block] = s;
+    }
+  }
+}
+
 // Sums the entry-major partials part[t * nb + b] over b < nb into sum[t]
 // (LDS) for t < nn: each wavefront takes 4 entries at a time and each lane 4
 // blocks of each (all 16 loads in flight, coalesced), then wavefront sums.
-// Contains a barrier.
+// Contains a barrier.  Agent: agent-coherent loads.
+template <bool Agent = false>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nb, int nn,
                                                 double* sum) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -112,7 +148,8 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int t = t0 + u, b = b0 + lane + 64 * r;
-          v[u][r] = (t < nn && b < nb) ? part[static_cast<size_t>(t) * nb + b] : 0.0;
+          const double* src = part + static_cast<size_t>(t) * nb + b;
+          v[u][r] = (t < nn && b < nb) ? (Agent ? load_agent(src) : *src) : 0.0;
         }
 #pragma unroll
       for (int u = 0; u < 4; ++u) s[u] += (v[u][0] + v[u][1]) + (v[u][2] + v[u][3]);
@@ -161,37 +198,117 @@ __device__ bool cholesky_rhs(const double* G, int ld, int k, double inv, double*
   return true;
 }
 
-// The same solve with the factor in registers (K known at compile time, all
-// loops unrolled): G in LDS, beta out.  Returns false (beta = 0) when a
-// pivot is not positive.
+// The gold's SVD tolerance for the LS normal equations: eigen-directions of
+// M below this fraction of the largest eigenvalue are dropped
+// (SVDSquareMatrix::SVDSolve, gold/math/svd_gold.cpp:560-596, called with
+// 1e-8 by gold/MultiLS_regression.cpp:172 and by the reference GPU's host
+// solver, LinearSolverConcrete_main.cu:127-188).
+constexpr double kSvdTolerance = 1e-8;
+
+// Eigen-solve of the symmetric k x k system M x = b with eigenvalues below
+// tol times the largest dropped (the truncated SVD solve of a symmetric
+// matrix): cyclic Jacobi (Golub and Van Loan, Matrix Computations, 4th ed.,
+// 8.5.2) on A (LDS, ld 16, overwritten) with eigenvectors in V (LDS).
+// Single lane.
+__device__ void jacobi_truncated(int k, double* A, double* V, const double* b, double tol,
+                                 double* x) {
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) V[i * 16 + j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        const double q = A[i * 16 + j] * A[i * 16 + j];
+        tot += q;
+        if (i != j) off += q;
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < k - 1; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = A[p * 16 + q];
+        if (apq == 0.0) continue;
+        const double tau = (A[q * 16 + q] - A[p * 16 + p]) / (2.0 * apq);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        for (int r = 0; r < k; ++r) {
+          const double arp = A[r * 16 + p], arq = A[r * 16 + q];
+          A[r * 16 + p] = c * arp - s * arq;
+          A[r * 16 + q] = s * arp + c * arq;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double apr = A[p * 16 + r], aqr = A[q * 16 + r];
+          A[p * 16 + r] = c * apr - s * aqr;
+          A[q * 16 + r] = s * apr + c * aqr;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double vrp = V[r * 16 + p], vrq = V[r * 16 + q];
+          V[r * 16 + p] = c * vrp - s * vrq;
+          V[r * 16 + q] = s * vrp + c * vrq;
+        }
+      }
+  }
+  double lmax = 0.0;
+  for (int i = 0; i < k; ++i) lmax = fmax(lmax, fabs(A[i * 16 + i]));
+  for (int i = 0; i < k; ++i) x[i] = 0.0;
+  for (int i = 0; i < k; ++i) {
+    const double l = A[i * 16 + i];
+    if (!(lmax > 0.0) || fabs(l) < tol * lmax) continue;
+    double e = 0.0;
+    for (int r = 0; r < k; ++r) e += V[r * 16 + i] * b[r];
+    e /= l;
+    for (int r = 0; r < k; ++r) x[r] += e * V[r * 16 + i];
+  }
+}
+
+// Cholesky solve with the factor in registers (K known at compile time, all
+// loops unrolled) of M beta = t, M = G[0..K)[0..K) * inv, t = G[.][K] * inv.
+// Succeeds only when M is positive definite and 1 / trace(M^-1) >= tol *
+// trace(M) -- a lower bound of the smallest eigenvalue against an upper bound
+// of the largest, so no eigenvalue is below tol times the largest and the
+// truncated solve would keep every direction: then it equals the truncated
+// solve.  Returns false (beta untouched) otherwise.
 template <int K>
-__device__ bool cholesky_rhs_reg(const double* G, int ld, double inv, double* beta) {
-  double L[K][K], y[K];
+__device__ bool cholesky_rhs_reg(const double* G, int ld, double inv, double tol, double* beta) {
+  double L[K][K], y[K], d[K];
+  double trM = 0.0;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
 #pragma unroll
     for (int j = 0; j <= i; ++j) {
       double s = G[i * ld + j] * inv;
+      if (i == j) trM += s;
 #pragma unroll
       for (int q = 0; q < j; ++q) s -= L[i][q] * L[j][q];
       if (i == j) {
-        if (!(s > 0.0)) {
-#pragma unroll
-          for (int r = 0; r < K; ++r) beta[r] = 0.0;
-          return false;
-        }
+        if (!(s > 0.0)) return false;
         L[i][i] = sqrt(s);
+        d[i] = 1.0 / L[i][i];
       } else {
-        L[i][j] = s / L[j][j];
+        L[i][j] = s * d[j];
       }
     }
   }
+  // trace(M^-1) = ||L^-1||_F^2, column by column of L^-1
+  double trInv = 0.0;
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    double z[K];
+#pragma unroll
+    for (int i = c; i < K; ++i) {
+      double s = i == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = c; q < i; ++q) s -= L[i][q] * z[q];
+      z[i] = s * d[i];
+      trInv += z[i] * z[i];
+    }
+  }
+  if (trInv * tol * trM > 1.0) return false;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     double s = G[i * ld + K] * inv;
 #pragma unroll
     for (int q = 0; q < i; ++q) s -= L[i][q] * y[q];
-    y[i] = s / L[i][i];
+    y[i] = s * d[i];
   }
   double x[K];
 #pragma unroll
@@ -199,20 +316,118 @@ __device__ bool cholesky_rhs_reg(const double* G, int ld, double inv, double* be
     double s = y[i];
 #pragma unroll
     for (int q = i + 1; q < K; ++q) s -= L[q][i] * x[q];
-    x[i] = s / L[i][i];
+    x[i] = s * d[i];
   }
 #pragma unroll
   for (int i = 0; i < K; ++i) beta[i] = x[i];
   return true;
 }
 
+// The same truncated eigen-solve with the matrix in registers (K known at
+// compile time, loops over the rotation pairs unrolled; only the upper
+// triangle is kept): each rotation J of the cyclic sweep (Golub and Van Loan
+// 8.4.2: the (p,q) entry zeroed, the diagonal updated by -/+ t a_pq) is also
+// applied to the right-hand side, b' = J^T b', and its (c, s) logged in LDS;
+// at convergence the kept eigenvalues give y = Lambda^+ b' and the solution
+// x = J_1 ... J_m y replays the log backwards.  M = G[0..K)[0..K) * inv,
+// rhs G[.][K] * inv.  Single lane; rot: 2 * kJacobiSweeps * K(K-1)/2 LDS
+// doubles.
+constexpr int kJacobiSweeps = 32;
+
+template <int K>
+__device__ void jacobi_truncated_reg(const double* G, int ld, double inv, double tol, double* x,
+                                     double* rot) {
+  constexpr int NP = K * (K - 1) / 2;
+  double a[K][K], b[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+#pragma unroll
+    for (int j = i; j < K; ++j) a[i][j] = G[i * ld + j] * inv;
+    b[i] = G[i * ld + K] * inv;
+  }
+  int sweeps = 0;
+  for (; sweeps < kJacobiSweeps; ++sweeps) {
+    double off = 0.0, tot = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      tot += a[i][i] * a[i][i];
+#pragma unroll
+      for (int j = i + 1; j < K; ++j) off += 2.0 * a[i][j] * a[i][j];
+    }
+    if (off <= 1e-30 * (tot + off)) break;
+    double* log = rot + 2 * sweeps * NP;
+    int idx = 0;
+#pragma unroll
+    for (int p = 0; p < K - 1; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < K; ++q, ++idx) {
+        const double apq = a[p][q];
+        double c = 1.0, sn = 0.0;
+        if (apq != 0.0) {
+          const double tau = (a[q][q] - a[p][p]) / (2.0 * apq);
+          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          sn = t * c;
+          a[p][p] -= t * apq;
+          a[q][q] += t * apq;
+          a[p][q] = 0.0;
+#pragma unroll
+          for (int r = 0; r < K; ++r) {
+            if (r == p || r == q) continue;
+            double& arp = r < p ? a[r][p] : a[p][r];
+            double& arq = r < q ? a[r][q] : a[q][r];
+            const double vp = arp, vq = arq;
+            arp = c * vp - sn * vq;
+            arq = sn * vp + c * vq;
+          }
+          const double bp = b[p], bq = b[q];
+          b[p] = c * bp - sn * bq;
+          b[q] = sn * bp + c * bq;
+        }
+        log[2 * idx] = c;
+        log[2 * idx + 1] = sn;
+      }
+    }
+  }
+  double lmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) lmax = fmax(lmax, fabs(a[i][i]));
+  double y[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+    y[i] = (lmax > 0.0 && !(fabs(a[i][i]) < tol * lmax)) ? b[i] / a[i][i] : 0.0;
+  for (int sw = sweeps - 1; sw >= 0; --sw) {
+    const double* log = rot + 2 * sw * NP;
+    int idx = NP - 1;
+#pragma unroll
+    for (int p = K - 2; p >= 0; --p) {
+#pragma unroll
+      for (int q = K - 1; q > p; --q, --idx) {
+        const double c = log[2 * idx], sn = log[2 * idx + 1];
+        const double yp = y[p], yq = y[q];
+        y[p] = c * yp + sn * yq;
+        y[q] = -sn * yp + c * yq;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i) x[i] = y[i];
+}
+
+// Scratch (LDS doubles) the truncated solves need: A and V (16 x 16) and b
+// for the LDS form, the rotation log of the register form.
+constexpr int kSolveScratch =
+    (2 * 256 + 16) > (2 * kJacobiSweeps * 21) ? (2 * 256 + 16) : (2 * kJacobiSweeps * 21);
+
 // LS normal equations from a (k+1) x (k+1) Gram matrix whose row 0 is the
-// constant function (so G[0][0] is the path count) and row k the target:
+// constant function (so G[0][0] is the path count) and row k the target,
 // mean-scaled as in GenerateLSRegressionMatrixSelectively_gold
-// (gold/LS_regression.h:431-519); beta = 0 when no path entered or the
-// system is not positive definite.
-__device__ void solve_normal_equations(const double* G, int k, double* beta, double* L,
-                                       double* y) {
+// (gold/LS_regression.h:431-519), solved like the reference's truncated SVD
+// with tolerance 1e-8: the register Cholesky when the trace bound shows no
+// direction is dropped (the usual case), else the Jacobi eigen-solve with the
+// cut.  beta = 0 when no path entered.  Single lane; scratch: kSolveScratch
+// LDS doubles.
+__device__ void solve_normal_equations(const double* G, int k, double* beta, double* scratch) {
   const double cnt = G[0];
   if (!(cnt > 0.0)) {
     for (int q = 0; q < k; ++q) beta[q] = 0.0;
@@ -220,14 +435,32 @@ __device__ void solve_normal_equations(const double* G, int k, double* beta, dou
   }
   const double inv = 1.0 / cnt;
   const int ld = k + 1;
-  switch (k) {  // the basis sizes of 0..4 variables, cross or plain
-    case 1: cholesky_rhs_reg<1>(G, ld, inv, beta); return;
-    case 3: cholesky_rhs_reg<3>(G, ld, inv, beta); return;
-    case 5: cholesky_rhs_reg<5>(G, ld, inv, beta); return;
-    case 6: cholesky_rhs_reg<6>(G, ld, inv, beta); return;
-    case 7: cholesky_rhs_reg<7>(G, ld, inv, beta); return;
-    default: cholesky_rhs(G, ld, k, inv, beta, L, y); return;
+  bool done = false;
+  switch (k) {  // the basis sizes of 0..3 variables, cross or plain
+    case 1: done = cholesky_rhs_reg<1>(G, ld, inv, kSvdTolerance, beta); break;
+    case 3: done = cholesky_rhs_reg<3>(G, ld, inv, kSvdTolerance, beta); break;
+    case 5: done = cholesky_rhs_reg<5>(G, ld, inv, kSvdTolerance, beta); break;
+    case 6: done = cholesky_rhs_reg<6>(G, ld, inv, kSvdTolerance, beta); break;
+    case 7: done = cholesky_rhs_reg<7>(G, ld, inv, kSvdTolerance, beta); break;
+    default: break;
   }
+  if (done) return;
+  switch (k) {
+    case 1: jacobi_truncated_reg<1>(G, ld, inv, kSvdTolerance, beta, scratch); return;
+    case 3: jacobi_truncated_reg<3>(G, ld, inv, kSvdTolerance, beta, scratch); return;
+    case 5: jacobi_truncated_reg<5>(G, ld, inv, kSvdTolerance, beta, scratch); return;
+    case 6: jacobi_truncated_reg<6>(G, ld, inv, kSvdTolerance, beta, scratch); return;
+    case 7: jacobi_truncated_reg<7>(G, ld, inv, kSvdTolerance, beta, scratch); return;
+    default: break;
+  }
+  double* A = scratch;
+  double* V = scratch + 256;
+  double* b = scratch + 512;
+  for (int i = 0; i < k; ++i) {
+    for (int j = 0; j < k; ++j) A[i * 16 + j] = G[i * ld + j] * inv;
+    b[i] = G[i * ld + k] * inv;
+  }
+  jacobi_truncated(k, A, V, b, kSvdTolerance, beta);
 }
 
 // Every block of a date kernel solves the later date's normal equations
@@ -235,16 +468,27 @@ __device__ void solve_normal_equations(const double* G, int k, double* beta, dou
 // the entry-major partials into LDS, the FP64 solve by lane 0, the
 // coefficients left in beta (LDS).  Block 0 also stores them.  Contains
 // barriers.
+template <bool Agent = false>
 __device__ __forceinline__ void solve_date(const double* __restrict__ part, int nb, int k,
-                                           double* gram, double* chol, double* beta,
+                                           double* gram, double* scratch, double* beta,
                                            double* __restrict__ coeffs_row, int maxd, int block) {
-  reduce_partials(part, nb, (k + 1) * (k + 1), gram);
+  reduce_partials<Agent>(part, nb, (k + 1) * (k + 1), gram);
   if (threadIdx.x == 0) {
-    solve_normal_equations(gram, k, beta, chol, chol + kMaxGram * kMaxGram);
+    solve_normal_equations(gram, k, beta, scratch);
     if (block == 0)
       for (int i = 0; i < maxd; ++i) coeffs_row[i] = i < k ? beta[i] : 0.0;
   }
   __syncthreads();
+}
+
+// Polls of a persistent kernel's waits: seconds, where a normal wait takes a
+// few microseconds; a wait that reaches it gives up so the kernel can end.
+constexpr unsigned kBarrierPolls = 1u << 22;
+
+// Zeroes n 32-bit words (the barrier counters before a persistent launch;
+// a kernel, not a memset, so the launch sequence can be captured in a graph).
+__global__ void k_zero_words(unsigned* __restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
 }  // namespace

@@ -47,7 +47,6 @@ constexpr int kMultiThreads = 1024;  // multi-block kernels (16 wavefronts)
 constexpr int kSearchThreads = 256;
 constexpr int kMaxDepth = 8;         // regressions per exercise date
 constexpr int kSelectMaxSteps = 1000;
-constexpr double kSvdTolerance = 1e-8;  // gold/MultiLS_regression.cpp:172
 constexpr unsigned long long kKeySentinel = 0x7FFFFFFFFFFFFFFFull;
 
 __device__ __forceinline__ doublex4 mfma16x16x4_f64(double a, double b, doublex4 c) {
@@ -319,62 +318,6 @@ __global__ void __launch_bounds__(kMultiThreads)
 // ---------------------------------------------------------------------------
 // Normal-equation solve and net continuation values.
 // ---------------------------------------------------------------------------
-
-// Eigen-solve of the symmetric k x k system M x = b with eigenvalues below
-// tol times the largest dropped (the truncated SVD solve of
-// SVDSquareMatrix::SVDSolve, gold/math/svd_gold.cpp:560-596, for a symmetric
-// matrix): cyclic Jacobi (Golub and Van Loan, Matrix Computations, 4th ed.,
-// 8.5.2) on A (LDS, ld 16, overwritten) with eigenvectors in V (LDS).
-// Single lane.
-__device__ void jacobi_truncated(int k, double* A, double* V, const double* b, double tol,
-                                 double* x) {
-  for (int i = 0; i < k; ++i)
-    for (int j = 0; j < k; ++j) V[i * 16 + j] = i == j ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < k; ++i)
-      for (int j = 0; j < k; ++j) {
-        const double q = A[i * 16 + j] * A[i * 16 + j];
-        tot += q;
-        if (i != j) off += q;
-      }
-    if (off <= 1e-30 * tot) break;
-    for (int p = 0; p < k - 1; ++p)
-      for (int q = p + 1; q < k; ++q) {
-        const double apq = A[p * 16 + q];
-        if (apq == 0.0) continue;
-        const double tau = (A[q * 16 + q] - A[p * 16 + p]) / (2.0 * apq);
-        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        for (int r = 0; r < k; ++r) {
-          const double arp = A[r * 16 + p], arq = A[r * 16 + q];
-          A[r * 16 + p] = c * arp - s * arq;
-          A[r * 16 + q] = s * arp + c * arq;
-        }
-        for (int r = 0; r < k; ++r) {
-          const double apr = A[p * 16 + r], aqr = A[q * 16 + r];
-          A[p * 16 + r] = c * apr - s * aqr;
-          A[q * 16 + r] = s * apr + c * aqr;
-        }
-        for (int r = 0; r < k; ++r) {
-          const double vrp = V[r * 16 + p], vrq = V[r * 16 + q];
-          V[r * 16 + p] = c * vrp - s * vrq;
-          V[r * 16 + q] = s * vrp + c * vrq;
-        }
-      }
-  }
-  double lmax = 0.0;
-  for (int i = 0; i < k; ++i) lmax = fmax(lmax, fabs(A[i * 16 + i]));
-  for (int i = 0; i < k; ++i) x[i] = 0.0;
-  for (int i = 0; i < k; ++i) {
-    const double l = A[i * 16 + i];
-    if (!(lmax > 0.0) || fabs(l) < tol * lmax) continue;
-    double e = 0.0;
-    for (int r = 0; r < k; ++r) e += V[r * 16 + i] * b[r];
-    e /= l;
-    for (int r = 0; r < k; ++r) x[r] += e * V[r * 16 + i];
-  }
-}
 
 // The normal equations M beta = t with M = G[0..k)[0..k) / n, t = G[.][k] / n
 // (G: (k+1) x (k+1) Gram of basis and target).  Cholesky first: when it
