@@ -547,6 +547,21 @@ int ls_first_pass_gpu(int n_ex, const float* basis_variables, const int* vars_ea
                       const float* var_shift, const float* var_scale, void* workspace,
                       double* coefficients, double* price_out, void* stream);
 
+/* No counterpart in the reference.  ls_first_pass_gpu for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): no workspace argument; the pass uses the
+ * library's own device array (16 MiB, one per GPU, held until the process
+ * ends), so ls_workspace(paths) must not exceed 16 MiB (up to 3 669 952
+ * paths on a 256-CU device; larger sizes are rejected with KOO_ERR_INVALID
+ * before anything runs).  Calls on one GPU share that array: calls that overlap on a GPU (two
+ * streams, two host threads) race and must not be issued.  Same results as
+ * ls_first_pass_gpu.  Calls no allocating routine. */
+int ls_first_pass_noalloc_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,
+                              int max_vars, const float* cash_flows, const float* exercise_values,
+                              const float* numeraires, int paths, int use_cross, int itm_only,
+                              const float* var_shift, const float* var_scale,
+                              double* coefficients, double* price_out, void* stream);
+
 /* generateMultiRegressionCoefficientsViaLS_gpu_flexisolve (LS_main_cula.h:
  * 565-838; the caller in the reference's test is
  * generateRegressionCoefficientsViaLSMultiquadratic_flexi_main,

@@ -136,6 +136,7 @@ _SIGNATURES = {
     "bermudan_portfolio_bwd": (_i, [_p, _p, _p]),
     "ls_workspace": (_i, [_i, _p]),
     "ls_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
+    "ls_first_pass_noalloc_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 5),
     "ls_multi_workspace": (_i, [_i, _p]),
     "ls_multi_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _i, _d, _d,
                                      _d, _d] + [_p] * 8),
@@ -518,6 +519,18 @@ def ls_first_pass_gpu(n_ex, basis_variables, vars_each_step, max_vars, cash, ev,
     call("ls_first_pass_gpu", n_ex, ptr(basis_variables), ves.ctypes.data, max_vars, ptr(cash),
          ptr(ev), ptr(num), paths, int(use_cross), int(itm_only), ptr(sh), ptr(sc),
          ptr(workspace), ptr(coefficients), ptr(price_out), stream_handle(stream))
+
+
+def ls_first_pass_noalloc_gpu(n_ex, basis_variables, vars_each_step, max_vars, cash, ev, num,
+                              paths, use_cross, itm_only, var_shift, var_scale, coefficients,
+                              price_out, stream=None):
+    """ls_first_pass_gpu with the library's own per-GPU scratch array as the
+    workspace (no allocation; calls on one GPU must not overlap)."""
+    ves = _host_ints(vars_each_step)
+    sh, sc = _host_f32(var_shift), _host_f32(var_scale)
+    call("ls_first_pass_noalloc_gpu", n_ex, ptr(basis_variables), ves.ctypes.data, max_vars,
+         ptr(cash), ptr(ev), ptr(num), paths, int(use_cross), int(itm_only), ptr(sh), ptr(sc),
+         ptr(coefficients), ptr(price_out), stream_handle(stream))
 
 
 def ls_multi_workspace(paths):
