@@ -58,7 +58,7 @@ def rate_updates_per_path(N):
     return N * (N + 1) // 2  # sum over steps s of alive rates N - s
 
 
-# Bytes one k_ls_date launch must move per path (FP32): this date's two basis
+# Bytes the LS first pass must move per path and exercise date (FP32): the two basis
 # variables, exercise value and numeraire, the later date's two variables,
 # exercise value and numeraire for its exercise decision, the continuation
 # value read and written.
@@ -80,6 +80,40 @@ def profiled_traffic(kernel, grid):
                 if rec.get("kernel") == kernel and rec.get("grid") == grid and rec.get("hbm_bytes"):
                     return float(rec["hbm_bytes"]), os.path.relpath(fn, ROOT)
     return None, None
+
+
+def profiled_mfma(kernel, grid):
+    """MFMA counter record of `kernel` at `grid` from the `mfma_json:` lines of
+    the counter summaries under profiles/ (executed MFMA flops = 512 x
+    SQ_INSTS_VALU_MFMA_MOPS_F32/_F64, busy = SQ_VALU_MFMA_BUSY_CYCLES over
+    GRBM_GUI_ACTIVE / 8 XCDs x 1024 SIMDs), with the file name; (None, None) when absent."""
+    import glob
+    for fn in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(fn) as f:
+            for line in f:
+                if line.startswith("mfma_json: "):
+                    rec = json.loads(line[len("mfma_json: "):])
+                    if rec.get("kernel") == kernel and rec.get("grid") == grid:
+                        return rec, os.path.relpath(fn, ROOT)
+    return None, None
+
+
+def with_mfma(roof, kernel, grid, kernel_ms):
+    """Adds `mfma`: the profiled executed MFMA flops per dispatch over this
+    run's live kernel time, against the dense MFMA peak of the type the
+    counters saw, and the profiled MFMA busy fraction."""
+    rec, src = profiled_mfma(kernel, grid)
+    if rec is None:
+        roof["mfma"] = None
+        return roof
+    if kernel_ms is None:  # no live time of this kernel alone: the trace's average
+        kernel_ms = rec["avg_us"] * 1e-3
+    tf = rec["mfma_flops"] / (kernel_ms * 1e-3) / 1e12
+    roof["mfma"] = {"kernel": kernel, "executed_flops": rec["mfma_flops"],
+                    "achieved": tf, "peak": rec["peak_tflops"], "unit": "TFLOP/s",
+                    "frac": tf / rec["peak_tflops"], "busy_profiled": rec["mfma_busy"],
+                    "source": src}
+    return roof
 
 
 def with_traffic(roof, kernel, grid):
@@ -157,9 +191,10 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
         "price_second_pass_same_paths": price2,
         "itm_fraction_mean": itm,
         "roofline": with_traffic({
-            "kernel": "LS first pass (ls_first_pass_gpu): per date k_ls_date (the later "
-                      "date's FP64 sum and Cholesky solve, decision update, MFMA normal "
-                      "equations), then k_ls_final",
+            "kernel": "LS first pass (ls_first_pass_gpu): k_zero_words, then one persistent "
+                      "k_ls_first_pass (per date: MFMA normal equations with FP64 accumulation, "
+                      "the last block to arrive solves and publishes, decision update; then "
+                      "the price)",
             "bound": "hbm",
             "achieved": gbs,
             "peak": HBM_PEAK_GBS,
@@ -170,6 +205,7 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
             "mfma_useful_tflops": mfma_flops / (regress * 1e-3) / 1e12,
         }, "ls_first_pass_gpu", P),
     }
+    with_mfma(res["roofline"], "k_ls_first_pass", P, regress)
     if not args.no_error_check:
         ints = O.scramble(O.sobol_ints(P, w.dims, 0), w.scramblers, w.dims)
         z = O.bridge_transform(O.invnorm_u32(ints), S, w.factors)
@@ -229,6 +265,7 @@ def multi_ls_section(args, torch, O, configs, pipeline, ev):
         "price_first_pass": price.item(),
         "nested_bands": int((up.view(E, R)[:, :2] > 0).sum().item()),
     }
+    with_mfma(res, "k_multi_gram<2>", P, None)
     if not args.no_error_check:
         bv = pr.bv.view(E, 2, P).cpu().double().numpy()
         evo = pr.ev.view(E, P).cpu().double().numpy()
@@ -386,6 +423,7 @@ def basket_section(args, torch, O, configs, pipeline, ev):
             "mfma_useful_tflops": mfma_tflops,
         }, "k_basket_asian<2>", P // 4),
     }
+    with_mfma(res["roofline"], "k_basket_asian<2>", P // 4, kern)
     if not args.no_error_check:
         ws = configs.basket_asian(paths=1 << 14)
         ps = pipeline.BasketPricer(ws)
@@ -481,6 +519,13 @@ def stage_section(args, torch, configs, pipeline, ev):
         ("ls_normal_eq_gpu (X^T X and X^T y, 6 basis functions + target, MFMA)",
          lambda: koo.ls_normal_eq_gpu(P, kb, basis, target, None, lsws, gram), four * (kb + 1) * P),
     ]
+    # a plain device-to-device copy of the same size as the quantile's cube
+    # (read + write 2 * 4 B * D * P): what the memory system gives a streaming
+    # kernel with nothing else to do, the practical ceiling for these stages
+    copy_src = torch.empty(D * P, dtype=torch.float32, device="cuda").fill_(1.0)
+    copy_dst = torch.empty_like(copy_src)
+    stages.append(("device copy (torch, reference point)", lambda: copy_dst.copy_(copy_src),
+                   2 * four * D * P))
     res = {}
     for name, fn, nbytes in stages:
         fn()
@@ -494,6 +539,8 @@ def stage_section(args, torch, configs, pipeline, ev):
         gbs = nbytes / (ms * 1e-3) / 1e9
         res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GB_per_s": gbs,
                      "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+        if name.startswith("ls_normal_eq_gpu"):
+            with_mfma(res[name], "k_gram<2>", P // 4, ms)
     return res
 
 
