@@ -17,6 +17,12 @@ int check_launch(const char* what);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// 16-byte vector of floats for streaming loads and stores: the nontemporal
+// builtins (global_load/store ... nt) take clang vector types, not float4.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x4 load_stream(const f32x4* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void store_stream(f32x4 v, f32x4* p) { __builtin_nontemporal_store(v, p); }
+
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
 

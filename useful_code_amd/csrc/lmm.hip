@@ -305,6 +305,26 @@ __global__ void k_swap_rate(const float* __restrict__ disc, const float* __restr
 }
 
 // B(0) = 1, B(i) = B(i-1) P(i-1,i-1)/P(i-1,i)  (LMM_evolver_gpu.cu:1785-1838).
+// Four paths per lane (16-byte loads and stores) when the path counts and
+// the offset keep every row 16-byte aligned; the ratios' loads do not depend
+// on the running product, so they are all issued ahead of it.
+__global__ void __launch_bounds__(256)
+    k_numeraires_vec4(const f32x4* __restrict__ disc, f32x4* __restrict__ num, int paths4,
+                      int out_paths4, int out_offset4, int n_rates) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths4) return;
+  const size_t P = static_cast<size_t>(paths4), Q = static_cast<size_t>(out_paths4);
+  f32x4 b = {1.0f, 1.0f, 1.0f, 1.0f};
+  store_stream(b, num + out_offset4 + p);
+  for (int i = 1; i < n_rates; ++i) {
+    const f32x4* dcol = disc + static_cast<size_t>(i - 1) * (n_rates + 1) * P + p;
+    const f32x4 a = load_stream(dcol + static_cast<size_t>(i - 1) * P);
+    const f32x4 c = load_stream(dcol + static_cast<size_t>(i) * P);
+    b *= a / c;
+    store_stream(b, num + static_cast<size_t>(i) * Q + out_offset4 + p);
+  }
+}
+
 __global__ void k_numeraires(const float* __restrict__ disc, float* __restrict__ num, int paths,
                              int out_paths, int out_offset, int n_rates) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,7 +339,20 @@ __global__ void k_numeraires(const float* __restrict__ disc, float* __restrict__
   }
 }
 
-// out(i,p) = f(step[i], index[i], p)  (LMM_evolver_gpu.cu:1910-2038).
+// out(i,p) = f(step[i], index[i], p)  (LMM_evolver_gpu.cu:1910-2038): a row
+// copy per output row, so the 16-byte form has one row per blockIdx.y and
+// four paths per lane, with streaming (nontemporal) loads and stores.
+__global__ void __launch_bounds__(256)
+    k_forward_extract_vec4(const f32x4* __restrict__ fwd, const IndexVec index,
+                           const IndexVec step, int paths4, int n_rates, f32x4* __restrict__ out) {
+  const int i = blockIdx.y;
+  const size_t P4 = static_cast<size_t>(paths4);
+  const f32x4* src = fwd + (static_cast<size_t>(step.v[i]) * n_rates + index.v[i]) * P4;
+  f32x4* dst = out + static_cast<size_t>(i) * P4;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < paths4; p += gridDim.x * blockDim.x)
+    store_stream(load_stream(src + p), dst + p);
+}
+
 __global__ void k_forward_extract(const float* __restrict__ fwd, const IndexVec index,
                                   const IndexVec step, int n_out, int paths, int n_rates,
                                   float* __restrict__ out) {
@@ -558,8 +591,18 @@ extern "C" int spot_measure_numeraires_computation_offset_gpu(float* discounts, 
   if (paths == 0) return KOO_OK;
   if (!discounts || !numeraires)
     return set_error(KOO_ERR_INVALID, "spot_measure_numeraires_computation: null pointer");
-  hipLaunchKernelGGL(k_numeraires, path_grid(paths, 256), dim3(256), 0, as_stream(stream),
-                     discounts, numeraires, paths, pathsForOutput, pathsOffsetForOutput, n_rates);
+  if (paths % 4 == 0 && pathsForOutput % 4 == 0 && pathsOffsetForOutput % 4 == 0 &&
+      (reinterpret_cast<uintptr_t>(discounts) & 15u) == 0 &&
+      (reinterpret_cast<uintptr_t>(numeraires) & 15u) == 0) {
+    hipLaunchKernelGGL(k_numeraires_vec4, path_grid(paths / 4, 256), dim3(256), 0,
+                       as_stream(stream), reinterpret_cast<const f32x4*>(discounts),
+                       reinterpret_cast<f32x4*>(numeraires), paths / 4, pathsForOutput / 4,
+                       pathsOffsetForOutput / 4, n_rates);
+  } else {
+    hipLaunchKernelGGL(k_numeraires, path_grid(paths, 256), dim3(256), 0, as_stream(stream),
+                       discounts, numeraires, paths, pathsForOutput, pathsOffsetForOutput,
+                       n_rates);
+  }
   return check_launch("spot_measure_numeraires_computation");
 }
 
@@ -584,8 +627,17 @@ extern "C" int forward_rate_extraction_gpu(float* all_forwards, const int* forwa
   }
   if (paths == 0 || steps == 0) return KOO_OK;
   if (!all_forwards || !out) return set_error(KOO_ERR_INVALID, "forward_rate_extraction_gpu: null");
-  hipLaunchKernelGGL(k_forward_extract, path_grid(paths, 256), dim3(256), 0, as_stream(stream),
-                     all_forwards, idx, st, steps, paths, rates, out);
+  if (paths % 4 == 0 && (reinterpret_cast<uintptr_t>(all_forwards) & 15u) == 0 &&
+      (reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
+    const int p4 = paths / 4;
+    const unsigned gx = grid_for(p4, 256, 512);  // at most 512 blocks per row
+    hipLaunchKernelGGL(k_forward_extract_vec4, dim3(gx, steps), dim3(256), 0, as_stream(stream),
+                       reinterpret_cast<const f32x4*>(all_forwards), idx, st, p4, rates,
+                       reinterpret_cast<f32x4*>(out));
+  } else {
+    hipLaunchKernelGGL(k_forward_extract, path_grid(paths, 256), dim3(256), 0, as_stream(stream),
+                       all_forwards, idx, st, steps, paths, rates, out);
+  }
   return check_launch("forward_rate_extraction_gpu");
 }
 
