@@ -176,23 +176,50 @@ struct BridgeOrder {
   }
 };
 
-template <int S, int I, class Z>
-__device__ __forceinline__ void bridge_point(float (&w)[S], const BridgeTables& T, Z& z) {
+// V is float (one path) or f32x4 (four paths; the scalar weights broadcast).
+template <int S, int I, class V, class Z>
+__device__ __forceinline__ void bridge_point(V (&w)[S], const BridgeTables& T, Z& z) {
   constexpr BridgeOrder<S> M{};
   if constexpr (I == 0) {
     w[S - 1] = T.sd[0] * z(0);
   } else {
     constexpr int j = M.left[I], k = M.right[I], l = M.bridge[I];
-    float v = T.rw[I] * w[k];
+    V v = T.rw[I] * w[k];
     if constexpr (j != 0) v = T.lw[I] * w[j - 1] + v;
     w[l] = v + T.sd[I] * z(I);
   }
 }
 
-template <int S, class Z, int... I>
-__device__ __forceinline__ void bridge_points(float (&w)[S], const BridgeTables& T, Z& z,
+template <int S, class V, class Z, int... I>
+__device__ __forceinline__ void bridge_points(V (&w)[S], const BridgeTables& T, Z& z,
                                               std::integer_sequence<int, I...>) {
   (bridge_point<S, I>(w, T, z), ...);
+}
+
+// In-place form for variates that come from memory: w[bridge(i)] holds
+// variate i on entry (so all S loads can be in flight at once, into the
+// registers the points end in), and point bridge(i) replaces it; the same
+// operations in the same order as bridge_point.
+template <int S, int I, class V>
+__device__ __forceinline__ void bridge_point_inplace(V (&w)[S], const BridgeTables& T) {
+  constexpr BridgeOrder<S> M{};
+  if constexpr (I == 0) {
+    w[S - 1] = T.sd[0] * w[S - 1];
+  } else {
+    constexpr int j = M.left[I], k = M.right[I], l = M.bridge[I];
+    V v = T.rw[I] * w[k];
+    if constexpr (j != 0) v = T.lw[I] * w[j - 1] + v;
+    w[l] = v + T.sd[I] * w[l];
+  }
+}
+
+template <int S, class V, class Load, int... I>
+__device__ __forceinline__ void bridge_points_loaded(V (&w)[S], const BridgeTables& T,
+                                                     Load load,
+                                                     std::integer_sequence<int, I...>) {
+  constexpr BridgeOrder<S> M{};
+  ((w[M.bridge[I]] = load(I)), ...);
+  (bridge_point_inplace<S, I>(w, T), ...);
 }
 
 // Builds the lane's path from z(i) (bridge variate i, i a constant) and
@@ -218,8 +245,41 @@ __global__ void __launch_bounds__(kBridgeThreads)
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= paths) return;
   const int* idx = alloc + blockIdx.y * S;
-  bridge_store<S>(T, [&](int i) { return in[static_cast<size_t>(idx[i]) * paths + p]; },
-                  factors, blockIdx.y, paths, p, true, out);
+  const size_t P = static_cast<size_t>(paths);
+  float w[S];
+  bridge_points_loaded<S>(
+      w, T, [&](int i) { return in[static_cast<size_t>(idx[i]) * P + p]; },
+      std::make_integer_sequence<int, S>{});
+  float prev = 0.0f;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    out[(static_cast<size_t>(i) * factors + blockIdx.y) * P + p] = w[i] - prev;
+    prev = w[i];
+  }
+}
+
+// Four paths per lane (16-byte streaming loads and stores, four times the
+// contiguous bytes per wavefront and row) for S <= kRegSteps4.
+constexpr int kRegSteps4 = 48;
+
+template <int S>
+__global__ void __launch_bounds__(kBridgeThreads)
+    k_bridge_reg4(const BridgeTables T, int factors, int paths4, const int* __restrict__ alloc,
+                  const f32x4* __restrict__ in, f32x4* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths4) return;
+  const size_t P4 = static_cast<size_t>(paths4);
+  const int* idx = alloc + blockIdx.y * S;
+  f32x4 w[S];
+  bridge_points_loaded<S>(
+      w, T, [&](int i) { return load_stream(in + static_cast<size_t>(idx[i]) * P4 + p); },
+      std::make_integer_sequence<int, S>{});
+  f32x4 prev = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    store_stream(w[i] - prev, out + (static_cast<size_t>(i) * factors + blockIdx.y) * P4 + p);
+    prev = w[i];
+  }
 }
 
 // Fused generator on the register bridge.  Each wavefront walks a contiguous
@@ -308,7 +368,13 @@ template <int... I>
 constexpr auto reg_gen_table(std::integer_sequence<int, I...>) {
   return std::array<RegGenFn, sizeof...(I)>{&k_sobol_bridge_reg<I + 1>...};
 }
+using RegBridge4Fn = void (*)(const BridgeTables, int, int, const int*, const f32x4*, f32x4*);
+template <int... I>
+constexpr auto reg_bridge4_table(std::integer_sequence<int, I...>) {
+  return std::array<RegBridge4Fn, sizeof...(I)>{&k_bridge_reg4<I + 1>...};
+}
 const auto kRegBridge = reg_bridge_table(std::make_integer_sequence<int, kRegSteps>{});
+const auto kRegBridge4 = reg_bridge4_table(std::make_integer_sequence<int, kRegSteps4>{});
 const auto kRegGen = reg_gen_table(std::make_integer_sequence<int, kRegSteps>{});
 
 // ---------------------------------------------------------------------------
@@ -521,6 +587,15 @@ extern "C" int brownian_bridge_gpu(int steps, const int* allocation_indices, con
   if (paths == 0) return KOO_OK;
   if (!allocation_indices || !in || !out)
     return set_error(KOO_ERR_INVALID, "brownian_bridge_gpu: null pointer");
+  if (steps <= kRegSteps4 && paths % 4 == 0 && aligned16(in) && aligned16(out)) {
+    const int paths4 = paths / 4;
+    hipLaunchKernelGGL(kRegBridge4[steps - 1],
+                       dim3((paths4 + kBridgeThreads - 1) / kBridgeThreads, factors),
+                       dim3(kBridgeThreads), 0, as_stream(stream), tables_for(steps), factors,
+                       paths4, allocation_indices, reinterpret_cast<const f32x4*>(in),
+                       reinterpret_cast<f32x4*>(out));
+    return check_launch("brownian_bridge_gpu");
+  }
   if (steps <= kRegSteps) {
     hipLaunchKernelGGL(kRegBridge[steps - 1], dim3((paths + kBridgeThreads - 1) / kBridgeThreads,
                                                    factors),

@@ -90,6 +90,37 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Four paths per lane (16-byte streaming loads and stores): one product
+// state machine per path, the same steps as k_cashflows.
+template <class Product>
+__global__ void __launch_bounds__(256)
+    k_cashflows_vec4(f32x4* __restrict__ f1, f32x4* __restrict__ f2, const float* __restrict__ aux,
+                     int paths4, int steps, const f32x4* __restrict__ r1,
+                     const f32x4* __restrict__ r2, const f32x4* __restrict__ r3) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths4) return;
+  Product obj[4] = {Product(aux, steps), Product(aux, steps), Product(aux, steps),
+                    Product(aux, steps)};
+  bool done[4] = {false, false, false, false};
+  const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  size_t o = p;
+  for (int i = 0; i < steps; ++i, o += paths4) {
+    const f32x4 x1 = load_stream(r1 + o);
+    const f32x4 x2 = r2 ? load_stream(r2 + o) : zero;
+    const f32x4 x3 = r3 ? load_stream(r3 + o) : zero;
+    f32x4 a = zero, b = zero;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float ac = 0.0f, bc = 0.0f;
+      if (!done[c]) done[c] = obj[c].flows(ac, bc, x1[c], x2[c], x3[c]);
+      a[c] = ac;
+      b[c] = bc;
+    }
+    store_stream(a, f1 + o);
+    store_stream(b, f2 + o);
+  }
+}
+
 // (r1)^(1 - theta) (r2)^theta, the log-linear interpolation of the discount
 // factor between two rate times.  theta is uniform over a launch's lanes:
 // the end points (0: payment on a rate time, the usual case) are exact, in
@@ -129,6 +160,39 @@ __global__ void __launch_bounds__(256)
     out[o] = fl;
   }
   summed[p] = total;
+}
+
+// The same, four paths per lane with 16-byte streaming loads and stores;
+// per path the same operations in the same order.
+__global__ void __launch_bounds__(256)
+    k_discounting_vec4(const int* __restrict__ first, const int* __restrict__ second,
+                       const float* __restrict__ theta, const f32x4* __restrict__ disc,
+                       const f32x4* __restrict__ flows, const f32x4* __restrict__ num, int paths4,
+                       int steps, f32x4* __restrict__ out, f32x4* __restrict__ summed) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= paths4) return;
+  const size_t P = static_cast<size_t>(paths4);
+  f32x4 total = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+  for (int s = 0; s < steps; ++s) {
+    const f32x4* dcol = disc + static_cast<size_t>(s) * (steps + 1) * P + p;
+    const f32x4 now = load_stream(dcol + static_cast<size_t>(s) * P);
+    const f32x4 d1 = load_stream(dcol + static_cast<size_t>(first[s]) * P);
+    const f32x4 d2 = load_stream(dcol + static_cast<size_t>(second[s]) * P);
+    const size_t o = static_cast<size_t>(s) * P + p;
+    const f32x4 g = load_stream(flows + o), b = load_stream(num + o);
+    const float th = theta[s];
+    f32x4 fl;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float inv_now = __builtin_amdgcn_rcpf(now[c]);
+      const float rdf = interp_df(d1[c] * inv_now, d2[c] * inv_now, th);
+      fl[c] = g[c] * (rdf * __builtin_amdgcn_rcpf(b[c]));
+    }
+    total += fl;
+    store_stream(fl, out + o);
+  }
+  store_stream(total, summed + p);
 }
 
 // Partial discounting to targetIndices via the numeraire
@@ -302,6 +366,16 @@ int launch_cashflows(const char* who, float* f1, float* f2, float* aux, int path
   if (paths < 0 || steps < 0) return set_error(KOO_ERR_INVALID, "%s: bad sizes", who);
   if (paths == 0 || steps == 0) return KOO_OK;
   if (!f1 || !f2 || !aux || !r1) return set_error(KOO_ERR_INVALID, "%s: null pointer", who);
+  if (paths % 4 == 0 && aligned16(f1) && aligned16(f2) && aligned16(r1) && aligned16(r2) &&
+      aligned16(r3)) {
+    const int paths4 = paths / 4;
+    hipLaunchKernelGGL(k_cashflows_vec4<Product>, dim3((paths4 + 255) / 256), dim3(256), 0,
+                       as_stream(stream), reinterpret_cast<f32x4*>(f1),
+                       reinterpret_cast<f32x4*>(f2), aux, paths4, steps,
+                       reinterpret_cast<const f32x4*>(r1), reinterpret_cast<const f32x4*>(r2),
+                       reinterpret_cast<const f32x4*>(r3));
+    return check_launch(who);
+  }
   hipLaunchKernelGGL(k_cashflows<Product>, dim3((paths + 255) / 256), dim3(256), 0,
                      as_stream(stream), f1, f2, aux, paths, steps, r1, r2, r3);
   return check_launch(who);
@@ -351,6 +425,19 @@ extern "C" int cashFlowDiscounting_gpu(int* firstIndex, int* secondIndex, float*
   if (!firstIndex || !secondIndex || !theta || !discountRatios || !genFlows || !numeraireValues ||
       !discountedFlows || !summedDiscountedFlows)
     return set_error(KOO_ERR_INVALID, "cashFlowDiscounting_gpu: null pointer");
+  if (paths % 4 == 0 && aligned16(discountRatios) && aligned16(genFlows) &&
+      aligned16(numeraireValues) && aligned16(discountedFlows) &&
+      aligned16(summedDiscountedFlows)) {
+    const int paths4 = paths / 4;
+    hipLaunchKernelGGL(k_discounting_vec4, dim3((paths4 + 255) / 256), dim3(256), 0,
+                       as_stream(stream), firstIndex, secondIndex, theta,
+                       reinterpret_cast<const f32x4*>(discountRatios),
+                       reinterpret_cast<const f32x4*>(genFlows),
+                       reinterpret_cast<const f32x4*>(numeraireValues), paths4, numberSteps,
+                       reinterpret_cast<f32x4*>(discountedFlows),
+                       reinterpret_cast<f32x4*>(summedDiscountedFlows));
+    return check_launch("cashFlowDiscounting_gpu");
+  }
   hipLaunchKernelGGL(k_discounting, dim3((paths + 255) / 256), dim3(256), 0, as_stream(stream),
                      firstIndex, secondIndex, theta, discountRatios, genFlows, numeraireValues,
                      paths, numberSteps, discountedFlows, summedDiscountedFlows);

@@ -20,8 +20,13 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // 16-byte vector of floats for streaming loads and stores: the nontemporal
 // builtins (global_load/store ... nt) take clang vector types, not float4.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ f32x4 load_stream(const f32x4* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void store_stream(f32x4 v, f32x4* p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ u32x4 load_stream(const u32x4* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void store_stream(u32x4 v, u32x4* p) { __builtin_nontemporal_store(v, p); }
+// 16-byte alignment of a device pointer (the condition of the vector paths)
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();

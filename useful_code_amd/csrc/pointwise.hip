@@ -56,8 +56,6 @@ __global__ void __launch_bounds__(kPwThreads)
   }
 }
 
-bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
-
 int grid_for(int n) {
   const int cap = 8 * device_cu_count();
   const int g = (n + kPwThreads - 1) / kPwThreads;

@@ -50,6 +50,62 @@ __device__ __forceinline__ void sobol_strided(const uint32_t* v, uint32_t n_vect
   }
 }
 
+// Four consecutive points per lane: Gray indices n .. n+3 (the Gray codes of
+// n and n+1 differ in bit ctz(n+1) only, so each point is the previous one
+// XOR one direction number), quads strided by a power of two with the skip
+// above at point stride 2^m = 4 x the quad stride.
+template <class Emit4>
+__device__ __forceinline__ void sobol_strided4(const uint32_t* v, uint32_t n_quads, uint32_t off1,
+                                               Emit4 emit) {
+  const uint32_t stride = gridDim.x * blockDim.x;  // quads, a power of two
+  const uint32_t m = floor_log2(stride) + 2u;
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_quads) return;
+  uint32_t n = 4u * q + off1;
+  uint32_t x = sobol_point(v, n);
+  const uint32_t v_stride = v[m - 1];
+  for (;;) {
+    const uint32_t x1 = x ^ v[__builtin_ctz(n + 1u)];
+    const uint32_t x2 = x1 ^ v[__builtin_ctz(n + 2u)];
+    const uint32_t x3 = x2 ^ v[__builtin_ctz(n + 3u)];
+    emit(q, x, x1, x2, x3);
+    q += stride;
+    if (q >= n_quads) break;
+    const uint32_t t = ~(n | ((1u << m) - 1u));
+    x ^= v_stride ^ v[t ? __builtin_ctz(t) : 31u];
+    n += 1u << m;
+  }
+}
+
+__global__ void __launch_bounds__(kSobolThreads) k_sobol_ints_vec4(uint32_t n_quads, uint32_t off1,
+                                                                   const uint32_t* __restrict__ dirs,
+                                                                   u32x4* __restrict__ out) {
+  __shared__ uint32_t v[32];
+  if (threadIdx.x < 32) v[threadIdx.x] = dirs[32u * blockIdx.y + threadIdx.x];
+  __syncthreads();
+  u32x4* o = out + static_cast<size_t>(blockIdx.y) * n_quads;
+  sobol_strided4(v, n_quads, off1, [&](uint32_t q, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    const u32x4 w = {a, b, c, d};
+    store_stream(w, o + q);
+  });
+}
+
+__global__ void __launch_bounds__(kSobolThreads)
+    k_sobol_normal_vec4(uint32_t n_quads, uint32_t off1, const uint32_t* __restrict__ dirs,
+                        const uint32_t* __restrict__ scramblers, int number_scrambled,
+                        f32x4* __restrict__ out) {
+  __shared__ uint32_t v[32];
+  if (threadIdx.x < 32) v[threadIdx.x] = dirs[32u * blockIdx.y + threadIdx.x];
+  __syncthreads();
+  const uint32_t s = (static_cast<int>(blockIdx.y) < number_scrambled) ? scramblers[blockIdx.y] : 0u;
+  f32x4* o = out + static_cast<size_t>(blockIdx.y) * n_quads;
+  sobol_strided4(v, n_quads, off1, [&](uint32_t q, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    const f32x4 w = {invnorm_u32(a ^ s), invnorm_u32(b ^ s), invnorm_u32(c ^ s),
+                     invnorm_u32(d ^ s)};
+    store_stream(w, o + q);
+  });
+}
+
 __global__ void __launch_bounds__(kSobolThreads) k_sobol_ints(uint32_t n_vectors, uint32_t off1,
                                                               const uint32_t* __restrict__ dirs,
                                                               uint32_t* __restrict__ out) {
@@ -75,18 +131,12 @@ __global__ void __launch_bounds__(kSobolThreads)
 // In-place XOR of the first number_dim rows; 4 words per lane when rows are
 // 16-byte aligned.  One row per blockIdx.y, so the scrambler is one scalar
 // load per block and no lane divides to find its row.
-__global__ void k_scramble_rows(uint4* __restrict__ data, const uint32_t* __restrict__ scr,
+__global__ void k_scramble_rows(u32x4* __restrict__ data, const uint32_t* __restrict__ scr,
                                 int paths4) {
   const uint32_t s = scr[blockIdx.y];
-  uint4* row = data + static_cast<size_t>(blockIdx.y) * paths4;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < paths4; i += gridDim.x * blockDim.x) {
-    uint4 w = row[i];
-    w.x ^= s;
-    w.y ^= s;
-    w.z ^= s;
-    w.w ^= s;
-    row[i] = w;
-  }
+  u32x4* row = data + static_cast<size_t>(blockIdx.y) * paths4;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < paths4; i += gridDim.x * blockDim.x)
+    store_stream(load_stream(row + i) ^ s, row + i);
 }
 
 __global__ void k_scramble(uint32_t* __restrict__ data, const uint32_t* __restrict__ scr,
@@ -96,11 +146,12 @@ __global__ void k_scramble(uint32_t* __restrict__ data, const uint32_t* __restri
     data[i] ^= scr[i / paths];
 }
 
-__global__ void k_invnorm_vec4(const uint4* __restrict__ in, float4* __restrict__ out, int64_t n4) {
+__global__ void k_invnorm_vec4(const u32x4* __restrict__ in, f32x4* __restrict__ out, int64_t n4) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n4;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const uint4 w = in[i];
-    out[i] = make_float4(invnorm_u32(w.x), invnorm_u32(w.y), invnorm_u32(w.z), invnorm_u32(w.w));
+    const u32x4 w = load_stream(in + i);
+    const f32x4 z = {invnorm_u32(w.x), invnorm_u32(w.y), invnorm_u32(w.z), invnorm_u32(w.w)};
+    store_stream(z, out + i);
   }
 }
 
@@ -115,6 +166,12 @@ unsigned next_pow2(unsigned x) {
   unsigned p = 1;
   while (p < x) p <<= 1;
   return p;
+}
+
+dim3 sobol_grid4(int n_quads, int n_dimensions) {
+  unsigned per_block = kSobolThreads * (kPointsPerThread / 4);
+  unsigned gx = next_pow2((static_cast<unsigned>(n_quads) + per_block - 1) / per_block);
+  return dim3(gx, static_cast<unsigned>(n_dimensions));
 }
 
 dim3 sobol_grid(int n_vectors, int n_dimensions) {
@@ -193,9 +250,15 @@ extern "C" int sobolGPUInts(int n_vectors, int n_dimensions, unsigned path_offse
     return set_error(KOO_ERR_INVALID, "sobolGPUInts: bad sizes (%d, %d)", n_vectors, n_dimensions);
   if (n_vectors == 0 || n_dimensions == 0) return KOO_OK;
   if (!d_directions || !d_output) return set_error(KOO_ERR_INVALID, "sobolGPUInts: null pointer");
-  hipLaunchKernelGGL(k_sobol_ints, sobol_grid(n_vectors, n_dimensions), dim3(kSobolThreads), 0,
-                     as_stream(stream), static_cast<uint32_t>(n_vectors), path_offset + 1u,
-                     d_directions, d_output);
+  if (n_vectors % 4 == 0 && aligned16(d_output))
+    hipLaunchKernelGGL(k_sobol_ints_vec4, sobol_grid4(n_vectors / 4, n_dimensions),
+                       dim3(kSobolThreads), 0, as_stream(stream),
+                       static_cast<uint32_t>(n_vectors / 4), path_offset + 1u, d_directions,
+                       reinterpret_cast<u32x4*>(d_output));
+  else
+    hipLaunchKernelGGL(k_sobol_ints, sobol_grid(n_vectors, n_dimensions), dim3(kSobolThreads), 0,
+                       as_stream(stream), static_cast<uint32_t>(n_vectors), path_offset + 1u,
+                       d_directions, d_output);
   return check_launch("sobolGPUInts");
 }
 
@@ -208,9 +271,15 @@ extern "C" int sobol_scramble_normal_gpu(int n_vectors, int n_dimensions, unsign
   if (n_vectors == 0 || n_dimensions == 0) return KOO_OK;
   if (!d_directions || !out || (number_scrambled > 0 && !scramblers))
     return set_error(KOO_ERR_INVALID, "sobol_scramble_normal_gpu: null pointer");
-  hipLaunchKernelGGL(k_sobol_normal, sobol_grid(n_vectors, n_dimensions), dim3(kSobolThreads), 0,
-                     as_stream(stream), static_cast<uint32_t>(n_vectors), path_offset + 1u,
-                     d_directions, scramblers, number_scrambled, out);
+  if (n_vectors % 4 == 0 && aligned16(out))
+    hipLaunchKernelGGL(k_sobol_normal_vec4, sobol_grid4(n_vectors / 4, n_dimensions),
+                       dim3(kSobolThreads), 0, as_stream(stream),
+                       static_cast<uint32_t>(n_vectors / 4), path_offset + 1u, d_directions,
+                       scramblers, number_scrambled, reinterpret_cast<f32x4*>(out));
+  else
+    hipLaunchKernelGGL(k_sobol_normal, sobol_grid(n_vectors, n_dimensions), dim3(kSobolThreads),
+                       0, as_stream(stream), static_cast<uint32_t>(n_vectors), path_offset + 1u,
+                       d_directions, scramblers, number_scrambled, out);
   return check_launch("sobol_scramble_normal_gpu");
 }
 
@@ -225,7 +294,7 @@ extern "C" int scramble_gpu(unsigned int* device_data, unsigned int* scramblers,
       number_dim <= 65535) {
     const int paths4 = paths / 4;
     hipLaunchKernelGGL(k_scramble_rows, dim3(grid_for(paths4, 256, 65535), number_dim), dim3(256),
-                       0, as_stream(stream), reinterpret_cast<uint4*>(device_data), scramblers,
+                       0, as_stream(stream), reinterpret_cast<u32x4*>(device_data), scramblers,
                        paths4);
   } else {
     hipLaunchKernelGGL(k_scramble, dim3(grid_for(n, 256, cap)), dim3(256), 0, as_stream(stream),
@@ -245,8 +314,8 @@ extern "C" int inverse_normal_u32_gpu(const unsigned int* in, float* out, long l
     const int64_t n4 = n / 4;
     if (n4 > 0)
       hipLaunchKernelGGL(k_invnorm_vec4, dim3(grid_for(n4, 256, cap)), dim3(256), 0,
-                         as_stream(stream), reinterpret_cast<const uint4*>(in),
-                         reinterpret_cast<float4*>(out), n4);
+                         as_stream(stream), reinterpret_cast<const u32x4*>(in),
+                         reinterpret_cast<f32x4*>(out), n4);
     done = n4 * 4;
   }
   if (done < n)
