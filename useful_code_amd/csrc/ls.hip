@@ -8,14 +8,17 @@
 // * k_gram: sum over paths of the outer product of up to 16 path-minor rows
 //   (X^T X and X^T y in one Gram matrix), on MFMA v_mfma_f32_16x16x4f32 with
 //   FP32 accumulation inside a block and FP64 block partials.
-// * k_ls_date: one backward-induction date of the LS first pass: applies the
-//   later date's exercise decision and numeraire deflation to the running
-//   continuation values and accumulates this date's normal equations
-//   (optionally over in-the-money paths only) on MFMA into FP64 block
-//   partials.  The next launch (the earlier date, or k_ls_final) starts by
-//   summing those partials in FP64 and solving them by FP64 Cholesky in every
-//   block (identical inputs and order, so identical coefficients): one launch
-//   per date, no atomics or grid-wide fences.
+// * LS first pass, in rounds (one per exercise date, last date first, then
+//   date 0's decision and the price): every block sums the previous round's
+//   FP64 block partials and solves that date's normal equations itself (same
+//   inputs and order, so identical coefficients), then applies the later
+//   date's exercise decision and numeraire deflation to its paths' running
+//   values and accumulates this date's normal equations (optionally over
+//   in-the-money paths only) on MFMA into its block partials.  One persistent
+//   launch (k_ls_first_pass; the partials exchanged through words armed with
+//   a sentinel, no counters) or one launch per round (k_ls_round_single).
+//   The portfolio's passes run the same per-block code (k_ls_portfolio,
+//   k_ls_round).
 // * k_ls_second: the frozen-strategy pass (price under given coefficients).
 // * the reference's small LS kernels (basis variables, exercise values, basis
 //   expansion, continuation values, update, deflation).
@@ -150,11 +153,21 @@ __global__ void __launch_bounds__(kGramThreads)
 }
 
 // ---------------------------------------------------------------------------
-// Fused LS first pass: one launch per exercise date, last date first.
+// LS first pass.  Round r = 0..E handles exercise date e = E - 1 - r (round
+// E: date 0's exercise decision and the price).  Every block of a round first
+// sums the previous round's block partials and solves that date's normal
+// equations itself (the same partials in the same order, so the same
+// coefficients in every block; block 0 stores them), then does its paths'
+// work for date e -- the later date's exercise decision and deflation of the
+// running values, this date's normal equations on MFMA -- into its own column
+// of the round's partials.  Partials: the upper triangle of the Gram, FP64,
+// in three slots used in turn (round r writes slot r % 3).  Either one
+// launch per round or one persistent launch; the per-block arithmetic is the
+// same, so the results are too.
 // ---------------------------------------------------------------------------
 struct DateArgs {
   int e, E, P, max_vars, maxd;
-  int v, k;    // variables and basis functions at date e
+  int v, k;    // variables and basis functions at date e (0 when e < 0)
   int vn, kn;  // at date e + 1 (when e + 1 < E)
   float sh[kMaxBasisVars], sc[kMaxBasisVars];    // x~ = (x - sh) * sc at date e
   float shn[kMaxBasisVars], scn[kMaxBasisVars];  // at date e + 1
@@ -176,253 +189,192 @@ __device__ __forceinline__ float continuation(const float* __restrict__ bv, size
   return c;
 }
 
-// The path work of one block (of nblocks) of one exercise date, given the
-// later date's coefficients in beta_n (LDS; unused at the last date): the
-// later date's exercise decision and deflation of the running values, then
-// this date's normal equations on MFMA into the block's partials.  Agent: the
-// partials are written with agent-coherent stores (the persistent kernels,
-// whose blocks exchange them without a kernel boundary).
-template <int GROUPS, bool Agent>
+// The same from variables already loaded (raw, before normalisation) and
+// coefficients already in registers (bf, FP32); KN >= k.
+template <int KN>
+__device__ __forceinline__ float continuation_of(const float (&xr)[kMaxBasisVars], int v, int k,
+                                                 bool cross, const float* sh, const float* sc,
+                                                 const float (&bf)[KN]) {
+  float x[kMaxBasisVars] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < kMaxBasisVars; ++i)
+    if (i < v) x[i] = (xr[i] - sh[i]) * sc[i];
+  float c = 0.0f;
+#pragma unroll
+  for (int j = 0; j < KN; ++j)
+    if (j < k) c += basis_value(j, v, cross, x[0], x[1], x[2], x[3]) * bf[j];
+  return c;
+}
+
+// One path's inputs to a round, loaded ahead of their use.
+struct PathIn {
+  float y;                      // running value (when date e + 1 exists)
+  float xn[kMaxBasisVars];      // date e + 1: raw variables,
+  float evn, numn, cashn;       //   exercise value, numeraire, cash flow
+  float x[kMaxBasisVars];       // date e (when e >= 0)
+  float eve, nume, cashe;
+};
+
+// Issues the loads of path p's inputs to the round of `a` (p < P); the
+// running value from Y unless Y is null (kept in a register by the caller).
+__device__ __forceinline__ PathIn load_path(const DateArgs& a, const float* __restrict__ bv,
+                                            const float* __restrict__ cash,
+                                            const float* __restrict__ ev,
+                                            const float* __restrict__ num,
+                                            const float* __restrict__ Y, int p) {
+  PathIn in{};
+  const size_t P = static_cast<size_t>(a.P);
+  if (a.e + 1 < a.E) {
+    const size_t on = static_cast<size_t>(a.e + 1) * P + p;
+    const size_t rn = static_cast<size_t>(a.e + 1) * a.max_vars;
+    if (Y) in.y = Y[p];
+#pragma unroll
+    for (int i = 0; i < kMaxBasisVars; ++i)
+      if (i < a.vn) in.xn[i] = bv[(rn + i) * P + p];
+    in.evn = ev[on];
+    in.numn = num[on];
+    if (cash) in.cashn = cash[on];
+  }
+  if (a.e >= 0) {
+    const size_t oe = static_cast<size_t>(a.e) * P + p;
+    const size_t re = static_cast<size_t>(a.e) * a.max_vars;
+#pragma unroll
+    for (int i = 0; i < kMaxBasisVars; ++i)
+      if (i < a.v) in.x[i] = bv[(re + i) * P + p];
+    in.eve = ev[oe];
+    in.nume = num[oe];
+    if (cash) in.cashe = cash[oe];
+  }
+  return in;
+}
+
+// The later date's exercise decision on a path under its coefficients beta_n
+// (updateDeflatedNextStepValuesMulti_gold, gold/MultiLS_regression.h:46-115):
+// the exercise value if the strategy exercises, else the running value plus
+// the later date's cash flow.
+template <int KN>
+__device__ __forceinline__ float later_value(const DateArgs& a, const PathIn& in, bool cash,
+                                             const float (&bf)[KN]) {
+  const float cont = continuation_of<KN>(in.xn, a.vn, a.kn, a.cross != 0, a.shn, a.scn, bf);
+  const bool allowed = !a.itm_only || in.evn > 0.0f;
+  if (allowed && in.evn > cont) return in.evn;
+  return cash ? in.y + in.cashn : in.y;
+}
+
+// The later date's coefficients (LDS, FP64) as FP32 registers, KN >= kn.
+template <int KN>
+__device__ __forceinline__ void coeffs_f32(const double* beta, int kn, float (&bf)[KN]) {
+#pragma unroll
+  for (int j = 0; j < KN; ++j) bf[j] = j < kn ? static_cast<float>(beta[j]) : 0.0f;
+}
+
+// The path work of one block (of nblocks) for date e >= 0 given the later
+// date's coefficients in beta_n (LDS; unused at the last date): the later
+// date's decision and deflation of the running values (to date e,
+// updateDeflationOfContinuationValues_gold, gold/LS_regression.cpp:98-115),
+// then this date's normal equations on MFMA into the block's column of out.
+// The block's paths are walked 64 per wavefront, the next chunk's loads
+// issued before the current chunk is used; `first` is the first chunk,
+// loaded by the caller.  yin: the running value of the lane's only chunk
+// is kept in the register yr instead of Y.  Each wavefront stages its 64
+// paths' weighted variables and target in its own 2 KB of LDS (basis_factors'
+// vector), from which the lane of MFMA row j reads the two factors of
+// function j for the path it supplies.  Agent: agent-coherent
+// partial stores (the persistent kernels).  KN >= the later date's basis size.
+template <int GROUPS, bool Agent, int KN>
 __device__ __forceinline__ void ls_date_work(const DateArgs& a, const float* __restrict__ bv,
                                              const float* __restrict__ cash,
                                              const float* __restrict__ ev,
                                              const float* __restrict__ num,
                                              const double* beta_n, float* __restrict__ Y,
-                                             double* __restrict__ part, int block, int nblocks) {
+                                             bool yin, float& yr, double* __restrict__ out,
+                                             int block, int nblocks, const PathIn& first) {
   __shared__ double lds_d[(kGramThreads / 64) * 256];
   constexpr int kPathsPerOp = 4 * GROUPS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const size_t P = static_cast<size_t>(a.P);
   const bool cross = a.cross != 0, itm = a.itm_only != 0;
   const bool later = a.e + 1 < a.E;
   const int n = a.k + 1;  // basis functions + target
   const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
   const int sub = GROUPS == 2 ? ((lane >> 3) & 1) * 4 + (lane >> 4) : (lane >> 4);
+  float bf[KN];
+  coeffs_f32<KN>(beta_n, later ? a.kn : 0, bf);
+  // per path: basis_factors' vector (8 floats, 32 bytes)
+  float* stage = reinterpret_cast<float*>(lds_d + wave * 256);
+  int ia, ib;
+  basis_factors(j, a.v, a.k, cross, ia, ib);
   // each MFMA's 4-path sums (FP32) accumulated in FP64
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int stride = nblocks * nw * 64;
+  PathIn cur = first;
+  if (yin) cur.y = yr;
   for (int base = (block * nw + wave) * 64; base < a.P; base += stride) {
+    const int nbase = base + stride;
+    PathIn nxt{};
+    if (nbase < a.P)  // uniform per wavefront
+      nxt = load_path(a, bv, cash, ev, num, Y, min(nbase + lane, a.P - 1));
     const int p = base + lane;
     float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f, x3 = 0.0f, t = 0.0f, w = 0.0f;
     if (p < a.P) {
       float y = 0.0f;
-      if (later) {
-        // exercise decision of date e+1 and deflation to date e
-        // (updateDeflatedNextStepValuesMulti_gold, gold/MultiLS_regression.h:46-115;
-        // updateDeflationOfContinuationValues_gold, gold/LS_regression.cpp:98-115)
-        const size_t on = static_cast<size_t>(a.e + 1) * P + p;
-        y = Y[p];
-        const float cont = continuation(bv, static_cast<size_t>(a.e + 1) * a.max_vars, P, p, a.vn,
-                                        a.kn, cross, a.shn, a.scn, beta_n);
-        const float exv = ev[on];
-        const bool allowed = !itm || exv > 0.0f;
-        if (allowed && exv > cont)
-          y = exv;
-        else if (cash)
-          y += cash[on];
-        y *= num[static_cast<size_t>(a.e) * P + p] / num[on];
-      }
-      Y[p] = y;
-      const size_t oe = static_cast<size_t>(a.e) * P + p;
-      w = (!itm || ev[oe] > 0.0f) ? 1.0f : 0.0f;
-      t = (cash ? cash[oe] : 0.0f) + y;
-      const size_t r0 = static_cast<size_t>(a.e) * a.max_vars;
-      if (a.v > 0) x0 = (bv[r0 * P + p] - a.sh[0]) * a.sc[0];
-      if (a.v > 1) x1 = (bv[(r0 + 1) * P + p] - a.sh[1]) * a.sc[1];
-      if (a.v > 2) x2 = (bv[(r0 + 2) * P + p] - a.sh[2]) * a.sc[2];
-      if (a.v > 3) x3 = (bv[(r0 + 3) * P + p] - a.sh[3]) * a.sc[3];
+      if (later) y = later_value<KN>(a, cur, cash != nullptr, bf) * (cur.nume / cur.numn);
+      if (yin)
+        yr = y;
+      else
+        Y[p] = y;
+      w = (!itm || cur.eve > 0.0f) ? 1.0f : 0.0f;
+      t = cur.cashe + y;
+      if (a.v > 0) x0 = (cur.x[0] - a.sh[0]) * a.sc[0];
+      if (a.v > 1) x1 = (cur.x[1] - a.sh[1]) * a.sc[1];
+      if (a.v > 2) x2 = (cur.x[2] - a.sh[2]) * a.sc[2];
+      if (a.v > 3) x3 = (cur.x[3] - a.sh[3]) * a.sc[3];
     }
+    reinterpret_cast<floatx4*>(stage)[2 * lane] = floatx4{w, w * x0, w * x1, w * x2};
+    reinterpret_cast<floatx4*>(stage)[2 * lane + 1] = floatx4{w * x3, w * t, 0.0f, 1.0f};
     // 64 paths of this wave -> 64 / (4 GROUPS) MFMAs; lane l takes row j of path `sub`.
-#pragma unroll 2
+#pragma unroll 4
     for (int q = 0; q < 64 / kPathsPerOp; ++q) {
-      const int srcl = q * kPathsPerOp + sub;
-      const float s0 = __shfl(x0, srcl, 64), s1 = __shfl(x1, srcl, 64);
-      float s2 = 0.0f, s3 = 0.0f;
-      if (a.v > 2) {  // uniform
-        s2 = __shfl(x2, srcl, 64);
-        s3 = __shfl(x3, srcl, 64);
-      }
-      const float st = __shfl(t, srcl, 64), sw = __shfl(w, srcl, 64);
-      float v = j < a.k ? basis_value(j, a.v, cross, s0, s1, s2, s3) : (j == a.k ? st : 0.0f);
-      v *= sw;
+      const float* sp = stage + 8 * (q * kPathsPerOp + sub);
+      const float v = sp[ia] * sp[ib];
       const floatx4 d = mfma16x16x4(v, v, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] += static_cast<double>(d[i]);
     }
+    cur = nxt;
   }
-  block_gram_d<GROUPS, Agent>(acc, n, lds_d, part, block, nblocks);
+  block_gram_tri<GROUPS, Agent>(acc, n, lds_d, out, block, nblocks);
 }
 
-// One block (of nblocks) of one exercise date: the later date's coefficients
-// from its partials (every block solves them itself: same partials, same
-// order, so the same coefficients; block 0 stores them), then the path work.
-template <int GROUPS>
-__device__ __forceinline__ void ls_date_block(const DateArgs& a, const float* __restrict__ bv,
-                                              const float* __restrict__ cash,
-                                              const float* __restrict__ ev,
-                                              const float* __restrict__ num,
-                                              double* __restrict__ coeffs, float* __restrict__ Y,
-                                              const double* __restrict__ part_later,
-                                              double* __restrict__ part, int block, int nblocks) {
-  __shared__ double gram[kMaxGram * kMaxGram];
-  __shared__ double chol[kSolveScratch];
-  __shared__ double beta_n[kMaxGram];
-  if (a.e + 1 < a.E)  // coefficients of date e+1 from its partials
-    solve_date(part_later, nblocks, a.kn, gram, chol, beta_n,
-               coeffs + static_cast<size_t>(a.e + 1) * a.maxd, a.maxd, block);
-  ls_date_work<GROUPS, false>(a, bv, cash, ev, num, beta_n, Y, part, block, nblocks);
-}
-
-template <int GROUPS>
-__global__ void __launch_bounds__(kGramThreads)
-    k_ls_date(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
-              const float* __restrict__ ev, const float* __restrict__ num,
-              double* __restrict__ coeffs, float* __restrict__ Y,
-              const double* __restrict__ part_later, double* __restrict__ part) {
-  ls_date_block<GROUPS>(a, bv, cash, ev, num, coeffs, Y, part_later, part, blockIdx.x, gridDim.x);
-}
-
-// Date 0's exercise decision under its coefficients beta0 (LDS), then
-// sum_p Y_p / B_0 into the block's price partial (one block of nblocks).
-// Agent as for ls_date_work.
-template <bool Agent>
+// Round E of one block: date 0's decision under its coefficients beta0 (LDS),
+// then sum_p Y_p / B_0 into the block's price partial out[block].  `a` is the
+// round's arguments (e = -1, date 0 as the later date); yin, yr as for
+// ls_date_work.  Agent as for ls_date_work.
+template <bool Agent, int KN>
 __device__ __forceinline__ void ls_final_work(const DateArgs& a, const float* __restrict__ bv,
                                               const float* __restrict__ cash,
                                               const float* __restrict__ ev,
                                               const float* __restrict__ num, const double* beta0,
-                                              const float* __restrict__ Y,
-                                              double* __restrict__ part, int block, int nblocks) {
+                                              const float* __restrict__ Y, bool yin, float yr,
+                                              double* __restrict__ out, int block, int nblocks,
+                                              const PathIn& first) {
   __shared__ double red[kGramThreads / 64];
-  const size_t P = static_cast<size_t>(a.P);
+  float bf[KN];
+  coeffs_f32<KN>(beta0, a.kn, bf);
   double s = 0.0;
-  for (int p = block * blockDim.x + threadIdx.x; p < a.P; p += nblocks * blockDim.x) {
-    float y = Y[p];
-    const float cont = continuation(bv, 0, P, p, a.v, a.k, a.cross != 0, a.sh, a.sc, beta0);
-    const float exv = ev[p];
-    const bool allowed = !a.itm_only || exv > 0.0f;
-    if (allowed && exv > cont)
-      y = exv;
-    else if (cash)
-      y += cash[p];
-    s += static_cast<double>(y) / static_cast<double>(num[p]);
+  const int stride = nblocks * blockDim.x;
+  PathIn cur = first;
+  if (yin) cur.y = yr;
+  for (int base = block * blockDim.x + (threadIdx.x & ~63); base < a.P; base += stride) {
+    const int nbase = base + stride;
+    PathIn nxt{};
+    if (nbase < a.P)
+      nxt = load_path(a, bv, cash, ev, num, Y, min(nbase + (threadIdx.x & 63), a.P - 1));
+    if (base + static_cast<int>(threadIdx.x & 63) < a.P)
+      s += static_cast<double>(later_value<KN>(a, cur, cash != nullptr, bf)) /
+           static_cast<double>(cur.numn);
+    cur = nxt;
   }
-  block_sum_to<Agent>(s, red, part + block);
-}
-
-// Date 0's coefficients from its partials (nb_date0 blocks; every block
-// solves, block 0 stores them), then the final decision and price partial.
-__device__ __forceinline__ void ls_final_block(const DateArgs& a, const float* __restrict__ bv,
-                                               const float* __restrict__ cash,
-                                               const float* __restrict__ ev,
-                                               const float* __restrict__ num,
-                                               double* __restrict__ coeffs,
-                                               const float* __restrict__ Y,
-                                               const double* __restrict__ part_date0,
-                                               int nb_date0, double* __restrict__ part, int block,
-                                               int nblocks) {
-  __shared__ double gram[kMaxGram * kMaxGram];
-  __shared__ double chol[kSolveScratch];
-  __shared__ double beta0[kMaxGram];
-  solve_date(part_date0, nb_date0, a.k, gram, chol, beta0, coeffs, a.maxd, block);
-  ls_final_work<false>(a, bv, cash, ev, num, beta0, Y, part, block, nblocks);
-}
-
-// ---------------------------------------------------------------------------
-// Persistent first passes.  One block solves each date's normal equations
-// (from the partials of all the date's blocks) and publishes the
-// coefficients; the blocks of the next date wait for them, do their path
-// work and count themselves done.  All exchanges between blocks go through
-// agent-coherent stores and loads (the partials, the coefficients, two
-// counter words), each wave waiting for its stores to be performed before
-// its block signals, so no cache write-back or invalidation is needed.  The
-// running values stay with the block that owns their paths.  Every wait polls
-// a bounded number of times (kBarrierPolls); a wait that gives up makes the
-// kernel stop with its error word set and NaN prices.
-// ---------------------------------------------------------------------------
-
-// Thread 0 waits until *word >= target; the outcome is returned on every lane.
-__device__ __forceinline__ bool wait_at_least(const unsigned* word, unsigned target, int* flag_lds) {
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    unsigned polls = 0;
-    while (__hip_atomic_load(const_cast<unsigned*>(word), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(4);
-      if (++polls > kBarrierPolls) {
-        ok = 0;
-        break;
-      }
-    }
-    *flag_lds = ok;
-  }
-  __syncthreads();
-  const bool ok = *flag_lds != 0;
-  __syncthreads();
-  return ok;
-}
-
-// Whether *word >= target now (thread 0 reads, every lane gets the answer).
-__device__ __forceinline__ bool reached(const unsigned* word, unsigned target, int* flag_lds) {
-  if (threadIdx.x == 0)
-    *flag_lds = __hip_atomic_load(const_cast<unsigned*>(word), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT) >= target;
-  __syncthreads();
-  const bool ok = *flag_lds != 0;
-  __syncthreads();
-  return ok;
-}
-
-// After the block's agent-coherent stores: each wave waits for its own stores
-// to be performed (s_waitcnt vmcnt(0)), then the block barrier, then one lane
-// adds v to *word.
-__device__ __forceinline__ void signal_add(unsigned* word, unsigned v) {
-  __builtin_amdgcn_s_waitcnt(0x0F70);
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(word, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// signal_add(word, 1) that also tells the block (every lane) whether its add
-// was the one that brought *word to target, i.e. it arrived last.
-__device__ __forceinline__ bool arrive_last(unsigned* word, unsigned target, int* flag_lds) {
-  __builtin_amdgcn_s_waitcnt(0x0F70);
-  __syncthreads();
-  if (threadIdx.x == 0)
-    *flag_lds = __hip_atomic_fetch_add(word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
-                target;
-  __syncthreads();
-  const bool last = *flag_lds != 0;
-  __syncthreads();
-  return last;
-}
-
-// One block: the normal equations of a date from its nb partials (agent
-// loads), solved, the coefficients row (maxd entries, zero past k) stored
-// with agent stores.
-__device__ __forceinline__ void ls_solve_publish(const double* part, int nb, int k, int maxd,
-                                                 double* coeffs_row) {
-  __shared__ double gram[kMaxGram * kMaxGram];
-  __shared__ double scratch[kSolveScratch];
-  __shared__ double beta[kMaxGram];
-  reduce_partials<true>(part, nb, (k + 1) * (k + 1), gram);
-  if (threadIdx.x == 0) {
-    solve_normal_equations(gram, k, beta, scratch);
-    for (int i = 0; i < maxd; ++i) store_agent(coeffs_row + i, i < k ? beta[i] : 0.0);
-  }
-}
-
-// The published coefficients row (agent loads) into LDS.
-__device__ __forceinline__ void ls_load_coeffs(const double* coeffs_row, int k, double* beta_lds) {
-  if (threadIdx.x < k) beta_lds[threadIdx.x] = load_agent(coeffs_row + threadIdx.x);
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(kGramThreads)
-    k_ls_final(const DateArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
-               const float* __restrict__ ev, const float* __restrict__ num,
-               double* __restrict__ coeffs, const float* __restrict__ Y,
-               const double* __restrict__ part_date0, int nb_date0, double* __restrict__ part) {
-  ls_final_block(a, bv, cash, ev, num, coeffs, Y, part_date0, nb_date0, part, blockIdx.x,
-                 gridDim.x);
+  block_sum_to<Agent>(s, red, out + block);
 }
 
 // Frozen-strategy pass (SecondPassPriceLSAMultiUsingAggregatedFlows_gold,
@@ -436,9 +388,10 @@ struct AllDates {
   float sh[kMaxRates][kMaxBasisVars], sc[kMaxRates][kMaxBasisVars];
 };
 
-// The date arguments of date e from the all-dates table.
-__device__ __forceinline__ DateArgs all_dates_date(const AllDates& d, int e) {
+// The arguments of round r (date e = E - 1 - r; e = -1 for round E).
+__device__ __forceinline__ DateArgs round_args(const AllDates& d, int r) {
   DateArgs a{};
+  const int e = d.E - 1 - r;
   a.e = e;
   a.E = d.E;
   a.P = d.P;
@@ -446,11 +399,13 @@ __device__ __forceinline__ DateArgs all_dates_date(const AllDates& d, int e) {
   a.maxd = d.maxd;
   a.cross = d.cross;
   a.itm_only = d.itm_only;
-  a.v = d.v[e];
-  a.k = d.k[e];
-  for (int i = 0; i < kMaxBasisVars; ++i) {
-    a.sh[i] = d.sh[e][i];
-    a.sc[i] = d.sc[e][i];
+  if (e >= 0) {
+    a.v = d.v[e];
+    a.k = d.k[e];
+    for (int i = 0; i < kMaxBasisVars; ++i) {
+      a.sh[i] = d.sh[e][i];
+      a.sc[i] = d.sc[e][i];
+    }
   }
   if (e + 1 < d.E) {
     a.vn = d.v[e + 1];
@@ -463,45 +418,102 @@ __device__ __forceinline__ DateArgs all_dates_date(const AllDates& d, int e) {
   return a;
 }
 
-// The whole LS first pass in one persistent launch.  Round r handles date
-// e = E - 1 - r (r = E: date 0's decision and the price): every block waits
-// for the later date's coefficients (sync[1] counts those published), does
-// its path work for date e into its partials and counts itself done
-// (sync[0]); the block that arrives last solves date e from all the partials
-// and publishes its coefficients -- or, after the last round, sums the price.
-// Each date's work and solve are those of the per-date launches (same
-// blocks, same order), so the results are identical.  Launched with at most
-// as many blocks as can be resident.  If a wait gives up, the price is NaN
-// and sync[2] = 1.
+// Partial slot of round r in a buffer of three, each kTriMax entries x nb.
+__host__ __device__ inline size_t ls_slot(int r, int nb) {
+  return static_cast<size_t>(r % 3) * kTriMax * nb;
+}
+
+// Entries of round r's partials: the Gram triangle of its date, or the price.
+__device__ __forceinline__ int round_entries(const AllDates& d, int r) {
+  const int e = d.E - 1 - r;
+  return e >= 0 ? tri_count(d.k[e] + 1) : 1;
+}
+
+// One block's round r of the first pass (launch per round): the later
+// date's coefficients from the previous round's partials (every block solves;
+// block 0 stores them), then the path work.
+template <int GROUPS>
+__global__ void __launch_bounds__(kGramThreads)
+    k_ls_round_single(const AllDates dv, int r, const float* __restrict__ bv,
+                      const float* __restrict__ cash, const float* __restrict__ ev,
+                      const float* __restrict__ num, double* __restrict__ coeffs,
+                      float* __restrict__ Y, double* __restrict__ xbuf) {
+  __shared__ double tri[kTriMax];
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double scratch[kSolveScratch];
+  __shared__ double beta[kMaxGram];
+  const AllDates& d = dv;
+  const int nb = gridDim.x;
+  const DateArgs a = round_args(d, r);
+  float unused = 0.0f;
+  const PathIn first =
+      load_path(a, bv, cash, ev, num, Y, min(static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), d.P - 1));
+  if (r > 0) {
+    sum_tri<false>(xbuf + ls_slot(r - 1, nb), nb, round_entries(d, r - 1), tri, nullptr);
+    solve_tri(tri, a.kn, gram, scratch, beta);
+    if (blockIdx.x == 0 && threadIdx.x < d.maxd)
+      coeffs[static_cast<size_t>(a.e + 1) * d.maxd + threadIdx.x] =
+          threadIdx.x < a.kn ? beta[threadIdx.x] : 0.0;
+  }
+  if (a.e >= 0)
+    ls_date_work<GROUPS, false, kMaxGram>(a, bv, cash, ev, num, beta, Y, false, unused,
+                                          xbuf + ls_slot(r, nb), blockIdx.x, nb, first);
+  else
+    ls_final_work<false, kMaxGram>(a, bv, cash, ev, num, beta, Y, false, 0.0f,
+                                   xbuf + ls_slot(r, nb), blockIdx.x, nb, first);
+}
+
+// The whole first pass in one persistent launch (at most as many blocks as
+// can be resident, every basis of at most 7 functions).  The round's
+// partials are exchanged through agent-coherent words armed with kUnsetBits
+// (k_fill_u64 before the launch): a block stores its column, then reads every
+// block's column of the round until none is unset.  A block re-arms its own
+// words of round r - 2 once it has read round r - 1 (every block stored
+// round r - 1 after reading round r - 2), and waits for those stores before
+// storing round r, so a reader of round r + 1 never sees round r - 2's value.
+// The next round's first chunk of path data is loaded before the wait.  If a
+// wait gives up: NaN price and sync[0] = 1.
 __global__ void __launch_bounds__(kGramThreads)
     k_ls_first_pass(const AllDates dv, const float* __restrict__ bv,
                     const float* __restrict__ cash, const float* __restrict__ ev,
                     const float* __restrict__ num, double* __restrict__ coeffs,
-                    float* __restrict__ Y, double* __restrict__ part0,
-                    double* __restrict__ part1, unsigned* __restrict__ sync,
-                    double* __restrict__ price_out) {
+                    float* __restrict__ Y, double* __restrict__ xbuf,
+                    unsigned* __restrict__ sync, double* __restrict__ price_out) {
   __shared__ int flag;
+  __shared__ double tri[kTriMax];
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double scratch[kSolveScratch];
   __shared__ double beta[kMaxGram];
   const AllDates& d = dv;
   const int nb = gridDim.x;
+  const int p0 = min(static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), d.P - 1);
+  // one chunk of paths per lane: the running value stays in a register
+  float yr = 0.0f;
+  const bool yin = static_cast<long long>(nb) * blockDim.x >= d.P;
+  const float* Yl = yin ? nullptr : Y;
+  PathIn first = load_path(round_args(d, 0), bv, cash, ev, num, Yl, p0);
   for (int r = 0; r <= d.E; ++r) {
-    const int e = d.E - 1 - r;  // -1: the final decision
+    const DateArgs a = round_args(d, r);
     if (r > 0) {
-      if (!wait_at_least(sync + 1, static_cast<unsigned>(r), &flag)) goto fail;
-      ls_load_coeffs(coeffs + static_cast<size_t>(e + 1) * d.maxd, d.k[e + 1], beta);
+      if (!sum_tri<true>(xbuf + ls_slot(r - 1, nb), nb, round_entries(d, r - 1), tri, &flag))
+        goto fail;
+      if (r >= 2) rearm_tri(xbuf + ls_slot(r - 2, nb), nb, round_entries(d, r - 2), blockIdx.x);
+      solve_tri(tri, a.kn, gram, scratch, beta);
+      // the re-arming stores performed before any wave stores this round's
+      // partials (after the barriers in the work)
+      if (r >= 2) __builtin_amdgcn_s_waitcnt(0x0F70);
+      if (blockIdx.x == 0 && threadIdx.x < d.maxd)
+        coeffs[static_cast<size_t>(a.e + 1) * d.maxd + threadIdx.x] =
+            threadIdx.x < a.kn ? beta[threadIdx.x] : 0.0;
     }
-    double* out = (e & 1) ? part1 : part0;  // e = -1: the price partials go to part1
-    if (e >= 0) {
-      // every date has k + 1 <= 8 rows (the host launches this kernel only then)
-      ls_date_work<2, true>(all_dates_date(d, e), bv, cash, ev, num, beta, Y, out, blockIdx.x, nb);
+    double* out = xbuf + ls_slot(r, nb);
+    if (a.e >= 0) {
+      ls_date_work<2, true, 8>(a, bv, cash, ev, num, beta, Y, yin, yr, out, blockIdx.x, nb, first);
+      first = load_path(round_args(d, r + 1), bv, cash, ev, num, Yl, p0);
     } else {
-      ls_final_work<true>(all_dates_date(d, 0), bv, cash, ev, num, beta, Y, out, blockIdx.x, nb);
-    }
-    if (arrive_last(sync, static_cast<unsigned>(r + 1) * nb, &flag)) {
-      if (e >= 0) {
-        ls_solve_publish(out, nb, d.k[e], d.maxd, coeffs + static_cast<size_t>(e) * d.maxd);
-        signal_add(sync + 1, 1u);
-      } else {
+      ls_final_work<true, 8>(a, bv, cash, ev, num, beta, Y, yin, yr, out, blockIdx.x, nb, first);
+      if (blockIdx.x == 0) {
+        if (!sum_tri<true>(out, nb, 1, tri, &flag)) goto fail;
         finalize_col<true>(out, nb, 1, 0, 1.0 / static_cast<double>(d.P), price_out);
       }
     }
@@ -509,7 +521,7 @@ __global__ void __launch_bounds__(kGramThreads)
   return;
 fail:
   if (threadIdx.x == 0) {
-    sync[2] = 1u;
+    sync[0] = 1u;
     *price_out = __builtin_nan("");
   }
 }
@@ -674,13 +686,30 @@ int gram_blocks(int paths) {
 
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-// Workspace: running continuation values (P floats), then two partial
-// buffers used alternately by consecutive dates (date e reads date e+1's
-// partials while writing its own), each kMaxGram^2 doubles per block.
-// Then the persistent first pass's three 32-bit sync words (blocks done,
-// solves published, error).
+// Blocks of the first pass: gram_blocks(paths), capped at what can be
+// resident at once when the persistent kernel can run at all (so both
+// launch forms use the same blocks); with fewer than one resident block per
+// CU the per-round launches use gram_blocks(paths).
+int ls_pass_blocks(int paths, bool* persistent_ok) {
+  int per_cu = 0;
+  const int want = gram_blocks(paths);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ls_first_pass, kGramThreads, 0) !=
+          hipSuccess ||
+      per_cu < 1) {
+    *persistent_ok = false;
+    return want;
+  }
+  *persistent_ok = true;
+  const int cap = per_cu * device_cu_count();
+  return want < cap ? want : cap;
+}
+
+// Workspace: running continuation values (P floats); block partials of the
+// reference-named Gram kernels and the second pass (kMaxGram^2 doubles per
+// block); the first pass's three partial slots (kTriMax doubles per block
+// each); the persistent first pass's error word.
 struct LsLayout {
-  size_t y, part, part2, bar, total;
+  size_t y, part, xbuf, bar, total;
 };
 
 LsLayout ls_layout(int paths) {
@@ -688,25 +717,10 @@ LsLayout ls_layout(int paths) {
   l.y = 0;
   l.part = align256(sizeof(float) * static_cast<size_t>(paths));
   const int nb = ls_blocks(paths) > gram_blocks(paths) ? ls_blocks(paths) : gram_blocks(paths);
-  const size_t half = align256(sizeof(double) * kMaxGram * kMaxGram * nb);
-  l.part2 = l.part + half;
-  l.bar = l.part2 + half;
+  l.xbuf = l.part + align256(sizeof(double) * kMaxGram * kMaxGram * nb);
+  l.bar = l.xbuf + align256(sizeof(double) * 3 * kTriMax * gram_blocks(paths));
   l.total = l.bar + 256;
   return l;
-}
-
-// Blocks of the persistent first pass: gram_blocks(paths), provided that
-// many 1024-lane blocks can be resident at once; 0 when the occupancy query
-// fails or allows none (the caller then launches one kernel per date).
-int ls_persistent_blocks(int paths) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ls_first_pass, kGramThreads, 0) !=
-          hipSuccess ||
-      per_cu < 1)
-    return 0;
-  const int want = gram_blocks(paths);
-  const int cap = per_cu * device_cu_count();
-  return want < cap ? want : cap;
 }
 
 int check_ls(const char* who, int n_ex, const int* ves, int max_vars, int paths, int use_cross) {
@@ -762,7 +776,7 @@ constexpr int kLsBatchMaxDates = 40;
 struct LsItem {
   g_cfloat *z, *f0, *taus, *disp, *A;
   g_float *bv, *ev, *num, *Y;
-  g_double *part0, *part1, *coeffs, *price_out;
+  g_double *xbuf, *coeffs, *price_out;  // xbuf: the LS partial slots
   float strike;
   int P, N, F, E, itm_only;
   int paths_block0, paths_blocks, ls_block0, ls_blocks;
@@ -787,22 +801,25 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// The date arguments of date e of a product (two variables per date, the
-// cross basis: 6 functions).
-__device__ __forceinline__ DateArgs item_date(const LsItem& it, int e) {
+// The arguments of a product's round r (two variables per date, the cross
+// basis: 6 functions; date e = E - 1 - r, e = -1 for round E).
+__device__ __forceinline__ DateArgs item_round_args(const LsItem& it, int r) {
   DateArgs a{};
+  const int e = it.E - 1 - r;
   a.e = e;
   a.E = it.E;
   a.P = it.P;
   a.max_vars = 2;
   a.maxd = 6;
-  a.v = 2;
-  a.k = 6;
   a.cross = 1;
   a.itm_only = it.itm_only;
-  for (int i = 0; i < kMaxBasisVars; ++i) {
-    a.sh[i] = i < 2 ? it.sh[e][i] : 0.0f;
-    a.sc[i] = i < 2 ? it.sc[e][i] : 1.0f;
+  if (e >= 0) {
+    a.v = 2;
+    a.k = 6;
+    for (int i = 0; i < kMaxBasisVars; ++i) {
+      a.sh[i] = i < 2 ? it.sh[e][i] : 0.0f;
+      a.sc[i] = i < 2 ? it.sc[e][i] : 1.0f;
+    }
   }
   if (e + 1 < it.E) {
     a.vn = 2;
@@ -815,124 +832,145 @@ __device__ __forceinline__ DateArgs item_date(const LsItem& it, int e) {
   return a;
 }
 
-// Block `block` (numbered product after product) of round `round`.
-__device__ __forceinline__ void ls_round_block(const LsItem* __restrict__ items, int n, int round,
-                                               int block) {
-  const LsItem& it = items[find_block_owner(items, n, block, &LsItem::ls_block0)];
-  const int e = it.E - 1 - round;
-  if (e < -1) return;  // this product finished in an earlier round
-  const int blk = block - it.ls_block0;
-  // the two partial buffers alternate between consecutive dates
-  g_double* even = it.part0;
-  g_double* odd = it.part1;
-  if (e >= 0) {
-    const DateArgs a = item_date(it, e);
-    ls_date_block<2>(a, gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), gmem(it.coeffs),
-                     gmem(it.Y), gmem((e & 1) ? even : odd), gmem((e & 1) ? odd : even), blk,
-                     it.ls_blocks);
-  } else {
-    const DateArgs a = item_date(it, 0);
-    ls_final_block(a, gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), gmem(it.coeffs),
-                   gmem(it.Y), gmem(even), it.ls_blocks, gmem(odd), blk, it.ls_blocks);
+// Entries of a product's round-r partials: the 7 x 7 triangle, or the price.
+__device__ __forceinline__ int item_round_entries(const LsItem& it, int r) {
+  return it.E - 1 - r >= 0 ? tri_count(7) : 1;
+}
+
+// Block blk of product it's round r < E (or r == E), given the previous
+// round's partials summed into tri (r > 0): the later date's coefficients
+// (block 0 stores them), then the path work into the round's slot.  Agent as
+// for ls_date_work.
+template <bool Agent>
+__device__ __forceinline__ void ls_item_round(const LsItem& it, int r, int blk, const double* tri,
+                                              double* gram, double* scratch, double* beta,
+                                              const PathIn& first) {
+  const DateArgs a = item_round_args(it, r);
+  float unused = 0.0f;
+  if (r > 0) {
+    solve_tri(tri, 6, gram, scratch, beta);
+    // the caller's re-arming stores performed before this round's partials
+    if (Agent && r >= 2) __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (blk == 0 && threadIdx.x < 6)
+      gmem(it.coeffs)[static_cast<size_t>(a.e + 1) * 6 + threadIdx.x] = beta[threadIdx.x];
   }
+  double* out = gmem(it.xbuf) + ls_slot(r, it.ls_blocks);
+  if (a.e >= 0)
+    ls_date_work<2, Agent, 8>(a, gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), beta,
+                              gmem(it.Y), false, unused, out, blk, it.ls_blocks, first);
+  else
+    ls_final_work<Agent, 8>(a, gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), beta, gmem(it.Y),
+                            false, 0.0f, out, blk, it.ls_blocks, first);
 }
 
+__device__ __forceinline__ PathIn item_first_path(const LsItem& it, int r, int blk) {
+  return load_path(item_round_args(it, r), gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num),
+                   gmem(it.Y), min(blk * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x),
+                                   it.P - 1));
+}
+
+// Round r of every product (one launch per round; blocks numbered product
+// after product).
 __global__ void __launch_bounds__(kGramThreads)
-    k_ls_round(const LsItem* __restrict__ items, int n, int round) {
-  ls_round_block(items, n, round, blockIdx.x);
+    k_ls_round(const LsItem* __restrict__ items, int n, int r) {
+  __shared__ double tri[kTriMax];
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double scratch[kSolveScratch];
+  __shared__ double beta[kMaxGram];
+  const LsItem& it = items[find_block_owner(items, n, static_cast<int>(blockIdx.x), &LsItem::ls_block0)];
+  if (it.E - 1 - r < -1) return;  // this product finished in an earlier round
+  const int blk = blockIdx.x - it.ls_block0;
+  const PathIn first = item_first_path(it, r, blk);
+  if (r > 0)
+    sum_tri<false>(gmem(it.xbuf) + ls_slot(r - 1, it.ls_blocks), it.ls_blocks,
+                   item_round_entries(it, r - 1), tri, nullptr);
+  ls_item_round<false>(it, r, blk, tri, gram, scratch, beta, first);
 }
 
-// First-pass price of each product (one block per product).
+// First-pass price of each product (one block per product) from its last
+// round's partials.
 __global__ void __launch_bounds__(256) k_ls_price_batch(const LsItem* __restrict__ items) {
   const LsItem& it = items[blockIdx.x];
-  finalize_col(gmem(it.part1), it.ls_blocks, 1, 0, 1.0 / static_cast<double>(it.P),
-               gmem(it.price_out));
+  finalize_col(gmem(it.xbuf) + ls_slot(it.E, it.ls_blocks), it.ls_blocks, 1, 0,
+               1.0 / static_cast<double>(it.P), gmem(it.price_out));
 }
 
-// LS block b (of product q) of round r in the persistent portfolio kernel:
-// the later date's published coefficients (r > 0) into beta (LDS), the
-// date's path work (or, past date 0, the final decision and price partial),
-// then q's round count; the block completing it solves and publishes q's
-// date (or sums q's price).
-__device__ __forceinline__ void ls_round_work(const LsItem* __restrict__ items, int q, int r,
-                                              int b, double* beta, unsigned* __restrict__ sync,
-                                              int* flag) {
-  const LsItem& it = items[q];
-  const int e = it.E - 1 - r;
-  if (r > 0) ls_load_coeffs(gmem(it.coeffs) + static_cast<size_t>(e + 1) * 6, 6, beta);
-  const int blk = b - it.ls_block0;
-  g_double* out = (e & 1) ? it.part1 : it.part0;  // e = -1: the price partials go to part1
-  if (e >= 0)
-    ls_date_work<2, true>(item_date(it, e), gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), beta,
-                          gmem(it.Y), gmem(out), blk, it.ls_blocks);
-  else
-    ls_final_work<true>(item_date(it, 0), gmem(it.bv), nullptr, gmem(it.ev), gmem(it.num), beta,
-                        gmem(it.Y), gmem(out), blk, it.ls_blocks);
-  if (arrive_last(sync + 2 * q, static_cast<unsigned>(r + 1) * it.ls_blocks, flag)) {
-    if (e >= 0) {
-      ls_solve_publish(gmem(out), it.ls_blocks, 6, 6, gmem(it.coeffs) + static_cast<size_t>(e) * 6);
-      signal_add(sync + 2 * q + 1, 1u);
-    } else {
-      finalize_col<true>(gmem(out), it.ls_blocks, 1, 0, 1.0 / static_cast<double>(it.P),
-                         gmem(it.price_out));
-    }
-  }
+// Arms every product's three partial slots with kUnsetBits (one block per
+// product) before the persistent launch.
+__global__ void __launch_bounds__(256) k_ls_portfolio_arm(const LsItem* __restrict__ items) {
+  const LsItem& it = items[blockIdx.x];
+  unsigned long long* x = reinterpret_cast<unsigned long long*>(gmem(it.xbuf));
+  const size_t words = 3 * static_cast<size_t>(kTriMax) * it.ls_blocks;
+  for (size_t i = threadIdx.x; i < words; i += blockDim.x) x[i] = kUnsetBits;
 }
 
 // Every LS first pass of the portfolio in one persistent launch.  The
 // resident blocks share out the products' LS blocks (block b takes LS blocks
-// b, b + gridDim.x, ...).  In round r an LS block of product q waits for q's
-// later-date coefficients (q's published-solve count sync[2q+1] reaching r),
-// does its path work for q's date E_q - 1 - r and counts itself done
-// (sync[2q]); the LS block that completes q's round solves q's date and
-// publishes the coefficients -- or, after q's last round, sums q's price.  A
-// block takes its LS blocks of a round in whichever order their products
-// become ready, so products wait only for themselves; the block that would
-// publish never waits first, so every wait ends.  Same per-block work and
-// solves as the per-round launches, so the same results.  If a wait gives
-// up: NaN prices and sync[2n] = 1.
+// b, b + gridDim.x, ...) and walk the rounds; round E_q + 1 is product q's
+// price, summed by its LS block 0.  In round r an LS block of product q
+// needs every column of q's round r - 1 (armed words, as in k_ls_first_pass);
+// a block tries its LS blocks of the round in turn (one read of the partials
+// each), works on each as soon as its product is ready, and sleeps only when
+// none was, so products wait only for themselves.  Every LS block of round r
+// - 1 belongs to a block that is in round r - 1 or later, and a block in the
+// lowest round waits only on finished work, so every wait ends.  Re-arming
+// as in k_ls_first_pass.  Same per-block work and solves as the per-round
+// launches.  If a wait gives up: NaN prices and sync[0] = 1.
 __global__ void __launch_bounds__(kGramThreads)
     k_ls_portfolio(const LsItem* __restrict__ items, int n, int total, int max_ex,
                    unsigned* __restrict__ sync) {
   __shared__ int flag;
+  __shared__ double tri[kTriMax];
+  __shared__ double gram[kMaxGram * kMaxGram];
+  __shared__ double scratch[kSolveScratch];
   __shared__ double beta[kMaxGram];
   const int g = gridDim.x;
   const int mine = (total - static_cast<int>(blockIdx.x) + g - 1) / g;
-  for (int r = 0; r <= max_ex; ++r) {
-    // the block's LS blocks of this round (at most 32 are tracked: each as
-    // soon as its product is ready; with more than 32, in order)
-    unsigned todo = mine >= 32 ? 0xFFFFFFFFu : ((1u << mine) - 1u);
+  for (int r = 0; r <= max_ex + 1; ++r) {
     unsigned polls = 0;
-    for (int i = 0; mine > 32 && i < mine; ++i) {
-      const int b = blockIdx.x + i * g;
-      const int q = find_block_owner(items, n, b, &LsItem::ls_block0);
-      if (items[q].E - 1 - r < -1) continue;
-      if (r > 0 && !wait_at_least(sync + 2 * q + 1, static_cast<unsigned>(r), &flag)) goto fail;
-      ls_round_work(items, q, r, b, beta, sync, &flag);
-    }
-    while (mine <= 32 && todo) {
-      bool progressed = false;
-      for (int i = 0; i < mine; ++i) {
-        if (!((todo >> i) & 1u)) continue;
-        const int b = blockIdx.x + i * g;
-        const int q = find_block_owner(items, n, b, &LsItem::ls_block0);
-        if (items[q].E - 1 - r >= -1) {  // else finished in an earlier round
-          if (r > 0 && !reached(sync + 2 * q + 1, static_cast<unsigned>(r), &flag)) continue;
-          ls_round_work(items, q, r, b, beta, sync, &flag);
+    // the block's LS blocks of this round, tracked 32 at a time
+    for (int i0 = 0; i0 < mine; i0 += 32) {
+      const int cnt = mine - i0 < 32 ? mine - i0 : 32;
+      unsigned todo = cnt == 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u);
+      while (todo) {
+        bool progressed = false;
+        for (int i = 0; i < cnt; ++i) {
+          if (!((todo >> i) & 1u)) continue;
+          const int b = blockIdx.x + (i0 + i) * g;
+          const LsItem& it = items[find_block_owner(items, n, b, &LsItem::ls_block0)];
+          const int blk = b - it.ls_block0;
+          const int nb = it.ls_blocks;
+          g_double* xb = it.xbuf;
+          bool skip = r > it.E + (blk == 0 ? 1 : 0);  // finished
+          if (!skip && r == it.E + 1) {
+            // q's price: its LS block 0, once every price partial is in
+            double* out = gmem(xb) + ls_slot(it.E, nb);
+            if (!sum_tri<true>(out, nb, 1, tri, &flag, 1)) continue;
+            finalize_col<true>(out, nb, 1, 0, 1.0 / static_cast<double>(it.P),
+                               gmem(it.price_out));
+          } else if (!skip) {
+            if (r > 0 && !sum_tri<true>(gmem(xb) + ls_slot(r - 1, nb), nb,
+                                        item_round_entries(it, r - 1), tri, &flag, 1))
+              continue;
+            const PathIn first = item_first_path(it, r, blk);
+            if (r >= 2) rearm_tri(gmem(xb) + ls_slot(r - 2, nb), nb, item_round_entries(it, r - 2), blk);
+            ls_item_round<true>(it, r, blk, tri, gram, scratch, beta, first);
+            __syncthreads();  // LDS reused by the next LS block
+          }
+          todo &= ~(1u << i);
+          progressed = true;
         }
-        todo &= ~(1u << i);
-        progressed = true;
-      }
-      if (!progressed) {
-        __builtin_amdgcn_s_sleep(4);
-        if (++polls > kBarrierPolls) goto fail;
+        if (!progressed) {
+          __builtin_amdgcn_s_sleep(4);
+          if (++polls > kBarrierPolls) goto fail;
+        }
       }
     }
   }
   return;
 fail:
   if (threadIdx.x == 0) {
-    sync[2 * n] = 1u;
+    sync[0] = 1u;
     for (int q = 0; q < n; ++q) *gmem(items[q].price_out) = __builtin_nan("");
   }
 }
@@ -1010,76 +1048,52 @@ extern "C" int ls_first_pass_gpu(int n_ex, const float* basis_variables, const i
   const LsLayout lay = ls_layout(paths);
   char* ws = static_cast<char*>(workspace);
   float* Y = reinterpret_cast<float*>(ws + lay.y);
-  double* parts[2] = {reinterpret_cast<double*>(ws + lay.part),
-                      reinterpret_cast<double*>(ws + lay.part2)};
-  const int ng = gram_blocks(paths);
+  double* xbuf = reinterpret_cast<double*>(ws + lay.xbuf);
   hipStream_t st = as_stream(stream);
   const bool cross = use_cross != 0;
-  // One persistent launch over all dates (after zeroing its sync words) when
-  // every date's basis has at most 7 functions, unless the blocks cannot all
-  // be resident or KOO_LS_LAUNCH_PER_DATE=1 asks for one launch per date
-  // (the same per-date code and partials).
-  const char* per_date = getenv("KOO_LS_LAUNCH_PER_DATE");
+  AllDates d{};
+  d.E = n_ex;
+  d.P = paths;
+  d.max_vars = max_vars;
+  d.maxd = basis_count(max_vars, cross);
+  d.cross = cross;
+  d.itm_only = itm_only != 0;
   bool small = true;  // the persistent kernel takes bases of at most 7 functions
-  for (int e = 0; e < n_ex; ++e) small = small && basis_count(vars_each_step[e], cross) + 1 <= 8;
-  const int np = (!small || (per_date && per_date[0] == '1')) ? 0 : ls_persistent_blocks(paths);
-  if (np > 0) {
-    AllDates d{};
-    d.E = n_ex;
-    d.P = paths;
-    d.max_vars = max_vars;
-    d.maxd = basis_count(max_vars, cross);
-    d.cross = cross;
-    d.itm_only = itm_only != 0;
-    for (int e = 0; e < n_ex; ++e) {
-      d.v[e] = static_cast<signed char>(vars_each_step[e]);
-      d.k[e] = static_cast<signed char>(basis_count(vars_each_step[e], cross));
-      fill_norm(var_shift, var_scale, e, vars_each_step[e], max_vars, d.sh[e], d.sc[e]);
-    }
+  for (int e = 0; e < n_ex; ++e) {
+    d.v[e] = static_cast<signed char>(vars_each_step[e]);
+    d.k[e] = static_cast<signed char>(basis_count(vars_each_step[e], cross));
+    fill_norm(var_shift, var_scale, e, vars_each_step[e], max_vars, d.sh[e], d.sc[e]);
+    small = small && d.k[e] + 1 <= 8;
+  }
+  bool resident = false;
+  const int nb = ls_pass_blocks(paths, &resident);
+  // One persistent launch over all rounds (after arming its partial words)
+  // when every basis has at most 7 functions and the blocks can all be
+  // resident, unless KOO_LS_LAUNCH_PER_DATE=1 asks for one launch per round.
+  const char* per_date = getenv("KOO_LS_LAUNCH_PER_DATE");
+  if (small && resident && !(per_date && per_date[0] == '1')) {
     unsigned* bar = reinterpret_cast<unsigned*>(ws + lay.bar);
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, bar, 3);
-    hipLaunchKernelGGL(k_ls_first_pass, dim3(np), dim3(kGramThreads), 0, st, d, basis_variables,
-                       cash_flows, exercise_values, numeraires, coefficients, Y, parts[0],
-                       parts[1], bar, price_out);
+    const size_t words = 3 * static_cast<size_t>(kTriMax) * nb;
+    hipLaunchKernelGGL(k_fill_u64, dim3((words + 255) / 256), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long*>(xbuf), words, kUnsetBits);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, bar, 1);
+    hipLaunchKernelGGL(k_ls_first_pass, dim3(nb), dim3(kGramThreads), 0, st, d, basis_variables,
+                       cash_flows, exercise_values, numeraires, coefficients, Y, xbuf, bar,
+                       price_out);
     return check_launch("ls_first_pass_gpu");
   }
-  DateArgs a{};
-  a.E = n_ex;
-  a.P = paths;
-  a.max_vars = max_vars;
-  a.maxd = basis_count(max_vars, cross);
-  a.cross = use_cross != 0;
-  a.itm_only = itm_only != 0;
-  for (int e = n_ex - 1; e >= 0; --e) {
-    a.e = e;
-    a.v = vars_each_step[e];
-    a.k = basis_count(a.v, cross);
-    fill_norm(var_shift, var_scale, e, a.v, max_vars, a.sh, a.sc);
-    if (e + 1 < n_ex) {
-      a.vn = vars_each_step[e + 1];
-      a.kn = basis_count(a.vn, cross);
-      fill_norm(var_shift, var_scale, e + 1, a.vn, max_vars, a.shn, a.scn);
-    } else {
-      a.vn = a.kn = 0;
-    }
-    double* out = parts[e & 1];
-    const double* later = parts[(e + 1) & 1];
-    if (a.k + 1 <= 8)
-      hipLaunchKernelGGL(k_ls_date<2>, dim3(ng), dim3(kGramThreads), 0, st, a, basis_variables,
-                         cash_flows, exercise_values, numeraires, coefficients, Y, later, out);
+  for (int r = 0; r <= n_ex; ++r) {
+    const int e = n_ex - 1 - r;
+    if (e < 0 || d.k[e] + 1 <= 8)
+      hipLaunchKernelGGL(k_ls_round_single<2>, dim3(nb), dim3(kGramThreads), 0, st, d, r,
+                         basis_variables, cash_flows, exercise_values, numeraires, coefficients,
+                         Y, xbuf);
     else
-      hipLaunchKernelGGL(k_ls_date<1>, dim3(ng), dim3(kGramThreads), 0, st, a, basis_variables,
-                         cash_flows, exercise_values, numeraires, coefficients, Y, later, out);
+      hipLaunchKernelGGL(k_ls_round_single<1>, dim3(nb), dim3(kGramThreads), 0, st, d, r,
+                         basis_variables, cash_flows, exercise_values, numeraires, coefficients,
+                         Y, xbuf);
   }
-  a.e = 0;
-  a.v = vars_each_step[0];
-  a.k = basis_count(a.v, cross);
-  fill_norm(var_shift, var_scale, 0, a.v, max_vars, a.sh, a.sc);
-  // date 0's partials are in parts[0]; the price partials go to parts[1]
-  hipLaunchKernelGGL(k_ls_final, dim3(ng), dim3(kGramThreads), 0, st, a, basis_variables,
-                     cash_flows, exercise_values, numeraires, coefficients, Y, parts[0], ng,
-                     parts[1]);
-  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, parts[1], ng, 1,
+  hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, xbuf + ls_slot(n_ex, nb), nb, 1,
                      1.0 / static_cast<double>(paths), price_out);
   return check_launch("ls_first_pass_gpu");
 }
@@ -1434,7 +1448,7 @@ extern "C" int reduceOuterProduct_cublas(int paths, int row_size, int col_size, 
 size_t ls_portfolio_bar_offset(int n_products) {
   return align256(sizeof(LsItem) * static_cast<size_t>(n_products));
 }
-int ls_portfolio_sync_words(int n_products) { return 2 * n_products + 1; }
+int ls_portfolio_sync_words(int) { return 1; }  // the error word
 
 extern "C" size_t bermudan_ls_portfolio_descriptor_bytes(int n_products) {
   return n_products > 0 ? ls_portfolio_bar_offset(n_products) +
@@ -1487,8 +1501,7 @@ extern "C" int bermudan_ls_portfolio_prepare(int n_products, const koo_ls_produc
     it.ev = (g_float*)q.exercise_values;
     it.num = (g_float*)q.numeraires;
     it.Y = (g_float*)(ws + lay.y);
-    it.part0 = (g_double*)(ws + lay.part);
-    it.part1 = (g_double*)(ws + lay.part2);
+    it.xbuf = (g_double*)(ws + lay.xbuf);
     it.coeffs = (g_double*)q.coefficients;
     it.price_out = (g_double*)q.price_out;
     it.strike = q.strike;
@@ -1570,9 +1583,10 @@ extern "C" int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan,
     hipLaunchKernelGGL(k_berm_paths_batch, dim3(nb), dim3(256), lds, st, items, plan->products,
                        b0);
   }
-  // The LS first passes: one persistent launch (after zeroing its barrier
-  // words) with as many blocks as can be resident, or, when none can or
-  // KOO_LS_LAUNCH_PER_DATE=1, one launch per round and one for the prices.
+  // The LS first passes: one persistent launch (after arming every
+  // product's partial words and zeroing the error word) with as many blocks
+  // as can be resident, or, when none can or KOO_LS_LAUNCH_PER_DATE=1, one
+  // launch per round and one for the prices.
   int per_cu = 0;
   const char* per_date = getenv("KOO_LS_LAUNCH_PER_DATE");
   if (!(per_date && per_date[0] == '1') &&
@@ -1584,8 +1598,8 @@ extern "C" int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan,
     unsigned* bar = reinterpret_cast<unsigned*>(static_cast<char*>(const_cast<void*>(
                                                     descriptors_device)) +
                                                 ls_portfolio_bar_offset(plan->products));
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, bar,
-                       ls_portfolio_sync_words(plan->products));
+    hipLaunchKernelGGL(k_ls_portfolio_arm, dim3(plan->products), dim3(256), 0, st, items);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, bar, 1);
     hipLaunchKernelGGL(k_ls_portfolio, dim3(grid), dim3(kGramThreads), 0, st, items,
                        plan->products, plan->ls_blocks, plan->max_ex, bar);
     return check_launch(who);

@@ -59,6 +59,42 @@ __device__ __forceinline__ T basis_value(int j, int v, bool cross, T x0, T x1, T
   return x * x;
 }
 
+// Basis function j of v variables (as basis_value; row k is the target and
+// rows past it are zero) as the product s[ia] * s[ib] of a staged path
+// vector s = [w, w x0, w x1, w x2, w x3, w t, 0, 1] (w the path's 0/1
+// weight, t its target): since w * w == w, s[ia] * s[ib] is w times the
+// function.  Lets a lane whose MFMA row is j read its two factors for any
+// path without branching on j.
+__device__ __forceinline__ void basis_factors(int j, int v, int k, bool cross, int& ia, int& ib) {
+  constexpr int kOne = 7, kZero = 6, kTarget = 5;
+  ia = 0;
+  ib = kOne;
+  if (j > k) {
+    ia = ib = kZero;
+  } else if (j == k) {
+    ia = kTarget;
+  } else if (j == 0) {
+  } else if (!cross) {
+    ia = ((j - 1) >> 1) + 1;
+    if (!(j & 1)) ib = ia;
+  } else if (j <= v) {
+    ia = j;
+  } else if (v == 1) {
+    ia = ib = 1;
+  } else if (v == 2) {
+    ia = j == 5 ? 2 : 1;
+    ib = j == 3 ? 1 : 2;
+  } else {
+    const int i = j - 1 - v;
+    if (i < v) {
+      ia = i + 1;
+      ib = (i + 1 == v ? 0 : i + 1) + 1;
+    } else {
+      ia = ib = i - v + 1;
+    }
+  }
+}
+
 __host__ __device__ inline int basis_count(int v, bool cross) {
   if (!cross) return 1 + 2 * v;
   return v == 0 ? 1 : (v == 1 ? 3 : (v == 2 ? 6 : 1 + 3 * v));
@@ -96,24 +132,64 @@ __device__ __forceinline__ void block_gram(const floatx4& acc, int n, float* lds
   }
 }
 
-// block_gram for per-lane FP64 accumulators (the LS first pass adds each
-// MFMA's 4-path FP32 result into FP64, so its Gram carries no FP32 rounding
-// beyond that of the products and 4-term sums): the same layout and
-// partials, the waves' tiles staged in FP64.
+// Entries of the upper triangle (i <= j, row-major) of an n x n symmetric
+// matrix, and the (i, j) of entry t.
+__host__ __device__ constexpr int tri_count(int n) { return n * (n + 1) / 2; }
+constexpr int kTriMax = tri_count(kMaxGram);
+__device__ __forceinline__ void tri_index(int t, int n, int& i, int& j) {
+  i = 0;
+  while (t >= n - i) {
+    t -= n - i;
+    ++i;
+  }
+  j = i + t;
+}
+
+// Block partials exchanged between the blocks of one running kernel carry no
+// counter: the buffer starts filled with this bit pattern (a signalling NaN,
+// which no sum of finite terms and no arithmetic NaN has), a block's partial
+// is there once the word no longer holds it.  64-bit aligned stores are
+// single-copy atomic, so a reader sees either the pattern or the value.
+constexpr unsigned long long kUnsetBits = 0x7FF4C0FFEE15C0DEull;
+__device__ __forceinline__ bool is_unset(double v) {
+  return static_cast<unsigned long long>(__double_as_longlong(v)) == kUnsetBits;
+}
+
+// Fills n 64-bit words with v (a kernel, not a memset, so a launch sequence
+// that uses it can be captured in a graph).
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, size_t n, unsigned long long v) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// The Gram of each wave's MFMA accumulators -- FP64 per lane (the LS first
+// pass adds each MFMA's 4-path FP32 result into FP64, so its Gram carries no
+// FP32 rounding beyond that of the products and 4-term sums) -- summed over
+// the block's waves in FP64, the upper triangle of the n x n matrix into the
+// block's column of the entry-major partials out[t * nblocks + block],
+// t < tri_count(n).  GROUPS == 2 adds the two diagonal 8x8 blocks of D,
+// GROUPS == 1 takes the 16x16 result.  Agent: agent-coherent stores (the
+// persistent kernels).  Every lane of the block calls it; the caller
+// synchronises before lds_d is written again.
 template <int GROUPS, bool Agent = false>
-__device__ __forceinline__ void block_gram_d(const double (&acc)[4], int n,
-                                             double* lds_d /* [nw][256] */, double* part,
-                                             int block, int nblocks) {
+__device__ __forceinline__ void block_gram_tri(const double (&acc)[4], int n,
+                                               double* lds_d /* [nw][256] */, double* out,
+                                               int block, int nblocks) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double* d = lds_d + wave * 256;
 #pragma unroll
   for (int i = 0; i < 4; ++i) d[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
   __syncthreads();
+  // 16 lanes per entry (4 entries per wavefront), lane q of the group adds
+  // the tiles of waves q, q+16, ...; then a 16-lane butterfly.
   const int q = lane & 15;
-  for (int t = wave * 4 + (lane >> 4); t - (lane >> 4) < n * n; t += nw * 4) {
+  const int T = tri_count(n);
+  for (int t = wave * 4 + (lane >> 4); t - (lane >> 4) < T; t += nw * 4) {
     double s = 0.0;
-    if (t < n * n) {
-      const int i = t / n, j = t - i * n;
+    if (t < T) {
+      int i, j;
+      tri_index(t, n, i, j);
       for (int w = q; w < nw; w += 16) {
         const double* dw = lds_d + w * 256;
         s += dw[i * 16 + j];
@@ -122,13 +198,79 @@ __device__ __forceinline__ void block_gram_d(const double (&acc)[4], int n,
     }
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (q == 0 && t < n * n) {
+    if (q == 0 && t < T) {
       if (Agent)
-        store_agent(part + static_cast<size_t>(t) * nblocks + block, s);
+        store_agent(out + static_cast<size_t>(t) * nblocks + block, s);
       else
-        part[static_cast<size_t>(t) * nblocks + block] = s;
+        out[static_cast<size_t>(t) * nblocks + block] = s;
     }
   }
+}
+
+// Sums the entry-major partials X[t * nb + b] over b < nb into sum[t] (LDS)
+// for t < T, in a fixed order (each wavefront 4 entries at a time, each lane 4
+// blocks of each, 16 loads in flight, then a wavefront butterfly), so every
+// block that sums the same partials gets the same bits.  Poll: agent loads,
+// re-read until no word holds kUnsetBits, at most `reads` times (1: a
+// single try); returns false on every lane if a word was still unset.
+// Contains barriers; every lane of the block calls it.
+constexpr unsigned kBarrierPolls = 1u << 22;
+
+template <bool Poll>
+__device__ __forceinline__ bool sum_tri(const double* X, int nb, int T, double* sum,
+                                        int* flag_lds, unsigned reads = kBarrierPolls) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (Poll) {
+    if (threadIdx.x == 0) *flag_lds = 1;
+    __syncthreads();
+  }
+  bool ok = true;
+  for (int t0 = wave * 4; ok && t0 < T; t0 += nw * 4) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = 0; ok && b0 < nb; b0 += 256) {
+      double v[4][4];
+      unsigned polls = 0;
+      for (;;) {
+        bool mine = true;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int t = t0 + u, b = b0 + lane + 64 * r;
+            const double* src = X + static_cast<size_t>(t) * nb + b;
+            const bool in = t < T && b < nb;
+            v[u][r] = in ? (Poll ? load_agent(src) : *src) : 0.0;
+            if (Poll && in && is_unset(v[u][r])) mine = false;
+          }
+        if (!Poll || __all(mine)) break;
+        if (++polls >= reads) {
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] += (v[u][0] + v[u][1]) + (v[u][2] + v[u][3]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double tot = wave_sum_f64(s[u]);
+      if (lane == 0 && t0 + u < T) sum[t0 + u] = tot;
+    }
+  }
+  if (Poll && !ok && lane == 0) *flag_lds = 0;
+  __syncthreads();
+  if (!Poll) return true;
+  ok = *flag_lds != 0;
+  __syncthreads();
+  return ok;
+}
+
+// Writes kUnsetBits over the block's own entries t < T of X (agent stores):
+// the owner re-arms its words once every block has read them.
+__device__ __forceinline__ void rearm_tri(double* X, int nb, int T, int block) {
+  for (int t = threadIdx.x; t < T; t += blockDim.x)
+    store_agent(X + static_cast<size_t>(t) * nb + block, __longlong_as_double(kUnsetBits));
 }
 
 // Sums the entry-major partials part[t * nb + b] over b < nb into sum[t]
@@ -463,27 +605,23 @@ __device__ void solve_normal_equations(const double* G, int k, double* beta, dou
   jacobi_truncated(k, A, V, b, kSvdTolerance, beta);
 }
 
-// Every block of a date kernel solves the later date's normal equations
-// itself (same partials, same order, so the same coefficients): the sum of
-// the entry-major partials into LDS, the FP64 solve by lane 0, the
-// coefficients left in beta (LDS).  Block 0 also stores them.  Contains
-// barriers.
-template <bool Agent = false>
-__device__ __forceinline__ void solve_date(const double* __restrict__ part, int nb, int k,
-                                           double* gram, double* scratch, double* beta,
-                                           double* __restrict__ coeffs_row, int maxd, int block) {
-  reduce_partials<Agent>(part, nb, (k + 1) * (k + 1), gram);
-  if (threadIdx.x == 0) {
-    solve_normal_equations(gram, k, beta, scratch);
-    if (block == 0)
-      for (int i = 0; i < maxd; ++i) coeffs_row[i] = i < k ? beta[i] : 0.0;
+// The normal equations of one date from the summed upper triangle (LDS,
+// tri_count(k + 1) entries, the target last): the full (k+1) x (k+1) Gram in
+// LDS, then the FP64 solve by lane 0 into beta (LDS).  Contains barriers;
+// every lane of the block calls it.
+__device__ __forceinline__ void solve_tri(const double* tri, int k, double* gram, double* scratch,
+                                          double* beta) {
+  const int n = k + 1;
+  for (int t = threadIdx.x; t < tri_count(n); t += blockDim.x) {
+    int i, j;
+    tri_index(t, n, i, j);
+    gram[i * n + j] = tri[t];
+    gram[j * n + i] = tri[t];
   }
   __syncthreads();
+  if (threadIdx.x == 0) solve_normal_equations(gram, k, beta, scratch);
+  __syncthreads();
 }
-
-// Polls of a persistent kernel's waits: seconds, where a normal wait takes a
-// few microseconds; a wait that reaches it gives up so the kernel can end.
-constexpr unsigned kBarrierPolls = 1u << 22;
 
 // Zeroes n 32-bit words (the barrier counters before a persistent launch;
 // a kernel, not a memset, so the launch sequence can be captured in a graph).
