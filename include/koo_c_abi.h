@@ -794,6 +794,51 @@ int bermudan_ls_portfolio_prepare(int n_products, const koo_ls_product* products
 int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan, const void* descriptors_device,
                               void* stream);
 
+/* =====================================================================
+ * A portfolio of Bermudan swaptions in one persistent launch (new; no
+ * counterpart in the reference, which prices product after product):
+ * for every product the fused Sobol -> scramble -> normal -> Brownian bridge
+ * generation (as sobol_bridge_gpu into its variates; skipped when gen is
+ * NULL and the variates are given), the path quantities and LS first pass
+ * (as bermudan_paths_gpu + ls_first_pass_gpu), and the frozen forward and
+ * adjoint backward under the first pass's coefficients (as
+ * bermudan_frozen_fwd/_bwd, checkpoint interval per product), all in one
+ * kernel whose resident blocks are split into one team per product.  Each
+ * product is described twice, as for the batched calls (ls[i] and adj[i]
+ * must agree on sizes, model arrays, strike, dates, itm_only and
+ * coefficients; its partials are the backward's exchange words, so they are
+ * overwritten), with factors <= 5, rates <= 45, dates <= 40.  Outputs per
+ * product: variates (with gen), basis variables, exercise values,
+ * numeraires, coefficients, first-pass price, frozen price and 2N Greeks.
+ * prepare() sizes the teams and writes the descriptors into device_buffer
+ * (bermudan_portfolio_persistent_bytes bytes) with synchronous copies; run()
+ * launches three kernels (arming, error word, the portfolio) and neither
+ * synchronises nor allocates, so it can be captured in a graph.  The
+ * frozen Greeks equal bermudan_frozen_fwd/_bwd on the same variates and
+ * coefficients bit for bit; the first pass sums its partials over the
+ * team's blocks, so its coefficients agree with ls_first_pass_gpu to
+ * rounding.  A wait that gives up leaves NaN prices.
+ * =================================================================== */
+typedef struct koo_gen_product {
+  unsigned path_offset;          /* Sobol points path_offset + 1, ... */
+  int number_scrambled;          /* dimensions scrambled */
+  const unsigned* directions;    /* device, dims x 32 */
+  const unsigned* scramblers;    /* device, number_scrambled */
+  const int* allocation_indices; /* device, F x S (bridge_allocate_diagonal) */
+} koo_gen_product;
+
+typedef struct koo_persistent_plan {
+  int products, grid, lds_bytes, tab_offset, blocks_per_cu;
+  size_t err_offset;
+} koo_persistent_plan;
+
+size_t bermudan_portfolio_persistent_bytes(int n_products, const int* paths);
+int bermudan_portfolio_persistent_prepare(int n_products, const koo_gen_product* gen,
+                                          const koo_ls_product* ls, const koo_berm_product* adj,
+                                          void* device_buffer, koo_persistent_plan* plan);
+int bermudan_portfolio_persistent_run(const koo_persistent_plan* plan, void* device_buffer,
+                                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

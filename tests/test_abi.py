@@ -107,7 +107,8 @@ def test_ctypes_structs_match_the_c_layout(tmp_path):
     """The ctypes mirrors of the descriptor structs have the C header's
     sizes and member offsets (gcc on the header itself)."""
     structs = {"koo_berm_product": koo.BermProduct, "koo_portfolio_plan": koo.PortfolioPlan,
-               "koo_ls_product": koo.LsProduct}
+               "koo_ls_product": koo.LsProduct, "koo_gen_product": koo.GenProduct,
+               "koo_persistent_plan": koo.PersistentPlan}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "koo_c_abi.h"',
              "int main(void) {"]
     for cname, py in structs.items():

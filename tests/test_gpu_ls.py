@@ -514,8 +514,9 @@ def test_batched_portfolio_equals_per_product_calls(torch_cuda):
     torch = torch_cuda
     prods = configs.bermudan_portfolio(n_products=9, paths=(1 << 12) + 77, seed=78)
     ks = lambda j: (1, 3, 8)[j % 3]  # noqa: E731
-    batch = pipeline.PortfolioPricer(prods, checkpoint_every=ks, n_streams=2, batched=True)
-    assert batch.batched
+    batch = pipeline.PortfolioPricer(prods, checkpoint_every=ks, n_streams=2, batched=True,
+                                     persistent=False)
+    assert batch.batched and not batch.persistent
     print("batched portfolio, 9 products", flush=True)
     batch.run(capture=False)
     torch.cuda.synchronize()
@@ -535,6 +536,66 @@ def test_batched_portfolio_equals_per_product_calls(torch_cuda):
         assert abs(pr.price1.item() - p1) <= 1e-12 * abs(p1), j
 
 
+@pytest.mark.parametrize("seed,paths", [(78, (1 << 12) + 77), (81, 1 << 14)])
+def test_persistent_portfolio_equals_stage_calls(torch_cuda, seed, paths):
+    """The whole portfolio in one persistent launch against the stage calls
+    run on its own outputs: the generated variates equal sobol_bridge_gpu's
+    to 2 ulp; the path quantities equal bermudan_paths_gpu's on those
+    variates bit for bit; the first pass agrees with ls_first_pass_gpu to rounding (its
+    partials are summed over the product's team, not the single call's
+    blocks); the frozen price and Greeks equal bermudan_frozen_fwd/_bwd on
+    the same variates and coefficients bit for bit (checkpoint intervals 1,
+    3 and 8 mixed)."""
+    torch = torch_cuda
+    prods = configs.bermudan_portfolio(n_products=9, paths=paths, seed=seed)
+    ks = lambda j: (1, 3, 8)[j % 3]  # noqa: E731
+    pf = pipeline.PortfolioPricer(prods, checkpoint_every=ks, n_streams=2)
+    assert pf.persistent
+    print(f"persistent portfolio, 9 products x {paths} paths", flush=True)
+    for pr in pf.pricers:
+        for t in (pr.z, pr.bv, pr.ev, pr.num):
+            t.fill_(float("nan"))
+    pf.run(capture=False)
+    torch.cuda.synchronize()
+    got = pf.results()
+    print(f"  grid {pf.pplan.grid} blocks ({pf.pplan.blocks_per_cu} per CU), "
+          f"LDS {pf.pplan.lds_bytes} B", flush=True)
+    for j, pr in enumerate(pf.pricers):
+        p1, p2, g = got[j]
+        assert np.isfinite(p1) and np.isfinite(p2) and np.all(np.isfinite(g)), j
+        z, bv, ev, num = (t.clone() for t in (pr.z, pr.bv, pr.ev, pr.num))
+        coeffs = pr.coeffs.clone()
+        pr.generate()
+        torch.cuda.synchronize()
+        # the same Sobol points, quantiles and bridge; the LDS bridge and the
+        # register bridge may round a bridge point (|W| up to about 25)
+        # differently by an ulp or two, which the increments inherit
+        d = (z - pr.z).abs()
+        assert d.max().item() <= 4e-6, (j, d.max().item())
+        if j == 0:
+            print(f"  variates vs sobol_bridge_gpu: {int((d > 0).sum())} of {d.numel()} differ, "
+                  f"max {d.max().item():.2e}", flush=True)
+        pr.z.copy_(z)  # the stage calls below on the persistent kernel's variates
+        pr.paths_kernel()
+        torch.cuda.synchronize()
+        assert torch.equal(bv, pr.bv) and torch.equal(ev, pr.ev) and torch.equal(num, pr.num), j
+        fp, fg = pr.greeks(checkpoint_every=ks(j))
+        torch.cuda.synchronize()
+        assert fp.item() == p2 and np.array_equal(fg.cpu().numpy(), g), j
+        pr.regress()
+        torch.cuda.synchronize()
+        # the FP64 normal equations summed in another order; a one-factor
+        # product's are close to singular (condition numbers up to 1e12), so
+        # its coefficients move by up to 1e-6 relative while the fitted
+        # continuation values and the price do not
+        c1, c0 = pr.coeffs.cpu().numpy(), coeffs.cpu().numpy()
+        worst = np.max(np.abs(c1 - c0) / (np.abs(c0) + 1e-12))
+        print(f"  product {j}: first pass {p1:.12f} vs {pr.price1.item():.12f}, coefficients "
+              f"max rel diff {worst:.1e}", flush=True)
+        assert np.allclose(c1, c0, rtol=1e-6, atol=1e-12), j
+        assert abs(pr.price1.item() - p1) <= 1e-12 * abs(p1), j
+
+
 def test_portfolio_persistent_ls_equals_per_round_launches(torch_cuda):
     """The portfolio's LS first passes as one persistent launch (resident
     blocks walking all products' blocks, a grid barrier per round) and as one
@@ -543,8 +604,8 @@ def test_portfolio_persistent_ls_equals_per_round_launches(torch_cuda):
     import os
     torch = torch_cuda
     prods = configs.bermudan_portfolio(n_products=12, paths=(1 << 13) + 5, seed=79)
-    pf = pipeline.PortfolioPricer(prods, checkpoint_every=4, n_streams=2)
-    assert pf.batched
+    pf = pipeline.PortfolioPricer(prods, checkpoint_every=4, n_streams=2, persistent=False)
+    assert pf.batched and not pf.persistent
     print("portfolio LS, 12 products", flush=True)
     runs = []
     for mode in ("0", "1"):

@@ -16,94 +16,11 @@
 #include <unordered_map>
 #include <vector>
 
-#include "koo_common.h"
+#include "bridge_device.h"
 
 namespace koo {
 
 namespace {
-
-constexpr int kBridgeMaxSteps = 128;
-
-// Construction schedule of brownian_bridge::initialize (brownian_bridge.cu:
-// 639-700) on t_i = i + 1 (brownian_bridge.cu:359-362), kept in float with
-// the same operations so the weights are the reference's bit for bit.
-struct BridgeTables {
-  int16_t bridge[kBridgeMaxSteps];
-  int16_t left[kBridgeMaxSteps];
-  int16_t right[kBridgeMaxSteps];
-  float lw[kBridgeMaxSteps];
-  float rw[kBridgeMaxSteps];
-  float sd[kBridgeMaxSteps];
-};
-
-BridgeTables make_tables(int size) {
-  BridgeTables T{};
-  std::vector<float> t(size);
-  for (int i = 0; i < size; ++i) t[i] = static_cast<float>(i + 1);
-  std::vector<int> map(size, 0);
-  map[size - 1] = 1;
-  T.bridge[0] = static_cast<int16_t>(size - 1);
-  T.sd[0] = std::sqrt(t[size - 1]);
-  T.lw[0] = T.rw[0] = 0.0f;
-  for (int j = 0, i = 1; i < size; ++i) {
-    while (map[j]) ++j;
-    int k = j;
-    while (!map[k]) ++k;
-    const int l = j + ((k - 1 - j) >> 1);
-    map[l] = i;
-    T.bridge[i] = static_cast<int16_t>(l);
-    T.left[i] = static_cast<int16_t>(j);
-    T.right[i] = static_cast<int16_t>(k);
-    if (j != 0) {
-      T.lw[i] = (t[k] - t[l]) / (t[k] - t[j - 1]);
-      T.rw[i] = (t[l] - t[j - 1]) / (t[k] - t[j - 1]);
-      T.sd[i] = std::sqrt(((t[l] - t[j - 1]) * (t[k] - t[l])) / (t[k] - t[j - 1]));
-    } else {
-      T.lw[i] = (t[k] - t[l]) / t[k];
-      T.rw[i] = t[l] / t[k];
-      T.sd[i] = std::sqrt(t[l] * (t[k] - t[l]) / t[k]);
-    }
-    j = k + 1;
-    if (j >= size) j = 0;
-  }
-  return T;
-}
-
-const BridgeTables& tables_for(int steps) {
-  static std::mutex mu;
-  static std::unordered_map<int, BridgeTables> cache;
-  std::lock_guard<std::mutex> lock(mu);
-  auto it = cache.find(steps);
-  if (it == cache.end()) it = cache.emplace(steps, make_tables(steps)).first;
-  return it->second;
-}
-
-constexpr int kBridgeThreads = 256;
-
-// Builds one factor's path for this lane's path index from the variate
-// source `z(i)` (bridge variate i), differences it and stores (S,F,P).
-template <class Source>
-__device__ __forceinline__ void bridge_lane(const BridgeTables& T, int steps, int factors,
-                                            int paths, int p, float* lds, Source z,
-                                            float* __restrict__ out) {
-  const int f = blockIdx.y;
-  const int lane = threadIdx.x;
-  float* path = lds + lane;  // path[i * blockDim.x]
-  const int bd = blockDim.x;
-  path[(steps - 1) * bd] = T.sd[0] * z(0);
-  for (int i = 1; i < steps; ++i) {
-    const int j = T.left[i], k = T.right[i], l = T.bridge[i];
-    float v = T.rw[i] * path[k * bd];
-    if (j != 0) v = T.lw[i] * path[(j - 1) * bd] + v;
-    path[l * bd] = v + T.sd[i] * z(i);
-  }
-  float prev = 0.0f;
-  for (int i = 0; i < steps; ++i) {
-    const float w = path[i * bd];
-    out[(static_cast<size_t>(i) * factors + f) * paths + p] = w - prev;
-    prev = w;
-  }
-}
 
 __global__ void __launch_bounds__(kBridgeThreads)
     k_bridge(const BridgeTables T, int steps, int factors, int paths,
@@ -113,7 +30,7 @@ __global__ void __launch_bounds__(kBridgeThreads)
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= paths) return;
   const int* idx = alloc + blockIdx.y * steps;
-  bridge_lane(T, steps, factors, paths, p, lds,
+  bridge_lane(T, steps, factors, paths, p, blockIdx.y, lds,
               [&](int i) { return in[static_cast<size_t>(idx[i]) * paths + p]; }, out);
 }
 
@@ -134,7 +51,7 @@ __global__ void __launch_bounds__(kBridgeThreads)
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= paths) return;
   const uint32_t n = static_cast<uint32_t>(p) + off1;
-  bridge_lane(T, steps, factors, paths, p, lds,
+  bridge_lane(T, steps, factors, paths, p, blockIdx.y, lds,
               [&](int i) {
                 const int d = idx[i];
                 const uint32_t s = d < number_scrambled ? scr[d] : 0u;
@@ -185,8 +102,8 @@ __device__ __forceinline__ void bridge_point(V (&w)[S], const BridgeTables& T, Z
   } else {
     constexpr int j = M.left[I], k = M.right[I], l = M.bridge[I];
     V v = T.rw[I] * w[k];
-    if constexpr (j != 0) v = T.lw[I] * w[j - 1] + v;
-    w[l] = v + T.sd[I] * z(I);
+    if constexpr (j != 0) v = bridge_fma(T.lw[I], w[j - 1], v);
+    w[l] = bridge_fma(T.sd[I], z(I), v);
   }
 }
 
@@ -208,8 +125,8 @@ __device__ __forceinline__ void bridge_point_inplace(V (&w)[S], const BridgeTabl
   } else {
     constexpr int j = M.left[I], k = M.right[I], l = M.bridge[I];
     V v = T.rw[I] * w[k];
-    if constexpr (j != 0) v = T.lw[I] * w[j - 1] + v;
-    w[l] = v + T.sd[I] * w[l];
+    if constexpr (j != 0) v = bridge_fma(T.lw[I], w[j - 1], v);
+    w[l] = bridge_fma(T.sd[I], w[l], v);
   }
 }
 

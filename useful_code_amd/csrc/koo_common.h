@@ -31,6 +31,12 @@ inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
 
+// LS workspace (ls_workspace): byte offset of the first pass's partial
+// slots, and the most blocks they hold a column for (the running values lead
+// the workspace).
+size_t ls_workspace_xbuf_offset(int paths);
+int ls_workspace_xbuf_columns(int paths);
+
 // 1-D launch helper: blocks for n items at `per_block` items each, capped so a
 // grid-stride loop covers the rest.
 inline unsigned grid_for(int64_t n, int per_block, int64_t cap = 1 << 20) {
@@ -68,8 +74,11 @@ __device__ __forceinline__ float invnorm_u32(uint32_t x) {
   // z = -ln(w) = -ln2 * log2(w).
   const float w = tf * 4.656612873077392578125e-10f;
   const float z = __builtin_amdgcn_logf(w) * -0.693147180559945309417f;
-  const float p = P1 + z * (P2 + z * (P3 + z * (P4 + (P5 + P6 * z) * z)));
-  const float q = 1.0f + z * (Q2 + z * (Q3 + z * (Q4 + (Q5 + Q6 * z) * z)));
+  // Horner with the multiply-adds written out, so every kernel that inlines
+  // this rounds alike (the same bits from the staged, fused and persistent
+  // generators)
+  const float p = fmaf(z, fmaf(z, fmaf(z, fmaf(fmaf(P6, z, P5), z, P4), P3), P2), P1);
+  const float q = fmaf(z, fmaf(z, fmaf(z, fmaf(fmaf(Q6, z, Q5), z, Q4), Q3), Q2), 1.0f);
   // z p / q by the hardware reciprocal and one Newton correction of the
   // quotient (q >= 1, so nothing overflows or underflows)
   const float n = z * p;

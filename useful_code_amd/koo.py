@@ -55,6 +55,18 @@ class LsProduct(ctypes.Structure):
                 ("coefficients", _p), ("price_out", _p), ("ls_blocks", _i)]
 
 
+class GenProduct(ctypes.Structure):
+    """koo_gen_product (include/koo_c_abi.h)."""
+    _fields_ = [("path_offset", _u), ("number_scrambled", _i), ("directions", _p),
+                ("scramblers", _p), ("allocation_indices", _p)]
+
+
+class PersistentPlan(ctypes.Structure):
+    """koo_persistent_plan (include/koo_c_abi.h)."""
+    _fields_ = [("products", _i), ("grid", _i), ("lds_bytes", _i), ("tab_offset", _i),
+                ("blocks_per_cu", _i), ("err_offset", _sz)]
+
+
 # name -> (restype, argtypes); pointers are passed as integers (c_void_p).
 _SIGNATURES = {
     "koo_last_error": (ctypes.c_char_p, []),
@@ -134,6 +146,9 @@ _SIGNATURES = {
     "bermudan_portfolio_prepare": (_i, [_i, _p, _p, _p]),
     "bermudan_portfolio_fwd": (_i, [_p, _p, _p]),
     "bermudan_portfolio_bwd": (_i, [_p, _p, _p]),
+    "bermudan_portfolio_persistent_bytes": (_sz, [_i, _p]),
+    "bermudan_portfolio_persistent_prepare": (_i, [_i, _p, _p, _p, _p, _p]),
+    "bermudan_portfolio_persistent_run": (_i, [_p, _p, _p]),
     "ls_workspace": (_i, [_i, _p]),
     "ls_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
     "ls_first_pass_noalloc_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 5),
@@ -756,6 +771,34 @@ def bermudan_portfolio_fwd(plan, descriptors, stream=None):
 
 def bermudan_portfolio_bwd(plan, descriptors, stream=None):
     call("bermudan_portfolio_bwd", ctypes.addressof(plan), ptr(descriptors), stream_handle(stream))
+
+
+def bermudan_portfolio_persistent_bytes(paths):
+    """Device buffer bytes of the persistent portfolio for products of the
+    given path counts."""
+    arr = np.ascontiguousarray(paths, np.int32)
+    return lib().bermudan_portfolio_persistent_bytes(len(arr), arr.ctypes.data)
+
+
+def bermudan_portfolio_persistent_prepare(gen, ls, adj, buffer):
+    """gen: list of GenProduct or None (variates given); ls: list of
+    LsProduct; adj: list of BermProduct (the same products in the same
+    order); buffer: a device buffer of bermudan_portfolio_persistent_bytes.
+    Returns the plan (host)."""
+    n = len(adj)
+    garr = (GenProduct * n)(*gen) if gen is not None else None
+    larr = (LsProduct * n)(*ls)
+    aarr = (BermProduct * n)(*adj)
+    plan = PersistentPlan()
+    call("bermudan_portfolio_persistent_prepare", n,
+         ctypes.addressof(garr) if garr is not None else None, ctypes.addressof(larr),
+         ctypes.addressof(aarr), ptr(buffer), ctypes.addressof(plan))
+    return plan
+
+
+def bermudan_portfolio_persistent_run(plan, buffer, stream=None):
+    call("bermudan_portfolio_persistent_run", ctypes.addressof(plan), ptr(buffer),
+         stream_handle(stream))
 
 
 def bermudan_frozen_bwd(paths, rates, factors, k, variates, f0, taus, disp, A, vols, strike,

@@ -162,6 +162,13 @@ class BermudanPricer:
             self.frozen_greeks = torch.zeros(2 * N, dtype=torch.float64, device=dev)
         return self.adj
 
+    def gen_product(self):
+        """This product's koo.GenProduct (the fused generator's inputs, as
+        generate())."""
+        w = self.w
+        return koo.GenProduct(w.path_offset, w.dims, self.dirs.data_ptr(), self.scr.data_ptr(),
+                              self.alloc.data_ptr())
+
     def ls_batch_product(self, ls_blocks=0):
         """This product's koo.LsProduct (path quantities and LS first pass,
         as paths_kernel() + regress()); ls_blocks as koo_ls_product."""
@@ -257,14 +264,18 @@ class BasketPricer:
 
 class PortfolioPricer:
     """A portfolio of Bermudan swaptions priced with Longstaff-Schwartz and
-    frozen-strategy adjoint Greeks, every product's whole sequence (generator,
-    path quantities, first pass, frozen forward and reverse sweep) issued on
-    one of ``n_streams`` streams and, with ``capture``, recorded once into a
-    single HIP graph that ``run`` replays.  ``checkpoint_every`` maps a
-    product index to its checkpoint interval (the autotuner's choice) or is
-    one interval for all."""
+    frozen-strategy adjoint Greeks (generator, path quantities, first pass,
+    frozen forward and reverse sweep per product).  ``persistent`` (the
+    default): the whole portfolio in one persistent launch
+    (bermudan_portfolio_persistent_run); else ``batched``: the generators on
+    ``n_streams`` streams, then batched launches per stage; else each
+    product's sequence on its stream.  With ``capture`` the launches are
+    recorded once into a HIP graph that ``run`` replays.
+    ``checkpoint_every`` maps a product index to its checkpoint interval
+    (the autotuner's choice) or is one interval for all."""
 
-    def __init__(self, products, checkpoint_every=4, n_streams=4, device="cuda", batched=True):
+    def __init__(self, products, checkpoint_every=4, n_streams=4, device="cuda", batched=True,
+                 persistent=True):
         self.products = products
         self.pricers = [BermudanPricer(w, device=device) for w in products]
         self.k = [checkpoint_every(j) if callable(checkpoint_every) else checkpoint_every
@@ -280,6 +291,8 @@ class PortfolioPricer:
         self.batched = batched and all(w.factors <= 5 and w.n_rates <= 45 for w in products)
         self.ls_blocks = [0] * len(products)  # LS tile per product (0: the library's share)
         self.plan = None
+        self.persistent = persistent and self.batched
+        self.pplan = None
 
     def apply_tuning(self, tuning):
         """Checkpoint interval, stream and LS tile per product from
@@ -289,6 +302,15 @@ class PortfolioPricer:
         self.ls_blocks = list(tuning.get("ls_blocks", [0] * len(self.pricers)))
         self.graph = None
         self.plan = None
+        self.pplan = None
+
+    def _prepare_persistent(self):
+        gen = [pr.gen_product() for pr in self.pricers]
+        ls = [pr.ls_batch_product() for pr in self.pricers]
+        adj = [pr.batch_product(self.k[j]) for j, pr in enumerate(self.pricers)]
+        nbytes = koo.bermudan_portfolio_persistent_bytes([w.paths for w in self.products])
+        self.pbuf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        self.pplan = koo.bermudan_portfolio_persistent_prepare(gen, ls, adj, self.pbuf)
 
     def _prepare_batch(self):
         n = len(self.pricers)
@@ -302,6 +324,11 @@ class PortfolioPricer:
         self.plan = koo.bermudan_portfolio_prepare(items, self.descriptors)
 
     def _issue(self):
+        if self.persistent:
+            if self.pplan is None:
+                self._prepare_persistent()
+            koo.bermudan_portfolio_persistent_run(self.pplan, self.pbuf)
+            return
         if self.batched and self.plan is None:
             self._prepare_batch()
         main = torch.cuda.current_stream()
