@@ -810,8 +810,12 @@ int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan, const void* descri
  * overwritten), with factors <= 5, rates <= 45, dates <= 40.  Outputs per
  * product: variates (with gen), basis variables, exercise values,
  * numeraires, coefficients, first-pass price, frozen price and 2N Greeks.
- * prepare() sizes the teams and writes the descriptors into device_buffer
- * (bermudan_portfolio_persistent_bytes bytes) with synchronous copies; run()
+ * prepare() sizes the teams in proportion to each product's work (work[i],
+ * any positive unit, e.g. block-time measured by an earlier run; NULL: a
+ * model of rate updates per phase) and writes the descriptors into
+ * device_buffer (bermudan_portfolio_persistent_bytes bytes) with
+ * synchronous copies; after a run the buffer holds each block's run time in
+ * 100 MHz ticks at ticks_offset (uint64 per block, grid entries); run()
  * launches three kernels (arming, error word, the portfolio) and neither
  * synchronises nor allocates, so it can be captured in a graph.  The
  * frozen Greeks equal bermudan_frozen_fwd/_bwd on the same variates and
@@ -829,13 +833,15 @@ typedef struct koo_gen_product {
 
 typedef struct koo_persistent_plan {
   int products, grid, lds_bytes, tab_offset, blocks_per_cu;
-  size_t err_offset;
+  size_t err_offset, ticks_offset;
+  int team_first[512], team_blocks[512]; /* per product, in the order given */
 } koo_persistent_plan;
 
 size_t bermudan_portfolio_persistent_bytes(int n_products, const int* paths);
 int bermudan_portfolio_persistent_prepare(int n_products, const koo_gen_product* gen,
                                           const koo_ls_product* ls, const koo_berm_product* adj,
-                                          void* device_buffer, koo_persistent_plan* plan);
+                                          const double* work, void* device_buffer,
+                                          koo_persistent_plan* plan);
 int bermudan_portfolio_persistent_run(const koo_persistent_plan* plan, void* device_buffer,
                                       void* stream);
 

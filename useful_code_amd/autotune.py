@@ -14,9 +14,17 @@ batched regression launches, a share of 1, 2, 4 or 8 blocks per CU over the
 portfolio -- is timed on the batched path-quantity and regression launches;
 and the generators are scheduled over the streams by LPT.
 
-The result -- k, stream and LS tile per product and the measured times -- is a plain
-dict that PortfolioPricer takes and that can be stored as JSON next to the
-results it produced.
+Persistent launch (the whole portfolio in one kernel, a team of resident
+blocks per product): with the batched mode's interval, the teams are sized
+by the library's work model, the launch is timed, and the teams are resized
+in proportion to each product's measured block-time (the kernel records
+every block's run time) for a few rounds, keeping the fastest sizing; then
+the persistent launch and the batched launches are compared and the faster
+one is what the pricer runs.
+
+The result -- k, stream and LS tile per product, the team weights, the
+launch form and the measured times -- is a plain dict that PortfolioPricer
+takes and that can be stored as JSON next to the results it produced.
 """
 import json
 
@@ -106,12 +114,47 @@ def tune_batched(pricer, memory_budget_bytes=64 << 30, n_streams=None):
             "stream_load_ms": load, "batched": True}
 
 
+def tune_persistent(pricer, rounds=3):
+    """Team weights of the persistent launch: the model's first, then each
+    round the measured block-time per product of the previous run; returns
+    (best ms, its weights, [ms per round])."""
+    pricer.work = None
+    best, best_w, hist = None, None, []
+    for _ in range(rounds):
+        pricer.pplan = None
+        pricer.graph = None
+        ms = _time(pricer._issue, reps=2)
+        hist.append(ms)
+        if best is None or ms < best:
+            best, best_w = ms, pricer.work
+        pricer.work = pricer.measured_work()
+    pricer.work = best_w
+    pricer.pplan = None
+    return best, best_w, hist
+
+
 def tune_portfolio(pricer, memory_budget_bytes=64 << 30, n_streams=None):
     """pricer: a PortfolioPricer (its buffers are reused).  Returns
     {"k": [...], "stream": [...], "ms": [...], "k_ms": [{k: ms}, ...]}.
-    A batched pricer is tuned by tune_batched."""
+    A batched pricer is tuned by tune_batched, and a persistent one also by
+    tune_persistent, which decides between the two launch forms."""
     if getattr(pricer, "batched", False):
-        return tune_batched(pricer, memory_budget_bytes, n_streams)
+        persistent = getattr(pricer, "persistent", False)
+        pricer.persistent = False
+        res = tune_batched(pricer, memory_budget_bytes, n_streams)
+        if persistent:
+            pricer.k = list(res["k"])
+            pricer.ls_blocks = list(res["ls_blocks"])
+            pricer.stream_of = list(res["stream"])
+            pricer.plan = None
+            batched_ms = _time(pricer._issue, reps=2)
+            pricer.persistent = True
+            ms, work, hist = tune_persistent(pricer)
+            res.update({"persistent": ms < batched_ms, "work": work,
+                        "persistent_ms": ms, "persistent_rounds_ms": hist,
+                        "batched_ms": batched_ms})
+        pricer.persistent = persistent
+        return res
     n_streams = n_streams or len(pricer.streams)
     ks, k_ms = [], []
     budget_each = memory_budget_bytes / max(1, len(pricer.pricers))

@@ -12,7 +12,9 @@ namespace {
 using namespace lmm;
 
 constexpr int kMaxBermDates = 40;
-constexpr int kVegaSlots = 8;
+// Vega partial sums per rate and wavefront in LDS: the 64 lanes folded to 4
+// (one per 16-lane row) before the LDS adds.
+constexpr int kVegaSlots = 4;
 constexpr int kStatePrefetch = 8;
 
 struct BermArgs {
@@ -149,7 +151,7 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
   float* lam = lds + threadIdx.x;
   float* ctab = lds + static_cast<size_t>(N) * bd;
   float* vslots = ctab + step_table_floats(N);
-  float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 3);
+  float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 4);
   fill_step_table<F>(m, ctab);
   for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
   for (int i = threadIdx.x; i < nw * N * kVegaSlots; i += bd) vslots[i] = 0.0f;
@@ -217,7 +219,8 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
             w += dpp_f32_quad_xor1(w);
             w += dpp_f32_quad_xor2(w);
             w += dpp_f32_row_half_mirror(w);
-            if ((lane & 7) == 0) atomicAdd(myslots + r * kVegaSlots, w);
+            w += dpp_f32_row_mirror(w);
+            if ((lane & 15) == 0) atomicAdd(myslots + r * kVegaSlots, w);
           });
     }
   }

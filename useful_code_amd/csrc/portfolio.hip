@@ -21,6 +21,7 @@
 // ls_device.h, berm_device.h), so given the same variates and coefficients
 // the forward and backward are bit for bit those of bermudan_frozen_fwd/bwd.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "berm_device.h"
@@ -139,11 +140,14 @@ constexpr int kPfLsDoubles = (kPfThreads / 64) * 256 + kTriMax + kMaxGram * kMax
 
 // The whole portfolio.  Dynamic LDS: [0, tab_off) floats for the phase in
 // progress, then the product's frozen coefficients as FP32 (E x 6), filled
-// by the LS rounds and read by the forward.  err[0] = 1 and NaN outputs if a
-// wait gave up.
+// by the LS rounds and read by the forward.  ticks[block]: the block's run
+// time in 100 MHz ticks (what the host rebalances the teams by).  err[0] = 1
+// and NaN outputs if a wait gave up.
 __global__ void __launch_bounds__(kPfThreads, 3)
-    k_portfolio(const PfItem* __restrict__ items, int n, int tab_off, unsigned* __restrict__ err) {
+    k_portfolio(const PfItem* __restrict__ items, int n, int tab_off, unsigned* __restrict__ err,
+                unsigned long long* __restrict__ ticks) {
   extern __shared__ float lds[];
+  const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
   __shared__ int flag;
   const PfItem& it = items[find_block_owner(items, n, static_cast<int>(blockIdx.x), &PfItem::team0)];
   const int t = static_cast<int>(blockIdx.x) - it.team0;
@@ -279,9 +283,11 @@ __global__ void __launch_bounds__(kPfThreads, 3)
       finalize_col<true>(gmem(it.pbwd), it.nvb, 2 * it.N, c, scale, gmem(it.greeks));
     }
   }
+  if (threadIdx.x == 0) ticks[blockIdx.x] = __builtin_amdgcn_s_memrealtime() - t_start;
   return;
 fail:
   if (threadIdx.x == 0) {
+    ticks[blockIdx.x] = __builtin_amdgcn_s_memrealtime() - t_start;
     err[0] = 1u;
     *gmem(it.price) = __builtin_nan("");
     *gmem(it.price1) = __builtin_nan("");
@@ -306,8 +312,10 @@ __global__ void __launch_bounds__(256) k_portfolio_arm(const PfItem* __restrict_
 
 // Device buffer: the items, the bridge tables, the forward price partials
 // (one column per product, nvb doubles each), the error word.
+constexpr int kPfMaxGrid = 4096;
+
 struct PfLayout {
-  size_t items, tables, pfwd, err, total;
+  size_t items, tables, pfwd, ticks, err, total;
 };
 
 PfLayout pf_layout(int n, size_t pfwd_doubles) {
@@ -316,7 +324,8 @@ PfLayout pf_layout(int n, size_t pfwd_doubles) {
   l.items = 0;
   l.tables = al(sizeof(PfItem) * n);
   l.pfwd = l.tables + al(sizeof(BridgeTables) * n);
-  l.err = l.pfwd + al(sizeof(double) * pfwd_doubles);
+  l.ticks = l.pfwd + al(sizeof(double) * pfwd_doubles);
+  l.err = l.ticks + al(sizeof(unsigned long long) * kPfMaxGrid);
   l.total = l.err + 256;
   return l;
 }
@@ -365,11 +374,12 @@ extern "C" size_t bermudan_portfolio_persistent_bytes(int n_products, const int*
 extern "C" int bermudan_portfolio_persistent_prepare(int n_products, const koo_gen_product* gen,
                                                      const koo_ls_product* ls,
                                                      const koo_berm_product* adj,
-                                                     void* device_buffer,
+                                                     const double* work, void* device_buffer,
                                                      koo_persistent_plan* plan) {
   const char* who = "bermudan_portfolio_persistent_prepare";
-  if (n_products < 1 || !ls || !adj || !device_buffer || !plan)
-    return set_error(KOO_ERR_INVALID, "%s: bad arguments", who);
+  constexpr int kMaxProducts = sizeof(plan->team_first) / sizeof(plan->team_first[0]);
+  if (n_products < 1 || n_products > kMaxProducts || !ls || !adj || !device_buffer || !plan)
+    return set_error(KOO_ERR_INVALID, "%s: bad arguments (1 to %d products)", who, kMaxProducts);
   std::vector<PfItem> items(n_products);
   std::vector<BridgeTables> tables(n_products);
   std::vector<double> cost(n_products);
@@ -466,7 +476,9 @@ extern "C" int bermudan_portfolio_persistent_prepare(int n_products, const koo_g
     pf_total += it.nvb;
     lds_main = std::max(lds_main, pf_phase_lds(q.rates));
     max_e = std::max(max_e, q.n_ex);
-    cost[i] = pf_cost(q, it.ex.last, it.gen);
+    cost[i] = work ? work[i] : pf_cost(q, it.ex.last, it.gen);
+    if (!(cost[i] > 0.0))
+      return set_error(KOO_ERR_INVALID, "%s: product %d: work must be positive", who, i);
   }
   // dynamic LDS: the phases, then the FP32 coefficient table
   const size_t lds = lds_main + sizeof(float) * 6 * max_e;
@@ -475,7 +487,25 @@ extern "C" int bermudan_portfolio_persistent_prepare(int n_products, const koo_g
           hipSuccess ||
       per_cu < 1)
     return set_error(KOO_ERR_INVALID, "%s: no block of %zu bytes of LDS can be resident", who, lds);
-  const int R = per_cu * device_cu_count();
+  // every block must be resident at once; bound the occupancy figure by the
+  // residency measured on gfx950 for 256-lane blocks (LDS per block incl.
+  // static: <= 40 KiB: 4 per CU, <= 53 120 B: 3, <= 80 KiB: 2), since the
+  // query has allowed a block more than fits just below the 3-per-CU limit
+  {
+    hipFuncAttributes fa{};
+    size_t total = lds;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_portfolio)) == hipSuccess)
+      total += fa.sharedSizeBytes;
+    const int by_lds = total <= 40960 ? 4 : (total <= 53120 ? 3 : (total <= 81920 ? 2 : 1));
+    if (by_lds < per_cu) per_cu = by_lds;
+  }
+  // KOO_PF_BLOCKS_PER_CU caps the blocks per CU below the occupancy figure
+  // (all blocks must be resident at once: the teams' waits rely on it)
+  if (const char* cap_env = getenv("KOO_PF_BLOCKS_PER_CU")) {
+    const int c = atoi(cap_env);
+    if (c >= 1 && c < per_cu) per_cu = c;
+  }
+  const int R = std::min(per_cu * device_cu_count(), kPfMaxGrid);
   if (n_products > R)
     return set_error(KOO_ERR_INVALID, "%s: %d products exceed the %d resident blocks", who,
                      n_products, R);
@@ -537,6 +567,11 @@ extern "C" int bermudan_portfolio_persistent_prepare(int n_products, const koo_g
   plan->tab_offset = static_cast<int>(lds_main / sizeof(float));
   plan->blocks_per_cu = per_cu;
   plan->err_offset = lay.err;
+  plan->ticks_offset = lay.ticks;
+  for (int i = 0; i < n_products; ++i) {
+    plan->team_first[i] = items[i].team0;
+    plan->team_blocks[i] = items[i].team;
+  }
   return KOO_OK;
 }
 
@@ -554,7 +589,8 @@ extern "C" int bermudan_portfolio_persistent_run(const koo_persistent_plan* plan
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_portfolio_arm, dim3(plan->products), dim3(256), 0, st, items);
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, err, 1);
+  unsigned long long* ticks = reinterpret_cast<unsigned long long*>(buf + plan->ticks_offset);
   hipLaunchKernelGGL(k_portfolio, dim3(plan->grid), dim3(kPfThreads), plan->lds_bytes, st, items,
-                     plan->products, plan->tab_offset, err);
+                     plan->products, plan->tab_offset, err, ticks);
   return check_launch(who);
 }

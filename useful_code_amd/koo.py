@@ -64,7 +64,8 @@ class GenProduct(ctypes.Structure):
 class PersistentPlan(ctypes.Structure):
     """koo_persistent_plan (include/koo_c_abi.h)."""
     _fields_ = [("products", _i), ("grid", _i), ("lds_bytes", _i), ("tab_offset", _i),
-                ("blocks_per_cu", _i), ("err_offset", _sz)]
+                ("blocks_per_cu", _i), ("err_offset", _sz), ("ticks_offset", _sz),
+                ("team_first", _i * 512), ("team_blocks", _i * 512)]
 
 
 # name -> (restype, argtypes); pointers are passed as integers (c_void_p).
@@ -147,7 +148,7 @@ _SIGNATURES = {
     "bermudan_portfolio_fwd": (_i, [_p, _p, _p]),
     "bermudan_portfolio_bwd": (_i, [_p, _p, _p]),
     "bermudan_portfolio_persistent_bytes": (_sz, [_i, _p]),
-    "bermudan_portfolio_persistent_prepare": (_i, [_i, _p, _p, _p, _p, _p]),
+    "bermudan_portfolio_persistent_prepare": (_i, [_i, _p, _p, _p, _p, _p, _p]),
     "bermudan_portfolio_persistent_run": (_i, [_p, _p, _p]),
     "ls_workspace": (_i, [_i, _p]),
     "ls_first_pass_gpu": (_i, [_i, _p, _p, _i, _p, _p, _p, _i, _i, _i] + [_p] * 6),
@@ -780,20 +781,31 @@ def bermudan_portfolio_persistent_bytes(paths):
     return lib().bermudan_portfolio_persistent_bytes(len(arr), arr.ctypes.data)
 
 
-def bermudan_portfolio_persistent_prepare(gen, ls, adj, buffer):
+def bermudan_portfolio_persistent_prepare(gen, ls, adj, buffer, work=None):
     """gen: list of GenProduct or None (variates given); ls: list of
     LsProduct; adj: list of BermProduct (the same products in the same
-    order); buffer: a device buffer of bermudan_portfolio_persistent_bytes.
-    Returns the plan (host)."""
+    order); buffer: a device buffer of bermudan_portfolio_persistent_bytes;
+    work: per-product work weights for the team sizes (None: the library's
+    model).  Returns the plan (host)."""
     n = len(adj)
     garr = (GenProduct * n)(*gen) if gen is not None else None
     larr = (LsProduct * n)(*ls)
     aarr = (BermProduct * n)(*adj)
+    w = None if work is None else np.ascontiguousarray(work, np.float64)
     plan = PersistentPlan()
     call("bermudan_portfolio_persistent_prepare", n,
          ctypes.addressof(garr) if garr is not None else None, ctypes.addressof(larr),
-         ctypes.addressof(aarr), ptr(buffer), ctypes.addressof(plan))
+         ctypes.addressof(aarr), None if w is None else w.ctypes.data, ptr(buffer),
+         ctypes.addressof(plan))
     return plan
+
+
+def persistent_block_ticks(plan, buffer):
+    """Per-block run times (100 MHz ticks) of the last persistent run, as a
+    host array of plan.grid entries (synchronises)."""
+    import torch
+    t = buffer[plan.ticks_offset:plan.ticks_offset + 8 * plan.grid].cpu().numpy()
+    return t.view(np.uint64).copy()
 
 
 def bermudan_portfolio_persistent_run(plan, buffer, stream=None):

@@ -293,13 +293,18 @@ class PortfolioPricer:
         self.plan = None
         self.persistent = persistent and self.batched
         self.pplan = None
+        self.work = None  # team work weights of the persistent launch (None: the model)
 
     def apply_tuning(self, tuning):
-        """Checkpoint interval, stream and LS tile per product from
+        """Checkpoint interval, stream and LS tile per product, and the
+        persistent launch's team weights and launch form, from
         autotune.tune_portfolio."""
         self.k = list(tuning["k"])
         self.stream_of = list(tuning["stream"])
         self.ls_blocks = list(tuning.get("ls_blocks", [0] * len(self.pricers)))
+        if "persistent" in tuning:
+            self.persistent = bool(tuning["persistent"]) and self.batched
+            self.work = tuning.get("work")
         self.graph = None
         self.plan = None
         self.pplan = None
@@ -310,7 +315,8 @@ class PortfolioPricer:
         adj = [pr.batch_product(self.k[j]) for j, pr in enumerate(self.pricers)]
         nbytes = koo.bermudan_portfolio_persistent_bytes([w.paths for w in self.products])
         self.pbuf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
-        self.pplan = koo.bermudan_portfolio_persistent_prepare(gen, ls, adj, self.pbuf)
+        self.pplan = koo.bermudan_portfolio_persistent_prepare(gen, ls, adj, self.pbuf,
+                                                               self.work)
 
     def _prepare_batch(self):
         n = len(self.pricers)
@@ -364,6 +370,15 @@ class PortfolioPricer:
             with torch.cuda.graph(self.graph):
                 self._issue()
         self.graph.replay()
+
+    def measured_work(self):
+        """Per-product block-time of the last persistent run (the sum of its
+        team's block run times): the weights that size the teams to the
+        work as it ran."""
+        t = koo.persistent_block_ticks(self.pplan, self.pbuf).astype(np.float64)
+        return [float(t[f:f + b].sum()) for f, b in
+                zip(self.pplan.team_first[:len(self.pricers)],
+                    self.pplan.team_blocks[:len(self.pricers)])]
 
     def results(self):
         """[(first-pass price, frozen-strategy price, greeks (2N))] per product."""
