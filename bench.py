@@ -281,12 +281,14 @@ def multi_ls_section(args, torch, O, configs, pipeline, ev):
 
 def portfolio_section(args, torch, configs, pipeline, ev):
     """Config 5: 64 Bermudan swaptions (10-40 rates, 4-20 exercise dates,
-    1-5 factors, 2^17 paths each): the generators over four streams, then the
-    batched launches -- path quantities (one per rate class), LS first passes
-    (one per date rank), frozen forward and adjoint backward (one per rate
-    class each) -- all captured into one HIP graph with the autotuner's
-    checkpoint interval and stream schedule; one step = one graph replay
-    (first-pass price, frozen price and 2N Greeks for every product)."""
+    1-5 factors, 2^17 paths each), in whichever launch form the autotuner
+    measured faster: the whole portfolio in one persistent launch (teams of
+    resident blocks per product), or the generators over four streams then
+    the batched launches (path quantities per rate class, the LS first
+    passes in one persistent launch, frozen forward and adjoint backward per
+    rate class) -- captured into one HIP graph with the autotuner's
+    checkpoint interval; one step = one graph replay (first-pass price,
+    frozen price and 2N Greeks for every product)."""
     from useful_code_amd import autotune
     prods = configs.bermudan_portfolio(paths=args.portfolio_paths)
     pf = pipeline.PortfolioPricer(prods, checkpoint_every=4, n_streams=4)
@@ -315,7 +317,7 @@ def portfolio_section(args, torch, configs, pipeline, ev):
         "workload": "bermudan_portfolio_64: 64 Bermudan payer swaptions, 10-40 rates, 4-20 "
                     "exercise dates, K 0.03-0.07, 1-5 factors, 2^17 Sobol paths each; LS first pass "
                     "(ITM, MFMA normal equations, FP64 Cholesky) + frozen-strategy adjoint deltas "
-                    "and vegas per product; batched launches in one graph",
+                    "and vegas per product; the autotuner's launch form, captured in one graph",
         "graph_replay_ms": ms,
         "eager_ms": eager_ms,
         "path_steps_per_s": path_steps / (ms * 1e-3),
@@ -324,6 +326,11 @@ def portfolio_section(args, torch, configs, pipeline, ev):
         "checkpoint_every": tuning["k"],
         "ls_blocks_per_product": tuning.get("ls_blocks"),
         "ls_blocks_per_cu_ms": tuning.get("ls_blocks_per_cu_ms"),
+        "launch": "persistent" if pf.persistent else "batched",
+        "launch_choice_ms": {"persistent (one launch, teams sized by measured work)":
+                             tuning.get("persistent_ms"),
+                             "batched (per rate class and stage)": tuning.get("batched_ms"),
+                             "persistent team-sizing rounds": tuning.get("persistent_rounds_ms")},
         "stream_load_ms": tuning["stream_load_ms"],
         "frozen_price_sum": float(sum(r[1] for r in res)),
         "batched": bool(pf.batched),
