@@ -197,3 +197,27 @@ def test_multi_first_pass_is_deterministic(torch_cuda):
     assert a[0] == b[0]
     for x, y in zip(a[1:], b[1:]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("P,sel", [(20000, 0), (1 << 18, 0), (20000, 1)])
+def test_multi_persistent_pass_equals_per_phase_launches(torch_cuda, P, sel):
+    """The pass as one persistent launch (KOO_LS_MULTI_PERSISTENT=1: grid
+    barriers between the phases) and as one launch per phase (the default)
+    run the same bodies over the same blocks, so
+    every output agrees bit for bit; a case whose search bisects and whose
+    deeper depths are skipped included (the persistent pass skips unreached
+    depths and unneeded bisection stages)."""
+    c = f32(M.ls_multi_case(23, E=8, P=P, max_vars=2, last_vars=1))
+    selector = M.SELECTOR if sel == 0 else (0.25, 0.26, 1.5, 0.8)
+    runs = []
+    for mode in ("1", "0"):
+        os.environ["KOO_LS_MULTI_PERSISTENT"] = mode
+        try:
+            runs.append(run_gpu(torch_cuda, c, 1, 3, 1, selector=selector, kind=sel))
+        finally:
+            os.environ.pop("KOO_LS_MULTI_PERSISTENT", None)
+    a, b = runs
+    print(f"persistent price {a[0]:.15f} per phase {b[0]:.15f}", flush=True)
+    assert np.isfinite(a[0]) and a[0] == b[0]
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)

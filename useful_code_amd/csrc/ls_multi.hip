@@ -36,6 +36,8 @@
 // takes the same steps as the FP64 gold: counts are exact and the net
 // values agree to rounding.  Depths past the last active one return at once;
 // the host launches every depth, so no result is read back during the pass.
+#include <cstdlib>
+
 #include "ls_common.h"
 
 namespace koo {
@@ -60,6 +62,20 @@ struct MultiArgs {
   int vn, kn;  // at date e + 1 (when e + 1 < E)
   double lower_frac, upper_frac, sd_guess, multiplier;
 };
+
+// Values that one block writes and others read within a pass (partials,
+// the search state, the outputs the later dates' decisions read, the keys)
+// go through agent-coherent loads and stores, so the persistent pass needs
+// no cache write-back or invalidation between its phases, only that each
+// wave's stores are performed before its block counts itself in.
+template <class T>
+__device__ __forceinline__ T ld_ag(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_ag(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Per depth of the current date: whether the depth is reached, its member
 // count, and the key bound of the band of depth - 1 that defines its
@@ -130,12 +146,12 @@ __device__ __forceinline__ void load_decision(const MultiArgs& a, int q,
                                               const double* __restrict__ norm, double* beta,
                                               double* nrm, double* lo, double* hi) {
   for (int t = threadIdx.x; t < a.R * a.maxd; t += blockDim.x)
-    beta[t] = coeffs[static_cast<size_t>(q) * a.R * a.maxd + t];
+    beta[t] = ld_ag(coeffs + static_cast<size_t>(q) * a.R * a.maxd + t);
   for (int t = threadIdx.x; t < a.R * 2 * a.max_vars; t += blockDim.x)
-    nrm[t] = norm[norm_at(q, a.R, 0, 0, a.max_vars) + t];
+    nrm[t] = ld_ag(norm + norm_at(q, a.R, 0, 0, a.max_vars) + t);
   for (int t = threadIdx.x; t < a.R; t += blockDim.x) {
-    lo[t] = lower[static_cast<size_t>(q) * a.R + t];
-    hi[t] = upper[static_cast<size_t>(q) * a.R + t];
+    lo[t] = ld_ag(lower + static_cast<size_t>(q) * a.R + t);
+    hi[t] = ld_ag(upper + static_cast<size_t>(q) * a.R + t);
   }
   __syncthreads();
 }
@@ -153,21 +169,20 @@ __device__ __forceinline__ void block_sums(const double* vals, int n, double* re
   if (threadIdx.x < n) {
     double s = 0.0;
     for (int w = 0; w < nw; ++w) s += red[w * 8 + threadIdx.x];
-    part[static_cast<size_t>(threadIdx.x) * nblocks + block] = s;
+    st_ag(part + static_cast<size_t>(threadIdx.x) * nblocks + block, s);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Membership, decision and variable sums.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kMultiThreads)
-    k_multi_prep(const MultiArgs a, int depth, const float* __restrict__ bv,
+__device__ __forceinline__ void multi_prep_body(const MultiArgs a, int depth, const float* __restrict__ bv,
                  const float* __restrict__ cash, const float* __restrict__ ev,
                  const float* __restrict__ num, const double* __restrict__ coeffs,
                  const double* __restrict__ lower, const double* __restrict__ upper,
                  const double* __restrict__ norm, double* __restrict__ Y,
                  unsigned char* __restrict__ mem, const unsigned long long* __restrict__ keys,
-                 DepthState* __restrict__ state, double* __restrict__ stats_part) {
+                 DepthState* __restrict__ state, double* __restrict__ stats_part, int blk, int nblk) {
   __shared__ double beta[kMaxDepth * kMaxGram];
   __shared__ double nrm[kMaxDepth * 2 * kMaxBasisVars];
   __shared__ double lo[kMaxDepth], hi[kMaxDepth];
@@ -175,20 +190,20 @@ __global__ void __launch_bounds__(kMultiThreads)
   const size_t P = static_cast<size_t>(a.P);
   unsigned long long cutbits = 0;
   if (depth == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      state[0].active = 1;
-      state[0].n_pre = a.P;
-      state[0].cutbits = kKeySentinel;
+    if (blk == 0 && threadIdx.x == 0) {
+      st_ag(&state[0].active, 1);
+      st_ag(&state[0].n_pre, static_cast<long long>(a.P));
+      st_ag(&state[0].cutbits, kKeySentinel);
     }
     if (a.e + 1 < a.E) load_decision(a, a.e + 1, coeffs, lower, upper, norm, beta, nrm, lo, hi);
   } else {
-    if (!state[depth].active) return;  // uniform over the grid
-    cutbits = state[depth].cutbits;
+    if (!ld_ag(&state[depth].active)) return;  // uniform over the grid
+    cutbits = ld_ag(&state[depth].cutbits);
   }
   double sums[2 * kMaxBasisVars];
 #pragma unroll
   for (int i = 0; i < 2 * kMaxBasisVars; ++i) sums[i] = 0.0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
+  for (int p = blk * blockDim.x + threadIdx.x; p < a.P; p += nblk * blockDim.x) {
     bool member;
     if (depth == 0) {
       double y = 0.0;
@@ -201,7 +216,7 @@ __global__ void __launch_bounds__(kMultiThreads)
       Y[p] = y;
       member = true;
     } else {
-      member = keys[p] <= cutbits;
+      member = ld_ag(keys + p) <= cutbits;
     }
     mem[p] = member ? 1 : 0;
     if (a.normalise && member) {
@@ -216,8 +231,20 @@ __global__ void __launch_bounds__(kMultiThreads)
     }
   }
   if (a.normalise && a.v > 0)
-    block_sums(sums, 2 * a.v, red, stats_part, blockIdx.x, gridDim.x);
+    block_sums(sums, 2 * a.v, red, stats_part, blk, nblk);
 }
+
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_prep(const MultiArgs a, int depth, const float* __restrict__ bv,
+                 const float* __restrict__ cash, const float* __restrict__ ev,
+                 const float* __restrict__ num, const double* __restrict__ coeffs,
+                 const double* __restrict__ lower, const double* __restrict__ upper,
+                 const double* __restrict__ norm, double* __restrict__ Y,
+                 unsigned char* __restrict__ mem, const unsigned long long* __restrict__ keys,
+                 DepthState* __restrict__ state, double* __restrict__ stats_part) {
+  multi_prep_body(a, depth, bv, cash, ev, num, coeffs, lower, upper, norm, Y, mem, keys, state, stats_part, blockIdx.x, gridDim.x);
+}
+
 
 // ---------------------------------------------------------------------------
 // Means and standard deviations; FP64 MFMA Gram of basis and target.
@@ -246,24 +273,23 @@ __device__ __forceinline__ void block_gram_f64(const doublex4& acc, int n, doubl
       s += dw[i * 16 + j];
       if (GROUPS == 2) s += dw[(8 + i) * 16 + 8 + j];
     }
-    part[static_cast<size_t>(t) * nblocks + block] = s;
+    st_ag(part + static_cast<size_t>(t) * nblocks + block, s);
   }
 }
 
 template <int GROUPS>
-__global__ void __launch_bounds__(kMultiThreads)
-    k_multi_gram(const MultiArgs a, int depth, const float* __restrict__ bv,
+__device__ __forceinline__ void multi_gram_body(const MultiArgs a, int depth, const float* __restrict__ bv,
                  const float* __restrict__ cash, const double* __restrict__ Y,
                  const unsigned char* __restrict__ mem, const DepthState* __restrict__ state,
                  const double* __restrict__ stats_part, double* __restrict__ norm,
                  double* __restrict__ means_out, double* __restrict__ sds_out,
-                 double* __restrict__ gram_part) {
+                 double* __restrict__ gram_part, int blk, int nblk) {
   __shared__ double lds_d[(kMultiThreads / 64) * 256];
   __shared__ double sums[2 * kMaxBasisVars];
   __shared__ double msd[2 * kMaxBasisVars];
-  if (!state[depth].active) return;
-  const double n_pre = static_cast<double>(state[depth].n_pre);
-  if (a.normalise && a.v > 0) reduce_partials(stats_part, gridDim.x, 2 * a.v, sums);
+  if (!ld_ag(&state[depth].active)) return;
+  const double n_pre = static_cast<double>(ld_ag(&state[depth].n_pre));
+  if (a.normalise && a.v > 0) reduce_partials<true>(stats_part, nblk, 2 * a.v, sums);
   if (threadIdx.x < kMaxBasisVars) {
     const int i = threadIdx.x;
     double m = 0.0, sd = 1.0;
@@ -274,12 +300,12 @@ __global__ void __launch_bounds__(kMultiThreads)
     }
     msd[i] = m;
     msd[kMaxBasisVars + i] = sd;
-    if (blockIdx.x == 0 && i < a.v) {
-      norm[norm_at(a.e, a.R, depth, 0, a.max_vars) + i] = m;
-      norm[norm_at(a.e, a.R, depth, 1, a.max_vars) + i] = sd;
+    if (blk == 0 && i < a.v) {
+      st_ag(norm + norm_at(a.e, a.R, depth, 0, a.max_vars) + i, m);
+      st_ag(norm + norm_at(a.e, a.R, depth, 1, a.max_vars) + i, sd);
       const size_t o = (static_cast<size_t>(a.e) * a.R + depth) * a.max_vars + i;
-      if (means_out) means_out[o] = m;
-      if (sds_out) sds_out[o] = sd;
+      if (means_out) st_ag(means_out + o, m);
+      if (sds_out) st_ag(sds_out + o, sd);
     }
   }
   __syncthreads();
@@ -291,8 +317,8 @@ __global__ void __launch_bounds__(kMultiThreads)
   const int j = GROUPS == 2 ? (lane & 7) : (lane & 15);
   const int sub = GROUPS == 2 ? ((lane >> 3) & 1) * 4 + (lane >> 4) : (lane >> 4);
   doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-  const int stride = gridDim.x * nw * 64;
-  for (int base = (blockIdx.x * nw + wave) * 64; base < a.P; base += stride) {
+  const int stride = nblk * nw * 64;
+  for (int base = (blk * nw + wave) * 64; base < a.P; base += stride) {
     const int p = base + lane;
     double x[kMaxBasisVars] = {0.0, 0.0, 0.0, 0.0};
     double t = 0.0, w = 0.0;
@@ -312,8 +338,20 @@ __global__ void __launch_bounds__(kMultiThreads)
       acc = mfma16x16x4_f64(val, val, acc);
     }
   }
-  block_gram_f64<GROUPS>(acc, n, lds_d, gram_part, blockIdx.x, gridDim.x);
+  block_gram_f64<GROUPS>(acc, n, lds_d, gram_part, blk, nblk);
 }
+
+template <int GROUPS>
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_gram(const MultiArgs a, int depth, const float* __restrict__ bv,
+                 const float* __restrict__ cash, const double* __restrict__ Y,
+                 const unsigned char* __restrict__ mem, const DepthState* __restrict__ state,
+                 const double* __restrict__ stats_part, double* __restrict__ norm,
+                 double* __restrict__ means_out, double* __restrict__ sds_out,
+                 double* __restrict__ gram_part) {
+  multi_gram_body<GROUPS>(a, depth, bv, cash, Y, mem, state, stats_part, norm, means_out, sds_out, gram_part, blockIdx.x, gridDim.x);
+}
+
 
 // ---------------------------------------------------------------------------
 // Normal-equation solve and net continuation values.
@@ -381,35 +419,35 @@ __device__ void solve_multi(const double* G, int k, double n, double* beta, doub
   jacobi_truncated(k, A, V, b, kSvdTolerance, beta);
 }
 
-__global__ void __launch_bounds__(kMultiThreads)
-    k_multi_net(const MultiArgs a, int depth, const float* __restrict__ bv,
+__device__ __forceinline__ void multi_net_body(const MultiArgs a, int depth, const float* __restrict__ bv,
                 const float* __restrict__ ev, const unsigned char* __restrict__ mem,
                 const DepthState* __restrict__ state, const double* __restrict__ norm,
                 const double* __restrict__ gram_part, double* __restrict__ coeffs,
-                unsigned long long* __restrict__ keys, double* __restrict__ net_part) {
+                unsigned long long* __restrict__ keys, double* __restrict__ net_part, int blk, int nblk) {
   __shared__ double gram[kMaxGram * kMaxGram];
   __shared__ double A[16 * 16], V[16 * 16], rhs[16], tmp[16];
   __shared__ double beta[kMaxGram];
   __shared__ double msd[2 * kMaxBasisVars];
   __shared__ double red[16 * 8];
-  if (!state[depth].active) return;
+  if (!ld_ag(&state[depth].active)) return;
   const int n = a.k + 1;
-  reduce_partials(gram_part, gridDim.x, n * n, gram);
+  reduce_partials<true>(gram_part, nblk, n * n, gram);
   if (threadIdx.x == 0) {
-    solve_multi(gram, a.k, static_cast<double>(state[depth].n_pre), beta, A, V, rhs, tmp);
-    if (blockIdx.x == 0)
+    solve_multi(gram, a.k, static_cast<double>(ld_ag(&state[depth].n_pre)), beta, A, V, rhs, tmp);
+    if (blk == 0)
       for (int i = 0; i < a.maxd; ++i)
-        coeffs[(static_cast<size_t>(a.e) * a.R + depth) * a.maxd + i] = i < a.k ? beta[i] : 0.0;
+        st_ag(coeffs + (static_cast<size_t>(a.e) * a.R + depth) * a.maxd + i,
+              i < a.k ? beta[i] : 0.0);
   }
   if (threadIdx.x < 2 * kMaxBasisVars) {
     const int which = threadIdx.x / kMaxBasisVars, i = threadIdx.x % kMaxBasisVars;
-    msd[threadIdx.x] = i < a.v ? norm[norm_at(a.e, a.R, depth, which, a.max_vars) + i]
+    msd[threadIdx.x] = i < a.v ? ld_ag(norm + norm_at(a.e, a.R, depth, which, a.max_vars) + i)
                                : (which ? 1.0 : 0.0);
   }
   __syncthreads();
   const size_t P = static_cast<size_t>(a.P);
   double s[2] = {0.0, 0.0};
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
+  for (int p = blk * blockDim.x + threadIdx.x; p < a.P; p += nblk * blockDim.x) {
     unsigned long long key = kKeySentinel;
     if (mem[p]) {
       double x[kMaxBasisVars];
@@ -420,10 +458,20 @@ __global__ void __launch_bounds__(kMultiThreads)
       s[1] += net * net;
       key = static_cast<unsigned long long>(__double_as_longlong(fabs(net)));
     }
-    keys[p] = key;
+    st_ag(keys + p, key);
   }
-  block_sums(s, 2, red, net_part, blockIdx.x, gridDim.x);
+  block_sums(s, 2, red, net_part, blk, nblk);
 }
+
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_net(const MultiArgs a, int depth, const float* __restrict__ bv,
+                const float* __restrict__ ev, const unsigned char* __restrict__ mem,
+                const DepthState* __restrict__ state, const double* __restrict__ norm,
+                const double* __restrict__ gram_part, double* __restrict__ coeffs,
+                unsigned long long* __restrict__ keys, double* __restrict__ net_part) {
+  multi_net_body(a, depth, bv, ev, mem, state, norm, gram_part, coeffs, keys, net_part, blockIdx.x, gridDim.x);
+}
+
 
 // ---------------------------------------------------------------------------
 // The selector's band search with exact counts, without ordering the keys.
@@ -475,11 +523,11 @@ __device__ void reduce_extents(const KeyExtent* __restrict__ ext, int nb, KeyExt
     unsigned long long mn = ~0ull, mx = 0ull;
     long long nz = 0, nv = 0;
     for (int b = threadIdx.x; b < nb; b += 64) {
-      const KeyExtent e = ext[b];
-      mn = e.minpos < mn ? e.minpos : mn;
-      mx = e.maxval > mx ? e.maxval : mx;
-      nz += e.nzero;
-      nv += e.nvalid;
+      const unsigned long long emn = ld_ag(&ext[b].minpos), emx = ld_ag(&ext[b].maxval);
+      mn = emn < mn ? emn : mn;
+      mx = emx > mx ? emx : mx;
+      nz += ld_ag(&ext[b].nzero);
+      nv += ld_ag(&ext[b].nvalid);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -589,26 +637,25 @@ __device__ __forceinline__ int ladder_bin(const unsigned long long* ts, int n,
 
 // Histogram of the member keys over the ladder: hist_part[bin * nb + block]
 // (FP64, exact) counts the members whose first bound >= key is bin.
-__global__ void __launch_bounds__(kMultiThreads)
-    k_multi_ladder(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+__device__ __forceinline__ void multi_ladder_body(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
                    const double* __restrict__ net_part, int nb_net,
                    const DepthState* __restrict__ state, double* __restrict__ hist_part,
-                   KeyExtent* __restrict__ ext_part) {
+                   KeyExtent* __restrict__ ext_part, int blk, int nblk) {
   __shared__ double sums[2];
   __shared__ double v[kLadderSlots];
   __shared__ unsigned long long tb[kLadderSlots], ts[kLadderSlots];
   __shared__ int hist[kLadderSlots];
   __shared__ KeyExtent wext[16];
-  if (!state[depth].active) return;
-  reduce_partials(net_part, nb_net, 2, sums);
-  const int n = ladder_build(a, state[depth].n_pre, sums, v, tb);
+  if (!ld_ag(&state[depth].active)) return;
+  reduce_partials<true>(net_part, nb_net, 2, sums);
+  const int n = ladder_build(a, ld_ag(&state[depth].n_pre), sums, v, tb);
   ladder_sort(n, tb, ts, nullptr);
   for (int t = threadIdx.x; t < n; t += blockDim.x) hist[t] = 0;
   __syncthreads();
   unsigned long long mn = ~0ull, mx = 0ull;
   long long nz = 0, nv = 0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
-    const unsigned long long key = keys[p];
+  for (int p = blk * blockDim.x + threadIdx.x; p < a.P; p += nblk * blockDim.x) {
+    const unsigned long long key = ld_ag(keys + p);
     if (key != kKeySentinel) {
       const int b = ladder_bin(ts, n, key);
       if (b < n) atomicAdd(&hist[b], 1);
@@ -633,7 +680,7 @@ __global__ void __launch_bounds__(kMultiThreads)
   if ((threadIdx.x & 63) == 0) wext[threadIdx.x >> 6] = KeyExtent{mn, mx, nz, nv};
   __syncthreads();
   for (int t = threadIdx.x; t < n; t += blockDim.x)
-    hist_part[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = static_cast<double>(hist[t]);
+    st_ag(hist_part + static_cast<size_t>(t) * nblk + blk, static_cast<double>(hist[t]));
   if (threadIdx.x == 0) {
     KeyExtent e = wext[0];
     for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
@@ -642,9 +689,21 @@ __global__ void __launch_bounds__(kMultiThreads)
       e.nzero += wext[w].nzero;
       e.nvalid += wext[w].nvalid;
     }
-    ext_part[blockIdx.x] = e;
+    st_ag(&ext_part[blk].minpos, e.minpos);
+    st_ag(&ext_part[blk].maxval, e.maxval);
+    st_ag(&ext_part[blk].nzero, e.nzero);
+    st_ag(&ext_part[blk].nvalid, e.nvalid);
   }
 }
+
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_ladder(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+                   const double* __restrict__ net_part, int nb_net,
+                   const DepthState* __restrict__ state, double* __restrict__ hist_part,
+                   KeyExtent* __restrict__ ext_part) {
+  multi_ladder_body(a, depth, keys, net_part, nb_net, state, hist_part, ext_part, blockIdx.x, gridDim.x);
+}
+
 
 // Count of member keys <= key_of(g) for g >= 0 from the extents when they
 // decide it, else -1.
@@ -661,7 +720,7 @@ __device__ long long block_count(const unsigned long long* __restrict__ keys, lo
                                  unsigned long long gb, long long* red /* [16] */) {
   long long c = 0;
   for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-    const unsigned long long k = keys[i];
+    const unsigned long long k = ld_ag(keys + i);
     c += (k != kKeySentinel && k <= gb) ? 1 : 0;
   }
 #pragma unroll
@@ -681,17 +740,17 @@ __device__ __forceinline__ void finish_select(const MultiArgs& a, int depth, dou
                                               long long after, DepthState* state, double* lower,
                                               double* upper) {
   if (after > a.cutoff) {
-    lower[static_cast<size_t>(a.e) * a.R + depth] = -cut;
-    upper[static_cast<size_t>(a.e) * a.R + depth] = cut;
+    st_ag(lower + static_cast<size_t>(a.e) * a.R + depth, -cut);
+    st_ag(upper + static_cast<size_t>(a.e) * a.R + depth, cut);
   }
   if (depth + 1 < a.R) {
-    state[depth + 1].active = after > a.cutoff ? 1 : 0;
-    state[depth + 1].n_pre = after;
+    st_ag(&state[depth + 1].active, after > a.cutoff ? 1 : 0);
+    st_ag(&state[depth + 1].n_pre, after);
     if (a.selector == 1)
-      state[depth + 1].cutbits =
-          cut > 0.0 ? static_cast<unsigned long long>(__double_as_longlong(cut)) - 1ull : 0ull;
+      st_ag(&state[depth + 1].cutbits,
+            cut > 0.0 ? static_cast<unsigned long long>(__double_as_longlong(cut)) - 1ull : 0ull);
     else
-      state[depth + 1].cutbits = cut >= 0.0 ? key_of(cut) : 0ull;
+      st_ag(&state[depth + 1].cutbits, cut >= 0.0 ? key_of(cut) : 0ull);
   }
 }
 
@@ -699,30 +758,29 @@ __device__ __forceinline__ void finish_select(const MultiArgs& a, int depth, dou
 // Regression_Selector_concrete_gold.cpp:13-183; RegressionSelectorStandardDeviations,
 // RegressionSelectorConcrete.cu:19-90).  One block; every thread runs the same
 // search on LDS values so the block-wide fallback counts stay uniform.
-__global__ void __launch_bounds__(kSearchThreads)
-    k_multi_select(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+__device__ __forceinline__ void multi_select_body(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
                    const double* __restrict__ net_part, int nb,
                    const double* __restrict__ hist_part, const KeyExtent* __restrict__ ext_part,
                    int nh, DepthState* __restrict__ state, SelectState* __restrict__ sel,
-                   double* __restrict__ lower, double* __restrict__ upper) {
+                   double* __restrict__ lower, double* __restrict__ upper, int blk, int nblk) {
   __shared__ double sums[2];
   __shared__ double v[kLadderSlots], cnt[kLadderSlots];
   __shared__ unsigned long long tb[kLadderSlots], ts[kLadderSlots];
   __shared__ int rank[kLadderSlots];
   __shared__ long long red[16];
   __shared__ KeyExtent ext;
-  if (!state[depth].active) {
+  if (!ld_ag(&state[depth].active)) {
     if (threadIdx.x == 0) {
-      sel->bisect = 0;
-      if (depth + 1 < a.R) state[depth + 1].active = 0;
+      st_ag(&sel->bisect, 0);
+      if (depth + 1 < a.R) st_ag(&state[depth + 1].active, 0);
     }
     return;
   }
-  const long long n_pre = state[depth].n_pre;
-  reduce_partials(net_part, nb, 2, sums);
+  const long long n_pre = ld_ag(&state[depth].n_pre);
+  reduce_partials<true>(net_part, nb, 2, sums);
   const int n = ladder_build(a, n_pre, sums, v, tb);
   ladder_sort(n, tb, ts, rank);
-  reduce_partials(hist_part, nh, n, cnt);  // cnt[k]: members in bin k
+  reduce_partials<true>(hist_part, nh, n, cnt);  // cnt[k]: members in bin k
   reduce_extents(ext_part, nh, &ext);
   if (threadIdx.x == 0)
     for (int k = 1; k < n; ++k) cnt[k] += cnt[k - 1];
@@ -814,17 +872,27 @@ __global__ void __launch_bounds__(kSearchThreads)
     }
   }
   if (threadIdx.x == 0) {
-    sel->bisect = bisect;
+    st_ag(&sel->bisect, bisect);
     if (bisect) {
-      sel->low = low;
-      sel->high = high;
-      sel->count_low = after;
-      sel->it = 0;
+      st_ag(&sel->low, low);
+      st_ag(&sel->high, high);
+      st_ag(&sel->count_low, after);
+      st_ag(&sel->it, 0);
     } else {
       finish_select(a, depth, cut, after, state, lower, upper);
     }
   }
 }
+
+__global__ void __launch_bounds__(kSearchThreads)
+    k_multi_select(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+                   const double* __restrict__ net_part, int nb,
+                   const double* __restrict__ hist_part, const KeyExtent* __restrict__ ext_part,
+                   int nh, DepthState* __restrict__ state, SelectState* __restrict__ sel,
+                   double* __restrict__ lower, double* __restrict__ upper) {
+  multi_select_body(a, depth, keys, net_part, nb, hist_part, ext_part, nh, state, sel, lower, upper, blockIdx.x, gridDim.x);
+}
+
 
 // The bisection tree: node 1 is the first guess 0.5 (low + high); node i
 // (heap order) splits its interval at its guess, its left child 2i takes the
@@ -845,15 +913,14 @@ __device__ __forceinline__ int tree_rank(int i) {
 
 // Per block: cumulative counts, in in-order position, of the member keys in
 // (low, high] that are <= each tree guess: cum_part[rank * nb + block].
-__global__ void __launch_bounds__(kMultiThreads)
-    k_multi_tree(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+__device__ __forceinline__ void multi_tree_body(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
                  const DepthState* __restrict__ state, const SelectState* __restrict__ sel,
-                 int* __restrict__ cum_part) {
+                 int* __restrict__ cum_part, int blk, int nblk) {
   __shared__ double lo[kTreeNodes + 1], hi[kTreeNodes + 1];
   __shared__ unsigned long long ts[kTreeNodes];
   __shared__ int hist[kTreeNodes + 1];
-  if (!state[depth].active || !sel->bisect) return;
-  const double low = sel->low, high = sel->high;
+  if (!ld_ag(&state[depth].active) || !ld_ag(&sel->bisect)) return;
+  const double low = ld_ag(&sel->low), high = ld_ag(&sel->high);
   for (int k = 0; k < kTreeDepth; ++k) {
     for (int i = (1 << k) + threadIdx.x; i < (2 << k); i += blockDim.x) {
       double l = low, h = high;
@@ -877,8 +944,8 @@ __global__ void __launch_bounds__(kMultiThreads)
   for (int t = threadIdx.x; t < kTreeNodes; t += blockDim.x) hist[t] = 0;
   __syncthreads();
   const unsigned long long lb = key_of(low), hb = key_of(high);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
-    const unsigned long long key = keys[p];
+  for (int p = blk * blockDim.x + threadIdx.x; p < a.P; p += nblk * blockDim.x) {
+    const unsigned long long key = ld_ag(keys + p);
     if (key != kKeySentinel && key > lb && key <= hb) {
       const int b = ladder_bin(ts, kTreeNodes, key);
       if (b < kTreeNodes) atomicAdd(&hist[b], 1);
@@ -889,15 +956,23 @@ __global__ void __launch_bounds__(kMultiThreads)
     for (int t = 1; t < kTreeNodes; ++t) hist[t] += hist[t - 1];
   __syncthreads();
   for (int t = threadIdx.x; t < kTreeNodes; t += blockDim.x)
-    cum_part[static_cast<size_t>(t) * gridDim.x + blockIdx.x] = hist[t];
+    st_ag(cum_part + static_cast<size_t>(t) * nblk + blk, hist[t]);
 }
+
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_tree(const MultiArgs a, int depth, const unsigned long long* __restrict__ keys,
+                 const DepthState* __restrict__ state, const SelectState* __restrict__ sel,
+                 int* __restrict__ cum_part) {
+  multi_tree_body(a, depth, keys, state, sel, cum_part, blockIdx.x, gridDim.x);
+}
+
 
 // Block-wide sum of cum_part[rank * nb + b] over the blocks.
 __device__ long long tree_count(const int* __restrict__ cum_part, int nb, int rank,
                                 long long* red) {
   long long c = 0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    c += cum_part[static_cast<size_t>(rank) * nb + b];
+    c += ld_ag(cum_part + static_cast<size_t>(rank) * nb + b);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
   __syncthreads();
@@ -914,23 +989,22 @@ __device__ long long tree_count(const int* __restrict__ cum_part, int nb, int ra
 // tree hands (low, high, count at low, step counter) to the next stage
 // (k_multi_tree + k_multi_bisect again); the last stage continues with
 // block-wide counts over all keys.
-__global__ void __launch_bounds__(kSearchThreads)
-    k_multi_bisect(const MultiArgs a, int depth, DepthState* __restrict__ state,
+__device__ __forceinline__ void multi_bisect_body(const MultiArgs a, int depth, DepthState* __restrict__ state,
                    SelectState* __restrict__ sel, const unsigned long long* __restrict__ keys,
                    const int* __restrict__ cum_part, const KeyExtent* __restrict__ ext_part,
                    int nb, double* __restrict__ lower, double* __restrict__ upper,
-                   int last_stage) {
+                   int last_stage, int blk, int nblk) {
   __shared__ long long red[16];
   __shared__ KeyExtent ext;
-  if (!state[depth].active || !sel->bisect) return;
+  if (!ld_ag(&state[depth].active) || !ld_ag(&sel->bisect)) return;
   reduce_extents(ext_part, nb, &ext);
-  const long long n_pre = state[depth].n_pre;
+  const long long n_pre = ld_ag(&state[depth].n_pre);
   const double np = static_cast<double>(n_pre);
   const long long lowerT = static_cast<long long>(np * a.lower_frac);
   const long long upperT = static_cast<long long>(np * a.upper_frac);
-  long long c_low = sel->count_low;
-  double low = sel->low, high = sel->high;
-  int it = sel->it;
+  long long c_low = ld_ag(&sel->count_low);
+  double low = ld_ag(&sel->low), high = ld_ag(&sel->high);
+  int it = ld_ag(&sel->it);
   int node = 1;
   const long long c0 = c_low;  // the tree counts keys in (low, high] of this stage
   auto count_at = [&](double g) -> long long {
@@ -953,10 +1027,10 @@ __global__ void __launch_bounds__(kSearchThreads)
     if (node > kTreeNodes && !last_stage) {
       __syncthreads();
       if (threadIdx.x == 0) {  // hand over; the next stage's root is 0.5 (low + high)
-        sel->low = low;
-        sel->high = high;
-        sel->count_low = c_low;
-        sel->it = it;
+        st_ag(&sel->low, low);
+        st_ag(&sel->high, high);
+        st_ag(&sel->count_low, c_low);
+        st_ag(&sel->it, it);
       }
       return;
     }
@@ -965,18 +1039,27 @@ __global__ void __launch_bounds__(kSearchThreads)
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    sel->bisect = 0;
+    st_ag(&sel->bisect, 0);
     finish_select(a, depth, guess, count, state, lower, upper);
   }
 }
 
+__global__ void __launch_bounds__(kSearchThreads)
+    k_multi_bisect(const MultiArgs a, int depth, DepthState* __restrict__ state,
+                   SelectState* __restrict__ sel, const unsigned long long* __restrict__ keys,
+                   const int* __restrict__ cum_part, const KeyExtent* __restrict__ ext_part,
+                   int nb, double* __restrict__ lower, double* __restrict__ upper,
+                   int last_stage) {
+  multi_bisect_body(a, depth, state, sel, keys, cum_part, ext_part, nb, lower, upper, last_stage, blockIdx.x, gridDim.x);
+}
+
+
 // Date 0's decision and sum_p Y_p / B_0 into block partials.
-__global__ void __launch_bounds__(kMultiThreads)
-    k_multi_final(const MultiArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
+__device__ __forceinline__ void multi_final_body(const MultiArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
                   const float* __restrict__ ev, const float* __restrict__ num,
                   const double* __restrict__ coeffs, const double* __restrict__ lower,
                   const double* __restrict__ upper, const double* __restrict__ norm,
-                  const double* __restrict__ Y, double* __restrict__ part) {
+                  const double* __restrict__ Y, double* __restrict__ part, int blk, int nblk) {
   __shared__ double beta[kMaxDepth * kMaxGram];
   __shared__ double nrm[kMaxDepth * 2 * kMaxBasisVars];
   __shared__ double lo[kMaxDepth], hi[kMaxDepth];
@@ -984,17 +1067,155 @@ __global__ void __launch_bounds__(kMultiThreads)
   load_decision(a, 0, coeffs, lower, upper, norm, beta, nrm, lo, hi);
   const size_t P = static_cast<size_t>(a.P);
   double s = 0.0;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += gridDim.x * blockDim.x) {
+  for (int p = blk * blockDim.x + threadIdx.x; p < a.P; p += nblk * blockDim.x) {
     const double y = decide(a, 0, a.v, a.k, bv, cash, ev, P, p, Y[p], beta, nrm, lo, hi);
     s += y / static_cast<double>(num[p]);
   }
-  block_sum_to(s, red, part + blockIdx.x);
+  block_sum_to<true>(s, red, part + blk);
 }
+
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_final(const MultiArgs a, const float* __restrict__ bv, const float* __restrict__ cash,
+                  const float* __restrict__ ev, const float* __restrict__ num,
+                  const double* __restrict__ coeffs, const double* __restrict__ lower,
+                  const double* __restrict__ upper, const double* __restrict__ norm,
+                  const double* __restrict__ Y, double* __restrict__ part) {
+  multi_final_body(a, bv, cash, ev, num, coeffs, lower, upper, norm, Y, part, blockIdx.x, gridDim.x);
+}
+
 
 __global__ void k_fill_zero(double* __restrict__ a, size_t n) {
   for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x)
     a[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// The whole pass in one persistent launch: the same phase bodies, one after
+// the other, every block of the grid taking part in the multi-block phases
+// and block 0 in the one-block searches, with a grid-wide barrier between
+// phases (what a phase hands to other blocks is stored and loaded
+// agent-coherent, so the barrier waits only for each wave's stores).  Each
+// lane's per-path values (running values, membership) stay with the lane
+// that wrote them.  Depths a date does not reach
+// and bisection stages the search does not need are skipped, uniformly over
+// the grid (every block reads the same state after a barrier).  Launched with
+// at most as many blocks as can be resident; every wait is bounded, and a
+// wait that gives up stops the pass with the error word set and a NaN price.
+// ---------------------------------------------------------------------------
+struct MultiDates {
+  signed char v[kMaxRates];  // variables at each exercise date
+};
+
+// Grid-wide barrier without a shared counter (a single word that every
+// block adds to serialises hundreds of atomics): block b stores the phase
+// number into flags[b] once its waves' stores are performed, and every block
+// waits until each of the grid's flags has reached the phase (thread t reads
+// flags[t]).  The values blocks exchange go through agent-coherent stores
+// and loads, so no cache write-back or invalidation is needed.  Bounded;
+// false on every lane if a wait gave up.
+__device__ __forceinline__ bool multi_grid_sync(unsigned* flags, unsigned phase) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores performed
+  __syncthreads();
+  if (threadIdx.x == 0) st_ag(flags + blockIdx.x, phase);
+  int ok = 1;
+  if (threadIdx.x < gridDim.x) {
+    unsigned polls = 0;
+    while (ld_ag(flags + threadIdx.x) < phase) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++polls > kBarrierPolls) {
+        ok = 0;
+        break;
+      }
+    }
+  }
+  return __syncthreads_and(ok) != 0;
+}
+
+__global__ void __launch_bounds__(kMultiThreads)
+    k_multi_pass(MultiArgs a, const MultiDates dates, int nb, int nh, const float* __restrict__ bv,
+                 const float* __restrict__ cash, const float* __restrict__ ev,
+                 const float* __restrict__ num, double* coeffs, double* lower, double* upper,
+                 double* norm, double* means, double* sds, double* Y, unsigned char* mem,
+                 unsigned long long* keys, int* cum, double* stats, double* gram, double* net,
+                 double* hist, KeyExtent* ext, double* price_part, DepthState* state,
+                 SelectState* sel, unsigned* bar, double* price_out) {
+  const int blk = blockIdx.x, G = gridDim.x;
+  const bool cross = a.cross != 0;
+  unsigned phase = 0;
+  // outputs start at zero (depths never reached stay zero)
+  {
+    const size_t ncoef = static_cast<size_t>(a.E) * a.R * a.maxd, ncut = static_cast<size_t>(a.E) * a.R;
+    const size_t nvar = static_cast<size_t>(a.E) * a.R * a.max_vars;
+    const size_t stride = static_cast<size_t>(G) * blockDim.x;
+    size_t nmax = 2 * nvar + 1;
+    nmax = ncoef > nmax ? ncoef : nmax;
+    nmax = ncut > nmax ? ncut : nmax;
+    for (size_t i = static_cast<size_t>(blk) * blockDim.x + threadIdx.x; i < nmax; i += stride) {
+      if (i < 2 * nvar + 1) st_ag(norm + i, 0.0);
+      if (i < ncoef) st_ag(coeffs + i, 0.0);
+      if (i < ncut) {
+        st_ag(lower + i, 0.0);
+        st_ag(upper + i, 0.0);
+      }
+      if (i < nvar && means) st_ag(means + i, 0.0);
+      if (i < nvar && sds) st_ag(sds + i, 0.0);
+    }
+  }
+  if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+  for (int e = a.E - 1; e >= 0; --e) {
+    a.e = e;
+    a.v = dates.v[e];
+    a.k = basis_count(a.v, cross);
+    a.vn = e + 1 < a.E ? dates.v[e + 1] : 0;
+    a.kn = e + 1 < a.E ? basis_count(a.vn, cross) : 0;
+    for (int d = 0; d < a.R; ++d) {
+      if (d > 0 && !ld_ag(&state[d].active)) break;  // no deeper regression at this date
+      if (blk < nb)
+        multi_prep_body(a, d, bv, cash, ev, num, coeffs, lower, upper, norm, Y, mem, keys, state,
+                        stats, blk, nb);
+      if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+      if (blk < nb) {
+        if (a.k + 1 <= 8)
+          multi_gram_body<2>(a, d, bv, cash, Y, mem, state, stats, norm, means, sds, gram, blk,
+                             nb);
+        else
+          multi_gram_body<1>(a, d, bv, cash, Y, mem, state, stats, norm, means, sds, gram, blk,
+                             nb);
+      }
+      if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+      if (blk < nb)
+        multi_net_body(a, d, bv, ev, mem, state, norm, gram, coeffs, keys, net, blk, nb);
+      if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+      if (blk < nh) multi_ladder_body(a, d, keys, net, nb, state, hist, ext, blk, nh);
+      if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+      if (blk == 0)
+        multi_select_body(a, d, keys, net, nb, hist, ext, nh, state, sel, lower, upper, 0, 1);
+      if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+      for (int stage = 0; stage < kBisectStages; ++stage) {
+        if (!ld_ag(&sel->bisect)) break;  // the search ended without (more) bisection
+        if (blk < nh) multi_tree_body(a, d, keys, state, sel, cum, blk, nh);
+        if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+        if (blk == 0)
+          multi_bisect_body(a, d, state, sel, keys, cum, ext, nh, lower, upper,
+                            stage + 1 == kBisectStages ? 1 : 0, 0, 1);
+        if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+      }
+    }
+  }
+  a.e = 0;
+  a.v = dates.v[0];
+  a.k = basis_count(a.v, cross);
+  if (blk < nb)
+    multi_final_body(a, bv, cash, ev, num, coeffs, lower, upper, norm, Y, price_part, blk, nb);
+  if (!multi_grid_sync(bar + 2, ++phase)) goto fail;
+  if (blk == 0) finalize_col<true>(price_part, nb, 1, 0, 1.0 / static_cast<double>(a.P), price_out);
+  return;
+fail:
+  if (threadIdx.x == 0) {
+    bar[1] = 1u;
+    if (blk == 0) *price_out = __builtin_nan("");
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,7 +1237,7 @@ int hist_blocks(int paths) {
 size_t al256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 struct MultiLayout {
-  size_t Y, mem, keys, cum, stats, gram, net, hist, ext, price, norm, state, sel, total;
+  size_t Y, mem, keys, cum, stats, gram, net, hist, ext, price, norm, state, sel, bar, total;
 };
 
 MultiLayout multi_layout(int paths) {
@@ -1050,6 +1271,8 @@ MultiLayout multi_layout(int paths) {
   o += al256(sizeof(DepthState) * kMaxDepth);
   l.sel = o;
   o += al256(sizeof(SelectState));
+  l.bar = o;  // error word, spare, then one barrier flag per block
+  o += al256(4 * (2 + nb));
   l.total = o;
   return l;
 }
@@ -1128,6 +1351,45 @@ extern "C" int ls_multi_first_pass_gpu(
   const int nb = multi_blocks(paths);
   const int nh = hist_blocks(paths);
   hipStream_t st = as_stream(stream);
+  MultiArgs a{};
+  a.E = n_ex;
+  a.P = paths;
+  a.max_vars = max_vars;
+  a.maxd = maxd;
+  a.R = R;
+  a.cross = cross ? 1 : 0;
+  a.normalise = normalise ? 1 : 0;
+  a.cutoff = lower_path_cutoff;
+  a.selector = selector;
+  a.lower_frac = lower_frac;
+  a.upper_frac = upper_frac;
+  a.sd_guess = sd_guess;
+  a.multiplier = multiplier;
+
+  // With KOO_LS_MULTI_PERSISTENT=1, one persistent launch (after zeroing
+  // its barrier words) when all nb blocks can be resident: the same bodies
+  // and blocks, bit for bit the same results.  Not the default: on the
+  // config-3 path quantities it measures 5.0 ms per pass against 3.7 ms for
+  // the launch per phase, the one-block band searches running 16 wavefronts
+  // of the same serial search where their own launches run 4.
+  {
+    int per_cu = 0;
+    const char* persistent = getenv("KOO_LS_MULTI_PERSISTENT");
+    if (persistent && persistent[0] == '1' &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_multi_pass, kMultiThreads, 0) ==
+            hipSuccess &&
+        per_cu * device_cu_count() >= nb) {
+      MultiDates dates{};
+      for (int e = 0; e < n_ex; ++e) dates.v[e] = static_cast<signed char>(vars_each_step[e]);
+      unsigned* bar = reinterpret_cast<unsigned*>(ws + lay.bar);
+      hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, bar, 2 + nb);
+      hipLaunchKernelGGL(k_multi_pass, dim3(nb), dim3(kMultiThreads), 0, st, a, dates, nb, nh,
+                         basis_variables, cash_flows, exercise_values, numeraires, coefficients,
+                         lower_cuts, upper_cuts, norm, means, sds, Y, mem, keys, cum, stats, gram,
+                         net, hist, ext, price_part, state, sel, bar, price_out);
+      return check_launch(who);
+    }
+  }
 
   // outputs start at zero (depths never reached stay zero)
   const size_t ncoef = static_cast<size_t>(n_ex) * R * maxd, ncut = static_cast<size_t>(n_ex) * R;
@@ -1145,20 +1407,6 @@ extern "C" int ls_multi_first_pass_gpu(
   if (sds && nvar)
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(nvar, 256, 256)), dim3(256), 0, st, sds, nvar);
 
-  MultiArgs a{};
-  a.E = n_ex;
-  a.P = paths;
-  a.max_vars = max_vars;
-  a.maxd = maxd;
-  a.R = R;
-  a.cross = cross ? 1 : 0;
-  a.normalise = normalise ? 1 : 0;
-  a.cutoff = lower_path_cutoff;
-  a.selector = selector;
-  a.lower_frac = lower_frac;
-  a.upper_frac = upper_frac;
-  a.sd_guess = sd_guess;
-  a.multiplier = multiplier;
   for (int e = n_ex - 1; e >= 0; --e) {
     a.e = e;
     a.v = vars_each_step[e];
