@@ -191,10 +191,10 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
         "price_second_pass_same_paths": price2,
         "itm_fraction_mean": itm,
         "roofline": with_traffic({
-            "kernel": "LS first pass (ls_first_pass_gpu): k_zero_words, then one persistent "
-                      "k_ls_first_pass (per date: MFMA normal equations with FP64 accumulation, "
-                      "the last block to arrive solves and publishes, decision update; then "
-                      "the price)",
+            "kernel": "LS first pass (ls_first_pass_gpu): k_fill_u64 arms the exchange slots, "
+                      "k_zero_words clears the flags, then one persistent k_ls_first_pass "
+                      "(per date: MFMA normal equations with FP64 accumulation, every block "
+                      "sums the published partials and solves, decision update; then the price)",
             "bound": "hbm",
             "achieved": gbs,
             "peak": HBM_PEAK_GBS,

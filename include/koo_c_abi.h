@@ -551,7 +551,7 @@ int ls_first_pass_gpu(int n_ex, const float* basis_variables, const int* vars_ea
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc): no workspace argument; the pass uses the
  * library's own device array (16 MiB, one per GPU, held until the process
- * ends), so ls_workspace(paths) must not exceed 16 MiB (up to 3 669 952
+ * ends), so ls_workspace(paths) must not exceed 16 MiB (up to 3 723 200
  * paths on a 256-CU device; larger sizes are rejected with KOO_ERR_INVALID
  * before anything runs).  Calls on one GPU share that array: calls that overlap on a GPU (two
  * streams, two host threads) race and must not be issued.  Same results as

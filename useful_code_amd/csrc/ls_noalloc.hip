@@ -6,16 +6,17 @@
 //
 // Every call on a GPU shares that GPU's array and the array keeps the last
 // call's contents: the first pass writes every word it reads before reading
-// it (running values, then the normal-equation partials date by date) and
-// zeroes its barrier words with a kernel at the start of each call, so no
-// state carries over.  Calls that overlap on one GPU (two streams, two host
+// it (running values, then the normal-equation partials date by date), and
+// at the start of each one-launch pass a fill kernel arms every exchange slot
+// with the "unset" pattern and another zeroes the flag word, so no state
+// carries over.  Calls that overlap on one GPU (two streams, two host
 // threads) race on the array and must not be issued.
 #include <mutex>
 
 #include "koo_common.h"
 
 // Largest workspace the array serves: ls_workspace(paths) <= 16 MiB, i.e. up
-// to 3 669 952 paths on a 256-CU device (the workspace's partials grow with
+// to 3 723 200 paths on a 256-CU device (the workspace's partials grow with
 // the CU count).
 constexpr size_t kLsScratchBytes = static_cast<size_t>(16) << 20;
 static __device__ __align__(256) unsigned char scratch[kLsScratchBytes];
