@@ -452,7 +452,8 @@ def basket_section(args, torch, O, configs, pipeline, ev):
 def stage_section(args, torch, configs, pipeline, ev):
     """The reference-ABI stage kernels at the config-2 shape (P = 2^20,
     N = S = 40, F = 5, D = 200): achieved HBM GB/s from algorithmic bytes over
-    HIP-event time (mean of 3 launches after one warm-up), against 8 TB/s."""
+    HIP-event time (mean of 5 launches after one warm-up, each after a read
+    that evicts the last-level cache), against 8 TB/s."""
     from useful_code_amd import koo
     w = configs.caplet_portfolio(paths=args.paths)
     P, N, F, S = w.paths, w.n_rates, w.factors, w.steps
@@ -533,11 +534,18 @@ def stage_section(args, torch, configs, pipeline, ev):
     copy_dst = torch.empty_like(copy_src)
     stages.append(("device copy (torch, reference point)", lambda: copy_dst.copy_(copy_src),
                    2 * four * D * P))
+    # Before each timed call a sum over 512 MiB evicts the stage's data from
+    # the 256 MB last-level (Infinity) cache, so its bytes come from HBM (a
+    # read, so no dirty lines are left to be written back during the timed
+    # call), and keeps the device busy while the host enqueues the call, so
+    # the events time the kernels and not the host's launch overhead.
+    scrub = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
     res = {}
     for name, fn, nbytes in stages:
         fn()
-        recs = [(ev(), ev()) for _ in range(3)]
+        recs = [(ev(), ev()) for _ in range(5)]
         for a, b in recs:
+            scrub.sum()
             a.record()
             fn()
             b.record()

@@ -288,10 +288,11 @@ def test_ls_first_pass_matches_fp64_oracle_on_same_inputs(torch_cuda, berm):
 
 @pytest.mark.parametrize("paths", [1 << 18, 5000, 77])
 def test_persistent_first_pass_equals_per_date_launches(torch_cuda, paths):
-    """The first pass as one persistent launch (grid barriers between dates)
-    and as one launch per date run the same per-date block code over the same
-    blocks, so coefficients and price agree bit for bit; repeated persistent
-    runs (the barrier words re-zeroed each time) agree too."""
+    """The first pass as one persistent launch (KOO_LS_LAUNCH_PER_DATE=0:
+    the partials exchanged through sentinel-armed slots between dates) and as
+    one launch per date (the default) run the same per-date block code over
+    the same blocks, so coefficients and price agree bit for bit; repeated
+    persistent runs (the slots re-armed each time) agree too."""
     import os
     torch = torch_cuda
     w = configs.bermudan_swaption(paths=paths)
@@ -598,9 +599,10 @@ def test_persistent_portfolio_equals_stage_calls(torch_cuda, seed, paths):
 
 def test_portfolio_persistent_ls_equals_per_round_launches(torch_cuda):
     """The portfolio's LS first passes as one persistent launch (resident
-    blocks walking all products' blocks, a grid barrier per round) and as one
-    launch per round: same per-block code and partials, so every product's
-    coefficients and first-pass price agree bit for bit."""
+    blocks walking all products' blocks, the partials exchanged through
+    sentinel-armed slots each round; KOO_LS_LAUNCH_PER_DATE=0) and as one
+    launch per round (the default): same per-block code and partials, so
+    every product's coefficients and first-pass price agree bit for bit."""
     import os
     torch = torch_cuda
     prods = configs.bermudan_portfolio(n_products=12, paths=(1 << 13) + 5, seed=79)

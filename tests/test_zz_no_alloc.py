@@ -1,8 +1,12 @@
 """ls_first_pass_noalloc_gpu (the LS first pass on the library's own scratch
 array, for callers that cannot allocate) against ls_first_pass_gpu with a
 workspace of the caller's: same kernels, same blocks, so coefficients and
-price agree bit for bit; two calls in a row, the second smaller, and a size
-beyond the array's limit rejected before anything runs."""
+price agree bit for bit; two calls in a row, the second smaller, in both
+launch forms (one launch per date, the default, and the persistent launch
+whose exchange slots live in the array), and a size beyond the array's limit
+rejected before anything runs."""
+import os
+
 import numpy as np
 import pytest
 
@@ -35,15 +39,20 @@ def _both(torch, w, pr):
     return (want_c, want_p), (got_c, got_p)
 
 
-def test_noalloc_first_pass_equals_workspace_first_pass(torch_cuda):
+@pytest.mark.parametrize("per_date", ["1", "0"])
+def test_noalloc_first_pass_equals_workspace_first_pass(torch_cuda, per_date):
     torch = torch_cuda
-    for paths, seed in ((1 << 18, 2026), ((1 << 12) + 3, 11)):
-        print(f"no-alloc LS first pass, {paths} paths", flush=True)
-        w, pr = _inputs(torch, paths, seed)
-        (wc, wp), (gc, gp) = _both(torch, w, pr)
-        print(f"  price workspace {wp.item():.12f} no-alloc {gp.item():.12f}", flush=True)
-        assert np.isfinite(wp.item()) and wp.item() == gp.item()
-        assert torch.equal(wc, gc)
+    os.environ["KOO_LS_LAUNCH_PER_DATE"] = per_date
+    try:
+        for paths, seed in ((1 << 18, 2026), ((1 << 12) + 3, 11)):
+            print(f"no-alloc LS first pass, {paths} paths, per date {per_date}", flush=True)
+            w, pr = _inputs(torch, paths, seed)
+            (wc, wp), (gc, gp) = _both(torch, w, pr)
+            print(f"  price workspace {wp.item():.12f} no-alloc {gp.item():.12f}", flush=True)
+            assert np.isfinite(wp.item()) and wp.item() == gp.item()
+            assert torch.equal(wc, gc)
+    finally:
+        os.environ.pop("KOO_LS_LAUNCH_PER_DATE", None)
 
 
 def test_noalloc_first_pass_rejects_sizes_beyond_the_array(torch_cuda):

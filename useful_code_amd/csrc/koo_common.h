@@ -88,6 +88,13 @@ __device__ __forceinline__ float invnorm_u32(uint32_t x) {
   return upper ? r : -r;
 }
 
+// Error-free sum: s + e == a + b exactly (Knuth's TwoSum).
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
 // Sobol point for Gray-code index n (x = XOR of v[k] over the set bits of
 // g = n ^ (n >> 1)); v in LDS.
 __device__ __forceinline__ uint32_t sobol_point(const uint32_t* v, uint32_t n) {

@@ -14,11 +14,11 @@
 //   inputs and order, so identical coefficients), then applies the later
 //   date's exercise decision and numeraire deflation to its paths' running
 //   values and accumulates this date's normal equations (optionally over
-//   in-the-money paths only) on MFMA into its block partials.  One persistent
-//   launch (k_ls_first_pass; the partials exchanged through words armed with
-//   a sentinel, no counters) or one launch per round (k_ls_round_single).
-//   The portfolio's passes run the same per-block code (k_ls_portfolio,
-//   k_ls_round).
+//   in-the-money paths only) on MFMA into its block partials.  One launch per
+//   round (k_ls_round_single, the default) or one persistent launch
+//   (k_ls_first_pass; the partials exchanged through words armed with a
+//   sentinel, no counters).  The portfolio's passes run the same per-block
+//   code (k_ls_portfolio, k_ls_round).
 // * k_ls_second: the frozen-strategy pass (price under given coefficients).
 // * the reference's small LS kernels (basis variables, exercise values, basis
 //   expansion, continuation values, update, deflation).
@@ -793,11 +793,14 @@ extern "C" int ls_first_pass_gpu(int n_ex, const float* basis_variables, const i
   }
   bool resident = false;
   const int nb = ls_pass_blocks(paths, &resident);
-  // One persistent launch over all rounds (after arming its partial words)
-  // when every basis has at most 7 functions and the blocks can all be
-  // resident, unless KOO_LS_LAUNCH_PER_DATE=1 asks for one launch per round.
+  // One launch per round by default: on MI355X at 2^18 paths, 10 dates, the
+  // launch boundary exchanges the partials faster than the persistent
+  // kernel's polling exchange (141 against 171 us per pass).  One persistent
+  // launch over all rounds (after arming its partial words) with
+  // KOO_LS_LAUNCH_PER_DATE=0, when every basis has at most 7 functions and
+  // the blocks can all be resident.
   const char* per_date = getenv("KOO_LS_LAUNCH_PER_DATE");
-  if (small && resident && !(per_date && per_date[0] == '1')) {
+  if (small && resident && per_date && per_date[0] == '0') {
     unsigned* bar = reinterpret_cast<unsigned*>(ws + lay.bar);
     const size_t words = 3 * static_cast<size_t>(kTriMax) * nb;
     hipLaunchKernelGGL(k_fill_u64, dim3((words + 255) / 256), dim3(256), 0, st,
@@ -1309,13 +1312,14 @@ extern "C" int bermudan_ls_portfolio_run(const koo_portfolio_plan* plan,
     hipLaunchKernelGGL(k_berm_paths_batch, dim3(nb), dim3(256), lds, st, items, plan->products,
                        b0);
   }
-  // The LS first passes: one persistent launch (after arming every
-  // product's partial words and zeroing the error word) with as many blocks
-  // as can be resident, or, when none can or KOO_LS_LAUNCH_PER_DATE=1, one
-  // launch per round and one for the prices.
+  // The LS first passes: one launch per round and one for the prices (the
+  // default: on MI355X the 64-product portfolio's passes take 7.0 ms so
+  // against 8.0 ms in one launch), or with KOO_LS_LAUNCH_PER_DATE=0 one
+  // persistent launch (after arming every product's partial words and
+  // zeroing the error word) with as many blocks as can be resident.
   int per_cu = 0;
   const char* per_date = getenv("KOO_LS_LAUNCH_PER_DATE");
-  if (!(per_date && per_date[0] == '1') &&
+  if (per_date && per_date[0] == '0' &&
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ls_portfolio, kGramThreads, 0) ==
           hipSuccess &&
       per_cu > 0) {

@@ -274,14 +274,17 @@ __device__ __forceinline__ void rearm_tri(double* X, int nb, int T, int block) {
 }
 
 // Sums the entry-major partials part[t * nb + b] over b < nb into sum[t]
-// (LDS) for t < nn: each wavefront takes 4 entries at a time and each lane 4
+// (LDS) for t < nn, on the wavefronts [w0, w0 + nws) of the block (the others
+// return at once): each wavefront takes 4 entries at a time and each lane 4
 // blocks of each (all 16 loads in flight, coalesced), then wavefront sums.
-// Contains a barrier.  Agent: agent-coherent loads.
+// An entry's sum does not depend on which wavefront forms it.  No barrier.
+// Agent: agent-coherent loads.
 template <bool Agent = false>
-__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nb, int nn,
-                                                double* sum) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int t0 = wave * 4; t0 < nn; t0 += nw * 4) {
+__device__ __forceinline__ void reduce_partials_waves(const double* __restrict__ part, int nb,
+                                                      int nn, double* sum, int w0, int nws) {
+  const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0;
+  if (wave < 0 || wave >= nws) return;
+  for (int t0 = wave * 4; t0 < nn; t0 += nws * 4) {
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b0 = 0; b0 < nb; b0 += 256) {
       double v[4][4];
@@ -302,6 +305,13 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
       if (lane == 0 && t0 + u < nn) sum[t0 + u] = tot;
     }
   }
+}
+
+// reduce_partials_waves on every wavefront of the block, then a barrier.
+template <bool Agent = false>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nb, int nn,
+                                                double* sum) {
+  reduce_partials_waves<Agent>(part, nb, nn, sum, 0, blockDim.x >> 6);
   __syncthreads();
 }
 

@@ -46,7 +46,7 @@ namespace {
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMultiThreads = 1024;  // multi-block kernels (16 wavefronts)
-constexpr int kSearchThreads = 256;
+constexpr int kSearchThreads = 1024;  // single-block search kernels (16 wavefronts)
 constexpr int kMaxDepth = 8;         // regressions per exercise date
 constexpr int kSelectMaxSteps = 1000;
 constexpr unsigned long long kKeySentinel = 0x7FFFFFFFFFFFFFFFull;
@@ -431,6 +431,21 @@ __device__ __forceinline__ void multi_net_body(const MultiArgs a, int depth, con
   __shared__ double red[16 * 8];
   if (!ld_ag(&state[depth].active)) return;
   const int n = a.k + 1;
+  const size_t P = static_cast<size_t>(a.P);
+  // the lane's first path's inputs are loaded before the Gram sums and the
+  // solve, so their latency overlaps them
+  const int p0 = blk * blockDim.x + threadIdx.x;
+  unsigned char m0 = 0;
+  float raw0[kMaxBasisVars], ev0 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kMaxBasisVars; ++i) raw0[i] = 0.0f;
+  if (p0 < a.P) {
+    m0 = mem[p0];
+#pragma unroll
+    for (int i = 0; i < kMaxBasisVars; ++i)
+      if (i < a.v) raw0[i] = bv[(static_cast<size_t>(a.e) * a.max_vars + i) * P + p0];
+    ev0 = ev[static_cast<size_t>(a.e) * P + p0];
+  }
   reduce_partials<true>(gram_part, nblk, n * n, gram);
   if (threadIdx.x == 0) {
     solve_multi(gram, a.k, static_cast<double>(ld_ag(&state[depth].n_pre)), beta, A, V, rhs, tmp);
@@ -439,21 +454,29 @@ __device__ __forceinline__ void multi_net_body(const MultiArgs a, int depth, con
         st_ag(coeffs + (static_cast<size_t>(a.e) * a.R + depth) * a.maxd + i,
               i < a.k ? beta[i] : 0.0);
   }
-  if (threadIdx.x < 2 * kMaxBasisVars) {
-    const int which = threadIdx.x / kMaxBasisVars, i = threadIdx.x % kMaxBasisVars;
-    msd[threadIdx.x] = i < a.v ? ld_ag(norm + norm_at(a.e, a.R, depth, which, a.max_vars) + i)
-                               : (which ? 1.0 : 0.0);
+  // the normalisation on the second wavefront, beside the solve on the first
+  if (threadIdx.x >= 64 && threadIdx.x < 64 + 2 * kMaxBasisVars) {
+    const int t = threadIdx.x - 64;
+    const int which = t / kMaxBasisVars, i = t % kMaxBasisVars;
+    msd[t] = i < a.v ? ld_ag(norm + norm_at(a.e, a.R, depth, which, a.max_vars) + i)
+                     : (which ? 1.0 : 0.0);
   }
   __syncthreads();
-  const size_t P = static_cast<size_t>(a.P);
   double s[2] = {0.0, 0.0};
-  for (int p = blk * blockDim.x + threadIdx.x; p < a.P; p += nblk * blockDim.x) {
+  for (int p = p0; p < a.P; p += nblk * blockDim.x) {
     unsigned long long key = kKeySentinel;
-    if (mem[p]) {
+    const bool first = p == p0;
+    if (first ? m0 != 0 : mem[p] != 0) {
       double x[kMaxBasisVars];
-      norm_vars(bv, a.e, a.max_vars, P, p, a.v, msd, msd + kMaxBasisVars, x);
+      if (first) {
+#pragma unroll
+        for (int i = 0; i < kMaxBasisVars; ++i)
+          x[i] = i < a.v ? (static_cast<double>(raw0[i]) - msd[i]) / msd[kMaxBasisVars + i] : 0.0;
+      } else {
+        norm_vars(bv, a.e, a.max_vars, P, p, a.v, msd, msd + kMaxBasisVars, x);
+      }
       const double net = basis_dot(a.v, a.k, a.cross != 0, x, beta) -
-                         static_cast<double>(ev[static_cast<size_t>(a.e) * P + p]);
+                         static_cast<double>(first ? ev0 : ev[static_cast<size_t>(a.e) * P + p]);
       s[0] += net;
       s[1] += net * net;
       key = static_cast<unsigned long long>(__double_as_longlong(fabs(net)));
@@ -492,7 +515,8 @@ constexpr int kLadder = 64;                   // growth / shrink steps held
 constexpr int kLadderSlots = 256;             // >= 3 kLadder - 1
 static_assert(3 * kLadder - 1 <= kLadderSlots, "ladder slots");
 // ladder_build uses lanes 0 and 64; block sums use one LDS slot per wavefront
-static_assert(kSearchThreads >= 128 && kMultiThreads >= 128, "ladder needs 128 lanes");
+static_assert(kSearchThreads >= 256 && kMultiThreads >= 256,
+              "ladder needs 128 lanes, the selection's sums 4 wavefronts");
 static_assert(kSearchThreads <= 1024 && kMultiThreads <= 1024, "16 wavefront slots");
 constexpr unsigned long long kEmpty = ~0ull;  // a cut that counts no path
 
@@ -517,28 +541,49 @@ struct KeyExtent {
 
 constexpr unsigned long long kInfKey = 0x7FF0000000000000ull;
 
-// Sums the per-block extents ext[0..nb) into LDS *out.  Whole block.
-__device__ void reduce_extents(const KeyExtent* __restrict__ ext, int nb, KeyExtent* out) {
-  if (threadIdx.x < 64) {
-    unsigned long long mn = ~0ull, mx = 0ull;
-    long long nz = 0, nv = 0;
-    for (int b = threadIdx.x; b < nb; b += 64) {
-      const unsigned long long emn = ld_ag(&ext[b].minpos), emx = ld_ag(&ext[b].maxval);
-      mn = emn < mn ? emn : mn;
-      mx = emx > mx ? emx : mx;
-      nz += ld_ag(&ext[b].nzero);
-      nv += ld_ag(&ext[b].nvalid);
+// Wavefront `wave` of the block sums the per-block extents ext[0..nb) into
+// LDS *out (its lanes load four blocks' extents at a time, all in flight);
+// the other wavefronts return at once.  No barrier.
+__device__ void reduce_extents_wave(const KeyExtent* __restrict__ ext, int nb, KeyExtent* out,
+                                    int wave) {
+  if ((threadIdx.x >> 6) != wave) return;
+  const int lane = threadIdx.x & 63;
+  unsigned long long mn = ~0ull, mx = 0ull;
+  long long nz = 0, nv = 0;
+  for (int b0 = 0; b0 < nb; b0 += 256) {
+    unsigned long long emn[4], emx[4];
+    long long enz[4], env[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int b = b0 + lane + 64 * r;
+      const bool ok = b < nb;
+      emn[r] = ok ? ld_ag(&ext[b].minpos) : ~0ull;
+      emx[r] = ok ? ld_ag(&ext[b].maxval) : 0ull;
+      enz[r] = ok ? ld_ag(&ext[b].nzero) : 0ll;
+      env[r] = ok ? ld_ag(&ext[b].nvalid) : 0ll;
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned long long a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
-      mn = a < mn ? a : mn;
-      mx = b > mx ? b : mx;
-      nz += __shfl_xor(nz, o, 64);
-      nv += __shfl_xor(nv, o, 64);
+    for (int r = 0; r < 4; ++r) {
+      mn = emn[r] < mn ? emn[r] : mn;
+      mx = emx[r] > mx ? emx[r] : mx;
+      nz += enz[r];
+      nv += env[r];
     }
-    if (threadIdx.x == 0) *out = KeyExtent{mn, mx, nz, nv};
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+    nz += __shfl_xor(nz, o, 64);
+    nv += __shfl_xor(nv, o, 64);
+  }
+  if (lane == 0) *out = KeyExtent{mn, mx, nz, nv};
+}
+
+// The same on wavefront 0, then a barrier.  Whole block.
+__device__ void reduce_extents(const KeyExtent* __restrict__ ext, int nb, KeyExtent* out) {
+  reduce_extents_wave(ext, nb, out, 0);
   __syncthreads();
 }
 
@@ -551,6 +596,16 @@ __device__ __forceinline__ double net_sd(const double* sums, long long n_pre) {
   const double np = static_cast<double>(n_pre);
   const double mean = sums[0] / np;
   return sqrt(sums[1] / np - mean * mean);
+}
+
+// Number of ladder values ladder_build makes for n_pre members: it depends
+// on the selector and the member count only, not on the net values.
+__device__ __forceinline__ int ladder_count(const MultiArgs& a, long long n_pre) {
+  const double np = static_cast<double>(n_pre);
+  const long long lowerT = static_cast<long long>(np * a.lower_frac);
+  const long long upperT = static_cast<long long>(np * a.upper_frac);
+  if (a.selector == 1 || (n_pre > lowerT && n_pre < upperT)) return 1;
+  return 3 * kLadder - 1;
 }
 
 // The ladder of raw cut values v[0..n) and their key bounds tb (a member
@@ -647,6 +702,10 @@ __device__ __forceinline__ void multi_ladder_body(const MultiArgs a, int depth, 
   __shared__ int hist[kLadderSlots];
   __shared__ KeyExtent wext[16];
   if (!ld_ag(&state[depth].active)) return;
+  // the lane's first key is loaded before the ladder is built, so its
+  // latency overlaps the net-value sums
+  const int p0 = blk * blockDim.x + threadIdx.x;
+  const unsigned long long key0 = p0 < a.P ? ld_ag(keys + p0) : kKeySentinel;
   reduce_partials<true>(net_part, nb_net, 2, sums);
   const int n = ladder_build(a, ld_ag(&state[depth].n_pre), sums, v, tb);
   ladder_sort(n, tb, ts, nullptr);
@@ -654,8 +713,8 @@ __device__ __forceinline__ void multi_ladder_body(const MultiArgs a, int depth, 
   __syncthreads();
   unsigned long long mn = ~0ull, mx = 0ull;
   long long nz = 0, nv = 0;
-  for (int p = blk * blockDim.x + threadIdx.x; p < a.P; p += nblk * blockDim.x) {
-    const unsigned long long key = ld_ag(keys + p);
+  for (int p = p0; p < a.P; p += nblk * blockDim.x) {
+    const unsigned long long key = p == p0 ? key0 : ld_ag(keys + p);
     if (key != kKeySentinel) {
       const int b = ladder_bin(ts, n, key);
       if (b < n) atomicAdd(&hist[b], 1);
@@ -777,11 +836,15 @@ __device__ __forceinline__ void multi_select_body(const MultiArgs a, int depth, 
     return;
   }
   const long long n_pre = ld_ag(&state[depth].n_pre);
-  reduce_partials<true>(net_part, nb, 2, sums);
+  // the three cross-block sums at once, on disjoint wavefronts: the ladder's
+  // bin counts (cnt[k]: members in bin k), the net-value sums, the extents
+  const int nw = blockDim.x >> 6;
+  reduce_partials_waves<true>(hist_part, nh, ladder_count(a, n_pre), cnt, 0, nw - 2);
+  reduce_partials_waves<true>(net_part, nb, 2, sums, nw - 2, 1);
+  reduce_extents_wave(ext_part, nh, &ext, nw - 1);
+  __syncthreads();
   const int n = ladder_build(a, n_pre, sums, v, tb);
   ladder_sort(n, tb, ts, rank);
-  reduce_partials<true>(hist_part, nh, n, cnt);  // cnt[k]: members in bin k
-  reduce_extents(ext_part, nh, &ext);
   if (threadIdx.x == 0)
     for (int k = 1; k < n; ++k) cnt[k] += cnt[k - 1];
   __syncthreads();

@@ -69,10 +69,16 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
+// FP64 sum of the four lanes of a float4.
+__device__ __forceinline__ double sum4(const f32x4 x) {
+  return (static_cast<double>(x.x) + static_cast<double>(x.y)) +
+         (static_cast<double>(x.z) + static_cast<double>(x.w));
+}
+
 // The same for FP32 input, 16 bytes per load: block b sums float4 elements
-// [b*chunk4, min((b+1)*chunk4, n4)) with four accumulators per lane and two
-// loads in flight (streaming, nontemporal); the last block also adds the
-// n_tail (< 4) elements past the last float4.
+// [b*chunk4, min((b+1)*chunk4, n4)) with four loads in flight per lane
+// (streaming, nontemporal) into four FP64 accumulators; the last block also
+// adds the n_tail (< 4) elements past the last float4.
 __global__ void __launch_bounds__(kRedThreads)
     k_block_sums_vec4(const f32x4* __restrict__ in, long long n4, long long chunk4,
                       const float* __restrict__ tail, int n_tail, double* __restrict__ part) {
@@ -80,23 +86,20 @@ __global__ void __launch_bounds__(kRedThreads)
   const long long lo = static_cast<long long>(blockIdx.x) * chunk4;
   long long hi = lo + chunk4;
   if (hi > n4) hi = n4;
+  const long long B = blockDim.x;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   long long i = lo + threadIdx.x;
-  for (; i + blockDim.x < hi; i += 2 * blockDim.x) {
+  for (; i + 3 * B < hi; i += 4 * B) {
     const f32x4 x = load_stream(in + i);
-    const f32x4 y = load_stream(in + i + blockDim.x);
-    a0 += static_cast<double>(x.x) + static_cast<double>(y.x);
-    a1 += static_cast<double>(x.y) + static_cast<double>(y.y);
-    a2 += static_cast<double>(x.z) + static_cast<double>(y.z);
-    a3 += static_cast<double>(x.w) + static_cast<double>(y.w);
+    const f32x4 y = load_stream(in + i + B);
+    const f32x4 u = load_stream(in + i + 2 * B);
+    const f32x4 v = load_stream(in + i + 3 * B);
+    a0 += sum4(x);
+    a1 += sum4(y);
+    a2 += sum4(u);
+    a3 += sum4(v);
   }
-  if (i < hi) {
-    const f32x4 x = load_stream(in + i);
-    a0 += static_cast<double>(x.x);
-    a1 += static_cast<double>(x.y);
-    a2 += static_cast<double>(x.z);
-    a3 += static_cast<double>(x.w);
-  }
+  for (; i < hi; i += B) a0 += sum4(load_stream(in + i));
   // the last block's first lane also takes the 0-3 elements past the float4s
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
     for (int t = 0; t < n_tail; ++t) a0 += static_cast<double>(tail[t]);
@@ -111,10 +114,14 @@ __global__ void __launch_bounds__(kRedThreads)
 }
 
 // One block: compensated sum of nb partials; result to out_d and/or out_f.
+// Each lane adds a strided slice with Neumaier's correction, then the lanes'
+// (sum, correction) pairs are combined by a wavefront butterfly of
+// error-free additions (TwoSum) and the four waves' pairs in LDS, so no lane
+// walks more than nb / 256 + 6 + 4 dependent additions.
 __global__ void __launch_bounds__(kRedThreads)
     k_final_sum(const double* __restrict__ part, int nb, double* __restrict__ out_d,
                 float* __restrict__ out_f) {
-  __shared__ double sv[kRedThreads], sc[kRedThreads];
+  __shared__ double sv[kRedThreads / 64], sc[kRedThreads / 64];
   double s = 0.0, c = 0.0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     const double x = part[i];
@@ -122,17 +129,26 @@ __global__ void __launch_bounds__(kRedThreads)
     c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
     s = t;
   }
-  sv[threadIdx.x] = s;
-  sc[threadIdx.x] = c;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double s2 = __shfl_xor(s, o, 64), c2 = __shfl_xor(c, o, 64);
+    double t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c += c2 + e;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sv[threadIdx.x >> 6] = s;
+    sc[threadIdx.x >> 6] = c;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     double S = 0.0, C = 0.0;
-    for (int i = 0; i < static_cast<int>(blockDim.x); ++i) {
-      const double x = sv[i];
-      const double t = S + x;
-      C += (fabs(S) >= fabs(x)) ? (S - t) + x : (x - t) + S;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+      double t, e;
+      two_sum(S, sv[w], t, e);
       S = t;
-      C += sc[i];
+      C += sc[w] + e;
     }
     if (out_d) *out_d = S + C;
     if (out_f) *out_f = static_cast<float>(S + C);

@@ -26,6 +26,11 @@ namespace koo {
 namespace {
 constexpr int kSobolThreads = 256;
 constexpr int kPointsPerThread = 8;
+// 16-byte kernels: quads of points per lane.  Each block first loads its
+// dimension's direction numbers and forms its lanes' first points from the
+// Gray code (a loop over the bits); eight quads per lane spread that set-up
+// over 128 bytes written per lane.
+constexpr int kQuadsPerThread = 8;
 
 __host__ __device__ inline unsigned floor_log2(unsigned x) { return 31u - __builtin_clz(x); }
 
@@ -169,7 +174,7 @@ unsigned next_pow2(unsigned x) {
 }
 
 dim3 sobol_grid4(int n_quads, int n_dimensions) {
-  unsigned per_block = kSobolThreads * (kPointsPerThread / 4);
+  unsigned per_block = kSobolThreads * kQuadsPerThread;
   unsigned gx = next_pow2((static_cast<unsigned>(n_quads) + per_block - 1) / per_block);
   return dim3(gx, static_cast<unsigned>(n_dimensions));
 }
