@@ -171,8 +171,9 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
     graph_ms = float(np.mean([a.elapsed_time(b) for a, b in grecs]))
     price1, price2 = pr.price1.item(), pr.price2.item()
     itm = float((pr.ev.view(E, P) > 0).float().mean().item())
-    # The whole first pass (E date kernels, E one-block solves, the final
-    # pass): algorithmic bytes of the E date kernels over its event time.
+    # The whole first pass (E + 1 round launches: every block sums the
+    # previous round's partials and solves, then its paths' date work; the
+    # price): algorithmic bytes of the E dates over its event time.
     stage_bytes = E * LS_DATE_BYTES_PER_PATH * P
     gbs = stage_bytes / (regress * 1e-3) / 1e9
     k = 6
@@ -191,10 +192,10 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
         "price_second_pass_same_paths": price2,
         "itm_fraction_mean": itm,
         "roofline": with_traffic({
-            "kernel": "LS first pass (ls_first_pass_gpu): k_fill_u64 arms the exchange slots, "
-                      "k_zero_words clears the flags, then one persistent k_ls_first_pass "
-                      "(per date: MFMA normal equations with FP64 accumulation, every block "
-                      "sums the published partials and solves, decision update; then the price)",
+            "kernel": "LS first pass (ls_first_pass_gpu): one k_ls_round_single<2> launch per "
+                      "exercise date and one for the price (per date: every block sums the "
+                      "previous date's FP64 partials and solves, decision update, MFMA normal "
+                      "equations with FP64 accumulation), then k_finalize_cols",
             "bound": "hbm",
             "achieved": gbs,
             "peak": HBM_PEAK_GBS,
@@ -205,7 +206,7 @@ def bermudan_section(args, torch, O, configs, pipeline, ev):
             "mfma_useful_tflops": mfma_flops / (regress * 1e-3) / 1e12,
         }, "ls_first_pass_gpu", P),
     }
-    with_mfma(res["roofline"], "k_ls_first_pass", P, regress)
+    with_mfma(res["roofline"], "ls_first_pass_gpu", P, regress)
     if not args.no_error_check:
         ints = O.scramble(O.sobol_ints(P, w.dims, 0), w.scramblers, w.dims)
         z = O.bridge_transform(O.invnorm_u32(ints), S, w.factors)

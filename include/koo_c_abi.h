@@ -539,7 +539,11 @@ int bermudan_paths_gpu(int paths, int rates, int factors, const float* variates,
  * FP64 and solved by FP64 Cholesky.  coefficients (device, E x
  * basis_count(max_vars)) apply to x~; price_out (device double) is the
  * first-pass price.  vars_each_step is a HOST array; cash_flows may be NULL
- * (no flows); workspace holds ls_workspace(paths) bytes.  max_vars <= 4. */
+ * (no flows); workspace holds ls_workspace(paths) bytes.  max_vars <= 4.
+ * Runs one launch per date and one for the price; with the environment
+ * variable KOO_LS_LAUNCH_PER_DATE=0 the whole pass is one persistent launch
+ * (bases of at most 7 functions, blocks all resident), with the same
+ * results bit for bit. */
 int ls_workspace(int paths, size_t* bytes);
 int ls_first_pass_gpu(int n_ex, const float* basis_variables, const int* vars_each_step,
                       int max_vars, const float* cash_flows, const float* exercise_values,
