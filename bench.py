@@ -286,7 +286,7 @@ def portfolio_section(args, torch, configs, pipeline, ev):
     measured faster: the whole portfolio in one persistent launch (teams of
     resident blocks per product), or the generators over four streams then
     the batched launches (path quantities per rate class, the LS first
-    passes in one persistent launch, frozen forward and adjoint backward per
+    passes one launch per date rank, frozen forward and adjoint backward per
     rate class) -- captured into one HIP graph with the autotuner's
     checkpoint interval; one step = one graph replay (first-pass price,
     frozen price and 2N Greeks for every product)."""
