@@ -245,6 +245,14 @@ fail:
   }
 }
 
+// Frozen-strategy pass: each path walks the dates until it exercises.  The
+// walk stops at a data-dependent date, so loading a date's inputs only after
+// the previous decision would make a chain of dependent memory round trips;
+// the inputs of kSecondChunk dates are loaded together ahead of their
+// decisions instead (a path that stops early leaves the rest unused).  Per
+// path the same operations in the same order as a date-by-date walk.
+constexpr int kSecondChunk = 4;
+
 __global__ void __launch_bounds__(kLsThreads)
     k_ls_second(const AllDates dv, const float* __restrict__ bv,
                 const float* __restrict__ cash, const float* __restrict__ ev,
@@ -256,18 +264,37 @@ __global__ void __launch_bounds__(kLsThreads)
   double s = 0.0;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < d->P; p += gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int e = 0; e < d->E; ++e) {
-      const size_t o = static_cast<size_t>(e) * P + p;
-      const float cont = continuation(bv, static_cast<size_t>(e) * d->max_vars, P, p, d->v[e],
-                                      d->k[e], d->cross != 0, d->sh[e], d->sc[e],
-                                      coeffs + static_cast<size_t>(e) * d->maxd);
-      const float exv = ev[o];
-      const bool allowed = !d->itm_only || exv > 0.0f;
-      if (allowed && cont - exv < 0.0f) {
-        acc += static_cast<double>(exv) / static_cast<double>(num[o]);
-        break;
+    bool done = false;
+    for (int e0 = 0; e0 < d->E && !done; e0 += kSecondChunk) {
+      float xr[kSecondChunk][kMaxBasisVars], exv[kSecondChunk], nm[kSecondChunk],
+          cf[kSecondChunk];
+#pragma unroll
+      for (int c = 0; c < kSecondChunk; ++c) {
+        const int e = e0 + c;
+        const bool in = e < d->E;
+        const size_t o = static_cast<size_t>(in ? e : 0) * P + p;
+        const int v = in ? d->v[e] : 0;
+#pragma unroll
+        for (int i = 0; i < kMaxBasisVars; ++i)
+          xr[c][i] = i < v ? bv[(static_cast<size_t>(e) * d->max_vars + i) * P + p] : 0.0f;
+        exv[c] = in ? ev[o] : 0.0f;
+        nm[c] = in ? num[o] : 1.0f;
+        cf[c] = (in && cash) ? cash[o] : 0.0f;
       }
-      if (cash) acc += static_cast<double>(cash[o]) / static_cast<double>(num[o]);
+#pragma unroll
+      for (int c = 0; c < kSecondChunk; ++c) {
+        const int e = e0 + c;
+        if (done || e >= d->E) break;
+        const float cont = continuation_raw(xr[c], d->v[e], d->k[e], d->cross != 0, d->sh[e],
+                                            d->sc[e], coeffs + static_cast<size_t>(e) * d->maxd);
+        const bool allowed = !d->itm_only || exv[c] > 0.0f;
+        if (allowed && cont - exv[c] < 0.0f) {
+          acc += static_cast<double>(exv[c]) / static_cast<double>(nm[c]);
+          done = true;
+        } else if (cash) {
+          acc += static_cast<double>(cf[c]) / static_cast<double>(nm[c]);
+        }
+      }
     }
     s += acc;
   }
@@ -853,7 +880,9 @@ extern "C" int ls_second_pass_gpu(int n_ex, const float* basis_variables, const 
     fill_norm(var_shift, var_scale, e, vars_each_step[e], max_vars, d.sh[e], d.sc[e]);
   }
   hipStream_t st = as_stream(stream);
-  const int nb = ls_blocks(paths);
+  // one lane per path up to 8 blocks per CU (the workspace's partial region
+  // holds kMaxGram^2 doubles per ls_blocks block, more than these blocks)
+  const int nb = static_cast<int>(grid_for(paths, kLsThreads, 8 * device_cu_count()));
   hipLaunchKernelGGL(k_ls_second, dim3(nb), dim3(kLsThreads), 0, st, d, basis_variables,
                      cash_flows, exercise_values, numeraires, coefficients, part);
   hipLaunchKernelGGL(k_finalize_cols, dim3(1), dim3(256), 0, st, part, nb, 1,

@@ -94,17 +94,29 @@ struct DateArgs {
 
 // Continuation value at a date for one path: basis of the normalised
 // variables dotted with that date's coefficients (FP64 in HBM, FP32 here).
-__device__ __forceinline__ float continuation(const float* __restrict__ bv, size_t row0, size_t P,
-                                              int p, int v, int k, bool cross, const float* sh,
-                                              const float* sc, const double* __restrict__ beta) {
+// The same from the date's variables already loaded (raw, before
+// normalisation; entries from v on are ignored).
+__device__ __forceinline__ float continuation_raw(const float (&xr)[kMaxBasisVars], int v, int k,
+                                                  bool cross, const float* sh, const float* sc,
+                                                  const double* __restrict__ beta) {
   float x[kMaxBasisVars] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int i = 0; i < kMaxBasisVars; ++i)
-    if (i < v) x[i] = (bv[(row0 + i) * P + p] - sh[i]) * sc[i];
+    if (i < v) x[i] = (xr[i] - sh[i]) * sc[i];
   float c = 0.0f;
   for (int j = 0; j < k; ++j)
     c += basis_value(j, v, cross, x[0], x[1], x[2], x[3]) * static_cast<float>(beta[j]);
   return c;
+}
+
+__device__ __forceinline__ float continuation(const float* __restrict__ bv, size_t row0, size_t P,
+                                              int p, int v, int k, bool cross, const float* sh,
+                                              const float* sc, const double* __restrict__ beta) {
+  float xr[kMaxBasisVars] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < kMaxBasisVars; ++i)
+    if (i < v) xr[i] = bv[(row0 + i) * P + p];
+  return continuation_raw(xr, v, k, cross, sh, sc, beta);
 }
 
 // The same from variables already loaded (raw, before normalisation) and
