@@ -101,10 +101,10 @@ def test_greeks_vs_reference_gold_bumped_prices(torch_cuda):
     computed by the reference's own gold pipeline (tests/gold_chain.py) on
     the same Sobol points (common random numbers): bump f_i(0), and sigma_i
     with the factor loadings held (row i of A scaled by sigma_i' / sigma_i).
-    The gold prices were computed in the build container and stored in
+    The gold prices were computed where the gold library builds and stored in
     tests/golden/caplet_gold_bumped.npz (tests/golden/make_golden.py).
-    1e-4 relative, the north-star bar; the unbumped gold price within 1e-5 of
-    the GPU price."""
+    1e-4 relative, the Greek accuracy the path targets; the unbumped gold
+    price within 1e-5 of the GPU price."""
     g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
                              "caplet_gold_bumped.npz"))
     w = configs.caplet_portfolio(paths=int(g["paths"]))
