@@ -364,13 +364,88 @@ __global__ void __launch_bounds__(kMultiThreads)
 // is below tol times the largest, the truncated solve keeps every direction
 // and equals the Cholesky solution.  Otherwise the truncated Jacobi solve.
 // Single lane; A, V are 16 x 16 LDS scratch, b and z 16.
+// The Cholesky branch of solve_multi with the factor in registers (K known
+// at compile time): the same operations in the same order (divisions by the
+// pivots, not reciprocals), so the same bits.  Returns false where
+// solve_multi goes on to the truncated eigen-solve.
+template <int K>
+__device__ bool cholesky_multi_reg(const double* G, int ld, double inv, double* beta) {
+  double L[K][K];
+  double trM = 0.0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    trM += G[i * ld + i] * inv;
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = G[i * ld + j] * inv;
+#pragma unroll
+      for (int q = 0; q < j; ++q) s -= L[i][q] * L[j][q];
+      if (i == j) {
+        if (!(s > 0.0)) return false;
+        L[i][i] = sqrt(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  }
+  // trace(M^-1) = ||L^-1||_F^2, column by column of L^-1
+  double trInv = 0.0;
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    double z[K];
+#pragma unroll
+    for (int i = c; i < K; ++i) {
+      double s = i == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = c; q < i; ++q) s -= L[i][q] * z[q];
+      z[i] = s / L[i][i];
+      trInv += z[i] * z[i];
+    }
+  }
+  if (!(trInv * kSvdTolerance * trM <= 1.0)) return false;
+  double z[K], x[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    double s = G[i * ld + K] * inv;
+#pragma unroll
+    for (int q = 0; q < i; ++q) s -= L[i][q] * z[q];
+    z[i] = s / L[i][i];
+  }
+#pragma unroll
+  for (int i = K - 1; i >= 0; --i) {
+    double s = z[i];
+#pragma unroll
+    for (int q = i + 1; q < K; ++q) s -= L[q][i] * x[q];
+    x[i] = s / L[i][i];
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i) beta[i] = x[i];
+  return true;
+}
+
 __device__ void solve_multi(const double* G, int k, double n, double* beta, double* A, double* V,
                             double* b, double* z) {
   const int ld = k + 1;
   const double inv = 1.0 / n;
+  // the basis sizes of one or two variables and three plain: the Cholesky
+  // branch in registers, else straight to the eigen-solve below
+  bool reg = true, done = false;
+  switch (k) {
+    case 1: done = cholesky_multi_reg<1>(G, ld, inv, beta); break;
+    case 3: done = cholesky_multi_reg<3>(G, ld, inv, beta); break;
+    case 5: done = cholesky_multi_reg<5>(G, ld, inv, beta); break;
+    case 6: done = cholesky_multi_reg<6>(G, ld, inv, beta); break;
+    case 7: done = cholesky_multi_reg<7>(G, ld, inv, beta); break;
+    default: reg = false; break;
+  }
+  if (done) return;
   for (int i = 0; i < k; ++i) {
     for (int j = 0; j < k; ++j) A[i * 16 + j] = G[i * ld + j] * inv;
     b[i] = G[i * ld + k] * inv;
+  }
+  if (reg) {
+    jacobi_truncated(k, A, V, b, kSvdTolerance, beta);
+    return;
   }
   // Cholesky into V (lower triangle)
   bool ok = true;
