@@ -744,6 +744,7 @@ __device__ __forceinline__ void ladder_sort(int n, const unsigned long long* tb,
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const unsigned long long x = tb[t];
     int r = 0;
+#pragma unroll 8
     for (int u = 0; u < n; ++u) r += (tb[u] < x || (tb[u] == x && u < t)) ? 1 : 0;
     ts[r] = x;
     if (rank) rank[t] = r;
@@ -920,8 +921,16 @@ __device__ __forceinline__ void multi_select_body(const MultiArgs a, int depth, 
   __syncthreads();
   const int n = ladder_build(a, n_pre, sums, v, tb);
   ladder_sort(n, tb, ts, rank);
-  if (threadIdx.x == 0)
-    for (int k = 1; k < n; ++k) cnt[k] += cnt[k - 1];
+  if (threadIdx.x == 0) {
+    // running count in a register, so the bins' loads are independent of
+    // the additions (the counts are integers, exact in FP64)
+    double run = cnt[0];
+#pragma unroll 8
+    for (int k = 1; k < n; ++k) {
+      run += cnt[k];
+      cnt[k] = run;
+    }
+  }
   __syncthreads();
   // exact count at ladder value t
   auto C = [&](int t) -> long long {
@@ -948,6 +957,18 @@ __device__ __forceinline__ void multi_select_body(const MultiArgs a, int depth, 
     double guess = v[0];
     long long count = C(0);
     int it = 0, j = 0;
+    const int lane = threadIdx.x & 63;
+    static_assert(kLadder == 64, "one ladder step per lane");
+    if (count < lowerT) {
+      // the growth loop's steps inside the ladder at once: it stops at the
+      // first j >= 1 whose count reaches lowerT (one ballot per wavefront,
+      // the same in every wavefront), else it has walked the whole ladder
+      const unsigned long long hit = __ballot(lane >= 1 && C(lane) >= lowerT);
+      j = hit ? __builtin_ctzll(hit) : L - 1;
+      it = j;
+      guess = v[j];
+      count = C(j);
+    }
     while (count < lowerT && it++ < kSelectMaxSteps) {
       ++j;
       if (j < L) {
@@ -981,6 +1002,15 @@ __device__ __forceinline__ void multi_select_body(const MultiArgs a, int depth, 
       }
       it = 0;
       int i = 0;
+      if (j == 0 && count > upperT) {
+        // the shrink loop's steps inside the ladder at once, as above: the
+        // first i >= 1 whose count falls to upperT, else the whole ladder
+        const unsigned long long hit = __ballot(lane >= 1 && C(L + lane) <= upperT);
+        i = hit ? __builtin_ctzll(hit) : L - 1;
+        it = i;
+        low = v[L + i];
+        count = C(L + i);
+      }
       while (count > upperT && it++ < kSelectMaxSteps) {
         ++i;
         if (j == 0 && i < L) {
@@ -1090,8 +1120,35 @@ __device__ __forceinline__ void multi_tree_body(const MultiArgs a, int depth, co
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0)  // inclusive prefix over the in-order bins
-    for (int t = 1; t < kTreeNodes; ++t) hist[t] += hist[t - 1];
+  if (threadIdx.x < 64) {
+    // inclusive prefix over the in-order bins on one wavefront: each lane
+    // sums a run of kScanRun bins, the lanes' sums are scanned across the
+    // wavefront, then each lane writes its run's prefixes (integer counts,
+    // so the order of the additions does not matter)
+    constexpr int kScanRun = (kTreeNodes + 63) / 64;
+    const int lane = threadIdx.x;
+    int v[kScanRun];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < kScanRun; ++q) {
+      const int t = lane * kScanRun + q;
+      v[q] = t < kTreeNodes ? hist[t] : 0;
+      sum += v[q];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    int run = incl - sum;
+#pragma unroll
+    for (int q = 0; q < kScanRun; ++q) {
+      const int t = lane * kScanRun + q;
+      run += v[q];
+      if (t < kTreeNodes) hist[t] = run;
+    }
+  }
   __syncthreads();
   for (int t = threadIdx.x; t < kTreeNodes; t += blockDim.x)
     st_ag(cum_part + static_cast<size_t>(t) * nblk + blk, hist[t]);
