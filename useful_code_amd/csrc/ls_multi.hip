@@ -1162,21 +1162,32 @@ __global__ void __launch_bounds__(kMultiThreads)
 }
 
 
-// Block-wide sum of cum_part[rank * nb + b] over the blocks.
-__device__ long long tree_count(const int* __restrict__ cum_part, int nb, int rank,
-                                long long* red) {
-  long long c = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    c += ld_ag(cum_part + static_cast<size_t>(rank) * nb + b);
+// Sums over the blocks of the counts of the tree nodes in the kSubtreeLevels
+// levels of the subtree rooted at `root` (heap order; nodes past the tree
+// are skipped) into sub[local], local = 2^l - 1 + j for node root 2^l + j:
+// one wavefront per node, all in flight at once.  Whole block, barrier.
+constexpr int kSubtreeLevels = 4;
+constexpr int kSubtreeNodes = (1 << kSubtreeLevels) - 1;
+static_assert(kSubtreeNodes <= kSearchThreads / 64 && kSubtreeNodes <= kMultiThreads / 64,
+              "one wavefront per subtree node");
+
+__device__ void subtree_counts(const int* __restrict__ cum_part, int nb, int root,
+                               long long* sub) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();  // every lane has read the previous subtree's counts
+  if (w < kSubtreeNodes) {
+    const int l = 31 - __builtin_clz(w + 1);   // level below the root
+    const int node = (root << l) + (w + 1 - (1 << l));
+    long long c = 0;
+    if (node <= kTreeNodes) {
+      const size_t row = static_cast<size_t>(tree_rank(node)) * nb;
+      for (int b = lane; b < nb; b += 64) c += ld_ag(cum_part + row + b);
+    }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) sub[w] = c;
+  }
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-  __syncthreads();
-  long long tot = 0;
-  for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) tot += red[w];
-  __syncthreads();
-  return tot;
 }
 
 // The gold's bisection loop on the tree counts.  Count at a guess g in
@@ -1190,6 +1201,7 @@ __device__ __forceinline__ void multi_bisect_body(const MultiArgs a, int depth, 
                    int nb, double* __restrict__ lower, double* __restrict__ upper,
                    int last_stage, int blk, int nblk) {
   __shared__ long long red[16];
+  __shared__ long long sub[kSubtreeNodes];
   __shared__ KeyExtent ext;
   if (!ld_ag(&state[depth].active) || !ld_ag(&sel->bisect)) return;
   reduce_extents(ext_part, nb, &ext);
@@ -1202,8 +1214,19 @@ __device__ __forceinline__ void multi_bisect_body(const MultiArgs a, int depth, 
   int it = ld_ag(&sel->it);
   int node = 1;
   const long long c0 = c_low;  // the tree counts keys in (low, high] of this stage
+  // the counts of a subtree of nodes are summed together, so kSubtreeLevels
+  // steps of the walk take one round of cross-block loads
+  int sub_root = 0;
   auto count_at = [&](double g) -> long long {
-    if (node <= kTreeNodes) return c0 + tree_count(cum_part, nb, tree_rank(node), red);
+    if (node <= kTreeNodes) {
+      const int dl = sub_root ? (31 - __builtin_clz(node)) - (31 - __builtin_clz(sub_root)) : -1;
+      if (dl < 0 || dl >= kSubtreeLevels || (node >> dl) != sub_root) {
+        sub_root = node;
+        subtree_counts(cum_part, nb, sub_root, sub);
+        return c0 + sub[0];
+      }
+      return c0 + sub[(1 << dl) - 1 + (node - (sub_root << dl))];
+    }
     if (!(g >= 0.0)) return 0;
     const long long c = extent_count(ext, g);
     return c >= 0 ? c : block_count(keys, a.P, key_of(g), red);
