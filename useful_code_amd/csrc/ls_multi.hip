@@ -690,8 +690,30 @@ __device__ __forceinline__ int ladder_count(const MultiArgs& a, long long n_pre)
 // (i < L, by repeated multiplication as the gold's shrink loop), v[2L + j -
 // 1] = G_j m (1 <= j < L); standard-deviation selector: v[0] = sd_guess sd
 // with the open bound.  Whole block; returns n.
+// The growth steps v[0..L) (G_0 divided by m again and again) and the shrink
+// steps v[L..2L) (G_0 m multiplied by m again and again), as ladder_build
+// forms them.  One lane; sums in LDS.
+__device__ void ladder_chains(const MultiArgs& a, long long n_pre, const double* sums, double* v) {
+  const int L = kLadder;
+  const double sd = net_sd(sums, n_pre);
+  double g = sd * a.sd_guess;
+  v[0] = g;
+  for (int j = 1; j < L; ++j) {
+    g /= a.multiplier;
+    v[j] = g;
+  }
+  double lo = (sd * a.sd_guess) * a.multiplier;
+  v[L] = lo;
+  for (int i = 1; i < L; ++i) {
+    lo *= a.multiplier;
+    v[L + i] = lo;
+  }
+}
+
+// chains_done: ladder_chains has already filled v[0..2L) (the full ladder
+// case only).
 __device__ int ladder_build(const MultiArgs& a, long long n_pre, const double* sums, double* v,
-                            unsigned long long* tb) {
+                            unsigned long long* tb, bool chains_done = false) {
   const int t = threadIdx.x, L = kLadder;
   const double sd = net_sd(sums, n_pre);
   const double np = static_cast<double>(n_pre);
@@ -713,7 +735,8 @@ __device__ int ladder_build(const MultiArgs& a, long long n_pre, const double* s
     }
   } else {
     n = 3 * L - 1;
-    if (t == 0) {
+    if (chains_done) {
+    } else if (t == 0) {
       double g = sd * a.sd_guess;
       v[0] = g;
       for (int j = 1; j < L; ++j) {
@@ -914,12 +937,16 @@ __device__ __forceinline__ void multi_select_body(const MultiArgs a, int depth, 
   const long long n_pre = ld_ag(&state[depth].n_pre);
   // the three cross-block sums at once, on disjoint wavefronts: the ladder's
   // bin counts (cnt[k]: members in bin k), the net-value sums, the extents
+  // (the net-value wavefront then builds the ladder's two serial chains
+  // while the bin counts are still being summed)
   const int nw = blockDim.x >> 6;
-  reduce_partials_waves<true>(hist_part, nh, ladder_count(a, n_pre), cnt, 0, nw - 2);
+  const int nlad = ladder_count(a, n_pre);
+  reduce_partials_waves<true>(hist_part, nh, nlad, cnt, 0, nw - 2);
   reduce_partials_waves<true>(net_part, nb, 2, sums, nw - 2, 1);
+  if (nlad > 1 && threadIdx.x == (nw - 2) * 64) ladder_chains(a, n_pre, sums, v);
   reduce_extents_wave(ext_part, nh, &ext, nw - 1);
   __syncthreads();
-  const int n = ladder_build(a, n_pre, sums, v, tb);
+  const int n = ladder_build(a, n_pre, sums, v, tb, nlad > 1);
   ladder_sort(n, tb, ts, rank);
   if (threadIdx.x == 0) {
     // running count in a register, so the bins' loads are independent of
