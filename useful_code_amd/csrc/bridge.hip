@@ -93,24 +93,43 @@ struct BridgeOrder {
   }
 };
 
+// The construction weights of step i, {rw, lw, sd}, from a copy of the
+// tables in LDS ({rw, lw, sd, 0} per step) read at an index offset the
+// compiler cannot see through.  A loop over chunks that reads the
+// kernel-argument tables gets all 3 S weights hoisted into scalar registers,
+// more than there are (spilled through vector lanes); the LDS copy is read
+// where each weight is used.
+struct LdsWeights {
+  const f32x4* w;  // LDS
+  int z;           // zero, opaque to the compiler
+  __device__ __forceinline__ void get(int i, float& rw, float& lw, float& sd) const {
+    const f32x4 q = w[i + z];
+    rw = q.x;
+    lw = q.y;
+    sd = q.z;
+  }
+};
+
 // V is float (one path) or f32x4 (four paths; the scalar weights broadcast).
-template <int S, int I, class V, class Z>
-__device__ __forceinline__ void bridge_point(V (&w)[S], const BridgeTables& T, Z& z) {
+template <int S, int I, class V, class Z, class W>
+__device__ __forceinline__ void bridge_point(V (&w)[S], const W& wt, Z& z) {
   constexpr BridgeOrder<S> M{};
+  float rw, lw, sd;
+  wt.get(I, rw, lw, sd);
   if constexpr (I == 0) {
-    w[S - 1] = T.sd[0] * z(0);
+    w[S - 1] = sd * z(0);
   } else {
     constexpr int j = M.left[I], k = M.right[I], l = M.bridge[I];
-    V v = T.rw[I] * w[k];
-    if constexpr (j != 0) v = bridge_fma(T.lw[I], w[j - 1], v);
-    w[l] = bridge_fma(T.sd[I], z(I), v);
+    V v = rw * w[k];
+    if constexpr (j != 0) v = bridge_fma(lw, w[j - 1], v);
+    w[l] = bridge_fma(sd, z(I), v);
   }
 }
 
-template <int S, class V, class Z, int... I>
-__device__ __forceinline__ void bridge_points(V (&w)[S], const BridgeTables& T, Z& z,
+template <int S, class V, class Z, class W, int... I>
+__device__ __forceinline__ void bridge_points(V (&w)[S], const W& wt, Z& z,
                                               std::integer_sequence<int, I...>) {
-  (bridge_point<S, I>(w, T, z), ...);
+  (bridge_point<S, I>(w, wt, z), ...);
 }
 
 // In-place form for variates that come from memory: w[bridge(i)] holds
@@ -141,16 +160,18 @@ __device__ __forceinline__ void bridge_points_loaded(V (&w)[S], const BridgeTabl
 
 // Builds the lane's path from z(i) (bridge variate i, i a constant) and
 // stores its increments to (S,F,P) at path p when `live`.
-template <int S, class Z>
-__device__ __forceinline__ void bridge_store(const BridgeTables& T, Z z, int factors, int f,
-                                             int paths, int p, bool live,
-                                             float* __restrict__ out) {
+template <int S, class Z, class W>
+__device__ __forceinline__ void bridge_store(const W& wt, Z z, int factors, int f, int paths,
+                                             int p, bool live, float* __restrict__ out) {
   float w[S];
-  bridge_points<S>(w, T, z, std::make_integer_sequence<int, S>{});
+  bridge_points<S>(w, wt, z, std::make_integer_sequence<int, S>{});
+  // the step index offset by the weights' opaque zero, so the S step
+  // offsets are formed where they are stored rather than hoisted into
+  // scalar registers across the chunk loop
   float prev = 0.0f;
 #pragma unroll
   for (int i = 0; i < S; ++i) {
-    if (live) out[(static_cast<size_t>(i) * factors + f) * paths + p] = w[i] - prev;
+    if (live) out[(static_cast<size_t>(i + wt.z) * factors + f) * paths + p] = w[i] - prev;
     prev = w[i];
   }
 }
@@ -216,10 +237,12 @@ __global__ void __launch_bounds__(kBridgeThreads)
                        float* __restrict__ out) {
   __shared__ uint32_t vall[S * 32];  // direction numbers [bridge variate][bit]
   __shared__ uint32_t tlo[S * 64];   // T(l) [bridge variate][l]
+  __shared__ f32x4 wts[S];           // construction weights {rw, lw, sd, 0}
   const int f = blockIdx.y;
   const int* idx = alloc + f * S;
   for (int q = threadIdx.x; q < S * 32; q += blockDim.x)
     vall[q] = dirs[static_cast<size_t>(idx[q >> 5]) * 32 + (q & 31)];
+  for (int q = threadIdx.x; q < S; q += blockDim.x) wts[q] = f32x4{T.rw[q], T.lw[q], T.sd[q], 0.0f};
   __syncthreads();
   for (int q = threadIdx.x; q < S * 64; q += blockDim.x) {
     const uint32_t l = static_cast<uint32_t>(q & 63), g = l ^ (l >> 1);
@@ -266,8 +289,12 @@ __global__ void __launch_bounds__(kBridgeThreads)
       return invnorm_u32((upper ? xb : xa) ^ tlo[i * 64 + l]);
     };
     // every lane computes (the readlanes see whole-wave values); only the
-    // lanes inside the path range store
-    bridge_store<S>(T, z, factors, f, paths, p, p < paths, out);
+    // lanes inside the path range store.  The weights come from LDS at an
+    // offset made opaque each chunk (a zero from s_mov), so their reads stay
+    // in the loop.
+    int zero;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+    bridge_store<S>(LdsWeights{wts, zero}, z, factors, f, paths, p, p < paths, out);
     xc = xn;
   }
 }
