@@ -64,8 +64,9 @@ __global__ void __launch_bounds__(kBridgeThreads)
 // Register-resident bridge for S <= kRegSteps.  The construction order
 // depends on S only, so for a compile-time S every index is a constant and
 // the S bridge points of a lane stay in VGPRs (no LDS path storage, which
-// held occupancy to one or two waves per SIMD).  The weights stay in the
-// kernel arguments (uniform, scalar loads).
+// held occupancy to one or two waves per SIMD).  The stand-alone bridge
+// reads its weights from the kernel arguments (uniform, scalar loads); the
+// fused generator, which loops over chunks, from an LDS copy (LdsWeights).
 // ---------------------------------------------------------------------------
 constexpr int kRegSteps = 64;
 
