@@ -384,7 +384,9 @@ int koo_sum_f64(const double* in, long long n, double* workspace, double* out, v
  * Forward: price (device double) and, when checkpoints != NULL, the
  * log-rate state at the start of every checkpoint_every-th step plus the
  * numeraire there (with checkpoint_every == 1 also each step's final drift
- * sums, so the backward sweep recomputes nothing).  Backward: one reverse sweep per path from those
+ * sums, so the backward sweep recomputes nothing), then the model's step
+ * table (fixed drifts and drift constants, rates*(rates+1) floats) that the
+ * backward reads.  Backward: one reverse sweep per path from those
  * checkpoints (recomputing the steps in between into segment_scratch), giving
  * greeks_out[0..N) = dV/df_i(0) and greeks_out[N..2N) = dV/dsigma_i with
  * A(s,i,f) proportional to sigma_i.  Sizes of all device buffers come from

@@ -136,10 +136,11 @@ def test_empty_work_is_a_no_op():
 
 def test_workspace_sizes():
     ck, sc, pa = koo.lmm_price_workspace(2**20, 40, 5, 8)
-    assert ck == 4 * 5 * 41 * 2**20
+    table = 4 * 40 * 41  # step table behind the checkpoints
+    assert ck == 4 * 5 * 41 * 2**20 + table
     assert sc == 4 * 8 * (41 + 10) * 2**20
     assert pa >= 8 * (2**20 // 256) * 80
     ck1, sc1, _ = koo.lmm_price_workspace(2**20, 40, 5, 1)
-    assert ck1 == 4 * 40 * (41 + 10) * 2**20 and sc1 == 4
+    assert ck1 == 4 * 40 * (41 + 10) * 2**20 + table and sc1 == 4
     assert koo.ls_workspace(2**18) >= 4 * 2**18 + 8 * 256
     assert koo.outer_product_workspace_floats(2**18) >= 2 * 256

@@ -208,8 +208,8 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
         ep[f] = sums[static_cast<size_t>(F + f) * P];
       }
       reverse_step<F, kStatePrefetch>(
-          m, s, ctab + s * N, [&](int r) { return src[static_cast<size_t>(r) * P]; }, lam, bd, z,
-          e, ep,
+          m, s, ctab + s * N, [&](int r) { return src[static_cast<size_t>(r) * P]; },
+          LdsLam{lam, bd}, z, e, ep,
           [&](float& lp, float xs, float fs, float inv3) {
             (void)fs;
             if (s < tau) lp -= m.taus[s] * inv3 * V * xs;  // fixing inside B_tau

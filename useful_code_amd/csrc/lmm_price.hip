@@ -57,6 +57,12 @@ __global__ void __launch_bounds__(256)
   // With k == 1 every step is a checkpoint and also keeps its final drift
   // sums (rows N+1..N+2F), so the reverse sweep needs no recomputation.
   const int rows = m.k == 1 ? m.N + 1 + 2 * F : m.N + 1;
+  // Block 0 also copies the step table behind the checkpoints, where the
+  // reverse sweep reads it (the same bits the forward stepped with).
+  if (ckpt && blockIdx.x == 0) {
+    float* tab = ckpt + static_cast<size_t>((m.N + m.k - 1) / m.k) * rows * P;
+    for (int i = threadIdx.x; i < step_table_floats(m.N); i += bd) tab[i] = ctab[i];
+  }
   for (int s = 0; s < m.N; ++s) {
     const bool save = ckpt && valid && (s % m.k) == 0;
     float* c = save ? ckpt + static_cast<size_t>(s / m.k) * rows * P + p : nullptr;
@@ -92,21 +98,53 @@ __global__ void __launch_bounds__(256)
 // Backward.  scratch slot j (j < k) holds rows [0,N): start log-rates of step
 // s0+j, row N: 1/B at that step, rows N+1..N+2F: final sums e, e^ of step
 // s0+j.  Slot 0's state rows are not used (the checkpoint is read instead).
-// LDS: adjoints lam [N][lanes], fixed drifts [N][N], vega partial sums
-// [wave][N][8] (each wavefront folds its 64 lanes to 8 with DPP and adds
-// those into its own slots).
+// The step table (fixed drifts [N][N], then kr [N]) is the forward's copy
+// behind the checkpoints, read with scalar loads.  LDS: adjoints of rates
+// 1..N-1 [N-1][lanes] (rate 0 only changes in step 0 and lives in a
+// register), one vega partial sum per wavefront and rate [wave][N] (each
+// wavefront folds its 64 lanes to one with DPP).  At 40 rates and 256 lanes
+// that is 40 608 bytes with the block-sum words: four blocks per CU.
 // ----------------------------------------------------------------------
-constexpr int kVegaSlots = 8;
+
+// Log-rate adjoints: rate 0 in a register, rate r >= 1 at p[(r - 1) * bd].
+// The rate index is uniform, so the branch is a scalar one.
+struct LamReg0 {
+  float* p;
+  int bd;
+  float& l0;
+  __device__ __forceinline__ float get(int r) const {
+    if (r > 0) return p[(r - 1) * bd];
+    return l0;
+  }
+  __device__ __forceinline__ void set(int r, float v) const {
+    if (r > 0) p[(r - 1) * bd] = v;
+    else l0 = v;
+  }
+};
+
+// Sum over the 64 lanes of a wavefront, left in lane 63 (row butterflies and
+// mirrors, then the row broadcasts 15 and 31).  All 64 lanes must be active.
+__device__ __forceinline__ float wave_sum_to_lane63(float w) {
+  w += dpp_f32_quad_xor1(w);
+  w += dpp_f32_quad_xor2(w);
+  w += dpp_f32_row_half_mirror(w);
+  w += dpp_f32_row_mirror(w);
+  w += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(w), 0x142, 0xA, 0xF, false));
+  w += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(w), 0x143, 0xC, 0xF, false));
+  return w;
+}
+
 // Depth of the queue of start log-rates read from checkpoints or scratch in
 // the backward kernel (HBM latency over the per-rate work of the resident
-// wavefronts).  Measured at 2^20 paths, 40 rates, 5 factors: 4.39 ms at 4,
-// 3.97 at 8, 3.89 at 12 and 16 (three 256-lane blocks per CU, LDS-bound).
+// wavefronts).  Measured at 2^20 paths, 40 rates, 5 factors with three
+// 256-lane blocks per CU: 4.39 ms at 4, 3.97 at 8, 3.89 at 12 and 16.
 constexpr int kStatePrefetch = 12;
 
 template <int F>
 __global__ void __launch_bounds__(256)
     k_price_bwd(KOO_MODEL_PARAMS, const float* __restrict__ vols, const float* __restrict__ ckpt,
-                float* __restrict__ scratch, double* __restrict__ part) {
+                const float* __restrict__ ctab, float* __restrict__ scratch,
+                double* __restrict__ part) {
   KOO_MODEL_LOCAL;
   extern __shared__ float lds[];
   __shared__ double red[4];
@@ -118,13 +156,12 @@ __global__ void __launch_bounds__(256)
   const int pc = valid ? p : m.P - 1;
   const int N = m.N;
   const size_t P = static_cast<size_t>(m.P);
-  float* lam = lds + threadIdx.x;                            // lam[r * bd]
-  float* ctab = lds + static_cast<size_t>(N) * bd;           // [N][N]
-  float* vslots = ctab + step_table_floats(N);         // [nw][N][8]
-  float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 3);
-  fill_step_table<F>(m, ctab);
-  for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
-  for (int i = threadIdx.x; i < nw * N * kVegaSlots; i += bd) vslots[i] = 0.0f;
+  float lam0 = 0.0f;
+  const LamReg0 lam{lds + threadIdx.x, bd, lam0};             // rates 1..N-1: [N-1][bd]
+  float* vslots = lds + static_cast<size_t>(N - 1) * bd;      // [nw][N]
+  float* myslots = vslots + static_cast<size_t>(wave) * N;
+  for (int r = 1; r < N; ++r) lam.set(r, 0.0f);
+  for (int i = threadIdx.x; i < nw * N; i += bd) vslots[i] = 0.0f;
   __syncthreads();
 
   const size_t slot = static_cast<size_t>(N + 1 + 2 * F) * P;
@@ -180,7 +217,7 @@ __global__ void __launch_bounds__(256)
         ep[f] = sums[static_cast<size_t>(F + f) * P];
       }
       reverse_step<F, kStatePrefetch>(
-          m, s, cs, [&](int r) { return src[static_cast<size_t>(r) * P]; }, lam, bd, z, e, ep,
+          m, s, cs, [&](int r) { return src[static_cast<size_t>(r) * P]; }, lam, z, e, ep,
           [&](float& lp, float xs, float fs, float inv3) {
             // The rate fixing at the end of this step: caplet value and the
             // numeraire it enters for later fixings.
@@ -191,37 +228,39 @@ __global__ void __launch_bounds__(256)
             tail += v * invBs;
           },
           [&](int r, float w) {
-            w = valid ? w : 0.0f;
-            w += dpp_f32_quad_xor1(w);
-            w += dpp_f32_quad_xor2(w);
-            w += dpp_f32_row_half_mirror(w);
-            if ((lane & 7) == 0) atomicAdd(myslots + r * kVegaSlots, w);
+            w = wave_sum_to_lane63(valid ? w : 0.0f);
+            if (lane == 63) atomicAdd(myslots + r, w);
           });
     }
   }
   __syncthreads();
   // Deltas: d/df0_r = lam_r / (f0_r + d_r); vegas: slot sums / sigma_r.
   for (int r = 0; r < N; ++r) {
-    const float dl = lam[r * bd] / (m.f0[r] + m.disp[r]);
+    const float dl = lam.get(r) / (m.f0[r] + m.disp[r]);
     block_sum_to(valid ? static_cast<double>(dl) : 0.0, red,
                  part + static_cast<size_t>(blockIdx.x) * 2 * N + r);
   }
   for (int r = threadIdx.x; r < N; r += bd) {
     double t = 0.0;
-    for (int w = 0; w < nw; ++w)
-      for (int q = 0; q < kVegaSlots; ++q)
-        t += vslots[(static_cast<size_t>(w) * N + r) * kVegaSlots + q];
+    for (int w = 0; w < nw; ++w) t += vslots[static_cast<size_t>(w) * N + r];
     part[static_cast<size_t>(blockIdx.x) * 2 * N + N + r] = t / vols[r];
   }
 }
 
 // Lanes per block keeping the dynamic LDS within 60 KiB:
-// forward N*lanes + N*(N+1) floats, backward N*lanes + N*(N+1) + (lanes/64)*N*8.
+// forward N*lanes + N*(N+1) floats, backward (N-1)*lanes + (lanes/64)*N
+// (256 lanes up to 60 rates; 58 keeps a margin).
 int fwd_threads(int N) { return N <= 50 ? 256 : (N <= 75 ? 128 : 64); }
-int bwd_threads(int N) { return N <= 45 ? 256 : (N <= 71 ? 128 : 64); }
+int bwd_threads(int N) { return N <= 58 ? 256 : 128; }
 size_t fwd_lds(int N, int t) { return sizeof(float) * (static_cast<size_t>(N) * t + step_table_floats(N)); }
 size_t bwd_lds(int N, int t) {
-  return sizeof(float) * (static_cast<size_t>(N) * t + step_table_floats(N) + (t / 64) * N * kVegaSlots);
+  return sizeof(float) * (static_cast<size_t>(N - 1) * t + (t / 64) * N);
+}
+// Floats of the checkpoint buffer before the step table the forward copies
+// behind them.
+size_t ckpt_floats(int paths, int N, int F, int k) {
+  const size_t rows = k == 1 ? N + 1 + 2 * F : N + 1;
+  return static_cast<size_t>((N + k - 1) / k) * rows * paths;
 }
 
 int check_model(const char* who, int paths, int rates, int factors, int k, int product) {
@@ -248,9 +287,9 @@ extern "C" int lmm_price_workspace(int paths, int rates, int factors, int checkp
     return e;
   const size_t P = static_cast<size_t>(paths);
   const int k = checkpoint_every;
-  const size_t nseg = (rates + k - 1) / k;
-  const size_t rows = k == 1 ? rates + 1 + 2 * factors : rates + 1;
-  if (checkpoint_bytes) *checkpoint_bytes = sizeof(float) * nseg * rows * P;
+  if (checkpoint_bytes)
+    *checkpoint_bytes =
+        sizeof(float) * (ckpt_floats(paths, rates, factors, k) + step_table_floats(rates));
   // k == 1 needs no recompute scratch; one float keeps the pointer non-null.
   if (scratch_bytes)
     *scratch_bytes = k == 1 ? sizeof(float)
@@ -312,12 +351,13 @@ extern "C" int lmm_price_bwd(int paths, int rates, int factors, int checkpoint_e
   const int threads = bwd_threads(rates);
   const int blocks = (paths + threads - 1) / threads;
   const size_t lds = bwd_lds(rates, threads);
+  const float* tab = checkpoints + ckpt_floats(paths, rates, factors, checkpoint_every);
   hipStream_t st = as_stream(stream);
   switch (factors) {
 #define KOO_CASE(F)                                                                              \
   case F:                                                                                        \
     hipLaunchKernelGGL(k_price_bwd<F>, dim3(blocks), dim3(threads), lds, st, m.z, m.f0, m.taus, \
-                       m.disp, m.A, m.strike, m.P, m.N, m.k, vols, checkpoints,                 \
+                       m.disp, m.A, m.strike, m.P, m.N, m.k, vols, checkpoints, tab,            \
                        segment_scratch, partials);                                              \
     break;
     KOO_CASE(1) KOO_CASE(2) KOO_CASE(3) KOO_CASE(4) KOO_CASE(5) KOO_CASE(6) KOO_CASE(7)

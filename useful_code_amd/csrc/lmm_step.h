@@ -107,23 +107,36 @@ __host__ __device__ constexpr size_t step_table_floats(int N) {
   return static_cast<size_t>(N) * N + N;
 }
 
+// Entry i of the step table (one function for every copy of the table, so
+// an LDS copy and a global copy hold the same bits).
+template <int F>
+__device__ __forceinline__ float step_table_entry(const Model& m, int i) {
+  const int nn = m.N * m.N;
+  if (i >= nn) {
+    const int r = i - nn;
+    return 1.0f / m.taus[r] - m.disp[r];
+  }
+  const float* a = m.A + static_cast<size_t>(i) * F;
+  float c = 0.0f;
+#pragma unroll
+  for (int f = 0; f < F; ++f) c -= 0.5f * a[f] * a[f];
+  return c;
+}
+
 // Every lane of the block takes part; the caller then synchronises.
 template <int F>
 __device__ __forceinline__ void fill_step_table(const Model& m, float* ctab) {
-  const int nn = m.N * m.N;
-  for (int i = threadIdx.x; i < nn + m.N; i += blockDim.x) {
-    if (i >= nn) {
-      const int r = i - nn;
-      ctab[i] = 1.0f / m.taus[r] - m.disp[r];
-      continue;
-    }
-    const float* a = m.A + static_cast<size_t>(i) * F;
-    float c = 0.0f;
-#pragma unroll
-    for (int f = 0; f < F; ++f) c -= 0.5f * a[f] * a[f];
-    ctab[i] = c;
-  }
+  for (int i = threadIdx.x; i < m.N * m.N + m.N; i += blockDim.x)
+    ctab[i] = step_table_entry<F>(m, i);
 }
+
+// Adjoints of the log-rates, lam[r * bd] in LDS (lane-minor rows).
+struct LdsLam {
+  float* p;
+  int bd;
+  __device__ __forceinline__ float get(int r) const { return p[r * bd]; }
+  __device__ __forceinline__ void set(int r, float v) const { p[r * bd] = v; }
+};
 
 // One predictor-corrector step s over rates r >= s.  `load(r)` gives the
 // start log-rate, issued PF rates ahead of its use (PF = 2 for LDS sources; a
@@ -181,18 +194,18 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
   }
 }
 
-// Reverse of PC step s over the rates N-1..s.  On entry lam[r * bd] (LDS)
-// holds the adjoint of the end-of-step log-rate r, on exit that of the
-// start-of-step one.  e / ep are the step's final drift sums (un-accumulated
+// Reverse of PC step s over the rates N-1..s.  On entry lam.get(r) holds
+// the adjoint of the end-of-step log-rate r, on exit that of the
+// start-of-step one (lam.set).  e / ep are the step's final drift sums (un-accumulated
 // here), z its variates, cs row s of the step table; `src(r)` gives the start
 // log-rate r (issued PF rates ahead).  `fixing(lp, x, f, 1/(1+tau f))` adds
 // the product's adjoint of the rate fixing at the end of the step (r == s) to
 // lp; `vega(r, w)` receives w = sum_f Abar(s,r,f) A(s,r,f) for the step, which
 // contracts to scalars the reverse already has (the mu^, mu, dW and
 // fixed-drift terms).  E, E^ are the adjoint running sums.
-template <int F, int PF, class Src, class Fixing, class Vega>
+template <int F, int PF, class Src, class Lam, class Fixing, class Vega>
 __device__ __forceinline__ void reverse_step(const Model& m, int s, const float* cs, Src src,
-                                             float* lam, int bd, const float (&z)[F],
+                                             const Lam& lam, const float (&z)[F],
                                              float (&e)[F], float (&ep)[F], Fixing fixing,
                                              Vega vega) {
   const int N = m.N;
@@ -207,7 +220,7 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
   float q[PF];
 #pragma unroll
   for (int i = 0; i < PF; ++i) q[i] = src(N - 1 - (i < cnt ? i : cnt - 1));
-  float lp_next = lam[(N - 1) * bd];
+  float lp_next = lam.get(N - 1);
   for (int b = 0; b < cnt; b += PF) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
@@ -215,7 +228,7 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
       const int r = N - 1 - b - u;
       const float l = q[u];
       float lp = lp_next;
-      if (r > s) lp_next = lam[(r - 1) * bd];
+      if (r > s) lp_next = lam.get(r - 1);
       float a[F];
 #pragma unroll
       for (int f = 0; f < F; ++f) a[f] = As[r * F + f];
@@ -251,7 +264,7 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
         E[f] += mub * a[f];
         gb += a[f] * E[f];
       }
-      lam[r * bd] = lhat + gb * (g - g * g);
+      lam.set(r, lhat + gb * (g - g * g));
       vega(r, 0.5f * lp * muh + gh * ghb + mub * mu + g * gb + lhat * (dw + 2.0f * c));
 #pragma unroll
       for (int f = 0; f < F; ++f) {
