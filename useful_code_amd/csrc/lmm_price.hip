@@ -137,7 +137,8 @@ __device__ __forceinline__ float wave_sum_to_lane63(float w) {
 // Depth of the queue of start log-rates read from checkpoints or scratch in
 // the backward kernel (HBM latency over the per-rate work of the resident
 // wavefronts).  Measured at 2^20 paths, 40 rates, 5 factors with three
-// 256-lane blocks per CU: 4.39 ms at 4, 3.97 at 8, 3.89 at 12 and 16.
+// 256-lane blocks per CU: 4.39 ms at 4, 3.97 at 8, 3.89 at 12 and 16; with
+// four per CU, 8, 12 and 16 are equal (3.70-3.79 ms).
 constexpr int kStatePrefetch = 12;
 
 template <int F>
