@@ -102,7 +102,9 @@ __global__ void __launch_bounds__(256)
 // behind the checkpoints, read with scalar loads.  LDS: adjoints of rates
 // 1..N-1 [N-1][lanes] (rate 0 only changes in step 0 and lives in a
 // register), one vega partial sum per wavefront and rate [wave][N] (each
-// wavefront folds its 64 lanes to one with DPP).  At 40 rates and 256 lanes
+// wavefront folds its lanes to the four 16-lane row sums with DPP, and the
+// last lane of each row adds its row's sum to the slot: one LDS atomic with
+// four lanes on one address; repeated runs give identical sums).  At 40 rates and 256 lanes
 // that is 40 608 bytes with the block-sum words: four blocks per CU.
 // ----------------------------------------------------------------------
 
@@ -121,18 +123,6 @@ struct LamReg0 {
     else l0 = v;
   }
 };
-
-// Sum over the 64 lanes of a wavefront, left in lane 63 (row butterflies and
-// mirrors, then the row broadcasts 15 and 31).  All 64 lanes must be active.
-__device__ __forceinline__ float wave_sum_to_lane63(float w) {
-  w += dpp_f32_quad_xor1(w);
-  w += dpp_f32_quad_xor2(w);
-  w += dpp_f32_row_half_mirror(w);
-  w += dpp_f32_row_mirror(w);
-  w += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(w), 0x142, 0xA, 0xF, false));
-  w += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(w), 0x143, 0xC, 0xF, false));
-  return w;
-}
 
 // Depth of the queue of start log-rates read from checkpoints or scratch in
 // the backward kernel (HBM latency over the per-rate work of the resident
@@ -229,8 +219,12 @@ __global__ void __launch_bounds__(256)
             tail += v * invBs;
           },
           [&](int r, float w) {
-            w = wave_sum_to_lane63(valid ? w : 0.0f);
-            if (lane == 63) atomicAdd(myslots + r, w);
+            w = valid ? w : 0.0f;
+            w += dpp_f32_quad_xor1(w);
+            w += dpp_f32_quad_xor2(w);
+            w += dpp_f32_row_half_mirror(w);
+            w += dpp_f32_row_mirror(w);
+            if ((lane & 15) == 15) atomicAdd(myslots + r, w);
           });
     }
   }
