@@ -159,7 +159,9 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
 
   const int tau = valid ? tau_in[p] : -1;
   const float V = valid ? vin[p] : 0.0f;
-  const int s_top = wave_max_int(tau);  // the wavefront's last exercise step
+  // The wavefront's last exercise step, made a scalar (the butterfly leaves
+  // it in every lane) so the loops below and their addresses are uniform.
+  const int s_top = __builtin_amdgcn_readfirstlane(wave_max_int(tau));
   const size_t slot = static_cast<size_t>(N + 1 + 2 * F) * P;
   const bool every = m.k == 1;
   for (int seg = s_top < 0 ? -1 : s_top / m.k; seg >= 0; --seg) {

@@ -89,6 +89,16 @@ __device__ __forceinline__ void block_sum_to(double v, double* scratch_lds, doub
   }
 }
 
+// Constant address space (4): the pseudo-roots are read through it in the
+// step loops, since no kernel writes them, so that their uniform reads use
+// the scalar cache also where the model comes from a device descriptor (the
+// batched and persistent kernels), not only from __restrict__ arguments.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KOO_CONSTANT __attribute__((address_space(4)))
+#else
+#define KOO_CONSTANT
+#endif
+
 struct Model {
   const float* z;     // (S,F,P) bridged variates
   const float* f0;    // N
@@ -153,7 +163,8 @@ template <int F, int PF = 2, class Load, class Store>
 __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
                                         const float (&z)[F], float (&e)[F], float (&ep)[F],
                                         Load load, Store store) {
-  const float* As = m.A + s * m.N * F;  // 32-bit: keeps the address scalar
+  // 32-bit offset: keeps the address scalar
+  const float KOO_CONSTANT* As = (const float KOO_CONSTANT*)m.A + s * m.N * F;
   const float* kt = cs + (m.N - s) * m.N;
   const int cnt = m.N - s;
 #pragma unroll
@@ -209,7 +220,8 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
                                              float (&e)[F], float (&ep)[F], Fixing fixing,
                                              Vega vega) {
   const int N = m.N;
-  const float* As = m.A + s * N * F;  // 32-bit: keeps the address scalar
+  // 32-bit offset: keeps the address scalar
+  const float KOO_CONSTANT* As = (const float KOO_CONSTANT*)m.A + s * N * F;
   const float* kt = cs + (N - s) * N;
   const int cnt = N - s;
   float E[F], Eh[F];
