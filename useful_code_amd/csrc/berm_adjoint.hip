@@ -71,7 +71,7 @@ __device__ __forceinline__ const BermItem& find_item(const BermItem* items, int 
 }
 
 __device__ __forceinline__ Model item_model(const BermItem& it) {
-  return Model{gmem(it.z), gmem(it.f0), gmem(it.taus), gmem(it.disp), gmem(it.A), it.strike,
+  return Model{gmem(it.z), cmem(it.f0), cmem(it.taus), cmem(it.disp), cmem(it.A), it.strike,
                it.P, it.N, it.k};
 }
 

@@ -308,7 +308,8 @@ extern "C" int lmm_price_fwd(int paths, int rates, int factors, int checkpoint_e
       !partials || !price_out)
     return set_error(KOO_ERR_INVALID, "lmm_price_fwd: null pointer");
   const float strike = product_aux[0];
-  Model m{variates, initial_rates, taus, displacements, pseudo_roots, strike, paths, rates,
+  Model m{variates, cmem(initial_rates), cmem(taus), cmem(displacements), cmem(pseudo_roots),
+          strike, paths, rates,
           checkpoint_every};
   const int threads = fwd_threads(rates);
   const int blocks = (paths + threads - 1) / threads;
@@ -341,7 +342,8 @@ extern "C" int lmm_price_bwd(int paths, int rates, int factors, int checkpoint_e
       !product_aux || !checkpoints || !segment_scratch || !partials || !greeks_out)
     return set_error(KOO_ERR_INVALID, "lmm_price_bwd: null pointer");
   const float strike = product_aux[0];
-  Model m{variates, initial_rates, taus, displacements, pseudo_roots, strike, paths, rates,
+  Model m{variates, cmem(initial_rates), cmem(taus), cmem(displacements), cmem(pseudo_roots),
+          strike, paths, rates,
           checkpoint_every};
   const int threads = bwd_threads(rates);
   const int blocks = (paths + threads - 1) / threads;

@@ -89,8 +89,8 @@ __device__ __forceinline__ void block_sum_to(double v, double* scratch_lds, doub
   }
 }
 
-// Constant address space (4): the pseudo-roots are read through it in the
-// step loops, since no kernel writes them, so that their uniform reads use
+// Constant address space (4): the model's curve and pseudo-roots are read
+// through it, since no kernel writes them, so that their uniform reads use
 // the scalar cache also where the model comes from a device descriptor (the
 // batched and persistent kernels), not only from __restrict__ arguments.
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -98,13 +98,18 @@ __device__ __forceinline__ void block_sum_to(double v, double* scratch_lds, doub
 #else
 #define KOO_CONSTANT
 #endif
+typedef const float KOO_CONSTANT* c_cfloat;
+template <class P>
+__host__ __device__ __forceinline__ c_cfloat cmem(P p) {
+  return (c_cfloat)p;
+}
 
 struct Model {
   const float* z;     // (S,F,P) bridged variates
-  const float* f0;    // N
-  const float* taus;  // N
-  const float* disp;  // N
-  const float* A;     // (S,N,F)
+  c_cfloat f0;        // N
+  c_cfloat taus;      // N
+  c_cfloat disp;      // N
+  c_cfloat A;         // (S,N,F)
   float strike;
   int P, N, k;
 };
@@ -126,7 +131,7 @@ __device__ __forceinline__ float step_table_entry(const Model& m, int i) {
     const int r = i - nn;
     return 1.0f / m.taus[r] - m.disp[r];
   }
-  const float* a = m.A + static_cast<size_t>(i) * F;
+  c_cfloat a = m.A + static_cast<size_t>(i) * F;
   float c = 0.0f;
 #pragma unroll
   for (int f = 0; f < F; ++f) c -= 0.5f * a[f] * a[f];
@@ -164,7 +169,7 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
                                         const float (&z)[F], float (&e)[F], float (&ep)[F],
                                         Load load, Store store) {
   // 32-bit offset: keeps the address scalar
-  const float KOO_CONSTANT* As = (const float KOO_CONSTANT*)m.A + s * m.N * F;
+  c_cfloat As = m.A + s * m.N * F;
   const float* kt = cs + (m.N - s) * m.N;
   const int cnt = m.N - s;
 #pragma unroll
@@ -221,7 +226,7 @@ __device__ __forceinline__ void reverse_step(const Model& m, int s, const float*
                                              Vega vega) {
   const int N = m.N;
   // 32-bit offset: keeps the address scalar
-  const float KOO_CONSTANT* As = (const float KOO_CONSTANT*)m.A + s * N * F;
+  c_cfloat As = m.A + s * N * F;
   const float* kt = cs + (N - s) * N;
   const int cnt = N - s;
   float E[F], Eh[F];
@@ -301,7 +306,8 @@ __device__ __forceinline__ void load_z(const Model& m, int s, int p, float (&z)[
   const float* __restrict__ z_, const float* __restrict__ f0_,                            \
       const float* __restrict__ taus_, const float* __restrict__ disp_,                   \
       const float* __restrict__ A_, float strike_, int P_, int N_, int k_
-#define KOO_MODEL_LOCAL const Model m{z_, f0_, taus_, disp_, A_, strike_, P_, N_, k_}
+#define KOO_MODEL_LOCAL \
+  const Model m{z_, cmem(f0_), cmem(taus_), cmem(disp_), cmem(A_), strike_, P_, N_, k_}
 
 // Batched kernels read the pointers of their device descriptor tables as
 // global-memory pointers (address space 1), so accesses through them are

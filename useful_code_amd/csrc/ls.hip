@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(256)
     k_berm_paths_batch(const LsItem* __restrict__ items, int n, int block_offset) {
   const int block = blockIdx.x + block_offset;
   const LsItem& it = items[find_block_owner(items, n, block, &LsItem::paths_block0)];
-  const Model m{gmem(it.z), gmem(it.f0), gmem(it.taus), gmem(it.disp), gmem(it.A), it.strike,
+  const Model m{gmem(it.z), cmem(it.f0), cmem(it.taus), cmem(it.disp), cmem(it.A), it.strike,
                 it.P, it.N, 1};
   const int blk = block - it.paths_block0;
   switch (it.F) {

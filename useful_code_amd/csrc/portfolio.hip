@@ -65,7 +65,7 @@ struct PfItem {
 };
 
 __device__ __forceinline__ Model pf_model(const PfItem& it, int k) {
-  return Model{gmem(it.z), gmem(it.f0), gmem(it.taus), gmem(it.disp), gmem(it.A), it.strike,
+  return Model{gmem(it.z), cmem(it.f0), cmem(it.taus), cmem(it.disp), cmem(it.A), it.strike,
                it.P, it.N, k};
 }
 
