@@ -137,7 +137,8 @@ extern "C" int bermudan_adjoint_workspace(int paths, int rates, int factors, int
   const int k = checkpoint_every;
   const size_t nseg = (rates + k - 1) / k;
   const size_t rows = k == 1 ? rates + 1 + 2 * factors : rates + 1;
-  if (checkpoint_bytes) *checkpoint_bytes = sizeof(float) * nseg * rows * P;
+  // the checkpoints, then the step table the forward copies for the backward
+  if (checkpoint_bytes) *checkpoint_bytes = sizeof(float) * (nseg * rows * P + step_table_floats(rates));
   if (scratch_bytes)
     *scratch_bytes = k == 1 ? sizeof(float)
                             : sizeof(float) * static_cast<size_t>(k < rates ? k : rates) *
@@ -209,7 +210,7 @@ extern "C" int bermudan_frozen_bwd(int paths, int rates, int factors, int checkp
   const float* vin = path_buffers + static_cast<size_t>(rates + 1) * P;
   const int threads = bwd_threads(rates);
   const int blocks = (paths + threads - 1) / threads;
-  const size_t lds = bwd_lds(rates, threads);
+  const size_t lds = bwd_lds(rates, threads, false);
   hipStream_t st = as_stream(stream);
   switch (factors) {
 #define KOO_CASE(F)                                                                            \
@@ -337,7 +338,7 @@ static int portfolio_launch(const char* who, bool fwd, const koo_portfolio_plan*
       hipLaunchKernelGGL(k_berm_fwd_batch, dim3(nb), dim3(256), fwd_lds(nc, 256), st, items,
                          plan->products, b0);
     else
-      hipLaunchKernelGGL(k_berm_bwd_batch, dim3(nb), dim3(256), bwd_lds(nc, 256), st, items,
+      hipLaunchKernelGGL(k_berm_bwd_batch, dim3(nb), dim3(256), bwd_lds(nc, 256, false), st, items,
                          plan->products, b0);
   }
   hipLaunchKernelGGL(k_berm_finalize_batch, dim3(fwd ? 1 : 2 * N, plan->products), dim3(256), 0,

@@ -12,9 +12,6 @@ namespace {
 using namespace lmm;
 
 constexpr int kMaxBermDates = 40;
-// Vega partial sums per rate and wavefront in LDS: the 64 lanes folded to 4
-// (one per 16-lane row) before the LDS adds.
-constexpr int kVegaSlots = 4;
 constexpr int kStatePrefetch = 8;
 
 struct BermArgs {
@@ -27,6 +24,14 @@ __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+// Floats of a product's checkpoint buffer before the step table that the
+// single-product and batched forwards copy behind them.
+template <int F>
+__device__ __forceinline__ size_t berm_ckpt_floats(const Model& m) {
+  const size_t rows = m.k == 1 ? m.N + 1 + 2 * F : m.N + 1;
+  return static_cast<size_t>((m.N + m.k - 1) / m.k) * rows * static_cast<size_t>(m.P);
 }
 
 // Coefficients of the frozen strategy, date ei, function j, as FP32: from
@@ -60,6 +65,14 @@ __device__ __forceinline__ void berm_fwd_block(const Model& m, const BermArgs& b
   fill_step_table<F>(m, ctab);
   for (int r = 0; r < m.N; ++r) L[r * bd] = logf(m.f0[r] + m.disp[r]);
   __syncthreads();
+  // Block 0 of a single-product or batched forward also copies the step
+  // table behind the checkpoints, where the backward reads it (the
+  // persistent kernel's backward keeps its own LDS copy: it runs in the same
+  // launch as this forward).
+  if (!Agent && ckpt && block == 0) {
+    float* tab = ckpt + berm_ckpt_floats<F>(m);
+    for (int i = threadIdx.x; i < step_table_floats(m.N); i += bd) tab[i] = ctab[i];
+  }
   float invB = 1.0f, V = 0.0f;
   int tau = -1;
   bool done = !valid;
@@ -129,7 +142,11 @@ __device__ __forceinline__ void berm_fwd_block(const Model& m, const BermArgs& b
 }
 
 // One block of the backward (paths [block * blockDim, ...)).  Agent: the
-// block's partials stored agent-coherent (persistent kernel).
+// block's partials stored agent-coherent (persistent kernel).  LDS: the
+// adjoints of rates 1..N-1 [N-1][lanes] (rate 0's in a register), the step
+// table [N][N+1] in the persistent kernel only (the others read the
+// forward's copy behind the checkpoints through the scalar cache), and one
+// vega partial sum per wavefront and rate [wave][N].
 template <int F, bool Agent = false>
 __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __restrict__ vols,
                                                const float* __restrict__ ckpt,
@@ -148,14 +165,19 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
   const int pc = valid ? p : m.P - 1;
   const int N = m.N;
   const size_t P = static_cast<size_t>(m.P);
-  float* lam = lds + threadIdx.x;
-  float* ctab = lds + static_cast<size_t>(N) * bd;
-  float* vslots = ctab + step_table_floats(N);
-  float* myslots = vslots + static_cast<size_t>(wave) * N * kVegaSlots + (lane >> 4);
-  fill_step_table<F>(m, ctab);
-  for (int r = 0; r < N; ++r) lam[r * bd] = 0.0f;
-  for (int i = threadIdx.x; i < nw * N * kVegaSlots; i += bd) vslots[i] = 0.0f;
+  float lam0 = 0.0f;
+  const LamReg0 lam{lds + threadIdx.x, bd, lam0};
+  float* ltab = lds + static_cast<size_t>(N - 1) * bd;
+  float* vslots = ltab + (Agent ? step_table_floats(N) : 0);
+  float* myslots = vslots + static_cast<size_t>(wave) * N;
+  if (Agent) fill_step_table<F>(m, ltab);
+  for (int r = 1; r < N; ++r) lam.set(r, 0.0f);
+  for (int i = threadIdx.x; i < nw * N; i += bd) vslots[i] = 0.0f;
   __syncthreads();
+  const auto ctab = [&]() {
+    if constexpr (Agent) return static_cast<const float*>(ltab);
+    else return cmem(ckpt + berm_ckpt_floats<F>(m));
+  }();
 
   const int tau = valid ? tau_in[p] : -1;
   const float V = valid ? vin[p] : 0.0f;
@@ -201,7 +223,7 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
       const float* src = (j == 0) ? ck : scratch + j * slot + pc;
       const float* sums = (every ? ck : scratch + j * slot + pc) + static_cast<size_t>(N + 1) * P;
       if (s == tau)  // seed lambda with the payoff adjoint recorded at exercise
-        for (int r = s; r < N; ++r) lam[r * bd] = lamt[static_cast<size_t>(r) * P + p];
+        for (int r = s; r < N; ++r) lam.set(r, lamt[static_cast<size_t>(r) * P + p]);
       float z[F], e[F], ep[F];
       load_z<F>(m, s, pc, z);
 #pragma unroll
@@ -211,7 +233,7 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
       }
       reverse_step<F, kStatePrefetch>(
           m, s, ctab + s * N, [&](int r) { return src[static_cast<size_t>(r) * P]; },
-          LdsLam{lam, bd}, z, e, ep,
+          lam, z, e, ep,
           [&](float& lp, float xs, float fs, float inv3) {
             (void)fs;
             if (s < tau) lp -= m.taus[s] * inv3 * V * xs;  // fixing inside B_tau
@@ -222,21 +244,21 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
             w += dpp_f32_quad_xor2(w);
             w += dpp_f32_row_half_mirror(w);
             w += dpp_f32_row_mirror(w);
-            if ((lane & 15) == 0) atomicAdd(myslots + r * kVegaSlots, w);
+            // every lane of a row holds the row's sum: the rows' last lanes
+            // add them to the wavefront's slot (one LDS atomic, four lanes)
+            if ((lane & 15) == 15) atomicAdd(myslots + r, w);
           });
     }
   }
   __syncthreads();
   for (int r = 0; r < N; ++r) {
-    const float dl = lam[r * bd] / (m.f0[r] + m.disp[r]);
+    const float dl = lam.get(r) / (m.f0[r] + m.disp[r]);
     block_sum_to<Agent>(valid ? static_cast<double>(dl) : 0.0, red,
                         part + static_cast<size_t>(block) * 2 * N + r);
   }
   for (int r = threadIdx.x; r < N; r += bd) {
     double t = 0.0;
-    for (int w = 0; w < nw; ++w)
-      for (int q = 0; q < kVegaSlots; ++q)
-        t += vslots[(static_cast<size_t>(w) * N + r) * kVegaSlots + q];
+    for (int w = 0; w < nw; ++w) t += vslots[static_cast<size_t>(w) * N + r];
     double* o = part + static_cast<size_t>(block) * 2 * N + N + r;
     if (Agent)
       store_agent(o, t / vols[r]);
@@ -248,8 +270,11 @@ __device__ __forceinline__ void berm_bwd_block(const Model& m, const float* __re
 int fwd_threads(int N) { return N <= 50 ? 256 : (N <= 75 ? 128 : 64); }
 int bwd_threads(int N) { return N <= 45 ? 256 : (N <= 71 ? 128 : 64); }
 size_t fwd_lds(int N, int t) { return sizeof(float) * (static_cast<size_t>(N) * t + step_table_floats(N)); }
-size_t bwd_lds(int N, int t) {
-  return sizeof(float) * (static_cast<size_t>(N) * t + step_table_floats(N) + (t / 64) * N * kVegaSlots);
+// lds_table: the persistent kernel's backward, which keeps the step table in
+// LDS.
+size_t bwd_lds(int N, int t, bool lds_table) {
+  return sizeof(float) * (static_cast<size_t>(N - 1) * t + (lds_table ? step_table_floats(N) : 0) +
+                          (t / 64) * N);
 }
 
 // Host checks and the strategy argument.

@@ -108,22 +108,6 @@ __global__ void __launch_bounds__(256)
 // that is 40 608 bytes with the block-sum words: four blocks per CU.
 // ----------------------------------------------------------------------
 
-// Log-rate adjoints: rate 0 in a register, rate r >= 1 at p[(r - 1) * bd].
-// The rate index is uniform, so the branch is a scalar one.
-struct LamReg0 {
-  float* p;
-  int bd;
-  float& l0;
-  __device__ __forceinline__ float get(int r) const {
-    if (r > 0) return p[(r - 1) * bd];
-    return l0;
-  }
-  __device__ __forceinline__ void set(int r, float v) const {
-    if (r > 0) p[(r - 1) * bd] = v;
-    else l0 = v;
-  }
-};
-
 // Depth of the queue of start log-rates read from checkpoints or scratch in
 // the backward kernel (HBM latency over the per-rate work of the resident
 // wavefronts).  Measured at 2^20 paths, 40 rates, 5 factors with three

@@ -145,12 +145,21 @@ __device__ __forceinline__ void fill_step_table(const Model& m, float* ctab) {
     ctab[i] = step_table_entry<F>(m, i);
 }
 
-// Adjoints of the log-rates, lam[r * bd] in LDS (lane-minor rows).
-struct LdsLam {
+// Log-rate adjoints: rate 0 in a register (it only changes in step 0), rate
+// r >= 1 at p[(r - 1) * bd] in LDS (lane-minor rows).  The rate index is
+// uniform, so the branch is a scalar one.
+struct LamReg0 {
   float* p;
   int bd;
-  __device__ __forceinline__ float get(int r) const { return p[r * bd]; }
-  __device__ __forceinline__ void set(int r, float v) const { p[r * bd] = v; }
+  float& l0;
+  __device__ __forceinline__ float get(int r) const {
+    if (r > 0) return p[(r - 1) * bd];
+    return l0;
+  }
+  __device__ __forceinline__ void set(int r, float v) const {
+    if (r > 0) p[(r - 1) * bd] = v;
+    else l0 = v;
+  }
 };
 
 // One predictor-corrector step s over rates r >= s.  `load(r)` gives the
@@ -164,13 +173,13 @@ struct LdsLam {
 // register whose load is still in flight waits for that load, which would
 // turn the queue back into one load in flight.  Tail loads are clamped to
 // the last rate (in bounds, never used).
-template <int F, int PF = 2, class Load, class Store>
-__device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
+template <int F, int PF = 2, class Tab, class Load, class Store>
+__device__ __forceinline__ void pc_step(const Model& m, int s, Tab cs,
                                         const float (&z)[F], float (&e)[F], float (&ep)[F],
                                         Load load, Store store) {
   // 32-bit offset: keeps the address scalar
   c_cfloat As = m.A + s * m.N * F;
-  const float* kt = cs + (m.N - s) * m.N;
+  const auto kt = cs + (m.N - s) * m.N;
   const int cnt = m.N - s;
 #pragma unroll
   for (int f = 0; f < F; ++f) e[f] = ep[f] = 0.0f;
@@ -219,15 +228,15 @@ __device__ __forceinline__ void pc_step(const Model& m, int s, const float* cs,
 // lp; `vega(r, w)` receives w = sum_f Abar(s,r,f) A(s,r,f) for the step, which
 // contracts to scalars the reverse already has (the mu^, mu, dW and
 // fixed-drift terms).  E, E^ are the adjoint running sums.
-template <int F, int PF, class Src, class Lam, class Fixing, class Vega>
-__device__ __forceinline__ void reverse_step(const Model& m, int s, const float* cs, Src src,
+template <int F, int PF, class Tab, class Src, class Lam, class Fixing, class Vega>
+__device__ __forceinline__ void reverse_step(const Model& m, int s, Tab cs, Src src,
                                              const Lam& lam, const float (&z)[F],
                                              float (&e)[F], float (&ep)[F], Fixing fixing,
                                              Vega vega) {
   const int N = m.N;
   // 32-bit offset: keeps the address scalar
   c_cfloat As = m.A + s * N * F;
-  const float* kt = cs + (N - s) * N;
+  const auto kt = cs + (N - s) * N;
   const int cnt = N - s;
   float E[F], Eh[F];
 #pragma unroll

@@ -340,7 +340,7 @@ size_t pf_phase_lds(int N) {
   const size_t gen = sizeof(float) * static_cast<size_t>(N) * kPfThreads + sizeof(uint32_t) * 32 * N;
   size_t m = gen;
   m = std::max(m, fwd_lds(N, kPfThreads));
-  m = std::max(m, bwd_lds(N, kPfThreads));
+  m = std::max(m, bwd_lds(N, kPfThreads, true));
   m = std::max(m, sizeof(double) * static_cast<size_t>(kPfLsDoubles));
   return (m + 15) & ~static_cast<size_t>(15);
 }
