@@ -684,7 +684,9 @@ int basket_asian_gpu(int paths, int assets, int steps, const float* variates,
  * ls_first_pass_gpu writes them); exercise_steps (increasing, <= 40 dates),
  * var_shift and var_scale (E x 2, NULL for 0 / 1) are HOST arrays; buffers
  * come from bermudan_adjoint_workspace().  The backward reads the path
- * buffers the forward wrote.  Nothing synchronises, so the whole first pass
+ * buffers and the checkpoints the forward wrote (the checkpoint buffer also
+ * carries the model's step table, written by the forward).  Nothing
+ * synchronises, so the whole first pass
  * plus adjoint can be captured in a graph.
  * =================================================================== */
 int bermudan_adjoint_workspace(int paths, int rates, int factors, int checkpoint_every,
